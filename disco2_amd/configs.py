@@ -1,0 +1,65 @@
+"""Named problem set-ups: the benchmark workloads and small test variants.
+
+Each entry returns ``(params, ic_kind, ic_kwargs)``.  ``params`` is a full
+parameter dictionary (see ``params.py``); ``N_p_round=1`` selects
+N_phi = round(2 pi r_c / dr) per annulus instead of the reference's
+truncating aspect-ratio formula (src/Sim/sim_set.c:17-29).
+"""
+from __future__ import annotations
+
+from .params import (make_params, C_RIGID, C_KEPLER, C_FIXED, A_KEPLER, GRAV_NONE,
+                     GRAV_SINGLE, GRAV_BINARY, HLLC, HLL)
+
+
+def vortex(numr=64, r_min=0.5, r_max=5.0, **over):
+    """2D isentropic vortex, annuli at the mass-weighted mean phi velocity."""
+    p = make_params(NumR=numr, R_Min=r_min, R_Max=r_max, NumZ=1, Adiabatic_Index=1.4,
+                    w_numeric_type=C_RIGID, GravMassType=GRAV_NONE, RiemannSolver=HLLC,
+                    PLM=1.0, CFL=0.4, N_p_round=1)
+    p.update(over)
+    return p, "vortex", {"seed": 1}
+
+
+def binary(numr=512, r_min=0.3, r_max=10.0, q=1.0, alpha=0.01, mach=20.0, **over):
+    """2D disk around an equal-mass binary with alpha viscosity, Keplerian annuli."""
+    p = make_params(NumR=numr, R_Min=r_min, R_Max=r_max, NumZ=1, Adiabatic_Index=5.0 / 3.0,
+                    GravMassType=GRAV_BINARY, MassRatio=q, G_EPS=0.05, Grav_2D=1,
+                    Explicit_Viscosity=alpha, ViscConst=0, w_numeric_type=C_KEPLER,
+                    RiemannSolver=HLLC, PLM=1.0, CFL=0.4, N_p_round=1)
+    p.update(over)
+    return p, "binary_disk", {"seed": 1, "mach": mach}
+
+
+def disk3d(numr=256, numz=64, npcap=1024, **over):
+    """3D vertically isothermal Keplerian disk around a single mass, HLL."""
+    p = make_params(NumR=numr, R_Min=0.5, R_Max=8.0, NumZ=numz, Z_Min=-0.5, Z_Max=0.5,
+                    NPCAP=npcap, Adiabatic_Index=5.0 / 3.0, GravMassType=GRAV_SINGLE,
+                    Grav_2D=0, G_EPS=0.0, Explicit_Viscosity=-1.0, w_numeric_type=C_KEPLER,
+                    RiemannSolver=HLL, PLM=1.0, CFL=0.4, N_p_round=1)
+    p.update(over)
+    return p, "disk3d", {"seed": 1}
+
+
+# The benchmark workloads (sizes of the measurement definition)
+def c1_vortex():
+    return vortex(64)
+
+
+def c2_binary():
+    return binary(512)
+
+
+def c3_disk3d():
+    return disk3d(256, 64, 1024)
+
+
+def c5_adjoint():
+    return binary(256, r_min=0.3, r_max=8.0, q=0.5, alpha=0.01)
+
+
+WORKLOADS = {
+    "vortex2d_r64": c1_vortex,
+    "binary2d_r512": c2_binary,
+    "disk3d_r256z64": c3_disk3d,
+    "binary2d_r256_adjoint": c5_adjoint,
+}

@@ -1,0 +1,110 @@
+"""Writes the golden fixtures of tests/golden/ from the reference build.
+
+Run in a container where the reference sources exist and oracle/_ref has
+been built (``make -C oracle ref``):
+
+    python tests/golden/make_golden.py [case ...]
+
+For every case it writes the parfile, asks the reference for its mesh,
+builds the seeded initial state (disco2_amd.ics), runs the reference's
+unit pieces (faces, PLM slopes, the first substep) and a few RK2 steps, and
+stores inputs and outputs as a compressed .npz.  The fixtures are data only.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from disco2_amd import configs, ics  # noqa: E402
+from disco2_amd.geometry import n_p_round  # noqa: E402
+from disco2_amd.params import write_parfile  # noqa: E402
+from oracle import oracle as orc  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def cases():
+    c = {}
+    c["vortex_r16"] = (configs.vortex(16), dict(steps=20, dumps=[1, 2, 20], unit=True))
+    p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=1, PLM=2.0, RiemannSolver=1, NoInnerBC=1)
+    c["vortex_scalar_hll_noinner"] = ((p, k, dict(kw, x0=1.8)), dict(steps=12, dumps=[1, 12], unit=True))
+    c["binary_r16"] = (configs.binary(16, r_min=0.3, r_max=3.0), dict(steps=20, dumps=[1, 2, 20], unit=True))
+    p, k, kw = configs.binary(16, r_min=0.3, r_max=3.0, q=0.5, w_numeric_type=0, w_analytic_type=1,
+                              RLogScale=1.0, RhoSinkOn=1, ViscConst=0)
+    c["binary_akepler_log_sink"] = ((p, k, kw), dict(steps=12, dumps=[1, 12], unit=True))
+    p, k, kw = configs.binary(16, r_min=0.3, r_max=3.0, alpha=0.02, ViscConst=1, w_numeric_type=2)
+    c["binary_viscconst_rigid"] = ((p, k, kw), dict(steps=8, dumps=[1, 8], unit=True))
+    p, k, kw = configs.disk3d(8, 4, 24, R_Min=0.5, R_Max=2.5)
+    c["disk3d_r8z4"] = ((p, k, kw), dict(steps=10, dumps=[1, 2, 10], unit=True))
+    p, k, kw = configs.disk3d(8, 4, 20, R_Min=0.5, R_Max=2.5, N_p_round=0, HiResFac=1.0,
+                              HiResSigma=0.3, HiResR0a=1.0, ZLogScale=0.5)
+    c["disk3d_refNp_hires"] = ((p, k, kw), dict(steps=6, dumps=[1, 6], unit=True))
+    c["vortex_r64_100"] = (configs.vortex(64), dict(steps=100, dumps=[100], unit=False, light=True))
+    return c
+
+
+def make_case(name, spec, opts, tmp):
+    params, ic_kind, ic_kw = spec
+    par = os.path.join(tmp, name + ".par")
+    write_parfile(par, params)
+    g0 = orc.read_records(_run(["geom", par, "-", os.path.join(tmp, "g0.bin")], tmp, "g0.bin"))
+    NR, NZ, NQ = (int(x) for x in g0["dims"][:3])
+    if params.get("N_p_round", 0):
+        N_p = n_p_round(g0["r_faces"], params["R_Min"], params["NPCAP"])
+    else:
+        N_p = g0["N_p"]
+    icg = os.path.join(tmp, "icg.bin")
+    orc.write_icfile(icg, NR, NZ, NQ, N_p=N_p)
+    g = orc.read_records(_run(["geom", par, icg, os.path.join(tmp, "g.bin")], tmp, "g.bin"))
+    geom = {"N_p": N_p, "r_faces": g["r_faces"], "z_faces": g["z_faces"],
+            "tiph": g["tiph0"], "dphi": g["dphi0"]}
+    prim0 = ics.make_ic(ic_kind, geom, params, **ic_kw)
+    ic = os.path.join(tmp, "ic.bin")
+    orc.write_icfile(ic, NR, NZ, NQ, N_p=N_p, prim=prim0)
+    out = {"params": np.array(json.dumps(params)), "N_p": N_p, "N_p_ref_formula": g0["N_p"],
+           "r_faces": g["r_faces"], "z_faces": g["z_faces"], "tiph0": g["tiph0"], "dphi0": g["dphi0"],
+           "prim0": prim0.ravel(), "dims": g["dims"]}
+    if opts.get("unit"):
+        u = orc.read_records(_run(["unit", par, ic, os.path.join(tmp, "u.bin")], tmp, "u.bin"))
+        for k, v in u.items():
+            if k.startswith(("faces_", "gradp", "grad_", "init/cons", "sub1/")):
+                out["unit/" + k] = v
+    args = ["run", par, ic, os.path.join(tmp, "r.bin"), str(opts["steps"])] + [str(d) for d in opts["dumps"]]
+    r = orc.read_records(_run(args, tmp, "r.bin"))
+    for k, v in r.items():
+        if k.startswith("step0/") or k in ("r_faces", "z_faces", "N_p", "dims"):
+            continue
+        if opts.get("light") and not k.endswith(("/prim", "/tiph", "/t_dt", "/gravmass")) and k != "dts":
+            continue
+        out["run/" + k] = v
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
+def _run(args, tmp, outname):
+    orc.run_reference(args, tmp)
+    return os.path.join(tmp, outname)
+
+
+def main(argv):
+    if not orc.have_reference():
+        raise SystemExit("build the reference harness first: make -C oracle ref")
+    want = set(argv)
+    with tempfile.TemporaryDirectory() as tmp:
+        for name, (spec, opts) in cases().items():
+            if want and name not in want:
+                continue
+            path = make_case(name, spec, opts, tmp)
+            print(f"{name}: {os.path.getsize(path)} bytes")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
