@@ -1,0 +1,142 @@
+"""Python front end of the C ABI: simulation set-up and batched runs.
+
+``Setup`` drives the reference's set-up sequence (src/main.c:60-71) through
+the class API; ``Batch`` holds one or many disks on the GPU and advances them
+with RK2 steps.  Field arrays use the reference's zone order (layer k,
+annulus i, cell j) and are q-major for the NQ-vectors.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._lib import check, dptr, iptr, lib
+
+FIELDS = {"prim": 0, "cons": 1, "RKcons": 2, "grad": 3, "gradp": 4,
+          "tiph": 5, "RKtiph": 6, "dphi": 7, "wiph": 8}
+VECTOR_FIELDS = {"prim", "cons", "RKcons", "grad", "gradp"}
+
+
+class Setup:
+    """A Sim built from a parameter dictionary (parfile keys)."""
+
+    def __init__(self, params: dict, N_p=None):
+        L = lib()
+        self.params = dict(params)
+        self.mpi = L.mpisetup_create(0, None)
+        self.sim = L.sim_create(self.mpi)
+        for k, v in params.items():
+            if L.sim_set_param(self.sim, k.encode(), float(v)) != 0:
+                raise KeyError(f"unknown parameter {k}")
+        L.sim_alloc_arr(self.sim, self.mpi)
+        L.sim_set_rz(self.sim, self.mpi)
+        L.sim_set_N_p(self.sim)
+        if N_p is not None:
+            arr = np.ascontiguousarray(N_p, dtype=np.int32)
+            L.sim_set_N_p_array(self.sim, iptr(arr))
+        L.sim_set_misc(self.sim, self.mpi)
+        check()
+        self.NR = L.sim_N(self.sim, 0)
+        self.NZ = L.sim_N(self.sim, 1)
+        self.NQ = L.sim_NUM_Q(self.sim)
+        self.N_p = np.array([L.sim_N_p(self.sim, i) for i in range(self.NR)], dtype=np.int32)
+        self.r_faces = np.array([L.sim_FacePos(self.sim, i - 1, 0) for i in range(self.NR + 1)])
+        self.z_faces = np.array([L.sim_FacePos(self.sim, k - 1, 1) for k in range(self.NZ + 1)])
+        self.nzones = int(self.N_p.sum()) * self.NZ
+        self.tiph0 = np.empty(self.nzones)
+        self.dphi0 = np.empty(self.nzones)
+        L.disco_initial_edges(self.sim, dptr(self.tiph0), dptr(self.dphi0))
+
+    def geometry(self) -> dict:
+        return {"N_p": self.N_p, "r_faces": self.r_faces, "z_faces": self.z_faces,
+                "tiph": self.tiph0, "dphi": self.dphi0}
+
+    def interior_zones(self) -> int:
+        L = lib()
+        i0, i1 = L.sim_Nghost_min(self.sim, 0), self.NR - L.sim_Nghost_max(self.sim, 0)
+        nz = L.sim_N_global(self.sim, 1)
+        return int(self.N_p[i0:i1].sum()) * nz
+
+    def close(self):
+        L = lib()
+        if getattr(self, "sim", None):
+            L.sim_destroy(self.sim)
+            L.mpisetup_destroy(self.mpi)
+            self.sim = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """One or many independent disks resident on the GPU."""
+
+    def __init__(self, setups):
+        self.setups = list(setups)
+        L = lib()
+        arr = (ctypes.c_void_p * len(self.setups))(*[s.sim for s in self.setups])
+        self.h = L.disco_batch_create(len(self.setups), arr)
+        check()
+        if not self.h:
+            raise RuntimeError("batch creation failed")
+        self.NQ = self.setups[0].NQ
+
+    def nzones(self, disk=0):
+        return self.setups[disk].nzones
+
+    def upload(self, field, arr, disk=0):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        lib().disco_batch_upload(self.h, disk, FIELDS[field], dptr(a))
+        check()
+
+    def download(self, field, disk=0):
+        n = self.setups[disk].nzones
+        out = np.empty(n * (self.NQ if field in VECTOR_FIELDS else 1))
+        lib().disco_batch_download(self.h, disk, FIELDS[field], dptr(out))
+        check()
+        return out.reshape(self.NQ, n) if field in VECTOR_FIELDS else out
+
+    def init(self):
+        lib().disco_batch_init(self.h)
+        check()
+
+    def run(self, nsteps, graph=True, sync=True):
+        lib().disco_batch_run(self.h, int(nsteps), 1 if graph else 0)
+        if sync:
+            self.synchronize()
+
+    def synchronize(self):
+        lib().disco_batch_synchronize(self.h)
+        check()
+
+    def last_ms(self):
+        return lib().disco_batch_last_ms(self.h)
+
+    def scalars(self, disk=0):
+        out = np.empty(23)
+        lib().disco_batch_scalars(self.h, disk, dptr(out))
+        check()
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().disco_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_disk(params: dict, ic_kind: str, ic_kwargs: dict, N_p=None):
+    """Setup + initial state for one disk: (setup, prim0)."""
+    from . import ics
+    s = Setup(params, N_p=N_p)
+    prim0 = ics.make_ic(ic_kind, s.geometry(), params, **ic_kwargs)
+    return s, prim0

@@ -1,0 +1,492 @@
+// The reference's Cell / Face / TimeStep class API on top of the Engine,
+// the batch API and error reporting.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+
+#include "../../include/disco.h"
+#include "engine.h"
+
+// ------------------------------------------------------------------ errors
+static int g_err = 0;
+static std::string g_msg;
+static std::mutex g_mu;
+void disco_set_error(int code, const std::string &msg) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_err == 0) {
+    g_err = code;
+    g_msg = msg;
+  }
+}
+extern "C" {
+int disco_last_error(void) { return g_err; }
+const char *disco_last_error_msg(void) { return g_msg.c_str(); }
+void disco_clear_error(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_err = 0;
+  g_msg.clear();
+}
+const char *disco_version(void) { return "disco2_amd 0.1 (gfx950, fp64)"; }
+}
+
+// ------------------------------------------------------------------ helpers
+static CellsHandle *H(Cell ***c) { return reinterpret_cast<CellsHandle *>(c); }
+
+static void mirror_pull(CellsHandle *h) {
+  if (!h->stale) return;
+  Engine *e = h->eng;
+  e->download(0, 0, h->prim.data());
+  e->download(0, 3, h->grad.data());
+  e->download(0, 4, h->gradp.data());
+  e->download(0, 5, h->tiph.data());
+  e->download(0, 7, h->dphi.data());
+  e->download(0, 8, h->wiph.data());
+  h->stale = false;
+}
+static void mirror_push(CellsHandle *h) {
+  if (h->dirty_prim) h->eng->upload(0, 0, h->prim.data());
+  if (h->dirty_tiph) h->eng->upload(0, 5, h->tiph.data());
+  h->dirty_prim = h->dirty_tiph = false;
+}
+static long long zone_of(CellsHandle *h, int i, int j, int k) {
+  const AnnInfo &A = h->eng->hann[k * h->eng->hdp[0].NR + i];
+  return A.zbase + j;
+}
+static void scalars_from_host(Engine *e, TimeStep *ts, GravMass *gm) {
+  DiskScalars s = e->get_scalars(0);
+  if (ts) {
+    s.t = ts->t;
+    s.dt = ts->dt;
+    s.RK = ts->RK;
+  }
+  if (gm)
+    for (int p = 0; p < 2; ++p) {
+      s.gm[p].r = gm[p].r; s.gm[p].phi = gm[p].phi; s.gm[p].M = gm[p].M; s.gm[p].omega = gm[p].omega;
+      s.gm[p].RK_r = gm[p].RK_r; s.gm[p].RK_phi = gm[p].RK_phi; s.gm[p].RK_M = gm[p].RK_M;
+      s.gm[p].RK_omega = gm[p].RK_omega; s.gm[p].Mdot = gm[p].Mdot; s.gm[p].Macc = gm[p].Macc;
+    }
+  e->put_scalars(0, s);
+}
+static void scalars_to_host(Engine *e, TimeStep *ts, GravMass *gm) {
+  DiskScalars s = e->get_scalars(0);
+  if (ts) {
+    ts->t = s.t;
+    ts->dt = s.dt;
+    ts->RK = s.RK;
+  }
+  if (gm)
+    for (int p = 0; p < 2; ++p) {
+      gm[p].r = s.gm[p].r; gm[p].phi = s.gm[p].phi; gm[p].M = s.gm[p].M; gm[p].omega = s.gm[p].omega;
+      gm[p].RK_r = s.gm[p].RK_r; gm[p].RK_phi = s.gm[p].RK_phi; gm[p].RK_M = s.gm[p].RK_M;
+      gm[p].RK_omega = s.gm[p].RK_omega; gm[p].Mdot = s.gm[p].Mdot; gm[p].Macc = s.gm[p].Macc;
+    }
+}
+
+extern "C" {
+// ------------------------------------------------------------------ Cell
+// cell_create, src/Cell/cell_create_destroy.c:23-95: zero primitives, one
+// random phi offset per annulus (srand(666) sequence), dphi = 2 pi / N_p;
+// the edges are also wrapped and z-synced as src/main.c:80-85 does next.
+Cell ***cell_create(Sim *s, MPIsetup *) {
+  if (!s->allocated) {
+    disco_set_error(10, "cell_create before sim_alloc_arr");
+    return nullptr;
+  }
+  Engine *e = new Engine(1, &s);
+  if (!e->ok) {
+    delete e;
+    return nullptr;
+  }
+  CellsHandle *h = new CellsHandle();
+  h->eng = e;
+  h->sim = s;
+  h->nz = e->nzones[0];
+  h->NQ = e->NQ;
+  h->prim.assign((size_t)h->NQ * h->nz, 0.0);
+  h->grad.assign((size_t)h->NQ * h->nz, 0.0);
+  h->gradp.assign((size_t)h->NQ * h->nz, 0.0);
+  h->tiph.assign(h->nz, 0.0);
+  h->dphi.assign(h->nz, 0.0);
+  h->wiph.assign(h->nz, 0.0);
+  h->stale = true;
+  return reinterpret_cast<Cell ***>(h);
+}
+
+void cell_destroy(Cell ***c, Sim *) {
+  CellsHandle *h = H(c);
+  if (!h) return;
+  delete h->eng;
+  delete[] h->refs;
+  delete h;
+}
+
+long long cell_nzones(Cell ***c) { return H(c)->nz; }
+
+Cell *cell_single(Cell ***c, int i, int j, int k) {
+  CellsHandle *h = H(c);
+  if (!h->refs) {
+    h->refs = new CellRef[h->nz];
+    for (long long z = 0; z < h->nz; ++z) h->refs[z] = CellRef{h, z};
+  }
+  return reinterpret_cast<Cell *>(&h->refs[zone_of(h, i, j, k)]);
+}
+long long cell_zone_index(Cell *c) { return reinterpret_cast<CellRef *>(c)->zone; }
+
+static CellRef *R(Cell *c) {
+  CellRef *r = reinterpret_cast<CellRef *>(c);
+  mirror_pull(r->h);
+  return r;
+}
+double cell_prim(Cell *c, int q) { CellRef *r = R(c); return r->h->prim[q * r->h->nz + r->zone]; }
+double cell_grad(Cell *c, int q) { CellRef *r = R(c); return r->h->grad[q * r->h->nz + r->zone]; }
+double cell_gradp(Cell *c, int q) { CellRef *r = R(c); return r->h->gradp[q * r->h->nz + r->zone]; }
+double cell_tiph(Cell *c) { CellRef *r = R(c); return r->h->tiph[r->zone]; }
+double cell_dphi(Cell *c) { CellRef *r = R(c); return r->h->dphi[r->zone]; }
+double cell_wiph(Cell *c) { CellRef *r = R(c); return r->h->wiph[r->zone]; }
+
+void cell_set_prim(Cell ***c, int i, int j, int k, int q, double v) {  // src/Cell/cell_modify_data.c:13-15
+  CellsHandle *h = H(c);
+  mirror_pull(h);
+  h->prim[q * h->nz + zone_of(h, i, j, k)] = v;
+  h->dirty_prim = true;
+}
+void cell_set_tiph(Cell ***c, int i, int j, int k, double v) {  // src/Cell/cell_modify_data.c:16-18
+  CellsHandle *h = H(c);
+  mirror_pull(h);
+  h->tiph[zone_of(h, i, j, k)] = v;
+  h->dirty_tiph = true;
+}
+
+void cell_upload(Cell ***c, int field, const double *src) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  h->eng->upload(0, field, src);
+  h->stale = true;
+}
+void cell_download(Cell ***c, int field, double *dst) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  h->eng->download(0, field, dst);
+}
+
+// src/Cell/cell_misc.c:12-26
+void cell_clean_pi(Cell ***c, Sim *) {
+  CellsHandle *h = H(c);
+  mirror_pull(h);
+  for (auto &phi : h->tiph) {
+    while (phi > 2.0 * M_PI) phi -= 2.0 * M_PI;
+    while (phi < 0.0) phi += 2.0 * M_PI;
+  }
+  h->dirty_tiph = true;
+  mirror_push(h);
+}
+void cell_syncproc_r(Cell ***, Sim *, MPIsetup *) {}  // one rank: no r neighbours
+// src/Cell/cell_sync.c:156-205 on one rank: periodic z copy of prim, RKcons, tiph
+void cell_syncproc_z(Cell ***c, Sim *s, MPIsetup *) {
+  CellsHandle *h = H(c);
+  if (s->N_global[1] <= 1) return;
+  mirror_pull(h);
+  mirror_push(h);
+  Engine *e = h->eng;
+  const DiskParams &P = e->hdp[0];
+  long long nz = h->nz, NL = P.NL;
+  std::vector<double> rk((size_t)h->NQ * nz);
+  e->download(0, 2, rk.data());
+  int NZ = P.NZ, kNm2g = NZ - 2 * P.NgMaxZ, kNmg = NZ - P.NgMaxZ, kg = P.NgMinZ;
+  auto cp = [&](int kd, int ks) {
+    for (long long x = 0; x < NL; ++x) {
+      for (int q = 0; q < h->NQ; ++q) {
+        h->prim[q * nz + kd * NL + x] = h->prim[q * nz + ks * NL + x];
+        rk[q * nz + kd * NL + x] = rk[q * nz + ks * NL + x];
+      }
+      h->tiph[kd * NL + x] = h->tiph[ks * NL + x];
+    }
+  };
+  for (int d = 0; d < P.NgMaxZ; ++d) cp(kNmg + d, kg + d);
+  for (int d = 0; d < P.NgMinZ; ++d) cp(d, kNm2g + d);
+  e->upload(0, 0, h->prim.data());
+  e->upload(0, 5, h->tiph.data());
+  e->upload(0, 2, rk.data());
+}
+// Conserved variables are always derived from the primitives on the device.
+void cell_calc_cons(Cell ***c, Sim *) { mirror_push(H(c)); }
+
+// src/Cell/cell_conversion.c:11-33
+void cell_prim2cons(double *prim, double *cons, double r, double dV, Sim *s) {
+  double G = s->GAMMALAW;
+  double rho = prim[RHO], Pp = prim[PPP], vr = prim[URR], vp = prim[UPP] * r, vz = prim[UZZ];
+  double v2 = vr * vr + vp * vp + vz * vz;
+  double rhoe = Pp / (G - 1.);
+  cons[DDD] = rho * dV;
+  cons[SRR] = rho * vr * dV;
+  cons[LLL] = r * rho * vp * dV;
+  cons[SZZ] = rho * vz * dV;
+  cons[TAU] = (.5 * rho * v2 + rhoe) * dV;
+  for (int q = s->NUM_C; q < s->NUM_Q; ++q) cons[q] = prim[q] * cons[DDD];
+}
+// src/Cell/cell_conversion.c:57-103
+void cell_cons2prim(double *cons, double *prim, double r, double dV, Sim *s) {
+  double G = s->GAMMALAW;
+  double rho = cons[DDD] / dV;
+  if (rho < s->RHO_FLOOR) rho = s->RHO_FLOOR;
+  double Sr = cons[SRR] / dV, Sp = cons[LLL] / dV / r, Sz = cons[SZZ] / dV, E = cons[TAU] / dV;
+  double vr = Sr / rho, vp = Sp / rho, vz = Sz / rho;
+  double KE = .5 * (Sr * vr + Sp * vp + Sz * vz);
+  double v_magnitude = sqrt(2. * KE / rho);
+  double rhoe = E - KE;
+  double Pp = (G - 1.) * rhoe;
+  if (Pp < s->CS_FLOOR * s->CS_FLOOR * rho / G) Pp = s->CS_FLOOR * s->CS_FLOOR * rho / G;
+  if (Pp > s->CS_CAP * s->CS_CAP * rho / G) Pp = s->CS_CAP * s->CS_CAP * rho / G;
+  if (v_magnitude > s->VEL_CAP) {
+    vr = vr * s->VEL_CAP / v_magnitude;
+    vp = vp * s->VEL_CAP / v_magnitude;
+    vz = vz * s->VEL_CAP / v_magnitude;
+  }
+  prim[RHO] = rho;
+  prim[PPP] = Pp;
+  prim[URR] = vr;
+  prim[UPP] = vp / r;
+  prim[UZZ] = vz;
+  for (int q = s->NUM_C; q < s->NUM_Q; ++q) prim[q] = cons[q] / cons[DDD];
+}
+// cell_calc_prim of cons = prim2cons(prim), src/Cell/cell_conversion.c:105-122
+void cell_calc_prim(Cell ***c, Sim *s) {
+  CellsHandle *h = H(c);
+  mirror_pull(h);
+  const DiskParams &P = h->eng->hdp[0];
+  for (int a = 0; a < P.NR * P.NZ; ++a) {
+    const AnnInfo &A = h->eng->hann[a];
+    double rp = A.rp, rm = A.rm, dz = A.zp - A.zm, r = .5 * (rp + rm);
+    for (int j = 0; j < A.N; ++j) {
+      long long z = A.zbase + j;
+      double p[8], u[8];
+      for (int q = 0; q < h->NQ; ++q) p[q] = h->prim[q * h->nz + z];
+      double dV = .5 * (rp * rp - rm * rm) * h->dphi[z] * dz;
+      cell_prim2cons(p, u, r, dV, s);
+      cell_cons2prim(u, p, r, dV, s);
+      for (int q = 0; q < h->NQ; ++q) h->prim[q * h->nz + z] = p[q];
+    }
+  }
+  h->dirty_prim = true;
+  mirror_push(h);
+}
+
+void cell_clear_w(Cell ***c, Sim *) {
+  CellsHandle *h = H(c);
+  hipMemsetAsync(h->eng->d_wiph, 0, sizeof(double) * h->eng->nann, h->eng->st);
+  h->stale = true;
+}
+void cell_set_w(Cell ***c, Sim *) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  launch_set_w(h->eng->mesh(), h->eng->plan, h->eng->st);
+  h->stale = true;
+}
+double cell_mindt(Cell ***c, Sim *, GravMass *gm) {  // src/Cell/cell_mindt.c:32-100
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  scalars_from_host(e, nullptr, gm);
+  launch_scalar(e->mesh(), e->plan, e->st, SC_RESET_DT);
+  launch_mindt(e->mesh(), e->plan, e->st, e->NQ);
+  DiskScalars s = e->get_scalars(0);
+  double dt;
+  memcpy(&dt, &s.dtmin_bits, 8);
+  return dt;
+}
+void cell_plm_p(Cell ***c, Sim *) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  launch_plm(h->eng->mesh(), h->eng->plan, h->eng->st, h->eng->NQ, -1);
+  h->stale = true;
+}
+void cell_plm_rz(Cell ***c, Sim *, Face *, TimeStep *, MPIsetup *, int direction) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  launch_plm(h->eng->mesh(), h->eng->plan, h->eng->st, h->eng->NQ, direction);
+  h->stale = true;
+}
+
+// ------------------------------------------------------------------ Face
+// face_create, src/Face/face_create_destroy.c:130-193, run on the device;
+// the host copy lists the faces in the reference's order (i outer, k inner
+// for r; k outer, i inner for z) and fills the TimeStep offsets nri / nzk.
+Face *face_create(Cell ***c, Sim *, TimeStep *ts, int direction) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  launch_faces(e->mesh(), e->plan, e->st, direction);
+  if (e->check_error()) return nullptr;
+  const DiskParams &P = e->hdp[0];
+  if (direction == 1 && P.NglobalZ <= 1) return nullptr;
+  std::vector<PairState> pst(e->hpairs.size());
+  hipMemcpy(pst.data(), e->d_pst, pst.size() * sizeof(PairState), hipMemcpyDeviceToHost);
+  std::vector<int> fL(e->NFtot), fR(e->NFtot);
+  std::vector<double> fd(e->NFtot), fc(e->NFtot);
+  hipMemcpy(fL.data(), e->d_fL, fL.size() * sizeof(int), hipMemcpyDeviceToHost);
+  hipMemcpy(fR.data(), e->d_fR, fR.size() * sizeof(int), hipMemcpyDeviceToHost);
+  hipMemcpy(fd.data(), e->d_fdphi, fd.size() * sizeof(double), hipMemcpyDeviceToHost);
+  hipMemcpy(fc.data(), e->d_fcm, fc.size() * sizeof(double), hipMemcpyDeviceToHost);
+  Face *F = new Face();
+  F->h = h;
+  F->dir = direction;
+  int NR = P.NR, NZ = P.NZ;
+  auto append = [&](int p) {
+    const PairInfo &Q = e->hpairs[p];
+    for (int n = 0; n < pst[p].count; ++n) {
+      long long g = Q.face_base + n;
+      F->L.push_back(fL[g]);
+      F->R.push_back(fR[g]);
+      F->r.push_back(Q.r);
+      F->dphi.push_back(fd[g]);
+      F->dA.push_back(Q.r * fd[g] * Q.deltaPerp);
+      F->deltaL.push_back(Q.deltaL);
+      F->deltaR.push_back(Q.deltaR);
+      F->cm.push_back(fc[g]);
+    }
+  };
+  if (direction == 0) {
+    if (ts) ts->nri.assign(NR, 0);
+    for (int i = 0; i < NR - 1; ++i) {
+      if (ts) ts->nri[i] = (int)F->L.size();
+      for (int k = 0; k < NZ; ++k) append(P.rpair_base + k * (NR - 1) + i);
+    }
+    if (ts) ts->nri[NR - 1] = (int)F->L.size();
+  } else {
+    if (ts) ts->nzk.assign(NZ, 0);
+    for (int k = 0; k < NZ - 1; ++k) {
+      if (ts) ts->nzk[k] = (int)F->L.size();
+      for (int i = 0; i < NR; ++i) append(P.zpair_base + k * NR + i);
+    }
+    if (ts) ts->nzk[NZ - 1] = (int)F->L.size();
+  }
+  return F;
+}
+void face_destroy(Face *f) { delete f; }
+int face_count(Face *f) { return f ? (int)f->L.size() : 0; }
+static Cell *ref_of(CellsHandle *h, long long z) {
+  if (!h->refs) {
+    h->refs = new CellRef[h->nz];
+    for (long long x = 0; x < h->nz; ++x) h->refs[x] = CellRef{h, x};
+  }
+  return reinterpret_cast<Cell *>(&h->refs[z]);
+}
+Cell *face_L_pointer(Face *f, int n) { return ref_of(f->h, f->L[n]); }
+Cell *face_R_pointer(Face *f, int n) { return ref_of(f->h, f->R[n]); }
+double face_deltaL(Face *f, int n) { return f->deltaL[n]; }
+double face_deltaR(Face *f, int n) { return f->deltaR[n]; }
+double face_cm(Face *f, int n) { return f->cm[n]; }
+double face_dA(Face *f, int n) { return f->dA[n]; }
+double face_r(Face *f, int n) { return f->r[n]; }
+
+// ------------------------------------------------------------------ TimeStep
+TimeStep *timestep_create(Sim *s) {  // src/TimeStep/timestep_create_destroy.c:7-13
+  TimeStep *t = new TimeStep();
+  t->nri.assign(sim_N(s, 0), 0);
+  t->nzk.assign(sim_N(s, 1), 0);
+  return t;
+}
+void timestep_destroy(TimeStep *t) { delete t; }
+double timestep_get_t(TimeStep *t) { return t->t; }
+double timestep_dt(TimeStep *t) { return t->dt; }
+void timestep_set_t(TimeStep *t, double v) { t->t = v; }
+void timestep_update_t(TimeStep *t) { t->t += t->dt; }
+void timestep_set_RK(TimeStep *t, double RK) { t->RK = RK; }
+void timestep_set_nri(TimeStep *t, int i, int n) { t->nri[i] = n; }
+void timestep_set_nzk(TimeStep *t, int k, int n) { t->nzk[k] = n; }
+int timestep_n(TimeStep *t, int index, int direction) {  // src/TimeStep/timestep_access.c:30-39
+  if (direction == 0) return t->nri[index];
+  if (direction == 1) return t->nzk[index];
+  disco_set_error(4, "timestep_n: bad direction");
+  return 0;
+}
+// src/TimeStep/timestep_set.c:16-21
+void timestep_set_dt(TimeStep *ts, Cell ***c, Sim *s, GravMass *gm) {
+  ts->dt = cell_mindt(c, s, gm);
+  if (ts->t + ts->dt > s->T_MAX) ts->dt = s->T_MAX - ts->t;
+}
+// One RK2 step on the device; t, dt and both masses come back to the host
+// objects at return, as the reference's loop reads t after every step.
+void timestep_rk2(TimeStep *ts, Sim *, Cell ***c, GravMass *gm, MPIsetup *) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  scalars_from_host(e, ts, gm);
+  e->enqueue_rk2();
+  scalars_to_host(e, ts, gm);
+  e->check_error();
+  h->stale = true;
+}
+void timestep_forward_euler(TimeStep *ts, Sim *, Cell ***c, GravMass *gm, MPIsetup *) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  scalars_from_host(e, ts, gm);
+  e->enqueue_euler();
+  scalars_to_host(e, ts, gm);
+  e->check_error();
+  h->stale = true;
+}
+// One substep with the RK snapshot already on the device (the reference
+// takes it with cell_copy before the first substep).
+void timestep_substep(TimeStep *ts, Cell ***c, Sim *, GravMass *gm, MPIsetup *, double fac) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  scalars_from_host(e, ts, gm);
+  e->enqueue_substep(fac, 0, 1);
+  scalars_to_host(e, ts, gm);
+  e->check_error();
+  h->stale = true;
+}
+
+// ------------------------------------------------------------------ batch
+struct DiscoBatch {
+  Engine *e;
+};
+DiscoBatch *disco_batch_create(int n, Sim **sims) {
+  for (int d = 0; d < n; ++d)
+    if (!sims[d]->allocated) {
+      disco_set_error(10, "batch disk without sim_alloc_arr");
+      return nullptr;
+    }
+  Engine *e = new Engine(n, sims);
+  if (!e->ok) {
+    delete e;
+    return nullptr;
+  }
+  return new DiscoBatch{e};
+}
+void disco_batch_destroy(DiscoBatch *b) {
+  if (!b) return;
+  delete b->e;
+  delete b;
+}
+long long disco_batch_nzones(DiscoBatch *b, int d) { return b->e->nzones[d]; }
+void disco_batch_upload(DiscoBatch *b, int d, int field, const double *src) { b->e->upload(d, field, src); }
+void disco_batch_download(DiscoBatch *b, int d, int field, double *dst) { b->e->download(d, field, dst); }
+void disco_batch_init(DiscoBatch *b) { b->e->init_state(); }
+void disco_batch_run(DiscoBatch *b, int nsteps, int graph) {
+  b->e->run(nsteps, graph != 0);
+}
+void disco_batch_synchronize(DiscoBatch *b) {
+  b->e->synchronize();
+  b->e->check_error();
+}
+double disco_batch_last_ms(DiscoBatch *b) { return b->e->last_ms; }
+void disco_batch_scalars(DiscoBatch *b, int d, double *o) {
+  DiskScalars s = b->e->get_scalars(d);
+  o[0] = s.t;
+  o[1] = s.dt;
+  o[2] = s.RK;
+  for (int p = 0; p < 2; ++p) {
+    double *g = o + 3 + 10 * p;
+    g[0] = s.gm[p].r; g[1] = s.gm[p].phi; g[2] = s.gm[p].M; g[3] = s.gm[p].omega;
+    g[4] = s.gm[p].RK_r; g[5] = s.gm[p].RK_phi; g[6] = s.gm[p].RK_M; g[7] = s.gm[p].RK_omega;
+    g[8] = s.gm[p].Mdot; g[9] = s.gm[p].Macc;
+  }
+}
+}

@@ -1,0 +1,81 @@
+// Types shared by the host engine and the HIP kernels.
+//
+// Device layout of a batch of D disks (a single simulation is D = 1):
+//   * zones of all disks are numbered consecutively; inside a disk,
+//     zone(k,i,j) = zone_base + k*NL + aoff[i] + j, NL = sum_i N_p[i]
+//     (the reference's cell order, src/Cell/cell_create_destroy.c:41-48);
+//   * every per-zone field is its own array (q-major for NQ-vectors);
+//   * an "annulus" is one (disk, layer k, ring i); a "pair" is two adjacent
+//     annuli that share overlap faces: (k,i)-(k,i+1) in r, (k,i)-(k+1,i) in z.
+#pragma once
+#include <stdint.h>
+
+#define DISCO_MAXQ 8
+#define DISCO_TILE 256
+
+enum { RHO = 0, PPP = 1, URR = 2, UPP = 3, UZZ = 4 };
+enum { DDD = 0, TAU = 1, SRR = 2, LLL = 3, SZZ = 4 };
+enum { C_FIXED, C_WCELL, C_RIGID, C_KEPLER, C_OMEGA20, C_MILOS, C_POWERLAW };
+enum { A_FIXED, A_KEPLER, A_OMEGA20, A_MILOS, A_POWERLAW };
+enum { BOUND_FIXED, BOUND_OUTFLOW, BOUND_PERIODIC };
+enum { HLLC_SOLVER = 0, HLL_SOLVER = 1 };
+
+// sticky device error codes
+enum {
+  DERR_NONE = 0,
+  DERR_FACE_OVERFLOW = 1,   // more overlap faces than the pair's capacity
+  DERR_VISC_VZ = 2,         // viscosity with |v_z| > 1e-12 (the reference exits)
+};
+
+struct DiskParams {
+  int NR, NZ, NQ, NC;         // annuli / layers incl. ghosts
+  int NgMinR, NgMaxR, NgMinZ, NgMaxZ;
+  int NglobalZ;
+  int ann_base;               // first annulus of this disk in the annulus table
+  int rpair_base, zpair_base; // first pair of this disk
+  int geo_r, geo_z;           // offsets into the r_faces / z_faces pools
+  long long zone_base;
+  int NL;                     // zones per layer
+  int w_numeric, w_analytic, riemann, visc_const, grav2d, sink, phi_order, cooling;
+  int initial_data_type, grav_type, bound_r, no_inner_bc;
+  double gamma, cfl, plm, visc, g_eps, rho_floor, cs_floor, cs_cap, vel_cap;
+  double orb_tscale, orb_t0, t_max, porho_r1;
+};
+
+// Per-disk scalar state living in device memory.
+struct GravState {
+  double r, phi, M, omega, RK_r, RK_phi, RK_M, RK_omega, Mdot, Macc;
+};
+struct DiskScalars {
+  double t, dt, RK, pad;
+  GravState gm[2];
+  unsigned long long dtmin_bits;  // CFL minimum of the step (as ordered bits)
+  int err, pad2;
+  double mdot_acc[2];
+};
+
+struct AnnInfo {
+  long long zbase;   // first zone of the annulus
+  int N;             // N_p
+  int disk, k, i;
+  int pad;
+  double rm, rp, zm, zp;  // face positions bounding the annulus
+};
+
+struct PairInfo {
+  int ann_in, ann_out;   // inner / outer annulus index
+  int disk, dir;         // dir 0: r, 1: z
+  long long face_base;   // first face slot
+  int cap;               // face capacity
+  int run_base;          // first run_start slot
+  int run_cap;
+  int pad;
+  double r, deltaL, deltaR, deltaPerp;  // the reference's build_jloop constants
+};
+
+struct PairState {   // written by the face builder each substep
+  int count;         // faces emitted
+  int jpmin;         // outer cell containing the inner annulus' start edge
+  int T;             // number of outer-cell advances (runs - 1)
+  int seq;           // 1 if the sequential fallback built this pair
+};

@@ -1,0 +1,500 @@
+// Engine construction, host<->device transfers and the step sequence.
+#include "engine.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/disco.h"
+
+bool hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  disco_set_error(20, std::string(what) + ": " + hipGetErrorString(e));
+  return false;
+}
+
+template <typename T>
+static bool dalloc(T **p, size_t n, const char *what) {
+  if (n == 0) n = 1;
+  return hip_check(hipMalloc((void **)p, n * sizeof(T)), what);
+}
+template <typename T>
+static bool h2d(T *d, const T *h, size_t n) {
+  if (n == 0) return true;
+  return hip_check(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
+  NQ = sims[0]->NUM_Q;
+  for (int d = 0; d < n; ++d) {
+    if (sims[d]->NUM_Q != NQ) {
+      disco_set_error(7, "all disks of a batch need the same NUM_Q");
+      return;
+    }
+  }
+  if (NQ != 5 && NQ != 6) {
+    disco_set_error(7, "NUM_Q must be 5 or 6 (NUM_C 5 with at most one passive scalar)");
+    return;
+  }
+  std::vector<double> hrf, hzf;
+  std::vector<int2> ztiles, itiles, rft, zft, bc0, bc1, szt;
+  long long zb = 0, fb = 0;
+  int rb = 0;
+  // r pairs of all disks first, then z pairs
+  std::vector<PairInfo> zpairs;
+  for (int d = 0; d < n; ++d) {
+    Sim *S = sims[d];
+    if (S->NUM_C != 5) {
+      disco_set_error(7, "NUM_C must be 5");
+      return;
+    }
+    if (S->BoundTypeR == BOUND_FIXED || (S->N_global[1] > 1 && S->BoundTypeZ == BOUND_FIXED) ||
+        S->DAMP_TIME > 0.0) {
+      disco_set_error(8, "fixed boundaries and damping need an initial-data plugin: not supported");
+      return;
+    }
+    if (S->W_NUMERIC_TYPE == C_WCELL) {
+      disco_set_error(8, "w_numeric_type 1 (C_WCELL) is rejected by the reference too");
+      return;
+    }
+    DiskParams P;
+    memset(&P, 0, sizeof(P));
+    sim_param_arrays(S, &P);
+    int NR = P.NR, NZ = P.NZ;
+    if (NR < 4) {
+      disco_set_error(8, "need at least one interior annulus");
+      return;
+    }
+    for (int i = 0; i < NR; ++i)
+      if (S->N_p[i] < 2) {
+        disco_set_error(8, "every annulus needs N_p >= 2 (check that R_Min exceeds the radial spacing)");
+        return;
+      }
+    P.geo_r = (int)hrf.size();
+    P.geo_z = (int)hzf.size();
+    hrf.insert(hrf.end(), S->r_faces.begin(), S->r_faces.end());
+    hzf.insert(hzf.end(), S->z_faces.begin(), S->z_faces.end());
+    long long NL = 0;
+    std::vector<long long> aoff(NR + 1, 0);
+    for (int i = 0; i < NR; ++i) aoff[i + 1] = aoff[i] + S->N_p[i];
+    NL = aoff[NR];
+    P.NL = (int)NL;
+    P.zone_base = zb;
+    P.ann_base = (int)hann.size();
+    zbase.push_back(zb);
+    nzones.push_back(NL * NZ);
+    for (int k = 0; k < NZ; ++k)
+      for (int i = 0; i < NR; ++i) {
+        AnnInfo A;
+        memset(&A, 0, sizeof(A));
+        A.zbase = zb + k * NL + aoff[i];
+        A.N = S->N_p[i];
+        A.disk = d;
+        A.k = k;
+        A.i = i;
+        A.rm = S->r_faces[i];
+        A.rp = S->r_faces[i + 1];
+        A.zm = S->z_faces[k];
+        A.zp = S->z_faces[k + 1];
+        int a = (int)hann.size();
+        hann.push_back(A);
+        for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) ztiles.push_back(make_int2(a, j0));
+        bool interior = i >= P.NgMinR && i < NR - P.NgMaxR && k >= P.NgMinZ && k < NZ - P.NgMaxZ;
+        if (interior)
+          for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) itiles.push_back(make_int2(a, j0));
+        if (P.bound_r == BOUND_OUTFLOW) {
+          if ((i == 0 && P.no_inner_bc != 1) || i == NR - 2)
+            for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) bc0.push_back(make_int2(a, j0));
+          if (i == NR - 1)
+            for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) bc1.push_back(make_int2(a, j0));
+        }
+        if (P.NglobalZ > 1 && (k < P.NgMinZ || k >= NZ - P.NgMaxZ))
+          for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) szt.push_back(make_int2(a, j0));
+      }
+    zb += NL * NZ;
+    // r pairs (k, i)-(k, i+1): constants of build_jloop, src/Face/face_create_destroy.c:60-68
+    P.rpair_base = (int)hpairs.size();
+    for (int k = 0; k < NZ; ++k)
+      for (int i = 0; i < NR - 1; ++i) {
+        PairInfo Q;
+        memset(&Q, 0, sizeof(Q));
+        Q.ann_in = P.ann_base + k * NR + i;
+        Q.ann_out = P.ann_base + k * NR + i + 1;
+        Q.disk = d;
+        Q.dir = 0;
+        int Nin = S->N_p[i], Nout = S->N_p[i + 1];
+        Q.cap = Nin + 2 * Nout + 4;
+        Q.run_cap = 2 * Nout + 4;
+        Q.deltaL = .5 * (sim_FacePos(S, i, 0) - sim_FacePos(S, i - 1, 0));
+        Q.deltaR = .5 * (sim_FacePos(S, i + 1, 0) - sim_FacePos(S, i, 0));
+        Q.r = sim_FacePos(S, i, 0);
+        double zp = sim_FacePos(S, k, 1), zm = sim_FacePos(S, k - 1, 1);
+        Q.deltaPerp = zp - zm;
+        hpairs.push_back(Q);
+      }
+    // z pairs (k, i)-(k+1, i): src/Face/face_create_destroy.c:70-78
+    if (P.NglobalZ > 1) {
+      P.zpair_base = (int)zpairs.size();  // rebased below
+      for (int k = 0; k < NZ - 1; ++k)
+        for (int i = 0; i < NR; ++i) {
+          PairInfo Q;
+          memset(&Q, 0, sizeof(Q));
+          Q.ann_in = P.ann_base + k * NR + i;
+          Q.ann_out = P.ann_base + (k + 1) * NR + i;
+          Q.disk = d;
+          Q.dir = 1;
+          int N = S->N_p[i];
+          Q.cap = 3 * N + 4;
+          Q.run_cap = 2 * N + 4;
+          Q.deltaL = .5 * (sim_FacePos(S, k, 1) - sim_FacePos(S, k - 1, 1));
+          Q.deltaR = .5 * (sim_FacePos(S, k + 1, 1) - sim_FacePos(S, k, 1));
+          double rp = sim_FacePos(S, i, 0), rm = sim_FacePos(S, i - 1, 0);
+          Q.deltaPerp = rp - rm;
+          Q.r = .5 * (rp + rm);
+          zpairs.push_back(Q);
+        }
+    }
+    hdp.push_back(P);
+  }
+  n_rpairs = (int)hpairs.size();
+  n_zpairs = (int)zpairs.size();
+  for (auto &P : hdp)
+    if (P.NglobalZ > 1) P.zpair_base += n_rpairs;
+  hpairs.insert(hpairs.end(), zpairs.begin(), zpairs.end());
+  for (int p = 0; p < (int)hpairs.size(); ++p) {
+    PairInfo &Q = hpairs[p];
+    Q.face_base = fb;
+    Q.run_base = rb;
+    fb += Q.cap;
+    rb += Q.run_cap;
+    for (int f0 = 0; f0 < Q.cap; f0 += DISCO_TILE) (Q.dir == 0 ? rft : zft).push_back(make_int2(p, f0));
+  }
+  Ntot = zb;
+  NFtot = fb;
+  nann = (int)hann.size();
+
+  bool good = true;
+  good &= hip_check(hipStreamCreate(&st), "hipStreamCreate");
+  good &= hip_check(hipEventCreate(&ev0), "hipEventCreate") && hip_check(hipEventCreate(&ev1), "hipEventCreate");
+  good &= dalloc(&d_dp, hdp.size(), "dp") && h2d(d_dp, hdp.data(), hdp.size());
+  good &= dalloc(&d_sc, (size_t)n, "scalars");
+  good &= dalloc(&d_ann, hann.size(), "ann") && h2d(d_ann, hann.data(), hann.size());
+  good &= dalloc(&d_pairs, hpairs.size(), "pairs") && h2d(d_pairs, hpairs.data(), hpairs.size());
+  good &= dalloc(&d_pst, hpairs.size(), "pair state");
+  good &= dalloc(&d_rf, hrf.size(), "rf") && h2d(d_rf, hrf.data(), hrf.size());
+  good &= dalloc(&d_zf, hzf.size(), "zf") && h2d(d_zf, hzf.data(), hzf.size());
+  size_t nq = (size_t)NQ * Ntot;
+  good &= dalloc(&d_prim, nq, "prim") && dalloc(&d_RKcons, nq, "RKcons");
+  good &= dalloc(&d_grad, nq, "grad") && dalloc(&d_gradp, nq, "gradp");
+  good &= dalloc(&d_tiph[0], Ntot, "tiph") && dalloc(&d_tiph[1], Ntot, "tiph");
+  good &= dalloc(&d_RKtiph, Ntot, "RKtiph") && dalloc(&d_dphi, Ntot, "dphi");
+  good &= dalloc(&d_wiph, (size_t)nann, "wiph");
+  good &= dalloc(&d_fL, (size_t)NFtot, "fL") && dalloc(&d_fR, (size_t)NFtot, "fR");
+  good &= dalloc(&d_fdphi, (size_t)NFtot, "fdphi") && dalloc(&d_fcm, (size_t)NFtot, "fcm");
+  good &= dalloc(&d_fo_r, Ntot, "fo_r") && dalloc(&d_fo_z, Ntot, "fo_z");
+  good &= dalloc(&d_run, (size_t)(rb > 0 ? rb : 1), "run_start");
+  good &= dalloc(&d_js, Ntot, "js");
+  good &= dalloc(&d_Fphi, (size_t)(NQ + 3) * Ntot, "Fphi");
+  good &= dalloc(&d_Fface, (size_t)(NQ + 3) * NFtot, "Fface");
+  good &= dalloc(&d_err, 1, "err");
+  if (!good) return;
+  hipMemset(d_err, 0, sizeof(int));
+  hipMemset(d_prim, 0, nq * sizeof(double));
+  hipMemset(d_RKcons, 0, nq * sizeof(double));
+  hipMemset(d_grad, 0, nq * sizeof(double));
+  hipMemset(d_gradp, 0, nq * sizeof(double));
+  hipMemset(d_wiph, 0, (size_t)nann * sizeof(double));
+  {  // no faces yet: every gather over a pair is empty until k_faces runs
+    std::vector<PairState> p0(hpairs.size());
+    for (auto &x : p0) x = PairState{0, 0, -1, 0};
+    h2d(d_pst, p0.data(), p0.size());
+  }
+  hipMemset(d_fo_r, 0, Ntot * sizeof(int));
+  hipMemset(d_fo_z, 0, Ntot * sizeof(int));
+  hipMemset(d_run, 0, (size_t)(rb > 0 ? rb : 1) * sizeof(int));
+  std::vector<DiskScalars> sc0(n);
+  memset(sc0.data(), 0, sizeof(DiskScalars) * n);
+  for (auto &s : sc0) {
+    double big = 1.e100;
+    memcpy(&s.dtmin_bits, &big, 8);
+  }
+  h2d(d_sc, sc0.data(), (size_t)n);
+
+  // one tile array on the device, sliced per list
+  std::vector<int2> all;
+  auto add = [&](std::vector<int2> &v, int &count) {
+    count = (int)v.size();
+    size_t off = all.size();
+    all.insert(all.end(), v.begin(), v.end());
+    return off;
+  };
+  size_t o_z = add(ztiles, plan.n_ztiles), o_i = add(itiles, plan.n_itiles), o_r = add(rft, plan.n_rftiles),
+         o_zf = add(zft, plan.n_zftiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
+         o_s = add(szt, plan.n_sz);
+  if (!dalloc(&d_tiles, all.size(), "tiles") || !h2d(d_tiles, all.data(), all.size())) return;
+  plan.ztiles = d_tiles + o_z;
+  plan.itiles = d_tiles + o_i;
+  plan.rftiles = d_tiles + o_r;
+  plan.zftiles = d_tiles + o_zf;
+  plan.bc0 = d_tiles + o_b0;
+  plan.bc1 = d_tiles + o_b1;
+  plan.sz = d_tiles + o_s;
+  plan.ndisk = n;
+  plan.nann = nann;
+  plan.n_rpairs = n_rpairs;
+  plan.n_zpairs = n_zpairs;
+  // initial mesh edges: cell_create + cell_clean_pi + tiph sync (src/main.c:77-85)
+  for (int d = 0; d < n; ++d) {
+    std::vector<double> t, dp;
+    disco_cell_offsets(sims[d], t, dp);
+    upload(d, 5, t.data());
+    upload(d, 7, dp.data());
+    upload(d, 6, t.data());
+  }
+  ok = hip_check(hipDeviceSynchronize(), "engine set-up");
+}
+
+Engine::~Engine() {
+  if (gexec) hipGraphExecDestroy(gexec);
+  if (graph) hipGraphDestroy(graph);
+  void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim, d_RKcons, d_grad, d_gradp,
+                  d_tiph[0], d_tiph[1], d_RKtiph, d_dphi, d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
+                  d_fo_z, d_run, d_js, d_Fphi, d_Fface, d_err, d_tiles};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (ev0) hipEventDestroy(ev0);
+  if (ev1) hipEventDestroy(ev1);
+  if (st) hipStreamDestroy(st);
+}
+
+DevMesh Engine::mesh() const {
+  DevMesh m;
+  m.dp = d_dp;
+  m.sc = d_sc;
+  m.ann = d_ann;
+  m.pairs = d_pairs;
+  m.pst = d_pst;
+  m.rf = d_rf;
+  m.zf = d_zf;
+  m.prim = d_prim;
+  m.RKcons = d_RKcons;
+  m.grad = d_grad;
+  m.gradp = d_gradp;
+  m.tiph = d_tiph[cur];
+  m.tiph_new = d_tiph[1 - cur];
+  m.RKtiph = d_RKtiph;
+  m.dphi = d_dphi;
+  m.wiph = d_wiph;
+  m.fL = d_fL;
+  m.fR = d_fR;
+  m.fdphi = d_fdphi;
+  m.fcm = d_fcm;
+  m.fo_r = d_fo_r;
+  m.fo_z = d_fo_z;
+  m.run_start = d_run;
+  m.js = d_js;
+  m.Fphi = d_Fphi;
+  m.Fface = d_Fface;
+  m.Ntot = Ntot;
+  m.NFtot = NFtot;
+  m.err = d_err;
+  return m;
+}
+
+// field ids: 0 prim, 1 cons (download only), 2 RKcons, 3 grad, 4 gradp,
+// 5 tiph, 6 RKtiph, 7 dphi, 8 wiph (download only, expanded per zone)
+void Engine::upload(int d, int field, const double *src) {
+  long long nz = nzones[d], z0 = zbase[d];
+  auto put = [&](double *base, int nq) {
+    for (int q = 0; q < nq; ++q)
+      hip_check(hipMemcpy(base + q * Ntot + z0, src + q * nz, nz * sizeof(double), hipMemcpyHostToDevice),
+                "upload");
+  };
+  switch (field) {
+    case 0: put(d_prim, NQ); break;
+    case 2: put(d_RKcons, NQ); break;
+    case 3: put(d_grad, NQ); break;
+    case 4: put(d_gradp, NQ); break;
+    case 5: put(d_tiph[cur], 1); break;
+    case 6: put(d_RKtiph, 1); break;
+    case 7: put(d_dphi, 1); break;
+    default: disco_set_error(9, "field cannot be uploaded");
+  }
+}
+
+void Engine::download(int d, int field, double *dst) {
+  long long nz = nzones[d], z0 = zbase[d];
+  hip_check(hipStreamSynchronize(st), "synchronize");
+  auto get = [&](const double *base, int nq, double *out) {
+    for (int q = 0; q < nq; ++q)
+      hip_check(hipMemcpy(out + q * nz, base + q * Ntot + z0, nz * sizeof(double), hipMemcpyDeviceToHost),
+                "download");
+  };
+  switch (field) {
+    case 0: get(d_prim, NQ, dst); break;
+    case 2: get(d_RKcons, NQ, dst); break;
+    case 3: get(d_grad, NQ, dst); break;
+    case 4: get(d_gradp, NQ, dst); break;
+    case 5: get(d_tiph[cur], 1, dst); break;
+    case 6: get(d_RKtiph, 1, dst); break;
+    case 7: get(d_dphi, 1, dst); break;
+    case 8: {
+      std::vector<double> w(nann);
+      hip_check(hipMemcpy(w.data(), d_wiph, nann * sizeof(double), hipMemcpyDeviceToHost), "download");
+      long long o = 0;
+      for (int a = hdp[d].ann_base; a < hdp[d].ann_base + hdp[d].NR * hdp[d].NZ; ++a)
+        for (int j = 0; j < hann[a].N; ++j) dst[o++] = w[a];
+      break;
+    }
+    case 1: {  // prim2cons with cell_calc_cons volumes (src/Cell/cell_conversion.c:35-55)
+      std::vector<double> pr((size_t)NQ * nz), dp(nz);
+      get(d_prim, NQ, pr.data());
+      get(d_dphi, 1, dp.data());
+      Sim *S = sims[d];
+      const DiskParams &P = hdp[d];
+      for (int a = P.ann_base; a < P.ann_base + P.NR * P.NZ; ++a) {
+        const AnnInfo &A = hann[a];
+        double rp = A.rp, rm = A.rm, dz = A.zp - A.zm, r = .5 * (rp + rm);
+        for (int j = 0; j < A.N; ++j) {
+          long long z = A.zbase - z0 + j;
+          double p[8], c[8];
+          for (int q = 0; q < NQ; ++q) p[q] = pr[q * nz + z];
+          double dV = .5 * (rp * rp - rm * rm) * dp[z] * dz;
+          cell_prim2cons(p, c, r, dV, S);
+          for (int q = 0; q < NQ; ++q) dst[q * nz + z] = c[q];
+        }
+      }
+      break;
+    }
+    default: disco_set_error(9, "unknown field");
+  }
+}
+
+void Engine::put_scalars(int d, const DiskScalars &s) {
+  hip_check(hipMemcpy(d_sc + d, &s, sizeof(s), hipMemcpyHostToDevice), "scalars H2D");
+}
+DiskScalars Engine::get_scalars(int d) {
+  DiskScalars s;
+  hip_check(hipStreamSynchronize(st), "synchronize");
+  hip_check(hipMemcpy(&s, d_sc + d, sizeof(s), hipMemcpyDeviceToHost), "scalars D2H");
+  return s;
+}
+
+// src/main.c:103-119: gravMass init, clean_pi, z sync of the initial
+// primitives and edges; conserved variables are derived on the fly.
+void Engine::init_state() {
+  for (int d = 0; d < ndisk; ++d) {
+    Sim *S = sims[d];
+    GravMass g[2];
+    memset(g, 0, sizeof(g));
+    (*gravMass_init_ptr(S))(g, S);
+    gravMass_clean_pi(g, S);
+    DiskScalars s;
+    memset(&s, 0, sizeof(s));
+    for (int p = 0; p < 2; ++p) {
+      s.gm[p].r = g[p].r;
+      s.gm[p].phi = g[p].phi;
+      s.gm[p].M = g[p].M;
+      s.gm[p].omega = g[p].omega;
+      s.gm[p].Mdot = g[p].Mdot;
+      s.gm[p].Macc = g[p].Macc;
+    }
+    double big = 1.e100;
+    memcpy(&s.dtmin_bits, &big, 8);
+    put_scalars(d, s);
+    const DiskParams &P = hdp[d];
+    if (P.NglobalZ > 1) {  // src/Cell/cell_sync.c:156-205 on one rank
+      long long nz = nzones[d], NL = P.NL;
+      std::vector<double> pr((size_t)NQ * nz), t(nz), rk((size_t)NQ * nz);
+      download(d, 0, pr.data());
+      download(d, 5, t.data());
+      download(d, 2, rk.data());
+      int NZ = P.NZ, kNm2g = NZ - 2 * P.NgMaxZ, kNmg = NZ - P.NgMaxZ, kg = P.NgMinZ;
+      auto cp = [&](int kd, int ks) {
+        for (long long x = 0; x < NL; ++x) {
+          for (int q = 0; q < NQ; ++q) {
+            pr[q * nz + kd * NL + x] = pr[q * nz + ks * NL + x];
+            rk[q * nz + kd * NL + x] = rk[q * nz + ks * NL + x];
+          }
+          t[kd * NL + x] = t[ks * NL + x];
+        }
+      };
+      for (int dd = 0; dd < P.NgMaxZ; ++dd) cp(kNmg + dd, kg + dd);
+      for (int dd = 0; dd < P.NgMinZ; ++dd) cp(dd, kNm2g + dd);
+      upload(d, 0, pr.data());
+      upload(d, 5, t.data());
+      upload(d, 2, rk.data());
+    }
+  }
+  hip_check(hipDeviceSynchronize(), "init");
+}
+
+void Engine::enqueue_substep(double fac, int copy_first, int recompute_w) {
+  launch_substep(mesh(), plan, st, NQ, fac, copy_first, recompute_w);
+  cur = 1 - cur;
+}
+
+// timestep_rk2, src/TimeStep/timestep_algorithm.c:11-41
+void Engine::enqueue_rk2() {
+  DevMesh m = mesh();
+  launch_set_w(m, plan, st);
+  launch_mindt(m, plan, st, NQ);
+  launch_scalar(m, plan, st, SC_STEP_BEGIN);
+  enqueue_substep(1.0, 1, 0);
+  launch_scalar(mesh(), plan, st, SC_SUB1_END);
+  enqueue_substep(0.5, 0, 1);
+  launch_scalar(mesh(), plan, st, SC_SUB2_END);
+}
+
+// timestep_forward_euler, src/TimeStep/timestep_algorithm.c:43-57
+void Engine::enqueue_euler() {
+  DevMesh m = mesh();
+  launch_set_w(m, plan, st);
+  launch_mindt(m, plan, st, NQ);
+  launch_scalar(m, plan, st, SC_STEP_BEGIN);
+  enqueue_substep(1.0, 1, 0);
+  launch_scalar(mesh(), plan, st, SC_EULER_END);
+}
+
+void Engine::run(int nsteps, bool use_graph) {
+  if (!ok) return;
+  if (use_graph && (gexec == nullptr || graph_cur != cur)) {
+    if (gexec) {
+      hipGraphExecDestroy(gexec);
+      hipGraphDestroy(graph);
+      gexec = nullptr;
+    }
+    int cur0 = cur;
+    if (!hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture")) return;
+    enqueue_rk2();
+    if (!hip_check(hipStreamEndCapture(st, &graph), "capture end")) return;
+    if (!hip_check(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "graph instantiate")) return;
+    cur = cur0;  // capture does not execute; a step swaps the edge buffers twice
+    graph_cur = cur;
+  }
+  hip_check(hipEventRecord(ev0, st), "event");
+  for (int s = 0; s < nsteps; ++s) {
+    if (use_graph)
+      hip_check(hipGraphLaunch(gexec, st), "graph launch");
+    else
+      enqueue_rk2();
+  }
+  hip_check(hipEventRecord(ev1, st), "event");
+}
+
+void Engine::synchronize() {
+  hip_check(hipStreamSynchronize(st), "synchronize");
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess) last_ms = ms;
+}
+
+int Engine::check_error() {
+  int e = 0;
+  hip_check(hipStreamSynchronize(st), "synchronize");
+  hip_check(hipMemcpy(&e, d_err, sizeof(int), hipMemcpyDeviceToHost), "err");
+  if (e == DERR_FACE_OVERFLOW) disco_set_error(30, "overlap faces exceed a pair's capacity (diverged mesh?)");
+  if (e == DERR_VISC_VZ)
+    disco_set_error(31, "viscosity needs v_z = 0 (the reference exits in riemann_visc_flux)");
+  return e;
+}

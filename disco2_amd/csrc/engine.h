@@ -1,0 +1,65 @@
+// Engine: device-resident state of a batch of disks and the launch sequence
+// of timestep_rk2 (src/TimeStep/timestep_algorithm.c:11-41).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "disco_internal.h"
+#include "kernels.h"
+
+struct Engine {
+  int ndisk = 0, NQ = 5;
+  std::vector<Sim *> sims;
+  std::vector<DiskParams> hdp;
+  std::vector<AnnInfo> hann;
+  std::vector<PairInfo> hpairs;
+  std::vector<long long> zbase, nzones;
+  long long Ntot = 0, NFtot = 0;
+  int nann = 0, n_rpairs = 0, n_zpairs = 0;
+
+  // device allocations
+  DiskParams *d_dp = nullptr;
+  DiskScalars *d_sc = nullptr;
+  AnnInfo *d_ann = nullptr;
+  PairInfo *d_pairs = nullptr;
+  PairState *d_pst = nullptr;
+  double *d_rf = nullptr, *d_zf = nullptr;
+  double *d_prim = nullptr, *d_RKcons = nullptr, *d_grad = nullptr, *d_gradp = nullptr;
+  double *d_tiph[2] = {nullptr, nullptr};
+  double *d_RKtiph = nullptr, *d_dphi = nullptr, *d_wiph = nullptr;
+  int *d_fL = nullptr, *d_fR = nullptr;
+  double *d_fdphi = nullptr, *d_fcm = nullptr;
+  int *d_fo_r = nullptr, *d_fo_z = nullptr, *d_run = nullptr;
+  int4 *d_js = nullptr;
+  double *d_Fphi = nullptr, *d_Fface = nullptr;
+  int *d_err = nullptr;
+  int2 *d_tiles = nullptr;
+  int cur = 0;  // index of the tiph buffer holding the current edges
+
+  LaunchPlan plan{};
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_cur = -1;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  bool ok = false;
+
+  Engine(int n, Sim **sims);
+  ~Engine();
+  DevMesh mesh() const;
+  void upload(int disk, int field, const double *src);
+  void download(int disk, int field, double *dst);
+  void put_scalars(int disk, const DiskScalars &s);
+  DiskScalars get_scalars(int disk);
+  void init_state();                 // gravMass init, z sync of prim/tiph, scalars
+  void enqueue_rk2();
+  void enqueue_euler();
+  void enqueue_substep(double fac, int copy_first, int recompute_w);
+  void run(int nsteps, bool use_graph);
+  void synchronize();
+  int check_error();                 // 0, or sets the sticky library error
+};
+
+bool hip_check(hipError_t e, const char *what);

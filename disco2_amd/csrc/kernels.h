@@ -1,0 +1,50 @@
+// Device-side view of a batch and the host-side launchers of kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "disco_types.h"
+
+struct DevMesh {
+  const DiskParams *dp;
+  DiskScalars *sc;
+  const AnnInfo *ann;
+  const PairInfo *pairs;
+  PairState *pst;
+  const double *rf, *zf;  // face-position pools (per-disk offsets geo_r / geo_z)
+  double *prim, *RKcons, *grad, *gradp;
+  const double *tiph;     // cell edges at the start of the substep
+  double *tiph_new;       // cell edges after the substep
+  double *RKtiph, *dphi;
+  double *wiph;           // per annulus
+  int *fL, *fR;
+  double *fdphi, *fcm;
+  int *fo_r, *fo_z;       // per zone: first face of the cell as inner cell of its r / z pair
+  int *run_start;
+  int4 *js;               // face-builder scratch, per zone
+  double *Fphi;           // [NQ+3][Ntot] increments of face (j, j+1)
+  double *Fface;          // [NQ+3][NFtot] increments of the r/z overlap faces
+  long long Ntot, NFtot;
+  int *err;
+};
+
+struct LaunchPlan {
+  int ndisk, nann;
+  int n_rpairs, n_zpairs;
+  const int2 *ztiles; int n_ztiles;     // every zone
+  const int2 *itiles; int n_itiles;     // interior zones (CFL)
+  const int2 *rftiles; int n_rftiles;   // r faces
+  const int2 *zftiles; int n_zftiles;   // z faces
+  const int2 *bc0; int n_bc0;           // ghost annuli 0 and NR-2
+  const int2 *bc1; int n_bc1;           // ghost annulus NR-1
+  const int2 *sz; int n_sz;             // z ghost layers
+};
+
+enum { SC_STEP_BEGIN, SC_SUB_BEGIN, SC_AFTER_UPDATE, SC_SUB1_END, SC_SUB2_END, SC_EULER_END, SC_RESET_DT };
+
+void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
+                    int recompute_w);
+void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op);
+void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st);
+void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ);
+void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir);
+void launch_plm(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int which);

@@ -1,0 +1,1125 @@
+// HIP kernels of the hydro substep for gfx950 (FP64).
+//
+// Every per-zone kernel runs on "zone tiles": one workgroup owns up to
+// DISCO_TILE consecutive cells of one annulus, so loads along phi are
+// coalesced and the annulus constants are uniform in the workgroup.
+// Per-cell sums are gathers in the reference's accumulation order (never
+// atomics), which keeps the results deterministic and bitwise comparable.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "physics.h"
+
+#define TILE DISCO_TILE
+
+__device__ __forceinline__ void set_err(int *err, int code) { atomicCAS(err, 0, code); }
+
+__device__ __forceinline__ const DiskParams &disk_of(const DevMesh &m, int d) { return m.dp[d]; }
+
+__device__ __forceinline__ int rpair_of(const DiskParams &P, int k, int i) { return P.rpair_base + k * (P.NR - 1) + i; }
+__device__ __forceinline__ int zpair_of(const DiskParams &P, int k, int i) { return P.zpair_base + k * P.NR + i; }
+
+// ----------------------------------------------------------------------------
+// Overlap faces between two adjacent annuli: build_jloop and face_create,
+// src/Face/face_create_destroy.c:22-193.
+//
+// The reference walks the inner annulus sequentially, carrying the index jp
+// of the outer cell that contains the current inner cell's left edge.  Here
+// every inner cell j guesses its starting jp (the outer cell containing its
+// left edge), runs the reference's per-cell step from that guess, and the
+// guesses are verified against the previous cell's final jp.  When every
+// guess matches, the result equals the sequential walk by induction; when
+// one does not (ties between edges), one thread re-runs the sequential walk.
+// Faces are stored per pair: L/R zone, width dphi and centre cm.  A face of
+// inner cell j that advances jp starts a new "run" of the outer cell; run t
+// of the pair begins at run_start[t], and the outer cell c owns runs
+// t = (c - jpmin) mod N_out (+ N_out ...), which gives the R-side gathers
+// the reference's accumulation order.
+// ----------------------------------------------------------------------------
+struct FaceOut {
+  int *fL, *fR, *run_start;
+  double *fdp, *fcm;
+  long long fbase;
+  int rbase, cap, run_cap;
+  long long zin, zout;
+  int *err;
+};
+
+template <bool EMIT>
+__device__ int jloop_cell(int j, int jps, const double *Tin, const double *Din, const double *Tout,
+                          const double *Dout, int Nin, int Nout, int foff, const FaceOut &o, int *jpe,
+                          int *bad) {
+  int jm = j - 1;
+  if (jm < 0) jm = Nin - 1;
+  int jp = jps;
+  int n = 0;
+  auto put = [&](int R, double dph, double tp) {
+    if (EMIT) {
+      int f = foff + n;
+      if (f < o.cap) {
+        long long g = o.fbase + f;
+        o.fL[g] = (int)(o.zin + j);
+        o.fR[g] = (int)(o.zout + R);
+        o.fdp[g] = dph;
+        o.fcm[g] = tp - .5 * dph;  // addFace, :11-20
+      } else {
+        set_err(o.err, DERR_FACE_OVERFLOW);
+      }
+      if (n >= 1) {
+        int t = foff - j + n;
+        if (t < o.run_cap)
+          o.run_start[o.rbase + t] = f;
+        else
+          set_err(o.err, DERR_FACE_OVERFLOW);
+      }
+    }
+    ++n;
+  };
+  double dp = wrap_p(Tout[jp] - Tin[jm]);
+  if (Din[j] < dp) {
+    put(jp, Din[j], Tin[j]);
+  } else {
+    put(jp, dp, Tout[jp]);  // step A
+    ++jp;
+    if (jp == Nout) jp = 0;
+    dp = wrap_pm(Tin[j] - Tout[jp]);
+    int guard = 0;
+    while (dp > 0.0) {  // step B
+      if (++guard > Nout) {
+        *bad = 1;
+        break;
+      }
+      put(jp, Dout[jp], Tout[jp]);
+      ++jp;
+      if (jp == Nout) jp = 0;
+      dp = wrap_pm(Tin[j] - Tout[jp]);
+    }
+    dp = Dout[jp] + dp;
+    put(jp, dp, Tin[j]);  // step C
+  }
+  *jpe = jp;
+  return n;
+}
+
+// outer cell whose right edge is the first one at or after angle e
+__device__ int guess_outer(double e, const double *Tout, int Nout) {
+  double x = wrap_p(e - Tout[Nout - 1]);
+  if (!(x >= 0.0)) x = 0.0;
+  int c = (int)(x * (double)Nout / TWO_PI);
+  if (c < 0) c = 0;
+  if (c >= Nout) c = Nout - 1;
+  for (int it = 0; it <= Nout && wrap_pm(e - Tout[c]) > 0.0; ++it) c = (c + 1 == Nout) ? 0 : c + 1;
+  for (int it = 0; it <= Nout; ++it) {
+    int cm = (c == 0) ? Nout - 1 : c - 1;
+    if (wrap_pm(e - Tout[cm]) <= 0.0)
+      c = cm;
+    else
+      break;
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(TILE) void k_faces(DevMesh m, int pair0) {
+  const int tid = threadIdx.x;
+  const int p = pair0 + blockIdx.x;
+  const PairInfo P = m.pairs[p];
+  const AnnInfo A = m.ann[P.ann_in];
+  const AnnInfo B = m.ann[P.ann_out];
+  const int Nin = A.N, Nout = B.N;
+  const double *Tin = m.tiph + A.zbase, *Din = m.dphi + A.zbase;
+  const double *Tout = m.tiph + B.zbase, *Dout = m.dphi + B.zbase;
+  __shared__ double s_d[TILE];
+  __shared__ int s_j[TILE];
+  __shared__ int s_scan[TILE];
+
+  // jpmin: first outer cell minimising wrap+(tiph_out - p0), :82-96
+  const double p0 = Tin[Nin - 1];
+  double bd = TWO_PI;
+  int bj = -1;
+  for (int jp = tid; jp < Nout; jp += TILE) {
+    double dp = wrap_p(Tout[jp] - p0);
+    if (bd > dp) {
+      bd = dp;
+      bj = jp;
+    }
+  }
+  s_d[tid] = bd;
+  s_j[tid] = bj;
+  __syncthreads();
+  for (int s = TILE / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      int j2 = s_j[tid + s];
+      if (j2 >= 0) {
+        int j1 = s_j[tid];
+        double d1 = s_d[tid], d2 = s_d[tid + s];
+        if (j1 < 0 || d2 < d1 || (d2 == d1 && j2 < j1)) {
+          s_d[tid] = d2;
+          s_j[tid] = j2;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const int jpmin = s_j[0] < 0 ? 0 : s_j[0];
+  __syncthreads();
+
+  FaceOut o;
+  o.fL = m.fL;
+  o.fR = m.fR;
+  o.fdp = m.fdphi;
+  o.fcm = m.fcm;
+  o.run_start = m.run_start;
+  o.fbase = P.face_base;
+  o.rbase = P.run_base;
+  o.cap = P.cap;
+  o.run_cap = P.run_cap;
+  o.zin = A.zbase;
+  o.zout = B.zbase;
+  o.err = m.err;
+  int *fo = (P.dir == 0 ? m.fo_r : m.fo_z) + A.zbase;
+  int4 *js = m.js + A.zbase;
+
+  // guess, count and verify
+  int bad = 0;
+  for (int j = tid; j < Nin; j += TILE) {
+    int jps = (j == 0) ? jpmin : guess_outer(Tin[j - 1], Tout, Nout);
+    int jpe;
+    int cnt = jloop_cell<false>(j, jps, Tin, Din, Tout, Dout, Nin, Nout, 0, o, &jpe, &bad);
+    js[j] = make_int4(jps, cnt, jpe, 0);
+  }
+  __syncthreads();
+  for (int j = tid; j + 1 < Nin; j += TILE)
+    if (js[j].z != js[j + 1].x) bad = 1;
+  bad = __syncthreads_or(bad);
+
+  int total = 0;
+  if (!bad) {
+    int carry = 0;
+    for (int base = 0; base < Nin; base += TILE) {
+      int j = base + tid;
+      int4 v = (j < Nin) ? js[j] : make_int4(0, 0, 0, 0);
+      s_scan[tid] = v.y;
+      __syncthreads();
+      for (int off = 1; off < TILE; off <<= 1) {
+        int add = (tid >= off) ? s_scan[tid - off] : 0;
+        __syncthreads();
+        s_scan[tid] += add;
+        __syncthreads();
+      }
+      int foff = carry + s_scan[tid] - v.y;
+      if (j < Nin) {
+        fo[j] = foff;
+        int jpe, b2 = 0;
+        jloop_cell<true>(j, v.x, Tin, Din, Tout, Dout, Nin, Nout, foff, o, &jpe, &b2);
+      }
+      carry += s_scan[TILE - 1];
+      __syncthreads();
+    }
+    total = carry;
+  } else if (tid == 0) {
+    // sequential walk of the reference
+    int jp = jpmin, foff = 0, b2 = 0;
+    for (int j = 0; j < Nin; ++j) {
+      fo[j] = foff;
+      int jpe;
+      foff += jloop_cell<true>(j, jp, Tin, Din, Tout, Dout, Nin, Nout, foff, o, &jpe, &b2);
+      jp = jpe;
+    }
+    total = foff;
+  }
+  if (tid == 0) {
+    int T = total - Nin;
+    if (total > P.cap || T + 1 >= P.run_cap || T < 0) {
+      set_err(m.err, DERR_FACE_OVERFLOW);
+      T = -1;
+      total = 0;
+    } else {
+      m.run_start[P.run_base] = 0;
+      m.run_start[P.run_base + T + 1] = total;
+    }
+    PairState st;
+    st.count = total;
+    st.jpmin = jpmin;
+    st.T = T;
+    st.seq = bad;
+    m.pst[p] = st;
+  }
+}
+
+// Faces of a pair that have inner cell j (local index) as L, and the runs
+// that have outer cell jo as R, in face order.
+template <typename F>
+__device__ __forceinline__ void faces_as_L(const DevMesh &m, int p, const int *fo, long long z, int j, int Nin,
+                                           F f) {
+  const PairInfo P = m.pairs[p];
+  const PairState S = m.pst[p];
+  int f1 = (j + 1 < Nin) ? fo[z + 1] : S.count;
+  if (f1 > S.count) f1 = S.count;
+  int f0 = fo[z];
+  if (f0 < 0) f0 = 0;
+  for (int n = f0; n < f1; ++n) f(P, P.face_base + n);
+}
+template <typename F>
+__device__ __forceinline__ void faces_as_R(const DevMesh &m, int p, int jo, int Nout, F f) {
+  const PairInfo P = m.pairs[p];
+  const PairState S = m.pst[p];
+  int t = jo - S.jpmin;
+  if (t < 0) t += Nout;
+  for (; t <= S.T; t += Nout) {
+    int f0 = m.run_start[P.run_base + t];
+    int f1 = m.run_start[P.run_base + t + 1];
+    for (int n = f0; n < f1; ++n) f(P, P.face_base + n);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Mesh velocity per annulus: cell_clear_w + cell_set_w, src/Cell/cell_set_w.c:12-102.
+// One wavefront per annulus; C_RIGID sums in the reference's cell order.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_set_w(DevMesh m, int ann0) {
+  const int a = ann0 + blockIdx.x;
+  const AnnInfo A = m.ann[a];
+  const DiskParams &P = disk_of(m, A.disk);
+  const int lane = threadIdx.x;
+  const double r = 0.5 * (A.rm + A.rp);
+  double w = 0.0;
+  if (P.w_numeric == C_RIGID) {
+    __shared__ double s_vm[64], s_m[64];
+    const double dz = A.zp - A.zm;
+    const double area = .5 * (A.rp * A.rp - A.rm * A.rm);
+    double ww = 0.0, Mring = 0.0;
+    for (int base = 0; base < A.N; base += 64) {
+      int j = base + lane;
+      if (j < A.N) {
+        long long z = A.zbase + j;
+        double dV = area * m.dphi[z] * dz;  // cell_calc_cons volume, cell_conversion.c:45
+        double mass = m.prim[RHO * m.Ntot + z] * dV;
+        double vp = r * m.prim[UPP * m.Ntot + z];
+        s_vm[lane] = vp * mass;
+        s_m[lane] = mass;
+      }
+      __syncthreads();
+      if (lane == 0) {
+        int n = min(64, A.N - base);
+        for (int x = 0; x < n; ++x) {
+          Mring += s_m[x];
+          ww += s_vm[x];
+        }
+      }
+      __syncthreads();
+    }
+    w = ww / Mring;
+  } else if (P.w_numeric == C_KEPLER) {
+    w = pow(r, -0.5);
+  } else if (P.w_numeric == C_OMEGA20) {
+    w = r * 20.;
+  } else if (P.w_numeric == C_MILOS) {
+    w = (1. - exp(-pow(r, 1.5))) / sqrt(r);
+  } else if (P.w_numeric == C_POWERLAW) {
+    w = pow(r, -1.0);
+  }
+  if (lane == 0) m.wiph[a] = w;
+}
+
+// ----------------------------------------------------------------------------
+// phi slopes, cell_plm_p, src/Cell/cell_plm.c:116-161
+// ----------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
+  const int2 tl = tiles[blockIdx.x];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const double PLM = m.dp[A.disk].plm;
+  const int N = A.N;
+  const long long c = A.zbase + j;
+  const long long cL = A.zbase + (j == 0 ? N - 1 : j - 1);
+  const long long cR = A.zbase + (j == N - 1 ? 0 : j + 1);
+  const double dpL = .5 * (m.dphi[c] + m.dphi[cL]);
+  const double dpR = .5 * (m.dphi[c] + m.dphi[cR]);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const double *pq = m.prim + q * m.Ntot;
+    double vc = pq[c], vL = pq[cL], vR = pq[cR];
+    double sL = (vc - vL) / dpL;
+    double sR = (vR - vc) / dpR;
+    double sM = (vR - vL) / (dpL + dpR);
+    double s = PLM * sL;
+    if (fabs(s) > PLM * fabs(sR)) s = PLM * sR;
+    if (fabs(s) > fabs(sM)) s = sM;
+    if (sL * sR < 0.0) s = 0.0;
+    m.gradp[q * m.Ntot + c] = s;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// r or z slopes, cell_plm_rz, src/Cell/cell_plm.c:13-114, as a per-cell
+// gather: the cell's faces as R (previous pair) then as L (next pair), in
+// face order, for the sum and again for the limiter.
+// ----------------------------------------------------------------------------
+template <int NQ>
+__device__ __forceinline__ void face_slope(const DevMesh &m, const PairInfo &P, long long g, double *S) {
+  const int cL = m.fL[g], cR = m.fR[g];
+  const double cm = m.fcm[g];
+  double pL = clean_pi(m.tiph[cL]) - .5 * m.dphi[cL];
+  double pR = clean_pi(m.tiph[cR]) - .5 * m.dphi[cR];
+  double dpL = wrap_pm(cm - pL);
+  double dpR = wrap_pm(pR - cm);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double WL = m.prim[q * m.Ntot + cL] + dpL * m.gradp[q * m.Ntot + cL];
+    double WR = m.prim[q * m.Ntot + cR] - dpR * m.gradp[q * m.Ntot + cR];
+    S[q] = (WR - WL) / (P.deltaR + P.deltaL);
+  }
+}
+
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, int dir) {
+  const int2 tl = tiles[blockIdx.x];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const DiskParams &P = disk_of(m, A.disk);
+  const long long z = A.zbase + j;
+  const int k = A.k, i = A.i;
+  double g[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) g[q] = 0.0;
+  int pR = -1, pLf = -1;
+  const int *fo = dir == 0 ? m.fo_r : m.fo_z;
+  if (dir == 0) {
+    if (i >= 1) pR = rpair_of(P, k, i - 1);
+    if (i <= P.NR - 2) pLf = rpair_of(P, k, i);
+  } else {
+    if (k >= 1) pR = zpair_of(P, k - 1, i);
+    if (k <= P.NZ - 2) pLf = zpair_of(P, k, i);
+  }
+  auto acc = [&](const PairInfo &PI, long long gi) {
+    double S[NQ];
+    face_slope<NQ>(m, PI, gi, S);
+    double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) g[q] += S[q] * dA;
+  };
+  if (pR >= 0) faces_as_R(m, pR, j, A.N, acc);
+  if (pLf >= 0) faces_as_L(m, pLf, fo, z, j, A.N, acc);
+  const double dz = A.zp - A.zm;
+  const double rp = A.rp, rm = A.rm;
+  const double dp = m.dphi[z];
+  double dAtot;
+  if (dir == 0) {
+    if (i == 0)
+      dAtot = dz * rp * dp;  // inner global edge (single rank)
+    else if (i == P.NR - 1)
+      dAtot = dz * rm * dp;
+    else
+      dAtot = dz * (rp + rm) * dp;
+  } else {
+    dAtot = 2. * .5 * (rp * rp - rm * rm) * dp;
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) g[q] /= dAtot;
+  const double PLM = P.plm;
+  auto lim = [&](const PairInfo &PI, long long gi) {
+    double S[NQ];
+    face_slope<NQ>(m, PI, gi, S);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      if (S[q] * g[q] < 0.0)
+        g[q] = 0.0;
+      else if (fabs(PLM * S[q]) < fabs(g[q]))
+        g[q] = PLM * S[q];
+    }
+  };
+  if (pR >= 0) faces_as_R(m, pR, j, A.N, lim);
+  if (pLf >= 0) faces_as_L(m, pLf, fo, z, j, A.N, lim);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) m.grad[q * m.Ntot + z] = g[q];
+}
+
+// ----------------------------------------------------------------------------
+// Fluxes.  Face (j, j+1) of every annulus (riemann_setup_p,
+// src/Riemann/Riemann_setup.c:46-80) and every r/z overlap face
+// (riemann_setup_rz, :11-44), each solved once; riemann_AddFlux's
+// increments dt*dA*F are stored per face and gathered by k_update.
+// ----------------------------------------------------------------------------
+template <int NQ>
+__device__ __forceinline__ void load_visc_cell(const DevMesh &m, long long c, double *pr, double *g, double *gp,
+                                               double *t, double *d) {
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    pr[q] = m.prim[q * m.Ntot + c];
+    g[q] = m.grad[q * m.Ntot + c];
+    gp[q] = m.gradp[q * m.Ntot + c];
+  }
+  *t = clean_pi(m.tiph[c]);
+  *d = m.dphi[c];
+}
+
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_flux_phi(DevMesh m, const int2 *tiles, double fac) {
+  const int2 tl = tiles[blockIdx.x];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const DiskParams &P = disk_of(m, A.disk);
+  const DiskScalars &sc = m.sc[A.disk];
+  const double dt = fac * sc.dt;
+  const long long cL = A.zbase + j;
+  const long long cR = A.zbase + (j == A.N - 1 ? 0 : j + 1);
+  const double dpL = m.dphi[cL], dpR = m.dphi[cR];
+  const double dz = A.zp - A.zm;
+  const double dr = A.rp - A.rm;
+  const double r = .5 * (A.rp + A.rm);
+  const double dA = dr * dz;
+  double primL[NQ], primR[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    primL[q] = m.prim[q * m.Ntot + cL] + 0.5 * m.gradp[q * m.Ntot + cL] * dpL;
+    primR[q] = m.prim[q * m.Ntot + cR] - 0.5 * m.gradp[q * m.Ntot + cR] * dpR;
+  }
+  ViscIn vi;
+  if (P.visc > 0.0) {
+    load_visc_cell<NQ>(m, cL, vi.pL, vi.gL, vi.gpL, &vi.tL, &vi.dL);
+    load_visc_cell<NQ>(m, cR, vi.pR, vi.gR, vi.gpR, &vi.tR, &vi.dR);
+    vi.cm = vi.tL;
+    vi.r_cell_L = r;
+    vi.r_cell_R = r;
+  }
+  double X[NQ], Y[3] = {0.0, 0.0, 0.0};
+  int e = d_face_flux<NQ>(primL, primR, 1, r, dA, m.wiph[tl.x], dt, P, sc.gm, &vi, X, Y);
+  if (e) set_err(m.err, e);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) m.Fphi[q * m.Ntot + cL] = X[q];
+  if (P.visc > 0.0) {
+#pragma unroll
+    for (int v = 0; v < 3; ++v) m.Fphi[(NQ + v) * m.Ntot + cL] = Y[v];
+  }
+}
+
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
+  const int2 tl = ftiles[blockIdx.x];
+  const int p = tl.x;
+  const int f = tl.y + threadIdx.x;
+  if (f >= m.pst[p].count) return;
+  const PairInfo PI = m.pairs[p];
+  const DiskParams &P = disk_of(m, PI.disk);
+  const DiskScalars &sc = m.sc[PI.disk];
+  const double dt = fac * sc.dt;
+  const long long g = PI.face_base + f;
+  const int cL = m.fL[g], cR = m.fR[g];
+  const double cm = m.fcm[g];
+  const double tL = clean_pi(m.tiph[cL]), tR = clean_pi(m.tiph[cR]);
+  const double dL = m.dphi[cL], dR = m.dphi[cR];
+  double pL = tL - .5 * dL;
+  double pR = tR - .5 * dR;
+  double dpL = cm - pL;
+  double dpR = -cm + pR;
+  dpL = wrap_pm(dpL);
+  dpR = wrap_pm(dpR);
+  const double r = PI.r;
+  const double dA = r * m.fdphi[g] * PI.deltaPerp;
+  double primL[NQ], primR[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    primL[q] = m.prim[q * m.Ntot + cL] + m.grad[q * m.Ntot + cL] * PI.deltaL + m.gradp[q * m.Ntot + cL] * dpL;
+    primR[q] = m.prim[q * m.Ntot + cR] - m.grad[q * m.Ntot + cR] * PI.deltaR - m.gradp[q * m.Ntot + cR] * dpR;
+  }
+  ViscIn vi;
+  if (P.visc > 0.0) {
+    load_visc_cell<NQ>(m, cL, vi.pL, vi.gL, vi.gpL, &vi.tL, &vi.dL);
+    load_visc_cell<NQ>(m, cR, vi.pR, vi.gR, vi.gpR, &vi.tR, &vi.dR);
+    vi.cm = cm;
+    if (PI.dir == 0) {
+      vi.r_cell_L = r - PI.deltaL;
+      vi.r_cell_R = r + PI.deltaR;
+    } else {
+      vi.r_cell_L = r;
+      vi.r_cell_R = r;
+    }
+  }
+  double X[NQ], Y[3] = {0.0, 0.0, 0.0};
+  int e = d_face_flux<NQ>(primL, primR, PI.dir == 0 ? 0 : 2, r, dA, 0.0, dt, P, sc.gm, &vi, X, Y);
+  if (e) set_err(m.err, e);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = X[q];
+  if (P.visc > 0.0) {
+#pragma unroll
+    for (int v = 0; v < 3; ++v) m.Fface[(NQ + v) * m.NFtot + g] = Y[v];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Per-cell update of one substep:
+//   cell_copy (first substep), cell_adjust_RK_cons, the flux increments in
+//   the reference's order, cell_add_src, cell_add_visc_src,
+//   cell_add_split_fictitious, cell_update_phi, cell_update_dphi and
+//   cell_calc_prim (src/TimeStep/timestep_routines.c:31-82).  Conserved
+//   variables at the start of the substep are prim2cons of the primitives,
+//   which is what cell_calc_cons left there (:112).
+// ----------------------------------------------------------------------------
+template <int NQ>
+__device__ __forceinline__ void add_face_R(const DevMesh &m, const double *F, long long stride, long long g,
+                                           double *u, double r_cell, bool visc) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) u[q] += F[q * stride + g];
+  if (visc) {
+    u[SRR] += F[NQ * stride + g] / r_cell;
+    u[LLL] += F[(NQ + 1) * stride + g];
+    u[TAU] += F[(NQ + 2) * stride + g];
+  }
+}
+template <int NQ>
+__device__ __forceinline__ void add_face_L(const DevMesh &m, const double *F, long long stride, long long g,
+                                           double *u, double r_cell, bool visc) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) u[q] += -F[q * stride + g];
+  if (visc) {
+    u[SRR] += -F[NQ * stride + g] / r_cell;
+    u[LLL] += -F[(NQ + 1) * stride + g];
+    u[TAU] += -F[(NQ + 2) * stride + g];
+  }
+}
+
+// new tiph of one cell, cell_update_phi, src/Cell/cell_misc.c:59-79
+__device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy_first, double RK, double wtot,
+                                           double dt, double r, double *rkt_out) {
+  double tb = clean_pi(traw);
+  double rkt = copy_first ? traw : rkt_in;
+  int n = 0;
+  while (tb - rkt > HALF_TURN && n++ < MAXWRAP) rkt += TWO_PI;
+  while (tb - rkt < -HALF_TURN && n++ < MAXWRAP) rkt -= TWO_PI;
+  *rkt_out = rkt;
+  double t = (1.0 - RK) * tb + RK * rkt;
+  t += wtot * dt / r;
+  return t;
+}
+
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
+  const int2 tl = tiles[blockIdx.x];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  __shared__ double s_md[2][TILE / 64];
+  double md0 = 0.0, md1 = 0.0;
+  const DiskParams &P = disk_of(m, A.disk);
+  const DiskScalars &sc = m.sc[A.disk];
+  if (j < A.N) {
+    const double dt = fac * sc.dt;
+    const double RK = sc.RK;
+    const bool visc = P.visc > 0.0;
+    const long long c = A.zbase + j;
+    const int N = A.N, k = A.k, i = A.i;
+    const double rp = A.rp, rm = A.rm, zp = A.zp, zm = A.zm;
+    const double r = .5 * (rp + rm);
+    const double dz = zp - zm;
+    const double dphi = m.dphi[c];
+    double pr[NQ], u[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) pr[q] = m.prim[q * m.Ntot + c];
+    // conserved state left by cell_calc_cons
+    d_prim2cons<NQ>(pr, u, r, .5 * (rp * rp - rm * rm) * dphi * dz, P.gamma, P.NC);
+    if (copy_first) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) m.RKcons[q * m.Ntot + c] = u[q];
+    }
+    // cell_adjust_RK_cons, cell_misc.c:43-57
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double rk = copy_first ? u[q] : m.RKcons[q * m.Ntot + c];
+      u[q] = (1.0 - RK) * u[q] + RK * rk;
+    }
+    // phi faces (timestep_routines.c:42-51): face j-1 has this cell as R,
+    // face j as L; for j = 0 the L face comes first.
+    {
+      const long long cm1 = A.zbase + (j == 0 ? N - 1 : j - 1);
+      if (j == 0) {
+        add_face_L<NQ>(m, m.Fphi, m.Ntot, c, u, r, visc);
+        add_face_R<NQ>(m, m.Fphi, m.Ntot, cm1, u, r, visc);
+      } else {
+        add_face_R<NQ>(m, m.Fphi, m.Ntot, cm1, u, r, visc);
+        add_face_L<NQ>(m, m.Fphi, m.Ntot, c, u, r, visc);
+      }
+    }
+    // r faces in face order: as R of pair (i-1, i), then as L of (i, i+1)
+    if (i >= 1)
+      faces_as_R(m, rpair_of(P, k, i - 1), j, N, [&](const PairInfo &PI, long long g) {
+        add_face_R<NQ>(m, m.Fface, m.NFtot, g, u, PI.r + PI.deltaR, visc);
+      });
+    if (i <= P.NR - 2)
+      faces_as_L(m, rpair_of(P, k, i), m.fo_r, c, j, N, [&](const PairInfo &PI, long long g) {
+        add_face_L<NQ>(m, m.Fface, m.NFtot, g, u, PI.r - PI.deltaL, visc);
+      });
+    if (P.NglobalZ != 1) {
+      if (k >= 1)
+        faces_as_R(m, zpair_of(P, k - 1, i), j, N, [&](const PairInfo &PI, long long g) {
+          add_face_R<NQ>(m, m.Fface, m.NFtot, g, u, PI.r, visc);
+        });
+      if (k <= P.NZ - 2)
+        faces_as_L(m, zpair_of(P, k, i), m.fo_z, c, j, N, [&](const PairInfo &PI, long long g) {
+          add_face_L<NQ>(m, m.Fface, m.NFtot, g, u, PI.r, visc);
+        });
+    }
+    // cell_add_src, src/Cell/cell_source.c:73-214
+    const double traw = m.tiph[c];
+    const double tcl = clean_pi(traw);
+    const GravState g0 = sc.gm[0], g1 = sc.gm[1];
+    const double a = g0.r + g1.r;
+    {
+      double phi = tcl - .5 * dphi;
+      double rho = pr[RHO], Pp = pr[PPP], vr = pr[URR], vz = pr[UZZ];
+      double vp = pr[UPP] * r;
+      double dV = dphi * .5 * (rp * rp - rm * rm) * dz;
+      double zc = .5 * (zp + zm);
+      double R = sqrt(r * r + zc * zc);
+      double sint, cost, gravdist;
+      if (P.grav2d == 1) {
+        sint = 1.0;
+        cost = 0.0;
+        gravdist = r;
+      } else {
+        sint = r / R;
+        cost = zc / R;
+        gravdist = R;
+      }
+      double fr, fp, Fr = 0.0, Fp = 0.0;
+      d_grav_force(g0, P.g_eps, P.phi_order, gravdist, phi, &fr, &fp);
+      Fr += fr;
+      Fp += fp;
+      d_grav_force(g1, P.g_eps, P.phi_order, gravdist, phi, &fr, &fp);
+      Fr += fr;
+      Fp += fp;
+      double w_a = d_rOm_a(P.w_analytic, r, a);
+      double rdrOm_a = d_rdrOm_a(P.w_analytic, r, a);
+      double F_centrifugal_r = w_a * w_a / r;
+      double dtOm_a = d_dtOm_a(P.w_analytic, r, a, P.orb_tscale);
+      double F_euler_phi = -(r * dtOm_a + vr * rdrOm_a);
+      u[SRR] += dt * dV * (rho * vp * vp + Pp) / r;
+      u[SRR] += dt * dV * rho * (Fr * sint + F_centrifugal_r);
+      u[LLL] += dt * dV * rho * (Fp - r * dtOm_a) * r;
+      u[SZZ] += dt * dV * rho * Fr * cost;
+      u[TAU] += dt * dV * rho * ((Fr * sint + F_centrifugal_r) * vr + Fr * vz * cost + (Fp + F_euler_phi) * vp);
+      if (P.sink == 1) {  // get_rho_sink, :15-47
+        double xpos = r * cos(phi), ypos = r * sin(phi);
+        double xbh0 = g0.r * cos(g0.phi), ybh0 = g0.r * sin(g0.phi);
+        double xbh1 = g1.r * cos(g1.phi), ybh1 = g1.r * sin(g1.phi);
+        double r0 = sqrt((xpos - xbh0) * (xpos - xbh0) + (ypos - ybh0) * (ypos - ybh0));
+        double r1 = sqrt((xpos - xbh1) * (xpos - xbh1) + (ypos - ybh1) * (ypos - ybh1));
+        double M0 = g0.M, M1 = g1.M;
+        double Mtotal = M0 + M1;
+        double alpha = P.visc;
+        double one_o_nu = 1.0 / (alpha * Pp / rho) * sqrt(pow(r0, -3) * M0 / Mtotal + pow(r1, -3) * M1 / Mtotal);
+        double t_visc0 = 1.0 * 2. / 3. * r0 * r0 * one_o_nu;
+        double t_visc1 = 1.0 * 2. / 3. * r1 * r1 * one_o_nu;
+        if (t_visc0 < 10. * dt) t_visc0 = 10. * dt;
+        if (t_visc1 < 10. * dt) t_visc1 = 10. * dt;
+        int imin = P.NgMinR == 1 ? 0 : P.NgMinR;
+        bool interior = i >= imin && i < P.NR - P.NgMaxR && k >= P.NgMinZ && k < P.NZ - P.NgMaxZ;
+        for (int pm = 0; pm < P.grav_type && pm < 2; ++pm) {
+          double drho = 0.0;
+          if (pm == 0 && r0 < 0.5) drho = rho / t_visc0;
+          if (pm == 1 && r1 < 0.5) drho = rho / t_visc1;
+          u[RHO] -= drho * dt * dV;
+          if (interior) {
+            if (pm == 0)
+              md0 += drho * dV;
+            else
+              md1 += drho * dV;
+          }
+        }
+      }
+      if (P.cooling == 1) {
+        if (P.initial_data_type == 0) {
+          double cooling_term = (Pp - .01) / (P.gamma - 1.) * dV / 1.e-2;
+          if (cooling_term < 0.0) cooling_term = 0.0;
+          u[TAU] -= cooling_term * dt;
+        } else {
+          double cooling_cap = u[TAU] * 1.0;
+          double cooling_term = 9. / 4. * P.visc * pow(P.porho_r1, -3) / rho * pow(Pp / rho, 4.) * dt * dV;
+          if (cooling_term > cooling_cap) cooling_term = cooling_cap;
+          u[TAU] -= cooling_term;
+        }
+      }
+      // cell_add_visc_src, :216-277
+      if (visc) {
+        double xpos = r * cos(phi), ypos = r * sin(phi);
+        double xbh0 = g0.r * cos(g0.phi), ybh0 = g0.r * sin(g0.phi);
+        double xbh1 = g1.r * cos(g1.phi), ybh1 = g1.r * sin(g1.phi);
+        double dist_bh0 = sqrt((xpos - xbh0) * (xpos - xbh0) + (ypos - ybh0) * (ypos - ybh0));
+        double dist_bh1 = sqrt((xpos - xbh1) * (xpos - xbh1) + (ypos - ybh1) * (ypos - ybh1));
+        double eps = P.g_eps;
+        double Sigma_nu;
+        if (P.visc_const == 1)
+          Sigma_nu = rho * P.visc;
+        else
+          Sigma_nu = P.visc * Pp /
+                     sqrt(pow(dist_bh0 * dist_bh0 + eps * eps, -1.5) * g0.M + pow(dist_bh1 * dist_bh1 + eps * eps, -1.5) * g1.M);
+        u[TAU] += dt * dV * Sigma_nu * rdrOm_a *
+                  (rdrOm_a + r * m.grad[UPP * m.Ntot + c] + m.gradp[URR * m.Ntot + c]);
+      }
+    }
+    // cell_add_split_fictitious, cell_add_split_fictitious.c:11-44
+    {
+      double rOm = d_rOm_a(P.w_analytic, r, a);
+      double Omega = rOm / r;
+      double dr_r2Om = 2. * rOm + r * d_rdrOm_a(P.w_analytic, r, a);
+      double kappa2 = 2. * Omega / r * dr_r2Om;
+      if (kappa2 > 0.) {
+        double kappa = sqrt(kappa2);
+        double l0 = u[LLL];
+        double rhov0 = u[SRR];
+        double l = l0 * cos(kappa * dt) - u[SRR] * dr_r2Om / kappa * sin(kappa * dt);
+        double rhov = rhov0 * cos(kappa * dt) + 2.0 * Omega * u[LLL] / r / kappa * sin(kappa * dt);
+        u[LLL] = l;
+        u[SRR] = rhov;
+      }
+    }
+    // cell_update_phi + cell_update_dphi (the left neighbour's new edge is
+    // recomputed here from its old state)
+    const double wtot = m.wiph[tl.x] + d_rOm_a(P.w_analytic, r, a);
+    double rkt_own, rkt_nb;
+    double tnew = new_tiph(traw, m.RKtiph[c], copy_first, RK, wtot, dt, r, &rkt_own);
+    const long long cm1 = A.zbase + (j == 0 ? N - 1 : j - 1);
+    double tnew_m1 = new_tiph(m.tiph[cm1], m.RKtiph[cm1], copy_first, RK, wtot, dt, r, &rkt_nb);
+    double dphi_new = wrap_p(tnew - tnew_m1);
+    m.RKtiph[c] = rkt_own;
+    m.tiph_new[c] = tnew;
+    m.dphi[c] = dphi_new;
+    // cell_calc_prim, cell_conversion.c:105-122
+    double out[NQ];
+    d_cons2prim<NQ>(u, out, r, .5 * (rp * rp - rm * rm) * dphi_new * dz, P);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) m.prim[q * m.Ntot + c] = out[q];
+  }
+  if (P.sink == 1) {
+    // Mdot partial sums of the block (one annulus, one disk)
+    for (int off = 32; off > 0; off >>= 1) {
+      md0 += __shfl_down(md0, off);
+      md1 += __shfl_down(md1, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      s_md[0][threadIdx.x >> 6] = md0;
+      s_md[1][threadIdx.x >> 6] = md1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t0 = 0.0, t1 = 0.0;
+      for (int w = 0; w < TILE / 64; ++w) {
+        t0 += s_md[0][w];
+        t1 += s_md[1][w];
+      }
+      atomicAdd(&m.sc[A.disk].mdot_acc[0], t0);
+      atomicAdd(&m.sc[A.disk].mdot_acc[1], t1);
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Outflow r boundary, cell_boundary_outflow_r, src/Cell/cell_boundary.c:14-139:
+// the ghost annulus takes the overlap-area-weighted primitives of its
+// neighbour.  stage 0: inner ghost (i = 0) and outer ghost NR-2; stage 1:
+// outer ghost NR-1 (from the new NR-2).
+// ----------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_bc_r(DevMesh m, const int2 *tiles) {
+  const int2 tl = tiles[blockIdx.x];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const DiskParams &P = disk_of(m, A.disk);
+  const long long c = A.zbase + j;
+  const int k = A.k, i = A.i;
+  const double dz = A.zp - A.zm;
+  double v[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) v[q] = 0.0;
+  double r_face;
+  if (i == 0) {
+    // cL->prim += (cR->prim - 0*grad*(...)/2.)*dA over the pair's faces
+    const double span = m.rf[P.geo_r + i + 2] - m.rf[P.geo_r + i];
+    r_face = A.rp;
+    faces_as_L(m, rpair_of(P, k, 0), m.fo_r, c, j, A.N, [&](const PairInfo &PI, long long g) {
+      const int cR = m.fR[g];
+      double dA = PI.r * m.fdphi[g] * PI.deltaPerp;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        v[q] += (m.prim[q * m.Ntot + cR] - 0 * m.grad[q * m.Ntot + cR] * span / 2.) * dA;
+    });
+  } else {
+    r_face = A.rm;
+    faces_as_R(m, rpair_of(P, k, i - 1), j, A.N, [&](const PairInfo &PI, long long g) {
+      const int cL = m.fL[g];
+      double dA = PI.r * m.fdphi[g] * PI.deltaPerp;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) v[q] += m.prim[q * m.Ntot + cL] * dA;
+    });
+  }
+  const double dA = dz * r_face * m.dphi[c];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) m.prim[q * m.Ntot + c] = v[q] / dA;
+}
+
+// z halo on one rank: periodic copy of (prim, RKcons, tiph) into the ghost
+// layers, src/Cell/cell_sync.c:156-205.
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_sync_z(DevMesh m, const int2 *tiles) {
+  const int2 tl = tiles[blockIdx.x];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const DiskParams &P = disk_of(m, A.disk);
+  const int k = A.k;
+  int ksrc;
+  if (k < P.NgMinZ)
+    ksrc = P.NZ - 2 * P.NgMaxZ + k;
+  else
+    ksrc = P.NgMinZ + (k - (P.NZ - P.NgMaxZ));
+  const long long c = A.zbase + j;
+  const long long s = c + (long long)(ksrc - k) * P.NL;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    m.prim[q * m.Ntot + c] = m.prim[q * m.Ntot + s];
+    m.RKcons[q * m.Ntot + c] = m.RKcons[q * m.Ntot + s];
+  }
+  m.tiph_new[c] = m.tiph_new[s];
+}
+
+// ----------------------------------------------------------------------------
+// CFL time step over interior cells, cell_mindt, src/Cell/cell_mindt.c:10-100.
+// The minimum of positive doubles is taken on their bit patterns.
+// ----------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
+  const int2 tl = tiles[blockIdx.x];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  const DiskParams &P = disk_of(m, A.disk);
+  double dt = 1.e100;
+  if (j < A.N) {
+    const long long c = A.zbase + j;
+    const double rm = A.rm, rp = A.rp;
+    const double dz = A.zp - A.zm;
+    const double dr = rp - rm;
+    const double r = .5 * (rp + rm);
+    const double wj = m.wiph[tl.x];
+    double w = .5 * (wj + wj);
+    double dx = dr;
+    double rdphi = .5 * (rp + rm) * m.dphi[c];
+    if (rdphi < dr) dx = rdphi;
+    if (dx > dz) dx = dz;
+    double Pp = m.prim[PPP * m.Ntot + c];
+    double rho = m.prim[RHO * m.Ntot + c];
+    double vp = m.prim[UPP * m.Ntot + c] * r - w;
+    double vr = m.prim[URR * m.Ntot + c];
+    double vz = m.prim[UZZ * m.Ntot + c];
+    double cs = sqrt(P.gamma * Pp / rho);
+    double cf2 = cs * cs;
+    double a = sqrt(cf2) + sqrt(vr * vr + vp * vp + vz * vz);
+    double d = P.cfl * dx / a;
+    if (P.visc > 0.0) {
+      double nu;
+      if (P.visc_const == 1) {
+        nu = P.visc;
+      } else {
+        double tiph = clean_pi(m.tiph[c]) - 0.5 * m.dphi[c];
+        if (P.initial_data_type == 0) {
+          nu = P.visc * P.gamma * Pp / rho * pow(fabs(r * cos(tiph)), 2.0);
+          if (r * cos(tiph) > 20.) nu = 0.000000001;
+        } else {
+          const DiskScalars &sc = m.sc[A.disk];
+          double d0 = d_grav_dist(sc.gm[0], r, tiph, 0.);
+          double d1 = d_grav_dist(sc.gm[1], r, tiph, 0.);
+          double eps = P.g_eps;
+          nu = P.visc * Pp / rho / sqrt(pow(d0 * d0 + eps * eps, -1.5) * sc.gm[0].M + pow(d1 * d1 + eps * eps, -1.5) * sc.gm[1].M);
+        }
+      }
+      double dt_visc = .25 * dx * dx / nu;
+      d = d / (1. + d / dt_visc);
+    }
+    if (d < dt) dt = d;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(dt, off);
+    if (o < dt) dt = o;
+  }
+  __shared__ double s_m[TILE / 64];
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = s_m[0];
+    for (int w = 1; w < TILE / 64; ++w)
+      if (s_m[w] < b) b = s_m[w];
+    if (b > 0.0 && b < 1.e100) atomicMin(&m.sc[A.disk].dtmin_bits, (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Per-disk scalar updates (time, dt, the two point masses):
+// src/TimeStep/timestep_algorithm.c:11-41, src/TimeStep/timestep_set.c:16-21,
+// src/GravMass/GravMass_misc.c:9-77.
+// ----------------------------------------------------------------------------
+__device__ void d_gm_move(const DiskParams &P, DiskScalars &s, double t, double dt) {
+  double m0 = s.gm[0].M, m1 = s.gm[1].M;
+  double M = m0 + m1;
+  double a_0 = 1.0, tau_0 = P.orb_tscale, T0 = P.orb_t0;
+  double a, omega;
+  if ((t - T0) / tau_0 < 1.) {
+    a = a_0 * pow((1.0 - (t - T0) / tau_0), 0.25);
+    omega = pow(a / M, -1.5);
+  } else {
+    a = 0.0;
+    omega = 0.0;
+  }
+  double r0, r1;
+  if (M < 1.e-12) {
+    r0 = 0.0;
+    r1 = 0.0;
+  } else {
+    r0 = m1 / M * a;
+    r1 = m0 / M * a;
+  }
+  s.gm[0].phi += omega * dt;
+  s.gm[1].phi += omega * dt;
+  s.gm[0].r = r0;
+  s.gm[1].r = r1;
+  s.gm[0].omega = omega;
+  s.gm[1].omega = omega;
+}
+
+__global__ void k_scalar(DevMesh m, int ndisk, int op, double arg) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndisk) return;
+  const DiskParams &P = m.dp[d];
+  DiskScalars &s = m.sc[d];
+  switch (op) {
+    case SC_STEP_BEGIN: {  // timestep_set_dt, gravMass_copy, RK = 0
+      double dt = __longlong_as_double((long long)s.dtmin_bits);
+      if (s.t + dt > P.t_max) dt = P.t_max - s.t;
+      s.dt = dt;
+      for (int p = 0; p < 2; ++p) {
+        s.gm[p].RK_r = s.gm[p].r;
+        s.gm[p].RK_phi = s.gm[p].phi;
+        s.gm[p].RK_M = s.gm[p].M;
+        s.gm[p].RK_omega = s.gm[p].omega;
+      }
+      s.RK = 0.0;
+      break;
+    }
+    case SC_SUB_BEGIN:  // gravMass_clean_pi
+      for (int p = 0; p < 2; ++p) {
+        double phi = s.gm[p].phi;
+        int n = 0;
+        while (phi > 2. * M_PI && n++ < MAXWRAP) phi -= 2. * M_PI;
+        while (phi < 0.0 && n++ < MAXWRAP) phi += 2. * M_PI;
+        s.gm[p].phi = phi;
+      }
+      s.mdot_acc[0] = 0.0;
+      s.mdot_acc[1] = 0.0;
+      break;
+    case SC_AFTER_UPDATE: {  // Mdot of cell_add_src, then gravMass_update_RK
+      s.gm[0].Mdot = s.mdot_acc[0];
+      s.gm[1].Mdot = s.mdot_acc[1];
+      double RK = s.RK;
+      for (int p = 0; p < 2; ++p) {
+        GravState &g = s.gm[p];
+        int n = 0;
+        while (g.phi - g.RK_phi > M_PI && n++ < MAXWRAP) g.RK_phi += 2. * M_PI;
+        while (g.phi - g.RK_phi < -M_PI && n++ < MAXWRAP) g.RK_phi -= 2. * M_PI;
+        g.r = (1.0 - RK) * g.r + RK * g.RK_r;
+        g.phi = (1.0 - RK) * g.phi + RK * g.RK_phi;
+        g.M = (1.0 - RK) * g.M + RK * g.RK_M;
+        g.omega = (1.0 - RK) * g.omega + RK * g.RK_omega;
+      }
+      break;
+    }
+    case SC_SUB1_END:  // gravMass_move(t, 1.0*dt); RK = 0.5
+      d_gm_move(P, s, s.t, 1.0 * s.dt);
+      s.RK = 0.5;
+      break;
+    case SC_SUB2_END:  // gravMass_move(t, 0.5*dt); t += dt; Macc += Mdot*dt
+      d_gm_move(P, s, s.t, 0.5 * s.dt);
+      s.t += s.dt;
+      for (int p = 0; p < 2; ++p) s.gm[p].Macc = s.gm[p].Macc + s.gm[p].Mdot * s.dt;
+      s.dtmin_bits = (unsigned long long)__double_as_longlong(1.e100);
+      break;
+    case SC_EULER_END:  // timestep_forward_euler: gravMass_move(t, dt); t += dt
+      d_gm_move(P, s, s.t, 1.0 * s.dt);
+      s.t += s.dt;
+      s.dtmin_bits = (unsigned long long)__double_as_longlong(1.e100);
+      break;
+    case SC_RESET_DT:
+      s.dtmin_bits = (unsigned long long)__double_as_longlong(1.e100);
+      break;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// host-side launchers
+// ----------------------------------------------------------------------------
+template <int NQ>
+static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, double fac, int copy_first,
+                             int recompute_w) {
+  const dim3 B(TILE);
+  hipLaunchKernelGGL(k_scalar, dim3((L.ndisk + 63) / 64), dim3(64), 0, st, m, L.ndisk, SC_SUB_BEGIN, 0.0);
+  if (L.n_rpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), B, 0, st, m, 0);
+  if (L.n_zpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), B, 0, st, m, L.n_rpairs);
+  if (recompute_w) hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0);
+  hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles);
+  hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0);
+  hipLaunchKernelGGL(k_flux_phi<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, fac);
+  if (L.n_rftiles) hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles, fac);
+  if (L.n_zpairs) {
+    hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 1);
+    hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_zftiles), B, 0, st, m, L.zftiles, fac);
+  }
+  hipLaunchKernelGGL(k_update<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, fac, copy_first);
+  hipLaunchKernelGGL(k_scalar, dim3((L.ndisk + 63) / 64), dim3(64), 0, st, m, L.ndisk, SC_AFTER_UPDATE, 0.0);
+  DevMesh m2 = m;  // after the update the new edges live in tiph_new
+  m2.tiph = m.tiph_new;
+  if (L.n_bc0) hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc0), B, 0, st, m2, L.bc0);
+  if (L.n_bc1) hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc1), B, 0, st, m2, L.bc1);
+  if (L.n_sz) hipLaunchKernelGGL(k_sync_z<NQ>, dim3(L.n_sz), B, 0, st, m, L.sz);
+}
+
+void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
+                    int recompute_w) {
+  if (NQ == 5)
+    launch_substep_q<5>(m, L, st, fac, copy_first, recompute_w);
+  else
+    launch_substep_q<6>(m, L, st, fac, copy_first, recompute_w);
+}
+
+void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op) {
+  hipLaunchKernelGGL(k_scalar, dim3((L.ndisk + 63) / 64), dim3(64), 0, st, m, L.ndisk, op, 0.0);
+}
+
+void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0);
+}
+
+void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ) {
+  if (NQ == 5)
+    hipLaunchKernelGGL(k_mindt<5>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles);
+  else
+    hipLaunchKernelGGL(k_mindt<6>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles);
+}
+
+void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir) {
+  if (dir == 0 && L.n_rpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), dim3(TILE), 0, st, m, 0);
+  if (dir == 1 && L.n_zpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), dim3(TILE), 0, st, m, L.n_rpairs);
+}
+
+void launch_plm(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int which) {
+  if (NQ == 5) {
+    if (which < 0)
+      hipLaunchKernelGGL(k_plm_p<5>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
+    else
+      hipLaunchKernelGGL(k_plm_rz<5>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles, which);
+  } else {
+    if (which < 0)
+      hipLaunchKernelGGL(k_plm_p<6>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
+    else
+      hipLaunchKernelGGL(k_plm_rz<6>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles, which);
+  }
+}

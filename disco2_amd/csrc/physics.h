@@ -1,0 +1,355 @@
+// Per-zone and per-face FP64 physics of the hydro substep, as device
+// functions.  Expression association follows the reference exactly (the
+// library is compiled with -ffp-contract=off), so a kernel that adds the
+// same terms in the same order reproduces the reference bit for bit,
+// except where libm transcendentals (sin, cos, pow, exp) differ by an ulp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "disco_types.h"
+
+#define TWO_PI (2.0 * M_PI)  // the reference's PHIMAX, src/Headers/header.h:21
+#define HALF_TURN (TWO_PI / 2.)
+#define MAXWRAP 64            // angle wraps beyond this mean a diverged state
+
+// while-loop angle wraps of the reference, with a bounded trip count
+__device__ __forceinline__ double clean_pi(double phi) {  // src/Cell/cell_misc.c:19-20
+  int n = 0;
+  while (phi > TWO_PI && n++ < MAXWRAP) phi -= TWO_PI;
+  while (phi < 0.0 && n++ < MAXWRAP) phi += TWO_PI;
+  return phi;
+}
+__device__ __forceinline__ double wrap_pm(double d) {  // e.g. src/Cell/cell_plm.c:39-40
+  int n = 0;
+  while (d > HALF_TURN && n++ < MAXWRAP) d -= TWO_PI;
+  while (d < -HALF_TURN && n++ < MAXWRAP) d += TWO_PI;
+  return d;
+}
+__device__ __forceinline__ double wrap_p(double d) {  // e.g. src/Face/face_create_destroy.c:88-89
+  int n = 0;
+  while (d > TWO_PI && n++ < MAXWRAP) d -= TWO_PI;
+  while (d < 0.0 && n++ < MAXWRAP) d += TWO_PI;
+  return d;
+}
+
+// analytic frame rotation, src/Sim/sim_w_a.c:8-73
+#define W_A_MILOS_INDEX 8
+__device__ inline double d_rOm_a(int type, double r, double a) {
+  switch (type) {
+    case A_KEPLER: return pow(r, -0.5);
+    case A_OMEGA20: return r * 20;
+    case A_MILOS:
+      return r * (pow(r, W_A_MILOS_INDEX - 1.5) + pow(a, W_A_MILOS_INDEX - 1.5)) /
+             (pow(r, W_A_MILOS_INDEX) + pow(a, W_A_MILOS_INDEX));
+    case A_POWERLAW: return pow(r, -1.0);
+    default: return 0.0;
+  }
+}
+__device__ inline double d_rdrOm_a(int type, double r, double a) {
+  switch (type) {
+    case A_KEPLER: return -1.5 * pow(r, -1.5);
+    case A_MILOS: {
+      int n = W_A_MILOS_INDEX;
+      double Omega_a = (pow(r, n - 1.5) + pow(a, n - 1.5)) / (pow(r, n) + pow(a, n));
+      double arn = pow(a / r, n);
+      return (-1.5 + n * arn / (1. + arn)) * Omega_a + (1.5 - n) * pow(a / r, n - 1.5) / (1. + arn) * pow(r, -1.5);
+    }
+    case A_POWERLAW: return -2. * pow(r, -2.0);
+    default: return 0.0;
+  }
+}
+__device__ inline double d_dtOm_a(int type, double r, double a, double tau) {
+  if (type == A_MILOS) {
+    double a0 = 1.0;
+    double dta_o_a = -.25 * pow(a / a0, -4) / tau;
+    int n = W_A_MILOS_INDEX;
+    double Omega_a = (pow(r, n - 1.5) + pow(a, n - 1.5)) / (pow(r, n) + pow(a, n));
+    double arn = pow(a / r, n);
+    if (a > 0.0) return dta_o_a * (Omega_a * (-1.5 + n / (1. + arn)) + (1.5 - n) * pow(r, -1.5) / (1. + arn));
+  }
+  return 0.0;
+}
+
+// src/GravMass/GravMass_access.c:25-31
+__device__ __forceinline__ double d_grav_dist(const GravState &g, double r, double phi, double z) {
+  double xbh = g.r * cos(g.phi);
+  double ybh = g.r * sin(g.phi);
+  double x = r * cos(phi);
+  double y = r * sin(phi);
+  return sqrt((x - xbh) * (x - xbh) + (y - ybh) * (y - ybh) + z * z);
+}
+
+// src/Cell/cell_conversion.c:11-33
+template <int NQ>
+__device__ __forceinline__ void d_prim2cons(const double *prim, double *cons, double r, double dV,
+                                            double G, int NC) {
+  double rho = prim[RHO];
+  double Pp = prim[PPP];
+  double vr = prim[URR];
+  double vp = prim[UPP] * r;
+  double vz = prim[UZZ];
+  double v2 = vr * vr + vp * vp + vz * vz;
+  double rhoe = Pp / (G - 1.);
+  cons[DDD] = rho * dV;
+  cons[SRR] = rho * vr * dV;
+  cons[LLL] = r * rho * vp * dV;
+  cons[SZZ] = rho * vz * dV;
+  cons[TAU] = (.5 * rho * v2 + rhoe) * dV;
+#pragma unroll
+  for (int q = 5; q < NQ; ++q) cons[q] = prim[q] * cons[DDD];
+}
+
+// src/Cell/cell_conversion.c:57-103
+template <int NQ>
+__device__ __forceinline__ void d_cons2prim(const double *cons, double *prim, double r, double dV,
+                                            const DiskParams &P) {
+  double G = P.gamma;
+  double rho = cons[DDD] / dV;
+  if (rho < P.rho_floor) rho = P.rho_floor;
+  double Sr = cons[SRR] / dV;
+  double Sp = cons[LLL] / dV / r;
+  double Sz = cons[SZZ] / dV;
+  double E = cons[TAU] / dV;
+  double vr = Sr / rho;
+  double vp = Sp / rho;
+  double vz = Sz / rho;
+  double KE = .5 * (Sr * vr + Sp * vp + Sz * vz);
+  double v_magnitude = sqrt(2. * KE / rho);
+  double rhoe = E - KE;
+  double Pp = (G - 1.) * rhoe;
+  if (Pp < P.cs_floor * P.cs_floor * rho / G) Pp = P.cs_floor * P.cs_floor * rho / G;
+  if (Pp > P.cs_cap * P.cs_cap * rho / G) Pp = P.cs_cap * P.cs_cap * rho / G;
+  if (v_magnitude > P.vel_cap) {
+    vr = vr * P.vel_cap / v_magnitude;
+    vp = vp * P.vel_cap / v_magnitude;
+    vz = vz * P.vel_cap / v_magnitude;
+  }
+  prim[RHO] = rho;
+  prim[PPP] = Pp;
+  prim[URR] = vr;
+  prim[UPP] = vp / r;
+  prim[UZZ] = vz;
+#pragma unroll
+  for (int q = 5; q < NQ; ++q) prim[q] = cons[q] / cons[DDD];
+}
+
+// Flux of one face, src/Riemann/Riemann_routines.c:14-397.  The caller has
+// reconstructed primL/primR (riemann_setup_*); n is the face normal
+// (0: r, 1: phi, 2: z), w the face velocity.  On return X[q] = dt*dA*F[q]
+// and, with viscosity, Y = dt*dA*{Fvisc[SRR], Fvisc[LLL], Fvisc[TAU]} --
+// the quantities riemann_AddFlux adds to the two cells.
+struct ViscIn {   // what riemann_visc_flux reads besides the reconstructed states
+  double cm, r_cell_L, r_cell_R;
+  double gpL[5], gpR[5], gL[5], gR[5];   // gradp, grad of the two cells (first 5 q)
+  double pL[5], pR[5];                   // cell-centre primitives
+  double tL, tR, dL, dR;                 // tiph, dphi of the two cells
+};
+
+template <int NQ>
+__device__ __forceinline__ void d_set_flux(const double *prim, double *F, double r, int n0, int n1, int n2,
+                                           double G) {  // Riemann_routines.c:157-191
+  double rho = prim[RHO];
+  double Pp = prim[PPP];
+  double vr = prim[URR];
+  double vp = prim[UPP] * r;
+  double vz = prim[UZZ];
+  double vn = vr * n0 + vp * n1 + vz * n2;
+  double rhoe = Pp / (G - 1.);
+  double v2 = vr * vr + vp * vp + vz * vz;
+  F[DDD] = rho * vn;
+  F[SRR] = rho * vr * vn + Pp * n0;
+  F[LLL] = r * (rho * vp * vn + Pp * n1);
+  F[SZZ] = rho * vz * vn + Pp * n2;
+  F[TAU] = (.5 * rho * v2 + rhoe + Pp) * vn;
+#pragma unroll
+  for (int q = 5; q < NQ; ++q) F[q] = prim[q] * F[DDD];
+}
+
+template <int NQ>
+__device__ __forceinline__ int d_face_flux(const double *primL, const double *primR, int dir, double r,
+                                           double dA, double w, double dt, const DiskParams &P,
+                                           const GravState *gm, const ViscIn *vi, double *X, double *Y) {
+  const double G = P.gamma;
+  const int n0 = dir == 0, n1 = dir == 1, n2 = dir == 2;
+  // riemann_set_vel, :30-55
+  double vnL, cf2L, vnR, cf2R;
+  {
+    double Pp = primL[PPP], rho = primL[RHO], vr = primL[URR], vp = r * primL[UPP], vz = primL[UZZ];
+    vnL = vr * n0 + vp * n1 + vz * n2;
+    cf2L = G * (Pp / rho);
+  }
+  double Sl = vnL - sqrt(cf2L);
+  double Sr = vnL + sqrt(cf2L);
+  {
+    double Pp = primR[PPP], rho = primR[RHO], vr = primR[URR], vp = r * primR[UPP], vz = primR[UZZ];
+    vnR = vr * n0 + vp * n1 + vz * n2;
+    cf2R = G * (Pp / rho);
+  }
+  if (Sl > vnR - sqrt(cf2R)) Sl = vnR - sqrt(cf2R);
+  if (Sr < vnR + sqrt(cf2R)) Sr = vnR + sqrt(cf2R);
+  double Ss = (primR[RHO] * vnR * (Sr - vnR) - primL[RHO] * vnL * (Sl - vnL) + primL[PPP] - primR[PPP]) /
+              (primR[RHO] * (Sr - vnR) - primL[RHO] * (Sl - vnL));
+  // riemann_set_state, :59-71
+  int state;
+  if (w < Sl)
+    state = 0;  // LEFT
+  else if (w > Sr)
+    state = 3;  // RIGHT
+  else if (w < Ss)
+    state = 1;  // LEFTSTAR
+  else
+    state = 2;  // RIGHTSTAR
+  double F[NQ];
+  if (state == 0 || state == 3) {
+    const double *pr = state == 0 ? primL : primR;
+    double Fk[NQ], Uk[NQ];
+    d_set_flux<NQ>(pr, Fk, r, n0, n1, n2, G);
+    d_prim2cons<NQ>(pr, Uk, r, 1.0, G, P.NC);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) F[q] = Fk[q] - w * Uk[q];
+  } else if (P.riemann == HLL_SOLVER) {  // Riemann_routines.c:74-85
+    double FL[NQ], FR[NQ], UL[NQ], UR[NQ];
+    d_set_flux<NQ>(primL, FL, r, n0, n1, n2, G);
+    d_set_flux<NQ>(primR, FR, r, n0, n1, n2, G);
+    d_prim2cons<NQ>(primL, UL, r, 1.0, G, P.NC);
+    d_prim2cons<NQ>(primR, UR, r, 1.0, G, P.NC);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double Ustar = (Sr * UR[q] - Sl * UL[q] + FL[q] - FR[q]) / (Sr - Sl);
+      double Fstar = (Sr * FL[q] - Sl * FR[q] - Sr * Sl * (UL[q] - UR[q])) / (Sr - Sl);
+      F[q] = Fstar - w * Ustar;
+    }
+  } else {  // HLLC, Riemann_routines.c:87-154
+    const double *prim = state == 1 ? primL : primR;
+    double Sk = state == 1 ? Sl : Sr;
+    double Fk[NQ], Uk[NQ], Ustar[NQ];
+    d_set_flux<NQ>(prim, Fk, r, n0, n1, n2, G);
+    d_prim2cons<NQ>(prim, Uk, r, 1.0, G, P.NC);
+    double rho = prim[RHO];
+    double vr = prim[URR];
+    double vp = prim[UPP] * r;
+    double vz = prim[UZZ];
+    double Pp = prim[PPP];
+    double v2 = vr * vr + vp * vp + vz * vz;
+    double vn = vr * n0 + vp * n1 + vz * n2;
+    double rhoe = Pp / (G - 1.);
+    double D = rho;
+    double mr = rho * vr;
+    double mp = rho * vp;
+    double mz = rho * vz;
+    double E_hydro = .5 * rho * v2 + rhoe;
+    double Ps = rho * (Sk - vn) * (Ss - vn) + Pp;
+    double Dstar = (Sk - vn) * D / (Sk - Ss);
+    double Msn = Dstar * Ss;
+    double Msr = (Sk - vn) * mr / (Sk - Ss);
+    double Msp = (Sk - vn) * mp / (Sk - Ss);
+    double Msz = (Sk - vn) * mz / (Sk - Ss);
+    double Estar = ((Sk - vn) * E_hydro + Ps * Ss - Pp * vn) / (Sk - Ss);
+    double mn = Msr * n0 + Msp * n1 + Msz * n2;
+    Msr += n0 * (Msn - mn);
+    Msp += n1 * (Msn - mn);
+    Msz += n2 * (Msn - mn);
+    Ustar[DDD] = Dstar;
+    Ustar[SRR] = Msr;
+    Ustar[LLL] = r * Msp;
+    Ustar[SZZ] = Msz;
+    Ustar[TAU] = Estar;
+#pragma unroll
+    for (int q = 5; q < NQ; ++q) Ustar[q] = prim[q] * Ustar[DDD];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double Fstar = Fk[q] + Sk * (Ustar[q] - Uk[q]);
+      F[q] = Fstar - w * Ustar[q];
+    }
+  }
+  double dtdA = dt * dA;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) X[q] = dtdA * F[q];
+  if (P.visc > 0.0) {  // riemann_visc_flux, :193-311
+    double Avg[5], Grr[5], Gph[5];
+    double tiph = vi->cm;
+    double M0 = gm[0].M, M1 = gm[1].M;
+    double a = gm[0].r + gm[1].r;
+    double dist_bh0 = d_grav_dist(gm[0], r, tiph, 0.);
+    double dist_bh1 = d_grav_dist(gm[1], r, tiph, 0.);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      Avg[q] = .5 * (primL[q] + primR[q]);
+      Gph[q] = .5 * (vi->gpL[q] + vi->gpR[q]) / r;
+      Grr[q] = .5 * (vi->gL[q] + vi->gR[q]);
+      if (n0 == 1) {
+        double pL = vi->tL - .5 * vi->dL;
+        double pR = vi->tR - .5 * vi->dR;
+        double dpL = vi->cm - pL;
+        double dpR = -vi->cm + pR;
+        dpL = wrap_pm(dpL);
+        dpR = wrap_pm(dpR);
+        double WL = vi->pL[q] + dpL * vi->gpL[q];
+        double WR = vi->pR[q] - dpR * vi->gpR[q];
+        double deltaL = r - vi->r_cell_L;
+        double deltaR = vi->r_cell_R - r;
+        Avg[q] = 0.5 * (WL + WR);
+        Grr[q] = (WR - WL) / (deltaR + deltaL);
+      }
+      if (n1 == 1) {
+        Avg[q] = 0.5 * (vi->pR[q] + vi->pL[q]);
+        Gph[q] = (vi->pR[q] - vi->pL[q]) / (vi->tR - vi->tL) / r;
+      }
+    }
+    double nu;
+    double alpha = P.visc;
+    if (P.visc_const == 1) {
+      nu = alpha;
+    } else if (P.initial_data_type == 0) {  // SHEAR
+      nu = alpha * G * Avg[PPP] / Avg[RHO] * pow(fabs(r * cos(tiph)), 2.0);
+      if (r * cos(tiph) > 20.) nu = 0.0;
+    } else {
+      double eps = P.g_eps;
+      nu = alpha * Avg[PPP] / Avg[RHO] /
+           sqrt(pow(dist_bh0 * dist_bh0 + eps * eps, -1.5) * M0 + pow(dist_bh1 * dist_bh1 + eps * eps, -1.5) * M1);
+    }
+    double rho = Avg[RHO], vr = Avg[URR], om = Avg[UPP], vz = Avg[UZZ];
+    if (fabs(vz) > 1.e-12) return DERR_VISC_VZ;
+    double Gr_vr = Grr[URR], Gr_om = Grr[UPP];
+    double Gp_vr = Gph[URR], Gp_om = Gph[UPP];
+    double rdr_om_cell = d_rdrOm_a(P.w_analytic, r, a);
+    double v_srr = 0.0, v_lll = 0.0, v_tau = 0.0;
+    if (n0 == 1) {
+      v_srr = -nu * rho * (r * Gr_vr - vr - r * r * Gp_om);
+      v_lll = -nu * rho * (r * (r * Gr_om + rdr_om_cell) + r * Gp_vr);
+      v_tau = -nu * rho * (vr * (Gr_vr - vr / r - r * Gp_om) + r * om * (Gp_vr + (rdr_om_cell + r * Gr_om)));
+    }
+    if (n1 == 1) {
+      v_srr = -nu * rho * (r * (r * Gr_om + rdr_om_cell) + r * Gp_vr);
+      v_lll = -nu * rho * (-r * Gr_vr + vr + r * r * Gp_om);
+      v_tau = -nu * rho * (-r * om * (Gr_vr - vr / r - r * Gp_om) + vr * (Gp_vr + (rdr_om_cell + r * Gr_om)));
+    }
+    Y[0] = dtdA * v_srr;
+    Y[1] = dtdA * v_lll;
+    Y[2] = dtdA * v_tau;
+  }
+  return 0;
+}
+
+// fgrav and gravMassForce, src/Cell/cell_source.c:11-13, 49-71
+__device__ __forceinline__ double d_fgrav(double M, double r, double eps, int n) {
+  return (M * pow(r, n - 1.) / pow(pow(r, n) + pow(eps, n), 1. + 1. / n));
+}
+__device__ __forceinline__ void d_grav_force(const GravState &g, double eps, int order, double r, double phi,
+                                             double *fr, double *fp) {
+  double rp = g.r;
+  double pp = g.phi;
+  double cosp = cos(phi);
+  double sinp = sin(phi);
+  double dx = r * cosp - rp * cos(pp);
+  double dy = r * sinp - rp * sin(pp);
+  double script_r = sqrt(dx * dx + dy * dy);
+  double cosa = dx / script_r;
+  double sina = dy / script_r;
+  double cosap = cosa * cosp + sina * sinp;
+  double sinap = sina * cosp - cosa * sinp;
+  double f1 = -d_fgrav(g.M, script_r, eps, order);
+  *fr = cosap * f1;
+  *fp = sinap * f1;
+}

@@ -1,0 +1,241 @@
+/*
+ * disco2_amd -- C ABI of the MI355X hydro substep library.
+ *
+ * The entry points keep the reference's opaque-struct "class" API
+ * (reference README.md "Notes"; src/Headers/*.h): every struct is an
+ * incomplete type, every object is made by X_create and freed by
+ * X_destroy, and main()-style drivers call the same functions in the same
+ * order as src/main.c:56-156.  Behind the API the cell state lives in GPU
+ * memory; struct Cell *** is an opaque handle that callers never
+ * dereference (as in the reference, where struct Cell is incomplete).
+ *
+ * Differences from the reference, by design:
+ *   - no exit() inside the library: a fatal condition sets a sticky error
+ *     (disco_last_error / disco_last_error_msg) and the call returns;
+ *   - one process drives one GPU; the MPI communicator is replaced by the
+ *     on-device annulus layout (mpisetup_* describe a single rank);
+ *   - conserved variables are always derived from the primitives (the
+ *     reference recomputes them with cell_calc_cons at the end of every
+ *     substep, src/TimeStep/timestep_routines.c:112).
+ * Each declaration cites the reference interface it replaces.
+ */
+#ifndef DISCO2_AMD_DISCO_H
+#define DISCO2_AMD_DISCO_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct Cell;
+struct Sim;
+struct Face;
+struct GravMass;
+struct MPIsetup;
+struct TimeStep;
+
+/* ---- MPIsetup: src/Headers/MPIsetup.h:21-38 (single rank) ---- */
+struct MPIsetup *mpisetup_create(int argc, char **argv);          /* MPIsetup.h:21 */
+void mpisetup_destroy(struct MPIsetup *);                         /* MPIsetup.h:22 */
+void mpisetup_setprocs(struct MPIsetup *, char *parfile);         /* MPIsetup.h:24 */
+void mpisetup_cart_create(struct MPIsetup *);                     /* MPIsetup.h:25 */
+void mpisetup_left_right(struct MPIsetup *);                      /* MPIsetup.h:26 */
+int mpisetup_check_rin_bndry(struct MPIsetup *);                  /* MPIsetup.h:28 */
+int mpisetup_check_rout_bndry(struct MPIsetup *);                 /* MPIsetup.h:29 */
+int mpisetup_check_zbot_bndry(struct MPIsetup *);                 /* MPIsetup.h:30 */
+int mpisetup_check_ztop_bndry(struct MPIsetup *);                 /* MPIsetup.h:31 */
+int mpisetup_MyProc(struct MPIsetup *);                           /* MPIsetup.h:33 */
+int mpisetup_NumProcs(struct MPIsetup *);                         /* MPIsetup.h:34 */
+int mpisetup_dim_MyProc(struct MPIsetup *, int);                  /* MPIsetup.h:35 */
+int mpisetup_dim_NumProcs(struct MPIsetup *, int);                /* MPIsetup.h:36 */
+int mpisetup_left_Proc(struct MPIsetup *, int);                   /* MPIsetup.h:37 */
+int mpisetup_right_Proc(struct MPIsetup *, int);                  /* MPIsetup.h:38 */
+
+/* ---- Sim: src/Headers/Sim.h:74-139 ---- */
+struct Sim *sim_create(struct MPIsetup *);                         /* Sim.h:75 */
+void sim_alloc_arr(struct Sim *, struct MPIsetup *);               /* Sim.h:76 */
+void sim_destroy(struct Sim *);                                    /* Sim.h:77 */
+int sim_read_par_file(struct Sim *, struct MPIsetup *, char *);    /* Sim.h:132 */
+void sim_set_N_p(struct Sim *);                                    /* Sim.h:133 */
+void sim_set_rz(struct Sim *, struct MPIsetup *);                  /* Sim.h:134 */
+void sim_set_misc(struct Sim *, struct MPIsetup *);                /* Sim.h:135 */
+double sim_MIN(struct Sim *, int);                                 /* Sim.h:79 */
+double sim_MAX(struct Sim *, int);                                 /* Sim.h:80 */
+int sim_N0(struct Sim *, int);                                     /* Sim.h:81 */
+int sim_N_p(struct Sim *, int);                                    /* Sim.h:82 */
+double sim_FacePos(struct Sim *, int, int);                        /* Sim.h:83 */
+int sim_N(struct Sim *, int);                                      /* Sim.h:84 */
+int sim_NoInnerBC(struct Sim *);                                   /* Sim.h:85 */
+int sim_Restart(struct Sim *);                                     /* Sim.h:86 */
+int sim_BoundTypeR(struct Sim *);                                  /* Sim.h:87 */
+int sim_BoundTypeZ(struct Sim *);                                  /* Sim.h:88 */
+int sim_N_global(struct Sim *, int);                               /* Sim.h:90 */
+int sim_Ncells(struct Sim *);                                      /* Sim.h:91 */
+int sim_Ncells_global(struct Sim *);                               /* Sim.h:92 */
+int sim_offset(struct Sim *);                                      /* Sim.h:93 */
+int sim_ng(struct Sim *);                                          /* Sim.h:94 */
+int sim_Nghost_min(struct Sim *, int);                             /* Sim.h:95 */
+int sim_Nghost_max(struct Sim *, int);                             /* Sim.h:96 */
+int sim_W_NUMERIC_TYPE(struct Sim *);                              /* Sim.h:97 */
+int sim_NumGravMass(struct Sim *);                                 /* Sim.h:98 */
+double sim_MassRatio(struct Sim *);                                /* Sim.h:99 */
+double sim_OrbShrinkTscale(struct Sim *);                          /* Sim.h:100 */
+double sim_OrbShrinkT0(struct Sim *);                              /* Sim.h:101 */
+int sim_Riemann(struct Sim *);                                     /* Sim.h:102 */
+double sim_GAMMALAW(struct Sim *);                                 /* Sim.h:103 */
+int sim_COOLING(struct Sim *);                                     /* Sim.h:104 */
+double sim_EXPLICIT_VISCOSITY(struct Sim *);                       /* Sim.h:105 */
+int sim_VISC_CONST(struct Sim *);                                  /* Sim.h:106 */
+double sim_PoRho_r1(struct Sim *);                                 /* Sim.h:107 */
+double sim_CFL(struct Sim *);                                      /* Sim.h:108 */
+double sim_PLM(struct Sim *);                                      /* Sim.h:109 */
+int sim_W_ANALYTIC_TYPE(struct Sim *);                             /* Sim.h:110 */
+int sim_GRAV2D(struct Sim *);                                      /* Sim.h:111 */
+double sim_G_EPS(struct Sim *);                                    /* Sim.h:112 */
+int sim_RhoSinkOn(struct Sim *);                                   /* Sim.h:113 */
+int sim_PHI_ORDER(struct Sim *);                                   /* Sim.h:114 */
+double sim_RHO_FLOOR(struct Sim *);                                /* Sim.h:115 */
+double sim_CS_FLOOR(struct Sim *);                                 /* Sim.h:116 */
+double sim_CS_CAP(struct Sim *);                                   /* Sim.h:117 */
+double sim_VEL_CAP(struct Sim *);                                  /* Sim.h:118 */
+int sim_NUM_C(struct Sim *);                                       /* Sim.h:119 */
+int sim_NUM_Q(struct Sim *);                                       /* Sim.h:120 */
+double sim_get_T_MAX(struct Sim *);                                /* Sim.h:121 */
+int sim_NUM_CHECKPOINTS(struct Sim *);                             /* Sim.h:122 */
+int sim_NUM_DIAG_DUMP(struct Sim *);                               /* Sim.h:123 */
+int sim_NUM_DIAG_MEASURE(struct Sim *);                            /* Sim.h:124 */
+int sim_InitialDataType(struct Sim *);                             /* Sim.h:125 */
+double sim_DAMP_TIME(struct Sim *);                                /* Sim.h:126 */
+double sim_RDAMP_INNER(struct Sim *);                              /* Sim.h:127 */
+double sim_RDAMP_OUTER(struct Sim *);                              /* Sim.h:128 */
+int sim_GravMassType(struct Sim *);                                /* Sim.h:130 */
+double sim_rOm_a(struct Sim *, double r, double a);                /* Sim.h:137 */
+double sim_rdrOm_a(struct Sim *, double r, double a);              /* Sim.h:138 */
+double sim_dtOm_a(struct Sim *, double r, double a);               /* Sim.h:139 */
+/* No reference counterpart: programmatic set-up instead of a parfile.
+ * sim_set_param takes any parfile key (src/Sim/sim_readpar.c:55-106) and
+ * returns 0, or 1 for an unknown key.  sim_set_N_p_array replaces the N_p
+ * computed by sim_set_N_p (call it after sim_set_N_p). */
+int sim_set_param(struct Sim *, const char *key, double value);
+void sim_set_N_p_array(struct Sim *, const int *N_p);
+/* No reference counterpart: the tiph and dphi of every zone that
+ * cell_create (src/Cell/cell_create_destroy.c:64-91) followed by
+ * cell_clean_pi and the z sync of src/main.c:80-85 produce; host only. */
+void disco_initial_edges(struct Sim *, double *tiph, double *dphi);
+
+/* ---- GravMass: src/Headers/GravMass.h:24-46 ---- */
+struct GravMass *gravMass_create(int);                                     /* GravMass.h:24 */
+void gravMass_destroy(struct GravMass *);                                  /* GravMass.h:25 */
+void gravMass_init_none(struct GravMass *, struct Sim *);                  /* GravMass.h:27 */
+void gravMass_init_single(struct GravMass *, struct Sim *);                /* GravMass.h:28 */
+void gravMass_init_binary(struct GravMass *, struct Sim *);                /* GravMass.h:29 */
+void (*gravMass_init_ptr(struct Sim *))(struct GravMass *, struct Sim *);  /* GravMass.h:30 */
+void gravMass_set_chkpt(struct GravMass *, int, double, double, double, double); /* GravMass.h:31 */
+double gravMass_r(struct GravMass *, int);                                 /* GravMass.h:33 */
+double gravMass_phi(struct GravMass *, int);                               /* GravMass.h:34 */
+double gravMass_M(struct GravMass *, int);                                 /* GravMass.h:35 */
+double gravMass_omega(struct GravMass *, int);                             /* GravMass.h:36 */
+double gravMass_Mdot(struct GravMass *, int);                              /* GravMass.h:37 */
+double gravMass_Macc(struct GravMass *, int);                              /* GravMass.h:38 */
+double gravMass_dist(struct GravMass *, int, double, double, double);      /* GravMass.h:39 */
+void gravMass_clean_pi(struct GravMass *, struct Sim *);                   /* GravMass.h:41 */
+void gravMass_copy(struct GravMass *, struct Sim *);                       /* GravMass.h:42 */
+void gravMass_move(struct Sim *, struct GravMass *, double, double);       /* GravMass.h:43 */
+void gravMass_update_RK(struct GravMass *, struct Sim *, double);          /* GravMass.h:44 */
+void gravMass_set_Mdot(struct GravMass *, double, int);                    /* GravMass.h:45 */
+void gravMass_set_Macc(struct GravMass *, double, int);                    /* GravMass.h:46 */
+
+/* ---- Cell: src/Headers/Cell.h:25-91 ---- */
+struct Cell ***cell_create(struct Sim *, struct MPIsetup *);       /* Cell.h:25 */
+void cell_destroy(struct Cell ***, struct Sim *);                  /* Cell.h:26 */
+struct Cell *cell_single(struct Cell ***, int i, int j, int k);    /* Cell.h:50 */
+double cell_prim(struct Cell *, int q);                            /* Cell.h:47 */
+double cell_grad(struct Cell *, int q);                            /* Cell.h:48 */
+double cell_gradp(struct Cell *, int q);                           /* Cell.h:49 */
+double cell_tiph(struct Cell *);                                   /* Cell.h:51 */
+double cell_dphi(struct Cell *);                                   /* Cell.h:52 */
+double cell_wiph(struct Cell *);                                   /* Cell.h:53 */
+void cell_set_prim(struct Cell ***, int i, int j, int k, int q, double v); /* Cell.h:89 */
+void cell_set_tiph(struct Cell ***, int i, int j, int k, double v);        /* Cell.h:90 */
+void cell_clean_pi(struct Cell ***, struct Sim *);                 /* Cell.h:60 */
+void cell_syncproc_r(struct Cell ***, struct Sim *, struct MPIsetup *); /* Cell.h:69 */
+void cell_syncproc_z(struct Cell ***, struct Sim *, struct MPIsetup *); /* Cell.h:70 */
+void cell_calc_cons(struct Cell ***, struct Sim *);                /* Cell.h:83 */
+void cell_calc_prim(struct Cell ***, struct Sim *);                /* Cell.h:81 */
+void cell_prim2cons(double *prim, double *cons, double r, double dV, struct Sim *); /* Cell.h:82 */
+void cell_cons2prim(double *cons, double *prim, double r, double dV, struct Sim *); /* Cell.h:84 */
+double cell_mindt(struct Cell ***, struct Sim *, struct GravMass *); /* Cell.h:86 */
+void cell_clear_w(struct Cell ***, struct Sim *);                  /* Cell.h:65 */
+void cell_set_w(struct Cell ***, struct Sim *);                    /* Cell.h:67 */
+void cell_plm_p(struct Cell ***, struct Sim *);                    /* Cell.h:73 */
+void cell_plm_rz(struct Cell ***, struct Sim *, struct Face *, struct TimeStep *,
+                 struct MPIsetup *, int direction);                /* Cell.h:72 */
+/* No reference counterpart: bulk transfer of whole fields, zones in the
+ * reference's (k, i, j) order, NQ-vectors q-major.  field: 0 prim, 3 grad,
+ * 4 gradp (NQ*Nzones); 5 tiph, 7 dphi, 8 wiph (Nzones); 1 cons (derived). */
+long long cell_nzones(struct Cell ***);
+void cell_upload(struct Cell ***, int field, const double *src);
+void cell_download(struct Cell ***, int field, double *dst);
+
+/* ---- Face: src/Headers/Face.h:22-31 ---- */
+struct Face *face_create(struct Cell ***, struct Sim *, struct TimeStep *, int direction); /* Face.h:22 */
+void face_destroy(struct Face *);                                  /* Face.h:23 */
+struct Cell *face_L_pointer(struct Face *, int n);                 /* Face.h:25 */
+struct Cell *face_R_pointer(struct Face *, int n);                 /* Face.h:26 */
+double face_deltaL(struct Face *, int n);                          /* Face.h:27 */
+double face_deltaR(struct Face *, int n);                          /* Face.h:28 */
+double face_cm(struct Face *, int n);                              /* Face.h:29 */
+double face_dA(struct Face *, int n);                              /* Face.h:30 */
+double face_r(struct Face *, int n);                               /* Face.h:31 */
+/* No reference counterpart: face count and the flat zone index of a cell
+ * returned by face_L_pointer / face_R_pointer / cell_single. */
+int face_count(struct Face *);
+long long cell_zone_index(struct Cell *);
+
+/* ---- TimeStep: src/Headers/TimeStep.h:20-39 ---- */
+struct TimeStep *timestep_create(struct Sim *);                    /* TimeStep.h:24 */
+void timestep_destroy(struct TimeStep *);                          /* TimeStep.h:25 */
+void timestep_rk2(struct TimeStep *, struct Sim *, struct Cell ***, struct GravMass *,
+                  struct MPIsetup *);                              /* TimeStep.h:21 */
+void timestep_forward_euler(struct TimeStep *, struct Sim *, struct Cell ***,
+                            struct GravMass *, struct MPIsetup *); /* TimeStep.h:22 */
+void timestep_set_dt(struct TimeStep *, struct Cell ***, struct Sim *, struct GravMass *); /* TimeStep.h:27 */
+void timestep_set_t(struct TimeStep *, double);                    /* TimeStep.h:28 */
+void timestep_update_t(struct TimeStep *);                         /* TimeStep.h:29 */
+void timestep_set_RK(struct TimeStep *, double);                   /* TimeStep.h:30 */
+void timestep_substep(struct TimeStep *, struct Cell ***, struct Sim *, struct GravMass *,
+                      struct MPIsetup *, double);                  /* TimeStep.h:32 */
+double timestep_get_t(struct TimeStep *);                          /* TimeStep.h:34 */
+double timestep_dt(struct TimeStep *);                             /* TimeStep.h:35 */
+int timestep_n(struct TimeStep *, int index, int direction);       /* TimeStep.h:36 */
+void timestep_set_nri(struct TimeStep *, int, int);                /* TimeStep.h:38 */
+void timestep_set_nzk(struct TimeStep *, int, int);                /* TimeStep.h:39 */
+
+/* ---- batched launch (no reference counterpart) ----
+ * Many independent disks in one set of launches: each disk has its own
+ * Sim (mesh and parameters), GravMass pair, time and dt.  The cell state
+ * of every disk is set with disco_batch_upload before disco_batch_init. */
+struct DiscoBatch;
+struct DiscoBatch *disco_batch_create(int ndisks, struct Sim **sims);
+void disco_batch_destroy(struct DiscoBatch *);
+long long disco_batch_nzones(struct DiscoBatch *, int disk);
+void disco_batch_upload(struct DiscoBatch *, int disk, int field, const double *src);
+void disco_batch_download(struct DiscoBatch *, int disk, int field, double *dst);
+/* gravMass init per disk type, z-periodic sync, derived conserved state */
+void disco_batch_init(struct DiscoBatch *);
+/* nsteps RK2 steps on every disk; graph = 1 replays one captured step */
+void disco_batch_run(struct DiscoBatch *, int nsteps, int graph);
+/* t, dt, RK, 2 x {r, phi, M, omega, RK_r, RK_phi, RK_M, RK_omega, Mdot, Macc} */
+void disco_batch_scalars(struct DiscoBatch *, int disk, double *out23);
+void disco_batch_synchronize(struct DiscoBatch *);
+/* time of the last disco_batch_run in ms (HIP events around the launches) */
+double disco_batch_last_ms(struct DiscoBatch *);
+
+/* ---- errors and version (no reference counterpart; the reference exits) ---- */
+int disco_last_error(void);
+const char *disco_last_error_msg(void);
+void disco_clear_error(void);
+const char *disco_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
