@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark: FP64 zone-updates/s of the hydro RK2 step on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu-baseline]
+
+One "step" is one timestep_rk2 (two substeps) over the whole mesh; a
+zone-update is one interior zone advanced by one step.  The default workload
+is the 2D binary disk with NumR = 512 on r in [0.3, 10], N_phi =
+round(2 pi r / dr) (874,488 interior zones), FP64, HLLC, alpha viscosity,
+synthetic seeded initial state.  The timed region replays a captured RK2
+step graph K times with the state resident in HBM.
+
+Printed: one JSON line with the metric, the HBM roofline of the dominant
+kernel (algorithmic bytes per launch over its HIP-event duration), the
+whole-step roofline, and the reference CPU path timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, published
+B_ALG_PER_ZONE_UPDATE = 320  # 8 * (6*NUM_Q + 10) bytes at NUM_Q = 5
+
+
+def kernel_alg_bytes(name, nz, nfaces_r, nq, visc):
+    """Algorithmic (compulsory) bytes of one launch of a kernel, counting every
+    array the kernel must read or write once.  nz: zones incl. ghosts."""
+    d = 8
+    fx = nq + (3 if visc else 0)  # flux increments stored per face
+    if name == "k_update":
+        # read prim, tiph, RKtiph, dphi, RKcons(avg of the two substeps: written
+        # in the first, read in the second), phi-face and r-face increments,
+        # face lists; write prim, tiph, RKtiph, dphi
+        rd = nz * d * (nq + 3 + nq / 2 + fx) + nfaces_r * (d * fx + 8)
+        wr = nz * d * (nq + 3)
+        return rd + wr
+    if name == "k_flux_r":
+        return nfaces_r * (24 + d * fx) + nz * d * (3 * nq + 2)
+    if name == "k_flux_phi":
+        return nz * d * (2 * nq + 2 + fx)
+    if name == "k_plm_r":
+        return nfaces_r * 24 + nz * d * (2 * nq + 2) + nz * d * nq
+    if name == "k_plm_p":
+        return nz * d * (nq + 1) + nz * d * nq
+    if name == "k_faces_r":
+        return nz * d * 2 + nfaces_r * 24 + nz * 4
+    return None
+
+
+def cpu_baseline(params, setup, prim0, steps):
+    """The reference C code (oracle/_ref build) on this host, single thread;
+    falls back to the CPU restatement of the oracle when the reference build
+    is missing."""
+    from oracle import oracle as orc
+    from disco2_amd.params import write_parfile
+    n_int = setup.interior_zones()
+    if orc.have_reference():
+        with tempfile.TemporaryDirectory() as tmp:
+            par = os.path.join(tmp, "w.par")
+            ic = os.path.join(tmp, "w.ic")
+            write_parfile(par, params)
+            orc.write_icfile(ic, setup.NR, setup.NZ, setup.NQ, N_p=setup.N_p, prim=prim0)
+            try:
+                r = orc.run_reference(["time", par, ic, str(steps)], tmp, timeout=900)
+                rec = json.loads(r.stderr.strip().splitlines()[-1])
+                return {"value": rec["zone_updates_per_s"], "unit": "zone-updates/s", "cores": 1,
+                        "kind": "reference",
+                        "sample": f"{steps} RK2 steps of the same workload ({rec['interior_zones']} interior "
+                                  f"zones), reference Disco2 C code built with gcc -O3, 1 MPI rank, stdout to "
+                                  f"/dev/null, step loop only"}
+            except Exception as e:  # fall through to the restatement
+                print(f"reference baseline unavailable: {e}", file=sys.stderr)
+    o = orc.Oracle(params, setup.N_p, setup.r_faces, setup.z_faces)
+    o.set_state(tiph=setup.tiph0, prim=prim0)
+    o.init()
+    t0 = time.perf_counter()
+    o.step(steps)
+    el = time.perf_counter() - t0
+    return {"value": n_int * steps / el, "unit": "zone-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} RK2 steps of the same workload, CPU restatement (oracle/), 1 thread"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="binary2d_r512")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one disk or batch per "
+              "GPU); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from disco2_amd import configs, ics
+    from disco2_amd.api import Batch, Setup
+
+    params, ic_kind, ic_kw = configs.WORKLOADS[args.workload]()
+    setup = Setup(params)
+    prim0 = ics.make_ic(ic_kind, setup.geometry(), params, **ic_kw)
+    n_int = setup.interior_zones()
+    batch = Batch([setup])
+    batch.upload("prim", prim0)
+    batch.init()
+    graph = not args.no_graph
+    print(f"[bench] {args.workload}: {setup.nzones} zones ({n_int} interior), warmup {args.warmup}",
+          file=sys.stderr, flush=True)
+    batch.run(args.warmup, graph=graph)
+
+    # timed region: K steps, synchronised on both sides
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.run(args.steps, graph=graph, sync=False)
+    batch.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = batch.last_ms()
+    ms_per_step = 1e3 * wall / args.steps
+    value = n_int * args.steps / wall
+    print(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step (events {ev_ms / args.steps:.3f})",
+          file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event durations on the batch's stream
+    prof = batch.profile(args.profile_steps)
+    nfr = 2 * setup.nzones  # about N_in + N_out faces per annulus pair
+    nq = setup.NQ
+    visc = params["Explicit_Viscosity"] > 0
+    dom = max(prof.items(), key=lambda kv: kv[1][0])
+    dom_name, (dom_ms, dom_n) = dom
+    dom_avg_ms = dom_ms / dom_n
+    alg = kernel_alg_bytes(dom_name, setup.nzones, nfr, nq, visc)
+    roof = {"kernel": dom_name, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": None, "traffic": None, "avg_launch_ms": round(dom_avg_ms, 5)}
+    if alg:
+        ach = alg / (dom_avg_ms * 1e-3) / 1e9
+        roof.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), alg_bytes_per_launch=int(alg))
+    step_ach = value * B_ALG_PER_ZONE_UPDATE / 1e9
+    kernels = {k: {"avg_ms": round(v[0] / v[1], 5), "share": round(v[0] / sum(x[0] for x in prof.values()), 4)}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(params, setup, prim0, args.cpu_steps)
+
+    out = {
+        "metric": "FP64 zone-updates/s on 1 GPU (plain single disk)",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "NumR": params["NumR"], "NumZ": params["NumZ"],
+                   "zones": setup.nzones, "interior_zones": n_int, "riemann": "HLLC" if params["RiemannSolver"] == 0 else "HLL",
+                   "graph": graph, "parallelism": "single disk, 1 GPU"},
+        "roofline": roof,
+        "step_roofline": {"alg_bytes_per_zone_update": B_ALG_PER_ZONE_UPDATE, "achieved": round(step_ach, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    batch.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

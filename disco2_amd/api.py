@@ -116,6 +116,17 @@ class Batch:
     def last_ms(self):
         return lib().disco_batch_last_ms(self.h)
 
+    def profile(self, nsteps):
+        """Per-kernel (total ms, launches) over nsteps eager steps."""
+        maxk = 64
+        names = ctypes.create_string_buffer(32 * maxk)
+        ms = np.zeros(maxk)
+        cnt = np.zeros(maxk, dtype=np.int32)
+        nk = lib().disco_batch_profile(self.h, int(nsteps), names, dptr(ms), iptr(cnt), maxk)
+        check()
+        raw = names.raw
+        return {raw[32 * k:32 * k + 32].split(b"\0")[0].decode(): (float(ms[k]), int(cnt[k])) for k in range(nk)}
+
     def scalars(self, disk=0):
         out = np.empty(23)
         lib().disco_batch_scalars(self.h, disk, dptr(out))

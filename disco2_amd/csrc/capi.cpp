@@ -477,6 +477,74 @@ void disco_batch_synchronize(DiscoBatch *b) {
   b->e->check_error();
 }
 double disco_batch_last_ms(DiscoBatch *b) { return b->e->last_ms; }
+
+// Eager steps with every kernel launch bracketed by HIP events on the
+// batch's stream; per kernel name: total ms and launch count.
+int disco_batch_profile(DiscoBatch *b, int nsteps, char *names, double *ms, int *counts, int maxk) {
+  Engine *e = b->e;
+  int cap = nsteps * 64 + 16;
+  std::vector<KMark> marks(cap);
+  for (auto &m : marks) {
+    hipEventCreate(&m.a);
+    hipEventCreate(&m.b);
+  }
+  KProf prof{marks.data(), 0, cap};
+  e->synchronize();
+  g_kprof = &prof;
+  for (int s = 0; s < nsteps; ++s) e->enqueue_rk2();
+  g_kprof = nullptr;
+  hip_check(hipStreamSynchronize(e->st), "profile");
+  int nk = 0;
+  std::vector<std::string> keys;
+  for (int i = 0; i < prof.n; ++i) {
+    float t = 0.f;
+    hipEventElapsedTime(&t, marks[i].a, marks[i].b);
+    std::string nm(marks[i].name);
+    int k = (int)(std::find(keys.begin(), keys.end(), nm) - keys.begin());
+    if (k == (int)keys.size()) {
+      if (nk >= maxk) continue;
+      keys.push_back(nm);
+      ms[nk] = 0.0;
+      counts[nk] = 0;
+      strncpy(names + 32 * nk, nm.c_str(), 31);
+      names[32 * nk + 31] = 0;
+      ++nk;
+    }
+    ms[k] += t;
+    counts[k] += 1;
+  }
+  for (auto &m : marks) {
+    hipEventDestroy(m.a);
+    hipEventDestroy(m.b);
+  }
+  e->check_error();
+  return nk;
+}
+
+// A plain device copy of n doubles (known HBM traffic: 8n read + 8n write),
+// used to calibrate hardware byte counters; returns the copy time in ms.
+double disco_calibrate_copy(long long n) {
+  double *a = nullptr, *b = nullptr;
+  if (!hip_check(hipMalloc((void **)&a, n * sizeof(double)), "calib") ||
+      !hip_check(hipMalloc((void **)&b, n * sizeof(double)), "calib"))
+    return -1.0;
+  hipMemset(a, 0, n * sizeof(double));
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  launch_copy(b, a, n, 0);
+  hipEventRecord(e0, 0);
+  launch_copy(b, a, n, 0);
+  hipEventRecord(e1, 0);
+  hipEventSynchronize(e1);
+  float t = 0.f;
+  hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  return t;
+}
 void disco_batch_scalars(DiscoBatch *b, int d, double *o) {
   DiskScalars s = b->e->get_scalars(d);
   o[0] = s.t;

@@ -39,6 +39,30 @@ struct LaunchPlan {
   const int2 *sz; int n_sz;             // z ghost layers
 };
 
+// Optional per-kernel timing: when g_kprof is set, every launch is bracketed
+// by HIP events on its stream (eager launches only, never under capture).
+struct KMark {
+  const char *name;
+  hipEvent_t a, b;
+};
+struct KProf {
+  KMark *marks;
+  int n, cap;
+};
+extern KProf *g_kprof;
+#define KLAUNCH(st, nm, ...)                                              \
+  do {                                                                    \
+    if (g_kprof && g_kprof->n < g_kprof->cap) {                           \
+      KMark &mk_ = g_kprof->marks[g_kprof->n++];                          \
+      mk_.name = nm;                                                      \
+      hipEventRecord(mk_.a, st);                                          \
+      __VA_ARGS__;                                                        \
+      hipEventRecord(mk_.b, st);                                          \
+    } else {                                                              \
+      __VA_ARGS__;                                                        \
+    }                                                                     \
+  } while (0)
+
 enum { SC_STEP_BEGIN, SC_SUB_BEGIN, SC_AFTER_UPDATE, SC_SUB1_END, SC_SUB2_END, SC_EULER_END, SC_RESET_DT };
 
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
@@ -48,3 +72,4 @@ void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st);
 void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ);
 void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir);
 void launch_plm(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int which);
+void launch_copy(double *dst, const double *src, long long n, hipStream_t st);

@@ -1057,29 +1057,34 @@ __global__ void k_scalar(DevMesh m, int ndisk, int op, double arg) {
 // ----------------------------------------------------------------------------
 // host-side launchers
 // ----------------------------------------------------------------------------
+KProf *g_kprof = nullptr;
+
 template <int NQ>
 static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, double fac, int copy_first,
                              int recompute_w) {
   const dim3 B(TILE);
-  hipLaunchKernelGGL(k_scalar, dim3((L.ndisk + 63) / 64), dim3(64), 0, st, m, L.ndisk, SC_SUB_BEGIN, 0.0);
-  if (L.n_rpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), B, 0, st, m, 0);
-  if (L.n_zpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), B, 0, st, m, L.n_rpairs);
-  if (recompute_w) hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0);
-  hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles);
-  hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0);
-  hipLaunchKernelGGL(k_flux_phi<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, fac);
-  if (L.n_rftiles) hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles, fac);
+  const dim3 SD((L.ndisk + 63) / 64), S64(64);
+  KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, SC_SUB_BEGIN, 0.0));
+  if (L.n_rpairs) KLAUNCH(st, "k_faces_r", hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), B, 0, st, m, 0));
+  if (L.n_zpairs) KLAUNCH(st, "k_faces_z", hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), B, 0, st, m, L.n_rpairs));
+  if (recompute_w) KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
+  KLAUNCH(st, "k_plm_p", hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
+  KLAUNCH(st, "k_plm_r", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0));
+  KLAUNCH(st, "k_flux_phi", hipLaunchKernelGGL(k_flux_phi<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, fac));
+  if (L.n_rftiles)
+    KLAUNCH(st, "k_flux_r", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles, fac));
   if (L.n_zpairs) {
-    hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 1);
-    hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_zftiles), B, 0, st, m, L.zftiles, fac);
+    KLAUNCH(st, "k_plm_z", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 1));
+    KLAUNCH(st, "k_flux_z", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_zftiles), B, 0, st, m, L.zftiles, fac));
   }
-  hipLaunchKernelGGL(k_update<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, fac, copy_first);
-  hipLaunchKernelGGL(k_scalar, dim3((L.ndisk + 63) / 64), dim3(64), 0, st, m, L.ndisk, SC_AFTER_UPDATE, 0.0);
+  KLAUNCH(st, "k_update",
+          hipLaunchKernelGGL(k_update<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, fac, copy_first));
+  KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, SC_AFTER_UPDATE, 0.0));
   DevMesh m2 = m;  // after the update the new edges live in tiph_new
   m2.tiph = m.tiph_new;
-  if (L.n_bc0) hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc0), B, 0, st, m2, L.bc0);
-  if (L.n_bc1) hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc1), B, 0, st, m2, L.bc1);
-  if (L.n_sz) hipLaunchKernelGGL(k_sync_z<NQ>, dim3(L.n_sz), B, 0, st, m, L.sz);
+  if (L.n_bc0) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc0), B, 0, st, m2, L.bc0));
+  if (L.n_bc1) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc1), B, 0, st, m2, L.bc1));
+  if (L.n_sz) KLAUNCH(st, "k_sync_z", hipLaunchKernelGGL(k_sync_z<NQ>, dim3(L.n_sz), B, 0, st, m, L.sz));
 }
 
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
@@ -1091,18 +1096,28 @@ void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int N
 }
 
 void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op) {
-  hipLaunchKernelGGL(k_scalar, dim3((L.ndisk + 63) / 64), dim3(64), 0, st, m, L.ndisk, op, 0.0);
+  KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, dim3((L.ndisk + 63) / 64), dim3(64), 0, st, m, L.ndisk, op, 0.0));
 }
 
 void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st) {
-  hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0);
+  KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
 }
 
 void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ) {
   if (NQ == 5)
-    hipLaunchKernelGGL(k_mindt<5>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles);
+    KLAUNCH(st, "k_mindt", hipLaunchKernelGGL(k_mindt<5>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles));
   else
-    hipLaunchKernelGGL(k_mindt<6>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles);
+    KLAUNCH(st, "k_mindt", hipLaunchKernelGGL(k_mindt<6>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles));
+}
+
+// plain copy of n doubles: the reference point for HBM byte counters
+__global__ __launch_bounds__(TILE) void k_copy(double *dst, const double *src, long long n) {
+  long long i = (long long)blockIdx.x * TILE + threadIdx.x;
+  long long stride = (long long)gridDim.x * TILE;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+void launch_copy(double *dst, const double *src, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy, dim3(2048), dim3(TILE), 0, st, dst, src, n);
 }
 
 void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir) {

@@ -228,6 +228,12 @@ void disco_batch_scalars(struct DiscoBatch *, int disk, double *out23);
 void disco_batch_synchronize(struct DiscoBatch *);
 /* time of the last disco_batch_run in ms (HIP events around the launches) */
 double disco_batch_last_ms(struct DiscoBatch *);
+/* nsteps eager RK2 steps with HIP events around every kernel launch; fills
+ * up to maxk distinct kernel names (32 chars each), total ms and launch
+ * counts; returns the number of names */
+int disco_batch_profile(struct DiscoBatch *, int nsteps, char *names, double *ms, int *counts, int maxk);
+/* device copy of n doubles (8n bytes read, 8n written); returns ms */
+double disco_calibrate_copy(long long n);
 
 /* ---- errors and version (no reference counterpart; the reference exits) ---- */
 int disco_last_error(void);
