@@ -88,6 +88,49 @@ def cpu_baseline(params, setup, prim0, steps):
             "sample": f"{steps} RK2 steps of the same workload, CPU restatement (oracle/), 1 thread"}
 
 
+def batched_leg(steps, warmup, ndisk=64):
+    """Many independent disks: one batched launch set against one launch set
+    per disk run disk after disk.  Both run the same disks for the same number
+    of RK2 steps from the same initial states."""
+    from disco2_amd import configs, ics
+    from disco2_amd.api import Batch, Setup
+    specs = configs.c4_batch(ndisk)
+    setups, prims = [], []
+    for p, kind, kw in specs:
+        s = Setup(p)
+        setups.append(s)
+        prims.append(ics.make_ic(kind, s.geometry(), p, **kw))
+    n_int = sum(s.interior_zones() for s in setups)
+    big = Batch(setups)
+    for d, pr in enumerate(prims):
+        big.upload("prim", pr, disk=d)
+    big.init()
+    big.run(warmup)
+    big.synchronize()
+    t0 = time.perf_counter()
+    big.run(steps, sync=False)
+    big.synchronize()
+    t_batched = time.perf_counter() - t0
+    big.close()
+    t_plain = 0.0
+    for s, pr in zip(setups, prims):
+        b = Batch([s])
+        b.upload("prim", pr)
+        b.init()
+        b.run(warmup)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.run(steps, sync=False)
+        b.synchronize()
+        t_plain += time.perf_counter() - t0
+        b.close()
+    return {"workload": f"{ndisk} binary disks, NumR uniform in [64, 384], q, alpha, Mach drawn per disk",
+            "interior_zones": n_int, "steps": steps,
+            "batched": {"value": n_int * steps / t_batched, "ms_per_step": 1e3 * t_batched / steps},
+            "per_disk_launches": {"value": n_int * steps / t_plain, "ms_per_step": 1e3 * t_plain / steps},
+            "speedup": t_plain / t_batched, "unit": "zone-updates/s"}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +141,8 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=20)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one disk or batch per "
@@ -149,6 +194,11 @@ def main(argv=None):
     kernels = {k: {"avg_ms": round(v[0] / v[1], 5), "share": round(v[0] / sum(x[0] for x in prof.values()), 4)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
 
+    batched = None
+    if not args.no_batched:
+        print("[bench] batched disks", file=sys.stderr, flush=True)
+        batched = batched_leg(args.batched_steps, 2)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -174,6 +224,7 @@ def main(argv=None):
         "step_roofline": {"alg_bytes_per_zone_update": B_ALG_PER_ZONE_UPDATE, "achieved": round(step_ach, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -57,6 +57,25 @@ def c5_adjoint():
     return binary(256, r_min=0.3, r_max=8.0, q=0.5, alpha=0.01)
 
 
+def c4_batch(ndisk=64, seed=2024, numr_range=(64, 384)):
+    """Independent binary disks for one batched launch: NumR uniform in
+    numr_range on r in [0.3, 10], mass ratio log-uniform in [0.01, 1], alpha
+    log-uniform in [1e-3, 1e-1], Mach uniform in [10, 30]; the density noise
+    of disk d is seeded with d."""
+    import numpy as np
+    rs = np.random.RandomState(seed)
+    out = []
+    for d in range(ndisk):
+        numr = int(rs.randint(numr_range[0], numr_range[1] + 1))
+        q = float(10 ** rs.uniform(-2.0, 0.0))
+        alpha = float(10 ** rs.uniform(-3.0, -1.0))
+        mach = float(rs.uniform(10.0, 30.0))
+        p, kind, kw = binary(numr, r_min=0.3, r_max=10.0, q=q, alpha=alpha, mach=mach)
+        kw = dict(kw, seed=1, disk=d)
+        out.append((p, kind, kw))
+    return out
+
+
 WORKLOADS = {
     "vortex2d_r64": c1_vortex,
     "binary2d_r512": c2_binary,

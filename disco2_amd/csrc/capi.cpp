@@ -291,6 +291,7 @@ double cell_mindt(Cell ***c, Sim *, GravMass *gm) {  // src/Cell/cell_mindt.c:32
   Engine *e = h->eng;
   scalars_from_host(e, nullptr, gm);
   launch_scalar(e->mesh(), e->plan, e->st, SC_RESET_DT);
+  launch_scalar(e->mesh(), e->plan, e->st, SC_MASS);
   launch_mindt(e->mesh(), e->plan, e->st, e->NQ);
   DiskScalars s = e->get_scalars(0);
   double dt;

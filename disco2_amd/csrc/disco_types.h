@@ -52,6 +52,9 @@ struct DiskScalars {
   unsigned long long dtmin_bits;  // CFL minimum of the step (as ordered bits)
   int err, pad2;
   double mdot_acc[2];
+  // launch-uniform terms, recomputed whenever the masses move:
+  // (xbh, ybh) = (r cos phi, r sin phi) of each mass, epsn = pow(G_EPS, PHI_ORDER)
+  double xbh[2], ybh[2], epsn, pad3;
 };
 
 struct AnnInfo {

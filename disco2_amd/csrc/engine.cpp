@@ -438,6 +438,7 @@ void Engine::enqueue_substep(double fac, int copy_first, int recompute_w) {
 // timestep_rk2, src/TimeStep/timestep_algorithm.c:11-41
 void Engine::enqueue_rk2() {
   DevMesh m = mesh();
+  launch_scalar(m, plan, st, SC_MASS);
   launch_set_w(m, plan, st);
   launch_mindt(m, plan, st, NQ);
   launch_scalar(m, plan, st, SC_STEP_BEGIN);
@@ -450,6 +451,7 @@ void Engine::enqueue_rk2() {
 // timestep_forward_euler, src/TimeStep/timestep_algorithm.c:43-57
 void Engine::enqueue_euler() {
   DevMesh m = mesh();
+  launch_scalar(m, plan, st, SC_MASS);
   launch_set_w(m, plan, st);
   launch_mindt(m, plan, st, NQ);
   launch_scalar(m, plan, st, SC_STEP_BEGIN);

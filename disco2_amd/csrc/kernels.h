@@ -63,7 +63,7 @@ extern KProf *g_kprof;
     }                                                                     \
   } while (0)
 
-enum { SC_STEP_BEGIN, SC_SUB_BEGIN, SC_AFTER_UPDATE, SC_SUB1_END, SC_SUB2_END, SC_EULER_END, SC_RESET_DT };
+enum { SC_STEP_BEGIN, SC_SUB_BEGIN, SC_AFTER_UPDATE, SC_SUB1_END, SC_SUB2_END, SC_EULER_END, SC_RESET_DT, SC_MASS };
 
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
                     int recompute_w);

@@ -14,6 +14,16 @@
 
 __device__ __forceinline__ void set_err(int *err, int code) { atomicCAS(err, 0, code); }
 
+// XCD-aware work order.  Workgroups are dispatched round-robin over the 8
+// XCDs (workgroup b runs on XCD b mod 8), each with its own L2.  Tiles are
+// listed annulus by annulus, so handing XCD x one contiguous band of tiles
+// keeps the neighbouring annuli that a tile's faces reach in the same L2.
+__device__ __forceinline__ int xcd_order() {
+  const int n = gridDim.x, b = blockIdx.x;
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 __device__ __forceinline__ const DiskParams &disk_of(const DevMesh &m, int d) { return m.dp[d]; }
 
 __device__ __forceinline__ int rpair_of(const DiskParams &P, int k, int i) { return P.rpair_base + k * (P.NR - 1) + i; }
@@ -119,24 +129,51 @@ __device__ int guess_outer(double e, const double *Tout, int Nout) {
   return c;
 }
 
-__global__ __launch_bounds__(TILE) void k_faces(DevMesh m, int pair0) {
+#define FTB 1024  // threads of the face builder (one workgroup per annulus pair)
+
+// exclusive prefix sum over the workgroup (wave shuffles, then wave totals)
+__device__ int block_excl_scan(int v, int *s_w, int *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    int t = lane < nw ? s_w[lane] : 0;
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(t, o);
+      if (lane >= o) t += y;
+    }
+    if (lane < nw) s_w[lane] = t;
+  }
+  __syncthreads();
+  int base = w > 0 ? s_w[w - 1] : 0;
+  *total = s_w[nw - 1];
+  __syncthreads();
+  return base + x - v;
+}
+
+__global__ __launch_bounds__(FTB) void k_faces(DevMesh m, int pair0) {
   const int tid = threadIdx.x;
-  const int p = pair0 + blockIdx.x;
+  const int p = pair0 + xcd_order();
   const PairInfo P = m.pairs[p];
   const AnnInfo A = m.ann[P.ann_in];
   const AnnInfo B = m.ann[P.ann_out];
   const int Nin = A.N, Nout = B.N;
   const double *Tin = m.tiph + A.zbase, *Din = m.dphi + A.zbase;
   const double *Tout = m.tiph + B.zbase, *Dout = m.dphi + B.zbase;
-  __shared__ double s_d[TILE];
-  __shared__ int s_j[TILE];
-  __shared__ int s_scan[TILE];
+  __shared__ double s_d[FTB];
+  __shared__ int s_j[FTB];
+  __shared__ int s_w[FTB / 64];
 
   // jpmin: first outer cell minimising wrap+(tiph_out - p0), :82-96
   const double p0 = Tin[Nin - 1];
   double bd = TWO_PI;
   int bj = -1;
-  for (int jp = tid; jp < Nout; jp += TILE) {
+  for (int jp = tid; jp < Nout; jp += FTB) {
     double dp = wrap_p(Tout[jp] - p0);
     if (bd > dp) {
       bd = dp;
@@ -146,7 +183,7 @@ __global__ __launch_bounds__(TILE) void k_faces(DevMesh m, int pair0) {
   s_d[tid] = bd;
   s_j[tid] = bj;
   __syncthreads();
-  for (int s = TILE / 2; s > 0; s >>= 1) {
+  for (int s = FTB / 2; s > 0; s >>= 1) {
     if (tid < s) {
       int j2 = s_j[tid + s];
       if (j2 >= 0) {
@@ -181,39 +218,31 @@ __global__ __launch_bounds__(TILE) void k_faces(DevMesh m, int pair0) {
 
   // guess, count and verify
   int bad = 0;
-  for (int j = tid; j < Nin; j += TILE) {
+  for (int j = tid; j < Nin; j += FTB) {
     int jps = (j == 0) ? jpmin : guess_outer(Tin[j - 1], Tout, Nout);
     int jpe;
     int cnt = jloop_cell<false>(j, jps, Tin, Din, Tout, Dout, Nin, Nout, 0, o, &jpe, &bad);
     js[j] = make_int4(jps, cnt, jpe, 0);
   }
   __syncthreads();
-  for (int j = tid; j + 1 < Nin; j += TILE)
+  for (int j = tid; j + 1 < Nin; j += FTB)
     if (js[j].z != js[j + 1].x) bad = 1;
   bad = __syncthreads_or(bad);
 
   int total = 0;
   if (!bad) {
     int carry = 0;
-    for (int base = 0; base < Nin; base += TILE) {
+    for (int base = 0; base < Nin; base += FTB) {
       int j = base + tid;
       int4 v = (j < Nin) ? js[j] : make_int4(0, 0, 0, 0);
-      s_scan[tid] = v.y;
-      __syncthreads();
-      for (int off = 1; off < TILE; off <<= 1) {
-        int add = (tid >= off) ? s_scan[tid - off] : 0;
-        __syncthreads();
-        s_scan[tid] += add;
-        __syncthreads();
-      }
-      int foff = carry + s_scan[tid] - v.y;
+      int chunk;
+      int foff = carry + block_excl_scan(v.y, s_w, &chunk);
       if (j < Nin) {
         fo[j] = foff;
         int jpe, b2 = 0;
         jloop_cell<true>(j, v.x, Tin, Din, Tout, Dout, Nin, Nout, foff, o, &jpe, &b2);
       }
-      carry += s_scan[TILE - 1];
-      __syncthreads();
+      carry += chunk;
     }
     total = carry;
   } else if (tid == 0) {
@@ -277,7 +306,7 @@ __device__ __forceinline__ void faces_as_R(const DevMesh &m, int p, int jo, int 
 // One wavefront per annulus; C_RIGID sums in the reference's cell order.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_set_w(DevMesh m, int ann0) {
-  const int a = ann0 + blockIdx.x;
+  const int a = ann0 + xcd_order();
   const AnnInfo A = m.ann[a];
   const DiskParams &P = disk_of(m, A.disk);
   const int lane = threadIdx.x;
@@ -326,7 +355,7 @@ __global__ __launch_bounds__(64) void k_set_w(DevMesh m, int ann0) {
 // ----------------------------------------------------------------------------
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
-  const int2 tl = tiles[blockIdx.x];
+  const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -373,9 +402,26 @@ __device__ __forceinline__ void face_slope(const DevMesh &m, const PairInfo &P, 
   }
 }
 
+// The face difference S = (W_R - W_L)/(deltaL + deltaR) of every overlap
+// face (cell_plm.c:36-49 and 86-99), computed once per face and stored in the
+// first NQ rows of the face-increment buffer (the fluxes overwrite them later).
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_face_slope(DevMesh m, const int2 *ftiles) {
+  const int2 tl = ftiles[xcd_order()];
+  const int p = tl.x;
+  const int f = tl.y + threadIdx.x;
+  if (f >= m.pst[p].count) return;
+  const PairInfo PI = m.pairs[p];
+  const long long g = PI.face_base + f;
+  double S[NQ];
+  face_slope<NQ>(m, PI, g, S);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = S[q];
+}
+
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, int dir) {
-  const int2 tl = tiles[blockIdx.x];
+  const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -396,7 +442,8 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
   }
   auto acc = [&](const PairInfo &PI, long long gi) {
     double S[NQ];
-    face_slope<NQ>(m, PI, gi, S);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) S[q] = m.Fface[q * m.NFtot + gi];
     double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) g[q] += S[q] * dA;
@@ -422,7 +469,8 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
   const double PLM = P.plm;
   auto lim = [&](const PairInfo &PI, long long gi) {
     double S[NQ];
-    face_slope<NQ>(m, PI, gi, S);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) S[q] = m.Fface[q * m.NFtot + gi];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       if (S[q] * g[q] < 0.0)
@@ -458,7 +506,7 @@ __device__ __forceinline__ void load_visc_cell(const DevMesh &m, long long c, do
 
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_flux_phi(DevMesh m, const int2 *tiles, double fac) {
-  const int2 tl = tiles[blockIdx.x];
+  const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -487,7 +535,7 @@ __global__ __launch_bounds__(TILE) void k_flux_phi(DevMesh m, const int2 *tiles,
     vi.r_cell_R = r;
   }
   double X[NQ], Y[3] = {0.0, 0.0, 0.0};
-  int e = d_face_flux<NQ>(primL, primR, 1, r, dA, m.wiph[tl.x], dt, P, sc.gm, &vi, X, Y);
+  int e = d_face_flux<NQ>(primL, primR, 1, r, dA, m.wiph[tl.x], dt, P, sc, &vi, X, Y);
   if (e) set_err(m.err, e);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) m.Fphi[q * m.Ntot + cL] = X[q];
@@ -499,7 +547,7 @@ __global__ __launch_bounds__(TILE) void k_flux_phi(DevMesh m, const int2 *tiles,
 
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
-  const int2 tl = ftiles[blockIdx.x];
+  const int2 tl = ftiles[xcd_order()];
   const int p = tl.x;
   const int f = tl.y + threadIdx.x;
   if (f >= m.pst[p].count) return;
@@ -540,7 +588,7 @@ __global__ __launch_bounds__(TILE) void k_flux_rz(DevMesh m, const int2 *ftiles,
     }
   }
   double X[NQ], Y[3] = {0.0, 0.0, 0.0};
-  int e = d_face_flux<NQ>(primL, primR, PI.dir == 0 ? 0 : 2, r, dA, 0.0, dt, P, sc.gm, &vi, X, Y);
+  int e = d_face_flux<NQ>(primL, primR, PI.dir == 0 ? 0 : 2, r, dA, 0.0, dt, P, sc, &vi, X, Y);
   if (e) set_err(m.err, e);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = X[q];
@@ -598,7 +646,7 @@ __device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy
 
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
-  const int2 tl = tiles[blockIdx.x];
+  const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   __shared__ double s_md[2][TILE / 64];
@@ -668,6 +716,7 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
     const double a = g0.r + g1.r;
     {
       double phi = tcl - .5 * dphi;
+      const double cphi = cos(phi), sphi = sin(phi);
       double rho = pr[RHO], Pp = pr[PPP], vr = pr[URR], vz = pr[UZZ];
       double vp = pr[UPP] * r;
       double dV = dphi * .5 * (rp * rp - rm * rm) * dz;
@@ -684,10 +733,10 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
         gravdist = R;
       }
       double fr, fp, Fr = 0.0, Fp = 0.0;
-      d_grav_force(g0, P.g_eps, P.phi_order, gravdist, phi, &fr, &fp);
+      d_grav_force(g0.M, sc.xbh[0], sc.ybh[0], sc.epsn, P.phi_order, gravdist, cphi, sphi, &fr, &fp);
       Fr += fr;
       Fp += fp;
-      d_grav_force(g1, P.g_eps, P.phi_order, gravdist, phi, &fr, &fp);
+      d_grav_force(g1.M, sc.xbh[1], sc.ybh[1], sc.epsn, P.phi_order, gravdist, cphi, sphi, &fr, &fp);
       Fr += fr;
       Fp += fp;
       double w_a = d_rOm_a(P.w_analytic, r, a);
@@ -701,9 +750,9 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
       u[SZZ] += dt * dV * rho * Fr * cost;
       u[TAU] += dt * dV * rho * ((Fr * sint + F_centrifugal_r) * vr + Fr * vz * cost + (Fp + F_euler_phi) * vp);
       if (P.sink == 1) {  // get_rho_sink, :15-47
-        double xpos = r * cos(phi), ypos = r * sin(phi);
-        double xbh0 = g0.r * cos(g0.phi), ybh0 = g0.r * sin(g0.phi);
-        double xbh1 = g1.r * cos(g1.phi), ybh1 = g1.r * sin(g1.phi);
+        double xpos = r * cphi, ypos = r * sphi;
+        double xbh0 = sc.xbh[0], ybh0 = sc.ybh[0];
+        double xbh1 = sc.xbh[1], ybh1 = sc.ybh[1];
         double r0 = sqrt((xpos - xbh0) * (xpos - xbh0) + (ypos - ybh0) * (ypos - ybh0));
         double r1 = sqrt((xpos - xbh1) * (xpos - xbh1) + (ypos - ybh1) * (ypos - ybh1));
         double M0 = g0.M, M1 = g1.M;
@@ -743,9 +792,9 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
       }
       // cell_add_visc_src, :216-277
       if (visc) {
-        double xpos = r * cos(phi), ypos = r * sin(phi);
-        double xbh0 = g0.r * cos(g0.phi), ybh0 = g0.r * sin(g0.phi);
-        double xbh1 = g1.r * cos(g1.phi), ybh1 = g1.r * sin(g1.phi);
+        double xpos = r * cphi, ypos = r * sphi;
+        double xbh0 = sc.xbh[0], ybh0 = sc.ybh[0];
+        double xbh1 = sc.xbh[1], ybh1 = sc.ybh[1];
         double dist_bh0 = sqrt((xpos - xbh0) * (xpos - xbh0) + (ypos - ybh0) * (ypos - ybh0));
         double dist_bh1 = sqrt((xpos - xbh1) * (xpos - xbh1) + (ypos - ybh1) * (ypos - ybh1));
         double eps = P.g_eps;
@@ -823,7 +872,7 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
 // ----------------------------------------------------------------------------
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_bc_r(DevMesh m, const int2 *tiles) {
-  const int2 tl = tiles[blockIdx.x];
+  const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -864,7 +913,7 @@ __global__ __launch_bounds__(TILE) void k_bc_r(DevMesh m, const int2 *tiles) {
 // layers, src/Cell/cell_sync.c:156-205.
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_sync_z(DevMesh m, const int2 *tiles) {
-  const int2 tl = tiles[blockIdx.x];
+  const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -891,7 +940,7 @@ __global__ __launch_bounds__(TILE) void k_sync_z(DevMesh m, const int2 *tiles) {
 // ----------------------------------------------------------------------------
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
-  const int2 tl = tiles[blockIdx.x];
+  const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   const DiskParams &P = disk_of(m, A.disk);
@@ -928,8 +977,9 @@ __global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
           if (r * cos(tiph) > 20.) nu = 0.000000001;
         } else {
           const DiskScalars &sc = m.sc[A.disk];
-          double d0 = d_grav_dist(sc.gm[0], r, tiph, 0.);
-          double d1 = d_grav_dist(sc.gm[1], r, tiph, 0.);
+          const double ct = cos(tiph), st = sin(tiph);
+          double d0 = d_grav_dist(sc.xbh[0], sc.ybh[0], r, ct, st, 0.);
+          double d1 = d_grav_dist(sc.xbh[1], sc.ybh[1], r, ct, st, 0.);
           double eps = P.g_eps;
           nu = P.visc * Pp / rho / sqrt(pow(d0 * d0 + eps * eps, -1.5) * sc.gm[0].M + pow(d1 * d1 + eps * eps, -1.5) * sc.gm[1].M);
         }
@@ -987,6 +1037,16 @@ __device__ void d_gm_move(const DiskParams &P, DiskScalars &s, double t, double 
   s.gm[1].omega = omega;
 }
 
+// launch-uniform gravity terms of the current mass positions (computed
+// with the same device functions the per-cell code would call)
+__device__ void d_mass_terms(const DiskParams &P, DiskScalars &s) {
+  for (int p = 0; p < 2; ++p) {
+    s.xbh[p] = s.gm[p].r * cos(s.gm[p].phi);
+    s.ybh[p] = s.gm[p].r * sin(s.gm[p].phi);
+  }
+  s.epsn = pow(P.g_eps, P.phi_order);
+}
+
 __global__ void k_scalar(DevMesh m, int ndisk, int op, double arg) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndisk) return;
@@ -1016,6 +1076,10 @@ __global__ void k_scalar(DevMesh m, int ndisk, int op, double arg) {
       }
       s.mdot_acc[0] = 0.0;
       s.mdot_acc[1] = 0.0;
+      d_mass_terms(P, s);
+      break;
+    case SC_MASS:
+      d_mass_terms(P, s);
       break;
     case SC_AFTER_UPDATE: {  // Mdot of cell_add_src, then gravMass_update_RK
       s.gm[0].Mdot = s.mdot_acc[0];
@@ -1065,15 +1129,18 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   const dim3 B(TILE);
   const dim3 SD((L.ndisk + 63) / 64), S64(64);
   KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, SC_SUB_BEGIN, 0.0));
-  if (L.n_rpairs) KLAUNCH(st, "k_faces_r", hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), B, 0, st, m, 0));
-  if (L.n_zpairs) KLAUNCH(st, "k_faces_z", hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), B, 0, st, m, L.n_rpairs));
+  if (L.n_rpairs) KLAUNCH(st, "k_faces_r", hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), dim3(FTB), 0, st, m, 0));
+  if (L.n_zpairs) KLAUNCH(st, "k_faces_z", hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), dim3(FTB), 0, st, m, L.n_rpairs));
   if (recompute_w) KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
   KLAUNCH(st, "k_plm_p", hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
+  if (L.n_rftiles)
+    KLAUNCH(st, "k_face_slope_r", hipLaunchKernelGGL(k_face_slope<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles));
   KLAUNCH(st, "k_plm_r", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0));
   KLAUNCH(st, "k_flux_phi", hipLaunchKernelGGL(k_flux_phi<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, fac));
   if (L.n_rftiles)
     KLAUNCH(st, "k_flux_r", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles, fac));
   if (L.n_zpairs) {
+    KLAUNCH(st, "k_face_slope_z", hipLaunchKernelGGL(k_face_slope<NQ>, dim3(L.n_zftiles), B, 0, st, m, L.zftiles));
     KLAUNCH(st, "k_plm_z", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 1));
     KLAUNCH(st, "k_flux_z", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_zftiles), B, 0, st, m, L.zftiles, fac));
   }
@@ -1121,20 +1188,28 @@ void launch_copy(double *dst, const double *src, long long n, hipStream_t st) {
 }
 
 void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir) {
-  if (dir == 0 && L.n_rpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), dim3(TILE), 0, st, m, 0);
-  if (dir == 1 && L.n_zpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), dim3(TILE), 0, st, m, L.n_rpairs);
+  if (dir == 0 && L.n_rpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), dim3(FTB), 0, st, m, 0);
+  if (dir == 1 && L.n_zpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), dim3(FTB), 0, st, m, L.n_rpairs);
 }
 
 void launch_plm(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int which) {
   if (NQ == 5) {
     if (which < 0)
       hipLaunchKernelGGL(k_plm_p<5>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
-    else
+    else {
+      const int2 *ft = which == 0 ? L.rftiles : L.zftiles;
+      int nft = which == 0 ? L.n_rftiles : L.n_zftiles;
+      if (nft) hipLaunchKernelGGL(k_face_slope<5>, dim3(nft), dim3(TILE), 0, st, m, ft);
       hipLaunchKernelGGL(k_plm_rz<5>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles, which);
+    }
   } else {
     if (which < 0)
       hipLaunchKernelGGL(k_plm_p<6>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
-    else
+    else {
+      const int2 *ft = which == 0 ? L.rftiles : L.zftiles;
+      int nft = which == 0 ? L.n_rftiles : L.n_zftiles;
+      if (nft) hipLaunchKernelGGL(k_face_slope<6>, dim3(nft), dim3(TILE), 0, st, m, ft);
       hipLaunchKernelGGL(k_plm_rz<6>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles, which);
+    }
   }
 }

@@ -71,12 +71,14 @@ __device__ inline double d_dtOm_a(int type, double r, double a, double tau) {
   return 0.0;
 }
 
-// src/GravMass/GravMass_access.c:25-31
-__device__ __forceinline__ double d_grav_dist(const GravState &g, double r, double phi, double z) {
-  double xbh = g.r * cos(g.phi);
-  double ybh = g.r * sin(g.phi);
-  double x = r * cos(phi);
-  double y = r * sin(phi);
+// gravMass_dist, src/GravMass/GravMass_access.c:25-31.  The mass position
+// (xbh, ybh) = (r_p cos phi_p, r_p sin phi_p) is uniform over a launch and is
+// computed once per disk and substep (DiskScalars::xbh/ybh); cphi, sphi are
+// cos and sin of the cell angle, computed once per cell.
+__device__ __forceinline__ double d_grav_dist(double xbh, double ybh, double r, double cphi, double sphi,
+                                             double z) {
+  double x = r * cphi;
+  double y = r * sphi;
   return sqrt((x - xbh) * (x - xbh) + (y - ybh) * (y - ybh) + z * z);
 }
 
@@ -169,7 +171,8 @@ __device__ __forceinline__ void d_set_flux(const double *prim, double *F, double
 template <int NQ>
 __device__ __forceinline__ int d_face_flux(const double *primL, const double *primR, int dir, double r,
                                            double dA, double w, double dt, const DiskParams &P,
-                                           const GravState *gm, const ViscIn *vi, double *X, double *Y) {
+                                           const DiskScalars &sc, const ViscIn *vi, double *X, double *Y) {
+  const GravState *gm = sc.gm;
   const double G = P.gamma;
   const int n0 = dir == 0, n1 = dir == 1, n2 = dir == 2;
   // riemann_set_vel, :30-55
@@ -271,8 +274,9 @@ __device__ __forceinline__ int d_face_flux(const double *primL, const double *pr
     double tiph = vi->cm;
     double M0 = gm[0].M, M1 = gm[1].M;
     double a = gm[0].r + gm[1].r;
-    double dist_bh0 = d_grav_dist(gm[0], r, tiph, 0.);
-    double dist_bh1 = d_grav_dist(gm[1], r, tiph, 0.);
+    const double ct = cos(tiph), st = sin(tiph);
+    double dist_bh0 = d_grav_dist(sc.xbh[0], sc.ybh[0], r, ct, st, 0.);
+    double dist_bh1 = d_grav_dist(sc.xbh[1], sc.ybh[1], r, ct, st, 0.);
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       Avg[q] = .5 * (primL[q] + primR[q]);
@@ -302,8 +306,8 @@ __device__ __forceinline__ int d_face_flux(const double *primL, const double *pr
     if (P.visc_const == 1) {
       nu = alpha;
     } else if (P.initial_data_type == 0) {  // SHEAR
-      nu = alpha * G * Avg[PPP] / Avg[RHO] * pow(fabs(r * cos(tiph)), 2.0);
-      if (r * cos(tiph) > 20.) nu = 0.0;
+      nu = alpha * G * Avg[PPP] / Avg[RHO] * pow(fabs(r * ct), 2.0);
+      if (r * ct > 20.) nu = 0.0;
     } else {
       double eps = P.g_eps;
       nu = alpha * Avg[PPP] / Avg[RHO] /
@@ -333,23 +337,21 @@ __device__ __forceinline__ int d_face_flux(const double *primL, const double *pr
 }
 
 // fgrav and gravMassForce, src/Cell/cell_source.c:11-13, 49-71
-__device__ __forceinline__ double d_fgrav(double M, double r, double eps, int n) {
-  return (M * pow(r, n - 1.) / pow(pow(r, n) + pow(eps, n), 1. + 1. / n));
+// epsn = pow(eps, n) is uniform over a launch (DiskScalars::epsn); xbh, ybh
+// are rp*cos(pp), rp*sin(pp); cosp, sinp are cos and sin of the cell angle.
+__device__ __forceinline__ double d_fgrav(double M, double r, double epsn, int n) {
+  return (M * pow(r, n - 1.) / pow(pow(r, n) + epsn, 1. + 1. / n));
 }
-__device__ __forceinline__ void d_grav_force(const GravState &g, double eps, int order, double r, double phi,
-                                             double *fr, double *fp) {
-  double rp = g.r;
-  double pp = g.phi;
-  double cosp = cos(phi);
-  double sinp = sin(phi);
-  double dx = r * cosp - rp * cos(pp);
-  double dy = r * sinp - rp * sin(pp);
+__device__ __forceinline__ void d_grav_force(double M, double xbh, double ybh, double epsn, int order, double r,
+                                             double cosp, double sinp, double *fr, double *fp) {
+  double dx = r * cosp - xbh;
+  double dy = r * sinp - ybh;
   double script_r = sqrt(dx * dx + dy * dy);
   double cosa = dx / script_r;
   double sina = dy / script_r;
   double cosap = cosa * cosp + sina * sinp;
   double sinap = sina * cosp - cosa * sinp;
-  double f1 = -d_fgrav(g.M, script_r, eps, order);
+  double f1 = -d_fgrav(M, script_r, epsn, order);
   *fr = cosap * f1;
   *fp = sinap * f1;
 }
