@@ -38,7 +38,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     return;
   }
   std::vector<double> hrf, hzf;
-  std::vector<int2> ztiles, itiles, rft, zft, bc0, bc1, szt;
+  std::vector<int2> ztiles, utiles, itiles, rft, zft, bc0, bc1, szt;
   long long zb = 0, fb = 0;
   int rb = 0;
   // r pairs of all disks first, then z pairs
@@ -100,6 +100,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
         int a = (int)hann.size();
         hann.push_back(A);
         for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) ztiles.push_back(make_int2(a, j0));
+        for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE - 1) utiles.push_back(make_int2(a, j0));
         bool interior = i >= P.NgMinR && i < NR - P.NgMaxR && k >= P.NgMinZ && k < NZ - P.NgMaxZ;
         if (interior)
           for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) itiles.push_back(make_int2(a, j0));
@@ -185,22 +186,22 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   good &= dalloc(&d_rf, hrf.size(), "rf") && h2d(d_rf, hrf.data(), hrf.size());
   good &= dalloc(&d_zf, hzf.size(), "zf") && h2d(d_zf, hzf.data(), hzf.size());
   size_t nq = (size_t)NQ * Ntot;
-  good &= dalloc(&d_prim, nq, "prim") && dalloc(&d_RKcons, nq, "RKcons");
+  good &= dalloc(&d_prim[0], nq, "prim") && dalloc(&d_prim[1], nq, "prim") && dalloc(&d_RKcons, nq, "RKcons");
   good &= dalloc(&d_grad, nq, "grad") && dalloc(&d_gradp, nq, "gradp");
   good &= dalloc(&d_tiph[0], Ntot, "tiph") && dalloc(&d_tiph[1], Ntot, "tiph");
-  good &= dalloc(&d_RKtiph, Ntot, "RKtiph") && dalloc(&d_dphi, Ntot, "dphi");
+  good &= dalloc(&d_RKtiph, Ntot, "RKtiph") && dalloc(&d_dphi[0], Ntot, "dphi") && dalloc(&d_dphi[1], Ntot, "dphi");
   good &= dalloc(&d_wiph, (size_t)nann, "wiph");
   good &= dalloc(&d_fL, (size_t)NFtot, "fL") && dalloc(&d_fR, (size_t)NFtot, "fR");
   good &= dalloc(&d_fdphi, (size_t)NFtot, "fdphi") && dalloc(&d_fcm, (size_t)NFtot, "fcm");
   good &= dalloc(&d_fo_r, Ntot, "fo_r") && dalloc(&d_fo_z, Ntot, "fo_z");
   good &= dalloc(&d_run, (size_t)(rb > 0 ? rb : 1), "run_start");
   good &= dalloc(&d_js, Ntot, "js");
-  good &= dalloc(&d_Fphi, (size_t)(NQ + 3) * Ntot, "Fphi");
   good &= dalloc(&d_Fface, (size_t)(NQ + 3) * NFtot, "Fface");
   good &= dalloc(&d_err, 1, "err");
   if (!good) return;
   hipMemset(d_err, 0, sizeof(int));
-  hipMemset(d_prim, 0, nq * sizeof(double));
+  hipMemset(d_prim[0], 0, nq * sizeof(double));
+  hipMemset(d_prim[1], 0, nq * sizeof(double));
   hipMemset(d_RKcons, 0, nq * sizeof(double));
   hipMemset(d_grad, 0, nq * sizeof(double));
   hipMemset(d_gradp, 0, nq * sizeof(double));
@@ -229,11 +230,12 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     all.insert(all.end(), v.begin(), v.end());
     return off;
   };
-  size_t o_z = add(ztiles, plan.n_ztiles), o_i = add(itiles, plan.n_itiles), o_r = add(rft, plan.n_rftiles),
+  size_t o_z = add(ztiles, plan.n_ztiles), o_u = add(utiles, plan.n_utiles), o_i = add(itiles, plan.n_itiles), o_r = add(rft, plan.n_rftiles),
          o_zf = add(zft, plan.n_zftiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
          o_s = add(szt, plan.n_sz);
   if (!dalloc(&d_tiles, all.size(), "tiles") || !h2d(d_tiles, all.data(), all.size())) return;
   plan.ztiles = d_tiles + o_z;
+  plan.utiles = d_tiles + o_u;
   plan.itiles = d_tiles + o_i;
   plan.rftiles = d_tiles + o_r;
   plan.zftiles = d_tiles + o_zf;
@@ -258,9 +260,9 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
 Engine::~Engine() {
   if (gexec) hipGraphExecDestroy(gexec);
   if (graph) hipGraphDestroy(graph);
-  void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim, d_RKcons, d_grad, d_gradp,
-                  d_tiph[0], d_tiph[1], d_RKtiph, d_dphi, d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
-                  d_fo_z, d_run, d_js, d_Fphi, d_Fface, d_err, d_tiles};
+  void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
+                  d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
+                  d_fo_z, d_run, d_js, d_Fface, d_err, d_tiles};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (ev0) hipEventDestroy(ev0);
@@ -277,14 +279,16 @@ DevMesh Engine::mesh() const {
   m.pst = d_pst;
   m.rf = d_rf;
   m.zf = d_zf;
-  m.prim = d_prim;
+  m.prim = d_prim[cur];
+  m.prim_new = d_prim[1 - cur];
   m.RKcons = d_RKcons;
   m.grad = d_grad;
   m.gradp = d_gradp;
   m.tiph = d_tiph[cur];
   m.tiph_new = d_tiph[1 - cur];
   m.RKtiph = d_RKtiph;
-  m.dphi = d_dphi;
+  m.dphi = d_dphi[cur];
+  m.dphi_new = d_dphi[1 - cur];
   m.wiph = d_wiph;
   m.fL = d_fL;
   m.fR = d_fR;
@@ -294,7 +298,6 @@ DevMesh Engine::mesh() const {
   m.fo_z = d_fo_z;
   m.run_start = d_run;
   m.js = d_js;
-  m.Fphi = d_Fphi;
   m.Fface = d_Fface;
   m.Ntot = Ntot;
   m.NFtot = NFtot;
@@ -312,13 +315,13 @@ void Engine::upload(int d, int field, const double *src) {
                 "upload");
   };
   switch (field) {
-    case 0: put(d_prim, NQ); break;
+    case 0: put(d_prim[cur], NQ); break;
     case 2: put(d_RKcons, NQ); break;
     case 3: put(d_grad, NQ); break;
     case 4: put(d_gradp, NQ); break;
     case 5: put(d_tiph[cur], 1); break;
     case 6: put(d_RKtiph, 1); break;
-    case 7: put(d_dphi, 1); break;
+    case 7: put(d_dphi[cur], 1); break;
     default: disco_set_error(9, "field cannot be uploaded");
   }
 }
@@ -332,13 +335,13 @@ void Engine::download(int d, int field, double *dst) {
                 "download");
   };
   switch (field) {
-    case 0: get(d_prim, NQ, dst); break;
+    case 0: get(d_prim[cur], NQ, dst); break;
     case 2: get(d_RKcons, NQ, dst); break;
     case 3: get(d_grad, NQ, dst); break;
     case 4: get(d_gradp, NQ, dst); break;
     case 5: get(d_tiph[cur], 1, dst); break;
     case 6: get(d_RKtiph, 1, dst); break;
-    case 7: get(d_dphi, 1, dst); break;
+    case 7: get(d_dphi[cur], 1, dst); break;
     case 8: {
       std::vector<double> w(nann);
       hip_check(hipMemcpy(w.data(), d_wiph, nann * sizeof(double), hipMemcpyDeviceToHost), "download");
@@ -349,8 +352,8 @@ void Engine::download(int d, int field, double *dst) {
     }
     case 1: {  // prim2cons with cell_calc_cons volumes (src/Cell/cell_conversion.c:35-55)
       std::vector<double> pr((size_t)NQ * nz), dp(nz);
-      get(d_prim, NQ, pr.data());
-      get(d_dphi, 1, dp.data());
+      get(d_prim[cur], NQ, pr.data());
+      get(d_dphi[cur], 1, dp.data());
       Sim *S = sims[d];
       const DiskParams &P = hdp[d];
       for (int a = P.ann_base; a < P.ann_base + P.NR * P.NZ; ++a) {

@@ -25,17 +25,17 @@ struct Engine {
   PairInfo *d_pairs = nullptr;
   PairState *d_pst = nullptr;
   double *d_rf = nullptr, *d_zf = nullptr;
-  double *d_prim = nullptr, *d_RKcons = nullptr, *d_grad = nullptr, *d_gradp = nullptr;
+  double *d_prim[2] = {nullptr, nullptr}, *d_RKcons = nullptr, *d_grad = nullptr, *d_gradp = nullptr;
   double *d_tiph[2] = {nullptr, nullptr};
-  double *d_RKtiph = nullptr, *d_dphi = nullptr, *d_wiph = nullptr;
+  double *d_RKtiph = nullptr, *d_dphi[2] = {nullptr, nullptr}, *d_wiph = nullptr;
   int *d_fL = nullptr, *d_fR = nullptr;
   double *d_fdphi = nullptr, *d_fcm = nullptr;
   int *d_fo_r = nullptr, *d_fo_z = nullptr, *d_run = nullptr;
   int4 *d_js = nullptr;
-  double *d_Fphi = nullptr, *d_Fface = nullptr;
+  double *d_Fface = nullptr;
   int *d_err = nullptr;
   int2 *d_tiles = nullptr;
-  int cur = 0;  // index of the tiph buffer holding the current edges
+  int cur = 0;  // index of the prim / tiph / dphi buffers holding the current state
 
   LaunchPlan plan{};
   hipStream_t st = nullptr;

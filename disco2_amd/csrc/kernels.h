@@ -12,16 +12,17 @@ struct DevMesh {
   PairState *pst;
   const double *rf, *zf;  // face-position pools (per-disk offsets geo_r / geo_z)
   double *prim, *RKcons, *grad, *gradp;
+  double *prim_new;       // primitives after the substep (k_update writes them)
   const double *tiph;     // cell edges at the start of the substep
   double *tiph_new;       // cell edges after the substep
   double *RKtiph, *dphi;
+  double *dphi_new;       // cell widths after the substep
   double *wiph;           // per annulus
   int *fL, *fR;
   double *fdphi, *fcm;
   int *fo_r, *fo_z;       // per zone: first face of the cell as inner cell of its r / z pair
   int *run_start;
   int4 *js;               // face-builder scratch, per zone
-  double *Fphi;           // [NQ+3][Ntot] increments of face (j, j+1)
   double *Fface;          // [NQ+3][NFtot] increments of the r/z overlap faces
   long long Ntot, NFtot;
   int *err;
@@ -31,6 +32,7 @@ struct LaunchPlan {
   int ndisk, nann;
   int n_rpairs, n_zpairs;
   const int2 *ztiles; int n_ztiles;     // every zone
+  const int2 *utiles; int n_utiles;     // every zone, DISCO_TILE - 1 cells per tile (k_update)
   const int2 *itiles; int n_itiles;     // interior zones (CFL)
   const int2 *rftiles; int n_rftiles;   // r faces
   const int2 *zftiles; int n_zftiles;   // z faces

@@ -491,28 +491,25 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
 // (riemann_setup_rz, :11-44), each solved once; riemann_AddFlux's
 // increments dt*dA*F are stored per face and gathered by k_update.
 // ----------------------------------------------------------------------------
-template <int NQ>
-__device__ __forceinline__ void load_visc_cell(const DevMesh &m, long long c, double *pr, double *g, double *gp,
-                                               double *t, double *d) {
-#pragma unroll
-  for (int q = 0; q < 5; ++q) {
-    pr[q] = m.prim[q * m.Ntot + c];
-    g[q] = m.grad[q * m.Ntot + c];
-    gp[q] = m.gradp[q * m.Ntot + c];
-  }
-  *t = clean_pi(m.tiph[c]);
-  *d = m.dphi[c];
+// the viscous flux reads cell-centre fields itself; set its pointers
+__device__ __forceinline__ void visc_cells(const DevMesh &m, ViscIn &vi, long long cL, long long cR) {
+  vi.prim = m.prim;
+  vi.grad = m.grad;
+  vi.gradp = m.gradp;
+  vi.stride = m.Ntot;
+  vi.cL = cL;
+  vi.cR = cR;
+  vi.tL = clean_pi(m.tiph[cL]);
+  vi.tR = clean_pi(m.tiph[cR]);
+  vi.dL = m.dphi[cL];
+  vi.dR = m.dphi[cR];
 }
 
+// Face (j, j+1) of an annulus, riemann_setup_p (src/Riemann/Riemann_setup.c:46-80)
+// + riemann_AddFlux: the increments X = dt*dA*F and viscous Y.
 template <int NQ>
-__global__ __launch_bounds__(TILE) void k_flux_phi(DevMesh m, const int2 *tiles, double fac) {
-  const int2 tl = tiles[xcd_order()];
-  const AnnInfo A = m.ann[tl.x];
-  const int j = tl.y + threadIdx.x;
-  if (j >= A.N) return;
-  const DiskParams &P = disk_of(m, A.disk);
-  const DiskScalars &sc = m.sc[A.disk];
-  const double dt = fac * sc.dt;
+__device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A, int a, int j, const DiskParams &P,
+                                             const DiskScalars &sc, double dt, double *X, double *Y) {
   const long long cL = A.zbase + j;
   const long long cR = A.zbase + (j == A.N - 1 ? 0 : j + 1);
   const double dpL = m.dphi[cL], dpR = m.dphi[cR];
@@ -528,21 +525,13 @@ __global__ __launch_bounds__(TILE) void k_flux_phi(DevMesh m, const int2 *tiles,
   }
   ViscIn vi;
   if (P.visc > 0.0) {
-    load_visc_cell<NQ>(m, cL, vi.pL, vi.gL, vi.gpL, &vi.tL, &vi.dL);
-    load_visc_cell<NQ>(m, cR, vi.pR, vi.gR, vi.gpR, &vi.tR, &vi.dR);
+    visc_cells(m, vi, cL, cR);
     vi.cm = vi.tL;
     vi.r_cell_L = r;
     vi.r_cell_R = r;
   }
-  double X[NQ], Y[3] = {0.0, 0.0, 0.0};
-  int e = d_face_flux<NQ>(primL, primR, 1, r, dA, m.wiph[tl.x], dt, P, sc, &vi, X, Y);
-  if (e) set_err(m.err, e);
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) m.Fphi[q * m.Ntot + cL] = X[q];
-  if (P.visc > 0.0) {
-#pragma unroll
-    for (int v = 0; v < 3; ++v) m.Fphi[(NQ + v) * m.Ntot + cL] = Y[v];
-  }
+  Y[0] = Y[1] = Y[2] = 0.0;
+  return d_face_flux<NQ>(primL, primR, 1, r, dA, m.wiph[a], dt, P, sc, &vi, X, Y);
 }
 
 template <int NQ>
@@ -576,8 +565,7 @@ __global__ __launch_bounds__(TILE) void k_flux_rz(DevMesh m, const int2 *ftiles,
   }
   ViscIn vi;
   if (P.visc > 0.0) {
-    load_visc_cell<NQ>(m, cL, vi.pL, vi.gL, vi.gpL, &vi.tL, &vi.dL);
-    load_visc_cell<NQ>(m, cR, vi.pR, vi.gR, vi.gpR, &vi.tR, &vi.dR);
+    visc_cells(m, vi, cL, cR);
     vi.cm = cm;
     if (PI.dir == 0) {
       vi.r_cell_L = r - PI.deltaL;
@@ -645,15 +633,32 @@ __device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy
 }
 
 template <int NQ>
-__global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
+__global__ __launch_bounds__(TILE, 4) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
   const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
-  const int j = tl.y + threadIdx.x;
+  const int t = threadIdx.x;
+  const int n = min(TILE - 1, A.N - tl.y);  // cells of this tile
+  const int j = tl.y + t;
   __shared__ double s_md[2][TILE / 64];
+  __shared__ double s_F[NQ + 3][TILE];      // increments of phi faces j0-1 .. j0+n-1
   double md0 = 0.0, md1 = 0.0;
   const DiskParams &P = disk_of(m, A.disk);
   const DiskScalars &sc = m.sc[A.disk];
-  if (j < A.N) {
+  // phi faces: each is solved once, by the thread of its left cell (thread 0
+  // also solves the face shared with the previous tile)
+  if (t <= n) {
+    int jf = tl.y - 1 + t;
+    if (jf < 0) jf += A.N;
+    double X[NQ], Y[3];
+    int e = phi_face_flux<NQ>(m, A, tl.x, jf, P, sc, fac * sc.dt, X, Y);
+    if (e) set_err(m.err, e);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) s_F[q][t] = X[q];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) s_F[NQ + v][t] = Y[v];
+  }
+  __syncthreads();
+  if (t < n) {
     const double dt = fac * sc.dt;
     const double RK = sc.RK;
     const bool visc = P.visc > 0.0;
@@ -681,13 +686,30 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
     // phi faces (timestep_routines.c:42-51): face j-1 has this cell as R,
     // face j as L; for j = 0 the L face comes first.
     {
-      const long long cm1 = A.zbase + (j == 0 ? N - 1 : j - 1);
+      auto phiR = [&]() {  // face (j-1, j): slot t
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) u[q] += s_F[q][t];
+        if (visc) {
+          u[SRR] += s_F[NQ][t] / r;
+          u[LLL] += s_F[NQ + 1][t];
+          u[TAU] += s_F[NQ + 2][t];
+        }
+      };
+      auto phiL = [&]() {  // face (j, j+1): slot t+1
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) u[q] += -s_F[q][t + 1];
+        if (visc) {
+          u[SRR] += -s_F[NQ][t + 1] / r;
+          u[LLL] += -s_F[NQ + 1][t + 1];
+          u[TAU] += -s_F[NQ + 2][t + 1];
+        }
+      };
       if (j == 0) {
-        add_face_L<NQ>(m, m.Fphi, m.Ntot, c, u, r, visc);
-        add_face_R<NQ>(m, m.Fphi, m.Ntot, cm1, u, r, visc);
+        phiL();
+        phiR();
       } else {
-        add_face_R<NQ>(m, m.Fphi, m.Ntot, cm1, u, r, visc);
-        add_face_L<NQ>(m, m.Fphi, m.Ntot, c, u, r, visc);
+        phiR();
+        phiL();
       }
     }
     // r faces in face order: as R of pair (i-1, i), then as L of (i, i+1)
@@ -716,7 +738,8 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
     const double a = g0.r + g1.r;
     {
       double phi = tcl - .5 * dphi;
-      const double cphi = cos(phi), sphi = sin(phi);
+      double cphi, sphi;
+      sincos(phi, &sphi, &cphi);
       double rho = pr[RHO], Pp = pr[PPP], vr = pr[URR], vz = pr[UZZ];
       double vp = pr[UPP] * r;
       double dV = dphi * .5 * (rp * rp - rm * rm) * dz;
@@ -803,7 +826,7 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
           Sigma_nu = rho * P.visc;
         else
           Sigma_nu = P.visc * Pp /
-                     sqrt(pow(dist_bh0 * dist_bh0 + eps * eps, -1.5) * g0.M + pow(dist_bh1 * dist_bh1 + eps * eps, -1.5) * g1.M);
+                     sqrt(pow_m1p5(dist_bh0 * dist_bh0 + eps * eps) * g0.M + pow_m1p5(dist_bh1 * dist_bh1 + eps * eps) * g1.M);
         u[TAU] += dt * dV * Sigma_nu * rdrOm_a *
                   (rdrOm_a + r * m.grad[UPP * m.Ntot + c] + m.gradp[URR * m.Ntot + c]);
       }
@@ -818,8 +841,10 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
         double kappa = sqrt(kappa2);
         double l0 = u[LLL];
         double rhov0 = u[SRR];
-        double l = l0 * cos(kappa * dt) - u[SRR] * dr_r2Om / kappa * sin(kappa * dt);
-        double rhov = rhov0 * cos(kappa * dt) + 2.0 * Omega * u[LLL] / r / kappa * sin(kappa * dt);
+        double skt, ckt;
+        sincos(kappa * dt, &skt, &ckt);
+        double l = l0 * ckt - u[SRR] * dr_r2Om / kappa * skt;
+        double rhov = rhov0 * ckt + 2.0 * Omega * u[LLL] / r / kappa * skt;
         u[LLL] = l;
         u[SRR] = rhov;
       }
@@ -834,12 +859,12 @@ __global__ __launch_bounds__(TILE) void k_update(DevMesh m, const int2 *tiles, d
     double dphi_new = wrap_p(tnew - tnew_m1);
     m.RKtiph[c] = rkt_own;
     m.tiph_new[c] = tnew;
-    m.dphi[c] = dphi_new;
+    m.dphi_new[c] = dphi_new;
     // cell_calc_prim, cell_conversion.c:105-122
     double out[NQ];
     d_cons2prim<NQ>(u, out, r, .5 * (rp * rp - rm * rm) * dphi_new * dz, P);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) m.prim[q * m.Ntot + c] = out[q];
+    for (int q = 0; q < NQ; ++q) m.prim_new[q * m.Ntot + c] = out[q];
   }
   if (P.sink == 1) {
     // Mdot partial sums of the block (one annulus, one disk)
@@ -977,11 +1002,12 @@ __global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
           if (r * cos(tiph) > 20.) nu = 0.000000001;
         } else {
           const DiskScalars &sc = m.sc[A.disk];
-          const double ct = cos(tiph), st = sin(tiph);
+          double ct, st;
+          sincos(tiph, &st, &ct);
           double d0 = d_grav_dist(sc.xbh[0], sc.ybh[0], r, ct, st, 0.);
           double d1 = d_grav_dist(sc.xbh[1], sc.ybh[1], r, ct, st, 0.);
           double eps = P.g_eps;
-          nu = P.visc * Pp / rho / sqrt(pow(d0 * d0 + eps * eps, -1.5) * sc.gm[0].M + pow(d1 * d1 + eps * eps, -1.5) * sc.gm[1].M);
+          nu = P.visc * Pp / rho / sqrt(pow_m1p5(d0 * d0 + eps * eps) * sc.gm[0].M + pow_m1p5(d1 * d1 + eps * eps) * sc.gm[1].M);
         }
       }
       double dt_visc = .25 * dx * dx / nu;
@@ -1136,7 +1162,6 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   if (L.n_rftiles)
     KLAUNCH(st, "k_face_slope_r", hipLaunchKernelGGL(k_face_slope<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles));
   KLAUNCH(st, "k_plm_r", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0));
-  KLAUNCH(st, "k_flux_phi", hipLaunchKernelGGL(k_flux_phi<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, fac));
   if (L.n_rftiles)
     KLAUNCH(st, "k_flux_r", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles, fac));
   if (L.n_zpairs) {
@@ -1145,13 +1170,15 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
     KLAUNCH(st, "k_flux_z", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_zftiles), B, 0, st, m, L.zftiles, fac));
   }
   KLAUNCH(st, "k_update",
-          hipLaunchKernelGGL(k_update<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, fac, copy_first));
+          hipLaunchKernelGGL(k_update<NQ>, dim3(L.n_utiles), B, 0, st, m, L.utiles, fac, copy_first));
   KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, SC_AFTER_UPDATE, 0.0));
-  DevMesh m2 = m;  // after the update the new edges live in tiph_new
+  DevMesh m2 = m;  // after the update the new state lives in the *_new buffers
   m2.tiph = m.tiph_new;
+  m2.prim = m.prim_new;
+  m2.dphi = m.dphi_new;
   if (L.n_bc0) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc0), B, 0, st, m2, L.bc0));
   if (L.n_bc1) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc1), B, 0, st, m2, L.bc1));
-  if (L.n_sz) KLAUNCH(st, "k_sync_z", hipLaunchKernelGGL(k_sync_z<NQ>, dim3(L.n_sz), B, 0, st, m, L.sz));
+  if (L.n_sz) KLAUNCH(st, "k_sync_z", hipLaunchKernelGGL(k_sync_z<NQ>, dim3(L.n_sz), B, 0, st, m2, L.sz));
 }
 
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,

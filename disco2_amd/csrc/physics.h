@@ -143,10 +143,16 @@ __device__ __forceinline__ void d_cons2prim(const double *cons, double *prim, do
 // the quantities riemann_AddFlux adds to the two cells.
 struct ViscIn {   // what riemann_visc_flux reads besides the reconstructed states
   double cm, r_cell_L, r_cell_R;
-  double gpL[5], gpR[5], gL[5], gR[5];   // gradp, grad of the two cells (first 5 q)
-  double pL[5], pR[5];                   // cell-centre primitives
-  double tL, tR, dL, dR;                 // tiph, dphi of the two cells
+  const double *prim, *grad, *gradp;     // q-major fields (read only where needed)
+  long long stride, cL, cR;
+  double tL, tR, dL, dR;                 // tiph (wrapped), dphi of the two cells
 };
+
+// x^-1.5 and x^1.5 through the correctly rounded sqrt (within two ulps of
+// pow); the reference calls pow (src/Riemann/Riemann_routines.c:262,
+// src/Cell/cell_source.c:12) and device and host pow differ by ulps anyway.
+__device__ __forceinline__ double pow_m1p5(double x) { return 1.0 / (x * sqrt(x)); }
+__device__ __forceinline__ double pow_p1p5(double x) { return x * sqrt(x); }
 
 template <int NQ>
 __device__ __forceinline__ void d_set_flux(const double *prim, double *F, double r, int n0, int n1, int n2,
@@ -270,36 +276,51 @@ __device__ __forceinline__ int d_face_flux(const double *primL, const double *pr
 #pragma unroll
   for (int q = 0; q < NQ; ++q) X[q] = dtdA * F[q];
   if (P.visc > 0.0) {  // riemann_visc_flux, :193-311
-    double Avg[5], Grr[5], Gph[5];
+    // only the entries the flux formulas use are formed: Avg for all five
+    // primitives, the r and phi gradients of v_r and omega
+    double Avg[5], Grr[2], Gph[2];
     double tiph = vi->cm;
     double M0 = gm[0].M, M1 = gm[1].M;
     double a = gm[0].r + gm[1].r;
-    const double ct = cos(tiph), st = sin(tiph);
+    double st, ct;
+    sincos(tiph, &st, &ct);
     double dist_bh0 = d_grav_dist(sc.xbh[0], sc.ybh[0], r, ct, st, 0.);
     double dist_bh1 = d_grav_dist(sc.xbh[1], sc.ybh[1], r, ct, st, 0.);
+    const double *pr = vi->prim, *gp = vi->gradp, *gr = vi->grad;
+    const long long S = vi->stride, cL = vi->cL, cR = vi->cR;
+    if (n0 == 1) {
+      double pL = vi->tL - .5 * vi->dL;
+      double pR = vi->tR - .5 * vi->dR;
+      double dpL = vi->cm - pL;
+      double dpR = -vi->cm + pR;
+      dpL = wrap_pm(dpL);
+      dpR = wrap_pm(dpR);
+      double deltaL = r - vi->r_cell_L;
+      double deltaR = vi->r_cell_R - r;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      Avg[q] = .5 * (primL[q] + primR[q]);
-      Gph[q] = .5 * (vi->gpL[q] + vi->gpR[q]) / r;
-      Grr[q] = .5 * (vi->gL[q] + vi->gR[q]);
-      if (n0 == 1) {
-        double pL = vi->tL - .5 * vi->dL;
-        double pR = vi->tR - .5 * vi->dR;
-        double dpL = vi->cm - pL;
-        double dpR = -vi->cm + pR;
-        dpL = wrap_pm(dpL);
-        dpR = wrap_pm(dpR);
-        double WL = vi->pL[q] + dpL * vi->gpL[q];
-        double WR = vi->pR[q] - dpR * vi->gpR[q];
-        double deltaL = r - vi->r_cell_L;
-        double deltaR = vi->r_cell_R - r;
+      for (int q = 0; q < 5; ++q) {
+        double WL = pr[q * S + cL] + dpL * gp[q * S + cL];
+        double WR = pr[q * S + cR] - dpR * gp[q * S + cR];
         Avg[q] = 0.5 * (WL + WR);
-        Grr[q] = (WR - WL) / (deltaR + deltaL);
+        if (q == URR || q == UPP) {
+          Grr[q - URR] = (WR - WL) / (deltaR + deltaL);
+          Gph[q - URR] = .5 * (gp[q * S + cL] + gp[q * S + cR]) / r;
+        }
       }
-      if (n1 == 1) {
-        Avg[q] = 0.5 * (vi->pR[q] + vi->pL[q]);
-        Gph[q] = (vi->pR[q] - vi->pL[q]) / (vi->tR - vi->tL) / r;
+    } else if (n1 == 1) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        double vL = pr[q * S + cL], vR = pr[q * S + cR];
+        Avg[q] = 0.5 * (vR + vL);
+        if (q == URR || q == UPP) {
+          Grr[q - URR] = .5 * (gr[q * S + cL] + gr[q * S + cR]);
+          Gph[q - URR] = (vR - vL) / (vi->tR - vi->tL) / r;
+        }
       }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) Avg[q] = .5 * (primL[q] + primR[q]);
+      Grr[0] = Grr[1] = Gph[0] = Gph[1] = 0.0;  // z faces: the flux formulas are not reached
     }
     double nu;
     double alpha = P.visc;
@@ -311,12 +332,12 @@ __device__ __forceinline__ int d_face_flux(const double *primL, const double *pr
     } else {
       double eps = P.g_eps;
       nu = alpha * Avg[PPP] / Avg[RHO] /
-           sqrt(pow(dist_bh0 * dist_bh0 + eps * eps, -1.5) * M0 + pow(dist_bh1 * dist_bh1 + eps * eps, -1.5) * M1);
+           sqrt(pow_m1p5(dist_bh0 * dist_bh0 + eps * eps) * M0 + pow_m1p5(dist_bh1 * dist_bh1 + eps * eps) * M1);
     }
     double rho = Avg[RHO], vr = Avg[URR], om = Avg[UPP], vz = Avg[UZZ];
     if (fabs(vz) > 1.e-12) return DERR_VISC_VZ;
-    double Gr_vr = Grr[URR], Gr_om = Grr[UPP];
-    double Gp_vr = Gph[URR], Gp_om = Gph[UPP];
+    double Gr_vr = Grr[0], Gr_om = Grr[1];
+    double Gp_vr = Gph[0], Gp_om = Gph[1];
     double rdr_om_cell = d_rdrOm_a(P.w_analytic, r, a);
     double v_srr = 0.0, v_lll = 0.0, v_tau = 0.0;
     if (n0 == 1) {
@@ -340,6 +361,7 @@ __device__ __forceinline__ int d_face_flux(const double *primL, const double *pr
 // epsn = pow(eps, n) is uniform over a launch (DiskScalars::epsn); xbh, ybh
 // are rp*cos(pp), rp*sin(pp); cosp, sinp are cos and sin of the cell angle.
 __device__ __forceinline__ double d_fgrav(double M, double r, double epsn, int n) {
+  if (n == 2) return (M * r / pow_p1p5(r * r + epsn));  // pow(r,1) = r, pow(r,2), pow(x,1.5)
   return (M * pow(r, n - 1.) / pow(pow(r, n) + epsn, 1. + 1. / n));
 }
 __device__ __forceinline__ void d_grav_force(double M, double xbh, double ybh, double epsn, int order, double r,
