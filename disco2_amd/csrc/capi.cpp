@@ -69,6 +69,7 @@ static void scalars_from_host(Engine *e, TimeStep *ts, GravMass *gm) {
       s.gm[p].RK_omega = gm[p].RK_omega; s.gm[p].Mdot = gm[p].Mdot; s.gm[p].Macc = gm[p].Macc;
     }
   e->put_scalars(0, s);
+  launch_scalar(e->mesh(), e->plan, e->st, SC_MASS);
 }
 static void scalars_to_host(Engine *e, TimeStep *ts, GravMass *gm) {
   DiskScalars s = e->get_scalars(0);
@@ -290,8 +291,7 @@ double cell_mindt(Cell ***c, Sim *, GravMass *gm) {  // src/Cell/cell_mindt.c:32
   mirror_push(h);
   Engine *e = h->eng;
   scalars_from_host(e, nullptr, gm);
-  launch_scalar(e->mesh(), e->plan, e->st, SC_RESET_DT);
-  launch_scalar(e->mesh(), e->plan, e->st, SC_MASS);
+  launch_scalar(e->mesh(), e->plan, e->st, SC_RESET_DT | SC_MASS);
   launch_mindt(e->mesh(), e->plan, e->st, e->NQ);
   DiskScalars s = e->get_scalars(0);
   double dt;
@@ -319,7 +319,7 @@ Face *face_create(Cell ***c, Sim *, TimeStep *ts, int direction) {
   CellsHandle *h = H(c);
   mirror_push(h);
   Engine *e = h->eng;
-  launch_faces(e->mesh(), e->plan, e->st, direction);
+  launch_faces(e->mesh(), e->plan, e->st, e->NQ, direction);
   if (e->check_error()) return nullptr;
   const DiskParams &P = e->hdp[0];
   if (direction == 1 && P.NglobalZ <= 1) return nullptr;
@@ -438,7 +438,7 @@ void timestep_substep(TimeStep *ts, Cell ***c, Sim *, GravMass *gm, MPIsetup *, 
   mirror_push(h);
   Engine *e = h->eng;
   scalars_from_host(e, ts, gm);
-  e->enqueue_substep(fac, 0, 1);
+  e->enqueue_substep(fac, 0, 1, SC_SUB_BEGIN, SC_AFTER_UPDATE);
   scalars_to_host(e, ts, gm);
   e->check_error();
   h->stale = true;

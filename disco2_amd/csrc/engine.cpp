@@ -38,7 +38,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     return;
   }
   std::vector<double> hrf, hzf;
-  std::vector<int2> ztiles, utiles, itiles, rft, zft, bc0, bc1, szt;
+  std::vector<int2> ztiles, utiles, itiles, rft, zft, rpt, zpt, bc0, bc1, szt;
   long long zb = 0, fb = 0;
   int rb = 0;
   // r pairs of all disks first, then z pairs
@@ -170,6 +170,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     fb += Q.cap;
     rb += Q.run_cap;
     for (int f0 = 0; f0 < Q.cap; f0 += DISCO_TILE) (Q.dir == 0 ? rft : zft).push_back(make_int2(p, f0));
+    for (int j0 = 0; j0 < hann[Q.ann_in].N; j0 += DISCO_TILE) (Q.dir == 0 ? rpt : zpt).push_back(make_int2(p, j0));
   }
   Ntot = zb;
   NFtot = fb;
@@ -231,7 +232,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     return off;
   };
   size_t o_z = add(ztiles, plan.n_ztiles), o_u = add(utiles, plan.n_utiles), o_i = add(itiles, plan.n_itiles), o_r = add(rft, plan.n_rftiles),
-         o_zf = add(zft, plan.n_zftiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
+         o_zf = add(zft, plan.n_zftiles), o_rp = add(rpt, plan.n_rptiles), o_zp = add(zpt, plan.n_zptiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
          o_s = add(szt, plan.n_sz);
   if (!dalloc(&d_tiles, all.size(), "tiles") || !h2d(d_tiles, all.data(), all.size())) return;
   plan.ztiles = d_tiles + o_z;
@@ -239,6 +240,8 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   plan.itiles = d_tiles + o_i;
   plan.rftiles = d_tiles + o_r;
   plan.zftiles = d_tiles + o_zf;
+  plan.rptiles = d_tiles + o_rp;
+  plan.zptiles = d_tiles + o_zp;
   plan.bc0 = d_tiles + o_b0;
   plan.bc1 = d_tiles + o_b1;
   plan.sz = d_tiles + o_s;
@@ -406,6 +409,7 @@ void Engine::init_state() {
     double big = 1.e100;
     memcpy(&s.dtmin_bits, &big, 8);
     put_scalars(d, s);
+    launch_scalar(mesh(), plan, st, SC_MASS);
     const DiskParams &P = hdp[d];
     if (P.NglobalZ > 1) {  // src/Cell/cell_sync.c:156-205 on one rank
       long long nz = nzones[d], NL = P.NL;
@@ -433,33 +437,27 @@ void Engine::init_state() {
   hip_check(hipDeviceSynchronize(), "init");
 }
 
-void Engine::enqueue_substep(double fac, int copy_first, int recompute_w) {
-  launch_substep(mesh(), plan, st, NQ, fac, copy_first, recompute_w);
+void Engine::enqueue_substep(double fac, int copy_first, int recompute_w, int pre_ops, int post_ops) {
+  launch_substep(mesh(), plan, st, NQ, fac, copy_first, recompute_w, pre_ops, post_ops);
   cur = 1 - cur;
 }
 
-// timestep_rk2, src/TimeStep/timestep_algorithm.c:11-41
+// timestep_rk2, src/TimeStep/timestep_algorithm.c:11-41.  The per-disk
+// scalar updates between substeps ride in one launch after each update.
 void Engine::enqueue_rk2() {
   DevMesh m = mesh();
-  launch_scalar(m, plan, st, SC_MASS);
   launch_set_w(m, plan, st);
   launch_mindt(m, plan, st, NQ);
-  launch_scalar(m, plan, st, SC_STEP_BEGIN);
-  enqueue_substep(1.0, 1, 0);
-  launch_scalar(mesh(), plan, st, SC_SUB1_END);
-  enqueue_substep(0.5, 0, 1);
-  launch_scalar(mesh(), plan, st, SC_SUB2_END);
+  enqueue_substep(1.0, 1, 0, SC_STEP_BEGIN | SC_SUB_BEGIN, SC_AFTER_UPDATE | SC_SUB1_END | SC_SUB_BEGIN);
+  enqueue_substep(0.5, 0, 1, 0, SC_AFTER_UPDATE | SC_SUB2_END);
 }
 
 // timestep_forward_euler, src/TimeStep/timestep_algorithm.c:43-57
 void Engine::enqueue_euler() {
   DevMesh m = mesh();
-  launch_scalar(m, plan, st, SC_MASS);
   launch_set_w(m, plan, st);
   launch_mindt(m, plan, st, NQ);
-  launch_scalar(m, plan, st, SC_STEP_BEGIN);
-  enqueue_substep(1.0, 1, 0);
-  launch_scalar(mesh(), plan, st, SC_EULER_END);
+  enqueue_substep(1.0, 1, 0, SC_STEP_BEGIN | SC_SUB_BEGIN, SC_AFTER_UPDATE | SC_EULER_END);
 }
 
 void Engine::run(int nsteps, bool use_graph) {

@@ -56,7 +56,7 @@ struct Engine {
   void init_state();                 // gravMass init, z sync of prim/tiph, scalars
   void enqueue_rk2();
   void enqueue_euler();
-  void enqueue_substep(double fac, int copy_first, int recompute_w);
+  void enqueue_substep(double fac, int copy_first, int recompute_w, int pre_ops, int post_ops);
   void run(int nsteps, bool use_graph);
   void synchronize();
   int check_error();                 // 0, or sets the sticky library error

@@ -34,6 +34,8 @@ struct LaunchPlan {
   const int2 *ztiles; int n_ztiles;     // every zone
   const int2 *utiles; int n_utiles;     // every zone, DISCO_TILE - 1 cells per tile (k_update)
   const int2 *itiles; int n_itiles;     // interior zones (CFL)
+  const int2 *rptiles; int n_rptiles;   // inner cells of r pairs (pair, j0)
+  const int2 *zptiles; int n_zptiles;   // inner cells of z pairs
   const int2 *rftiles; int n_rftiles;   // r faces
   const int2 *zftiles; int n_zftiles;   // z faces
   const int2 *bc0; int n_bc0;           // ghost annuli 0 and NR-2
@@ -65,13 +67,24 @@ extern KProf *g_kprof;
     }                                                                     \
   } while (0)
 
-enum { SC_STEP_BEGIN, SC_SUB_BEGIN, SC_AFTER_UPDATE, SC_SUB1_END, SC_SUB2_END, SC_EULER_END, SC_RESET_DT, SC_MASS };
+// per-disk scalar operations (bit flags; one launch runs the set in the
+// order listed)
+enum {
+  SC_STEP_BEGIN = 1,    // dt from the CFL minimum, gravMass_copy, RK = 0
+  SC_AFTER_UPDATE = 2,  // Mdot, gravMass_update_RK
+  SC_SUB1_END = 4,      // gravMass_move(t, dt), RK = 0.5
+  SC_SUB2_END = 8,      // gravMass_move(t, dt/2), t += dt, Macc
+  SC_EULER_END = 16,    // gravMass_move(t, dt), t += dt
+  SC_SUB_BEGIN = 32,    // gravMass_clean_pi, Mdot accumulators
+  SC_RESET_DT = 64,
+  SC_MASS = 128,        // launch-uniform gravity terms
+};
 
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
-                    int recompute_w);
+                    int recompute_w, int pre_ops, int post_ops);
 void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op);
 void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st);
 void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ);
-void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir);
+void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int dir);
 void launch_plm(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int which);
 void launch_copy(double *dst, const double *src, long long n, hipStream_t st);

@@ -156,97 +156,129 @@ __device__ int block_excl_scan(int v, int *s_w, int *total) {
   return base + x - v;
 }
 
-__global__ __launch_bounds__(FTB) void k_faces(DevMesh m, int pair0) {
-  const int tid = threadIdx.x;
-  const int p = pair0 + xcd_order();
+// S = (W_R - W_L)/(deltaL + deltaR) of one face, cell_plm.c:36-49 and 86-99
+template <int NQ>
+__device__ __forceinline__ void face_slope(const DevMesh &m, const PairInfo &P, long long g, double *S) {
+  const int cL = m.fL[g], cR = m.fR[g];
+  const double cm = m.fcm[g];
+  double pL = clean_pi(m.tiph[cL]) - .5 * m.dphi[cL];
+  double pR = clean_pi(m.tiph[cR]) - .5 * m.dphi[cR];
+  double dpL = wrap_pm(cm - pL);
+  double dpR = wrap_pm(pR - cm);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double WL = m.prim[q * m.Ntot + cL] + dpL * m.gradp[q * m.Ntot + cL];
+    double WR = m.prim[q * m.Ntot + cR] - dpR * m.gradp[q * m.Ntot + cR];
+    S[q] = (WR - WL) / (P.deltaR + P.deltaL);
+  }
+}
+
+// Stage 1 (pair tiles: up to DISCO_TILE inner cells of one pair per
+// workgroup): every inner cell guesses its start cursor and counts its faces.
+// The workgroup holding cell 0 also finds jpmin (face_create_destroy.c:82-96).
+__global__ __launch_bounds__(TILE) void k_faces_count(DevMesh m, const int2 *ptiles) {
+  const int2 tl = ptiles[xcd_order()];
+  const int p = tl.x, tid = threadIdx.x;
   const PairInfo P = m.pairs[p];
   const AnnInfo A = m.ann[P.ann_in];
   const AnnInfo B = m.ann[P.ann_out];
   const int Nin = A.N, Nout = B.N;
   const double *Tin = m.tiph + A.zbase, *Din = m.dphi + A.zbase;
   const double *Tout = m.tiph + B.zbase, *Dout = m.dphi + B.zbase;
-  __shared__ double s_d[FTB];
-  __shared__ int s_j[FTB];
-  __shared__ int s_w[FTB / 64];
-
-  // jpmin: first outer cell minimising wrap+(tiph_out - p0), :82-96
-  const double p0 = Tin[Nin - 1];
-  double bd = TWO_PI;
-  int bj = -1;
-  for (int jp = tid; jp < Nout; jp += FTB) {
-    double dp = wrap_p(Tout[jp] - p0);
-    if (bd > dp) {
-      bd = dp;
-      bj = jp;
-    }
-  }
-  s_d[tid] = bd;
-  s_j[tid] = bj;
-  __syncthreads();
-  for (int s = FTB / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-      int j2 = s_j[tid + s];
-      if (j2 >= 0) {
-        int j1 = s_j[tid];
-        double d1 = s_d[tid], d2 = s_d[tid + s];
-        if (j1 < 0 || d2 < d1 || (d2 == d1 && j2 < j1)) {
-          s_d[tid] = d2;
-          s_j[tid] = j2;
-        }
+  int jpmin = 0;
+  if (tl.y == 0) {
+    __shared__ double s_d[TILE];
+    __shared__ int s_j[TILE];
+    const double p0 = Tin[Nin - 1];
+    double bd = TWO_PI;
+    int bj = -1;
+    for (int jp = tid; jp < Nout; jp += TILE) {
+      double dp = wrap_p(Tout[jp] - p0);
+      if (bd > dp) {
+        bd = dp;
+        bj = jp;
       }
     }
+    s_d[tid] = bd;
+    s_j[tid] = bj;
     __syncthreads();
+    for (int s = TILE / 2; s > 0; s >>= 1) {
+      if (tid < s) {
+        int j2 = s_j[tid + s];
+        if (j2 >= 0) {
+          int j1 = s_j[tid];
+          double d1 = s_d[tid], d2 = s_d[tid + s];
+          if (j1 < 0 || d2 < d1 || (d2 == d1 && j2 < j1)) {
+            s_d[tid] = d2;
+            s_j[tid] = j2;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    jpmin = s_j[0] < 0 ? 0 : s_j[0];
+    if (tid == 0) m.pst[p].jpmin = jpmin;
   }
-  const int jpmin = s_j[0] < 0 ? 0 : s_j[0];
-  __syncthreads();
-
+  const int j = tl.y + tid;
+  if (j >= Nin) return;
   FaceOut o;
-  o.fL = m.fL;
-  o.fR = m.fR;
-  o.fdp = m.fdphi;
-  o.fcm = m.fcm;
-  o.run_start = m.run_start;
-  o.fbase = P.face_base;
-  o.rbase = P.run_base;
-  o.cap = P.cap;
-  o.run_cap = P.run_cap;
-  o.zin = A.zbase;
-  o.zout = B.zbase;
   o.err = m.err;
-  int *fo = (P.dir == 0 ? m.fo_r : m.fo_z) + A.zbase;
-  int4 *js = m.js + A.zbase;
+  int bad = 0;
+  int jps = (j == 0) ? jpmin : guess_outer(Tin[j - 1], Tout, Nout);
+  int jpe;
+  int cnt = jloop_cell<false>(j, jps, Tin, Din, Tout, Dout, Nin, Nout, 0, o, &jpe, &bad);
+  m.js[A.zbase + j] = make_int4(jps, cnt, jpe, bad);
+}
 
-  // guess, count and verify
+// Stage 2 (one workgroup per pair): verify every guess against the previous
+// cell's end cursor, prefix-sum the counts into first-face offsets, or replay
+// the reference's sequential walk when a guess is off (ties between edges).
+__global__ __launch_bounds__(FTB) void k_faces_scan(DevMesh m, int pair0) {
+  __shared__ int s_w[FTB / 64];
+  const int tid = threadIdx.x;
+  const int p = pair0 + xcd_order();
+  const PairInfo P = m.pairs[p];
+  const AnnInfo A = m.ann[P.ann_in];
+  const AnnInfo B = m.ann[P.ann_out];
+  const int Nin = A.N, Nout = B.N;
+  const int4 *js = m.js + A.zbase;
+  int *fo = (P.dir == 0 ? m.fo_r : m.fo_z) + A.zbase;
+  const int jpmin = m.pst[p].jpmin;
   int bad = 0;
   for (int j = tid; j < Nin; j += FTB) {
-    int jps = (j == 0) ? jpmin : guess_outer(Tin[j - 1], Tout, Nout);
-    int jpe;
-    int cnt = jloop_cell<false>(j, jps, Tin, Din, Tout, Dout, Nin, Nout, 0, o, &jpe, &bad);
-    js[j] = make_int4(jps, cnt, jpe, 0);
+    int4 v = js[j];
+    if (v.w) bad = 1;
+    if (j + 1 < Nin && v.z != js[j + 1].x) bad = 1;
   }
-  __syncthreads();
-  for (int j = tid; j + 1 < Nin; j += FTB)
-    if (js[j].z != js[j + 1].x) bad = 1;
   bad = __syncthreads_or(bad);
-
   int total = 0;
   if (!bad) {
     int carry = 0;
     for (int base = 0; base < Nin; base += FTB) {
       int j = base + tid;
-      int4 v = (j < Nin) ? js[j] : make_int4(0, 0, 0, 0);
+      int c = (j < Nin) ? js[j].y : 0;
       int chunk;
-      int foff = carry + block_excl_scan(v.y, s_w, &chunk);
-      if (j < Nin) {
-        fo[j] = foff;
-        int jpe, b2 = 0;
-        jloop_cell<true>(j, v.x, Tin, Din, Tout, Dout, Nin, Nout, foff, o, &jpe, &b2);
-      }
+      int foff = carry + block_excl_scan(c, s_w, &chunk);
+      if (j < Nin) fo[j] = foff;
       carry += chunk;
     }
     total = carry;
   } else if (tid == 0) {
-    // sequential walk of the reference
+    const double *Tin = m.tiph + A.zbase, *Din = m.dphi + A.zbase;
+    const double *Tout = m.tiph + B.zbase, *Dout = m.dphi + B.zbase;
+    FaceOut o;
+    o.fL = m.fL;
+    o.fR = m.fR;
+    o.fdp = m.fdphi;
+    o.fcm = m.fcm;
+    o.run_start = m.run_start;
+    o.fbase = P.face_base;
+    o.rbase = P.run_base;
+    o.cap = P.cap;
+    o.run_cap = P.run_cap;
+    o.zin = A.zbase;
+    o.zout = B.zbase;
+    o.err = m.err;
     int jp = jpmin, foff = 0, b2 = 0;
     for (int j = 0; j < Nin; ++j) {
       fo[j] = foff;
@@ -273,6 +305,43 @@ __global__ __launch_bounds__(FTB) void k_faces(DevMesh m, int pair0) {
     st.seq = bad;
     m.pst[p] = st;
   }
+}
+
+// Stage 3 (pair tiles): every inner cell writes its faces (and the runs they
+// start) at its offset.
+__global__ __launch_bounds__(TILE) void k_faces_emit(DevMesh m, const int2 *ptiles) {
+  const int2 tl = ptiles[xcd_order()];
+  const int p = tl.x;
+  const PairState ps = m.pst[p];
+  const PairInfo P = m.pairs[p];
+  const AnnInfo A = m.ann[P.ann_in];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const int *fo = (P.dir == 0 ? m.fo_r : m.fo_z) + A.zbase;
+  const int foff = fo[j];
+  int cnt;
+  if (!ps.seq) {
+    const AnnInfo B = m.ann[P.ann_out];
+    FaceOut o;
+    o.fL = m.fL;
+    o.fR = m.fR;
+    o.fdp = m.fdphi;
+    o.fcm = m.fcm;
+    o.run_start = m.run_start;
+    o.fbase = P.face_base;
+    o.rbase = P.run_base;
+    o.cap = P.cap;
+    o.run_cap = P.run_cap;
+    o.zin = A.zbase;
+    o.zout = B.zbase;
+    o.err = m.err;
+    int jpe, b2 = 0;
+    cnt = jloop_cell<true>(j, m.js[A.zbase + j].x, m.tiph + A.zbase, m.dphi + A.zbase, m.tiph + B.zbase,
+                           m.dphi + B.zbase, A.N, B.N, foff, o, &jpe, &b2);
+  } else {
+    cnt = ((j + 1 < A.N) ? fo[j + 1] : ps.count) - foff;
+  }
+  (void)cnt;
 }
 
 // Faces of a pair that have inner cell j (local index) as L, and the runs
@@ -386,22 +455,6 @@ __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
 // gather: the cell's faces as R (previous pair) then as L (next pair), in
 // face order, for the sum and again for the limiter.
 // ----------------------------------------------------------------------------
-template <int NQ>
-__device__ __forceinline__ void face_slope(const DevMesh &m, const PairInfo &P, long long g, double *S) {
-  const int cL = m.fL[g], cR = m.fR[g];
-  const double cm = m.fcm[g];
-  double pL = clean_pi(m.tiph[cL]) - .5 * m.dphi[cL];
-  double pR = clean_pi(m.tiph[cR]) - .5 * m.dphi[cR];
-  double dpL = wrap_pm(cm - pL);
-  double dpR = wrap_pm(pR - cm);
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    double WL = m.prim[q * m.Ntot + cL] + dpL * m.gradp[q * m.Ntot + cL];
-    double WR = m.prim[q * m.Ntot + cR] - dpR * m.gradp[q * m.Ntot + cR];
-    S[q] = (WR - WL) / (P.deltaR + P.deltaL);
-  }
-}
-
 // The face difference S = (W_R - W_L)/(deltaL + deltaR) of every overlap
 // face (cell_plm.c:36-49 and 86-99), computed once per face and stored in the
 // first NQ rows of the face-increment buffer (the fluxes overwrite them later).
@@ -1073,75 +1126,69 @@ __device__ void d_mass_terms(const DiskParams &P, DiskScalars &s) {
   s.epsn = pow(P.g_eps, P.phi_order);
 }
 
-__global__ void k_scalar(DevMesh m, int ndisk, int op, double arg) {
+__global__ void k_scalar(DevMesh m, int ndisk, int ops, double arg) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndisk) return;
   const DiskParams &P = m.dp[d];
   DiskScalars &s = m.sc[d];
-  switch (op) {
-    case SC_STEP_BEGIN: {  // timestep_set_dt, gravMass_copy, RK = 0
-      double dt = __longlong_as_double((long long)s.dtmin_bits);
-      if (s.t + dt > P.t_max) dt = P.t_max - s.t;
-      s.dt = dt;
-      for (int p = 0; p < 2; ++p) {
-        s.gm[p].RK_r = s.gm[p].r;
-        s.gm[p].RK_phi = s.gm[p].phi;
-        s.gm[p].RK_M = s.gm[p].M;
-        s.gm[p].RK_omega = s.gm[p].omega;
-      }
-      s.RK = 0.0;
-      break;
+  if (ops & SC_STEP_BEGIN) {  // timestep_set_dt, gravMass_copy, RK = 0
+    double dt = __longlong_as_double((long long)s.dtmin_bits);
+    if (s.t + dt > P.t_max) dt = P.t_max - s.t;
+    s.dt = dt;
+    for (int p = 0; p < 2; ++p) {
+      s.gm[p].RK_r = s.gm[p].r;
+      s.gm[p].RK_phi = s.gm[p].phi;
+      s.gm[p].RK_M = s.gm[p].M;
+      s.gm[p].RK_omega = s.gm[p].omega;
     }
-    case SC_SUB_BEGIN:  // gravMass_clean_pi
-      for (int p = 0; p < 2; ++p) {
-        double phi = s.gm[p].phi;
-        int n = 0;
-        while (phi > 2. * M_PI && n++ < MAXWRAP) phi -= 2. * M_PI;
-        while (phi < 0.0 && n++ < MAXWRAP) phi += 2. * M_PI;
-        s.gm[p].phi = phi;
-      }
-      s.mdot_acc[0] = 0.0;
-      s.mdot_acc[1] = 0.0;
-      d_mass_terms(P, s);
-      break;
-    case SC_MASS:
-      d_mass_terms(P, s);
-      break;
-    case SC_AFTER_UPDATE: {  // Mdot of cell_add_src, then gravMass_update_RK
-      s.gm[0].Mdot = s.mdot_acc[0];
-      s.gm[1].Mdot = s.mdot_acc[1];
-      double RK = s.RK;
-      for (int p = 0; p < 2; ++p) {
-        GravState &g = s.gm[p];
-        int n = 0;
-        while (g.phi - g.RK_phi > M_PI && n++ < MAXWRAP) g.RK_phi += 2. * M_PI;
-        while (g.phi - g.RK_phi < -M_PI && n++ < MAXWRAP) g.RK_phi -= 2. * M_PI;
-        g.r = (1.0 - RK) * g.r + RK * g.RK_r;
-        g.phi = (1.0 - RK) * g.phi + RK * g.RK_phi;
-        g.M = (1.0 - RK) * g.M + RK * g.RK_M;
-        g.omega = (1.0 - RK) * g.omega + RK * g.RK_omega;
-      }
-      break;
-    }
-    case SC_SUB1_END:  // gravMass_move(t, 1.0*dt); RK = 0.5
-      d_gm_move(P, s, s.t, 1.0 * s.dt);
-      s.RK = 0.5;
-      break;
-    case SC_SUB2_END:  // gravMass_move(t, 0.5*dt); t += dt; Macc += Mdot*dt
-      d_gm_move(P, s, s.t, 0.5 * s.dt);
-      s.t += s.dt;
-      for (int p = 0; p < 2; ++p) s.gm[p].Macc = s.gm[p].Macc + s.gm[p].Mdot * s.dt;
-      s.dtmin_bits = (unsigned long long)__double_as_longlong(1.e100);
-      break;
-    case SC_EULER_END:  // timestep_forward_euler: gravMass_move(t, dt); t += dt
-      d_gm_move(P, s, s.t, 1.0 * s.dt);
-      s.t += s.dt;
-      s.dtmin_bits = (unsigned long long)__double_as_longlong(1.e100);
-      break;
-    case SC_RESET_DT:
-      s.dtmin_bits = (unsigned long long)__double_as_longlong(1.e100);
-      break;
+    s.RK = 0.0;
   }
+  if (ops & SC_AFTER_UPDATE) {  // Mdot of cell_add_src, then gravMass_update_RK
+    s.gm[0].Mdot = s.mdot_acc[0];
+    s.gm[1].Mdot = s.mdot_acc[1];
+    double RK = s.RK;
+    for (int p = 0; p < 2; ++p) {
+      GravState &g = s.gm[p];
+      int n = 0;
+      while (g.phi - g.RK_phi > M_PI && n++ < MAXWRAP) g.RK_phi += 2. * M_PI;
+      while (g.phi - g.RK_phi < -M_PI && n++ < MAXWRAP) g.RK_phi -= 2. * M_PI;
+      g.r = (1.0 - RK) * g.r + RK * g.RK_r;
+      g.phi = (1.0 - RK) * g.phi + RK * g.RK_phi;
+      g.M = (1.0 - RK) * g.M + RK * g.RK_M;
+      g.omega = (1.0 - RK) * g.omega + RK * g.RK_omega;
+    }
+  }
+  if (ops & SC_SUB1_END) {  // gravMass_move(t, 1.0*dt); RK = 0.5
+    d_gm_move(P, s, s.t, 1.0 * s.dt);
+    s.RK = 0.5;
+  }
+  if (ops & SC_SUB2_END) {  // gravMass_move(t, 0.5*dt); t += dt; Macc += Mdot*dt
+    d_gm_move(P, s, s.t, 0.5 * s.dt);
+    s.t += s.dt;
+    for (int p = 0; p < 2; ++p) s.gm[p].Macc = s.gm[p].Macc + s.gm[p].Mdot * s.dt;
+    s.dtmin_bits = (unsigned long long)__double_as_longlong(1.e100);
+    d_mass_terms(P, s);
+  }
+  if (ops & SC_EULER_END) {  // timestep_forward_euler: gravMass_move(t, dt); t += dt
+    d_gm_move(P, s, s.t, 1.0 * s.dt);
+    s.t += s.dt;
+    s.dtmin_bits = (unsigned long long)__double_as_longlong(1.e100);
+    d_mass_terms(P, s);
+  }
+  if (ops & SC_SUB_BEGIN) {  // gravMass_clean_pi
+    for (int p = 0; p < 2; ++p) {
+      double phi = s.gm[p].phi;
+      int n = 0;
+      while (phi > 2. * M_PI && n++ < MAXWRAP) phi -= 2. * M_PI;
+      while (phi < 0.0 && n++ < MAXWRAP) phi += 2. * M_PI;
+      s.gm[p].phi = phi;
+    }
+    s.mdot_acc[0] = 0.0;
+    s.mdot_acc[1] = 0.0;
+    d_mass_terms(P, s);
+  }
+  if (ops & SC_RESET_DT) s.dtmin_bits = (unsigned long long)__double_as_longlong(1.e100);
+  if (ops & SC_MASS) d_mass_terms(P, s);
 }
 
 // ----------------------------------------------------------------------------
@@ -1149,29 +1196,47 @@ __global__ void k_scalar(DevMesh m, int ndisk, int op, double arg) {
 // ----------------------------------------------------------------------------
 KProf *g_kprof = nullptr;
 
+// overlap faces of the r (dir 0) or z (dir 1) pairs, with their slopes S
+template <int NQ>
+static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir) {
+  const int2 *pt = dir == 0 ? L.rptiles : L.zptiles;
+  const int npt = dir == 0 ? L.n_rptiles : L.n_zptiles;
+  const int np = dir == 0 ? L.n_rpairs : L.n_zpairs;
+  const int p0 = dir == 0 ? 0 : L.n_rpairs;
+  if (!np) return;
+  KLAUNCH(st, dir == 0 ? "k_faces_count_r" : "k_faces_count_z",
+          hipLaunchKernelGGL(k_faces_count, dim3(npt), dim3(TILE), 0, st, m, pt));
+  KLAUNCH(st, dir == 0 ? "k_faces_scan_r" : "k_faces_scan_z",
+          hipLaunchKernelGGL(k_faces_scan, dim3(np), dim3(FTB), 0, st, m, p0));
+  KLAUNCH(st, dir == 0 ? "k_faces_emit_r" : "k_faces_emit_z",
+          hipLaunchKernelGGL(k_faces_emit, dim3(npt), dim3(TILE), 0, st, m, pt));
+  const int2 *ft = dir == 0 ? L.rftiles : L.zftiles;
+  const int nft = dir == 0 ? L.n_rftiles : L.n_zftiles;
+  if (nft)
+    KLAUNCH(st, dir == 0 ? "k_face_slope_r" : "k_face_slope_z",
+            hipLaunchKernelGGL(k_face_slope<NQ>, dim3(nft), dim3(TILE), 0, st, m, ft));
+}
+
 template <int NQ>
 static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, double fac, int copy_first,
-                             int recompute_w) {
+                             int recompute_w, int pre_ops, int post_ops) {
   const dim3 B(TILE);
   const dim3 SD((L.ndisk + 63) / 64), S64(64);
-  KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, SC_SUB_BEGIN, 0.0));
-  if (L.n_rpairs) KLAUNCH(st, "k_faces_r", hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), dim3(FTB), 0, st, m, 0));
-  if (L.n_zpairs) KLAUNCH(st, "k_faces_z", hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), dim3(FTB), 0, st, m, L.n_rpairs));
-  if (recompute_w) KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
+  if (pre_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, pre_ops, 0.0));
   KLAUNCH(st, "k_plm_p", hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
-  if (L.n_rftiles)
-    KLAUNCH(st, "k_face_slope_r", hipLaunchKernelGGL(k_face_slope<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles));
+  launch_faces_q<NQ>(m, L, st, 0);
+  launch_faces_q<NQ>(m, L, st, 1);
+  if (recompute_w) KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
   KLAUNCH(st, "k_plm_r", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0));
   if (L.n_rftiles)
     KLAUNCH(st, "k_flux_r", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles, fac));
   if (L.n_zpairs) {
-    KLAUNCH(st, "k_face_slope_z", hipLaunchKernelGGL(k_face_slope<NQ>, dim3(L.n_zftiles), B, 0, st, m, L.zftiles));
     KLAUNCH(st, "k_plm_z", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 1));
     KLAUNCH(st, "k_flux_z", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_zftiles), B, 0, st, m, L.zftiles, fac));
   }
   KLAUNCH(st, "k_update",
           hipLaunchKernelGGL(k_update<NQ>, dim3(L.n_utiles), B, 0, st, m, L.utiles, fac, copy_first));
-  KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, SC_AFTER_UPDATE, 0.0));
+  if (post_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, post_ops, 0.0));
   DevMesh m2 = m;  // after the update the new state lives in the *_new buffers
   m2.tiph = m.tiph_new;
   m2.prim = m.prim_new;
@@ -1182,11 +1247,11 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
 }
 
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
-                    int recompute_w) {
+                    int recompute_w, int pre_ops, int post_ops) {
   if (NQ == 5)
-    launch_substep_q<5>(m, L, st, fac, copy_first, recompute_w);
+    launch_substep_q<5>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
   else
-    launch_substep_q<6>(m, L, st, fac, copy_first, recompute_w);
+    launch_substep_q<6>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
 }
 
 void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op) {
@@ -1214,9 +1279,11 @@ void launch_copy(double *dst, const double *src, long long n, hipStream_t st) {
   hipLaunchKernelGGL(k_copy, dim3(2048), dim3(TILE), 0, st, dst, src, n);
 }
 
-void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir) {
-  if (dir == 0 && L.n_rpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_rpairs), dim3(FTB), 0, st, m, 0);
-  if (dir == 1 && L.n_zpairs) hipLaunchKernelGGL(k_faces, dim3(L.n_zpairs), dim3(FTB), 0, st, m, L.n_rpairs);
+void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int dir) {
+  if (NQ == 5)
+    launch_faces_q<5>(m, L, st, dir);
+  else
+    launch_faces_q<6>(m, L, st, dir);
 }
 
 void launch_plm(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int which) {
