@@ -30,28 +30,45 @@ HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, published
 B_ALG_PER_ZONE_UPDATE = 320  # 8 * (6*NUM_Q + 10) bytes at NUM_Q = 5
 
 
+def r_face_count(n_p, nz_layers):
+    """Overlap faces between neighbouring annuli: N_in + N_out per pair (plus
+    coincident edges, which have measure zero for the seeded offsets)."""
+    return int(nz_layers * (2 * int(n_p.sum()) - int(n_p[0]) - int(n_p[-1])))
+
+
 def kernel_alg_bytes(name, nz, nfaces_r, nq, visc):
-    """Algorithmic (compulsory) bytes of one launch of a kernel, counting every
-    array the kernel must read or write once.  nz: zones incl. ghosts."""
+    """Algorithmic (compulsory) bytes of one launch of a kernel: every array
+    the kernel must touch, read or written once, whatever the re-reads of
+    its gathers.  nz: zones incl. ghosts; nfaces_r: r overlap faces."""
     d = 8
-    fx = nq + (3 if visc else 0)  # flux increments stored per face
+    fx = nq + (3 if visc else 0)  # flux increments stored per r face
     if name == "k_update":
-        # read prim, tiph, RKtiph, dphi, RKcons(avg of the two substeps: written
-        # in the first, read in the second), phi-face and r-face increments,
-        # face lists; write prim, tiph, RKtiph, dphi
-        rd = nz * d * (nq + 3 + nq / 2 + fx) + nfaces_r * (d * fx + 8)
-        wr = nz * d * (nq + 3)
-        return rd + wr
+        # per zone read: prim, gradp (phi-face states), tiph, RKtiph, dphi,
+        # grad[vr, omega] (viscous flux and source), face-list cursors (2 ints),
+        # RKcons (read in the second substep only: half a vector per launch);
+        # per zone written: prim, tiph, RKtiph, dphi, RKcons (first substep:
+        # half a vector per launch); per r face: the stored increments
+        rd = nz * (d * (2 * nq + 3 + (2 if visc else 0) + nq / 2) + 8) + nfaces_r * d * fx
+        wr = nz * d * (nq + 3 + nq / 2)
+        return int(rd + wr)
     if name == "k_flux_r":
-        return nfaces_r * (24 + d * fx) + nz * d * (3 * nq + 2)
-    if name == "k_flux_phi":
-        return nz * d * (2 * nq + 2 + fx)
+        # per face: cell indices, centre phi, overlap length in; increments out;
+        # per zone: prim, grad, gradp, tiph, dphi
+        return nfaces_r * (24 + 8 + d * fx) + nz * d * (3 * nq + 2)
     if name == "k_plm_r":
-        return nfaces_r * 24 + nz * d * (2 * nq + 2) + nz * d * nq
+        # per face: slope S (NQ) and overlap length; per zone: dphi, cursors in, grad out
+        return nfaces_r * d * (nq + 1) + nz * (d + 8) + nz * d * nq
+    if name == "k_face_slope_r":
+        # per face: cell indices, centre phi in, S out; per zone: prim, gradp, tiph, dphi
+        return nfaces_r * (24 + d * nq) + nz * d * (2 * nq + 2)
     if name == "k_plm_p":
         return nz * d * (nq + 1) + nz * d * nq
-    if name == "k_faces_r":
-        return nz * d * 2 + nfaces_r * 24 + nz * 4
+    if name == "k_mindt":
+        return nz * d * (nq + 2)
+    if name == "k_faces_count_r":
+        return nz * (2 * d + 16)          # tiph, dphi in; per inner cell (start, count, end, flag)
+    if name == "k_faces_emit_r":
+        return nz * (2 * d + 16 + 4 + 4) + nfaces_r * 24
     return None
 
 
@@ -178,7 +195,7 @@ def main(argv=None):
 
     # per-kernel HIP-event durations on the batch's stream
     prof = batch.profile(args.profile_steps)
-    nfr = 2 * setup.nzones  # about N_in + N_out faces per annulus pair
+    nfr = r_face_count(setup.N_p, setup.NZ)
     nq = setup.NQ
     visc = params["Explicit_Viscosity"] > 0
     dom = max(prof.items(), key=lambda kv: kv[1][0])

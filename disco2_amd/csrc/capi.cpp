@@ -293,6 +293,7 @@ double cell_mindt(Cell ***c, Sim *, GravMass *gm) {  // src/Cell/cell_mindt.c:32
   scalars_from_host(e, nullptr, gm);
   launch_scalar(e->mesh(), e->plan, e->st, SC_RESET_DT | SC_MASS);
   launch_mindt(e->mesh(), e->plan, e->st, e->NQ);
+  launch_scalar(e->mesh(), e->plan, e->st, SC_DT_FOLD);
   DiskScalars s = e->get_scalars(0);
   double dt;
   memcpy(&dt, &s.dtmin_bits, 8);

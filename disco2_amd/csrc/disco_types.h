@@ -36,6 +36,7 @@ struct DiskParams {
   int geo_r, geo_z;           // offsets into the r_faces / z_faces pools
   long long zone_base;
   int NL;                     // zones per layer
+  int itile0, itile1;         // this disk's range in the interior (CFL) tile list
   int w_numeric, w_analytic, riemann, visc_const, grav2d, sink, phi_order, cooling;
   int initial_data_type, grav_type, bound_r, no_inner_bc;
   double gamma, cfl, plm, visc, g_eps, rho_floor, cs_floor, cs_cap, vel_cap;

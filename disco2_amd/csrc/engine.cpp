@@ -82,6 +82,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     P.NL = (int)NL;
     P.zone_base = zb;
     P.ann_base = (int)hann.size();
+    P.itile0 = (int)itiles.size();
     zbase.push_back(zb);
     nzones.push_back(NL * NZ);
     for (int k = 0; k < NZ; ++k)
@@ -113,6 +114,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
         if (P.NglobalZ > 1 && (k < P.NgMinZ || k >= NZ - P.NgMaxZ))
           for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) szt.push_back(make_int2(a, j0));
       }
+    P.itile1 = (int)itiles.size();
     zb += NL * NZ;
     // r pairs (k, i)-(k, i+1): constants of build_jloop, src/Face/face_create_destroy.c:60-68
     P.rpair_base = (int)hpairs.size();
@@ -199,6 +201,8 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   good &= dalloc(&d_js, Ntot, "js");
   good &= dalloc(&d_Fface, (size_t)(NQ + 3) * NFtot, "Fface");
   good &= dalloc(&d_err, 1, "err");
+  good &= dalloc(&d_dt_tile, itiles.empty() ? 1 : itiles.size(), "dt tiles");
+  good &= dalloc(&d_ann_rot, (size_t)nann, "annulus rotation");
   if (!good) return;
   hipMemset(d_err, 0, sizeof(int));
   hipMemset(d_prim[0], 0, nq * sizeof(double));
@@ -207,6 +211,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   hipMemset(d_grad, 0, nq * sizeof(double));
   hipMemset(d_gradp, 0, nq * sizeof(double));
   hipMemset(d_wiph, 0, (size_t)nann * sizeof(double));
+  hipMemset(d_ann_rot, 0, (size_t)nann * sizeof(double4));
   {  // no faces yet: every gather over a pair is empty until k_faces runs
     std::vector<PairState> p0(hpairs.size());
     for (auto &x : p0) x = PairState{0, 0, -1, 0};
@@ -222,6 +227,10 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     memcpy(&s.dtmin_bits, &big, 8);
   }
   h2d(d_sc, sc0.data(), (size_t)n);
+  {
+    std::vector<double> none(itiles.empty() ? 1 : itiles.size(), 1.e100);
+    h2d(d_dt_tile, none.data(), none.size());
+  }
 
   // one tile array on the device, sliced per list
   std::vector<int2> all;
@@ -265,7 +274,7 @@ Engine::~Engine() {
   if (graph) hipGraphDestroy(graph);
   void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
-                  d_fo_z, d_run, d_js, d_Fface, d_err, d_tiles};
+                  d_fo_z, d_run, d_js, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (ev0) hipEventDestroy(ev0);
@@ -305,6 +314,8 @@ DevMesh Engine::mesh() const {
   m.Ntot = Ntot;
   m.NFtot = NFtot;
   m.err = d_err;
+  m.dt_tile = d_dt_tile;
+  m.ann_rot = d_ann_rot;
   return m;
 }
 

@@ -34,6 +34,8 @@ struct Engine {
   int4 *d_js = nullptr;
   double *d_Fface = nullptr;
   int *d_err = nullptr;
+  double *d_dt_tile = nullptr;
+  double4 *d_ann_rot = nullptr;
   int2 *d_tiles = nullptr;
   int cur = 0;  // index of the prim / tiph / dphi buffers holding the current state
 

@@ -24,9 +24,15 @@ struct DevMesh {
   int *run_start;
   int4 *js;               // face-builder scratch, per zone
   double *Fface;          // [NQ+3][NFtot] increments of the r/z overlap faces
+  double *dt_tile;        // CFL minimum of each interior tile (k_mindt), reduced per disk by k_scalar
+  double4 *ann_rot;       // per annulus: rOm_a, rdrOm_a, dtOm_a at the centre, rdrOm_a at the outer face
   long long Ntot, NFtot;
   int *err;
 };
+
+// k_scalar runs one workgroup of SCALAR_THREADS per disk: all of them reduce
+// the disk's tile minima (SC_DT_FOLD), thread 0 does the scalar updates.
+#define SCALAR_THREADS 256
 
 struct LaunchPlan {
   int ndisk, nann;
@@ -78,6 +84,7 @@ enum {
   SC_SUB_BEGIN = 32,    // gravMass_clean_pi, Mdot accumulators
   SC_RESET_DT = 64,
   SC_MASS = 128,        // launch-uniform gravity terms
+  SC_DT_FOLD = 256,     // CFL minimum from the tile minima (runs first; STEP_BEGIN implies it)
 };
 
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,

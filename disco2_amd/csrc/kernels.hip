@@ -544,6 +544,13 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
 // (riemann_setup_rz, :11-44), each solved once; riemann_AddFlux's
 // increments dt*dA*F are stored per face and gathered by k_update.
 // ----------------------------------------------------------------------------
+// analytic frame rotation of an annulus (filled by k_scalar whenever the
+// masses move; zero for the non-rotating frame)
+__device__ __forceinline__ double4 ann_rotation(const DevMesh &m, const DiskParams &P, int a) {
+  if (P.w_analytic == A_FIXED) return make_double4(0.0, 0.0, 0.0, 0.0);
+  return m.ann_rot[a];
+}
+
 // the viscous flux reads cell-centre fields itself; set its pointers
 __device__ __forceinline__ void visc_cells(const DevMesh &m, ViscIn &vi, long long cL, long long cR) {
   vi.prim = m.prim;
@@ -562,7 +569,7 @@ __device__ __forceinline__ void visc_cells(const DevMesh &m, ViscIn &vi, long lo
 // + riemann_AddFlux: the increments X = dt*dA*F and viscous Y.
 template <int NQ>
 __device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A, int a, int j, const DiskParams &P,
-                                             const DiskScalars &sc, double dt, double *X, double *Y) {
+                                             const DiskScalars &sc, double dt, double rdr_om, double *X, double *Y) {
   const long long cL = A.zbase + j;
   const long long cR = A.zbase + (j == A.N - 1 ? 0 : j + 1);
   const double dpL = m.dphi[cL], dpR = m.dphi[cR];
@@ -582,6 +589,7 @@ __device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A,
     vi.cm = vi.tL;
     vi.r_cell_L = r;
     vi.r_cell_R = r;
+    vi.rdr_om = rdr_om;
   }
   Y[0] = Y[1] = Y[2] = 0.0;
   return d_face_flux<NQ>(primL, primR, 1, r, dA, m.wiph[a], dt, P, sc, &vi, X, Y);
@@ -620,12 +628,15 @@ __global__ __launch_bounds__(TILE) void k_flux_rz(DevMesh m, const int2 *ftiles,
   if (P.visc > 0.0) {
     visc_cells(m, vi, cL, cR);
     vi.cm = cm;
+    const double4 rot = ann_rotation(m, P, PI.ann_in);
     if (PI.dir == 0) {
       vi.r_cell_L = r - PI.deltaL;
       vi.r_cell_R = r + PI.deltaR;
+      vi.rdr_om = rot.w;  // at the face between the annuli
     } else {
       vi.r_cell_L = r;
       vi.r_cell_R = r;
+      vi.rdr_om = rot.y;  // z faces sit at the annulus centre
     }
   }
   double X[NQ], Y[3] = {0.0, 0.0, 0.0};
@@ -675,10 +686,18 @@ __device__ __forceinline__ void add_face_L(const DevMesh &m, const double *F, lo
 __device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy_first, double RK, double wtot,
                                            double dt, double r, double *rkt_out) {
   double tb = clean_pi(traw);
-  double rkt = copy_first ? traw : rkt_in;
-  int n = 0;
-  while (tb - rkt > HALF_TURN && n++ < MAXWRAP) rkt += TWO_PI;
-  while (tb - rkt < -HALF_TURN && n++ < MAXWRAP) rkt -= TWO_PI;
+  const double rk0 = copy_first ? traw : rkt_in;
+  // the unwrap loops of cell_misc.c:65-66, one trip by selects and the loops
+  // from the start only when that trip does not end them (as wrap_range)
+  double rkt = tb - rk0 > HALF_TURN ? rk0 + TWO_PI : (tb - rk0 < -HALF_TURN ? rk0 - TWO_PI : rk0);
+  if (tb - rkt > HALF_TURN || tb - rkt < -HALF_TURN) {
+    rkt = rk0;
+    int n = 0;
+#pragma unroll 1
+    while (tb - rkt > HALF_TURN && n++ < MAXWRAP) rkt += TWO_PI;
+#pragma unroll 1
+    while (tb - rkt < -HALF_TURN && n++ < MAXWRAP) rkt -= TWO_PI;
+  }
   *rkt_out = rkt;
   double t = (1.0 - RK) * tb + RK * rkt;
   t += wtot * dt / r;
@@ -697,13 +716,14 @@ __global__ __launch_bounds__(TILE, 4) void k_update(DevMesh m, const int2 *tiles
   double md0 = 0.0, md1 = 0.0;
   const DiskParams &P = disk_of(m, A.disk);
   const DiskScalars &sc = m.sc[A.disk];
+  const double4 rot = ann_rotation(m, P, tl.x);
   // phi faces: each is solved once, by the thread of its left cell (thread 0
   // also solves the face shared with the previous tile)
   if (t <= n) {
     int jf = tl.y - 1 + t;
     if (jf < 0) jf += A.N;
     double X[NQ], Y[3];
-    int e = phi_face_flux<NQ>(m, A, tl.x, jf, P, sc, fac * sc.dt, X, Y);
+    int e = phi_face_flux<NQ>(m, A, tl.x, jf, P, sc, fac * sc.dt, rot.y, X, Y);
     if (e) set_err(m.err, e);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) s_F[q][t] = X[q];
@@ -788,7 +808,6 @@ __global__ __launch_bounds__(TILE, 4) void k_update(DevMesh m, const int2 *tiles
     const double traw = m.tiph[c];
     const double tcl = clean_pi(traw);
     const GravState g0 = sc.gm[0], g1 = sc.gm[1];
-    const double a = g0.r + g1.r;
     {
       double phi = tcl - .5 * dphi;
       double cphi, sphi;
@@ -815,10 +834,10 @@ __global__ __launch_bounds__(TILE, 4) void k_update(DevMesh m, const int2 *tiles
       d_grav_force(g1.M, sc.xbh[1], sc.ybh[1], sc.epsn, P.phi_order, gravdist, cphi, sphi, &fr, &fp);
       Fr += fr;
       Fp += fp;
-      double w_a = d_rOm_a(P.w_analytic, r, a);
-      double rdrOm_a = d_rdrOm_a(P.w_analytic, r, a);
+      double w_a = rot.x;
+      double rdrOm_a = rot.y;
       double F_centrifugal_r = w_a * w_a / r;
-      double dtOm_a = d_dtOm_a(P.w_analytic, r, a, P.orb_tscale);
+      double dtOm_a = rot.z;
       double F_euler_phi = -(r * dtOm_a + vr * rdrOm_a);
       u[SRR] += dt * dV * (rho * vp * vp + Pp) / r;
       u[SRR] += dt * dV * rho * (Fr * sint + F_centrifugal_r);
@@ -886,9 +905,9 @@ __global__ __launch_bounds__(TILE, 4) void k_update(DevMesh m, const int2 *tiles
     }
     // cell_add_split_fictitious, cell_add_split_fictitious.c:11-44
     {
-      double rOm = d_rOm_a(P.w_analytic, r, a);
+      double rOm = rot.x;
       double Omega = rOm / r;
-      double dr_r2Om = 2. * rOm + r * d_rdrOm_a(P.w_analytic, r, a);
+      double dr_r2Om = 2. * rOm + r * rot.y;
       double kappa2 = 2. * Omega / r * dr_r2Om;
       if (kappa2 > 0.) {
         double kappa = sqrt(kappa2);
@@ -904,7 +923,7 @@ __global__ __launch_bounds__(TILE, 4) void k_update(DevMesh m, const int2 *tiles
     }
     // cell_update_phi + cell_update_dphi (the left neighbour's new edge is
     // recomputed here from its old state)
-    const double wtot = m.wiph[tl.x] + d_rOm_a(P.w_analytic, r, a);
+    const double wtot = m.wiph[tl.x] + rot.x;
     double rkt_own, rkt_nb;
     double tnew = new_tiph(traw, m.RKtiph[c], copy_first, RK, wtot, dt, r, &rkt_own);
     const long long cm1 = A.zbase + (j == 0 ? N - 1 : j - 1);
@@ -1014,12 +1033,12 @@ __global__ __launch_bounds__(TILE) void k_sync_z(DevMesh m, const int2 *tiles) {
 
 // ----------------------------------------------------------------------------
 // CFL time step over interior cells, cell_mindt, src/Cell/cell_mindt.c:10-100.
-// The minimum of positive doubles is taken on their bit patterns.
+// Each workgroup stores the minimum of its tile with a plain store (no
+// device-wide atomics: thousands of them on one word serialise); k_scalar's
+// SC_DT_FOLD reduces a disk's tiles.  min is exact, so the order is free.
 // ----------------------------------------------------------------------------
 template <int NQ>
-__global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
-  const int2 tl = tiles[xcd_order()];
-  const AnnInfo A = m.ann[tl.x];
+__device__ __forceinline__ double cell_dt(const DevMesh &m, const int2 tl, const AnnInfo &A) {
   const int j = tl.y + threadIdx.x;
   const DiskParams &P = disk_of(m, A.disk);
   double dt = 1.e100;
@@ -1068,18 +1087,53 @@ __global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
     }
     if (d < dt) dt = d;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    double o = __shfl_down(dt, off);
-    if (o < dt) dt = o;
-  }
+  return dt;
+}
+
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
   __shared__ double s_m[TILE / 64];
-  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = dt;
+  const int it = xcd_order();
+  const int2 tl = tiles[it];
+  const AnnInfo A = m.ann[tl.x];
+  double v = cell_dt<NQ>(m, tl, A);
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(v, off);
+    if (o < v) v = o;
+  }
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double b = s_m[0];
     for (int w = 1; w < TILE / 64; ++w)
       if (s_m[w] < b) b = s_m[w];
-    if (b > 0.0 && b < 1.e100) atomicMin(&m.sc[A.disk].dtmin_bits, (unsigned long long)__double_as_longlong(b));
+    m.dt_tile[it] = b;
+  }
+}
+
+// SC_DT_FOLD: dtmin of disk d = min(dtmin, the positive tile minima below
+// 1e100) over the disk's interior tiles; every thread of the workgroup calls it.
+__device__ __forceinline__ void fold_dt(const DevMesh &m, const DiskParams &P, DiskScalars &s) {
+  __shared__ double s_m[SCALAR_THREADS / 64];
+  double v = 1.e100;
+  for (int it = P.itile0 + (int)threadIdx.x; it < P.itile1; it += SCALAR_THREADS) {
+    const double b = m.dt_tile[it];
+    if (b > 0.0 && b < v) v = b;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(v, off);
+    if (o < v) v = o;
+  }
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = s_m[0];
+    for (int w = 1; w < SCALAR_THREADS / 64; ++w)
+      if (s_m[w] < b) b = s_m[w];
+    if (b < 1.e100) {
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(b);
+      if (bits < s.dtmin_bits) s.dtmin_bits = bits;
+    }
   }
 }
 
@@ -1126,11 +1180,7 @@ __device__ void d_mass_terms(const DiskParams &P, DiskScalars &s) {
   s.epsn = pow(P.g_eps, P.phi_order);
 }
 
-__global__ void k_scalar(DevMesh m, int ndisk, int ops, double arg) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= ndisk) return;
-  const DiskParams &P = m.dp[d];
-  DiskScalars &s = m.sc[d];
+__device__ void scalar_ops(const DiskParams &P, DiskScalars &s, int ops) {
   if (ops & SC_STEP_BEGIN) {  // timestep_set_dt, gravMass_copy, RK = 0
     double dt = __longlong_as_double((long long)s.dtmin_bits);
     if (s.t + dt > P.t_max) dt = P.t_max - s.t;
@@ -1191,6 +1241,28 @@ __global__ void k_scalar(DevMesh m, int ndisk, int ops, double arg) {
   if (ops & SC_MASS) d_mass_terms(P, s);
 }
 
+__global__ __launch_bounds__(SCALAR_THREADS) void k_scalar(DevMesh m, int ndisk, int ops, double arg) {
+  const int d = blockIdx.x;  // one workgroup per disk
+  const DiskParams &P = m.dp[d];
+  DiskScalars &s = m.sc[d];
+  if (ops & (SC_DT_FOLD | SC_STEP_BEGIN)) fold_dt(m, P, s);
+  if (threadIdx.x == 0) scalar_ops(P, s, ops);
+  // the analytic frame rotation (src/Sim/sim_w_a.c) of every annulus, at the
+  // mass separation a the next kernels will see: after any mass move
+  if ((ops & (SC_SUB2_END | SC_EULER_END | SC_SUB_BEGIN | SC_MASS)) && P.w_analytic != A_FIXED) {
+    __syncthreads();
+    const double a = s.gm[0].r + s.gm[1].r;
+    const int na = P.NR * P.NZ;
+    for (int k = threadIdx.x; k < na; k += SCALAR_THREADS) {
+      const AnnInfo &A = m.ann[P.ann_base + k];
+      const double r = .5 * (A.rp + A.rm);
+      m.ann_rot[P.ann_base + k] =
+          make_double4(d_rOm_a(P.w_analytic, r, a), d_rdrOm_a(P.w_analytic, r, a),
+                       d_dtOm_a(P.w_analytic, r, a, P.orb_tscale), d_rdrOm_a(P.w_analytic, A.rp, a));
+    }
+  }
+}
+
 // ----------------------------------------------------------------------------
 // host-side launchers
 // ----------------------------------------------------------------------------
@@ -1221,7 +1293,7 @@ template <int NQ>
 static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, double fac, int copy_first,
                              int recompute_w, int pre_ops, int post_ops) {
   const dim3 B(TILE);
-  const dim3 SD((L.ndisk + 63) / 64), S64(64);
+  const dim3 SD(L.ndisk), S64(SCALAR_THREADS);
   if (pre_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, pre_ops, 0.0));
   KLAUNCH(st, "k_plm_p", hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
   launch_faces_q<NQ>(m, L, st, 0);
@@ -1255,7 +1327,7 @@ void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int N
 }
 
 void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op) {
-  KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, dim3((L.ndisk + 63) / 64), dim3(64), 0, st, m, L.ndisk, op, 0.0));
+  KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, dim3(L.ndisk), dim3(SCALAR_THREADS), 0, st, m, L.ndisk, op, 0.0));
 }
 
 void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st) {
@@ -1263,6 +1335,7 @@ void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st) {
 }
 
 void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ) {
+  if (!L.n_itiles) return;
   if (NQ == 5)
     KLAUNCH(st, "k_mindt", hipLaunchKernelGGL(k_mindt<5>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles));
   else

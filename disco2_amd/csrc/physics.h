@@ -13,29 +13,37 @@
 #define HALF_TURN (TWO_PI / 2.)
 #define MAXWRAP 64            // angle wraps beyond this mean a diverged state
 
-// while-loop angle wraps of the reference, with a bounded trip count
+// while-loop angle wraps of the reference, with a bounded trip count.  The
+// loops almost never run more than once, so one trip is taken with selects;
+// when its result is in range it is what the loops return (one trip of the
+// first loop, or of the second, ending both), and otherwise the loops run
+// from the start: a rarely taken branch instead of two divergent loops in
+// every call, with the same result.
+__device__ __forceinline__ double wrap_range(double x, double lo, double hi) {
+  double y = x > hi ? x - TWO_PI : (x < lo ? x + TWO_PI : x);
+  if (y > hi || y < lo) {
+    y = x;
+    int n = 0;
+#pragma unroll 1
+    while (y > hi && n++ < MAXWRAP) y -= TWO_PI;
+#pragma unroll 1
+    while (y < lo && n++ < MAXWRAP) y += TWO_PI;
+  }
+  return y;
+}
 __device__ __forceinline__ double clean_pi(double phi) {  // src/Cell/cell_misc.c:19-20
-  int n = 0;
-  while (phi > TWO_PI && n++ < MAXWRAP) phi -= TWO_PI;
-  while (phi < 0.0 && n++ < MAXWRAP) phi += TWO_PI;
-  return phi;
+  return wrap_range(phi, 0.0, TWO_PI);
 }
 __device__ __forceinline__ double wrap_pm(double d) {  // e.g. src/Cell/cell_plm.c:39-40
-  int n = 0;
-  while (d > HALF_TURN && n++ < MAXWRAP) d -= TWO_PI;
-  while (d < -HALF_TURN && n++ < MAXWRAP) d += TWO_PI;
-  return d;
+  return wrap_range(d, -HALF_TURN, HALF_TURN);
 }
 __device__ __forceinline__ double wrap_p(double d) {  // e.g. src/Face/face_create_destroy.c:88-89
-  int n = 0;
-  while (d > TWO_PI && n++ < MAXWRAP) d -= TWO_PI;
-  while (d < 0.0 && n++ < MAXWRAP) d += TWO_PI;
-  return d;
+  return wrap_range(d, 0.0, TWO_PI);
 }
 
 // analytic frame rotation, src/Sim/sim_w_a.c:8-73
 #define W_A_MILOS_INDEX 8
-__device__ inline double d_rOm_a(int type, double r, double a) {
+__device__ __noinline__ double d_rOm_a_any(int type, double r, double a) {
   switch (type) {
     case A_KEPLER: return pow(r, -0.5);
     case A_OMEGA20: return r * 20;
@@ -46,7 +54,7 @@ __device__ inline double d_rOm_a(int type, double r, double a) {
     default: return 0.0;
   }
 }
-__device__ inline double d_rdrOm_a(int type, double r, double a) {
+__device__ __noinline__ double d_rdrOm_a_any(int type, double r, double a) {
   switch (type) {
     case A_KEPLER: return -1.5 * pow(r, -1.5);
     case A_MILOS: {
@@ -59,7 +67,14 @@ __device__ inline double d_rdrOm_a(int type, double r, double a) {
     default: return 0.0;
   }
 }
-__device__ inline double d_dtOm_a(int type, double r, double a, double tau) {
+// the common non-rotating frame inline, the rest out of line
+__device__ __forceinline__ double d_rOm_a(int type, double r, double a) {
+  return type == A_FIXED ? 0.0 : d_rOm_a_any(type, r, a);
+}
+__device__ __forceinline__ double d_rdrOm_a(int type, double r, double a) {
+  return type == A_FIXED ? 0.0 : d_rdrOm_a_any(type, r, a);
+}
+__device__ __forceinline__ double d_dtOm_a(int type, double r, double a, double tau) {
   if (type == A_MILOS) {
     double a0 = 1.0;
     double dta_o_a = -.25 * pow(a / a0, -4) / tau;
@@ -146,6 +161,7 @@ struct ViscIn {   // what riemann_visc_flux reads besides the reconstructed stat
   const double *prim, *grad, *gradp;     // q-major fields (read only where needed)
   long long stride, cL, cR;
   double tL, tR, dL, dR;                 // tiph (wrapped), dphi of the two cells
+  double rdr_om;                         // sim_rdrOm_a at the face radius
 };
 
 // x^-1.5 and x^1.5 through the correctly rounded sqrt (within two ulps of
@@ -281,7 +297,6 @@ __device__ __forceinline__ int d_face_flux(const double *primL, const double *pr
     double Avg[5], Grr[2], Gph[2];
     double tiph = vi->cm;
     double M0 = gm[0].M, M1 = gm[1].M;
-    double a = gm[0].r + gm[1].r;
     double st, ct;
     sincos(tiph, &st, &ct);
     double dist_bh0 = d_grav_dist(sc.xbh[0], sc.ybh[0], r, ct, st, 0.);
@@ -338,7 +353,7 @@ __device__ __forceinline__ int d_face_flux(const double *primL, const double *pr
     if (fabs(vz) > 1.e-12) return DERR_VISC_VZ;
     double Gr_vr = Grr[0], Gr_om = Grr[1];
     double Gp_vr = Gph[0], Gp_om = Gph[1];
-    double rdr_om_cell = d_rdrOm_a(P.w_analytic, r, a);
+    double rdr_om_cell = vi->rdr_om;
     double v_srr = 0.0, v_lll = 0.0, v_tau = 0.0;
     if (n0 == 1) {
       v_srr = -nu * rho * (r * Gr_vr - vr - r * r * Gp_om);

@@ -314,7 +314,16 @@ void sim_set_N_p_array(Sim *s, const int *N_p) {
 void sim_set_misc(Sim *s, MPIsetup *) {
   int NR = sim_N(s, 0), NZ = sim_N(s, 1);
   int Ncells = 0;
-  for (int i = 0; i < NR; ++i) Ncells += s->N_p[i] * NZ;
+  for (int i = 0; i < NR; ++i) {
+    if (s->N_p[i] < 2) {
+      // the reference goes on with a negative count and crashes later (an
+      // inner ghost face at r < 0 when R_Min is below the radial spacing)
+      disco_set_error(8, "annulus " + std::to_string(i) + " has N_p = " + std::to_string(s->N_p[i]) +
+                             ": every annulus needs N_p >= 2 (check that R_Min exceeds the radial spacing)");
+      return;
+    }
+    Ncells += s->N_p[i] * NZ;
+  }
   s->Ncells = Ncells;
   s->Ncells_global = Ncells;
   s->offset = 0;
@@ -414,7 +423,15 @@ void disco_cell_offsets(const Sim *s, std::vector<double> &tiph, std::vector<dou
   int NR = (int)s->N_p.size();
   int NZ = (int)s->z_faces.size() - 1;
   long long NL = 0;
-  for (int i = 0; i < NR; ++i) NL += s->N_p[i];
+  for (int i = 0; i < NR; ++i) {
+    if (s->N_p[i] < 1) {
+      disco_set_error(8, "cell offsets: annulus with N_p < 1");
+      tiph.clear();
+      dphi.clear();
+      return;
+    }
+    NL += s->N_p[i];
+  }
   tiph.assign((size_t)NL * NZ, 0.0);
   dphi.assign((size_t)NL * NZ, 0.0);
   GlibcRand g(666);

@@ -2,7 +2,7 @@
  * disco2_amd -- C ABI of the MI355X hydro substep library.
  *
  * The entry points keep the reference's opaque-struct "class" API
- * (reference README.md "Notes"; src/Headers/*.h): every struct is an
+ * (reference README.md "Notes"; the headers in src/Headers): every struct is an
  * incomplete type, every object is made by X_create and freed by
  * X_destroy, and main()-style drivers call the same functions in the same
  * order as src/main.c:56-156.  Behind the API the cell state lives in GPU

@@ -15,9 +15,13 @@ def max_rel(a, b, floor=1e-300):
     return float(np.max(d / np.maximum(np.abs(b), floor))) if d.size else 0.0
 
 
-def rel_per_zone(a, b, scale_axis=None):
-    """max over zones of |a-b| / max(|b|, tiny), per row if 2-D."""
-    return max_rel(a, b)
+def zone_rel(a, b):
+    """max over zones and rows of |a-b| / |b_zone|, where |b_zone| is the norm
+    of the zone's vector (the column of a (NQ, Nzones) array)."""
+    a = np.asarray(a, dtype=np.float64).reshape(-1, np.shape(a)[-1])
+    b = np.asarray(b, dtype=np.float64).reshape(-1, np.shape(b)[-1])
+    norm = np.sqrt(np.sum(b * b, axis=0))
+    return float(np.max(np.abs(a - b) / np.maximum(norm, 1e-300)))
 
 
 def describe(a, b):

@@ -11,7 +11,7 @@ ABI and must agree:
 import numpy as np
 import pytest
 
-from compare import bitwise_equal, describe, max_rel
+from compare import bitwise_equal, describe, max_rel, zone_rel
 from golden_util import Golden, golden_names
 
 pytestmark = pytest.mark.gpu
@@ -40,14 +40,6 @@ def run_case(name, graph):
         out[st]["scalars"] = b.scalars()
     b.close()
     return g, out
-
-
-def zone_rel(a, b):
-    """max over zones of |a-b| / |b| for each row, rows scaled by the zone norm."""
-    a = np.asarray(a).reshape(-1, a.shape[-1])
-    b = np.asarray(b).reshape(-1, b.shape[-1])
-    norm = np.sqrt(np.sum(b * b, axis=0))
-    return float(np.max(np.abs(a - b) / np.maximum(norm, 1e-300)))
 
 
 @pytest.mark.parametrize("name", golden_names())

@@ -130,3 +130,18 @@ def test_gravmass_host_orbit():
     assert abs(L.gravMass_phi(gm, 0) - 1.0) < 1e-14
     assert abs(L.gravMass_phi(gm, 1) - (np.pi + 1.0)) < 1e-14
     L.gravMass_destroy(gm)
+
+
+def test_negative_ghost_radius_is_rejected():
+    """NumR 32 on r in [0.3, 10]: the inner ghost face would sit at r < 0 and
+    the reference's N_p formula gives -3 for that annulus; set-up must fail
+    with an error instead of sizing arrays with it."""
+    from disco2_amd import configs
+    from disco2_amd._lib import DiscoError
+    from disco2_amd.api import Setup
+    p, _, _ = configs.binary(32)
+    with pytest.raises(DiscoError, match="N_p"):
+        Setup(p)
+    p, _, _ = configs.binary(48)  # dr = 0.2 < R_Min: fine
+    s = Setup(p)
+    assert s.N_p.min() >= 2
