@@ -65,10 +65,10 @@ def kernel_alg_bytes(name, nz, nfaces_r, nq, visc):
         return nz * d * (nq + 1) + nz * d * nq
     if name == "k_mindt":
         return nz * d * (nq + 2)
-    if name == "k_faces_count_r":
-        return nz * (2 * d + 16)          # tiph, dphi in; per inner cell (start, count, end, flag)
-    if name == "k_faces_emit_r":
-        return nz * (2 * d + 16 + 4 + 4) + nfaces_r * 24
+    if name == "k_faces_r":
+        # tiph, dphi in; per inner cell its first-face offset, per outer cell
+        # its run start out; per face cell indices, width and centre out
+        return nz * (2 * d + 4 + 4) + nfaces_r * 24
     return None
 
 

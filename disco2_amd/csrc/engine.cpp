@@ -38,7 +38,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     return;
   }
   std::vector<double> hrf, hzf;
-  std::vector<int2> ztiles, utiles, itiles, rft, zft, rpt, zpt, bc0, bc1, szt;
+  std::vector<int2> ztiles, utiles, itiles, rft, zft, bc0, bc1, szt;
   long long zb = 0, fb = 0;
   int rb = 0;
   // r pairs of all disks first, then z pairs
@@ -172,7 +172,6 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     fb += Q.cap;
     rb += Q.run_cap;
     for (int f0 = 0; f0 < Q.cap; f0 += DISCO_TILE) (Q.dir == 0 ? rft : zft).push_back(make_int2(p, f0));
-    for (int j0 = 0; j0 < hann[Q.ann_in].N; j0 += DISCO_TILE) (Q.dir == 0 ? rpt : zpt).push_back(make_int2(p, j0));
   }
   Ntot = zb;
   NFtot = fb;
@@ -198,7 +197,6 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   good &= dalloc(&d_fdphi, (size_t)NFtot, "fdphi") && dalloc(&d_fcm, (size_t)NFtot, "fcm");
   good &= dalloc(&d_fo_r, Ntot, "fo_r") && dalloc(&d_fo_z, Ntot, "fo_z");
   good &= dalloc(&d_run, (size_t)(rb > 0 ? rb : 1), "run_start");
-  good &= dalloc(&d_js, Ntot, "js");
   good &= dalloc(&d_Fface, (size_t)(NQ + 3) * NFtot, "Fface");
   good &= dalloc(&d_err, 1, "err");
   good &= dalloc(&d_dt_tile, itiles.empty() ? 1 : itiles.size(), "dt tiles");
@@ -241,7 +239,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     return off;
   };
   size_t o_z = add(ztiles, plan.n_ztiles), o_u = add(utiles, plan.n_utiles), o_i = add(itiles, plan.n_itiles), o_r = add(rft, plan.n_rftiles),
-         o_zf = add(zft, plan.n_zftiles), o_rp = add(rpt, plan.n_rptiles), o_zp = add(zpt, plan.n_zptiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
+         o_zf = add(zft, plan.n_zftiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
          o_s = add(szt, plan.n_sz);
   if (!dalloc(&d_tiles, all.size(), "tiles") || !h2d(d_tiles, all.data(), all.size())) return;
   plan.ztiles = d_tiles + o_z;
@@ -249,13 +247,25 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   plan.itiles = d_tiles + o_i;
   plan.rftiles = d_tiles + o_r;
   plan.zftiles = d_tiles + o_zf;
-  plan.rptiles = d_tiles + o_rp;
-  plan.zptiles = d_tiles + o_zp;
   plan.bc0 = d_tiles + o_b0;
   plan.bc1 = d_tiles + o_b1;
   plan.sz = d_tiles + o_s;
   plan.ndisk = n;
   plan.nann = nann;
+  {  // largest outer annulus of each pair direction, staged in LDS if it fits
+    int mr = 0, mz = 0;
+    for (auto &Q : hpairs) {
+      int N = hann[Q.ann_out].N;
+      if (Q.dir == 0)
+        mr = N > mr ? N : mr;
+      else
+        mz = N > mz ? N : mz;
+    }
+    const int cap = FACE_STAGE_BYTES / (2 * (int)sizeof(double));
+    plan.stage_r = mr <= cap ? mr : 0;
+    plan.stage_z = mz <= cap ? mz : 0;
+    if (!prepare_launches(plan)) plan.stage_r = plan.stage_z = 0;
+  }
   plan.n_rpairs = n_rpairs;
   plan.n_zpairs = n_zpairs;
   // initial mesh edges: cell_create + cell_clean_pi + tiph sync (src/main.c:77-85)
@@ -274,7 +284,7 @@ Engine::~Engine() {
   if (graph) hipGraphDestroy(graph);
   void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
-                  d_fo_z, d_run, d_js, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot};
+                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (ev0) hipEventDestroy(ev0);
@@ -309,7 +319,6 @@ DevMesh Engine::mesh() const {
   m.fo_r = d_fo_r;
   m.fo_z = d_fo_z;
   m.run_start = d_run;
-  m.js = d_js;
   m.Fface = d_Fface;
   m.Ntot = Ntot;
   m.NFtot = NFtot;

@@ -31,7 +31,6 @@ struct Engine {
   int *d_fL = nullptr, *d_fR = nullptr;
   double *d_fdphi = nullptr, *d_fcm = nullptr;
   int *d_fo_r = nullptr, *d_fo_z = nullptr, *d_run = nullptr;
-  int4 *d_js = nullptr;
   double *d_Fface = nullptr;
   int *d_err = nullptr;
   double *d_dt_tile = nullptr;

@@ -22,13 +22,16 @@ struct DevMesh {
   double *fdphi, *fcm;
   int *fo_r, *fo_z;       // per zone: first face of the cell as inner cell of its r / z pair
   int *run_start;
-  int4 *js;               // face-builder scratch, per zone
   double *Fface;          // [NQ+3][NFtot] increments of the r/z overlap faces
   double *dt_tile;        // CFL minimum of each interior tile (k_mindt), reduced per disk by k_scalar
   double4 *ann_rot;       // per annulus: rOm_a, rdrOm_a, dtOm_a at the centre, rdrOm_a at the outer face
   long long Ntot, NFtot;
   int *err;
 };
+
+// LDS the face builder may use to stage an outer annulus (two doubles per
+// cell), below the 160 KiB of a gfx950 workgroup minus its static arrays
+#define FACE_STAGE_BYTES (144 * 1024)
 
 // k_scalar runs one workgroup of SCALAR_THREADS per disk: all of them reduce
 // the disk's tile minima (SC_DT_FOLD), thread 0 does the scalar updates.
@@ -40,10 +43,9 @@ struct LaunchPlan {
   const int2 *ztiles; int n_ztiles;     // every zone
   const int2 *utiles; int n_utiles;     // every zone, DISCO_TILE - 1 cells per tile (k_update)
   const int2 *itiles; int n_itiles;     // interior zones (CFL)
-  const int2 *rptiles; int n_rptiles;   // inner cells of r pairs (pair, j0)
-  const int2 *zptiles; int n_zptiles;   // inner cells of z pairs
   const int2 *rftiles; int n_rftiles;   // r faces
   const int2 *zftiles; int n_zftiles;   // z faces
+  int stage_r, stage_z;                 // face builder: outer-annulus cells staged in LDS (0: none)
   const int2 *bc0; int n_bc0;           // ghost annuli 0 and NR-2
   const int2 *bc1; int n_bc1;           // ghost annulus NR-1
   const int2 *sz; int n_sz;             // z ghost layers
@@ -89,6 +91,7 @@ enum {
 
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
                     int recompute_w, int pre_ops, int post_ops);
+bool prepare_launches(const LaunchPlan &L);  // kernel attributes the plan needs
 void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op);
 void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st);
 void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ);

@@ -11,6 +11,14 @@
 #include "physics.h"
 
 #define TILE DISCO_TILE
+// minimum waves per SIMD the register allocation of the two largest kernels
+// must allow (occupancy against spills)
+#ifndef UPDATE_WAVES
+#define UPDATE_WAVES 4
+#endif
+#ifndef FLUX_WAVES
+#define FLUX_WAVES 1
+#endif
 
 __device__ __forceinline__ void set_err(int *err, int code) { atomicCAS(err, 0, code); }
 
@@ -173,26 +181,32 @@ __device__ __forceinline__ void face_slope(const DevMesh &m, const PairInfo &P, 
   }
 }
 
-// Stage 1 (pair tiles: up to DISCO_TILE inner cells of one pair per
-// workgroup): every inner cell guesses its start cursor and counts its faces.
-// The workgroup holding cell 0 also finds jpmin (face_create_destroy.c:82-96).
-__global__ __launch_bounds__(TILE) void k_faces_count(DevMesh m, const int2 *ptiles) {
-  const int2 tl = ptiles[xcd_order()];
-  const int p = tl.x, tid = threadIdx.x;
-  const PairInfo P = m.pairs[p];
-  const AnnInfo A = m.ann[P.ann_in];
-  const AnnInfo B = m.ann[P.ann_out];
+// One workgroup of FTB threads per annulus pair builds the pair's faces.
+// Inner cell j's walk starts at the outer cell holding its left edge and
+// ends at the one holding its right edge, emitting one face more than the
+// cursor advances.  So with g(j) = guess_outer(Tin[j]) (and g(-1) = jpmin,
+// face_create_destroy.c:82-96) the face count of cell j is
+// 1 + (g(j) - g(j-1)) mod N_out.  Cells are taken FTB at a time: counts,
+// workgroup prefix sum, then every cell walks from g(j-1) with the
+// reference's per-cell step writing its faces, and checks that it wrote
+// that many faces and ended at g(j).  If all cells agree, every walk began
+// where the sequential walk would have (induction over j) and the faces are
+// the reference's; if any did not (ties between edges), thread 0 replays the
+// sequential walk.  The outer annulus's edges are staged in LDS when they
+// fit, so the walks' dependent loads are LDS reads.
+__device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, const AnnInfo &A, const AnnInfo &B,
+                                           int p, const double *Tout, const double *Dout, double *s_d, int *s_j,
+                                           int *s_w) {
+  const int tid = threadIdx.x;
   const int Nin = A.N, Nout = B.N;
   const double *Tin = m.tiph + A.zbase, *Din = m.dphi + A.zbase;
-  const double *Tout = m.tiph + B.zbase, *Dout = m.dphi + B.zbase;
-  int jpmin = 0;
-  if (tl.y == 0) {
-    __shared__ double s_d[TILE];
-    __shared__ int s_j[TILE];
+  int *fo = (P.dir == 0 ? m.fo_r : m.fo_z) + A.zbase;
+  // jpmin: smallest wrap_p(Tout[jp] - Tin[Nin-1]), first index on ties
+  {
     const double p0 = Tin[Nin - 1];
     double bd = TWO_PI;
     int bj = -1;
-    for (int jp = tid; jp < Nout; jp += TILE) {
+    for (int jp = tid; jp < Nout; jp += FTB) {
       double dp = wrap_p(Tout[jp] - p0);
       if (bd > dp) {
         bd = dp;
@@ -202,7 +216,7 @@ __global__ __launch_bounds__(TILE) void k_faces_count(DevMesh m, const int2 *pti
     s_d[tid] = bd;
     s_j[tid] = bj;
     __syncthreads();
-    for (int s = TILE / 2; s > 0; s >>= 1) {
+    for (int s = FTB / 2; s > 0; s >>= 1) {
       if (tid < s) {
         int j2 = s_j[tid + s];
         if (j2 >= 0) {
@@ -216,77 +230,53 @@ __global__ __launch_bounds__(TILE) void k_faces_count(DevMesh m, const int2 *pti
       }
       __syncthreads();
     }
-    jpmin = s_j[0] < 0 ? 0 : s_j[0];
-    if (tid == 0) m.pst[p].jpmin = jpmin;
   }
-  const int j = tl.y + tid;
-  if (j >= Nin) return;
+  const int jpmin = s_j[0] < 0 ? 0 : s_j[0];
   FaceOut o;
+  o.fL = m.fL;
+  o.fR = m.fR;
+  o.fdp = m.fdphi;
+  o.fcm = m.fcm;
+  o.run_start = m.run_start;
+  o.fbase = P.face_base;
+  o.rbase = P.run_base;
+  o.cap = P.cap;
+  o.run_cap = P.run_cap;
+  o.zin = A.zbase;
+  o.zout = B.zbase;
   o.err = m.err;
-  int bad = 0;
-  int jps = (j == 0) ? jpmin : guess_outer(Tin[j - 1], Tout, Nout);
-  int jpe;
-  int cnt = jloop_cell<false>(j, jps, Tin, Din, Tout, Dout, Nin, Nout, 0, o, &jpe, &bad);
-  m.js[A.zbase + j] = make_int4(jps, cnt, jpe, bad);
-}
-
-// Stage 2 (one workgroup per pair): verify every guess against the previous
-// cell's end cursor, prefix-sum the counts into first-face offsets, or replay
-// the reference's sequential walk when a guess is off (ties between edges).
-__global__ __launch_bounds__(FTB) void k_faces_scan(DevMesh m, int pair0) {
-  __shared__ int s_w[FTB / 64];
-  const int tid = threadIdx.x;
-  const int p = pair0 + xcd_order();
-  const PairInfo P = m.pairs[p];
-  const AnnInfo A = m.ann[P.ann_in];
-  const AnnInfo B = m.ann[P.ann_out];
-  const int Nin = A.N, Nout = B.N;
-  const int4 *js = m.js + A.zbase;
-  int *fo = (P.dir == 0 ? m.fo_r : m.fo_z) + A.zbase;
-  const int jpmin = m.pst[p].jpmin;
-  int bad = 0;
-  for (int j = tid; j < Nin; j += FTB) {
-    int4 v = js[j];
-    if (v.w) bad = 1;
-    if (j + 1 < Nin && v.z != js[j + 1].x) bad = 1;
+  int bad = 0, carry = 0;
+  for (int base = 0; base < Nin; base += FTB) {
+    const int j = base + tid;
+    int gs = 0, ge = 0, cnt = 0;
+    if (j < Nin) {
+      gs = (j == 0) ? jpmin : guess_outer(Tin[j - 1], Tout, Nout);
+      ge = guess_outer(Tin[j], Tout, Nout);
+      int adv = ge - gs;
+      if (adv < 0) adv += Nout;
+      cnt = 1 + adv;
+    }
+    int chunk;
+    const int foff = carry + block_excl_scan(cnt, s_w, &chunk);
+    if (j < Nin) {
+      fo[j] = foff;
+      int jpe, b2 = 0;
+      int n = jloop_cell<true>(j, gs, Tin, Din, Tout, Dout, Nin, Nout, foff, o, &jpe, &b2);
+      if (b2 || n != cnt || (j + 1 < Nin && jpe != ge)) bad = 1;
+    }
+    carry += chunk;
   }
   bad = __syncthreads_or(bad);
-  int total = 0;
-  if (!bad) {
-    int carry = 0;
-    for (int base = 0; base < Nin; base += FTB) {
-      int j = base + tid;
-      int c = (j < Nin) ? js[j].y : 0;
-      int chunk;
-      int foff = carry + block_excl_scan(c, s_w, &chunk);
-      if (j < Nin) fo[j] = foff;
-      carry += chunk;
-    }
-    total = carry;
-  } else if (tid == 0) {
-    const double *Tin = m.tiph + A.zbase, *Din = m.dphi + A.zbase;
-    const double *Tout = m.tiph + B.zbase, *Dout = m.dphi + B.zbase;
-    FaceOut o;
-    o.fL = m.fL;
-    o.fR = m.fR;
-    o.fdp = m.fdphi;
-    o.fcm = m.fcm;
-    o.run_start = m.run_start;
-    o.fbase = P.face_base;
-    o.rbase = P.run_base;
-    o.cap = P.cap;
-    o.run_cap = P.run_cap;
-    o.zin = A.zbase;
-    o.zout = B.zbase;
-    o.err = m.err;
-    int jp = jpmin, foff = 0, b2 = 0;
+  int total = carry;
+  if (bad && tid == 0) {
+    int f = 0, b2 = 0, jp = jpmin;
     for (int j = 0; j < Nin; ++j) {
-      fo[j] = foff;
+      fo[j] = f;
       int jpe;
-      foff += jloop_cell<true>(j, jp, Tin, Din, Tout, Dout, Nin, Nout, foff, o, &jpe, &b2);
+      f += jloop_cell<true>(j, jp, Tin, Din, Tout, Dout, Nin, Nout, f, o, &jpe, &b2);
       jp = jpe;
     }
-    total = foff;
+    total = f;
   }
   if (tid == 0) {
     int T = total - Nin;
@@ -307,41 +297,27 @@ __global__ __launch_bounds__(FTB) void k_faces_scan(DevMesh m, int pair0) {
   }
 }
 
-// Stage 3 (pair tiles): every inner cell writes its faces (and the runs they
-// start) at its offset.
-__global__ __launch_bounds__(TILE) void k_faces_emit(DevMesh m, const int2 *ptiles) {
-  const int2 tl = ptiles[xcd_order()];
-  const int p = tl.x;
-  const PairState ps = m.pst[p];
+// nstage: LDS doubles per array for the outer annulus (0: none staged)
+__global__ __launch_bounds__(FTB) void k_faces(DevMesh m, int pair0, int nstage) {
+  extern __shared__ double s_stage[];  // Tout [nstage], Dout [nstage]
+  __shared__ double s_d[FTB];
+  __shared__ int s_j[FTB];
+  __shared__ int s_w[FTB / 64];
+  const int p = pair0 + xcd_order();
   const PairInfo P = m.pairs[p];
   const AnnInfo A = m.ann[P.ann_in];
-  const int j = tl.y + threadIdx.x;
-  if (j >= A.N) return;
-  const int *fo = (P.dir == 0 ? m.fo_r : m.fo_z) + A.zbase;
-  const int foff = fo[j];
-  int cnt;
-  if (!ps.seq) {
-    const AnnInfo B = m.ann[P.ann_out];
-    FaceOut o;
-    o.fL = m.fL;
-    o.fR = m.fR;
-    o.fdp = m.fdphi;
-    o.fcm = m.fcm;
-    o.run_start = m.run_start;
-    o.fbase = P.face_base;
-    o.rbase = P.run_base;
-    o.cap = P.cap;
-    o.run_cap = P.run_cap;
-    o.zin = A.zbase;
-    o.zout = B.zbase;
-    o.err = m.err;
-    int jpe, b2 = 0;
-    cnt = jloop_cell<true>(j, m.js[A.zbase + j].x, m.tiph + A.zbase, m.dphi + A.zbase, m.tiph + B.zbase,
-                           m.dphi + B.zbase, A.N, B.N, foff, o, &jpe, &b2);
+  const AnnInfo B = m.ann[P.ann_out];
+  const double *Tg = m.tiph + B.zbase, *Dg = m.dphi + B.zbase;
+  if (B.N <= nstage) {
+    for (int i = threadIdx.x; i < B.N; i += FTB) {
+      s_stage[i] = Tg[i];
+      s_stage[nstage + i] = Dg[i];
+    }
+    __syncthreads();
+    faces_pair(m, P, A, B, p, s_stage, s_stage + nstage, s_d, s_j, s_w);
   } else {
-    cnt = ((j + 1 < A.N) ? fo[j + 1] : ps.count) - foff;
+    faces_pair(m, P, A, B, p, Tg, Dg, s_d, s_j, s_w);
   }
-  (void)cnt;
 }
 
 // Faces of a pair that have inner cell j (local index) as L, and the runs
@@ -596,7 +572,7 @@ __device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A,
 }
 
 template <int NQ>
-__global__ __launch_bounds__(TILE) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
+__global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
   const int2 tl = ftiles[xcd_order()];
   const int p = tl.x;
   const int f = tl.y + threadIdx.x;
@@ -705,7 +681,7 @@ __device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy
 }
 
 template <int NQ>
-__global__ __launch_bounds__(TILE, 4) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
+__global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
   const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
   const int t = threadIdx.x;
@@ -1271,17 +1247,12 @@ KProf *g_kprof = nullptr;
 // overlap faces of the r (dir 0) or z (dir 1) pairs, with their slopes S
 template <int NQ>
 static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir) {
-  const int2 *pt = dir == 0 ? L.rptiles : L.zptiles;
-  const int npt = dir == 0 ? L.n_rptiles : L.n_zptiles;
   const int np = dir == 0 ? L.n_rpairs : L.n_zpairs;
   const int p0 = dir == 0 ? 0 : L.n_rpairs;
   if (!np) return;
-  KLAUNCH(st, dir == 0 ? "k_faces_count_r" : "k_faces_count_z",
-          hipLaunchKernelGGL(k_faces_count, dim3(npt), dim3(TILE), 0, st, m, pt));
-  KLAUNCH(st, dir == 0 ? "k_faces_scan_r" : "k_faces_scan_z",
-          hipLaunchKernelGGL(k_faces_scan, dim3(np), dim3(FTB), 0, st, m, p0));
-  KLAUNCH(st, dir == 0 ? "k_faces_emit_r" : "k_faces_emit_z",
-          hipLaunchKernelGGL(k_faces_emit, dim3(npt), dim3(TILE), 0, st, m, pt));
+  const int nst = dir == 0 ? L.stage_r : L.stage_z;
+  KLAUNCH(st, dir == 0 ? "k_faces_r" : "k_faces_z",
+          hipLaunchKernelGGL(k_faces, dim3(np), dim3(FTB), (size_t)nst * 2 * sizeof(double), st, m, p0, nst));
   const int2 *ft = dir == 0 ? L.rftiles : L.zftiles;
   const int nft = dir == 0 ? L.n_rftiles : L.n_zftiles;
   if (nft)
@@ -1324,6 +1295,13 @@ void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int N
     launch_substep_q<5>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
   else
     launch_substep_q<6>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
+}
+
+bool prepare_launches(const LaunchPlan &L) {
+  const int nst = L.stage_r > L.stage_z ? L.stage_r : L.stage_z;
+  if (!nst) return true;
+  return hipFuncSetAttribute((const void *)k_faces, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)(nst * 2 * sizeof(double))) == hipSuccess;
 }
 
 void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op) {
