@@ -29,6 +29,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E, published
 B_ALG_PER_ZONE_UPDATE = 320  # 8 * (6*NUM_Q + 10) bytes at NUM_Q = 5
 
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE, then
+# WRITE_SIZE (KiB per dispatch, one counter per run, commands and per-kernel
+# table in profiles/binary2d_r512_hbm_traffic.md), scaled by what the same
+# counters read for a plain 1 GiB copy: FETCH_SIZE x 2.0, WRITE_SIZE x 1.0.
+# Counters cannot be read from inside the timed run, so the figure is
+# reported only for the workload and kernel it was measured on.
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+MEASURED_TRAFFIC_KIB = {("binary2d_r512", "k_update"): (136797, 72647)}
+
 
 def r_face_count(n_p, nz_layers):
     """Overlap faces between neighbouring annuli: N_in + N_out per pair (plus
@@ -207,6 +216,9 @@ def main(argv=None):
     if alg:
         ach = alg / (dom_avg_ms * 1e-3) / 1e9
         roof.update(achieved=round(ach, 1), frac=round(ach / HBM_PEAK_GBS, 4), alg_bytes_per_launch=int(alg))
+    meas = MEASURED_TRAFFIC_KIB.get((args.workload, dom_name))
+    if meas:
+        roof["traffic"] = int((meas[0] * FETCH_SCALE + meas[1] * WRITE_SCALE) * 1024)
     step_ach = value * B_ALG_PER_ZONE_UPDATE / 1e9
     kernels = {k: {"avg_ms": round(v[0] / v[1], 5), "share": round(v[0] / sum(x[0] for x in prof.values()), 4)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
