@@ -325,6 +325,8 @@ DevMesh Engine::mesh() const {
   m.err = d_err;
   m.dt_tile = d_dt_tile;
   m.ann_rot = d_ann_rot;
+  m.u_save = nullptr;
+  m.torque_tile = nullptr;
   return m;
 }
 

@@ -25,6 +25,9 @@ struct DevMesh {
   double *Fface;          // [NQ+3][NFtot] increments of the r/z overlap faces
   double *dt_tile;        // CFL minimum of each interior tile (k_mindt), reduced per disk by k_scalar
   double4 *ann_rot;       // per annulus: rOm_a, rdrOm_a, dtOm_a at the centre, rdrOm_a at the outer face
+  // adjoint recording (null in plain runs): k_update stores the conserved
+  // state it converts (u_save, NQ x Ntot) and its tiles' torque sums
+  double *u_save, *torque_tile;
   long long Ntot, NFtot;
   int *err;
 };

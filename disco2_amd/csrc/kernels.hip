@@ -7,35 +7,7 @@
 // atomics), which keeps the results deterministic and bitwise comparable.
 #include <hip/hip_runtime.h>
 
-#include "kernels.h"
-#include "physics.h"
-
-#define TILE DISCO_TILE
-// minimum waves per SIMD the register allocation of the two largest kernels
-// must allow (occupancy against spills)
-#ifndef UPDATE_WAVES
-#define UPDATE_WAVES 4
-#endif
-#ifndef FLUX_WAVES
-#define FLUX_WAVES 1
-#endif
-
-__device__ __forceinline__ void set_err(int *err, int code) { atomicCAS(err, 0, code); }
-
-// XCD-aware work order.  Workgroups are dispatched round-robin over the 8
-// XCDs (workgroup b runs on XCD b mod 8), each with its own L2.  Tiles are
-// listed annulus by annulus, so handing XCD x one contiguous band of tiles
-// keeps the neighbouring annuli that a tile's faces reach in the same L2.
-__device__ __forceinline__ int xcd_order() {
-  const int n = gridDim.x, b = blockIdx.x;
-  const int q = n >> 3, r = n & 7, x = b & 7;
-  return x * q + min(x, r) + (b >> 3);
-}
-
-__device__ __forceinline__ const DiskParams &disk_of(const DevMesh &m, int d) { return m.dp[d]; }
-
-__device__ __forceinline__ int rpair_of(const DiskParams &P, int k, int i) { return P.rpair_base + k * (P.NR - 1) + i; }
-__device__ __forceinline__ int zpair_of(const DiskParams &P, int k, int i) { return P.zpair_base + k * P.NR + i; }
+#include "device_common.h"
 
 // ----------------------------------------------------------------------------
 // Overlap faces between two adjacent annuli: build_jloop and face_create,
@@ -162,23 +134,6 @@ __device__ int block_excl_scan(int v, int *s_w, int *total) {
   *total = s_w[nw - 1];
   __syncthreads();
   return base + x - v;
-}
-
-// S = (W_R - W_L)/(deltaL + deltaR) of one face, cell_plm.c:36-49 and 86-99
-template <int NQ>
-__device__ __forceinline__ void face_slope(const DevMesh &m, const PairInfo &P, long long g, double *S) {
-  const int cL = m.fL[g], cR = m.fR[g];
-  const double cm = m.fcm[g];
-  double pL = clean_pi(m.tiph[cL]) - .5 * m.dphi[cL];
-  double pR = clean_pi(m.tiph[cR]) - .5 * m.dphi[cR];
-  double dpL = wrap_pm(cm - pL);
-  double dpR = wrap_pm(pR - cm);
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    double WL = m.prim[q * m.Ntot + cL] + dpL * m.gradp[q * m.Ntot + cL];
-    double WR = m.prim[q * m.Ntot + cR] - dpR * m.gradp[q * m.Ntot + cR];
-    S[q] = (WR - WL) / (P.deltaR + P.deltaL);
-  }
 }
 
 // One workgroup of FTB threads per annulus pair builds the pair's faces.
@@ -317,32 +272,6 @@ __global__ __launch_bounds__(FTB) void k_faces(DevMesh m, int pair0, int nstage)
     faces_pair(m, P, A, B, p, s_stage, s_stage + nstage, s_d, s_j, s_w);
   } else {
     faces_pair(m, P, A, B, p, Tg, Dg, s_d, s_j, s_w);
-  }
-}
-
-// Faces of a pair that have inner cell j (local index) as L, and the runs
-// that have outer cell jo as R, in face order.
-template <typename F>
-__device__ __forceinline__ void faces_as_L(const DevMesh &m, int p, const int *fo, long long z, int j, int Nin,
-                                           F f) {
-  const PairInfo P = m.pairs[p];
-  const PairState S = m.pst[p];
-  int f1 = (j + 1 < Nin) ? fo[z + 1] : S.count;
-  if (f1 > S.count) f1 = S.count;
-  int f0 = fo[z];
-  if (f0 < 0) f0 = 0;
-  for (int n = f0; n < f1; ++n) f(P, P.face_base + n);
-}
-template <typename F>
-__device__ __forceinline__ void faces_as_R(const DevMesh &m, int p, int jo, int Nout, F f) {
-  const PairInfo P = m.pairs[p];
-  const PairState S = m.pst[p];
-  int t = jo - S.jpmin;
-  if (t < 0) t += Nout;
-  for (; t <= S.T; t += Nout) {
-    int f0 = m.run_start[P.run_base + t];
-    int f1 = m.run_start[P.run_base + t + 1];
-    for (int n = f0; n < f1; ++n) f(P, P.face_base + n);
   }
 }
 
@@ -520,27 +449,6 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
 // (riemann_setup_rz, :11-44), each solved once; riemann_AddFlux's
 // increments dt*dA*F are stored per face and gathered by k_update.
 // ----------------------------------------------------------------------------
-// analytic frame rotation of an annulus (filled by k_scalar whenever the
-// masses move; zero for the non-rotating frame)
-__device__ __forceinline__ double4 ann_rotation(const DevMesh &m, const DiskParams &P, int a) {
-  if (P.w_analytic == A_FIXED) return make_double4(0.0, 0.0, 0.0, 0.0);
-  return m.ann_rot[a];
-}
-
-// the viscous flux reads cell-centre fields itself; set its pointers
-__device__ __forceinline__ void visc_cells(const DevMesh &m, ViscIn &vi, long long cL, long long cR) {
-  vi.prim = m.prim;
-  vi.grad = m.grad;
-  vi.gradp = m.gradp;
-  vi.stride = m.Ntot;
-  vi.cL = cL;
-  vi.cR = cR;
-  vi.tL = clean_pi(m.tiph[cL]);
-  vi.tR = clean_pi(m.tiph[cR]);
-  vi.dL = m.dphi[cL];
-  vi.dR = m.dphi[cR];
-}
-
 // Face (j, j+1) of an annulus, riemann_setup_p (src/Riemann/Riemann_setup.c:46-80)
 // + riemann_AddFlux: the increments X = dt*dA*F and viscous Y.
 template <int NQ>
@@ -682,14 +590,15 @@ __device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy
 
 template <int NQ>
 __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
-  const int2 tl = tiles[xcd_order()];
+  const int ti = xcd_order();
+  const int2 tl = tiles[ti];
   const AnnInfo A = m.ann[tl.x];
   const int t = threadIdx.x;
   const int n = min(TILE - 1, A.N - tl.y);  // cells of this tile
   const int j = tl.y + t;
   __shared__ double s_md[2][TILE / 64];
   __shared__ double s_F[NQ + 3][TILE];      // increments of phi faces j0-1 .. j0+n-1
-  double md0 = 0.0, md1 = 0.0;
+  double md0 = 0.0, md1 = 0.0, tq = 0.0;
   const DiskParams &P = disk_of(m, A.disk);
   const DiskScalars &sc = m.sc[A.disk];
   const double4 rot = ann_rotation(m, P, tl.x);
@@ -788,38 +697,13 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
       double phi = tcl - .5 * dphi;
       double cphi, sphi;
       sincos(phi, &sphi, &cphi);
-      double rho = pr[RHO], Pp = pr[PPP], vr = pr[URR], vz = pr[UZZ];
-      double vp = pr[UPP] * r;
+      double rho = pr[RHO], Pp = pr[PPP];
       double dV = dphi * .5 * (rp * rp - rm * rm) * dz;
       double zc = .5 * (zp + zm);
-      double R = sqrt(r * r + zc * zc);
-      double sint, cost, gravdist;
-      if (P.grav2d == 1) {
-        sint = 1.0;
-        cost = 0.0;
-        gravdist = r;
-      } else {
-        sint = r / R;
-        cost = zc / R;
-        gravdist = R;
-      }
-      double fr, fp, Fr = 0.0, Fp = 0.0;
-      d_grav_force(g0.M, sc.xbh[0], sc.ybh[0], sc.epsn, P.phi_order, gravdist, cphi, sphi, &fr, &fp);
-      Fr += fr;
-      Fp += fp;
-      d_grav_force(g1.M, sc.xbh[1], sc.ybh[1], sc.epsn, P.phi_order, gravdist, cphi, sphi, &fr, &fp);
-      Fr += fr;
-      Fp += fp;
-      double w_a = rot.x;
       double rdrOm_a = rot.y;
-      double F_centrifugal_r = w_a * w_a / r;
-      double dtOm_a = rot.z;
-      double F_euler_phi = -(r * dtOm_a + vr * rdrOm_a);
-      u[SRR] += dt * dV * (rho * vp * vp + Pp) / r;
-      u[SRR] += dt * dV * rho * (Fr * sint + F_centrifugal_r);
-      u[LLL] += dt * dV * rho * (Fp - r * dtOm_a) * r;
-      u[SZZ] += dt * dV * rho * Fr * cost;
-      u[TAU] += dt * dV * rho * ((Fr * sint + F_centrifugal_r) * vr + Fr * vz * cost + (Fp + F_euler_phi) * vp);
+      double Fp;
+      add_geom_grav_src(u, pr, r, dV, dt, cphi, sphi, zc, rot.x, rot.y, rot.z, P, sc, &Fp);
+      if (m.torque_tile && interior_cell(P, i, k)) tq += dt * dV * rho * Fp * r;
       if (P.sink == 1) {  // get_rho_sink, :15-47
         double xpos = r * cphi, ypos = r * sphi;
         double xbh0 = sc.xbh[0], ybh0 = sc.ybh[0];
@@ -908,11 +792,26 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
     m.RKtiph[c] = rkt_own;
     m.tiph_new[c] = tnew;
     m.dphi_new[c] = dphi_new;
+    if (m.u_save) {  // the conserved state cons2prim sees (recorded for the adjoint)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) m.u_save[q * m.Ntot + c] = u[q];
+    }
     // cell_calc_prim, cell_conversion.c:105-122
     double out[NQ];
     d_cons2prim<NQ>(u, out, r, .5 * (rp * rp - rm * rm) * dphi_new * dz, P);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) m.prim_new[q * m.Ntot + c] = out[q];
+  }
+  if (m.torque_tile) {  // the tile's gravitational torque on the gas (adjoint loss)
+    for (int off = 32; off > 0; off >>= 1) tq += __shfl_down(tq, off);
+    if ((threadIdx.x & 63) == 0) s_md[0][threadIdx.x >> 6] = tq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t0 = 0.0;
+      for (int w = 0; w < TILE / 64; ++w) t0 += s_md[0][w];
+      m.torque_tile[ti] = t0;
+    }
+    __syncthreads();
   }
   if (P.sink == 1) {
     // Mdot partial sums of the block (one annulus, one disk)
@@ -1030,37 +929,11 @@ __device__ __forceinline__ double cell_dt(const DevMesh &m, const int2 tl, const
     double rdphi = .5 * (rp + rm) * m.dphi[c];
     if (rdphi < dr) dx = rdphi;
     if (dx > dz) dx = dz;
-    double Pp = m.prim[PPP * m.Ntot + c];
-    double rho = m.prim[RHO * m.Ntot + c];
-    double vp = m.prim[UPP * m.Ntot + c] * r - w;
-    double vr = m.prim[URR * m.Ntot + c];
-    double vz = m.prim[UZZ * m.Ntot + c];
-    double cs = sqrt(P.gamma * Pp / rho);
-    double cf2 = cs * cs;
-    double a = sqrt(cf2) + sqrt(vr * vr + vp * vp + vz * vz);
-    double d = P.cfl * dx / a;
-    if (P.visc > 0.0) {
-      double nu;
-      if (P.visc_const == 1) {
-        nu = P.visc;
-      } else {
-        double tiph = clean_pi(m.tiph[c]) - 0.5 * m.dphi[c];
-        if (P.initial_data_type == 0) {
-          nu = P.visc * P.gamma * Pp / rho * pow(fabs(r * cos(tiph)), 2.0);
-          if (r * cos(tiph) > 20.) nu = 0.000000001;
-        } else {
-          const DiskScalars &sc = m.sc[A.disk];
-          double ct, st;
-          sincos(tiph, &st, &ct);
-          double d0 = d_grav_dist(sc.xbh[0], sc.ybh[0], r, ct, st, 0.);
-          double d1 = d_grav_dist(sc.xbh[1], sc.ybh[1], r, ct, st, 0.);
-          double eps = P.g_eps;
-          nu = P.visc * Pp / rho / sqrt(pow_m1p5(d0 * d0 + eps * eps) * sc.gm[0].M + pow_m1p5(d1 * d1 + eps * eps) * sc.gm[1].M);
-        }
-      }
-      double dt_visc = .25 * dx * dx / nu;
-      d = d / (1. + d / dt_visc);
-    }
+    double pr[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) pr[q] = m.prim[q * m.Ntot + c];
+    const double tc = (P.visc > 0.0 && P.visc_const != 1) ? clean_pi(m.tiph[c]) - 0.5 * m.dphi[c] : 0.0;
+    double d = cell_dt_value(P, m.sc[A.disk], pr, r, dx, w, tc);
     if (d < dt) dt = d;
   }
   return dt;

@@ -9,6 +9,10 @@
 
 #include "disco_types.h"
 
+// every function here also compiles for the host, where tests check the
+// hand-written derivatives against finite differences of these forms
+#define DHD __host__ __device__
+
 #define TWO_PI (2.0 * M_PI)  // the reference's PHIMAX, src/Headers/header.h:21
 #define HALF_TURN (TWO_PI / 2.)
 #define MAXWRAP 64            // angle wraps beyond this mean a diverged state
@@ -19,7 +23,7 @@
 // first loop, or of the second, ending both), and otherwise the loops run
 // from the start: a rarely taken branch instead of two divergent loops in
 // every call, with the same result.
-__device__ __forceinline__ double wrap_range(double x, double lo, double hi) {
+DHD __forceinline__ double wrap_range(double x, double lo, double hi) {
   double y = x > hi ? x - TWO_PI : (x < lo ? x + TWO_PI : x);
   if (y > hi || y < lo) {
     y = x;
@@ -31,19 +35,19 @@ __device__ __forceinline__ double wrap_range(double x, double lo, double hi) {
   }
   return y;
 }
-__device__ __forceinline__ double clean_pi(double phi) {  // src/Cell/cell_misc.c:19-20
+DHD __forceinline__ double clean_pi(double phi) {  // src/Cell/cell_misc.c:19-20
   return wrap_range(phi, 0.0, TWO_PI);
 }
-__device__ __forceinline__ double wrap_pm(double d) {  // e.g. src/Cell/cell_plm.c:39-40
+DHD __forceinline__ double wrap_pm(double d) {  // e.g. src/Cell/cell_plm.c:39-40
   return wrap_range(d, -HALF_TURN, HALF_TURN);
 }
-__device__ __forceinline__ double wrap_p(double d) {  // e.g. src/Face/face_create_destroy.c:88-89
+DHD __forceinline__ double wrap_p(double d) {  // e.g. src/Face/face_create_destroy.c:88-89
   return wrap_range(d, 0.0, TWO_PI);
 }
 
 // analytic frame rotation, src/Sim/sim_w_a.c:8-73
 #define W_A_MILOS_INDEX 8
-__device__ __noinline__ double d_rOm_a_any(int type, double r, double a) {
+DHD __noinline__ double d_rOm_a_any(int type, double r, double a) {
   switch (type) {
     case A_KEPLER: return pow(r, -0.5);
     case A_OMEGA20: return r * 20;
@@ -54,7 +58,7 @@ __device__ __noinline__ double d_rOm_a_any(int type, double r, double a) {
     default: return 0.0;
   }
 }
-__device__ __noinline__ double d_rdrOm_a_any(int type, double r, double a) {
+DHD __noinline__ double d_rdrOm_a_any(int type, double r, double a) {
   switch (type) {
     case A_KEPLER: return -1.5 * pow(r, -1.5);
     case A_MILOS: {
@@ -68,13 +72,13 @@ __device__ __noinline__ double d_rdrOm_a_any(int type, double r, double a) {
   }
 }
 // the common non-rotating frame inline, the rest out of line
-__device__ __forceinline__ double d_rOm_a(int type, double r, double a) {
+DHD __forceinline__ double d_rOm_a(int type, double r, double a) {
   return type == A_FIXED ? 0.0 : d_rOm_a_any(type, r, a);
 }
-__device__ __forceinline__ double d_rdrOm_a(int type, double r, double a) {
+DHD __forceinline__ double d_rdrOm_a(int type, double r, double a) {
   return type == A_FIXED ? 0.0 : d_rdrOm_a_any(type, r, a);
 }
-__device__ __forceinline__ double d_dtOm_a(int type, double r, double a, double tau) {
+DHD __forceinline__ double d_dtOm_a(int type, double r, double a, double tau) {
   if (type == A_MILOS) {
     double a0 = 1.0;
     double dta_o_a = -.25 * pow(a / a0, -4) / tau;
@@ -90,16 +94,22 @@ __device__ __forceinline__ double d_dtOm_a(int type, double r, double a, double 
 // (xbh, ybh) = (r_p cos phi_p, r_p sin phi_p) is uniform over a launch and is
 // computed once per disk and substep (DiskScalars::xbh/ybh); cphi, sphi are
 // cos and sin of the cell angle, computed once per cell.
-__device__ __forceinline__ double d_grav_dist(double xbh, double ybh, double r, double cphi, double sphi,
+DHD __forceinline__ double d_grav_dist(double xbh, double ybh, double r, double cphi, double sphi,
                                              double z) {
   double x = r * cphi;
   double y = r * sphi;
   return sqrt((x - xbh) * (x - xbh) + (y - ybh) * (y - ybh) + z * z);
 }
 
+// x^-1.5 and x^1.5 through the correctly rounded sqrt (within two ulps of
+// pow); the reference calls pow (src/Riemann/Riemann_routines.c:262,
+// src/Cell/cell_source.c:12) and device and host pow differ by ulps anyway.
+DHD __forceinline__ double pow_m1p5(double x) { return 1.0 / (x * sqrt(x)); }
+DHD __forceinline__ double pow_p1p5(double x) { return x * sqrt(x); }
+
 // src/Cell/cell_conversion.c:11-33
 template <int NQ>
-__device__ __forceinline__ void d_prim2cons(const double *prim, double *cons, double r, double dV,
+DHD __forceinline__ void d_prim2cons(const double *prim, double *cons, double r, double dV,
                                             double G, int NC) {
   double rho = prim[RHO];
   double Pp = prim[PPP];
@@ -119,7 +129,7 @@ __device__ __forceinline__ void d_prim2cons(const double *prim, double *cons, do
 
 // src/Cell/cell_conversion.c:57-103
 template <int NQ>
-__device__ __forceinline__ void d_cons2prim(const double *cons, double *prim, double r, double dV,
+DHD __forceinline__ void d_cons2prim(const double *cons, double *prim, double r, double dV,
                                             const DiskParams &P) {
   double G = P.gamma;
   double rho = cons[DDD] / dV;
@@ -151,6 +161,101 @@ __device__ __forceinline__ void d_cons2prim(const double *cons, double *prim, do
   for (int q = 5; q < NQ; ++q) prim[q] = cons[q] / cons[DDD];
 }
 
+// fgrav and gravMassForce of one mass at a cell, projected on r and phi,
+// src/Cell/cell_source.c:11-13, 49-71.  epsn = pow(eps, n) is uniform over a
+// launch (DiskScalars::epsn); xbh, ybh are rp*cos(pp), rp*sin(pp); cosp,
+// sinp are cos and sin of the cell angle.
+DHD __forceinline__ double d_fgrav(double M, double r, double epsn, int n) {
+  if (n == 2) return (M * r / pow_p1p5(r * r + epsn));  // pow(r,1) = r, pow(r,2), pow(x,1.5)
+  return (M * pow(r, n - 1.) / pow(pow(r, n) + epsn, 1. + 1. / n));
+}
+DHD __forceinline__ void d_grav_force(double M, double xbh, double ybh, double epsn, int order, double r,
+                                      double cosp, double sinp, double *fr, double *fp) {
+  double dx = r * cosp - xbh;
+  double dy = r * sinp - ybh;
+  double script_r = sqrt(dx * dx + dy * dy);
+  double cosa = dx / script_r;
+  double sina = dy / script_r;
+  double cosap = cosa * cosp + sina * sinp;
+  double sinap = sina * cosp - cosa * sinp;
+  double f1 = -d_fgrav(M, script_r, epsn, order);
+  *fr = cosap * f1;
+  *fp = sinap * f1;
+}
+
+// The geometric, gravity and frame terms of cell_add_src
+// (src/Cell/cell_source.c:117-173), added to u in the reference's order.
+// pr: the cell's primitives, (cphi, sphi): cos and sin of its centre angle,
+// zc its centre height, (w_a, rdrOm_a, dtOm_a) the frame rotation at r.
+// Returns F_phi of the two masses in *Fp_out.
+DHD __forceinline__ void add_geom_grav_src(double *u, const double *pr, double r, double dV, double dt, double cphi,
+                                           double sphi, double zc, double w_a, double rdrOm_a, double dtOm_a,
+                                           const DiskParams &P, const DiskScalars &sc, double *Fp_out) {
+  double rho = pr[RHO], Pp = pr[PPP], vr = pr[URR], vz = pr[UZZ];
+  double vp = pr[UPP] * r;
+  double R = sqrt(r * r + zc * zc);
+  double sint, cost, gravdist;
+  if (P.grav2d == 1) {
+    sint = 1.0;
+    cost = 0.0;
+    gravdist = r;
+  } else {
+    sint = r / R;
+    cost = zc / R;
+    gravdist = R;
+  }
+  double fr, fp, Fr = 0.0, Fp = 0.0;
+  d_grav_force(sc.gm[0].M, sc.xbh[0], sc.ybh[0], sc.epsn, P.phi_order, gravdist, cphi, sphi, &fr, &fp);
+  Fr += fr;
+  Fp += fp;
+  d_grav_force(sc.gm[1].M, sc.xbh[1], sc.ybh[1], sc.epsn, P.phi_order, gravdist, cphi, sphi, &fr, &fp);
+  Fr += fr;
+  Fp += fp;
+  double F_centrifugal_r = w_a * w_a / r;
+  double F_euler_phi = -(r * dtOm_a + vr * rdrOm_a);
+  u[SRR] += dt * dV * (rho * vp * vp + Pp) / r;
+  u[SRR] += dt * dV * rho * (Fr * sint + F_centrifugal_r);
+  u[LLL] += dt * dV * rho * (Fp - r * dtOm_a) * r;
+  u[SZZ] += dt * dV * rho * Fr * cost;
+  u[TAU] += dt * dV * rho * ((Fr * sint + F_centrifugal_r) * vr + Fr * vz * cost + (Fp + F_euler_phi) * vp);
+  *Fp_out = Fp;
+}
+
+// CFL time step of one cell, src/Cell/cell_mindt.c:44-90: pr its
+// primitives, r its centre radius, dx its smallest extent, w the mesh
+// velocity, tc its centre angle (read only by the alpha viscosity).
+DHD __forceinline__ double cell_dt_value(const DiskParams &P, const DiskScalars &sc, const double *pr, double r,
+                                         double dx, double w, double tc) {
+  double Pp = pr[PPP];
+  double rho = pr[RHO];
+  double vp = pr[UPP] * r - w;
+  double vr = pr[URR];
+  double vz = pr[UZZ];
+  double cs = sqrt(P.gamma * Pp / rho);
+  double cf2 = cs * cs;
+  double a = sqrt(cf2) + sqrt(vr * vr + vp * vp + vz * vz);
+  double d = P.cfl * dx / a;
+  if (P.visc > 0.0) {
+    double nu;
+    if (P.visc_const == 1) {
+      nu = P.visc;
+    } else if (P.initial_data_type == 0) {
+      nu = P.visc * P.gamma * Pp / rho * pow(fabs(r * cos(tc)), 2.0);
+      if (r * cos(tc) > 20.) nu = 0.000000001;
+    } else {
+      double ct, st;
+      sincos(tc, &st, &ct);
+      double d0 = d_grav_dist(sc.xbh[0], sc.ybh[0], r, ct, st, 0.);
+      double d1 = d_grav_dist(sc.xbh[1], sc.ybh[1], r, ct, st, 0.);
+      double eps = P.g_eps;
+      nu = P.visc * Pp / rho / sqrt(pow_m1p5(d0 * d0 + eps * eps) * sc.gm[0].M + pow_m1p5(d1 * d1 + eps * eps) * sc.gm[1].M);
+    }
+    double dt_visc = .25 * dx * dx / nu;
+    d = d / (1. + d / dt_visc);
+  }
+  return d;
+}
+
 // Flux of one face, src/Riemann/Riemann_routines.c:14-397.  The caller has
 // reconstructed primL/primR (riemann_setup_*); n is the face normal
 // (0: r, 1: phi, 2: z), w the face velocity.  On return X[q] = dt*dA*F[q]
@@ -164,14 +269,9 @@ struct ViscIn {   // what riemann_visc_flux reads besides the reconstructed stat
   double rdr_om;                         // sim_rdrOm_a at the face radius
 };
 
-// x^-1.5 and x^1.5 through the correctly rounded sqrt (within two ulps of
-// pow); the reference calls pow (src/Riemann/Riemann_routines.c:262,
-// src/Cell/cell_source.c:12) and device and host pow differ by ulps anyway.
-__device__ __forceinline__ double pow_m1p5(double x) { return 1.0 / (x * sqrt(x)); }
-__device__ __forceinline__ double pow_p1p5(double x) { return x * sqrt(x); }
 
 template <int NQ>
-__device__ __forceinline__ void d_set_flux(const double *prim, double *F, double r, int n0, int n1, int n2,
+DHD __forceinline__ void d_set_flux(const double *prim, double *F, double r, int n0, int n1, int n2,
                                            double G) {  // Riemann_routines.c:157-191
   double rho = prim[RHO];
   double Pp = prim[PPP];
@@ -191,7 +291,7 @@ __device__ __forceinline__ void d_set_flux(const double *prim, double *F, double
 }
 
 template <int NQ>
-__device__ __forceinline__ int d_face_flux(const double *primL, const double *primR, int dir, double r,
+DHD __forceinline__ int d_face_flux(const double *primL, const double *primR, int dir, double r,
                                            double dA, double w, double dt, const DiskParams &P,
                                            const DiskScalars &sc, const ViscIn *vi, double *X, double *Y) {
   const GravState *gm = sc.gm;
@@ -372,23 +472,3 @@ __device__ __forceinline__ int d_face_flux(const double *primL, const double *pr
   return 0;
 }
 
-// fgrav and gravMassForce, src/Cell/cell_source.c:11-13, 49-71
-// epsn = pow(eps, n) is uniform over a launch (DiskScalars::epsn); xbh, ybh
-// are rp*cos(pp), rp*sin(pp); cosp, sinp are cos and sin of the cell angle.
-__device__ __forceinline__ double d_fgrav(double M, double r, double epsn, int n) {
-  if (n == 2) return (M * r / pow_p1p5(r * r + epsn));  // pow(r,1) = r, pow(r,2), pow(x,1.5)
-  return (M * pow(r, n - 1.) / pow(pow(r, n) + epsn, 1. + 1. / n));
-}
-__device__ __forceinline__ void d_grav_force(double M, double xbh, double ybh, double epsn, int order, double r,
-                                             double cosp, double sinp, double *fr, double *fp) {
-  double dx = r * cosp - xbh;
-  double dy = r * sinp - ybh;
-  double script_r = sqrt(dx * dx + dy * dy);
-  double cosa = dx / script_r;
-  double sina = dy / script_r;
-  double cosap = cosa * cosp + sina * sinp;
-  double sinap = sina * cosp - cosa * sinp;
-  double f1 = -d_fgrav(M, script_r, epsn, order);
-  *fr = cosap * f1;
-  *fp = sinap * f1;
-}
