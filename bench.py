@@ -157,6 +157,42 @@ def batched_leg(steps, warmup, ndisk=64):
             "speedup": t_plain / t_batched, "unit": "zone-updates/s"}
 
 
+def adjoint_leg(steps=200, every=20):
+    """The adjoint at the binary2d_r256_adjoint size: 200 RK2 steps forward and
+    the hand-written adjoint back (checkpoints every 20 steps, each segment
+    recomputed), against the same 200 steps forward alone (graph replay)."""
+    from disco2_amd import configs, ics
+    from disco2_amd.api import Batch, Setup
+    p, kind, kw = configs.c5_adjoint()
+    s = Setup(p)
+    prim0 = ics.make_ic(kind, s.geometry(), p, **kw)
+    b = Batch([s])
+    b.upload("prim", prim0)
+    b.init()
+    b.run(2)  # warm the graph and the kernels
+    b.upload("prim", prim0)
+    b.init()
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(steps, sync=False)
+    b.synchronize()
+    t_fwd = time.perf_counter() - t0
+    b.close()
+    b = Batch([s])
+    b.upload("prim", prim0)
+    b.init()
+    b.synchronize()
+    t0 = time.perf_counter()
+    loss, ga, gq, gp = b.adjoint(steps, checkpoint_every=every)
+    t_adj = time.perf_counter() - t0
+    b.close()
+    return {"workload": "binary2d_r256_adjoint", "interior_zones": s.interior_zones(), "steps": steps,
+            "checkpoint_every": every, "forward_ms_per_step": 1e3 * t_fwd / steps,
+            "adjoint_ms_per_step": 1e3 * t_adj / steps, "ratio": t_adj / t_fwd,
+            "loss": loss, "grad_alpha": ga, "grad_q": gq,
+            "note": "adjoint time includes its own forward pass (loss), the segment recomputes and the backward"}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +205,7 @@ def main(argv=None):
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one disk or batch per "
@@ -228,6 +265,11 @@ def main(argv=None):
         print("[bench] batched disks", file=sys.stderr, flush=True)
         batched = batched_leg(args.batched_steps, 2)
 
+    adjoint = None
+    if not args.no_adjoint:
+        print("[bench] adjoint", file=sys.stderr, flush=True)
+        adjoint = adjoint_leg()
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -254,6 +296,7 @@ def main(argv=None):
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
         "batched": batched,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -114,6 +114,8 @@ SIGNATURES = {
     "disco_batch_last_ms": (cd, [vp]),
     "disco_batch_profile": (ci, [vp, ci, ctypes.c_char_p, pd, pi, ci]),
     "disco_calibrate_copy": (cd, [cll]),
+    "disco_batch_torque_loss": (ci, [vp, ci, pd]),
+    "disco_batch_adjoint": (ci, [vp, ci, ci, pd, pd, pd, pd]),
     "disco_last_error": (ci, []),
     "disco_last_error_msg": (ccp, []),
     "disco_clear_error": (None, []),

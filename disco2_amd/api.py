@@ -133,6 +133,29 @@ class Batch:
         check()
         return out
 
+    def torque_loss(self, nsteps):
+        """Run nsteps RK2 steps; the time-averaged gravitational torque on the
+        gas (the loss of the adjoint)."""
+        out = np.zeros(1)
+        rc = lib().disco_batch_torque_loss(self.h, int(nsteps), dptr(out))
+        check()
+        if rc != 0:
+            raise RuntimeError("torque_loss failed")
+        return float(out[0])
+
+    def adjoint(self, nsteps, checkpoint_every=20):
+        """Run nsteps RK2 steps forward and the adjoint back: returns
+        (loss, dL/d alpha, dL/d q, dL/d prim0 as (NQ, Nzones))."""
+        n = self.nzones(0)
+        loss, ga, gq = np.zeros(1), np.zeros(1), np.zeros(1)
+        gp = np.zeros(self.setups[0].NQ * n)
+        rc = lib().disco_batch_adjoint(self.h, int(nsteps), int(checkpoint_every), dptr(loss), dptr(ga), dptr(gq),
+                                       dptr(gp))
+        check()
+        if rc != 0:
+            raise RuntimeError("adjoint failed")
+        return float(loss[0]), float(ga[0]), float(gq[0]), gp.reshape(self.setups[0].NQ, n)
+
     def close(self):
         if getattr(self, "h", None):
             lib().disco_batch_destroy(self.h)

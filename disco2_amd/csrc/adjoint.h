@@ -298,8 +298,10 @@ DHD __forceinline__ double visc_nu(const DiskParams &P, const DiskScalars &sc, d
   if (Qout) *Qout = Q;
   return P.visc * Pp / rho / sqrt(Q);
 }
+// angb (optional): adjoint of the angle whose cos / sin are ct / st
 DHD __forceinline__ void vjp_visc_nu(const DiskParams &P, const DiskScalars &sc, double r, double ct, double st,
-                                     double Pp, double rho, double nub, double *Ppb, double *rhob, ScalarAdj &sb) {
+                                     double Pp, double rho, double nub, double *Ppb, double *rhob, ScalarAdj &sb,
+                                     double *angb = nullptr) {
   if (P.visc_const == 1) {
     sb.alpha += nub;
     return;
@@ -319,9 +321,10 @@ DHD __forceinline__ void vjp_visc_nu(const DiskParams &P, const DiskScalars &sc,
     const double s = d * d + eps * eps;
     sb.M[p] += pow_m1p5(s) * Qb;
     const double sbar = sc.gm[p].M * (-1.5) * pow_m1p5(s) / s * Qb;
-    // s = d^2 + eps^2 with d^2 = dx^2 + dy^2
+    // s = d^2 + eps^2 with d^2 = dx^2 + dy^2, (x, y) = r (cos, sin)(angle)
     sb.xbh[p] += -2. * dx * sbar;
     sb.ybh[p] += -2. * dy * sbar;
+    if (angb) *angb += 2. * (dx * (-y) + dy * x) * sbar;
   }
 }
 
@@ -351,7 +354,8 @@ DHD __forceinline__ void visc_flux_v(const DiskParams &P, const DiskScalars &sc,
 }
 DHD __forceinline__ void vjp_visc_flux(const DiskParams &P, const DiskScalars &sc, int n0, double r, double cm,
                                        double rdr, const double *Avg, const double *Grr, const double *Gph,
-                                       const double *vb, double *Avgb, double *Grrb, double *Gphb, ScalarAdj &sb) {
+                                       const double *vb, double *Avgb, double *Grrb, double *Gphb, ScalarAdj &sb,
+                                       double *cmb) {
   double st, ct;
   sincos(cm, &st, &ct);
   const double nu = visc_nu(P, sc, r, ct, st, Avg[PPP], Avg[RHO], nullptr);
@@ -392,13 +396,13 @@ DHD __forceinline__ void vjp_visc_flux(const DiskParams &P, const DiskScalars &s
   Avgb[UPP] += omb;
   Avgb[RHO] += -nu * Kb;
   const double nub = -rho * Kb;
-  vjp_visc_nu(P, sc, r, ct, st, Avg[PPP], Avg[RHO], nub, &Avgb[PPP], &Avgb[RHO], sb);
+  vjp_visc_nu(P, sc, r, ct, st, Avg[PPP], Avg[RHO], nub, &Avgb[PPP], &Avgb[RHO], sb, cmb);
 }
 
 // Gravity of one mass at a cell, projected on r and phi (cell_source.c:49-71).
 DHD __forceinline__ void vjp_grav_force(double M, double xbh, double ybh, double epsn, int order, double r,
                                         double cosp, double sinp, double frb, double fpb, double *Mb, double *xbhb,
-                                        double *ybhb) {
+                                        double *ybhb, double *cospb, double *sinpb) {
   const double dx = r * cosp - xbh, dy = r * sinp - ybh;
   const double s = sqrt(dx * dx + dy * dy);
   const double cosa = dx / s, sina = dy / s;
@@ -429,6 +433,9 @@ DHD __forceinline__ void vjp_grav_force(double M, double xbh, double ybh, double
   dyb += dy / s * sb;
   *xbhb += -dxb;
   *ybhb += -dyb;
+  // cosp, sinp enter dx = r cosp - xbh, dy = r sinp - ybh and the projection
+  *cospb += r * dxb + cosa * cosapb + sina * sinapb;
+  *sinpb += r * dyb + sina * cosapb - cosa * sinapb;
 }
 
 // X = dt*dA*F and Y = dt*dA*(viscous SRR, LLL, TAU) of one face
@@ -438,7 +445,7 @@ DHD __forceinline__ void vjp_face_flux(const DiskParams &P, const DiskScalars &s
                                        int dir, double r, double dA, double w, double dt, double cm, double rdr,
                                        const double *Avg, const double *Grr, const double *Gph, const double *Xb,
                                        const double *Yb, double *pLb, double *pRb, double *Avgb, double *Grrb,
-                                       double *Gphb, ScalarAdj &sb) {
+                                       double *Gphb, ScalarAdj &sb, double *dAb = nullptr, double *cmb = nullptr) {
   const double dtdA = dt * dA;
   double Fb[5], F[5];
   for (int q = 0; q < 5; ++q) Fb[q] = dtdA * Xb[q];
@@ -452,9 +459,10 @@ DHD __forceinline__ void vjp_face_flux(const DiskParams &P, const DiskScalars &s
       vb[k] = dtdA * Yb[k];
       sdt += v[k] * Yb[k];
     }
-    vjp_visc_flux(P, sc, dir == 0, r, cm, rdr, Avg, Grr, Gph, vb, Avgb, Grrb, Gphb, sb);
+    vjp_visc_flux(P, sc, dir == 0, r, cm, rdr, Avg, Grr, Gph, vb, Avgb, Grrb, Gphb, sb, cmb);
   }
   sb.dt += dA * sdt;
+  if (dAb) *dAb += dt * sdt;
 }
 
 // VJP of add_geom_grav_src (physics.h) with the frame terms of
@@ -462,7 +470,7 @@ DHD __forceinline__ void vjp_face_flux(const DiskParams &P, const DiskScalars &s
 // an interior cell (wl = 0 elsewhere).  phi: cell-centre angle.
 DHD __forceinline__ void vjp_cell_sources(const DiskParams &P, const DiskScalars &sc, const double *pr, double r,
                                           double dV, double dt, double phi, double zc, const double *ub, double wl,
-                                          double *prb, ScalarAdj &sb) {
+                                          double *prb, ScalarAdj &sb, double *phib) {
   const double epsn = sc.epsn;
   double cphi, sphi;
   sincos(phi, &sphi, &cphi);
@@ -496,14 +504,16 @@ DHD __forceinline__ void vjp_cell_sources(const DiskParams &P, const DiskScalars
   prb[UZZ] += k * rho * Fr * cost * uT;
   const double Frb = k * rho * (sint * uS + cost * uZ + (sint * vr + vz * cost) * uT);
   const double Fpb = k * rho * (r * uL + vp * uT);
+  double cb = 0.0, sbb = 0.0;
   for (int p = 0; p < 2; ++p)
     vjp_grav_force(sc.gm[p].M, sc.xbh[p], sc.ybh[p], epsn, P.phi_order, gdist, cphi, sphi, Frb, Fpb, &sb.M[p],
-                   &sb.xbh[p], &sb.ybh[p]);
+                   &sb.xbh[p], &sb.ybh[p], &cb, &sbb);
+  *phib += -sphi * cb + cphi * sbb;
 }
 
 // VJP of cell_dt_value (physics.h) for constant or alpha-law viscosity.
 DHD __forceinline__ void vjp_cell_dt(const DiskParams &P, const DiskScalars &sc, const double *pr, double r, double dx,
-                                     double w, double tc, double db, double *prb, ScalarAdj &sb) {
+                                     double w, double tc, double db, double *prb, ScalarAdj &sb, double *tcb) {
   const double Pp = pr[PPP], rho = pr[RHO], vp = pr[UPP] * r - w, vr = pr[URR], vz = pr[UZZ];
   const double c = sqrt(P.gamma * Pp / rho);
   const double vm = sqrt(vr * vr + vp * vp + vz * vz);
@@ -520,7 +530,7 @@ DHD __forceinline__ void vjp_cell_dt(const DiskParams &P, const DiskScalars &sc,
     d0b = db / (den * den);
     const double tvb = db * d0 * d0 / (tv * tv * den * den);
     const double nub = -tvb * tv / nu;
-    vjp_visc_nu(P, sc, r, ct, st, Pp, rho, nub, &Ppb, &rhob, sb);
+    vjp_visc_nu(P, sc, r, ct, st, Pp, rho, nub, &Ppb, &rhob, sb, tcb);
   }
   const double ab = -d0 / a * d0b;
   if (c > 0.0) {
@@ -622,9 +632,11 @@ DHD __forceinline__ void rface_flux(const DiskParams &P, const DiskScalars &sc, 
   Y[0] = Y[1] = Y[2] = 0.0;
   d_face_flux<5>(primL, primR, 0, g.r, g.dA, 0.0, dt, P, sc, &vi, X, Y);
 }
+// gb accumulates the adjoints of the face area dA, centre cm and the cells'
+// edges tL, tR (the mesh geometry, which moves with dt)
 DHD __forceinline__ void vjp_rface_flux(const DiskParams &P, const DiskScalars &sc, const FacePair &c,
                                         const RFaceGeo &g, double dt, const double *Xb, const double *Yb,
-                                        FacePair &cb, ScalarAdj &sb) {
+                                        FacePair &cb, ScalarAdj &sb, RFaceGeo &gb) {
   const double dpL = wrap_pm(g.cm - (g.tL - .5 * g.dL));
   const double dpR = wrap_pm(-g.cm + (g.tR - .5 * g.dR));
   double primL[5], primR[5], WL[5], WR[5], Avg[5], Grr[2], Gph[2];
@@ -643,7 +655,7 @@ DHD __forceinline__ void vjp_rface_flux(const DiskParams &P, const DiskScalars &
   double pLb[5] = {0, 0, 0, 0, 0}, pRb[5] = {0, 0, 0, 0, 0}, Avgb[5] = {0, 0, 0, 0, 0};
   double Grrb[2] = {0, 0}, Gphb[2] = {0, 0};
   vjp_face_flux(P, sc, primL, primR, 0, g.r, g.dA, 0.0, dt, g.cm, g.rdr, Avg, Grr, Gph, Xb, Yb, pLb, pRb, Avgb, Grrb,
-                Gphb, sb);
+                Gphb, sb, &gb.dA, &gb.cm);
   double WLb[5], WRb[5];
   for (int q = 0; q < 5; ++q) {
     cb.pL[q] += pLb[q];
@@ -661,12 +673,19 @@ DHD __forceinline__ void vjp_rface_flux(const DiskParams &P, const DiskScalars &
     cb.gpL[URR + k] += .5 * Gphb[k] / g.r;
     cb.gpR[URR + k] += .5 * Gphb[k] / g.r;
   }
+  double dpLb = 0.0, dpRb = 0.0;
   for (int q = 0; q < 5; ++q) {
     cb.pL[q] += WLb[q];
     cb.gpL[q] += dpL * WLb[q];
     cb.pR[q] += WRb[q];
     cb.gpR[q] -= dpR * WRb[q];
+    dpLb += (pLb[q] + WLb[q]) * c.gpL[q];
+    dpRb -= (pRb[q] + WRb[q]) * c.gpR[q];
   }
+  // dpL = cm - (tL - dL/2), dpR = (tR - dR/2) - cm
+  gb.cm += dpLb - dpRb;
+  gb.tL -= dpLb;
+  gb.tR += dpRb;
 }
 
 // phi face (j, j+1): riemann_setup_p (Riemann_setup.c:46-80) + the flux, as
@@ -697,9 +716,10 @@ DHD __forceinline__ void phiface_flux(const DiskParams &P, const DiskScalars &sc
   Y[0] = Y[1] = Y[2] = 0.0;
   d_face_flux<5>(primL, primR, 1, g.r, g.dA, g.w, dt, P, sc, &vi, X, Y);
 }
+// *tLb: adjoint of the face's angle (the viscosity is evaluated at cm = tL)
 DHD __forceinline__ void vjp_phiface_flux(const DiskParams &P, const DiskScalars &sc, const FacePair &c,
                                           const PhiFaceGeo &g, double dt, const double *Xb, const double *Yb,
-                                          FacePair &cb, ScalarAdj &sb) {
+                                          FacePair &cb, ScalarAdj &sb, double *tLb) {
   double primL[5], primR[5], Avg[5], Grr[2], Gph[2];
   for (int q = 0; q < 5; ++q) {
     primL[q] = c.pL[q] + 0.5 * c.gpL[q] * g.dL;
@@ -714,7 +734,7 @@ DHD __forceinline__ void vjp_phiface_flux(const DiskParams &P, const DiskScalars
   double pLb[5] = {0, 0, 0, 0, 0}, pRb[5] = {0, 0, 0, 0, 0}, Avgb[5] = {0, 0, 0, 0, 0};
   double Grrb[2] = {0, 0}, Gphb[2] = {0, 0};
   vjp_face_flux(P, sc, primL, primR, 1, g.r, g.dA, g.w, dt, g.tL, g.rdr, Avg, Grr, Gph, Xb, Yb, pLb, pRb, Avgb,
-                Grrb, Gphb, sb);
+                Grrb, Gphb, sb, nullptr, tLb);
   for (int q = 0; q < 5; ++q) {
     cb.pL[q] += pLb[q] + .5 * Avgb[q];
     cb.gpL[q] += .5 * g.dL * pLb[q];

@@ -1,0 +1,532 @@
+// Backward kernels of one substep for the hand-written adjoint (2D, NUM_Q 5,
+// non-rotating analytic frame, state-independent mesh motion).
+//
+// The forward substep is re-run first (with k_update recording the conserved
+// state it converts), so the faces, the phi and r slopes and the state are
+// in place.  Then, in reverse: the outflow remap, the cell update (cons2prim,
+// RK blend, prim2cons, sources, the torque of the loss), the phi faces, the
+// r faces, the r limiter, the face slopes, the phi limiter.  Cells receive
+// the contributions of their faces through atomic adds (the adjoint is not
+// bitwise reproducible run to run; it does not need to be).  Launch-uniform
+// scalars (dt, alpha, the masses and their positions) are summed per tile
+// and reduced per substep.
+#include "adjoint.h"
+#include "adjoint_kernels.h"
+#include "device_common.h"
+
+__device__ __forceinline__ void load5(const double *a, long long N, long long c, double *v) {
+#pragma unroll
+  for (int q = 0; q < 5; ++q) v[q] = a[q * N + c];
+}
+__device__ __forceinline__ void store5(double *a, long long N, long long c, const double *v) {
+#pragma unroll
+  for (int q = 0; q < 5; ++q) a[q * N + c] = v[q];
+}
+__device__ __forceinline__ void add5(double *a, long long N, long long c, const double *v) {
+#pragma unroll
+  for (int q = 0; q < 5; ++q)
+    if (v[q] != 0.0) atomicAdd(&a[q * N + c], v[q]);
+}
+
+// workgroup sum of the scalar adjoints into one slot (every thread calls)
+__device__ void reduce_store(const ScalarAdj &s, ScalarAdj *out) {
+  __shared__ double sh[9][TILE / 64];
+  double v[9] = {s.dt, s.alpha, s.M[0], s.M[1], s.xbh[0], s.xbh[1], s.ybh[0], s.ybh[1], s.loss};
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sh[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[9];
+    for (int k = 0; k < 9; ++k) {
+      t[k] = 0.0;
+      for (int i = 0; i < nw; ++i) t[k] += sh[k][i];
+    }
+    out->dt = t[0];
+    out->alpha = t[1];
+    out->M[0] = t[2];
+    out->M[1] = t[3];
+    out->xbh[0] = t[4];
+    out->xbh[1] = t[5];
+    out->ybh[0] = t[6];
+    out->ybh[1] = t[7];
+    out->loss = t[8];
+  }
+}
+
+__device__ __forceinline__ void load_pair(const DevMesh &m, long long cL, long long cR, FacePair &c) {
+  load5(m.prim, m.Ntot, cL, c.pL);
+  load5(m.prim, m.Ntot, cR, c.pR);
+  load5(m.grad, m.Ntot, cL, c.gL);
+  load5(m.grad, m.Ntot, cR, c.gR);
+  load5(m.gradp, m.Ntot, cL, c.gpL);
+  load5(m.gradp, m.Ntot, cR, c.gpR);
+}
+__device__ __forceinline__ void scatter_pair(const DevMesh &m, const AdjMesh &a, long long cL, long long cR,
+                                             const FacePair &cb) {
+  add5(a.aP, m.Ntot, cL, cb.pL);
+  add5(a.aP, m.Ntot, cR, cb.pR);
+  add5(a.agrad, m.Ntot, cL, cb.gL);
+  add5(a.agrad, m.Ntot, cR, cb.gR);
+  add5(a.agradp, m.Ntot, cL, cb.gpL);
+  add5(a.agradp, m.Ntot, cR, cb.gpR);
+}
+
+// A face's width and centre follow the rotations of the annuli whose edges
+// bound it (face_create_destroy.c:74-125): d(width)/d(theta_in, theta_out)
+// and d(centre)/d(theta_in, theta_out) by face type.  The adjoints of a face's
+// width (wb), centre (cb) and of its cells' own edges (tLb, tRb) go to the
+// rotation adjoint of the inner (L cell) and outer (R cell) annulus.
+__device__ __forceinline__ void face_geo_to_rot(const DevMesh &m, const AdjMesh &a, long long g, long long cL,
+                                                long long cR, double wb, double cb, double tLb, double tRb) {
+  double w_in = 0.0, w_out = 0.0, c_in = 0.0, c_out = 0.0;
+  switch (m.ftype[g]) {
+    case FACE_II: c_in = 1.0; break;
+    case FACE_IO: w_in = -1.0; w_out = 1.0; c_in = c_out = 0.5; break;
+    case FACE_OO: c_out = 1.0; break;
+    default: w_in = 1.0; w_out = -1.0; c_in = c_out = 0.5; break;  // FACE_OI
+  }
+  const double vin = wb * w_in + cb * c_in + tLb, vout = wb * w_out + cb * c_out + tRb;
+  if (vin != 0.0) atomicAdd(&a.arot[cL], vin);
+  if (vout != 0.0) atomicAdd(&a.arot[cR], vout);
+}
+
+// Outflow remap backward (k_bc_r): a ghost cell's adjoint goes to the cells
+// it was averaged from, weighted as the forward weighted their primitives.
+// The mesh view has the substep's faces and the new dphi.
+__global__ __launch_bounds__(TILE) void ka_bc(DevMesh m, AdjMesh a, const int2 *tiles) {
+  const int2 tl = tiles[xcd_order()];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const DiskParams &P = disk_of(m, A.disk);
+  const long long c = A.zbase + j;
+  const int k = A.k, i = A.i;
+  double lam[5];
+  load5(a.aPn, m.Ntot, c, lam);
+  const double dAc = (A.zp - A.zm) * (i == 0 ? A.rp : A.rm) * m.dphi_new[c];
+  // ghost = sum over faces of prim[src] * dA_f / dAc, dA_f = r * width * dperp
+  if (i == 0) {
+    faces_as_L(m, rpair_of(P, k, 0), m.fo_r, c, j, A.N, [&](const PairInfo &PI, long long g) {
+      const long long s = m.fR[g];
+      const double w = PI.r * m.fdphi[g] * PI.deltaPerp / dAc;
+      double v[5], wb = 0.0;
+      for (int q = 0; q < 5; ++q) {
+        v[q] = lam[q] * w;
+        wb += lam[q] * m.prim_new[q * m.Ntot + s];
+      }
+      add5(a.aPn, m.Ntot, s, v);
+      face_geo_to_rot(m, a, g, c, s, wb * PI.r * PI.deltaPerp / dAc, 0.0, 0.0, 0.0);
+    });
+  } else {
+    faces_as_R(m, rpair_of(P, k, i - 1), j, A.N, [&](const PairInfo &PI, long long g) {
+      const long long s = m.fL[g];
+      const double w = PI.r * m.fdphi[g] * PI.deltaPerp / dAc;
+      double v[5], wb = 0.0;
+      for (int q = 0; q < 5; ++q) {
+        v[q] = lam[q] * w;
+        wb += lam[q] * m.prim_new[q * m.Ntot + s];
+      }
+      add5(a.aPn, m.Ntot, s, v);
+      face_geo_to_rot(m, a, g, s, c, wb * PI.r * PI.deltaPerp / dAc, 0.0, 0.0, 0.0);
+    });
+  }
+  const double z[5] = {0, 0, 0, 0, 0};
+  store5(a.aPn, m.Ntot, c, z);
+}
+
+// Cell update backward (k_update phase 2): cons2prim, the RK blend and the
+// RK snapshot, prim2cons, the sources and the torque term.  Writes the
+// adjoint of the cell's conserved state (au) and starts aP / agrad / agradp.
+__global__ __launch_bounds__(TILE) void ka_update(DevMesh m, AdjMesh a, const int2 *tiles) {
+  const int ti = xcd_order();
+  const int2 tl = tiles[ti];
+  const AnnInfo A = m.ann[tl.x];
+  const DiskParams &P = disk_of(m, A.disk);
+  const DiskScalars &sc = m.sc[A.disk];
+  const int j = tl.y + threadIdx.x;
+  ScalarAdj sb = {};
+  if (j < A.N) {
+    const long long c = A.zbase + j;
+    const double rp = A.rp, rm = A.rm, zp = A.zp, zm = A.zm;
+    const double r = .5 * (rp + rm), dz = zp - zm;
+    const double dphi = m.dphi[c], dphin = m.dphi_new[c];
+    double pr[5], u[5], ob[5], ub[5] = {0, 0, 0, 0, 0}, prb[5] = {0, 0, 0, 0, 0}, u0b[5];
+    load5(m.prim, m.Ntot, c, pr);
+    load5(m.u_save, m.Ntot, c, u);
+    load5(a.aPn, m.Ntot, c, ob);
+    vjp_cons2prim(u, r, .5 * (rp * rp - rm * rm) * dphin * dz, P, ob, ub);
+    store5(a.au, m.Ntot, c, ub);
+    for (int q = 0; q < 5; ++q) {
+      if (a.copy_first) {
+        u0b[q] = ub[q] + a.aR[q * m.Ntot + c];  // u = u0 (RK = 0) and RKcons = u0
+      } else {
+        u0b[q] = (1.0 - a.RK) * ub[q];
+        a.aR[q * m.Ntot + c] = a.RK * ub[q];
+      }
+    }
+    vjp_prim2cons(pr, r, .5 * (rp * rp - rm * rm) * dphi * dz, P.gamma, u0b, prb);
+    const double phi = clean_pi(m.tiph[c]) - .5 * dphi;
+    const double dVs = dphi * .5 * (rp * rp - rm * rm) * dz;
+    const double wl = interior_cell(P, A.i, A.k) ? a.wl : 0.0;
+    double phib = 0.0;
+    vjp_cell_sources(P, sc, pr, r, dVs, a.fac * sc.dt, phi, .5 * (zp + zm), ub, wl, prb, sb, &phib);
+    if (phib != 0.0) atomicAdd(&a.arot[c], phib);
+    store5(a.aP, m.Ntot, c, prb);
+    const double z[5] = {0, 0, 0, 0, 0};
+    store5(a.agrad, m.Ntot, c, z);
+    store5(a.agradp, m.Ntot, c, z);
+  }
+  reduce_store(sb, &a.part[ti]);
+}
+
+// phi faces backward: face (j, j+1), with cell j+1 receiving +X and j -X.
+__global__ __launch_bounds__(TILE) void ka_phi(DevMesh m, AdjMesh a, const int2 *tiles, int slot0) {
+  const int ti = xcd_order();
+  const int2 tl = tiles[ti];
+  const AnnInfo A = m.ann[tl.x];
+  const DiskParams &P = disk_of(m, A.disk);
+  const DiskScalars &sc = m.sc[A.disk];
+  const int j = tl.y + threadIdx.x;
+  ScalarAdj sb = {};
+  if (j < A.N) {
+    const long long cL = A.zbase + j, cR = A.zbase + (j == A.N - 1 ? 0 : j + 1);
+    const double r = .5 * (A.rp + A.rm);
+    double Xb[5], Yb[3];
+    for (int q = 0; q < 5; ++q) Xb[q] = a.au[q * m.Ntot + cR] - a.au[q * m.Ntot + cL];
+    Yb[0] = a.au[SRR * m.Ntot + cR] / r - a.au[SRR * m.Ntot + cL] / r;
+    Yb[1] = a.au[LLL * m.Ntot + cR] - a.au[LLL * m.Ntot + cL];
+    Yb[2] = a.au[TAU * m.Ntot + cR] - a.au[TAU * m.Ntot + cL];
+    FacePair c, cb = {};
+    load_pair(m, cL, cR, c);
+    const double4 rot = ann_rotation(m, P, tl.x);
+    PhiFaceGeo g;
+    g.r = r;
+    g.dA = (A.rp - A.rm) * (A.zp - A.zm);
+    g.w = m.wiph[tl.x];
+    g.tL = clean_pi(m.tiph[cL]);
+    g.tR = clean_pi(m.tiph[cR]);
+    g.dL = m.dphi[cL];
+    g.dR = m.dphi[cR];
+    g.rdr = rot.y;
+    double tLb = 0.0;
+    vjp_phiface_flux(P, sc, c, g, a.fac * sc.dt, Xb, Yb, cb, sb, &tLb);
+    scatter_pair(m, a, cL, cR, cb);
+    if (tLb != 0.0) atomicAdd(&a.arot[cL], tLb);
+  }
+  reduce_store(sb, &a.part[slot0 + ti]);
+}
+
+// r faces backward: the R cell received +X (+Y/r_R), the L cell -X.
+__global__ __launch_bounds__(TILE) void ka_rflux(DevMesh m, AdjMesh a, const int2 *ftiles, int slot0) {
+  const int ti = xcd_order();
+  const int2 tl = ftiles[ti];
+  const int p = tl.x;
+  const int f = tl.y + threadIdx.x;
+  ScalarAdj sb = {};
+  if (f < m.pst[p].count) {
+    const PairInfo PI = m.pairs[p];
+    const DiskParams &P = disk_of(m, PI.disk);
+    const DiskScalars &sc = m.sc[PI.disk];
+    const long long g = PI.face_base + f;
+    const long long cL = m.fL[g], cR = m.fR[g];
+    double Xb[5], Yb[3];
+    for (int q = 0; q < 5; ++q) Xb[q] = a.au[q * m.Ntot + cR] - a.au[q * m.Ntot + cL];
+    Yb[0] = a.au[SRR * m.Ntot + cR] / (PI.r + PI.deltaR) - a.au[SRR * m.Ntot + cL] / (PI.r - PI.deltaL);
+    Yb[1] = a.au[LLL * m.Ntot + cR] - a.au[LLL * m.Ntot + cL];
+    Yb[2] = a.au[TAU * m.Ntot + cR] - a.au[TAU * m.Ntot + cL];
+    FacePair c, cb = {};
+    load_pair(m, cL, cR, c);
+    const double4 rot = ann_rotation(m, P, PI.ann_in);
+    RFaceGeo geo;
+    geo.r = PI.r;
+    geo.deltaL = PI.deltaL;
+    geo.deltaR = PI.deltaR;
+    geo.dA = PI.r * m.fdphi[g] * PI.deltaPerp;
+    geo.cm = m.fcm[g];
+    geo.tL = clean_pi(m.tiph[cL]);
+    geo.dL = m.dphi[cL];
+    geo.tR = clean_pi(m.tiph[cR]);
+    geo.dR = m.dphi[cR];
+    geo.rdr = rot.w;
+    RFaceGeo gb = {};
+    vjp_rface_flux(P, sc, c, geo, a.fac * sc.dt, Xb, Yb, cb, sb, gb);
+    scatter_pair(m, a, cL, cR, cb);
+    face_geo_to_rot(m, a, g, cL, cR, gb.dA * PI.r * PI.deltaPerp, gb.cm, gb.tL, gb.tR);
+  }
+  reduce_store(sb, &a.part[slot0 + ti]);
+}
+
+// r slope backward (k_plm_rz, r direction): replay the sum and the
+// sequential limiter, then hand each face its share of the slope's adjoint:
+// dA/dAtot to every face when the sum survived, PLM to the face whose S the
+// limiter took last, nothing when it was zeroed.  Each face gets one value
+// from its L cell (aSL) and one from its R cell (aSR).
+__global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int2 *tiles) {
+  const int2 tl = tiles[xcd_order()];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const DiskParams &P = disk_of(m, A.disk);
+  const long long z = A.zbase + j;
+  const int k = A.k, i = A.i;
+  const int pR = i >= 1 ? rpair_of(P, k, i - 1) : -1;
+  const int pLf = i <= P.NR - 2 ? rpair_of(P, k, i) : -1;
+  double g[5] = {0, 0, 0, 0, 0};
+  auto acc = [&](const PairInfo &PI, long long gi) {
+    double S[5];
+    face_slope<5>(m, PI, gi, S);
+    const double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
+    for (int q = 0; q < 5; ++q) g[q] += S[q] * dA;
+  };
+  if (pR >= 0) faces_as_R(m, pR, j, A.N, acc);
+  if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, acc);
+  const double dz = A.zp - A.zm, dp = m.dphi[z];
+  double dAtot;
+  if (i == 0)
+    dAtot = dz * A.rp * dp;
+  else if (i == P.NR - 1)
+    dAtot = dz * A.rm * dp;
+  else
+    dAtot = dz * (A.rp + A.rm) * dp;
+  for (int q = 0; q < 5; ++q) g[q] /= dAtot;
+  const double PLM = P.plm;
+  int src[5] = {-1, -1, -1, -1, -1};
+  int n = 0;
+  auto lim = [&](const PairInfo &PI, long long gi) {
+    double S[5];
+    face_slope<5>(m, PI, gi, S);
+    for (int q = 0; q < 5; ++q) {
+      if (S[q] * g[q] < 0.0) {
+        g[q] = 0.0;
+        src[q] = -2;
+      } else if (fabs(PLM * S[q]) < fabs(g[q])) {
+        g[q] = PLM * S[q];
+        src[q] = n;
+      }
+    }
+    ++n;
+  };
+  if (pR >= 0) faces_as_R(m, pR, j, A.N, lim);
+  if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, lim);
+  double gb[5];
+  load5(a.agrad, m.Ntot, z, gb);
+  n = 0;
+  auto put = [&](const PairInfo &PI, long long gi, double *dst) {
+    const double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
+    double S[5], wb = 0.0;
+    face_slope<5>(m, PI, gi, S);
+    for (int q = 0; q < 5; ++q) {
+      dst[q * m.NFtot + gi] = src[q] == -1 ? dA / dAtot * gb[q] : (src[q] == n ? PLM * gb[q] : 0.0);
+      if (src[q] == -1) wb += gb[q] * S[q] / dAtot;  // d g / d dA_f
+    }
+    if (wb != 0.0) face_geo_to_rot(m, a, gi, m.fL[gi], m.fR[gi], wb * PI.r * PI.deltaPerp, 0.0, 0.0, 0.0);
+    ++n;
+  };
+  if (pR >= 0) faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) { put(PI, gi, a.aSR); });
+  if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) { put(PI, gi, a.aSL); });
+}
+
+// face slope S = (W_R - W_L)/(deltaL + deltaR) backward
+__global__ __launch_bounds__(TILE) void ka_slope(DevMesh m, AdjMesh a, const int2 *ftiles) {
+  const int2 tl = ftiles[xcd_order()];
+  const int p = tl.x;
+  const int f = tl.y + threadIdx.x;
+  if (f >= m.pst[p].count) return;
+  const PairInfo PI = m.pairs[p];
+  const long long g = PI.face_base + f;
+  const long long cL = m.fL[g], cR = m.fR[g];
+  const double cm = m.fcm[g];
+  const double dpL = wrap_pm(cm - (clean_pi(m.tiph[cL]) - .5 * m.dphi[cL]));
+  const double dpR = wrap_pm((clean_pi(m.tiph[cR]) - .5 * m.dphi[cR]) - cm);
+  const double d = PI.deltaR + PI.deltaL;
+  double pLb[5], pRb[5], gpLb[5], gpRb[5], dpLb = 0.0, dpRb = 0.0;
+  for (int q = 0; q < 5; ++q) {
+    const double Sb = a.aSL[q * m.NFtot + g] + a.aSR[q * m.NFtot + g];
+    const double WRb = Sb / d, WLb = -Sb / d;
+    pLb[q] = WLb;
+    gpLb[q] = dpL * WLb;
+    pRb[q] = WRb;
+    gpRb[q] = -dpR * WRb;
+    dpLb += WLb * m.gradp[q * m.Ntot + cL];
+    dpRb -= WRb * m.gradp[q * m.Ntot + cR];
+  }
+  // dpL = cm - (tL - dL/2), dpR = (tR - dR/2) - cm
+  face_geo_to_rot(m, a, g, cL, cR, 0.0, dpLb - dpRb, -dpLb, dpRb);
+  add5(a.aP, m.Ntot, cL, pLb);
+  add5(a.agradp, m.Ntot, cL, gpLb);
+  add5(a.aP, m.Ntot, cR, pRb);
+  add5(a.agradp, m.Ntot, cR, gpRb);
+}
+
+// phi slope backward (k_plm_p)
+__global__ __launch_bounds__(TILE) void ka_plm_p(DevMesh m, AdjMesh a, const int2 *tiles) {
+  const int2 tl = tiles[xcd_order()];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const DiskParams &P = disk_of(m, A.disk);
+  const int N = A.N;
+  const long long c = A.zbase + j;
+  const long long cL = A.zbase + (j == 0 ? N - 1 : j - 1);
+  const long long cR = A.zbase + (j == N - 1 ? 0 : j + 1);
+  const double dpL = .5 * (m.dphi[c] + m.dphi[cL]);
+  const double dpR = .5 * (m.dphi[c] + m.dphi[cR]);
+  double bL[5], bc[5], bR[5];
+  for (int q = 0; q < 5; ++q) {
+    const double gb = a.agradp[q * m.Ntot + c];
+    double wL, wc, wR;
+    plm_p_weights(m.prim[q * m.Ntot + cL], m.prim[q * m.Ntot + c], m.prim[q * m.Ntot + cR], dpL, dpR, P.plm, &wL, &wc,
+                  &wR);
+    bL[q] = wL * gb;
+    bc[q] = wc * gb;
+    bR[q] = wR * gb;
+  }
+  add5(a.aP, m.Ntot, cL, bL);
+  add5(a.aP, m.Ntot, c, bc);
+  add5(a.aP, m.Ntot, cR, bR);
+}
+
+// CFL step backward.  The step's dt is the minimum of the interior cells'
+// steps; on a smooth disk whole annuli can tie bit for bit, and the minimum
+// is then the step of any one of them: the first tied cell (lowest zone)
+// takes lam_dt times its gradient.  ka_dt_pick finds it, ka_dt applies it.
+// m.sc holds the scalars the forward k_mindt saw.
+__device__ __forceinline__ double cfl_cell(const DevMesh &m, const int2 tl, const AnnInfo &A, const DiskParams &P,
+                                           const DiskScalars &sc, long long c, double *pr, double *r_out,
+                                           double *dx_out, double *w_out, double *tc_out) {
+  const double rm = A.rm, rp = A.rp, dz = A.zp - A.zm, dr = rp - rm, r = .5 * (rp + rm);
+  const double wj = m.wiph[tl.x];
+  const double w = .5 * (wj + wj);
+  double dx = dr;
+  const double rdphi = .5 * (rp + rm) * m.dphi[c];
+  if (rdphi < dr) dx = rdphi;
+  if (dx > dz) dx = dz;
+  load5(m.prim, m.Ntot, c, pr);
+  const double tc = (P.visc > 0.0 && P.visc_const != 1) ? clean_pi(m.tiph[c]) - 0.5 * m.dphi[c] : 0.0;
+  *r_out = r;
+  *dx_out = dx;
+  *w_out = w;
+  *tc_out = tc;
+  return cell_dt_value(P, sc, pr, r, dx, w, tc);
+}
+__global__ __launch_bounds__(TILE) void ka_dt_pick(DevMesh m, const int2 *tiles, unsigned long long dtbits,
+                                                   int *pick) {
+  const int2 tl = tiles[xcd_order()];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const long long c = A.zbase + j;
+  double pr[5], r, dx, w, tc;
+  const double d = cfl_cell(m, tl, A, disk_of(m, A.disk), m.sc[A.disk], c, pr, &r, &dx, &w, &tc);
+  if ((unsigned long long)__double_as_longlong(d) == dtbits) atomicMin(pick, (int)c);
+}
+__global__ __launch_bounds__(TILE) void ka_dt(DevMesh m, AdjMesh a, const int2 *tiles, const int *pick,
+                                              int slot0) {
+  const int ti = xcd_order();
+  const int2 tl = tiles[ti];
+  const AnnInfo A = m.ann[tl.x];
+  const DiskParams &P = disk_of(m, A.disk);
+  const DiskScalars &sc = m.sc[A.disk];
+  const int j = tl.y + threadIdx.x;
+  ScalarAdj sb = {};
+  if (j < A.N && (long long)*pick == A.zbase + j) {
+    const long long c = A.zbase + j;
+    double pr[5], r, dx, w, tc;
+    cfl_cell(m, tl, A, P, sc, c, pr, &r, &dx, &w, &tc);
+    {
+      double prb[5] = {0, 0, 0, 0, 0};
+      double tcb = 0.0;
+      vjp_cell_dt(P, sc, pr, r, dx, w, tc, a.lam_dt, prb, sb, &tcb);
+      add5(a.aP, m.Ntot, c, prb);
+      if (tcb != 0.0) atomicAdd(&a.arot[c], tcb);
+    }
+  }
+  reduce_store(sb, &a.part[slot0 + ti]);
+}
+
+// sum of n tile partials into *out (one workgroup)
+__global__ __launch_bounds__(TILE) void ka_reduce(const ScalarAdj *part, int n, ScalarAdj *out) {
+  ScalarAdj s = {};
+  for (int i = threadIdx.x; i < n; i += TILE) {
+    const ScalarAdj &p = part[i];
+    s.dt += p.dt;
+    s.alpha += p.alpha;
+    s.loss += p.loss;
+    for (int k = 0; k < 2; ++k) {
+      s.M[k] += p.M[k];
+      s.xbh[k] += p.xbh[k];
+      s.ybh[k] += p.ybh[k];
+    }
+  }
+  reduce_store(s, out);
+}
+
+// sum of the tiles' torque into *acc (one workgroup; acc accumulates)
+__global__ __launch_bounds__(TILE) void ka_sum(const double *v, int n, double *acc) {
+  __shared__ double sh[TILE / 64];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += TILE) s += v[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < TILE / 64; ++w) t += sh[w];
+    *acc += t;
+  }
+}
+
+// per annulus sum of the cells' rotation adjoints (one workgroup per annulus)
+__global__ __launch_bounds__(TILE) void ka_rot(DevMesh m, AdjMesh a, double *out) {
+  __shared__ double sh[TILE / 64];
+  const AnnInfo A = m.ann[blockIdx.x];
+  double s = 0.0;
+  for (int j = threadIdx.x; j < A.N; j += TILE) s += a.arot[A.zbase + j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < TILE / 64; ++w) t += sh[w];
+    out[blockIdx.x] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
+                            ScalarAdj *out) {
+  const dim3 B(TILE);
+  if (L.n_bc1) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc1), B, 0, st, m, a, L.bc1);
+  if (L.n_bc0) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc0), B, 0, st, m, a, L.bc0);
+  hipLaunchKernelGGL(ka_update, dim3(L.n_ztiles), B, 0, st, m, a, L.ztiles);
+  hipLaunchKernelGGL(ka_phi, dim3(L.n_ztiles), B, 0, st, m, a, L.ztiles, L.n_ztiles);
+  if (L.n_rftiles) hipLaunchKernelGGL(ka_rflux, dim3(L.n_rftiles), B, 0, st, m, a, L.rftiles, 2 * L.n_ztiles);
+  hipLaunchKernelGGL(ka_plm_r, dim3(L.n_ztiles), B, 0, st, m, a, L.ztiles);
+  if (L.n_rftiles) hipLaunchKernelGGL(ka_slope, dim3(L.n_rftiles), B, 0, st, m, a, L.rftiles);
+  hipLaunchKernelGGL(ka_plm_p, dim3(L.n_ztiles), B, 0, st, m, a, L.ztiles);
+  hipLaunchKernelGGL(ka_reduce, dim3(1), B, 0, st, a.part, 2 * L.n_ztiles + L.n_rftiles, out);
+}
+
+void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
+                       unsigned long long dtbits, int *pick, ScalarAdj *out) {
+  if (!L.n_itiles) return;
+  hipMemsetAsync(pick, 0x7f, sizeof(int), st);  // "no cell yet": a large positive index
+  hipLaunchKernelGGL(ka_dt_pick, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles, dtbits, pick);
+  hipLaunchKernelGGL(ka_dt, dim3(L.n_itiles), dim3(TILE), 0, st, m, a, L.itiles, (const int *)pick, 0);
+  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(TILE), 0, st, a.part, L.n_itiles, out);
+}
+
+void launch_torque_sum(const double *tiles, int n, double *acc, hipStream_t st) {
+  hipLaunchKernelGGL(ka_sum, dim3(1), dim3(TILE), 0, st, tiles, n, acc);
+}
+
+void launch_adjoint_rot(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st, double *out) {
+  hipLaunchKernelGGL(ka_rot, dim3(L.nann), dim3(TILE), 0, st, m, a, out);
+}

@@ -479,6 +479,11 @@ void disco_batch_synchronize(DiscoBatch *b) {
   b->e->check_error();
 }
 double disco_batch_last_ms(DiscoBatch *b) { return b->e->last_ms; }
+int disco_batch_torque_loss(DiscoBatch *b, int nsteps, double *loss) { return b->e->torque_loss(nsteps, loss); }
+int disco_batch_adjoint(DiscoBatch *b, int nsteps, int every, double *loss, double *g_alpha, double *g_q,
+                        double *g_prim0) {
+  return b->e->adjoint(nsteps, every, loss, g_alpha, g_q, g_prim0);
+}
 
 // Eager steps with every kernel launch bracketed by HIP events on the
 // batch's stream; per kernel name: total ms and launch count.

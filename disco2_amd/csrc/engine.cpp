@@ -325,8 +325,9 @@ DevMesh Engine::mesh() const {
   m.err = d_err;
   m.dt_tile = d_dt_tile;
   m.ann_rot = d_ann_rot;
-  m.u_save = nullptr;
-  m.torque_tile = nullptr;
+  m.u_save = adj_usave;
+  m.torque_tile = adj_torque;
+  m.ftype = adj_ftype;
   return m;
 }
 

@@ -8,6 +8,12 @@
 #include "disco_internal.h"
 #include "kernels.h"
 
+// scalars of one RK2 step the adjoint records: as k_mindt saw them, at the
+// start of substep 1 (after the step's scalar ops) and of substep 2
+struct AdjStepRec {
+  DiskScalars pre, A, B;
+};
+
 struct Engine {
   int ndisk = 0, NQ = 5;
   std::vector<Sim *> sims;
@@ -37,6 +43,8 @@ struct Engine {
   double4 *d_ann_rot = nullptr;
   int2 *d_tiles = nullptr;
   int cur = 0;  // index of the prim / tiph / dphi buffers holding the current state
+  double *adj_usave = nullptr, *adj_torque = nullptr;  // adjoint recording (mesh() passes them on)
+  signed char *adj_ftype = nullptr;
 
   LaunchPlan plan{};
   hipStream_t st = nullptr;
@@ -61,6 +69,14 @@ struct Engine {
   void run(int nsteps, bool use_graph);
   void synchronize();
   int check_error();                 // 0, or sets the sticky library error
+
+  // adjoint (adjoint_host.cpp)
+  DevMesh mesh_prev() const;         // after a substep: its input as prim/tiph/dphi, output as *_new
+  bool adjoint_scope();
+  void adjoint_step(AdjStepRec *rec, double *snapA, double *snapB, double *torque, double *acc);
+  void adjoint_restore(const double *src, bool rk);
+  int torque_loss(int nsteps, double *loss);
+  int adjoint(int nsteps, int every, double *loss, double *g_alpha, double *g_q, double *g_prim0);
 };
 
 bool hip_check(hipError_t e, const char *what);

@@ -28,9 +28,14 @@ struct DevMesh {
   // adjoint recording (null in plain runs): k_update stores the conserved
   // state it converts (u_save, NQ x Ntot) and its tiles' torque sums
   double *u_save, *torque_tile;
+  signed char *ftype;     // the face builder records each face's bounding edges here
   long long Ntot, NFtot;
   int *err;
 };
+
+// which edges bound an overlap face, left then right (I: inner annulus,
+// O: outer): the face's width and centre move with the two annuli's rotations
+enum { FACE_II = 0, FACE_IO = 1, FACE_OO = 2, FACE_OI = 3 };
 
 // LDS the face builder may use to stage an outer annulus (two doubles per
 // cell), below the 160 KiB of a gfx950 workgroup minus its static arrays

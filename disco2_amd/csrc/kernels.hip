@@ -28,6 +28,7 @@
 // ----------------------------------------------------------------------------
 struct FaceOut {
   int *fL, *fR, *run_start;
+  signed char *ftype;  // optional: which edges bound the face (FACE_II, _IO, _OO, _OI)
   double *fdp, *fcm;
   long long fbase;
   int rbase, cap, run_cap;
@@ -43,7 +44,7 @@ __device__ int jloop_cell(int j, int jps, const double *Tin, const double *Din, 
   if (jm < 0) jm = Nin - 1;
   int jp = jps;
   int n = 0;
-  auto put = [&](int R, double dph, double tp) {
+  auto put = [&](int R, double dph, double tp, int type) {
     if (EMIT) {
       int f = foff + n;
       if (f < o.cap) {
@@ -52,6 +53,7 @@ __device__ int jloop_cell(int j, int jps, const double *Tin, const double *Din, 
         o.fR[g] = (int)(o.zout + R);
         o.fdp[g] = dph;
         o.fcm[g] = tp - .5 * dph;  // addFace, :11-20
+        if (o.ftype) o.ftype[g] = (signed char)type;
       } else {
         set_err(o.err, DERR_FACE_OVERFLOW);
       }
@@ -67,9 +69,9 @@ __device__ int jloop_cell(int j, int jps, const double *Tin, const double *Din, 
   };
   double dp = wrap_p(Tout[jp] - Tin[jm]);
   if (Din[j] < dp) {
-    put(jp, Din[j], Tin[j]);
+    put(jp, Din[j], Tin[j], FACE_II);
   } else {
-    put(jp, dp, Tout[jp]);  // step A
+    put(jp, dp, Tout[jp], FACE_IO);  // step A
     ++jp;
     if (jp == Nout) jp = 0;
     dp = wrap_pm(Tin[j] - Tout[jp]);
@@ -79,13 +81,13 @@ __device__ int jloop_cell(int j, int jps, const double *Tin, const double *Din, 
         *bad = 1;
         break;
       }
-      put(jp, Dout[jp], Tout[jp]);
+      put(jp, Dout[jp], Tout[jp], FACE_OO);
       ++jp;
       if (jp == Nout) jp = 0;
       dp = wrap_pm(Tin[j] - Tout[jp]);
     }
     dp = Dout[jp] + dp;
-    put(jp, dp, Tin[j]);  // step C
+    put(jp, dp, Tin[j], FACE_OI);  // step C
   }
   *jpe = jp;
   return n;
@@ -190,6 +192,7 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
   FaceOut o;
   o.fL = m.fL;
   o.fR = m.fR;
+  o.ftype = m.ftype;
   o.fdp = m.fdphi;
   o.fcm = m.fcm;
   o.run_start = m.run_start;

@@ -47,7 +47,7 @@ DHD __forceinline__ double wrap_p(double d) {  // e.g. src/Face/face_create_dest
 
 // analytic frame rotation, src/Sim/sim_w_a.c:8-73
 #define W_A_MILOS_INDEX 8
-DHD __noinline__ double d_rOm_a_any(int type, double r, double a) {
+static DHD __noinline__ double d_rOm_a_any(int type, double r, double a) {
   switch (type) {
     case A_KEPLER: return pow(r, -0.5);
     case A_OMEGA20: return r * 20;
@@ -58,7 +58,7 @@ DHD __noinline__ double d_rOm_a_any(int type, double r, double a) {
     default: return 0.0;
   }
 }
-DHD __noinline__ double d_rdrOm_a_any(int type, double r, double a) {
+static DHD __noinline__ double d_rdrOm_a_any(int type, double r, double a) {
   switch (type) {
     case A_KEPLER: return -1.5 * pow(r, -1.5);
     case A_MILOS: {

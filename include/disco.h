@@ -235,6 +235,21 @@ int disco_batch_profile(struct DiscoBatch *, int nsteps, char *names, double *ms
 /* device copy of n doubles (8n bytes read, 8n written); returns ms */
 double disco_calibrate_copy(long long n);
 
+/* No reference counterpart: the hand-written adjoint of timestep_rk2.
+ * Loss L = (1/T) sum over steps, substeps and interior cells of
+ * dt_s*dV*rho*F_phi*r (the gravitational torque on the gas, the LLL source of
+ * src/Cell/cell_source.c:171; T the simulated time).  From the batch's
+ * current state, disco_batch_torque_loss runs nsteps RK2 steps and returns L;
+ * disco_batch_adjoint also returns dL/d(Explicit_Viscosity), dL/d(MassRatio)
+ * and dL/d(initial primitives) (NUM_Q*Nzones, q-major, reference cell order),
+ * recomputing the forward from a checkpoint every checkpoint_every steps.
+ * One 2D disk, NUM_Q 5, w_analytic_type 0, a mesh velocity independent of the
+ * state, no sink or cooling, outflow r boundaries; otherwise the calls set
+ * the error and return -1.  The batch's state afterwards is unspecified. */
+int disco_batch_torque_loss(struct DiscoBatch *, int nsteps, double *loss);
+int disco_batch_adjoint(struct DiscoBatch *, int nsteps, int checkpoint_every, double *loss, double *grad_alpha,
+                        double *grad_q, double *grad_prim0);
+
 /* ---- errors and version (no reference counterpart; the reference exits) ---- */
 int disco_last_error(void);
 const char *disco_last_error_msg(void);

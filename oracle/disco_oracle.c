@@ -51,6 +51,7 @@ struct OracleSim {
   int nfaces[2];
   int *nri, *nzk;
   int err;
+  double torque; /* sum of dt*dV*rho*F_phi*r over interior cells and substeps (the adjoint's loss) */
 };
 
 /* sim_FacePos (src/Sim/sim_access.c:26-36): index -1 is the first face */
@@ -1153,6 +1154,9 @@ static void add_src(OracleSim *s, double dt) { /* :73-214 */
         U(s, SRR, c) += dt * dV * (rho * vp * vp + Pp) / r;
         U(s, SRR, c) += dt * dV * rho * (Fr * sint + F_centrifugal_r);
         U(s, LLL, c) += dt * dV * rho * (Fp - r * dtOm) * r;
+        if (i >= s->ip[OI_NgMinR] && i < s->NR - s->ip[OI_NgMaxR] && k >= s->ip[OI_NgMinZ] &&
+            k < s->NZ - s->ip[OI_NgMaxZ])
+          s->torque += dt * dV * rho * Fp * r;
         U(s, SZZ, c) += dt * dV * rho * Fr * cost;
         U(s, TAU, c) += dt * dV * rho * ((Fr * sint + F_centrifugal_r) * vr + Fr * vz * cost + (Fp + F_euler_phi) * vp);
         if (s->ip[OI_RhoSinkOn] == 1) {
@@ -1424,3 +1428,6 @@ int oracle_run(OracleSim *s, int nsteps) {
   }
   return 0;
 }
+
+/* accumulated torque term (test infrastructure for the adjoint's loss) */
+double oracle_torque(OracleSim *s) { return s->torque; }

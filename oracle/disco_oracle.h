@@ -45,6 +45,7 @@ void oracle_init(OracleSim *s);
 /* one timestep_rk2; returns 0, or a nonzero error code */
 int oracle_rk2(OracleSim *s);
 int oracle_run(OracleSim *s, int nsteps);
+double oracle_torque(OracleSim *s);
 /* field: 0 prim, 1 cons, 2 RKcons, 3 grad, 4 gradp (NQ*Nz each);
  *        5 tiph, 6 RKtiph, 7 dphi, 8 wiph (Nz each) */
 void oracle_get(OracleSim *s, int field, double *out);

@@ -61,6 +61,8 @@ def lib():
         L.oracle_face_create.restype = ctypes.c_int
         L.oracle_face_export.argtypes = [vp, ctypes.c_int, ip, dp, ip]
         L.oracle_plm_rz.argtypes = [vp, ctypes.c_int]
+        L.oracle_torque.argtypes = [vp]
+        L.oracle_torque.restype = ctypes.c_double
         L.oracle_mindt.argtypes = [vp]
         L.oracle_mindt.restype = ctypes.c_double
         L.oracle_last_error.argtypes = [vp]
@@ -136,6 +138,10 @@ class Oracle:
 
     def put(self, field, arr):
         lib().oracle_put(self.h, FIELDS[field], _dp(np.ascontiguousarray(arr, dtype=np.float64)))
+
+    def torque(self):
+        """sum of dt*dV*rho*F_phi*r over interior cells and substeps so far"""
+        return lib().oracle_torque(self.h)
 
     def scalars(self):
         out = np.empty(23)

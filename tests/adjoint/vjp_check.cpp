@@ -183,8 +183,9 @@ int main() {
     Stat st{kind == 0 ? "rface_visc" : "phiface_visc"};
     for (int t = 0; t < T; ++t) {
       const int solver = t % 2;
-      // x: pL pR gL gR gpL gpR (30) | dt alpha | M0 M1 x0 y0 x1 y1
-      std::vector<double> x(38);
+      // x: pL pR gL gR gpL gpR (30) | dt alpha | M0 M1 x0 y0 x1 y1 | geometry:
+      // r face: dA cm tL tR; phi face: a rigid rotation of the annulus
+      std::vector<double> x(kind == 0 ? 42 : 39);
       rand_prim(&x[0]);
       rand_prim(&x[5]);
       for (int q = 0; q < 5; ++q) {  // no v_z anywhere: the viscous flux refuses it
@@ -217,6 +218,14 @@ int main() {
       pg.dR = U(0.01, 0.05);
       pg.tR = pg.tL + pg.dR;
       pg.rdr = 0.0;
+      if (kind == 0) {
+        x[38] = rg.dA;
+        x[39] = rg.cm;
+        x[40] = rg.tL;
+        x[41] = rg.tR;
+      } else {
+        x[38] = 0.0;
+      }
       auto unpack = [&](const double *v, FacePair &c, DiskParams &P, DiskScalars &sc) {
         for (int q = 0; q < 5; ++q) {
           c.pL[q] = v[q];
@@ -234,10 +243,19 @@ int main() {
         DiskParams P;
         DiskScalars sc;
         unpack(v, c, P, sc);
-        if (kind == 0)
-          rface_flux(P, sc, c, rg, v[30], y, y + 5);
-        else
-          phiface_flux(P, sc, c, pg, v[30], y, y + 5);
+        if (kind == 0) {
+          RFaceGeo g = rg;
+          g.dA = v[38];
+          g.cm = v[39];
+          g.tL = v[40];
+          g.tR = v[41];
+          rface_flux(P, sc, c, g, v[30], y, y + 5);
+        } else {
+          PhiFaceGeo g = pg;
+          g.tL += v[38];
+          g.tR += v[38];
+          phiface_flux(P, sc, c, g, v[30], y, y + 5);
+        }
       };
       auto g = [&](const double *v, const double *yb, double *gx) {
         FacePair c, cb = {};
@@ -245,10 +263,25 @@ int main() {
         DiskScalars sc;
         unpack(v, c, P, sc);
         ScalarAdj sb = {};
-        if (kind == 0)
-          vjp_rface_flux(P, sc, c, rg, v[30], yb, yb + 5, cb, sb);
-        else
-          vjp_phiface_flux(P, sc, c, pg, v[30], yb, yb + 5, cb, sb);
+        if (kind == 0) {
+          RFaceGeo g = rg, gb = {};
+          g.dA = v[38];
+          g.cm = v[39];
+          g.tL = v[40];
+          g.tR = v[41];
+          vjp_rface_flux(P, sc, c, g, v[30], yb, yb + 5, cb, sb, gb);
+          gx[38] += gb.dA;
+          gx[39] += gb.cm;
+          gx[40] += gb.tL;
+          gx[41] += gb.tR;
+        } else {
+          PhiFaceGeo g = pg;
+          g.tL += v[38];
+          g.tR += v[38];
+          double tb = 0.0;
+          vjp_phiface_flux(P, sc, c, g, v[30], yb, yb + 5, cb, sb, &tb);
+          gx[38] += tb;
+        }
         for (int q = 0; q < 5; ++q) {
           gx[q] += cb.pL[q];
           gx[5 + q] += cb.pR[q];
@@ -268,18 +301,19 @@ int main() {
   for (int g2d = 1; g2d >= 0; --g2d) {  // sources and the torque term: prim, dt, masses
     Stat st{g2d ? "sources_2d" : "sources_3d"};
     for (int t = 0; t < T; ++t) {
-      std::vector<double> x(12);
+      std::vector<double> x(13);
       rand_prim(&x[0]);
       x[UZZ] = U(-0.1, 0.1);
       x[5] = U(1e-3, 1e-2);
       rand_masses(&x[6]);
-      const double r = U(0.6, 3.0), dV = U(0.001, 0.05), phi = U(0, 6.28), zc = g2d ? 0.0 : U(-0.3, 0.3);
+      x[12] = U(0, 6.28);  // the cell-centre angle
+      const double r = U(0.6, 3.0), dV = U(0.001, 0.05), zc = g2d ? 0.0 : U(-0.3, 0.3);
       DiskParams P = params(0, -1);
       P.grav2d = g2d;
       auto f = [&](const double *v, double *y) {
         DiskScalars sc = masses(v + 6);
         double u[5] = {0, 0, 0, 0, 0}, Fp, s, c;
-        sincos(phi, &s, &c);
+        sincos(v[12], &s, &c);
         add_geom_grav_src(u, v, r, dV, v[5], c, s, zc, 0.0, 0.0, 0.0, P, sc, &Fp);
         for (int q = 0; q < 5; ++q) y[q] = u[q];
         y[5] = v[5] * dV * v[RHO] * Fp * r;
@@ -287,7 +321,7 @@ int main() {
       auto g = [&](const double *v, const double *yb, double *gx) {
         DiskScalars sc = masses(v + 6);
         ScalarAdj sb = {};
-        vjp_cell_sources(P, sc, v, r, dV, v[5], phi, zc, yb, yb[5], gx, sb);
+        vjp_cell_sources(P, sc, v, r, dV, v[5], v[12], zc, yb, yb[5], gx, sb, &gx[12]);
         gx[5] += sb.dt;
         add_scalars(sb, gx + 6);
       };
@@ -298,12 +332,13 @@ int main() {
   for (int vmode = 0; vmode < 3; ++vmode) {  // CFL dt: no viscosity, constant, alpha law
     Stat st{vmode == 0 ? "cell_dt" : (vmode == 1 ? "cell_dt_nu_const" : "cell_dt_alpha")};
     for (int t = 0; t < T; ++t) {
-      std::vector<double> x(12);
+      std::vector<double> x(13);
       rand_prim(&x[0]);
       x[UZZ] = U(-0.1, 0.1);
       x[5] = U(1e-3, 1e-1);
       rand_masses(&x[6]);
-      const double r = U(0.6, 3.0), dx = U(0.01, 0.05), w = U(0.0, 1.0), tc = U(0, 6.28);
+      x[12] = U(0, 6.28);  // the cell-centre angle
+      const double r = U(0.6, 3.0), dx = U(0.01, 0.05), w = U(0.0, 1.0);
       auto mkP = [&](double alpha) {
         DiskParams P = params(0, vmode == 0 ? -1.0 : alpha);
         P.visc_const = vmode == 1;
@@ -312,13 +347,13 @@ int main() {
       auto f = [&](const double *v, double *y) {
         DiskParams P = mkP(v[5]);
         DiskScalars sc = masses(v + 6);
-        y[0] = cell_dt_value(P, sc, v, r, dx, w, tc);
+        y[0] = cell_dt_value(P, sc, v, r, dx, w, v[12]);
       };
       auto g = [&](const double *v, const double *yb, double *gx) {
         DiskParams P = mkP(v[5]);
         DiskScalars sc = masses(v + 6);
         ScalarAdj sb = {};
-        vjp_cell_dt(P, sc, v, r, dx, w, tc, yb[0], gx, sb);
+        vjp_cell_dt(P, sc, v, r, dx, w, v[12], yb[0], gx, sb, &gx[12]);
         gx[5] += sb.alpha;
         add_scalars(sb, gx + 6);
       };
