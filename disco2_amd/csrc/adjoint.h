@@ -632,17 +632,50 @@ DHD __forceinline__ void rface_flux(const DiskParams &P, const DiskScalars &sc, 
   Y[0] = Y[1] = Y[2] = 0.0;
   d_face_flux<5>(primL, primR, 0, g.r, g.dA, 0.0, dt, P, sc, &vi, X, Y);
 }
-// gb accumulates the adjoints of the face area dA, centre cm and the cells'
-// edges tL, tR (the mesh geometry, which moves with dt)
-DHD __forceinline__ void vjp_rface_flux(const DiskParams &P, const DiskScalars &sc, const FacePair &c,
-                                        const RFaceGeo &g, double dt, const double *Xb, const double *Yb,
-                                        FacePair &cb, ScalarAdj &sb, RFaceGeo &gb) {
+// The r face's VJP in two parts (the kernels run them as two launches to keep
+// the register count down): the Riemann flux of the reconstructed states and
+// the viscous flux of the phi-shifted states.  Each returns the adjoints of
+// the quantities it read from the two cells, of the face area dA, of the
+// offsets dpL, dpR (which move with the face centre and the cells' edges)
+// and, for the viscous part, of the face centre cm.
+struct RFaceHydroAdj {
+  double pLb[5], pRb[5];  // of primL = p_L + g_L dL + gp_L dpL and primR = p_R - g_R dR - gp_R dpR
+  double dAb, dpLb, dpRb;
+};
+DHD __forceinline__ void vjp_rface_hydro(const DiskParams &P, const FacePair &c, const RFaceGeo &g, double dt,
+                                         const double *Xb, ScalarAdj &sb, RFaceHydroAdj &h) {
   const double dpL = wrap_pm(g.cm - (g.tL - .5 * g.dL));
   const double dpR = wrap_pm(-g.cm + (g.tR - .5 * g.dR));
-  double primL[5], primR[5], WL[5], WR[5], Avg[5], Grr[2], Gph[2];
+  double primL[5], primR[5], Fb[5], F[5];
   for (int q = 0; q < 5; ++q) {
     primL[q] = c.pL[q] + c.gL[q] * g.deltaL + c.gpL[q] * dpL;
     primR[q] = c.pR[q] - c.gR[q] * g.deltaR - c.gpR[q] * dpR;
+    Fb[q] = dt * g.dA * Xb[q];
+    h.pLb[q] = h.pRb[q] = 0.0;
+  }
+  vjp_riemann(primL, primR, 0, g.r, 0.0, P, Fb, h.pLb, h.pRb, F);
+  double sdt = 0.0;
+  for (int q = 0; q < 5; ++q) sdt += F[q] * Xb[q];
+  sb.dt += g.dA * sdt;
+  h.dAb = dt * sdt;
+  h.dpLb = h.dpRb = 0.0;
+  for (int q = 0; q < 5; ++q) {
+    h.dpLb += h.pLb[q] * c.gpL[q];
+    h.dpRb -= h.pRb[q] * c.gpR[q];
+  }
+}
+struct RFaceViscAdj {
+  double WLb[5], WRb[5];  // of W_L = p_L + dpL gp_L and W_R = p_R - dpR gp_R
+  double Gphb[2];         // of Gph = (gp_L + gp_R)/(2r) for v_r, omega
+  double dAb, cmb, dpLb, dpRb;
+};
+DHD __forceinline__ void vjp_rface_visc(const DiskParams &P, const DiskScalars &sc, const FacePair &c,
+                                        const RFaceGeo &g, double dt, const double *Yb, ScalarAdj &sb,
+                                        RFaceViscAdj &v) {
+  const double dpL = wrap_pm(g.cm - (g.tL - .5 * g.dL));
+  const double dpR = wrap_pm(-g.cm + (g.tR - .5 * g.dR));
+  double WL[5], WR[5], Avg[5], Grr[2], Gph[2];
+  for (int q = 0; q < 5; ++q) {
     WL[q] = c.pL[q] + dpL * c.gpL[q];
     WR[q] = c.pR[q] - dpR * c.gpR[q];
     Avg[q] = 0.5 * (WL[q] + WR[q]);
@@ -652,35 +685,66 @@ DHD __forceinline__ void vjp_rface_flux(const DiskParams &P, const DiskScalars &
     Grr[k] = (WR[URR + k] - WL[URR + k]) / dsum;
     Gph[k] = .5 * (c.gpL[URR + k] + c.gpR[URR + k]) / g.r;
   }
-  double pLb[5] = {0, 0, 0, 0, 0}, pRb[5] = {0, 0, 0, 0, 0}, Avgb[5] = {0, 0, 0, 0, 0};
-  double Grrb[2] = {0, 0}, Gphb[2] = {0, 0};
-  vjp_face_flux(P, sc, primL, primR, 0, g.r, g.dA, 0.0, dt, g.cm, g.rdr, Avg, Grr, Gph, Xb, Yb, pLb, pRb, Avgb, Grrb,
-                Gphb, sb, &gb.dA, &gb.cm);
-  double WLb[5], WRb[5];
-  for (int q = 0; q < 5; ++q) {
-    cb.pL[q] += pLb[q];
-    cb.gL[q] += g.deltaL * pLb[q];
-    cb.gpL[q] += dpL * pLb[q];
-    cb.pR[q] += pRb[q];
-    cb.gR[q] -= g.deltaR * pRb[q];
-    cb.gpR[q] -= dpR * pRb[q];
-    WLb[q] = .5 * Avgb[q];
-    WRb[q] = .5 * Avgb[q];
+  double vf[3], vb[3], Avgb[5] = {0, 0, 0, 0, 0}, Grrb[2] = {0, 0};
+  visc_flux_v(P, sc, 1, g.r, g.cm, g.rdr, Avg, Grr, Gph, vf);
+  double sdt = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    vb[k] = dt * g.dA * Yb[k];
+    sdt += vf[k] * Yb[k];
   }
+  sb.dt += g.dA * sdt;
+  v.dAb = dt * sdt;
+  v.cmb = 0.0;
+  v.Gphb[0] = v.Gphb[1] = 0.0;
+  vjp_visc_flux(P, sc, 1, g.r, g.cm, g.rdr, Avg, Grr, Gph, vb, Avgb, Grrb, v.Gphb, sb, &v.cmb);
+  for (int q = 0; q < 5; ++q) v.WLb[q] = v.WRb[q] = .5 * Avgb[q];
   for (int k = 0; k < 2; ++k) {
-    WLb[URR + k] -= Grrb[k] / dsum;
-    WRb[URR + k] += Grrb[k] / dsum;
-    cb.gpL[URR + k] += .5 * Gphb[k] / g.r;
-    cb.gpR[URR + k] += .5 * Gphb[k] / g.r;
+    v.WLb[URR + k] -= Grrb[k] / dsum;
+    v.WRb[URR + k] += Grrb[k] / dsum;
   }
-  double dpLb = 0.0, dpRb = 0.0;
+  v.dpLb = v.dpRb = 0.0;
   for (int q = 0; q < 5; ++q) {
-    cb.pL[q] += WLb[q];
-    cb.gpL[q] += dpL * WLb[q];
-    cb.pR[q] += WRb[q];
-    cb.gpR[q] -= dpR * WRb[q];
-    dpLb += (pLb[q] + WLb[q]) * c.gpL[q];
-    dpRb -= (pRb[q] + WRb[q]) * c.gpR[q];
+    v.dpLb += v.WLb[q] * c.gpL[q];
+    v.dpRb -= v.WRb[q] * c.gpR[q];
+  }
+}
+
+// gb accumulates the adjoints of the face area dA, centre cm and the cells'
+// edges tL, tR (the mesh geometry, which moves with dt); both parts together
+DHD __forceinline__ void vjp_rface_flux(const DiskParams &P, const DiskScalars &sc, const FacePair &c,
+                                        const RFaceGeo &g, double dt, const double *Xb, const double *Yb,
+                                        FacePair &cb, ScalarAdj &sb, RFaceGeo &gb) {
+  const double dpL = wrap_pm(g.cm - (g.tL - .5 * g.dL));
+  const double dpR = wrap_pm(-g.cm + (g.tR - .5 * g.dR));
+  RFaceHydroAdj h;
+  vjp_rface_hydro(P, c, g, dt, Xb, sb, h);
+  double dpLb = h.dpLb, dpRb = h.dpRb;
+  gb.dA += h.dAb;
+  for (int q = 0; q < 5; ++q) {
+    cb.pL[q] += h.pLb[q];
+    cb.gL[q] += g.deltaL * h.pLb[q];
+    cb.gpL[q] += dpL * h.pLb[q];
+    cb.pR[q] += h.pRb[q];
+    cb.gR[q] -= g.deltaR * h.pRb[q];
+    cb.gpR[q] -= dpR * h.pRb[q];
+  }
+  if (P.visc > 0.0) {
+    RFaceViscAdj v;
+    vjp_rface_visc(P, sc, c, g, dt, Yb, sb, v);
+    gb.dA += v.dAb;
+    gb.cm += v.cmb;
+    dpLb += v.dpLb;
+    dpRb += v.dpRb;
+    for (int q = 0; q < 5; ++q) {
+      cb.pL[q] += v.WLb[q];
+      cb.gpL[q] += dpL * v.WLb[q];
+      cb.pR[q] += v.WRb[q];
+      cb.gpR[q] -= dpR * v.WRb[q];
+    }
+    for (int k = 0; k < 2; ++k) {
+      cb.gpL[URR + k] += .5 * v.Gphb[k] / g.r;
+      cb.gpR[URR + k] += .5 * v.Gphb[k] / g.r;
+    }
   }
   // dpL = cm - (tL - dL/2), dpR = (tR - dR/2) - cm
   gb.cm += dpLb - dpRb;
@@ -716,35 +780,78 @@ DHD __forceinline__ void phiface_flux(const DiskParams &P, const DiskScalars &sc
   Y[0] = Y[1] = Y[2] = 0.0;
   d_face_flux<5>(primL, primR, 1, g.r, g.dA, g.w, dt, P, sc, &vi, X, Y);
 }
-// *tLb: adjoint of the face's angle (the viscosity is evaluated at cm = tL)
-DHD __forceinline__ void vjp_phiface_flux(const DiskParams &P, const DiskScalars &sc, const FacePair &c,
-                                          const PhiFaceGeo &g, double dt, const double *Xb, const double *Yb,
-                                          FacePair &cb, ScalarAdj &sb, double *tLb) {
-  double primL[5], primR[5], Avg[5], Grr[2], Gph[2];
+// The phi face's VJP in two parts, as for the r face: the Riemann flux of
+// primL = p_L + gp_L dL/2, primR = p_R - gp_R dR/2, and the viscous flux of
+// the cell averages with the r slopes and the (unwrapped) phi difference;
+// the viscosity is evaluated at the face's angle tL.
+DHD __forceinline__ void vjp_phiface_hydro(const DiskParams &P, const FacePair &c, const PhiFaceGeo &g, double dt,
+                                           const double *Xb, ScalarAdj &sb, double *pLb, double *pRb) {
+  double primL[5], primR[5], Fb[5], F[5];
   for (int q = 0; q < 5; ++q) {
     primL[q] = c.pL[q] + 0.5 * c.gpL[q] * g.dL;
     primR[q] = c.pR[q] - 0.5 * c.gpR[q] * g.dR;
-    Avg[q] = 0.5 * (c.pR[q] + c.pL[q]);
+    Fb[q] = dt * g.dA * Xb[q];
+    pLb[q] = pRb[q] = 0.0;
   }
-  const double dt_phi = (g.tR - g.tL) * g.r;  // (vR - vL) / (tR - tL) / r
+  vjp_riemann(primL, primR, 1, g.r, g.w, P, Fb, pLb, pRb, F);
+  double sdt = 0.0;
+  for (int q = 0; q < 5; ++q) sdt += F[q] * Xb[q];
+  sb.dt += g.dA * sdt;
+}
+struct PhiFaceViscAdj {
+  double Avgb[5], Grrb[2], Gphb[2], tLb;
+};
+DHD __forceinline__ void vjp_phiface_visc(const DiskParams &P, const DiskScalars &sc, const FacePair &c,
+                                          const PhiFaceGeo &g, double dt, const double *Yb, ScalarAdj &sb,
+                                          PhiFaceViscAdj &v) {
+  double Avg[5], Grr[2], Gph[2], vf[3], vb[3];
+  for (int q = 0; q < 5; ++q) {
+    Avg[q] = 0.5 * (c.pR[q] + c.pL[q]);
+    v.Avgb[q] = 0.0;
+  }
   for (int k = 0; k < 2; ++k) {
     Grr[k] = .5 * (c.gL[URR + k] + c.gR[URR + k]);
     Gph[k] = (c.pR[URR + k] - c.pL[URR + k]) / (g.tR - g.tL) / g.r;
+    v.Grrb[k] = v.Gphb[k] = 0.0;
   }
-  double pLb[5] = {0, 0, 0, 0, 0}, pRb[5] = {0, 0, 0, 0, 0}, Avgb[5] = {0, 0, 0, 0, 0};
-  double Grrb[2] = {0, 0}, Gphb[2] = {0, 0};
-  vjp_face_flux(P, sc, primL, primR, 1, g.r, g.dA, g.w, dt, g.tL, g.rdr, Avg, Grr, Gph, Xb, Yb, pLb, pRb, Avgb,
-                Grrb, Gphb, sb, nullptr, tLb);
+  visc_flux_v(P, sc, 0, g.r, g.tL, g.rdr, Avg, Grr, Gph, vf);
+  double sdt = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    vb[k] = dt * g.dA * Yb[k];
+    sdt += vf[k] * Yb[k];
+  }
+  sb.dt += g.dA * sdt;
+  v.tLb = 0.0;
+  vjp_visc_flux(P, sc, 0, g.r, g.tL, g.rdr, Avg, Grr, Gph, vb, v.Avgb, v.Grrb, v.Gphb, sb, &v.tLb);
+}
+
+// *tLb: adjoint of the face's angle (the viscosity is evaluated at cm = tL);
+// both parts together
+DHD __forceinline__ void vjp_phiface_flux(const DiskParams &P, const DiskScalars &sc, const FacePair &c,
+                                          const PhiFaceGeo &g, double dt, const double *Xb, const double *Yb,
+                                          FacePair &cb, ScalarAdj &sb, double *tLb) {
+  double pLb[5], pRb[5];
+  vjp_phiface_hydro(P, c, g, dt, Xb, sb, pLb, pRb);
   for (int q = 0; q < 5; ++q) {
-    cb.pL[q] += pLb[q] + .5 * Avgb[q];
+    cb.pL[q] += pLb[q];
     cb.gpL[q] += .5 * g.dL * pLb[q];
-    cb.pR[q] += pRb[q] + .5 * Avgb[q];
+    cb.pR[q] += pRb[q];
     cb.gpR[q] -= .5 * g.dR * pRb[q];
   }
-  for (int k = 0; k < 2; ++k) {
-    cb.gL[URR + k] += .5 * Grrb[k];
-    cb.gR[URR + k] += .5 * Grrb[k];
-    cb.pR[URR + k] += Gphb[k] / dt_phi;
-    cb.pL[URR + k] -= Gphb[k] / dt_phi;
+  if (P.visc > 0.0) {
+    PhiFaceViscAdj v;
+    vjp_phiface_visc(P, sc, c, g, dt, Yb, sb, v);
+    const double dt_phi = (g.tR - g.tL) * g.r;
+    for (int q = 0; q < 5; ++q) {
+      cb.pL[q] += .5 * v.Avgb[q];
+      cb.pR[q] += .5 * v.Avgb[q];
+    }
+    for (int k = 0; k < 2; ++k) {
+      cb.gL[URR + k] += .5 * v.Grrb[k];
+      cb.gR[URR + k] += .5 * v.Grrb[k];
+      cb.pR[URR + k] += v.Gphb[k] / dt_phi;
+      cb.pL[URR + k] -= v.Gphb[k] / dt_phi;
+    }
+    *tLb += v.tLb;
   }
 }

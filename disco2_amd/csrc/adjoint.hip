@@ -95,6 +95,20 @@ __device__ __forceinline__ void face_geo_to_rot(const DevMesh &m, const AdjMesh 
   if (vout != 0.0) atomicAdd(&a.arot[cR], vout);
 }
 
+// the same split, returned: vin to the L cell's annulus, vout to the R cell's
+__device__ __forceinline__ void face_rot_split(signed char type, double wb, double cb, double tLb, double tRb,
+                                               double *vin, double *vout) {
+  double w_in = 0.0, w_out = 0.0, c_in = 0.0, c_out = 0.0;
+  switch (type) {
+    case FACE_II: c_in = 1.0; break;
+    case FACE_IO: w_in = -1.0; w_out = 1.0; c_in = c_out = 0.5; break;
+    case FACE_OO: c_out = 1.0; break;
+    default: w_in = 1.0; w_out = -1.0; c_in = c_out = 0.5; break;  // FACE_OI
+  }
+  *vin = wb * w_in + cb * c_in + tLb;
+  *vout = wb * w_out + cb * c_out + tRb;
+}
+
 // Outflow remap backward (k_bc_r): a ghost cell's adjoint goes to the cells
 // it was averaged from, weighted as the forward weighted their primitives.
 // The mesh view has the substep's faces and the new dphi.
@@ -175,7 +189,7 @@ __global__ __launch_bounds__(TILE) void ka_update(DevMesh m, AdjMesh a, const in
     const double wl = interior_cell(P, A.i, A.k) ? a.wl : 0.0;
     double phib = 0.0;
     vjp_cell_sources(P, sc, pr, r, dVs, a.fac * sc.dt, phi, .5 * (zp + zm), ub, wl, prb, sb, &phib);
-    if (phib != 0.0) atomicAdd(&a.arot[c], phib);
+    a.arot[c] += phib;
     store5(a.aP, m.Ntot, c, prb);
     const double z[5] = {0, 0, 0, 0, 0};
     store5(a.agrad, m.Ntot, c, z);
@@ -184,45 +198,183 @@ __global__ __launch_bounds__(TILE) void ka_update(DevMesh m, AdjMesh a, const in
   reduce_store(sb, &a.part[ti]);
 }
 
-// phi faces backward: face (j, j+1), with cell j+1 receiving +X and j -X.
-__global__ __launch_bounds__(TILE) void ka_phi(DevMesh m, AdjMesh a, const int2 *tiles, int slot0) {
+// phi face (j, j+1): cell j+1 received +X (+Y), cell j -X.
+struct PhiFaceAt {
+  long long cL, cR;
+  double Xb[5], Yb[3];
+  PhiFaceGeo g;
+};
+__device__ __forceinline__ void phi_face_at(const DevMesh &m, const AdjMesh &a, const AnnInfo &A, int ann,
+                                            const DiskParams &P, int j, PhiFaceAt &F) {
+  F.cL = A.zbase + j;
+  F.cR = A.zbase + (j == A.N - 1 ? 0 : j + 1);
+  const double r = .5 * (A.rp + A.rm);
+  for (int q = 0; q < 5; ++q) F.Xb[q] = a.au[q * m.Ntot + F.cR] - a.au[q * m.Ntot + F.cL];
+  F.Yb[0] = a.au[SRR * m.Ntot + F.cR] / r - a.au[SRR * m.Ntot + F.cL] / r;
+  F.Yb[1] = a.au[LLL * m.Ntot + F.cR] - a.au[LLL * m.Ntot + F.cL];
+  F.Yb[2] = a.au[TAU * m.Ntot + F.cR] - a.au[TAU * m.Ntot + F.cL];
+  F.g.r = r;
+  F.g.dA = (A.rp - A.rm) * (A.zp - A.zm);
+  F.g.w = m.wiph[ann];
+  F.g.tL = clean_pi(m.tiph[F.cL]);
+  F.g.tR = clean_pi(m.tiph[F.cR]);
+  F.g.dL = m.dphi[F.cL];
+  F.g.dR = m.dphi[F.cR];
+  F.g.rdr = ann_rotation(m, P, ann).y;
+}
+
+// The phi-face backward kernels run over tiles of TILE - 1 cells j0 .. j0+n-1:
+// thread t solves face slot t = (j0-1+t, j0+t) for t = 0..n, keeps what each
+// of its two cells receives in LDS, and thread t then adds slot t (its cell
+// is R) and slot t+1 (its cell is L) into its own cell: no atomics.  A face
+// on a tile boundary is solved by both tiles; only slots 0..n-1 add to the
+// scalar adjoints, which counts every face once.
+
+// phi faces backward, Riemann part
+__global__ __launch_bounds__(TILE) void ka_phi_h(DevMesh m, AdjMesh a, const int2 *tiles, int slot0) {
+  const int ti = xcd_order();
+  const int2 tl = tiles[ti];
+  const AnnInfo A = m.ann[tl.x];
+  const DiskParams &P = disk_of(m, A.disk);
+  const int t = threadIdx.x;
+  const int n = min(TILE - 1, A.N - tl.y);
+  __shared__ double sL[5][TILE], sR[5][TILE];
+  ScalarAdj sb = {};
+  if (t <= n) {
+    int jf = tl.y - 1 + t;
+    if (jf < 0) jf += A.N;
+    PhiFaceAt F;
+    phi_face_at(m, a, A, tl.x, P, jf, F);
+    FacePair c;
+    load_pair(m, F.cL, F.cR, c);
+    double pLb[5], pRb[5];
+    ScalarAdj fs = {};
+    vjp_phiface_hydro(P, c, F.g, a.fac * m.sc[A.disk].dt, F.Xb, fs, pLb, pRb);
+    if (t < n) sb = fs;
+    for (int q = 0; q < 5; ++q) {
+      sL[q][t] = pLb[q];
+      sR[q][t] = pRb[q];
+    }
+  }
+  __syncthreads();
+  if (t < n) {
+    const long long c = A.zbase + tl.y + t;
+    const double dph = m.dphi[c];
+    for (int q = 0; q < 5; ++q) {
+      const double bR = sR[q][t], bL = sL[q][t + 1];
+      a.aP[q * m.Ntot + c] += bR + bL;
+      a.agradp[q * m.Ntot + c] += .5 * dph * bL - .5 * dph * bR;
+    }
+  }
+  reduce_store(sb, &a.part[slot0 + ti]);
+}
+
+// phi faces backward, viscous part
+__global__ __launch_bounds__(TILE) void ka_phi_v(DevMesh m, AdjMesh a, const int2 *tiles, int slot0) {
   const int ti = xcd_order();
   const int2 tl = tiles[ti];
   const AnnInfo A = m.ann[tl.x];
   const DiskParams &P = disk_of(m, A.disk);
   const DiskScalars &sc = m.sc[A.disk];
-  const int j = tl.y + threadIdx.x;
+  const int t = threadIdx.x;
+  const int n = min(TILE - 1, A.N - tl.y);
+  // per slot: Avgb/2 (5), Gphb/(r dphi) (2), Grrb/2 (2), the face angle's adjoint
+  __shared__ double sv[10][TILE];
   ScalarAdj sb = {};
-  if (j < A.N) {
-    const long long cL = A.zbase + j, cR = A.zbase + (j == A.N - 1 ? 0 : j + 1);
-    const double r = .5 * (A.rp + A.rm);
-    double Xb[5], Yb[3];
-    for (int q = 0; q < 5; ++q) Xb[q] = a.au[q * m.Ntot + cR] - a.au[q * m.Ntot + cL];
-    Yb[0] = a.au[SRR * m.Ntot + cR] / r - a.au[SRR * m.Ntot + cL] / r;
-    Yb[1] = a.au[LLL * m.Ntot + cR] - a.au[LLL * m.Ntot + cL];
-    Yb[2] = a.au[TAU * m.Ntot + cR] - a.au[TAU * m.Ntot + cL];
-    FacePair c, cb = {};
-    load_pair(m, cL, cR, c);
-    const double4 rot = ann_rotation(m, P, tl.x);
-    PhiFaceGeo g;
-    g.r = r;
-    g.dA = (A.rp - A.rm) * (A.zp - A.zm);
-    g.w = m.wiph[tl.x];
-    g.tL = clean_pi(m.tiph[cL]);
-    g.tR = clean_pi(m.tiph[cR]);
-    g.dL = m.dphi[cL];
-    g.dR = m.dphi[cR];
-    g.rdr = rot.y;
-    double tLb = 0.0;
-    vjp_phiface_flux(P, sc, c, g, a.fac * sc.dt, Xb, Yb, cb, sb, &tLb);
-    scatter_pair(m, a, cL, cR, cb);
-    if (tLb != 0.0) atomicAdd(&a.arot[cL], tLb);
+  if (t <= n) {
+    int jf = tl.y - 1 + t;
+    if (jf < 0) jf += A.N;
+    PhiFaceAt F;
+    phi_face_at(m, a, A, tl.x, P, jf, F);
+    FacePair c;
+    load_pair(m, F.cL, F.cR, c);
+    PhiFaceViscAdj v;
+    ScalarAdj fs = {};
+    vjp_phiface_visc(P, sc, c, F.g, a.fac * sc.dt, F.Yb, fs, v);
+    if (t < n) sb = fs;
+    const double dt_phi = (F.g.tR - F.g.tL) * F.g.r;
+    for (int q = 0; q < 5; ++q) sv[q][t] = .5 * v.Avgb[q];
+    for (int k = 0; k < 2; ++k) {
+      sv[5 + k][t] = v.Gphb[k] / dt_phi;
+      sv[7 + k][t] = .5 * v.Grrb[k];
+    }
+    sv[9][t] = v.tLb;
+  }
+  __syncthreads();
+  if (t < n) {
+    const long long c = A.zbase + tl.y + t;
+    for (int q = 0; q < 5; ++q) a.aP[q * m.Ntot + c] += sv[q][t] + sv[q][t + 1];
+    for (int k = 0; k < 2; ++k) {
+      a.aP[(URR + k) * m.Ntot + c] += sv[5 + k][t] - sv[5 + k][t + 1];
+      a.agrad[(URR + k) * m.Ntot + c] += sv[7 + k][t] + sv[7 + k][t + 1];
+    }
+    a.arot[c] += sv[9][t + 1];
   }
   reduce_store(sb, &a.part[slot0 + ti]);
 }
 
-// r faces backward: the R cell received +X (+Y/r_R), the L cell -X.
-__global__ __launch_bounds__(TILE) void ka_rflux(DevMesh m, AdjMesh a, const int2 *ftiles, int slot0) {
+// r face g of a pair: the R cell received +X (+Y/r_R), the L cell -X.
+struct RFaceAt {
+  long long g, cL, cR;
+  double Xb[5], Yb[3];
+  RFaceGeo geo;
+  double wscale;  // d(dA)/d(face width): r deltaPerp
+};
+__device__ __forceinline__ void r_face_at(const DevMesh &m, const AdjMesh &a, const PairInfo &PI,
+                                          const DiskParams &P, long long g, RFaceAt &F) {
+  F.g = g;
+  F.cL = m.fL[g];
+  F.cR = m.fR[g];
+  for (int q = 0; q < 5; ++q) F.Xb[q] = a.au[q * m.Ntot + F.cR] - a.au[q * m.Ntot + F.cL];
+  F.Yb[0] = a.au[SRR * m.Ntot + F.cR] / (PI.r + PI.deltaR) - a.au[SRR * m.Ntot + F.cL] / (PI.r - PI.deltaL);
+  F.Yb[1] = a.au[LLL * m.Ntot + F.cR] - a.au[LLL * m.Ntot + F.cL];
+  F.Yb[2] = a.au[TAU * m.Ntot + F.cR] - a.au[TAU * m.Ntot + F.cL];
+  F.geo.r = PI.r;
+  F.geo.deltaL = PI.deltaL;
+  F.geo.deltaR = PI.deltaR;
+  F.geo.dA = PI.r * m.fdphi[g] * PI.deltaPerp;
+  F.geo.cm = m.fcm[g];
+  F.geo.tL = clean_pi(m.tiph[F.cL]);
+  F.geo.dL = m.dphi[F.cL];
+  F.geo.tR = clean_pi(m.tiph[F.cR]);
+  F.geo.dR = m.dphi[F.cR];
+  F.geo.rdr = ann_rotation(m, P, PI.ann_in).w;
+  F.wscale = PI.r * PI.deltaPerp;
+}
+
+// The r-face backward kernels write what each face hands its two cells into
+// per-face slots of a.fr (q-major over faces, AFR_* offsets in units of
+// NFtot); ka_plm_r gathers them per cell over the cell's face ranges.
+__global__ __launch_bounds__(TILE) void ka_rflux_h(DevMesh m, AdjMesh a, const int2 *ftiles, int slot0) {
+  const int ti = xcd_order();
+  const int2 tl = ftiles[ti];
+  const int p = tl.x;
+  const int f = tl.y + threadIdx.x;
+  ScalarAdj sb = {};
+  if (f < m.pst[p].count) {
+    const PairInfo PI = m.pairs[p];
+    const DiskParams &P = disk_of(m, PI.disk);
+    RFaceAt F;
+    r_face_at(m, a, PI, P, PI.face_base + f, F);
+    FacePair c;
+    load_pair(m, F.cL, F.cR, c);
+    RFaceHydroAdj h;
+    vjp_rface_hydro(P, c, F.geo, a.fac * m.sc[PI.disk].dt, F.Xb, sb, h);
+    const long long NF = m.NFtot, g = F.g;
+    for (int q = 0; q < 5; ++q) {
+      a.fr[(AFR_HL + q) * NF + g] = h.pLb[q];
+      a.fr[(AFR_HR + q) * NF + g] = h.pRb[q];
+    }
+    // dpL = cm - (tL - dL/2), dpR = (tR - dR/2) - cm
+    double vin, vout;
+    face_rot_split(m.ftype[g], h.dAb * F.wscale, h.dpLb - h.dpRb, -h.dpLb, h.dpRb, &vin, &vout);
+    a.fr[AFR_ROT * NF + g] = vin;
+    a.fr[(AFR_ROT + 1) * NF + g] = vout;
+  }
+  reduce_store(sb, &a.part[slot0 + ti]);
+}
+
+__global__ __launch_bounds__(TILE) void ka_rflux_v(DevMesh m, AdjMesh a, const int2 *ftiles, int slot0) {
   const int ti = xcd_order();
   const int2 tl = ftiles[ti];
   const int p = tl.x;
@@ -232,40 +384,34 @@ __global__ __launch_bounds__(TILE) void ka_rflux(DevMesh m, AdjMesh a, const int
     const PairInfo PI = m.pairs[p];
     const DiskParams &P = disk_of(m, PI.disk);
     const DiskScalars &sc = m.sc[PI.disk];
-    const long long g = PI.face_base + f;
-    const long long cL = m.fL[g], cR = m.fR[g];
-    double Xb[5], Yb[3];
-    for (int q = 0; q < 5; ++q) Xb[q] = a.au[q * m.Ntot + cR] - a.au[q * m.Ntot + cL];
-    Yb[0] = a.au[SRR * m.Ntot + cR] / (PI.r + PI.deltaR) - a.au[SRR * m.Ntot + cL] / (PI.r - PI.deltaL);
-    Yb[1] = a.au[LLL * m.Ntot + cR] - a.au[LLL * m.Ntot + cL];
-    Yb[2] = a.au[TAU * m.Ntot + cR] - a.au[TAU * m.Ntot + cL];
-    FacePair c, cb = {};
-    load_pair(m, cL, cR, c);
-    const double4 rot = ann_rotation(m, P, PI.ann_in);
-    RFaceGeo geo;
-    geo.r = PI.r;
-    geo.deltaL = PI.deltaL;
-    geo.deltaR = PI.deltaR;
-    geo.dA = PI.r * m.fdphi[g] * PI.deltaPerp;
-    geo.cm = m.fcm[g];
-    geo.tL = clean_pi(m.tiph[cL]);
-    geo.dL = m.dphi[cL];
-    geo.tR = clean_pi(m.tiph[cR]);
-    geo.dR = m.dphi[cR];
-    geo.rdr = rot.w;
-    RFaceGeo gb = {};
-    vjp_rface_flux(P, sc, c, geo, a.fac * sc.dt, Xb, Yb, cb, sb, gb);
-    scatter_pair(m, a, cL, cR, cb);
-    face_geo_to_rot(m, a, g, cL, cR, gb.dA * PI.r * PI.deltaPerp, gb.cm, gb.tL, gb.tR);
+    RFaceAt F;
+    r_face_at(m, a, PI, P, PI.face_base + f, F);
+    FacePair c;
+    load_pair(m, F.cL, F.cR, c);
+    RFaceViscAdj v;
+    vjp_rface_visc(P, sc, c, F.geo, a.fac * sc.dt, F.Yb, sb, v);
+    const long long NF = m.NFtot, g = F.g;
+    for (int q = 0; q < 5; ++q) {
+      a.fr[(AFR_WL + q) * NF + g] = v.WLb[q];
+      a.fr[(AFR_WR + q) * NF + g] = v.WRb[q];
+    }
+    for (int k = 0; k < 2; ++k) a.fr[(AFR_G + k) * NF + g] = .5 * v.Gphb[k] / F.geo.r;
+    double vin, vout;
+    face_rot_split(m.ftype[g], v.dAb * F.wscale, v.cmb + v.dpLb - v.dpRb, -v.dpLb, v.dpRb, &vin, &vout);
+    a.fr[AFR_ROT * NF + g] += vin;
+    a.fr[(AFR_ROT + 1) * NF + g] += vout;
   }
   reduce_store(sb, &a.part[slot0 + ti]);
 }
 
-// r slope backward (k_plm_rz, r direction): replay the sum and the
-// sequential limiter, then hand each face its share of the slope's adjoint:
-// dA/dAtot to every face when the sum survived, PLM to the face whose S the
-// limiter took last, nothing when it was zeroed.  Each face gets one value
-// from its L cell (aSL) and one from its R cell (aSR).
+// r slope backward (k_plm_rz, r direction).  First the cell gathers what
+// its r faces' flux adjoints hand it (primitives, r and phi slopes, its
+// annulus rotation).  Then it replays the sum and the sequential limiter and
+// hands each face its share of the slope's adjoint: dA/dAtot to every face
+// when the sum survived, PLM to the face whose S the limiter took last,
+// nothing when it was zeroed.  Each face gets one value from its L cell
+// (aSL, and the adjoint of its width AFR_WB) and one from its R cell (aSR,
+// AFR_WB + 1).
 __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int2 *tiles) {
   const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
@@ -273,9 +419,51 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
   if (j >= A.N) return;
   const DiskParams &P = disk_of(m, A.disk);
   const long long z = A.zbase + j;
+  const long long NF = m.NFtot;
   const int k = A.k, i = A.i;
   const int pR = i >= 1 ? rpair_of(P, k, i - 1) : -1;
   const int pLf = i <= P.NR - 2 ? rpair_of(P, k, i) : -1;
+  const bool visc = a.visc;
+  {
+    const double tc = clean_pi(m.tiph[z]), dc = m.dphi[z];
+    double ap[5] = {0, 0, 0, 0, 0}, ag[5] = {0, 0, 0, 0, 0}, agp[5] = {0, 0, 0, 0, 0}, rot = 0.0;
+    // as R: primR = p - g deltaR - gp dpR (hydro), W_R = p - dpR gp (viscous)
+    if (pR >= 0)
+      faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) {
+        const double dpR = wrap_pm(-m.fcm[gi] + (tc - .5 * dc));
+        for (int q = 0; q < 5; ++q) {
+          const double h = a.fr[(AFR_HR + q) * NF + gi];
+          const double s = visc ? h + a.fr[(AFR_WR + q) * NF + gi] : h;
+          ap[q] += s;
+          ag[q] -= PI.deltaR * h;
+          agp[q] -= dpR * s;
+        }
+        if (visc)
+          for (int kk = 0; kk < 2; ++kk) agp[URR + kk] += a.fr[(AFR_G + kk) * NF + gi];
+        rot += a.fr[(AFR_ROT + 1) * NF + gi];
+      });
+    // as L: primL = p + g deltaL + gp dpL, W_L = p + dpL gp
+    if (pLf >= 0)
+      faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) {
+        const double dpL = wrap_pm(m.fcm[gi] - (tc - .5 * dc));
+        for (int q = 0; q < 5; ++q) {
+          const double h = a.fr[(AFR_HL + q) * NF + gi];
+          const double s = visc ? h + a.fr[(AFR_WL + q) * NF + gi] : h;
+          ap[q] += s;
+          ag[q] += PI.deltaL * h;
+          agp[q] += dpL * s;
+        }
+        if (visc)
+          for (int kk = 0; kk < 2; ++kk) agp[URR + kk] += a.fr[(AFR_G + kk) * NF + gi];
+        rot += a.fr[AFR_ROT * NF + gi];
+      });
+    for (int q = 0; q < 5; ++q) {
+      a.aP[q * m.Ntot + z] += ap[q];
+      a.agrad[q * m.Ntot + z] += ag[q];
+      a.agradp[q * m.Ntot + z] += agp[q];
+    }
+    a.arot[z] += rot;
+  }
   double g[5] = {0, 0, 0, 0, 0};
   auto acc = [&](const PairInfo &PI, long long gi) {
     double S[5];
@@ -316,54 +504,89 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
   double gb[5];
   load5(a.agrad, m.Ntot, z, gb);
   n = 0;
-  auto put = [&](const PairInfo &PI, long long gi, double *dst) {
-    const double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
+  auto put = [&](const PairInfo &PI, long long gi, double *dst, int side) {
     double S[5], wb = 0.0;
     face_slope<5>(m, PI, gi, S);
+    const double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
     for (int q = 0; q < 5; ++q) {
-      dst[q * m.NFtot + gi] = src[q] == -1 ? dA / dAtot * gb[q] : (src[q] == n ? PLM * gb[q] : 0.0);
+      dst[q * NF + gi] = src[q] == -1 ? dA / dAtot * gb[q] : (src[q] == n ? PLM * gb[q] : 0.0);
       if (src[q] == -1) wb += gb[q] * S[q] / dAtot;  // d g / d dA_f
     }
-    if (wb != 0.0) face_geo_to_rot(m, a, gi, m.fL[gi], m.fR[gi], wb * PI.r * PI.deltaPerp, 0.0, 0.0, 0.0);
+    a.fr[(AFR_WB + side) * NF + gi] = wb * PI.r * PI.deltaPerp;
     ++n;
   };
-  if (pR >= 0) faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) { put(PI, gi, a.aSR); });
-  if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) { put(PI, gi, a.aSL); });
+  if (pR >= 0) faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) { put(PI, gi, a.aSR, 1); });
+  if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) { put(PI, gi, a.aSL, 0); });
 }
 
-// face slope S = (W_R - W_L)/(deltaL + deltaR) backward
-__global__ __launch_bounds__(TILE) void ka_slope(DevMesh m, AdjMesh a, const int2 *ftiles) {
-  const int2 tl = ftiles[xcd_order()];
-  const int p = tl.x;
-  const int f = tl.y + threadIdx.x;
-  if (f >= m.pst[p].count) return;
-  const PairInfo PI = m.pairs[p];
-  const long long g = PI.face_base + f;
-  const long long cL = m.fL[g], cR = m.fR[g];
-  const double cm = m.fcm[g];
-  const double dpL = wrap_pm(cm - (clean_pi(m.tiph[cL]) - .5 * m.dphi[cL]));
-  const double dpR = wrap_pm((clean_pi(m.tiph[cR]) - .5 * m.dphi[cR]) - cm);
-  const double d = PI.deltaR + PI.deltaL;
-  double pLb[5], pRb[5], gpLb[5], gpRb[5], dpLb = 0.0, dpRb = 0.0;
+// face slope S = (W_R - W_L)/(deltaL + deltaR), W_L = p_L + dpL gp_L,
+// W_R = p_R - dpR gp_R, backward, gathered per cell over its r faces
+__global__ __launch_bounds__(TILE) void ka_slope(DevMesh m, AdjMesh a, const int2 *tiles) {
+  const int2 tl = tiles[xcd_order()];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const DiskParams &P = disk_of(m, A.disk);
+  const long long z = A.zbase + j;
+  const long long NF = m.NFtot;
+  const int k = A.k, i = A.i;
+  const int pR = i >= 1 ? rpair_of(P, k, i - 1) : -1;
+  const int pLf = i <= P.NR - 2 ? rpair_of(P, k, i) : -1;
+  const double tc = clean_pi(m.tiph[z]), dc = m.dphi[z];
+  double gpc[5], ap[5] = {0, 0, 0, 0, 0}, agp[5] = {0, 0, 0, 0, 0}, rot = 0.0;
+  load5(m.gradp, m.Ntot, z, gpc);
+  // one face: Sb the slope's adjoint, W_Lb = -Sb/d, W_Rb = Sb/d; dpLb, dpRb
+  // the adjoints of the two offsets, which move the face centre and edges
+  auto face = [&](const PairInfo &PI, long long gi, bool asL) {
+    const long long o = asL ? m.fR[gi] : m.fL[gi];
+    const double cm = m.fcm[gi];
+    const double d = PI.deltaR + PI.deltaL;
+    const double dpc = asL ? wrap_pm(cm - (tc - .5 * dc)) : wrap_pm((tc - .5 * dc) - cm);
+    double dpLb = 0.0, dpRb = 0.0;
+    for (int q = 0; q < 5; ++q) {
+      const double Sb = a.aSL[q * NF + gi] + a.aSR[q * NF + gi];
+      const double gpo = m.gradp[q * m.Ntot + o];
+      if (asL) {
+        const double WLb = -Sb / d;
+        ap[q] += WLb;
+        agp[q] += dpc * WLb;
+        dpLb += WLb * gpc[q];
+        dpRb -= Sb / d * gpo;
+      } else {
+        const double WRb = Sb / d;
+        ap[q] += WRb;
+        agp[q] -= dpc * WRb;
+        dpLb += -Sb / d * gpo;
+        dpRb -= WRb * gpc[q];
+      }
+    }
+    const double wb = a.fr[AFR_WB * NF + gi] + a.fr[(AFR_WB + 1) * NF + gi];
+    double vin, vout;
+    // dpL = cm - (tL - dL/2), dpR = (tR - dR/2) - cm
+    face_rot_split(m.ftype[gi], wb, dpLb - dpRb, -dpLb, dpRb, &vin, &vout);
+    rot += asL ? vin : vout;
+  };
+  if (pR >= 0) faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) { face(PI, gi, false); });
+  if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) { face(PI, gi, true); });
   for (int q = 0; q < 5; ++q) {
-    const double Sb = a.aSL[q * m.NFtot + g] + a.aSR[q * m.NFtot + g];
-    const double WRb = Sb / d, WLb = -Sb / d;
-    pLb[q] = WLb;
-    gpLb[q] = dpL * WLb;
-    pRb[q] = WRb;
-    gpRb[q] = -dpR * WRb;
-    dpLb += WLb * m.gradp[q * m.Ntot + cL];
-    dpRb -= WRb * m.gradp[q * m.Ntot + cR];
+    a.aP[q * m.Ntot + z] += ap[q];
+    a.agradp[q * m.Ntot + z] += agp[q];
   }
-  // dpL = cm - (tL - dL/2), dpR = (tR - dR/2) - cm
-  face_geo_to_rot(m, a, g, cL, cR, 0.0, dpLb - dpRb, -dpLb, dpRb);
-  add5(a.aP, m.Ntot, cL, pLb);
-  add5(a.agradp, m.Ntot, cL, gpLb);
-  add5(a.aP, m.Ntot, cR, pRb);
-  add5(a.agradp, m.Ntot, cR, gpRb);
+  a.arot[z] += rot;
 }
 
-// phi slope backward (k_plm_p)
+// phi slope backward (k_plm_p), gathered: cell c receives the weight on it
+// of its own slope and of its two neighbours' slopes
+__device__ __forceinline__ void plm_p_cell(const DevMesh &m, const AnnInfo &A, double plm, int j, int q,
+                                           double *wL, double *wc, double *wR) {
+  const int N = A.N;
+  const long long c = A.zbase + j;
+  const long long cL = A.zbase + (j == 0 ? N - 1 : j - 1);
+  const long long cR = A.zbase + (j == N - 1 ? 0 : j + 1);
+  const double dpL = .5 * (m.dphi[c] + m.dphi[cL]);
+  const double dpR = .5 * (m.dphi[c] + m.dphi[cR]);
+  plm_p_weights(m.prim[q * m.Ntot + cL], m.prim[q * m.Ntot + c], m.prim[q * m.Ntot + cR], dpL, dpR, plm, wL, wc, wR);
+}
 __global__ __launch_bounds__(TILE) void ka_plm_p(DevMesh m, AdjMesh a, const int2 *tiles) {
   const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
@@ -371,24 +594,18 @@ __global__ __launch_bounds__(TILE) void ka_plm_p(DevMesh m, AdjMesh a, const int
   if (j >= A.N) return;
   const DiskParams &P = disk_of(m, A.disk);
   const int N = A.N;
+  const int jl = j == 0 ? N - 1 : j - 1, jr = j == N - 1 ? 0 : j + 1;
   const long long c = A.zbase + j;
-  const long long cL = A.zbase + (j == 0 ? N - 1 : j - 1);
-  const long long cR = A.zbase + (j == N - 1 ? 0 : j + 1);
-  const double dpL = .5 * (m.dphi[c] + m.dphi[cL]);
-  const double dpR = .5 * (m.dphi[c] + m.dphi[cR]);
-  double bL[5], bc[5], bR[5];
   for (int q = 0; q < 5; ++q) {
-    const double gb = a.agradp[q * m.Ntot + c];
-    double wL, wc, wR;
-    plm_p_weights(m.prim[q * m.Ntot + cL], m.prim[q * m.Ntot + c], m.prim[q * m.Ntot + cR], dpL, dpR, P.plm, &wL, &wc,
-                  &wR);
-    bL[q] = wL * gb;
-    bc[q] = wc * gb;
-    bR[q] = wR * gb;
+    double wL, wc, wR, s;
+    plm_p_cell(m, A, P.plm, j, q, &wL, &wc, &wR);
+    s = wc * a.agradp[q * m.Ntot + c];
+    plm_p_cell(m, A, P.plm, jl, q, &wL, &wc, &wR);  // left neighbour: its right weight
+    s += wR * a.agradp[q * m.Ntot + A.zbase + jl];
+    plm_p_cell(m, A, P.plm, jr, q, &wL, &wc, &wR);  // right neighbour: its left weight
+    s += wL * a.agradp[q * m.Ntot + A.zbase + jr];
+    a.aP[q * m.Ntot + c] += s;
   }
-  add5(a.aP, m.Ntot, cL, bL);
-  add5(a.aP, m.Ntot, c, bc);
-  add5(a.aP, m.Ntot, cR, bR);
 }
 
 // CFL step backward.  The step's dt is the minimum of the interior cells'
@@ -505,13 +722,20 @@ void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan
   const dim3 B(TILE);
   if (L.n_bc1) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc1), B, 0, st, m, a, L.bc1);
   if (L.n_bc0) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc0), B, 0, st, m, a, L.bc0);
-  hipLaunchKernelGGL(ka_update, dim3(L.n_ztiles), B, 0, st, m, a, L.ztiles);
-  hipLaunchKernelGGL(ka_phi, dim3(L.n_ztiles), B, 0, st, m, a, L.ztiles, L.n_ztiles);
-  if (L.n_rftiles) hipLaunchKernelGGL(ka_rflux, dim3(L.n_rftiles), B, 0, st, m, a, L.rftiles, 2 * L.n_ztiles);
-  hipLaunchKernelGGL(ka_plm_r, dim3(L.n_ztiles), B, 0, st, m, a, L.ztiles);
-  if (L.n_rftiles) hipLaunchKernelGGL(ka_slope, dim3(L.n_rftiles), B, 0, st, m, a, L.rftiles);
-  hipLaunchKernelGGL(ka_plm_p, dim3(L.n_ztiles), B, 0, st, m, a, L.ztiles);
-  hipLaunchKernelGGL(ka_reduce, dim3(1), B, 0, st, a.part, 2 * L.n_ztiles + L.n_rftiles, out);
+  // part slots: ka_update [0, nz), ka_phi_h [nz, nz+nu), ka_rflux_h [nz+nu, nz+nu+nrf),
+  // and with viscosity ka_phi_v and ka_rflux_v after them
+  const int nz = L.n_ztiles, nu = L.n_utiles, nrf = L.n_rftiles;
+  hipLaunchKernelGGL(ka_update, dim3(nz), B, 0, st, m, a, L.ztiles);
+  hipLaunchKernelGGL(ka_phi_h, dim3(nu), B, 0, st, m, a, L.utiles, nz);
+  if (nrf) hipLaunchKernelGGL(ka_rflux_h, dim3(nrf), B, 0, st, m, a, L.rftiles, nz + nu);
+  if (a.visc) {
+    hipLaunchKernelGGL(ka_phi_v, dim3(nu), B, 0, st, m, a, L.utiles, nz + nu + nrf);
+    if (nrf) hipLaunchKernelGGL(ka_rflux_v, dim3(nrf), B, 0, st, m, a, L.rftiles, nz + 2 * nu + nrf);
+  }
+  hipLaunchKernelGGL(ka_plm_r, dim3(nz), B, 0, st, m, a, L.ztiles);
+  hipLaunchKernelGGL(ka_slope, dim3(nz), B, 0, st, m, a, L.ztiles);
+  hipLaunchKernelGGL(ka_plm_p, dim3(nz), B, 0, st, m, a, L.ztiles);
+  hipLaunchKernelGGL(ka_reduce, dim3(1), B, 0, st, a.part, a.visc ? nz + 2 * nu + 2 * nrf : nz + nu + nrf, out);
 }
 
 void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,

@@ -138,10 +138,10 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   const size_t nz = (size_t)Ntot, nq = (size_t)NQ * Ntot, nf = (size_t)NQ * NFtot;
   const size_t wA = nq + 3 * nz, wB = 2 * nq + 3 * nz;  // doubles per snapshot
   const int nseg = (nsteps + every - 1) / every;
-  const int nz_t = plan.n_ztiles, nrf_t = plan.n_rftiles, ni_t = plan.n_itiles;
-  const size_t npart = std::max((size_t)(2 * nz_t + nrf_t), (size_t)ni_t);
+  const int nz_t = plan.n_ztiles, nu_t = plan.n_utiles, nrf_t = plan.n_rftiles, ni_t = plan.n_itiles;
+  const size_t npart = std::max((size_t)(nz_t + 2 * nu_t + 2 * nrf_t), (size_t)ni_t);
 
-  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *usave, *torque, *acc, *ckpt, *snapA, *snapB;
+  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr, *usave, *torque, *acc, *ckpt, *snapA, *snapB;
   double *arot, *rotout;
   signed char *ftype;
   int *pick;
@@ -149,7 +149,8 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   DiskScalars *sc_tmp;
   bool good = dalloc0(&aP, nq, "adjoint") && dalloc0(&aPn, nq, "adjoint") && dalloc0(&aR, nq, "adjoint") &&
               dalloc0(&au, nq, "adjoint") && dalloc0(&agrad, nq, "adjoint") && dalloc0(&agradp, nq, "adjoint") &&
-              dalloc0(&aSL, nf, "adjoint") && dalloc0(&aSR, nf, "adjoint") && dalloc0(&usave, nq, "adjoint") &&
+              dalloc0(&aSL, nf, "adjoint") && dalloc0(&aSR, nf, "adjoint") &&
+              dalloc0(&fr, (size_t)AFR_N * NFtot, "adjoint") && dalloc0(&usave, nq, "adjoint") &&
               dalloc0(&torque, (size_t)plan.n_utiles, "adjoint") && dalloc0(&acc, 1, "adjoint") &&
               dalloc0(&ckpt, wA * nseg, "adjoint checkpoints") && dalloc0(&snapA, wA * every, "adjoint steps") &&
               dalloc0(&snapB, wB * every, "adjoint steps") && dalloc0(&part, npart, "adjoint") &&
@@ -213,10 +214,12 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
       a.agradp = agradp;
       a.aSL = aSL;
       a.aSR = aSR;
+      a.fr = fr;
       a.part = part;
       a.wl = wl;
       a.lam_dt = 0.0;
       a.arot = arot;
+      a.visc = P.visc > 0.0;
       // substep 2 (fac 1/2, RK 1/2)
       adjoint_restore(snapB + wB * (n - n0), true);
       put_scalars(0, R.B);
@@ -302,7 +305,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   }
   *g_q = gq;
   for (void *p : {(void *)aP, (void *)aPn, (void *)aR, (void *)au, (void *)agrad, (void *)agradp, (void *)aSL,
-                  (void *)aSR, (void *)usave, (void *)torque, (void *)acc, (void *)ckpt, (void *)snapA,
+                  (void *)aSR, (void *)fr, (void *)usave, (void *)torque, (void *)acc, (void *)ckpt, (void *)snapA,
                   (void *)snapB, (void *)part, (void *)out, (void *)sc_tmp, (void *)arot, (void *)rotout,
                   (void *)ftype, (void *)pick})
     hipFree(p);

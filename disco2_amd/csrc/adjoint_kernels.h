@@ -10,15 +10,27 @@
 // aP: of its input primitives (written); aR: of RKcons (written by the
 // second substep, read by the first); au: of each cell's conserved state
 // before cons2prim; agrad / agradp: of the r and phi slopes; aSL / aSR: of
-// each face's slope S, from its L and its R cell.
+// each face's slope S, from its L and its R cell; fr: per r face, what its
+// flux and slope backward hand its two cells (AFR_* slots of NFtot each).
+enum {
+  AFR_HL = 0,    // Riemann part: adjoint of primL (5)
+  AFR_HR = 5,    // and of primR (5)
+  AFR_WL = 10,   // viscous part: adjoint of W_L (5)
+  AFR_WR = 15,   // and of W_R (5)
+  AFR_G = 20,    // viscous part: to both cells' phi slopes of v_r, omega (2)
+  AFR_ROT = 22,  // flux parts: to the L and the R cell's annulus rotation (2)
+  AFR_WB = 24,   // r slope sums: adjoint of the face width from the L and the R cell (2)
+  AFR_N = 26
+};
 struct AdjMesh {
-  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR;
+  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr;
   ScalarAdj *part;     // per-tile partial sums of the scalar adjoints
   double wl;           // weight of the torque term in the loss (1/T)
   double fac, RK;      // the substep's dt factor and RK weight
   int copy_first;      // first substep: RKcons = prim2cons(prim)
   double lam_dt;       // adjoint of the step's dt (CFL backward)
   double *arot;        // per cell: adjoint of its annulus's rotation angle (summed per annulus)
+  int visc;            // the disk is viscous: launch the viscous face backward kernels
 };
 
 void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
