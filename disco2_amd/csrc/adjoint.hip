@@ -179,8 +179,8 @@ __global__ __launch_bounds__(TILE) void ka_update(DevMesh m, AdjMesh a, const in
       if (a.copy_first) {
         u0b[q] = ub[q] + a.aR[q * m.Ntot + c];  // u = u0 (RK = 0) and RKcons = u0
       } else {
-        u0b[q] = (1.0 - a.RK) * ub[q];
-        a.aR[q * m.Ntot + c] = a.RK * ub[q];
+        u0b[q] = (1.0 - sc.RK) * ub[q];
+        a.aR[q * m.Ntot + c] = sc.RK * ub[q];
       }
     }
     vjp_prim2cons(pr, r, .5 * (rp * rp - rm * rm) * dphi * dz, P.gamma, u0b, prb);
@@ -631,8 +631,9 @@ __device__ __forceinline__ double cfl_cell(const DevMesh &m, const int2 tl, cons
   *tc_out = tc;
   return cell_dt_value(P, sc, pr, r, dx, w, tc);
 }
-__global__ __launch_bounds__(TILE) void ka_dt_pick(DevMesh m, const int2 *tiles, unsigned long long dtbits,
+__global__ __launch_bounds__(TILE) void ka_dt_pick(DevMesh m, const int2 *tiles, const unsigned long long *dtp,
                                                    int *pick) {
+  const unsigned long long dtbits = *dtp;
   const int2 tl = tiles[xcd_order()];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
@@ -658,7 +659,7 @@ __global__ __launch_bounds__(TILE) void ka_dt(DevMesh m, AdjMesh a, const int2 *
     {
       double prb[5] = {0, 0, 0, 0, 0};
       double tcb = 0.0;
-      vjp_cell_dt(P, sc, pr, r, dx, w, tc, a.lam_dt, prb, sb, &tcb);
+      vjp_cell_dt(P, sc, pr, r, dx, w, tc, *a.lam_dt, prb, sb, &tcb);
       add5(a.aP, m.Ntot, c, prb);
       if (tcb != 0.0) atomicAdd(&a.arot[c], tcb);
     }
@@ -703,7 +704,10 @@ __global__ __launch_bounds__(TILE) void ka_rot(DevMesh m, AdjMesh a, double *out
   __shared__ double sh[TILE / 64];
   const AnnInfo A = m.ann[blockIdx.x];
   double s = 0.0;
-  for (int j = threadIdx.x; j < A.N; j += TILE) s += a.arot[A.zbase + j];
+  for (int j = threadIdx.x; j < A.N; j += TILE) {
+    s += a.arot[A.zbase + j];
+    a.arot[A.zbase + j] = 0.0;
+  }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -711,6 +715,69 @@ __global__ __launch_bounds__(TILE) void ka_rot(DevMesh m, AdjMesh a, double *out
     double t = 0.0;
     for (int w = 0; w < TILE / 64; ++w) t += sh[w];
     out[blockIdx.x] = t;
+  }
+}
+
+__global__ void ka_copy_scalars(DiskScalars *dst, const DiskScalars *src) {
+  const int n = sizeof(DiskScalars) / sizeof(double);
+  if ((int)threadIdx.x < n) ((double *)dst)[threadIdx.x] = ((const double *)src)[threadIdx.x];
+}
+
+// adjoints of a mass's phase and radius through xbh = r cos(phi), ybh = r sin(phi)
+__device__ __forceinline__ void chain_pos(const DiskScalars &sc, const ScalarAdj &s, double *lphi, double *lr) {
+  for (int p = 0; p < 2; ++p) {
+    lphi[p] = -sc.ybh[p] * s.xbh[p] + sc.xbh[p] * s.ybh[p];
+    const double r = sc.gm[p].r;
+    lr[p] = r != 0.0 ? (sc.xbh[p] * s.xbh[p] + sc.ybh[p] * s.ybh[p]) / r : 0.0;
+  }
+}
+
+// The step's dt enters: both substeps' dt_s (dt, dt/2), the phase the binary
+// advances (omega dt) and every annulus's turn (Omega_a dt, Omega_a = w_a / r_a,
+// cell_misc.c:73-75), and T (lamT).  lam_dt = 0 when dt was clipped (the CFL
+// minimum did not set it).
+__global__ __launch_bounds__(TILE) void ka_chain_dt(DevMesh m, int nann, const DiskScalars *A,
+                                                    const DiskScalars *B, const ScalarAdj *S1, const ScalarAdj *S2,
+                                                    const double *th2, const double *Ltheta, double lamT,
+                                                    AdjChain *cs) {
+  __shared__ double sh[TILE / 64];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nann; k += TILE) {
+    const AnnInfo &an = m.ann[k];
+    s += m.wiph[k] / (.5 * (an.rp + an.rm)) * (Ltheta[k] + th2[k]);
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < TILE / 64; ++w) t += sh[w];
+    double lphi2[2], lr2[2];
+    chain_pos(*B, *S2, lphi2, lr2);
+    const double omega = A->gm[0].omega;
+    const double lam = 1.0 * S1->dt + 0.5 * S2->dt + omega * (cs->Lphi[0] + cs->Lphi[1] + lphi2[0] + lphi2[1]) +
+                       lamT + t;
+    const double dtmin = __longlong_as_double((long long)A->dtmin_bits);
+    cs->lam_dt = A->dt == dtmin ? lam : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(TILE) void ka_chain_acc(int nann, const DiskScalars *pre, const DiskScalars *A,
+                                                     const DiskScalars *B, const ScalarAdj *S0, const ScalarAdj *S1,
+                                                     const ScalarAdj *S2, const double *th1, const double *th2,
+                                                     double *Ltheta, AdjChain *cs) {
+  for (int k = threadIdx.x; k < nann; k += TILE) Ltheta[k] += th1[k] + th2[k];
+  if (threadIdx.x == 0) {
+    double lphi0[2], lr0[2], lphi1[2], lr1[2], lphi2[2], lr2[2];
+    chain_pos(*pre, *S0, lphi0, lr0);
+    chain_pos(*A, *S1, lphi1, lr1);
+    chain_pos(*B, *S2, lphi2, lr2);
+    for (int p = 0; p < 2; ++p) {
+      cs->Lphi[p] += lphi1[p] + lphi2[p] + lphi0[p];
+      cs->Lr[p] += lr1[p] + lr2[p] + lr0[p];
+      cs->LM[p] += S1->M[p] + S2->M[p] + S0->M[p];
+    }
+    cs->Lalpha += S1->alpha + S2->alpha + S0->alpha;
   }
 }
 
@@ -739,7 +806,7 @@ void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan
 }
 
 void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
-                       unsigned long long dtbits, int *pick, ScalarAdj *out) {
+                       const unsigned long long *dtbits, int *pick, ScalarAdj *out) {
   if (!L.n_itiles) return;
   hipMemsetAsync(pick, 0x7f, sizeof(int), st);  // "no cell yet": a large positive index
   hipLaunchKernelGGL(ka_dt_pick, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles, dtbits, pick);
@@ -753,4 +820,22 @@ void launch_torque_sum(const double *tiles, int n, double *acc, hipStream_t st) 
 
 void launch_adjoint_rot(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st, double *out) {
   hipLaunchKernelGGL(ka_rot, dim3(L.nann), dim3(TILE), 0, st, m, a, out);
+}
+
+void launch_copy_scalars(DiskScalars *dst, const DiskScalars *src, hipStream_t st) {
+  static_assert(sizeof(DiskScalars) % sizeof(double) == 0 && sizeof(DiskScalars) <= 64 * sizeof(double),
+                "one wave copies the scalars");
+  hipLaunchKernelGGL(ka_copy_scalars, dim3(1), dim3(64), 0, st, dst, src);
+}
+
+void launch_adjoint_chain_dt(const DevMesh &m, int nann, const DiskScalars *A, const DiskScalars *B,
+                             const ScalarAdj *S1, const ScalarAdj *S2, const double *th2, const double *Ltheta,
+                             double lamT, AdjChain *cs, hipStream_t st) {
+  hipLaunchKernelGGL(ka_chain_dt, dim3(1), dim3(TILE), 0, st, m, nann, A, B, S1, S2, th2, Ltheta, lamT, cs);
+}
+
+void launch_adjoint_chain_acc(int nann, const DiskScalars *pre, const DiskScalars *A, const DiskScalars *B,
+                              const ScalarAdj *S0, const ScalarAdj *S1, const ScalarAdj *S2, const double *th1,
+                              const double *th2, double *Ltheta, AdjChain *cs, hipStream_t st) {
+  hipLaunchKernelGGL(ka_chain_acc, dim3(1), dim3(TILE), 0, st, nann, pre, A, B, S0, S1, S2, th1, th2, Ltheta, cs);
 }

@@ -16,6 +16,9 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cstdint>
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -31,17 +34,6 @@ bool dalloc0(T **p, size_t n, const char *what) {
 }
 
 }  // namespace
-
-DevMesh Engine::mesh_prev() const {
-  DevMesh m = mesh();
-  m.prim = d_prim[1 - cur];
-  m.prim_new = d_prim[cur];
-  m.tiph = d_tiph[1 - cur];
-  m.tiph_new = d_tiph[cur];
-  m.dphi = d_dphi[1 - cur];
-  m.dphi_new = d_dphi[cur];
-  return m;
-}
 
 bool Engine::adjoint_scope() {
   const DiskParams &P = hdp[0];
@@ -70,38 +62,44 @@ bool Engine::adjoint_scope() {
   return true;
 }
 
+void Engine::adjoint_redirect(const AdjSubRec &r) {
+  d_prim[1 - cur] = r.prim;
+  d_tiph[1 - cur] = r.tiph;
+  d_dphi[1 - cur] = r.dphi;
+  d_grad = r.grad;
+  d_gradp = r.gradp;
+  d_fL = r.fL;
+  d_fR = r.fR;
+  d_fdphi = r.fdphi;
+  d_fcm = r.fcm;
+  d_fo_r = r.fo_r;
+  d_run = r.run;
+  d_pst = r.pst;
+  adj_usave = r.usave;  // k_update records what cons2prim converts
+  adj_ftype = r.ftype;  // the face builder records the face types
+}
+
 // One RK2 step exactly as enqueue_rk2 launches it, with the pre-substep
-// scalar ops as their own launch so the state can be recorded between them.
-// snapA / snapB (when non-null) receive the state at the start of each
-// substep and rec its scalars; every k_update adds its torque to acc.
-void Engine::adjoint_step(AdjStepRec *rec, double *snapA, double *snapB, double *torque, double *acc) {
-  const size_t nz = (size_t)Ntot, nq = (size_t)NQ * Ntot;
-  auto snap = [&](double *dst, bool rk) {
-    hipMemcpyAsync(dst, d_prim[cur], nq * 8, hipMemcpyDeviceToDevice, st);
-    hipMemcpyAsync(dst + nq, d_tiph[cur], nz * 8, hipMemcpyDeviceToDevice, st);
-    hipMemcpyAsync(dst + nq + nz, d_dphi[cur], nz * 8, hipMemcpyDeviceToDevice, st);
-    hipMemcpyAsync(dst + nq + 2 * nz, d_RKtiph, nz * 8, hipMemcpyDeviceToDevice, st);
-    if (rk) hipMemcpyAsync(dst + nq + 3 * nz, d_RKcons, nq * 8, hipMemcpyDeviceToDevice, st);
-  };
+// scalar ops as their own launch so the scalars can be recorded between
+// them; every k_update adds its torque to acc.
+void Engine::adjoint_step(const AdjSubRec *rec, DiskScalars *screc, double *torque, double *acc) {
   DevMesh m = mesh();
   launch_set_w(m, plan, st);
   launch_mindt(m, plan, st, NQ);
-  if (rec) rec->pre = get_scalars(0);
+  if (screc) launch_copy_scalars(screc, d_sc, st);
   launch_scalar(m, plan, st, SC_STEP_BEGIN | SC_SUB_BEGIN);
-  if (rec) {
-    snap(snapA, false);
-    rec->A = get_scalars(0);
-  }
+  if (screc) launch_copy_scalars(screc + 1, d_sc, st);
   adj_torque = torque;
+  if (rec) adjoint_redirect(rec[0]);
   enqueue_substep(1.0, 1, 0, 0, SC_AFTER_UPDATE | SC_SUB1_END | SC_SUB_BEGIN);
   launch_torque_sum(torque, plan.n_utiles, acc, st);
-  if (rec) {
-    snap(snapB, true);
-    rec->B = get_scalars(0);
-  }
+  if (screc) launch_copy_scalars(screc + 2, d_sc, st);
+  if (rec) adjoint_redirect(rec[1]);
   enqueue_substep(0.5, 0, 1, 0, SC_AFTER_UPDATE | SC_SUB2_END);
   launch_torque_sum(torque, plan.n_utiles, acc, st);
   adj_torque = nullptr;
+  adj_usave = nullptr;
+  adj_ftype = nullptr;
 }
 
 void Engine::adjoint_restore(const double *src, bool rk) {
@@ -118,7 +116,7 @@ int Engine::torque_loss(int nsteps, double *loss) {
   double *torque = nullptr, *acc = nullptr;
   if (!dalloc0(&torque, (size_t)plan.n_utiles, "torque") || !dalloc0(&acc, 1, "torque sum")) return -1;
   const double t0 = get_scalars(0).t;
-  for (int n = 0; n < nsteps; ++n) adjoint_step(nullptr, nullptr, nullptr, torque, acc);
+  for (int n = 0; n < nsteps; ++n) adjoint_step(nullptr, nullptr, torque, acc);
   const double T = get_scalars(0).t - t0;
   double I = 0.0;
   hip_check(hipMemcpy(&I, acc, 8, hipMemcpyDeviceToHost), "torque D2H");
@@ -128,6 +126,23 @@ int Engine::torque_loss(int nsteps, double *loss) {
   return check_error() ? -1 : 0;
 }
 
+namespace {
+
+// device buffers of one adjoint run, carved from one allocation
+struct Arena {
+  char *base = nullptr;
+  size_t off = 0, cap = 0;
+  template <class T>
+  T *take(size_t n) {
+    off = (off + 255) & ~(size_t)255;
+    T *p = (T *)((uintptr_t)base + off);
+    off += (n ? n : 1) * sizeof(T);
+    return p;
+  }
+};
+
+}  // namespace
+
 int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double *g_q, double *g_prim0) {
   if (!ok || !adjoint_scope()) return -1;
   if (nsteps < 1) {
@@ -135,179 +150,247 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     return -1;
   }
   if (every < 1 || every > nsteps) every = nsteps;
-  const size_t nz = (size_t)Ntot, nq = (size_t)NQ * Ntot, nf = (size_t)NQ * NFtot;
-  const size_t wA = nq + 3 * nz, wB = 2 * nq + 3 * nz;  // doubles per snapshot
+  const size_t nz = (size_t)Ntot, nq = (size_t)NQ * Ntot, nf = (size_t)NFtot;
+  const size_t wA = nq + 3 * nz;  // doubles per checkpoint: prim, tiph, dphi, RKtiph
   const int nseg = (nsteps + every - 1) / every;
   const int nz_t = plan.n_ztiles, nu_t = plan.n_utiles, nrf_t = plan.n_rftiles, ni_t = plan.n_itiles;
   const size_t npart = std::max((size_t)(nz_t + 2 * nu_t + 2 * nrf_t), (size_t)ni_t);
+  const size_t npairs = hpairs.size();
 
-  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr, *usave, *torque, *acc, *ckpt, *snapA, *snapB;
-  double *arot, *rotout;
-  signed char *ftype;
+  // one allocation: the adjoint fields, the checkpoints, the record of one
+  // segment (2 * every substeps); laid out once to size it, once to carve it
+  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr, *arot, *torque, *acc, *ckpt, *fin, *th1, *th2,
+      *Ltheta;
+  DiskScalars *screc, *ckpt_sc, *fin_sc;
+  ScalarAdj *part;
+  AdjChain *cs;
   int *pick;
-  ScalarAdj *part, *out;
-  DiskScalars *sc_tmp;
-  bool good = dalloc0(&aP, nq, "adjoint") && dalloc0(&aPn, nq, "adjoint") && dalloc0(&aR, nq, "adjoint") &&
-              dalloc0(&au, nq, "adjoint") && dalloc0(&agrad, nq, "adjoint") && dalloc0(&agradp, nq, "adjoint") &&
-              dalloc0(&aSL, nf, "adjoint") && dalloc0(&aSR, nf, "adjoint") &&
-              dalloc0(&fr, (size_t)AFR_N * NFtot, "adjoint") && dalloc0(&usave, nq, "adjoint") &&
-              dalloc0(&torque, (size_t)plan.n_utiles, "adjoint") && dalloc0(&acc, 1, "adjoint") &&
-              dalloc0(&ckpt, wA * nseg, "adjoint checkpoints") && dalloc0(&snapA, wA * every, "adjoint steps") &&
-              dalloc0(&snapB, wB * every, "adjoint steps") && dalloc0(&part, npart, "adjoint") &&
-              dalloc0(&out, 1, "adjoint") && dalloc0(&sc_tmp, 1, "adjoint") && dalloc0(&arot, nz, "adjoint") &&
-              dalloc0(&rotout, (size_t)nann, "adjoint") && dalloc0(&ftype, (size_t)NFtot, "adjoint") && dalloc0(&pick, 1, "adjoint");
-  if (!good) return -1;
-  std::vector<DiskScalars> ckpt_sc(nseg);
-  std::vector<AdjStepRec> rec(every);
+  std::vector<AdjSubRec> rec(2 * every);
+  Arena ar;
+  auto layout = [&]() {
+    ar.off = 0;
+    aP = ar.take<double>(nq);
+    aPn = ar.take<double>(nq);
+    aR = ar.take<double>(nq);
+    au = ar.take<double>(nq);
+    agrad = ar.take<double>(nq);
+    agradp = ar.take<double>(nq);
+    aSL = ar.take<double>(NQ * nf);
+    aSR = ar.take<double>(NQ * nf);
+    fr = ar.take<double>((size_t)AFR_N * nf);
+    arot = ar.take<double>(nz);
+    torque = ar.take<double>((size_t)plan.n_utiles);
+    acc = ar.take<double>(1);
+    ckpt = ar.take<double>(wA * nseg);
+    fin = ar.take<double>(nseg > 1 ? wA : 0);
+    th1 = ar.take<double>((size_t)nann);
+    th2 = ar.take<double>((size_t)nann);
+    Ltheta = ar.take<double>((size_t)nann);
+    screc = ar.take<DiskScalars>((size_t)3 * every);
+    ckpt_sc = ar.take<DiskScalars>((size_t)nseg);
+    fin_sc = ar.take<DiskScalars>(1);
+    part = ar.take<ScalarAdj>(npart + 3);
+    cs = ar.take<AdjChain>(1);
+    pick = ar.take<int>(1);
+    for (auto &r : rec) {
+      r.prim = ar.take<double>(nq);
+      r.grad = ar.take<double>(nq);
+      r.gradp = ar.take<double>(nq);
+      r.usave = ar.take<double>(nq);
+      r.tiph = ar.take<double>(nz);
+      r.dphi = ar.take<double>(nz);
+      r.fdphi = ar.take<double>(nf);
+      r.fcm = ar.take<double>(nf);
+      r.fL = ar.take<int>(nf);
+      r.fR = ar.take<int>(nf);
+      r.fo_r = ar.take<int>(nz);
+      r.run = ar.take<int>((size_t)nrun);
+      r.pst = ar.take<PairState>(npairs);
+      r.ftype = ar.take<signed char>(nf);
+    }
+  };
+  layout();
+  ar.cap = ar.off;
+  if (hipMalloc((void **)&ar.base, ar.cap) != hipSuccess) {
+    (void)hipGetLastError();
+    disco_set_error(13, "adjoint: the record of a segment (" + std::to_string(ar.cap >> 20) +
+                            " MiB) does not fit in device memory; lower checkpoint_every");
+    return -1;
+  }
+  layout();
+  if (!hip_check(hipMemsetAsync(ar.base, 0, ar.cap, st), "adjoint")) {
+    hipFree(ar.base);
+    return -1;
+  }
+  ScalarAdj *out = part + npart;  // S2, S1, S0 of the current step
 
-  // forward: the loss, with a checkpoint at the start of every segment
+  // the engine's own buffers, pointed back at after each recorded segment
+  struct Own {
+    double *prim[2], *tiph[2], *dphi[2], *grad, *gradp, *fdphi, *fcm;
+    int *fL, *fR, *fo_r, *run;
+    PairState *pst;
+  } own = {{d_prim[0], d_prim[1]}, {d_tiph[0], d_tiph[1]}, {d_dphi[0], d_dphi[1]}, d_grad, d_gradp, d_fdphi, d_fcm,
+           d_fL, d_fR, d_fo_r, d_run, d_pst};
+  auto end_record = [&]() {
+    hipMemcpyAsync(own.prim[cur], d_prim[cur], nq * 8, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(own.tiph[cur], d_tiph[cur], nz * 8, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(own.dphi[cur], d_dphi[cur], nz * 8, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(own.grad, d_grad, nq * 8, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(own.gradp, d_gradp, nq * 8, hipMemcpyDeviceToDevice, st);
+    for (int b = 0; b < 2; ++b) {
+      d_prim[b] = own.prim[b];
+      d_tiph[b] = own.tiph[b];
+      d_dphi[b] = own.dphi[b];
+    }
+    d_grad = own.grad;
+    d_gradp = own.gradp;
+    d_fdphi = own.fdphi;
+    d_fcm = own.fcm;
+    d_fL = own.fL;
+    d_fR = own.fR;
+    d_fo_r = own.fo_r;
+    d_run = own.run;
+    d_pst = own.pst;
+  };
+  auto snapshot = [&](double *c) {
+    hipMemcpyAsync(c, d_prim[cur], nq * 8, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(c + nq, d_tiph[cur], nz * 8, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(c + nq + nz, d_dphi[cur], nz * 8, hipMemcpyDeviceToDevice, st);
+    hipMemcpyAsync(c + nq + 2 * nz, d_RKtiph, nz * 8, hipMemcpyDeviceToDevice, st);
+  };
+
+  // forward: the loss, a checkpoint at the start of every segment, and the
+  // record of the last segment
+  const int last0 = (nseg - 1) * every;
   const double t0 = get_scalars(0).t;
   for (int n = 0; n < nsteps; ++n) {
     if (n % every == 0) {
-      double *c = ckpt + wA * (n / every);
-      hipMemcpyAsync(c, d_prim[cur], nq * 8, hipMemcpyDeviceToDevice, st);
-      hipMemcpyAsync(c + nq, d_tiph[cur], nz * 8, hipMemcpyDeviceToDevice, st);
-      hipMemcpyAsync(c + nq + nz, d_dphi[cur], nz * 8, hipMemcpyDeviceToDevice, st);
-      hipMemcpyAsync(c + nq + 2 * nz, d_RKtiph, nz * 8, hipMemcpyDeviceToDevice, st);
-      ckpt_sc[n / every] = get_scalars(0);
+      snapshot(ckpt + wA * (n / every));
+      launch_copy_scalars(ckpt_sc + n / every, d_sc, st);
     }
-    adjoint_step(nullptr, nullptr, nullptr, torque, acc);
+    if (n >= last0)
+      adjoint_step(&rec[2 * (n - last0)], screc + 3 * (n - last0), torque, acc);
+    else
+      adjoint_step(nullptr, nullptr, torque, acc);
   }
+  end_record();
   const double T = get_scalars(0).t - t0;
   double I = 0.0;
   hip_check(hipMemcpy(&I, acc, 8, hipMemcpyDeviceToHost), "torque D2H");
-  if (check_error() || !(T > 0.0)) return -1;
+  auto release = [&]() { hipFree(ar.base); };
+  if (check_error() || !(T > 0.0)) {
+    release();
+    return -1;
+  }
   *loss = I / T;
   const double wl = 1.0 / T, lamT = -I / (T * T);
-  // each annulus turns by Omega = (wiph + rOm_a) / r per unit time
-  // (cell_update_phi, cell_misc.c:73-75; rOm_a = 0 here); wiph is analytic
-  // for the mesh motions the adjoint accepts
-  std::vector<double> wi(nann), Omega(nann), Ltheta(nann, 0.0), th1(nann), th2(nann);
-  hip_check(hipMemcpy(wi.data(), d_wiph, nann * 8, hipMemcpyDeviceToHost), "wiph D2H");
-  for (int a2 = 0; a2 < nann; ++a2) Omega[a2] = wi[a2] / (.5 * (hann[a2].rp + hann[a2].rm));
+  if (nseg > 1) {  // the state after nsteps, put back at the end
+    snapshot(fin);
+    launch_copy_scalars(fin_sc, d_sc, st);
+  }
+
+  // one substep's mesh: input state `in`, everything else from its record
+  const DevMesh base = mesh();
+  auto sub_mesh = [&](const AdjSubRec &in, const AdjSubRec &o, DiskScalars *sc) {
+    DevMesh m = base;
+    m.prim = in.prim;
+    m.tiph = in.tiph;
+    m.dphi = in.dphi;
+    m.prim_new = o.prim;
+    m.tiph_new = o.tiph;
+    m.dphi_new = o.dphi;
+    m.grad = o.grad;
+    m.gradp = o.gradp;
+    m.fL = o.fL;
+    m.fR = o.fR;
+    m.fdphi = o.fdphi;
+    m.fcm = o.fcm;
+    m.fo_r = o.fo_r;
+    m.run_start = o.run;
+    m.pst = o.pst;
+    m.u_save = o.usave;
+    m.ftype = o.ftype;
+    m.sc = sc;
+    m.torque_tile = nullptr;
+    return m;
+  };
 
   // reverse sweep
   const DiskParams &P = hdp[0];
-  double Lphi[2] = {0, 0}, Lr[2] = {0, 0}, LM[2] = {0, 0}, Lalpha = 0.0;
-  ScalarAdj S0, S1, S2;
-  auto fetch = [&](ScalarAdj &s) { hip_check(hipMemcpy(&s, out, sizeof(s), hipMemcpyDeviceToHost), "adjoint D2H"); };
-  // adjoints of the mass phases and radii through xbh = r cos(phi), ybh = r sin(phi)
-  auto chain_pos = [&](const DiskScalars &sc, const ScalarAdj &s, double *lphi, double *lr) {
-    for (int p = 0; p < 2; ++p) {
-      lphi[p] = -sc.ybh[p] * s.xbh[p] + sc.xbh[p] * s.ybh[p];
-      const double r = sc.gm[p].r;
-      lr[p] = r != 0.0 ? (sc.xbh[p] * s.xbh[p] + sc.ybh[p] * s.ybh[p]) / r : 0.0;
-    }
-  };
+  AdjMesh a;
+  a.aR = aR;
+  a.au = au;
+  a.agrad = agrad;
+  a.agradp = agradp;
+  a.aSL = aSL;
+  a.aSR = aSR;
+  a.fr = fr;
+  a.part = part;
+  a.wl = wl;
+  a.lam_dt = &cs->lam_dt;
+  a.arot = arot;
+  a.visc = P.visc > 0.0;
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int n0 = seg * every, n1 = std::min(nsteps, n0 + every);
-    adjoint_restore(ckpt + wA * seg, false);
-    put_scalars(0, ckpt_sc[seg]);
-    for (int n = n0; n < n1; ++n)
-      adjoint_step(&rec[n - n0], snapA + wA * (n - n0), snapB + wB * (n - n0), torque, acc);
+    if (seg != nseg - 1) {  // recompute the segment from its checkpoint, recording it
+      adjoint_restore(ckpt + wA * seg, false);
+      launch_copy_scalars(d_sc, ckpt_sc + seg, st);
+      for (int n = n0; n < n1; ++n) adjoint_step(&rec[2 * (n - n0)], screc + 3 * (n - n0), torque, acc);
+      end_record();
+    }
+    AdjSubRec start = rec[0];
+    start.prim = ckpt + wA * seg;
+    start.tiph = start.prim + nq;
+    start.dphi = start.tiph + nz;
     for (int n = n1 - 1; n >= n0; --n) {
-      const AdjStepRec &R = rec[n - n0];
-      AdjMesh a;
-      a.aR = aR;
-      a.au = au;
-      a.agrad = agrad;
-      a.agradp = agradp;
-      a.aSL = aSL;
-      a.aSR = aSR;
-      a.fr = fr;
-      a.part = part;
-      a.wl = wl;
-      a.lam_dt = 0.0;
-      a.arot = arot;
-      a.visc = P.visc > 0.0;
-      // substep 2 (fac 1/2, RK 1/2)
-      adjoint_restore(snapB + wB * (n - n0), true);
-      put_scalars(0, R.B);
-      adj_usave = usave;  // k_update records what cons2prim converts
-      adj_ftype = ftype;  // the face builder records the face types
-      enqueue_substep(0.5, 0, 1, 0, 0);
-      adj_usave = nullptr;
-      adj_ftype = nullptr;
-      DevMesh mb = mesh_prev();
-      mb.u_save = usave;
-      mb.ftype = ftype;
-      hipMemsetAsync(arot, 0, nz * 8, st);
+      const int k = n - n0;
+      DiskScalars *scP = screc + 3 * k, *scA = scP + 1, *scB = scP + 2;
+      const AdjSubRec &in1 = k == 0 ? start : rec[2 * k - 1];
+      // substep 2 (fac 1/2)
+      const DevMesh m2 = sub_mesh(rec[2 * k], rec[2 * k + 1], scB);
       a.aP = aP;
       a.aPn = aPn;
       a.fac = 0.5;
-      a.RK = R.B.RK;
       a.copy_first = 0;
-      launch_adjoint_substep(mb, a, plan, st, out);
-      launch_adjoint_rot(mb, a, plan, st, rotout);
-      fetch(S2);
-      hip_check(hipMemcpy(th2.data(), rotout, nann * 8, hipMemcpyDeviceToHost), "adjoint D2H");
+      launch_adjoint_substep(m2, a, plan, st, out + 0);
+      launch_adjoint_rot(m2, a, plan, st, th2);
       std::swap(aP, aPn);
-      // substep 1 (fac 1, RK 0, RKcons = prim2cons(prim))
-      adjoint_restore(snapA + wA * (n - n0), false);
-      put_scalars(0, R.A);
-      adj_usave = usave;
-      adj_ftype = ftype;
-      enqueue_substep(1.0, 1, 0, 0, 0);
-      adj_usave = nullptr;
-      adj_ftype = nullptr;
-      mb = mesh_prev();
-      mb.u_save = usave;
-      mb.ftype = ftype;
-      hipMemsetAsync(arot, 0, nz * 8, st);
+      // substep 1 (fac 1, RKcons = prim2cons(prim))
+      const DevMesh m1 = sub_mesh(in1, rec[2 * k], scA);
       a.aP = aP;
       a.aPn = aPn;
       a.fac = 1.0;
-      a.RK = R.A.RK;
       a.copy_first = 1;
-      launch_adjoint_substep(mb, a, plan, st, out);
-      fetch(S1);
+      launch_adjoint_substep(m1, a, plan, st, out + 1);
       std::swap(aP, aPn);
-      // dt of the step: both substeps' dt_s, the phase it advances, T
-      const double omega = R.A.gm[0].omega;
-      double lphi1[2], lr1[2], lphi2[2], lr2[2], lphi0[2] = {0, 0}, lr0[2] = {0, 0};
-      chain_pos(R.A, S1, lphi1, lr1);
-      chain_pos(R.B, S2, lphi2, lr2);
-      double lam_dt = 1.0 * S1.dt + 0.5 * S2.dt + omega * (Lphi[0] + Lphi[1] + lphi2[0] + lphi2[1]) + lamT;
-      for (int a2 = 0; a2 < nann; ++a2) lam_dt += Omega[a2] * (Ltheta[a2] + th2[a2]);
-      memset(&S0, 0, sizeof(S0));
-      double dtmin;
-      memcpy(&dtmin, &R.A.dtmin_bits, 8);
-      if (R.A.dt == dtmin) {  // not clipped at T_MAX
-        hip_check(hipMemcpyAsync(sc_tmp, &R.pre, sizeof(DiskScalars), hipMemcpyHostToDevice, st), "adjoint");
-        DevMesh md = mesh_prev();  // step-start state, scalars as k_mindt saw them
-        md.sc = sc_tmp;
-        md.ftype = ftype;
-        a.aP = aPn;  // the step-start adjoint
-        a.lam_dt = lam_dt;
-        launch_adjoint_dt(md, a, plan, st, R.A.dtmin_bits, pick, out);
-        fetch(S0);
-        chain_pos(R.pre, S0, lphi0, lr0);
-      }
-      launch_adjoint_rot(mb, a, plan, st, rotout);  // substep 1 and the CFL step
-      hip_check(hipMemcpy(th1.data(), rotout, nann * 8, hipMemcpyDeviceToHost), "adjoint D2H");
-      for (int a2 = 0; a2 < nann; ++a2) Ltheta[a2] += th1[a2] + th2[a2];
-      for (int p = 0; p < 2; ++p) {
-        Lphi[p] += lphi1[p] + lphi2[p] + lphi0[p];
-        Lr[p] += lr1[p] + lr2[p] + lr0[p];
-        LM[p] += S1.M[p] + S2.M[p] + S0.M[p];
-      }
-      Lalpha += S1.alpha + S2.alpha + S0.alpha;
+      // the step's dt: its CFL minimum, at the step-start state with the
+      // scalars k_mindt saw
+      launch_adjoint_chain_dt(m1, nann, scA, scB, out + 1, out + 0, th2, Ltheta, lamT, cs, st);
+      DevMesh md = m1;
+      md.sc = scP;
+      a.aP = aPn;  // the step-start adjoint
+      launch_adjoint_dt(md, a, plan, st, &scA->dtmin_bits, pick, out + 2);
+      launch_adjoint_rot(m1, a, plan, st, th1);  // substep 1 and the CFL step
+      launch_adjoint_chain_acc(nann, scP, scA, scB, out + 2, out + 1, out + 0, th1, th2, Ltheta, cs, st);
     }
   }
+  AdjChain C;
+  hip_check(hipMemcpy(&C, cs, sizeof(C), hipMemcpyDeviceToHost), "adjoint D2H");
   hip_check(hipMemcpy(g_prim0, aPn, nq * 8, hipMemcpyDeviceToHost), "adjoint D2H");
-  *g_alpha = Lalpha;
+  if (nseg > 1) {
+    adjoint_restore(fin, false);
+    launch_copy_scalars(d_sc, fin_sc, st);
+  }
+  *g_alpha = C.Lalpha;
   // M0 = 1/(1+q), M1 = q/(1+q), r0 = M1, r1 = M0 (GravMass_init.c:53-84; the
   // binary keeps them), so each derivative is +-1/(1+q)^2
   double gq = 0.0;
   if (P.grav_type == 2) {
     const double q = sims[0]->MassRatio;
-    gq = (Lr[0] - Lr[1] - LM[0] + LM[1]) / ((1. + q) * (1. + q));
+    gq = (C.Lr[0] - C.Lr[1] - C.LM[0] + C.LM[1]) / ((1. + q) * (1. + q));
   }
   *g_q = gq;
-  for (void *p : {(void *)aP, (void *)aPn, (void *)aR, (void *)au, (void *)agrad, (void *)agradp, (void *)aSL,
-                  (void *)aSR, (void *)fr, (void *)usave, (void *)torque, (void *)acc, (void *)ckpt, (void *)snapA,
-                  (void *)snapB, (void *)part, (void *)out, (void *)sc_tmp, (void *)arot, (void *)rotout,
-                  (void *)ftype, (void *)pick})
-    hipFree(p);
-  return check_error() ? -1 : 0;
+  const int e = check_error();
+  release();
+  return e ? -1 : 0;
 }

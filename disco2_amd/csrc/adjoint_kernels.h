@@ -26,18 +26,35 @@ struct AdjMesh {
   double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr;
   ScalarAdj *part;     // per-tile partial sums of the scalar adjoints
   double wl;           // weight of the torque term in the loss (1/T)
-  double fac, RK;      // the substep's dt factor and RK weight
+  double fac;          // the substep's dt factor (its RK weight is m.sc's)
   int copy_first;      // first substep: RKcons = prim2cons(prim)
-  double lam_dt;       // adjoint of the step's dt (CFL backward)
+  const double *lam_dt;  // device: adjoint of the step's dt (CFL backward)
   double *arot;        // per cell: adjoint of its annulus's rotation angle (summed per annulus)
   int visc;            // the disk is viscous: launch the viscous face backward kernels
 };
 
+// reverse-sweep sums of the adjoints of the launch-uniform scalars: the
+// masses' phases and radii, their masses, alpha; lam_dt of the current step
+struct AdjChain {
+  double Lphi[2], Lr[2], LM[2], Lalpha, lam_dt;
+};
+
 void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
                             ScalarAdj *out);
-// pick: device int scratch for the index of the cell that sets dt
+// pick: device int scratch for the index of the cell that sets dt; dtbits:
+// the step's CFL minimum (device)
 void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
-                       unsigned long long dtbits, int *pick, ScalarAdj *out);
+                       const unsigned long long *dtbits, int *pick, ScalarAdj *out);
 void launch_torque_sum(const double *tiles, int n, double *acc, hipStream_t st);
-// per annulus sum of a.arot into out[annulus]
+// per annulus sum of a.arot into out[annulus]; clears a.arot
 void launch_adjoint_rot(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st, double *out);
+void launch_copy_scalars(DiskScalars *dst, const DiskScalars *src, hipStream_t st);
+// lam_dt of a step from its substeps' scalar adjoints S1, S2, the second
+// substep's annulus rotation adjoints th2 and the sums so far (m: annuli, wiph)
+void launch_adjoint_chain_dt(const DevMesh &m, int nann, const DiskScalars *A, const DiskScalars *B,
+                             const ScalarAdj *S1, const ScalarAdj *S2, const double *th2, const double *Ltheta,
+                             double lamT, AdjChain *cs, hipStream_t st);
+// adds a step's contributions (S0: the CFL step's) to the sums
+void launch_adjoint_chain_acc(int nann, const DiskScalars *pre, const DiskScalars *A, const DiskScalars *B,
+                              const ScalarAdj *S0, const ScalarAdj *S1, const ScalarAdj *S2, const double *th1,
+                              const double *th2, double *Ltheta, AdjChain *cs, hipStream_t st);

@@ -175,6 +175,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   }
   Ntot = zb;
   NFtot = fb;
+  nrun = rb > 0 ? rb : 1;
   nann = (int)hann.size();
 
   bool good = true;

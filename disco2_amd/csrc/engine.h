@@ -8,10 +8,15 @@
 #include "disco_internal.h"
 #include "kernels.h"
 
-// scalars of one RK2 step the adjoint records: as k_mindt saw them, at the
-// start of substep 1 (after the step's scalar ops) and of substep 2
-struct AdjStepRec {
-  DiskScalars pre, A, B;
+// What the adjoint keeps of one substep: the state it produced (ghosts
+// included) and everything its backward reads.  The forward writes these
+// directly (the engine's buffers are pointed at them while it records).
+struct AdjSubRec {
+  double *prim, *tiph, *dphi;
+  double *grad, *gradp, *usave, *fdphi, *fcm;
+  int *fL, *fR, *fo_r, *run;
+  PairState *pst;
+  signed char *ftype;
 };
 
 struct Engine {
@@ -22,6 +27,7 @@ struct Engine {
   std::vector<PairInfo> hpairs;
   std::vector<long long> zbase, nzones;
   long long Ntot = 0, NFtot = 0;
+  long long nrun = 1;  // run_start slots
   int nann = 0, n_rpairs = 0, n_zpairs = 0;
 
   // device allocations
@@ -71,9 +77,11 @@ struct Engine {
   int check_error();                 // 0, or sets the sticky library error
 
   // adjoint (adjoint_host.cpp)
-  DevMesh mesh_prev() const;         // after a substep: its input as prim/tiph/dphi, output as *_new
   bool adjoint_scope();
-  void adjoint_step(AdjStepRec *rec, double *snapA, double *snapB, double *torque, double *acc);
+  // one RK2 step; with rec (two substep records) and screc (three scalar
+  // records: as k_mindt saw them, at the start of substep 1 and 2) it records
+  void adjoint_step(const AdjSubRec *rec, DiskScalars *screc, double *torque, double *acc);
+  void adjoint_redirect(const AdjSubRec &r);  // the next substep writes into r
   void adjoint_restore(const double *src, bool rk);
   int torque_loss(int nsteps, double *loss);
   int adjoint(int nsteps, int every, double *loss, double *g_alpha, double *g_q, double *g_prim0);
