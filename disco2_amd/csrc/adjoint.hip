@@ -30,7 +30,7 @@ __device__ __forceinline__ void add5(double *a, long long N, long long c, const 
 
 // workgroup sum of the scalar adjoints into one slot (every thread calls)
 __device__ void reduce_store(const ScalarAdj &s, ScalarAdj *out) {
-  __shared__ double sh[9][TILE / 64];
+  __shared__ double sh[9][1024 / 64];
   double v[9] = {s.dt, s.alpha, s.M[0], s.M[1], s.xbh[0], s.xbh[1], s.ybh[0], s.ybh[1], s.loss};
 #pragma unroll
   for (int k = 0; k < 9; ++k)
@@ -361,6 +361,11 @@ __global__ __launch_bounds__(TILE) void ka_rflux_h(DevMesh m, AdjMesh a, const i
     RFaceHydroAdj h;
     vjp_rface_hydro(P, c, F.geo, a.fac * m.sc[PI.disk].dt, F.Xb, sb, h);
     const long long NF = m.NFtot, g = F.g;
+    {  // the face slope, as k_face_slope stores it for k_plm_rz
+      double S[5];
+      face_slope<5>(m, PI, g, S);
+      for (int q = 0; q < 5; ++q) m.Fface[q * NF + g] = S[q];
+    }
     for (int q = 0; q < 5; ++q) {
       a.fr[(AFR_HL + q) * NF + g] = h.pLb[q];
       a.fr[(AFR_HR + q) * NF + g] = h.pRb[q];
@@ -404,9 +409,10 @@ __global__ __launch_bounds__(TILE) void ka_rflux_v(DevMesh m, AdjMesh a, const i
   reduce_store(sb, &a.part[slot0 + ti]);
 }
 
-// r slope backward (k_plm_rz, r direction).  First the cell gathers what
-// its r faces' flux adjoints hand it (primitives, r and phi slopes, its
-// annulus rotation).  Then it replays the sum and the sequential limiter and
+// r slope backward (k_plm_rz, r direction).  The cell gathers what its r
+// faces' flux adjoints hand it (primitives, r and phi slopes, its annulus
+// rotation) in the same walk that replays the slope sum; then it replays
+// the sequential limiter (on the face slopes ka_rflux_h left in Fface) and
 // hands each face its share of the slope's adjoint: dA/dAtot to every face
 // when the sum survived, PLM to the face whose S the limiter took last,
 // nothing when it was zeroed.  Each face gets one value from its L cell
@@ -424,55 +430,53 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
   const int pR = i >= 1 ? rpair_of(P, k, i - 1) : -1;
   const int pLf = i <= P.NR - 2 ? rpair_of(P, k, i) : -1;
   const bool visc = a.visc;
-  {
-    const double tc = clean_pi(m.tiph[z]), dc = m.dphi[z];
-    double ap[5] = {0, 0, 0, 0, 0}, ag[5] = {0, 0, 0, 0, 0}, agp[5] = {0, 0, 0, 0, 0}, rot = 0.0;
-    // as R: primR = p - g deltaR - gp dpR (hydro), W_R = p - dpR gp (viscous)
-    if (pR >= 0)
-      faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) {
-        const double dpR = wrap_pm(-m.fcm[gi] + (tc - .5 * dc));
-        for (int q = 0; q < 5; ++q) {
-          const double h = a.fr[(AFR_HR + q) * NF + gi];
-          const double s = visc ? h + a.fr[(AFR_WR + q) * NF + gi] : h;
-          ap[q] += s;
-          ag[q] -= PI.deltaR * h;
-          agp[q] -= dpR * s;
-        }
-        if (visc)
-          for (int kk = 0; kk < 2; ++kk) agp[URR + kk] += a.fr[(AFR_G + kk) * NF + gi];
-        rot += a.fr[(AFR_ROT + 1) * NF + gi];
-      });
-    // as L: primL = p + g deltaL + gp dpL, W_L = p + dpL gp
-    if (pLf >= 0)
-      faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) {
-        const double dpL = wrap_pm(m.fcm[gi] - (tc - .5 * dc));
-        for (int q = 0; q < 5; ++q) {
-          const double h = a.fr[(AFR_HL + q) * NF + gi];
-          const double s = visc ? h + a.fr[(AFR_WL + q) * NF + gi] : h;
-          ap[q] += s;
-          ag[q] += PI.deltaL * h;
-          agp[q] += dpL * s;
-        }
-        if (visc)
-          for (int kk = 0; kk < 2; ++kk) agp[URR + kk] += a.fr[(AFR_G + kk) * NF + gi];
-        rot += a.fr[AFR_ROT * NF + gi];
-      });
-    for (int q = 0; q < 5; ++q) {
-      a.aP[q * m.Ntot + z] += ap[q];
-      a.agrad[q * m.Ntot + z] += ag[q];
-      a.agradp[q * m.Ntot + z] += agp[q];
-    }
-    a.arot[z] += rot;
-  }
+  const double tc = clean_pi(m.tiph[z]), dc = m.dphi[z];
+  double ap[5] = {0, 0, 0, 0, 0}, ag[5] = {0, 0, 0, 0, 0}, agp[5] = {0, 0, 0, 0, 0}, rot = 0.0;
   double g[5] = {0, 0, 0, 0, 0};
+  // the slope sum (faces as R, then as L, as k_plm_rz) and the gather
   auto acc = [&](const PairInfo &PI, long long gi) {
-    double S[5];
-    face_slope<5>(m, PI, gi, S);
     const double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
-    for (int q = 0; q < 5; ++q) g[q] += S[q] * dA;
+    for (int q = 0; q < 5; ++q) g[q] += m.Fface[q * NF + gi] * dA;
   };
-  if (pR >= 0) faces_as_R(m, pR, j, A.N, acc);
-  if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, acc);
+  // as R: primR = p - g deltaR - gp dpR (Riemann part), W_R = p - dpR gp (viscous)
+  if (pR >= 0)
+    faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) {
+      acc(PI, gi);
+      const double dpR = wrap_pm(-m.fcm[gi] + (tc - .5 * dc));
+      for (int q = 0; q < 5; ++q) {
+        const double h = a.fr[(AFR_HR + q) * NF + gi];
+        const double s = visc ? h + a.fr[(AFR_WR + q) * NF + gi] : h;
+        ap[q] += s;
+        ag[q] -= PI.deltaR * h;
+        agp[q] -= dpR * s;
+      }
+      if (visc)
+        for (int kk = 0; kk < 2; ++kk) agp[URR + kk] += a.fr[(AFR_G + kk) * NF + gi];
+      rot += a.fr[(AFR_ROT + 1) * NF + gi];
+    });
+  // as L: primL = p + g deltaL + gp dpL, W_L = p + dpL gp
+  if (pLf >= 0)
+    faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) {
+      acc(PI, gi);
+      const double dpL = wrap_pm(m.fcm[gi] - (tc - .5 * dc));
+      for (int q = 0; q < 5; ++q) {
+        const double h = a.fr[(AFR_HL + q) * NF + gi];
+        const double s = visc ? h + a.fr[(AFR_WL + q) * NF + gi] : h;
+        ap[q] += s;
+        ag[q] += PI.deltaL * h;
+        agp[q] += dpL * s;
+      }
+      if (visc)
+        for (int kk = 0; kk < 2; ++kk) agp[URR + kk] += a.fr[(AFR_G + kk) * NF + gi];
+      rot += a.fr[AFR_ROT * NF + gi];
+    });
+  double gb[5];
+  for (int q = 0; q < 5; ++q) {
+    a.aP[q * m.Ntot + z] += ap[q];
+    gb[q] = a.agrad[q * m.Ntot + z] + ag[q];
+    a.agradp[q * m.Ntot + z] += agp[q];
+  }
+  a.arot[z] += rot;
   const double dz = A.zp - A.zm, dp = m.dphi[z];
   double dAtot;
   if (i == 0)
@@ -486,14 +490,13 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
   int src[5] = {-1, -1, -1, -1, -1};
   int n = 0;
   auto lim = [&](const PairInfo &PI, long long gi) {
-    double S[5];
-    face_slope<5>(m, PI, gi, S);
     for (int q = 0; q < 5; ++q) {
-      if (S[q] * g[q] < 0.0) {
+      const double S = m.Fface[q * NF + gi];
+      if (S * g[q] < 0.0) {
         g[q] = 0.0;
         src[q] = -2;
-      } else if (fabs(PLM * S[q]) < fabs(g[q])) {
-        g[q] = PLM * S[q];
+      } else if (fabs(PLM * S) < fabs(g[q])) {
+        g[q] = PLM * S;
         src[q] = n;
       }
     }
@@ -501,16 +504,13 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
   };
   if (pR >= 0) faces_as_R(m, pR, j, A.N, lim);
   if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, lim);
-  double gb[5];
-  load5(a.agrad, m.Ntot, z, gb);
   n = 0;
   auto put = [&](const PairInfo &PI, long long gi, double *dst, int side) {
-    double S[5], wb = 0.0;
-    face_slope<5>(m, PI, gi, S);
     const double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
+    double wb = 0.0;
     for (int q = 0; q < 5; ++q) {
       dst[q * NF + gi] = src[q] == -1 ? dA / dAtot * gb[q] : (src[q] == n ? PLM * gb[q] : 0.0);
-      if (src[q] == -1) wb += gb[q] * S[q] / dAtot;  // d g / d dA_f
+      if (src[q] == -1) wb += gb[q] * m.Fface[q * NF + gi] / dAtot;  // d g / d dA_f
     }
     a.fr[(AFR_WB + side) * NF + gi] = wb * PI.r * PI.deltaPerp;
     ++n;
@@ -668,9 +668,10 @@ __global__ __launch_bounds__(TILE) void ka_dt(DevMesh m, AdjMesh a, const int2 *
 }
 
 // sum of n tile partials into *out (one workgroup)
-__global__ __launch_bounds__(TILE) void ka_reduce(const ScalarAdj *part, int n, ScalarAdj *out) {
+#define REDUCE_THREADS 1024
+__global__ __launch_bounds__(REDUCE_THREADS) void ka_reduce(const ScalarAdj *part, int n, ScalarAdj *out) {
   ScalarAdj s = {};
-  for (int i = threadIdx.x; i < n; i += TILE) {
+  for (int i = threadIdx.x; i < n; i += REDUCE_THREADS) {
     const ScalarAdj &p = part[i];
     s.dt += p.dt;
     s.alpha += p.alpha;
@@ -802,7 +803,8 @@ void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan
   hipLaunchKernelGGL(ka_plm_r, dim3(nz), B, 0, st, m, a, L.ztiles);
   hipLaunchKernelGGL(ka_slope, dim3(nz), B, 0, st, m, a, L.ztiles);
   hipLaunchKernelGGL(ka_plm_p, dim3(nz), B, 0, st, m, a, L.ztiles);
-  hipLaunchKernelGGL(ka_reduce, dim3(1), B, 0, st, a.part, a.visc ? nz + 2 * nu + 2 * nrf : nz + nu + nrf, out);
+  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(REDUCE_THREADS), 0, st, a.part,
+                     a.visc ? nz + 2 * nu + 2 * nrf : nz + nu + nrf, out);
 }
 
 void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
@@ -811,7 +813,7 @@ void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, 
   hipMemsetAsync(pick, 0x7f, sizeof(int), st);  // "no cell yet": a large positive index
   hipLaunchKernelGGL(ka_dt_pick, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles, dtbits, pick);
   hipLaunchKernelGGL(ka_dt, dim3(L.n_itiles), dim3(TILE), 0, st, m, a, L.itiles, (const int *)pick, 0);
-  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(TILE), 0, st, a.part, L.n_itiles, out);
+  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(REDUCE_THREADS), 0, st, a.part, L.n_itiles, out);
 }
 
 void launch_torque_sum(const double *tiles, int n, double *acc, hipStream_t st) {
