@@ -28,33 +28,24 @@ __device__ __forceinline__ void add5(double *a, long long N, long long c, const 
     if (v[q] != 0.0) atomicAdd(&a[q * N + c], v[q]);
 }
 
-// workgroup sum of the scalar adjoints into one slot (every thread calls)
-__device__ void reduce_store(const ScalarAdj &s, ScalarAdj *out) {
-  __shared__ double sh[9][1024 / 64];
-  double v[9] = {s.dt, s.alpha, s.M[0], s.M[1], s.xbh[0], s.xbh[1], s.ybh[0], s.ybh[1], s.loss};
+// workgroup sum of the scalar adjoints into slot `slot` of the partial sums
+// (every thread calls; row k of a.part holds field k of every slot)
+#define SA_FIELDS 9
+__device__ void reduce_store(const ScalarAdj &s, const AdjMesh &a, int slot) {
+  __shared__ double sh[SA_FIELDS][1024 / 64];
+  double v[SA_FIELDS] = {s.dt, s.alpha, s.M[0], s.M[1], s.xbh[0], s.xbh[1], s.ybh[0], s.ybh[1], s.loss};
 #pragma unroll
-  for (int k = 0; k < 9; ++k)
+  for (int k = 0; k < SA_FIELDS; ++k)
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
-    for (int k = 0; k < 9; ++k) sh[k][w] = v[k];
+    for (int k = 0; k < SA_FIELDS; ++k) sh[k][w] = v[k];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[9];
-    for (int k = 0; k < 9; ++k) {
-      t[k] = 0.0;
-      for (int i = 0; i < nw; ++i) t[k] += sh[k][i];
-    }
-    out->dt = t[0];
-    out->alpha = t[1];
-    out->M[0] = t[2];
-    out->M[1] = t[3];
-    out->xbh[0] = t[4];
-    out->xbh[1] = t[5];
-    out->ybh[0] = t[6];
-    out->ybh[1] = t[7];
-    out->loss = t[8];
+  if (threadIdx.x < SA_FIELDS) {
+    double t = 0.0;
+    for (int i = 0; i < nw; ++i) t += sh[threadIdx.x][i];
+    a.part[threadIdx.x * a.npart + slot] = t;
   }
 }
 
@@ -195,7 +186,7 @@ __global__ __launch_bounds__(TILE) void ka_update(DevMesh m, AdjMesh a, const in
     store5(a.agrad, m.Ntot, c, z);
     store5(a.agradp, m.Ntot, c, z);
   }
-  reduce_store(sb, &a.part[ti]);
+  reduce_store(sb, a, ti);
 }
 
 // phi face (j, j+1): cell j+1 received +X (+Y), cell j -X.
@@ -231,7 +222,11 @@ __device__ __forceinline__ void phi_face_at(const DevMesh &m, const AdjMesh &a, 
 // scalar adjoints, which counts every face once.
 
 // phi faces backward, Riemann part
-__global__ __launch_bounds__(TILE) void ka_phi_h(DevMesh m, AdjMesh a, const int2 *tiles, int slot0) {
+// minimum waves per SIMD of the Riemann-part face kernels (1: no bound)
+#ifndef ADJ_HYDRO_WAVES
+#define ADJ_HYDRO_WAVES 1
+#endif
+__global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_phi_h(DevMesh m, AdjMesh a, const int2 *tiles, int slot0) {
   const int ti = xcd_order();
   const int2 tl = tiles[ti];
   const AnnInfo A = m.ann[tl.x];
@@ -266,7 +261,7 @@ __global__ __launch_bounds__(TILE) void ka_phi_h(DevMesh m, AdjMesh a, const int
       a.agradp[q * m.Ntot + c] += .5 * dph * bL - .5 * dph * bR;
     }
   }
-  reduce_store(sb, &a.part[slot0 + ti]);
+  reduce_store(sb, a, slot0 + ti);
 }
 
 // phi faces backward, viscous part
@@ -310,7 +305,7 @@ __global__ __launch_bounds__(TILE) void ka_phi_v(DevMesh m, AdjMesh a, const int
     }
     a.arot[c] += sv[9][t + 1];
   }
-  reduce_store(sb, &a.part[slot0 + ti]);
+  reduce_store(sb, a, slot0 + ti);
 }
 
 // r face g of a pair: the R cell received +X (+Y/r_R), the L cell -X.
@@ -345,7 +340,7 @@ __device__ __forceinline__ void r_face_at(const DevMesh &m, const AdjMesh &a, co
 // The r-face backward kernels write what each face hands its two cells into
 // per-face slots of a.fr (q-major over faces, AFR_* offsets in units of
 // NFtot); ka_plm_r gathers them per cell over the cell's face ranges.
-__global__ __launch_bounds__(TILE) void ka_rflux_h(DevMesh m, AdjMesh a, const int2 *ftiles, int slot0) {
+__global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_rflux_h(DevMesh m, AdjMesh a, const int2 *ftiles, int slot0) {
   const int ti = xcd_order();
   const int2 tl = ftiles[ti];
   const int p = tl.x;
@@ -376,7 +371,7 @@ __global__ __launch_bounds__(TILE) void ka_rflux_h(DevMesh m, AdjMesh a, const i
     a.fr[AFR_ROT * NF + g] = vin;
     a.fr[(AFR_ROT + 1) * NF + g] = vout;
   }
-  reduce_store(sb, &a.part[slot0 + ti]);
+  reduce_store(sb, a, slot0 + ti);
 }
 
 __global__ __launch_bounds__(TILE) void ka_rflux_v(DevMesh m, AdjMesh a, const int2 *ftiles, int slot0) {
@@ -406,7 +401,7 @@ __global__ __launch_bounds__(TILE) void ka_rflux_v(DevMesh m, AdjMesh a, const i
     a.fr[AFR_ROT * NF + g] += vin;
     a.fr[(AFR_ROT + 1) * NF + g] += vout;
   }
-  reduce_store(sb, &a.part[slot0 + ti]);
+  reduce_store(sb, a, slot0 + ti);
 }
 
 // r slope backward (k_plm_rz, r direction).  The cell gathers what its r
@@ -664,25 +659,43 @@ __global__ __launch_bounds__(TILE) void ka_dt(DevMesh m, AdjMesh a, const int2 *
       if (tcb != 0.0) atomicAdd(&a.arot[c], tcb);
     }
   }
-  reduce_store(sb, &a.part[slot0 + ti]);
+  reduce_store(sb, a, slot0 + ti);
 }
 
 // sum of n tile partials into *out (one workgroup)
 #define REDUCE_THREADS 1024
-__global__ __launch_bounds__(REDUCE_THREADS) void ka_reduce(const ScalarAdj *part, int n, ScalarAdj *out) {
-  ScalarAdj s = {};
-  for (int i = threadIdx.x; i < n; i += REDUCE_THREADS) {
-    const ScalarAdj &p = part[i];
-    s.dt += p.dt;
-    s.alpha += p.alpha;
-    s.loss += p.loss;
-    for (int k = 0; k < 2; ++k) {
-      s.M[k] += p.M[k];
-      s.xbh[k] += p.xbh[k];
-      s.ybh[k] += p.ybh[k];
-    }
+__global__ __launch_bounds__(REDUCE_THREADS) void ka_reduce(const double *part, long long npart, int n,
+                                                             ScalarAdj *out) {
+  __shared__ double sh[SA_FIELDS][REDUCE_THREADS / 64];
+  double v[SA_FIELDS];
+#pragma unroll
+  for (int k = 0; k < SA_FIELDS; ++k) {
+    double t = 0.0;
+    for (int i = threadIdx.x; i < n; i += REDUCE_THREADS) t += part[k * npart + i];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off);
+    v[k] = t;
   }
-  reduce_store(s, out);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < SA_FIELDS; ++k) sh[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t[SA_FIELDS];
+    for (int k = 0; k < SA_FIELDS; ++k) {
+      t[k] = 0.0;
+      for (int i = 0; i < REDUCE_THREADS / 64; ++i) t[k] += sh[k][i];
+    }
+    out->dt = t[0];
+    out->alpha = t[1];
+    out->M[0] = t[2];
+    out->M[1] = t[3];
+    out->xbh[0] = t[4];
+    out->xbh[1] = t[5];
+    out->ybh[0] = t[6];
+    out->ybh[1] = t[7];
+    out->loss = t[8];
+  }
 }
 
 // sum of the tiles' torque into *acc (one workgroup; acc accumulates)
@@ -803,7 +816,7 @@ void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan
   hipLaunchKernelGGL(ka_plm_r, dim3(nz), B, 0, st, m, a, L.ztiles);
   hipLaunchKernelGGL(ka_slope, dim3(nz), B, 0, st, m, a, L.ztiles);
   hipLaunchKernelGGL(ka_plm_p, dim3(nz), B, 0, st, m, a, L.ztiles);
-  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(REDUCE_THREADS), 0, st, a.part,
+  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(REDUCE_THREADS), 0, st, (const double *)a.part, a.npart,
                      a.visc ? nz + 2 * nu + 2 * nrf : nz + nu + nrf, out);
 }
 
@@ -813,7 +826,8 @@ void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, 
   hipMemsetAsync(pick, 0x7f, sizeof(int), st);  // "no cell yet": a large positive index
   hipLaunchKernelGGL(ka_dt_pick, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles, dtbits, pick);
   hipLaunchKernelGGL(ka_dt, dim3(L.n_itiles), dim3(TILE), 0, st, m, a, L.itiles, (const int *)pick, 0);
-  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(REDUCE_THREADS), 0, st, a.part, L.n_itiles, out);
+  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(REDUCE_THREADS), 0, st, (const double *)a.part, a.npart, L.n_itiles,
+                     out);
 }
 
 void launch_torque_sum(const double *tiles, int n, double *acc, hipStream_t st) {

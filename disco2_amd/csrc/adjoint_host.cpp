@@ -131,7 +131,7 @@ namespace {
 // device buffers of one adjoint run, carved from one allocation
 struct Arena {
   char *base = nullptr;
-  size_t off = 0, cap = 0;
+  size_t off = 0;
   template <class T>
   T *take(size_t n) {
     off = (off + 255) & ~(size_t)255;
@@ -162,15 +162,24 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr, *arot, *torque, *acc, *ckpt, *fin, *th1, *th2,
       *Ltheta;
   DiskScalars *screc, *ckpt_sc, *fin_sc;
-  ScalarAdj *part;
+  double *part;
+  ScalarAdj *out;
   AdjChain *cs;
   int *pick;
   std::vector<AdjSubRec> rec(2 * every);
   Arena ar;
+  size_t zero_bytes = 0;
   auto layout = [&]() {
     ar.off = 0;
-    aP = ar.take<double>(nq);
+    // first the fields that start at zero (one memset)
     aPn = ar.take<double>(nq);
+    arot = ar.take<double>(nz);
+    acc = ar.take<double>(1);
+    Ltheta = ar.take<double>((size_t)nann);
+    out = ar.take<ScalarAdj>(3);  // S2, S1, S0 of the current step
+    cs = ar.take<AdjChain>(1);
+    zero_bytes = ar.off;
+    aP = ar.take<double>(nq);
     aR = ar.take<double>(nq);
     au = ar.take<double>(nq);
     agrad = ar.take<double>(nq);
@@ -178,19 +187,15 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     aSL = ar.take<double>(NQ * nf);
     aSR = ar.take<double>(NQ * nf);
     fr = ar.take<double>((size_t)AFR_N * nf);
-    arot = ar.take<double>(nz);
     torque = ar.take<double>((size_t)plan.n_utiles);
-    acc = ar.take<double>(1);
     ckpt = ar.take<double>(wA * nseg);
     fin = ar.take<double>(nseg > 1 ? wA : 0);
     th1 = ar.take<double>((size_t)nann);
     th2 = ar.take<double>((size_t)nann);
-    Ltheta = ar.take<double>((size_t)nann);
     screc = ar.take<DiskScalars>((size_t)3 * every);
     ckpt_sc = ar.take<DiskScalars>((size_t)nseg);
     fin_sc = ar.take<DiskScalars>(1);
-    part = ar.take<ScalarAdj>(npart + 3);
-    cs = ar.take<AdjChain>(1);
+    part = ar.take<double>(9 * npart);
     pick = ar.take<int>(1);
     for (auto &r : rec) {
       r.prim = ar.take<double>(nq);
@@ -210,19 +215,22 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     }
   };
   layout();
-  ar.cap = ar.off;
-  if (hipMalloc((void **)&ar.base, ar.cap) != hipSuccess) {
-    (void)hipGetLastError();
-    disco_set_error(13, "adjoint: the record of a segment (" + std::to_string(ar.cap >> 20) +
-                            " MiB) does not fit in device memory; lower checkpoint_every");
-    return -1;
+  if (ar.off > adj_cap) {  // the working memory is kept for later calls
+    if (adj_mem) hipFree(adj_mem);
+    adj_mem = nullptr;
+    adj_cap = 0;
+    if (hipMalloc(&adj_mem, ar.off) != hipSuccess) {
+      (void)hipGetLastError();
+      adj_mem = nullptr;
+      disco_set_error(13, "adjoint: the record of a segment (" + std::to_string(ar.off >> 20) +
+                              " MiB) does not fit in device memory; lower checkpoint_every");
+      return -1;
+    }
+    adj_cap = ar.off;
   }
+  ar.base = (char *)adj_mem;
   layout();
-  if (!hip_check(hipMemsetAsync(ar.base, 0, ar.cap, st), "adjoint")) {
-    hipFree(ar.base);
-    return -1;
-  }
-  ScalarAdj *out = part + npart;  // S2, S1, S0 of the current step
+  if (!hip_check(hipMemsetAsync(ar.base, 0, zero_bytes, st), "adjoint")) return -1;
 
   // the engine's own buffers, pointed back at after each recorded segment
   struct Own {
@@ -277,11 +285,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   const double T = get_scalars(0).t - t0;
   double I = 0.0;
   hip_check(hipMemcpy(&I, acc, 8, hipMemcpyDeviceToHost), "torque D2H");
-  auto release = [&]() { hipFree(ar.base); };
-  if (check_error() || !(T > 0.0)) {
-    release();
-    return -1;
-  }
+  if (check_error() || !(T > 0.0)) return -1;
   *loss = I / T;
   const double wl = 1.0 / T, lamT = -I / (T * T);
   if (nseg > 1) {  // the state after nsteps, put back at the end
@@ -326,6 +330,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   a.aSR = aSR;
   a.fr = fr;
   a.part = part;
+  a.npart = (long long)npart;
   a.wl = wl;
   a.lam_dt = &cs->lam_dt;
   a.arot = arot;
@@ -390,7 +395,5 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     gq = (C.Lr[0] - C.Lr[1] - C.LM[0] + C.LM[1]) / ((1. + q) * (1. + q));
   }
   *g_q = gq;
-  const int e = check_error();
-  release();
-  return e ? -1 : 0;
+  return check_error() ? -1 : 0;
 }

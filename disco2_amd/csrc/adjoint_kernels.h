@@ -24,7 +24,8 @@ enum {
 };
 struct AdjMesh {
   double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr;
-  ScalarAdj *part;     // per-tile partial sums of the scalar adjoints
+  double *part;        // per-tile partial sums of the scalar adjoints: 9 rows of npart
+  long long npart;
   double wl;           // weight of the torque term in the loss (1/T)
   double fac;          // the substep's dt factor (its RK weight is m.sc's)
   int copy_first;      // first substep: RKcons = prim2cons(prim)

@@ -51,6 +51,8 @@ struct Engine {
   int cur = 0;  // index of the prim / tiph / dphi buffers holding the current state
   double *adj_usave = nullptr, *adj_torque = nullptr;  // adjoint recording (mesh() passes them on)
   signed char *adj_ftype = nullptr;
+  void *adj_mem = nullptr;  // the adjoint's working memory, kept for later calls
+  size_t adj_cap = 0;
 
   LaunchPlan plan{};
   hipStream_t st = nullptr;
