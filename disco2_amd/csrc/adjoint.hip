@@ -57,36 +57,11 @@ __device__ __forceinline__ void load_pair(const DevMesh &m, long long cL, long l
   load5(m.gradp, m.Ntot, cL, c.gpL);
   load5(m.gradp, m.Ntot, cR, c.gpR);
 }
-__device__ __forceinline__ void scatter_pair(const DevMesh &m, const AdjMesh &a, long long cL, long long cR,
-                                             const FacePair &cb) {
-  add5(a.aP, m.Ntot, cL, cb.pL);
-  add5(a.aP, m.Ntot, cR, cb.pR);
-  add5(a.agrad, m.Ntot, cL, cb.gL);
-  add5(a.agrad, m.Ntot, cR, cb.gR);
-  add5(a.agradp, m.Ntot, cL, cb.gpL);
-  add5(a.agradp, m.Ntot, cR, cb.gpR);
-}
-
 // A face's width and centre follow the rotations of the annuli whose edges
 // bound it (face_create_destroy.c:74-125): d(width)/d(theta_in, theta_out)
 // and d(centre)/d(theta_in, theta_out) by face type.  The adjoints of a face's
 // width (wb), centre (cb) and of its cells' own edges (tLb, tRb) go to the
-// rotation adjoint of the inner (L cell) and outer (R cell) annulus.
-__device__ __forceinline__ void face_geo_to_rot(const DevMesh &m, const AdjMesh &a, long long g, long long cL,
-                                                long long cR, double wb, double cb, double tLb, double tRb) {
-  double w_in = 0.0, w_out = 0.0, c_in = 0.0, c_out = 0.0;
-  switch (m.ftype[g]) {
-    case FACE_II: c_in = 1.0; break;
-    case FACE_IO: w_in = -1.0; w_out = 1.0; c_in = c_out = 0.5; break;
-    case FACE_OO: c_out = 1.0; break;
-    default: w_in = 1.0; w_out = -1.0; c_in = c_out = 0.5; break;  // FACE_OI
-  }
-  const double vin = wb * w_in + cb * c_in + tLb, vout = wb * w_out + cb * c_out + tRb;
-  if (vin != 0.0) atomicAdd(&a.arot[cL], vin);
-  if (vout != 0.0) atomicAdd(&a.arot[cR], vout);
-}
-
-// the same split, returned: vin to the L cell's annulus, vout to the R cell's
+// rotation adjoint of the inner (L cell, vin) and outer (R cell, vout) annulus.
 __device__ __forceinline__ void face_rot_split(signed char type, double wb, double cb, double tLb, double tRb,
                                                double *vin, double *vout) {
   double w_in = 0.0, w_out = 0.0, c_in = 0.0, c_out = 0.0;
@@ -98,6 +73,14 @@ __device__ __forceinline__ void face_rot_split(signed char type, double wb, doub
   }
   *vin = wb * w_in + cb * c_in + tLb;
   *vout = wb * w_out + cb * c_out + tRb;
+}
+// the same, added to the two cells' entries of a.arot
+__device__ __forceinline__ void face_geo_to_rot(const DevMesh &m, const AdjMesh &a, long long g, long long cL,
+                                                long long cR, double wb, double cb, double tLb, double tRb) {
+  double vin, vout;
+  face_rot_split(m.ftype[g], wb, cb, tLb, tRb, &vin, &vout);
+  if (vin != 0.0) atomicAdd(&a.arot[cL], vin);
+  if (vout != 0.0) atomicAdd(&a.arot[cR], vout);
 }
 
 // Outflow remap backward (k_bc_r): a ghost cell's adjoint goes to the cells
@@ -146,7 +129,7 @@ __global__ __launch_bounds__(TILE) void ka_bc(DevMesh m, AdjMesh a, const int2 *
 
 // Cell update backward (k_update phase 2): cons2prim, the RK blend and the
 // RK snapshot, prim2cons, the sources and the torque term.  Writes the
-// adjoint of the cell's conserved state (au) and starts aP / agrad / agradp.
+// adjoint of the cell's conserved state (au) and starts aP.
 __global__ __launch_bounds__(TILE) void ka_update(DevMesh m, AdjMesh a, const int2 *tiles) {
   const int ti = xcd_order();
   const int2 tl = tiles[ti];
@@ -182,9 +165,6 @@ __global__ __launch_bounds__(TILE) void ka_update(DevMesh m, AdjMesh a, const in
     vjp_cell_sources(P, sc, pr, r, dVs, a.fac * sc.dt, phi, .5 * (zp + zm), ub, wl, prb, sb, &phib);
     a.arot[c] += phib;
     store5(a.aP, m.Ntot, c, prb);
-    const double z[5] = {0, 0, 0, 0, 0};
-    store5(a.agrad, m.Ntot, c, z);
-    store5(a.agradp, m.Ntot, c, z);
   }
   reduce_store(sb, a, ti);
 }
@@ -258,7 +238,7 @@ __global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_phi_h(DevMesh m, Adj
     for (int q = 0; q < 5; ++q) {
       const double bR = sR[q][t], bL = sL[q][t + 1];
       a.aP[q * m.Ntot + c] += bR + bL;
-      a.agradp[q * m.Ntot + c] += .5 * dph * bL - .5 * dph * bR;
+      a.agradp[q * m.Ntot + c] = .5 * dph * bL - .5 * dph * bR;  // first writer of agradp
     }
   }
   reduce_store(sb, a, slot0 + ti);
@@ -301,7 +281,7 @@ __global__ __launch_bounds__(TILE) void ka_phi_v(DevMesh m, AdjMesh a, const int
     for (int q = 0; q < 5; ++q) a.aP[q * m.Ntot + c] += sv[q][t] + sv[q][t + 1];
     for (int k = 0; k < 2; ++k) {
       a.aP[(URR + k) * m.Ntot + c] += sv[5 + k][t] - sv[5 + k][t + 1];
-      a.agrad[(URR + k) * m.Ntot + c] += sv[7 + k][t] + sv[7 + k][t + 1];
+      a.agrad[(URR + k) * m.Ntot + c] = sv[7 + k][t] + sv[7 + k][t + 1];  // read by ka_plm_r
     }
     a.arot[c] += sv[9][t + 1];
   }
@@ -465,12 +445,15 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
         for (int kk = 0; kk < 2; ++kk) agp[URR + kk] += a.fr[(AFR_G + kk) * NF + gi];
       rot += a.fr[AFR_ROT * NF + gi];
     });
+  // the r slope's adjoint: its r faces' plus (viscous) its phi faces' (agrad)
   double gb[5];
   for (int q = 0; q < 5; ++q) {
     a.aP[q * m.Ntot + z] += ap[q];
-    gb[q] = a.agrad[q * m.Ntot + z] + ag[q];
+    gb[q] = ag[q];
     a.agradp[q * m.Ntot + z] += agp[q];
   }
+  if (visc)
+    for (int kk = 0; kk < 2; ++kk) gb[URR + kk] += a.agrad[(URR + kk) * m.Ntot + z];
   a.arot[z] += rot;
   const double dz = A.zp - A.zm, dp = m.dphi[z];
   double dAtot;
