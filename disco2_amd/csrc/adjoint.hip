@@ -652,12 +652,23 @@ __global__ __launch_bounds__(REDUCE_THREADS) void ka_reduce(const double *part, 
   __shared__ double sh[SA_FIELDS][REDUCE_THREADS / 64];
   double v[SA_FIELDS];
 #pragma unroll
-  for (int k = 0; k < SA_FIELDS; ++k) {
-    double t = 0.0;
-    for (int i = threadIdx.x; i < n; i += REDUCE_THREADS) t += part[k * npart + i];
-    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off);
-    v[k] = t;
+  for (int k = 0; k < SA_FIELDS; ++k) v[k] = 0.0;
+  // 4 slots per thread and pass, all 9 fields: 36 loads in flight
+  for (int i0 = threadIdx.x; i0 < n; i0 += 4 * REDUCE_THREADS) {
+    double x[SA_FIELDS][4];
+#pragma unroll
+    for (int k = 0; k < SA_FIELDS; ++k)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * REDUCE_THREADS;
+        x[k][u] = i < n ? part[k * npart + i] : 0.0;
+      }
+#pragma unroll
+    for (int k = 0; k < SA_FIELDS; ++k) v[k] += (x[k][0] + x[k][1]) + (x[k][2] + x[k][3]);
   }
+#pragma unroll
+  for (int k = 0; k < SA_FIELDS; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
 #pragma unroll
