@@ -214,11 +214,18 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=50)
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg and print its JSON (for profiling it)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one disk or batch per "
               "GPU); run with --gpus 1", file=sys.stderr)
         return 2
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(args.adjoint_steps, args.adjoint_every)))
+        return 0
 
     from disco2_amd import configs, ics
     from disco2_amd.api import Batch, Setup
@@ -276,7 +283,7 @@ def main(argv=None):
     adjoint = None
     if not args.no_adjoint:
         print("[bench] adjoint", file=sys.stderr, flush=True)
-        adjoint = adjoint_leg()
+        adjoint = adjoint_leg(args.adjoint_steps, args.adjoint_every)
 
     cpu = None
     if not args.no_cpu_baseline:

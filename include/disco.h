@@ -242,10 +242,13 @@ double disco_calibrate_copy(long long n);
  * current state, disco_batch_torque_loss runs nsteps RK2 steps and returns L;
  * disco_batch_adjoint also returns dL/d(Explicit_Viscosity), dL/d(MassRatio)
  * and dL/d(initial primitives) (NUM_Q*Nzones, q-major, reference cell order),
- * recomputing the forward from a checkpoint every checkpoint_every steps.
- * One 2D disk, NUM_Q 5, w_analytic_type 0, a mesh velocity independent of the
- * state, no sink or cooling, outflow r boundaries; otherwise the calls set
- * the error and return -1.  The batch's state afterwards is unspecified. */
+ * keeping a checkpoint every checkpoint_every steps (checkpoint_every >=
+ * nsteps, or < 1: the whole run is recorded, nothing is recomputed).  The
+ * batch keeps the adjoint's device working memory (the record of one
+ * segment) for later calls until it is destroyed.  One 2D disk, NUM_Q 5,
+ * w_analytic_type 0, a mesh velocity independent of the state, no sink or
+ * cooling, outflow r boundaries; otherwise the calls set the error and
+ * return -1.  Afterwards the batch holds the state after nsteps steps. */
 int disco_batch_torque_loss(struct DiscoBatch *, int nsteps, double *loss);
 int disco_batch_adjoint(struct DiscoBatch *, int nsteps, int checkpoint_every, double *loss, double *grad_alpha,
                         double *grad_q, double *grad_prim0);
