@@ -84,7 +84,7 @@ void Engine::adjoint_redirect(const AdjSubRec &r) {
 // them; every k_update adds its torque to acc.
 void Engine::adjoint_step(const AdjSubRec *rec, DiskScalars *screc, double *torque, double *acc) {
   DevMesh m = mesh();
-  launch_set_w(m, plan, st);
+  enqueue_set_w();
   launch_mindt(m, plan, st, NQ);
   if (screc) launch_copy_scalars(screc, d_sc, st);
   launch_scalar(m, plan, st, SC_STEP_BEGIN | SC_SUB_BEGIN);
@@ -95,7 +95,7 @@ void Engine::adjoint_step(const AdjSubRec *rec, DiskScalars *screc, double *torq
   launch_torque_sum(torque, plan.n_utiles, acc, st);
   if (screc) launch_copy_scalars(screc + 2, d_sc, st);
   if (rec) adjoint_redirect(rec[1]);
-  enqueue_substep(0.5, 0, 1, 0, SC_AFTER_UPDATE | SC_SUB2_END);
+  enqueue_substep(0.5, 0, w_static ? 0 : 1, 0, SC_AFTER_UPDATE | SC_SUB2_END);
   launch_torque_sum(torque, plan.n_utiles, acc, st);
   adj_torque = nullptr;
   adj_usave = nullptr;

@@ -175,6 +175,8 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   }
   Ntot = zb;
   NFtot = fb;
+  w_static = true;
+  for (auto &P : hdp) w_static = w_static && P.w_numeric != C_RIGID;
   nrun = rb > 0 ? rb : 1;
   nann = (int)hann.size();
 
@@ -470,16 +472,25 @@ void Engine::enqueue_substep(double fac, int copy_first, int recompute_w, int pr
 // scalar updates between substeps ride in one launch after each update.
 void Engine::enqueue_rk2() {
   DevMesh m = mesh();
-  launch_set_w(m, plan, st);
+  enqueue_set_w();
   launch_mindt(m, plan, st, NQ);
   enqueue_substep(1.0, 1, 0, SC_STEP_BEGIN | SC_SUB_BEGIN, SC_AFTER_UPDATE | SC_SUB1_END | SC_SUB_BEGIN);
-  enqueue_substep(0.5, 0, 1, 0, SC_AFTER_UPDATE | SC_SUB2_END);
+  enqueue_substep(0.5, 0, w_static ? 0 : 1, 0, SC_AFTER_UPDATE | SC_SUB2_END);
+}
+
+// cell_clear_w + cell_set_w (cell_set_w.c:12-102).  Every mesh motion but
+// C_RIGID is a function of r alone: once computed, wiph is what set_w would
+// write again, bit for bit, so later steps skip the launch.
+void Engine::enqueue_set_w() {
+  if (w_static && w_ready) return;
+  launch_set_w(mesh(), plan, st);
+  w_ready = true;
 }
 
 // timestep_forward_euler, src/TimeStep/timestep_algorithm.c:43-57
 void Engine::enqueue_euler() {
   DevMesh m = mesh();
-  launch_set_w(m, plan, st);
+  enqueue_set_w();
   launch_mindt(m, plan, st, NQ);
   enqueue_substep(1.0, 1, 0, SC_STEP_BEGIN | SC_SUB_BEGIN, SC_AFTER_UPDATE | SC_EULER_END);
 }

@@ -49,6 +49,8 @@ struct Engine {
   double4 *d_ann_rot = nullptr;
   int2 *d_tiles = nullptr;
   int cur = 0;  // index of the prim / tiph / dphi buffers holding the current state
+  bool w_static = false;  // no disk moves its annuli with the gas (C_RIGID): wiph depends on r alone
+  bool w_ready = false;   // wiph has been set (a launch for it is enqueued)
   double *adj_usave = nullptr, *adj_torque = nullptr;  // adjoint recording (mesh() passes them on)
   signed char *adj_ftype = nullptr;
   void *adj_mem = nullptr;  // the adjoint's working memory, kept for later calls
@@ -72,6 +74,7 @@ struct Engine {
   DiskScalars get_scalars(int disk);
   void init_state();                 // gravMass init, z sync of prim/tiph, scalars
   void enqueue_rk2();
+  void enqueue_set_w();
   void enqueue_euler();
   void enqueue_substep(double fac, int copy_first, int recompute_w, int pre_ops, int post_ops);
   void run(int nsteps, bool use_graph);

@@ -380,6 +380,12 @@ __global__ __launch_bounds__(TILE) void k_face_slope(DevMesh m, const int2 *ftil
   for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = S[q];
 }
 
+// The slopes of a cell's first PLM_CACHE faces stay in registers between the
+// sum and the limiter's walk (slot selects on constant indices); the rest are
+// read again.
+#ifndef PLM_CACHE
+#define PLM_CACHE 4
+#endif
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, int dir) {
   const int2 tl = tiles[xcd_order()];
@@ -401,16 +407,62 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
     if (k >= 1) pR = zpair_of(P, k - 1, i);
     if (k <= P.NZ - 2) pLf = zpair_of(P, k, i);
   }
-  auto acc = [&](const PairInfo &PI, long long gi) {
-    double S[NQ];
+  // The cell's faces in the reference's order: as R, its runs in the pair
+  // below (faces_as_R: at most two, the second is the wrap), then as L one
+  // range in the pair above (faces_as_L); range u is faces [fb[u] + b[u],
+  // fb[u] + e[u]) with its pair's r and deltaPerp.
+  long long fb[3] = {0, 0, 0};
+  int b[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+  double pr[3] = {0, 0, 0}, pp[3] = {0, 0, 0};
+  if (pR >= 0) {
+    const PairInfo PI = m.pairs[pR];
+    const PairState S = m.pst[pR];
+    int t = j - S.jpmin;
+    if (t < 0) t += A.N;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) S[q] = m.Fface[q * m.NFtot + gi];
-    double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
+    for (int u = 0; u < 2; ++u) {
+      if (t <= S.T) {
+        fb[u] = PI.face_base;
+        b[u] = m.run_start[PI.run_base + t];
+        e[u] = m.run_start[PI.run_base + t + 1];
+        pr[u] = PI.r;
+        pp[u] = PI.deltaPerp;
+      }
+      t += A.N;
+    }
+  }
+  if (pLf >= 0) {
+    const PairInfo PI = m.pairs[pLf];
+    const PairState S = m.pst[pLf];
+    int f1 = (j + 1 < A.N) ? fo[z + 1] : S.count;
+    if (f1 > S.count) f1 = S.count;
+    int f0 = fo[z];
+    if (f0 < 0) f0 = 0;
+    fb[2] = PI.face_base;
+    b[2] = f0;
+    e[2] = f1;
+    pr[2] = PI.r;
+    pp[2] = PI.deltaPerp;
+  }
+  double cS[PLM_CACHE > 0 ? PLM_CACHE : 1][NQ];
+  int n = 0;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) g[q] += S[q] * dA;
-  };
-  if (pR >= 0) faces_as_R(m, pR, j, A.N, acc);
-  if (pLf >= 0) faces_as_L(m, pLf, fo, z, j, A.N, acc);
+  for (int u = 0; u < 3; ++u)
+    for (int f = b[u]; f < e[u]; ++f) {
+      const long long gi = fb[u] + f;
+      double S[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) S[q] = m.Fface[q * m.NFtot + gi];
+      double dA = pr[u] * m.fdphi[gi] * pp[u];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) g[q] += S[q] * dA;
+#pragma unroll
+      for (int c = 0; c < PLM_CACHE; ++c)
+        if (n == c)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) cS[c][q] = S[q];
+      ++n;
+    }
   const double dz = A.zp - A.zm;
   const double rp = A.rp, rm = A.rm;
   const double dp = m.dphi[z];
@@ -428,20 +480,31 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
 #pragma unroll
   for (int q = 0; q < NQ; ++q) g[q] /= dAtot;
   const double PLM = P.plm;
-  auto lim = [&](const PairInfo &PI, long long gi) {
-    double S[NQ];
+  n = 0;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) S[q] = m.Fface[q * m.NFtot + gi];
+  for (int u = 0; u < 3; ++u)
+    for (int f = b[u]; f < e[u]; ++f) {
+      double S[NQ];
+      if (n < PLM_CACHE) {
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      if (S[q] * g[q] < 0.0)
-        g[q] = 0.0;
-      else if (fabs(PLM * S[q]) < fabs(g[q]))
-        g[q] = PLM * S[q];
+        for (int c = 0; c < PLM_CACHE; ++c)
+          if (n == c)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) S[q] = cS[c][q];
+      } else {
+        const long long gi = fb[u] + f;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) S[q] = m.Fface[q * m.NFtot + gi];
+      }
+      ++n;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        if (S[q] * g[q] < 0.0)
+          g[q] = 0.0;
+        else if (fabs(PLM * S[q]) < fabs(g[q]))
+          g[q] = PLM * S[q];
+      }
     }
-  };
-  if (pR >= 0) faces_as_R(m, pR, j, A.N, lim);
-  if (pLf >= 0) faces_as_L(m, pLf, fo, z, j, A.N, lim);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) m.grad[q * m.Ntot + z] = g[q];
 }
