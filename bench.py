@@ -36,7 +36,16 @@ B_ALG_PER_ZONE_UPDATE = 320  # 8 * (6*NUM_Q + 10) bytes at NUM_Q = 5
 # Counters cannot be read from inside the timed run, so the figure is
 # reported only for the workload and kernel it was measured on.
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-MEASURED_TRAFFIC_KIB = {("binary2d_r512", "k_update"): (136797, 72647)}
+MEASURED_TRAFFIC_KIB = {("binary2d_r512", "k_update"): (136831, 72642)}
+
+# FP64 arithmetic, measured the same way (rocprofv3 --pmc
+# SQ_INSTS_VALU_FLOPS_FP64 ..., profiles/binary2d_r512_fp64_flops.md): the
+# counter gives FLOPs per wave instruction (an FMA counts 2; it equals
+# ADD + MUL + 2 FMA + TRANS of the per-type instruction counters), times 64
+# lanes; masked lanes at tile ends are counted, so these are upper bounds.
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector, published
+MEASURED_FLOP_PER_LAUNCH = {("binary2d_r512", "k_update"): 2.258e7 * 64}
+MEASURED_FLOP_PER_STEP = {"binary2d_r512": 1.0902e8 * 64}  # every kernel of one RK2 step
 
 
 def r_face_count(n_p, nz_layers):
@@ -271,6 +280,19 @@ def main(argv=None):
     meas = MEASURED_TRAFFIC_KIB.get((args.workload, dom_name))
     if meas:
         roof["traffic"] = int((meas[0] * FETCH_SCALE + meas[1] * WRITE_SCALE) * 1024)
+    flop_roof = None
+    fl = MEASURED_FLOP_PER_LAUNCH.get((args.workload, dom_name))
+    if fl:
+        fl_ach = fl / (dom_avg_ms * 1e-3) / 1e12
+        flop_roof = {"kernel": dom_name, "bound": "fp64", "achieved": round(fl_ach, 2), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(fl_ach / FP64_PEAK_TFLOPS, 4), "flop_per_launch": fl}
+    fs = MEASURED_FLOP_PER_STEP.get(args.workload)
+    if fs:
+        st_ach = fs / (ms_per_step * 1e-3) / 1e12
+        flop_step = {"flop_per_zone_update": round(fs / n_int, 1), "achieved": round(st_ach, 2),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(st_ach / FP64_PEAK_TFLOPS, 4)}
+    else:
+        flop_step = None
     step_ach = value * B_ALG_PER_ZONE_UPDATE / 1e9
     kernels = {k: {"avg_ms": round(v[0] / v[1], 5), "share": round(v[0] / sum(x[0] for x in prof.values()), 4)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])}
@@ -309,6 +331,8 @@ def main(argv=None):
         "roofline": roof,
         "step_roofline": {"alg_bytes_per_zone_update": B_ALG_PER_ZONE_UPDATE, "achieved": round(step_ach, 1),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)},
+        "flop_roofline": flop_roof,
+        "step_flop_roofline": flop_step,
         "kernels": kernels,
         "batched": batched,
         "adjoint": adjoint,

@@ -254,6 +254,8 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   plan.bc1 = d_tiles + o_b1;
   plan.sz = d_tiles + o_s;
   plan.ndisk = n;
+  plan.any_visc = 0;
+  for (auto &P : hdp) plan.any_visc |= P.visc > 0.0;
   plan.nann = nann;
   {  // largest outer annulus of each pair direction, staged in LDS if it fits
     int mr = 0, mz = 0;

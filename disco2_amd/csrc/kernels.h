@@ -54,6 +54,7 @@ struct LaunchPlan {
   const int2 *rftiles; int n_rftiles;   // r faces
   const int2 *zftiles; int n_zftiles;   // z faces
   int stage_r, stage_z;                 // face builder: outer-annulus cells staged in LDS (0: none)
+  int any_visc;                         // some disk has viscosity
   const int2 *bc0; int n_bc0;           // ghost annuli 0 and NR-2
   const int2 *bc1; int n_bc1;           // ghost annulus NR-1
   const int2 *sz; int n_sz;             // z ghost layers

@@ -545,7 +545,9 @@ __device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A,
   return d_face_flux<NQ>(primL, primR, 1, r, dA, m.wiph[a], dt, P, sc, &vi, X, Y);
 }
 
-template <int NQ>
+// PART 0: Riemann and viscous increments; 1: Riemann only; 2: viscous only
+// (with FLUX_SPLIT the two parts run as two launches)
+template <int NQ, int PART>
 __global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
   const int2 tl = ftiles[xcd_order()];
   const int p = tl.x;
@@ -553,6 +555,7 @@ __global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const i
   if (f >= m.pst[p].count) return;
   const PairInfo PI = m.pairs[p];
   const DiskParams &P = disk_of(m, PI.disk);
+  if (PART == 2 && !(P.visc > 0.0)) return;
   const DiskScalars &sc = m.sc[PI.disk];
   const double dt = fac * sc.dt;
   const long long g = PI.face_base + f;
@@ -590,11 +593,12 @@ __global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const i
     }
   }
   double X[NQ], Y[3] = {0.0, 0.0, 0.0};
-  int e = d_face_flux<NQ>(primL, primR, PI.dir == 0 ? 0 : 2, r, dA, 0.0, dt, P, sc, &vi, X, Y);
+  int e = d_face_flux<NQ, PART>(primL, primR, PI.dir == 0 ? 0 : 2, r, dA, 0.0, dt, P, sc, &vi, X, Y);
   if (e) set_err(m.err, e);
+  if (PART != 2)
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = X[q];
-  if (P.visc > 0.0) {
+    for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = X[q];
+  if (PART != 1 && P.visc > 0.0) {
 #pragma unroll
     for (int v = 0; v < 3; ++v) m.Fface[(NQ + v) * m.NFtot + g] = Y[v];
   }
@@ -1199,6 +1203,23 @@ static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st
             hipLaunchKernelGGL(k_face_slope<NQ>, dim3(nft), dim3(TILE), 0, st, m, ft));
 }
 
+// the r or z faces' increments: one kernel, or (FLUX_SPLIT) the Riemann and
+// the viscous parts as two, each with fewer registers
+#ifndef FLUX_SPLIT
+#define FLUX_SPLIT 0
+#endif
+template <int NQ>
+static void launch_flux_q(const DevMesh &m, const int2 *ft, int nft, int any_visc, hipStream_t st, double fac,
+                          const char *name, const char *name_visc) {
+  const dim3 B(TILE);
+  if (FLUX_SPLIT) {
+    KLAUNCH(st, name, hipLaunchKernelGGL((k_flux_rz<NQ, 1>), dim3(nft), B, 0, st, m, ft, fac));
+    if (any_visc) KLAUNCH(st, name_visc, hipLaunchKernelGGL((k_flux_rz<NQ, 2>), dim3(nft), B, 0, st, m, ft, fac));
+  } else {
+    KLAUNCH(st, name, hipLaunchKernelGGL((k_flux_rz<NQ, 0>), dim3(nft), B, 0, st, m, ft, fac));
+  }
+}
+
 template <int NQ>
 static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, double fac, int copy_first,
                              int recompute_w, int pre_ops, int post_ops) {
@@ -1211,10 +1232,10 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   if (recompute_w) KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
   KLAUNCH(st, "k_plm_r", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0));
   if (L.n_rftiles)
-    KLAUNCH(st, "k_flux_r", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_rftiles), B, 0, st, m, L.rftiles, fac));
+    launch_flux_q<NQ>(m, L.rftiles, L.n_rftiles, L.any_visc, st, fac, "k_flux_r", "k_flux_r_visc");
   if (L.n_zpairs) {
     KLAUNCH(st, "k_plm_z", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 1));
-    KLAUNCH(st, "k_flux_z", hipLaunchKernelGGL(k_flux_rz<NQ>, dim3(L.n_zftiles), B, 0, st, m, L.zftiles, fac));
+    launch_flux_q<NQ>(m, L.zftiles, L.n_zftiles, L.any_visc, st, fac, "k_flux_z", "k_flux_z_visc");
   }
   KLAUNCH(st, "k_update",
           hipLaunchKernelGGL(k_update<NQ>, dim3(L.n_utiles), B, 0, st, m, L.utiles, fac, copy_first));

@@ -290,108 +290,112 @@ DHD __forceinline__ void d_set_flux(const double *prim, double *F, double r, int
   for (int q = 5; q < NQ; ++q) F[q] = prim[q] * F[DDD];
 }
 
-template <int NQ>
+// PART 0: the whole flux; 1: the Riemann increments X only; 2: the viscous
+// increments Y only (the two parts share no intermediate but dt*dA)
+template <int NQ, int PART = 0>
 DHD __forceinline__ int d_face_flux(const double *primL, const double *primR, int dir, double r,
                                            double dA, double w, double dt, const DiskParams &P,
                                            const DiskScalars &sc, const ViscIn *vi, double *X, double *Y) {
   const GravState *gm = sc.gm;
   const double G = P.gamma;
   const int n0 = dir == 0, n1 = dir == 1, n2 = dir == 2;
-  // riemann_set_vel, :30-55
-  double vnL, cf2L, vnR, cf2R;
-  {
-    double Pp = primL[PPP], rho = primL[RHO], vr = primL[URR], vp = r * primL[UPP], vz = primL[UZZ];
-    vnL = vr * n0 + vp * n1 + vz * n2;
-    cf2L = G * (Pp / rho);
-  }
-  double Sl = vnL - sqrt(cf2L);
-  double Sr = vnL + sqrt(cf2L);
-  {
-    double Pp = primR[PPP], rho = primR[RHO], vr = primR[URR], vp = r * primR[UPP], vz = primR[UZZ];
-    vnR = vr * n0 + vp * n1 + vz * n2;
-    cf2R = G * (Pp / rho);
-  }
-  if (Sl > vnR - sqrt(cf2R)) Sl = vnR - sqrt(cf2R);
-  if (Sr < vnR + sqrt(cf2R)) Sr = vnR + sqrt(cf2R);
-  double Ss = (primR[RHO] * vnR * (Sr - vnR) - primL[RHO] * vnL * (Sl - vnL) + primL[PPP] - primR[PPP]) /
-              (primR[RHO] * (Sr - vnR) - primL[RHO] * (Sl - vnL));
-  // riemann_set_state, :59-71
-  int state;
-  if (w < Sl)
-    state = 0;  // LEFT
-  else if (w > Sr)
-    state = 3;  // RIGHT
-  else if (w < Ss)
-    state = 1;  // LEFTSTAR
-  else
-    state = 2;  // RIGHTSTAR
-  double F[NQ];
-  if (state == 0 || state == 3) {
-    const double *pr = state == 0 ? primL : primR;
-    double Fk[NQ], Uk[NQ];
-    d_set_flux<NQ>(pr, Fk, r, n0, n1, n2, G);
-    d_prim2cons<NQ>(pr, Uk, r, 1.0, G, P.NC);
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) F[q] = Fk[q] - w * Uk[q];
-  } else if (P.riemann == HLL_SOLVER) {  // Riemann_routines.c:74-85
-    double FL[NQ], FR[NQ], UL[NQ], UR[NQ];
-    d_set_flux<NQ>(primL, FL, r, n0, n1, n2, G);
-    d_set_flux<NQ>(primR, FR, r, n0, n1, n2, G);
-    d_prim2cons<NQ>(primL, UL, r, 1.0, G, P.NC);
-    d_prim2cons<NQ>(primR, UR, r, 1.0, G, P.NC);
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      double Ustar = (Sr * UR[q] - Sl * UL[q] + FL[q] - FR[q]) / (Sr - Sl);
-      double Fstar = (Sr * FL[q] - Sl * FR[q] - Sr * Sl * (UL[q] - UR[q])) / (Sr - Sl);
-      F[q] = Fstar - w * Ustar;
-    }
-  } else {  // HLLC, Riemann_routines.c:87-154
-    const double *prim = state == 1 ? primL : primR;
-    double Sk = state == 1 ? Sl : Sr;
-    double Fk[NQ], Uk[NQ], Ustar[NQ];
-    d_set_flux<NQ>(prim, Fk, r, n0, n1, n2, G);
-    d_prim2cons<NQ>(prim, Uk, r, 1.0, G, P.NC);
-    double rho = prim[RHO];
-    double vr = prim[URR];
-    double vp = prim[UPP] * r;
-    double vz = prim[UZZ];
-    double Pp = prim[PPP];
-    double v2 = vr * vr + vp * vp + vz * vz;
-    double vn = vr * n0 + vp * n1 + vz * n2;
-    double rhoe = Pp / (G - 1.);
-    double D = rho;
-    double mr = rho * vr;
-    double mp = rho * vp;
-    double mz = rho * vz;
-    double E_hydro = .5 * rho * v2 + rhoe;
-    double Ps = rho * (Sk - vn) * (Ss - vn) + Pp;
-    double Dstar = (Sk - vn) * D / (Sk - Ss);
-    double Msn = Dstar * Ss;
-    double Msr = (Sk - vn) * mr / (Sk - Ss);
-    double Msp = (Sk - vn) * mp / (Sk - Ss);
-    double Msz = (Sk - vn) * mz / (Sk - Ss);
-    double Estar = ((Sk - vn) * E_hydro + Ps * Ss - Pp * vn) / (Sk - Ss);
-    double mn = Msr * n0 + Msp * n1 + Msz * n2;
-    Msr += n0 * (Msn - mn);
-    Msp += n1 * (Msn - mn);
-    Msz += n2 * (Msn - mn);
-    Ustar[DDD] = Dstar;
-    Ustar[SRR] = Msr;
-    Ustar[LLL] = r * Msp;
-    Ustar[SZZ] = Msz;
-    Ustar[TAU] = Estar;
-#pragma unroll
-    for (int q = 5; q < NQ; ++q) Ustar[q] = prim[q] * Ustar[DDD];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      double Fstar = Fk[q] + Sk * (Ustar[q] - Uk[q]);
-      F[q] = Fstar - w * Ustar[q];
-    }
-  }
   double dtdA = dt * dA;
+  if (PART != 2) {
+    // riemann_set_vel, :30-55
+    double vnL, cf2L, vnR, cf2R;
+    {
+      double Pp = primL[PPP], rho = primL[RHO], vr = primL[URR], vp = r * primL[UPP], vz = primL[UZZ];
+      vnL = vr * n0 + vp * n1 + vz * n2;
+      cf2L = G * (Pp / rho);
+    }
+    double Sl = vnL - sqrt(cf2L);
+    double Sr = vnL + sqrt(cf2L);
+    {
+      double Pp = primR[PPP], rho = primR[RHO], vr = primR[URR], vp = r * primR[UPP], vz = primR[UZZ];
+      vnR = vr * n0 + vp * n1 + vz * n2;
+      cf2R = G * (Pp / rho);
+    }
+    if (Sl > vnR - sqrt(cf2R)) Sl = vnR - sqrt(cf2R);
+    if (Sr < vnR + sqrt(cf2R)) Sr = vnR + sqrt(cf2R);
+    double Ss = (primR[RHO] * vnR * (Sr - vnR) - primL[RHO] * vnL * (Sl - vnL) + primL[PPP] - primR[PPP]) /
+                (primR[RHO] * (Sr - vnR) - primL[RHO] * (Sl - vnL));
+    // riemann_set_state, :59-71
+    int state;
+    if (w < Sl)
+      state = 0;  // LEFT
+    else if (w > Sr)
+      state = 3;  // RIGHT
+    else if (w < Ss)
+      state = 1;  // LEFTSTAR
+    else
+      state = 2;  // RIGHTSTAR
+    double F[NQ];
+    if (state == 0 || state == 3) {
+      const double *pr = state == 0 ? primL : primR;
+      double Fk[NQ], Uk[NQ];
+      d_set_flux<NQ>(pr, Fk, r, n0, n1, n2, G);
+      d_prim2cons<NQ>(pr, Uk, r, 1.0, G, P.NC);
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) X[q] = dtdA * F[q];
-  if (P.visc > 0.0) {  // riemann_visc_flux, :193-311
+      for (int q = 0; q < NQ; ++q) F[q] = Fk[q] - w * Uk[q];
+    } else if (P.riemann == HLL_SOLVER) {  // Riemann_routines.c:74-85
+      double FL[NQ], FR[NQ], UL[NQ], UR[NQ];
+      d_set_flux<NQ>(primL, FL, r, n0, n1, n2, G);
+      d_set_flux<NQ>(primR, FR, r, n0, n1, n2, G);
+      d_prim2cons<NQ>(primL, UL, r, 1.0, G, P.NC);
+      d_prim2cons<NQ>(primR, UR, r, 1.0, G, P.NC);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double Ustar = (Sr * UR[q] - Sl * UL[q] + FL[q] - FR[q]) / (Sr - Sl);
+        double Fstar = (Sr * FL[q] - Sl * FR[q] - Sr * Sl * (UL[q] - UR[q])) / (Sr - Sl);
+        F[q] = Fstar - w * Ustar;
+      }
+    } else {  // HLLC, Riemann_routines.c:87-154
+      const double *prim = state == 1 ? primL : primR;
+      double Sk = state == 1 ? Sl : Sr;
+      double Fk[NQ], Uk[NQ], Ustar[NQ];
+      d_set_flux<NQ>(prim, Fk, r, n0, n1, n2, G);
+      d_prim2cons<NQ>(prim, Uk, r, 1.0, G, P.NC);
+      double rho = prim[RHO];
+      double vr = prim[URR];
+      double vp = prim[UPP] * r;
+      double vz = prim[UZZ];
+      double Pp = prim[PPP];
+      double v2 = vr * vr + vp * vp + vz * vz;
+      double vn = vr * n0 + vp * n1 + vz * n2;
+      double rhoe = Pp / (G - 1.);
+      double D = rho;
+      double mr = rho * vr;
+      double mp = rho * vp;
+      double mz = rho * vz;
+      double E_hydro = .5 * rho * v2 + rhoe;
+      double Ps = rho * (Sk - vn) * (Ss - vn) + Pp;
+      double Dstar = (Sk - vn) * D / (Sk - Ss);
+      double Msn = Dstar * Ss;
+      double Msr = (Sk - vn) * mr / (Sk - Ss);
+      double Msp = (Sk - vn) * mp / (Sk - Ss);
+      double Msz = (Sk - vn) * mz / (Sk - Ss);
+      double Estar = ((Sk - vn) * E_hydro + Ps * Ss - Pp * vn) / (Sk - Ss);
+      double mn = Msr * n0 + Msp * n1 + Msz * n2;
+      Msr += n0 * (Msn - mn);
+      Msp += n1 * (Msn - mn);
+      Msz += n2 * (Msn - mn);
+      Ustar[DDD] = Dstar;
+      Ustar[SRR] = Msr;
+      Ustar[LLL] = r * Msp;
+      Ustar[SZZ] = Msz;
+      Ustar[TAU] = Estar;
+#pragma unroll
+      for (int q = 5; q < NQ; ++q) Ustar[q] = prim[q] * Ustar[DDD];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        double Fstar = Fk[q] + Sk * (Ustar[q] - Uk[q]);
+        F[q] = Fstar - w * Ustar[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) X[q] = dtdA * F[q];
+  }
+  if (PART != 1 && P.visc > 0.0) {  // riemann_visc_flux, :193-311
     // only the entries the flux formulas use are formed: Avg for all five
     // primitives, the r and phi gradients of v_r and omega
     double Avg[5], Grr[2], Gph[2];
