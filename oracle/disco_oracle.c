@@ -1431,3 +1431,141 @@ int oracle_run(OracleSim *s, int nsteps) {
 
 /* accumulated torque term (test infrastructure for the adjoint's loss) */
 double oracle_torque(OracleSim *s) { return s->torque; }
+
+/* ---------------- diagnostics: src/Diagnostics/diagnostics_set.c ---------------- */
+/* number of equatorial cells, diagnostics_create_destroy.c:34-51 */
+int oracle_diag_neq(OracleSim *s) {
+  int n = 0;
+  const int imin = s->ip[OI_NgMinR], imax = s->NR - s->ip[OI_NgMaxR];
+  const int kmin = s->ip[OI_NgMinZ], kmax = s->NZ - s->ip[OI_NgMaxZ];
+  for (int k = kmin; k < kmax; ++k) {
+    double zp = zpos(s, k), zm = zpos(s, k - 1), z = 0.5 * (zm + zp);
+    for (int i = imin; i < imax; ++i)
+      for (int j = 0; j < s->N_p[i]; ++j)
+        if ((fabs(zp) < 0.0000001) || (fabs(z) < 0.0000001)) ++n;
+  }
+  return n;
+}
+
+/* One measurement of diagnostics_set (diagnostics_set.c:13-303) on the
+ * current state, one rank.  dtout = t - toutprev; rz = {R_Min, R_Max, Z_Min,
+ * Z_Max} (sim_MIN/sim_MAX).  Outputs, as the reference forms them before
+ * adding them to its running sums:
+ *   vec[NRg * 17]   VectorDiag_reduce after its dtout/(ZMAX-ZMIN) scaling
+ *   scal[16]        ScalarDiag_reduce after its scaling
+ *   equat[Neq * 18] EquatDiag
+ *   mdot[12]        the DiagMdot.dat line after its t: Mdot_near_req1,
+ *                   r_near_req1, mass_inside_1, mass_inner_wedge, Mdot0,
+ *                   Mdot1, Macc0, Macc1, masses within 0.2 and 0.4 of each mass
+ * The passive scalar column reads prim[5]; with NUM_Q = 5 the reference reads
+ * past the primitive array there, this restatement writes 0. */
+void oracle_diag_measure(OracleSim *s, double dtout, const double *rz, double *vec, double *scal,
+                         double *equat, double *mdot) {
+  const int NV = 17, NE = 18, NS = 16;
+  const int imin = s->ip[OI_NgMinR], imax = s->NR - s->ip[OI_NgMaxR];
+  const int kmin = s->ip[OI_NgMinZ], kmax = s->NZ - s->ip[OI_NgMaxZ];
+  const int nrg = imax - imin;
+  const double G = s->dp[OD_GAMMALAW];
+  double m01[2] = {0, 0}, m02[2] = {0, 0}, m04[2] = {0, 0}, inside1 = 0.0;
+  for (int n = 0; n < nrg * NV; ++n) vec[n] = 0.0;
+  for (int n = 0; n < NS; ++n) scal[n] = 0.0;
+  int position = 0;
+  for (int k = kmin; k < kmax; ++k) {
+    double zp = zpos(s, k), zm = zpos(s, k - 1);
+    double z = 0.5 * (zm + zp), dz = zp - zm;
+    for (int i = imin; i < imax; ++i) {
+      double rp = rpos(s, i), rm = rpos(s, i - 1);
+      double r = 0.5 * (rm + rp);
+      double a = s->gm[0].r + s->gm[1].r;
+      for (int j = 0; j < s->N_p[i]; ++j) {
+        int c = zid(s, k, i, j);
+        double dphi = s->dphi[c];
+        double phi = s->tiph[c] - 0.5 * dphi;
+        double rho = s->prim[RHO * s->Nz + c];
+        double press = s->prim[PPP * s->Nz + c];
+        double vr = s->prim[URR * s->Nz + c];
+        double vp_minus_w = s->prim[UPP * s->Nz + c] * r;
+        double vp = s->prim[UPP * s->Nz + c] * r + rOm_a(s, r, a);
+        double vz = s->prim[UZZ * s->Nz + c];
+        double v2 = vr * vr + vp * vp + vz * vz;
+        double rhoe = press / (G - 1.);
+        double KE = 0.5 * rho * v2;
+        double E_hydro = rhoe + KE;
+        double Cool = s->Cool[c];
+        double dV = 0.5 * (rp * rp - rm * rm) * dphi;
+        if (rp <= 1.0) inside1 += rho * dV;
+        double passive = s->NQ > 5 ? s->prim[5 * s->Nz + c] : 0.0;
+        double d0 = grav_dist(s, 0, r, phi, 0.), d1 = grav_dist(s, 1, r, phi, 0.);
+        if (d0 < 0.1) m01[0] += rho * dV;
+        if (d1 < 0.1) m01[1] += rho * dV;
+        if (d0 < 0.2) m02[0] += rho * dV;
+        if (d1 < 0.2) m02[1] += rho * dV;
+        if (d0 < 0.4) m04[0] += rho * dV;
+        if (d1 < 0.4) m04[1] += rho * dV;
+        if ((fabs(zp) < 0.0000001) || (fabs(z) < 0.0000001)) {
+          double *e = equat + (size_t)position * NE;
+          e[0] = r; e[1] = phi; e[2] = rho; e[3] = press; e[4] = vr; e[5] = vp; e[6] = vz;
+          e[7] = vp_minus_w; e[8] = Cool; e[9] = passive; e[10] = E_hydro; e[11] = KE; e[12] = rhoe;
+          e[13] = e[14] = e[15] = e[16] = e[17] = 0.0;
+          ++position;
+        }
+        double *v = vec + (size_t)(i - imin) * NV;
+        double np = s->N_p[i];
+        v[0] += (r / np * dz);
+        v[1] += (rho / np * dz);
+        v[2] += (press / np * dz);
+        v[3] += (vr / np * dz);
+        v[4] += (vp / np * dz);
+        v[5] += (vz / np * dz);
+        v[6] += (vp_minus_w / np * dz);
+        v[7] += (Cool / np * dz);
+        v[8] += (passive / np * dz);
+        v[9] += (E_hydro / np * dz);
+        v[10] += (KE / np * dz);
+        v[11] += (rhoe / np * dz);
+        for (int n = 12; n < NV; ++n) v[n] += 0.0;
+      }
+    }
+  }
+  int i;
+  for (i = imin; i < imax; ++i) {
+    double rp = rpos(s, i), rm = rpos(s, i - 1);
+    for (int n = 0; n < NS; ++n) scal[n] += vec[(size_t)(i - imin) * NV + n + 1] * (rp * rp - rm * rm);
+  }
+  /* :201-211: the loop bound is N_p of the index the previous loop left (imax),
+   * and cell (0, j, 0) runs on through the layer's contiguous cells */
+  double wedge = 0.0;
+  {
+    double rp = rpos(s, 0), rm = rpos(s, -1);
+    for (int j = 0; j < s->N_p[i]; ++j) {
+      double dV = 0.5 * (rp * rp - rm * rm) * s->dphi[j];
+      wedge += s->prim[RHO * s->Nz + j] * dV;
+    }
+  }
+  double RMIN = rz[0], RMAX = rz[1], ZMIN = rz[2], ZMAX = rz[3];
+  for (int n = 0; n < NS; ++n) scal[n] *= dtout / ((ZMAX - ZMIN) * (RMAX * RMAX - RMIN * RMIN));
+  int found = 0;
+  double Mdot_req1 = 0.0, r_req1 = 0.0;
+  for (int ii = 0; ii < nrg; ++ii) {
+    double r = vec[(size_t)ii * NV] / (ZMAX - ZMIN);
+    if (r > 1.0 && found == 0) {
+      Mdot_req1 = vec[(size_t)ii * NV + 5] * 2. * M_PI * r / (ZMAX - ZMIN);
+      r_req1 = r;
+      found = 1;
+    }
+    for (int n = 0; n < NV; ++n) vec[(size_t)ii * NV + n] *= dtout / (ZMAX - ZMIN);
+  }
+  mdot[0] = Mdot_req1;
+  mdot[1] = r_req1;
+  mdot[2] = inside1;
+  mdot[3] = wedge;
+  mdot[4] = s->gm[0].Mdot;
+  mdot[5] = s->gm[1].Mdot;
+  mdot[6] = s->gm[0].Macc;
+  mdot[7] = s->gm[1].Macc;
+  mdot[8] = m02[0];
+  mdot[9] = m02[1];
+  mdot[10] = m04[0];
+  mdot[11] = m04[1];
+  (void)m01;
+}

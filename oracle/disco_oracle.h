@@ -63,6 +63,10 @@ void oracle_plm_rz(OracleSim *s, int dir);
 double oracle_mindt(OracleSim *s);
 void oracle_set_w(OracleSim *s);
 int oracle_last_error(OracleSim *s);
+/* diagnostics_set (src/Diagnostics/diagnostics_set.c): see disco_oracle.c */
+int oracle_diag_neq(OracleSim *s);
+void oracle_diag_measure(OracleSim *s, double dtout, const double *rz, double *vec, double *scal,
+                         double *equat, double *mdot);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,10 @@ def lib():
         L.oracle_mindt.restype = ctypes.c_double
         L.oracle_last_error.argtypes = [vp]
         L.oracle_last_error.restype = ctypes.c_int
+        L.oracle_diag_neq.argtypes = [vp]
+        L.oracle_diag_neq.restype = ctypes.c_int
+        L.oracle_diag_measure.argtypes = [vp, ctypes.c_double, dp, dp, dp, dp, dp]
+        L.oracle_diag_measure.restype = None
         _lib = L
     return _lib
 
@@ -148,6 +152,10 @@ class Oracle:
         lib().oracle_scalars(self.h, _dp(out))
         return out
 
+    def set_scalars(self, v):
+        """t, dt, RK, then 2 x {r, phi, M, omega, RK_r, RK_phi, RK_M, RK_omega, Mdot, Macc}"""
+        lib().oracle_set_scalars(self.h, _dp(np.ascontiguousarray(v, dtype=np.float64)))
+
     def faces(self, direction):
         L = lib()
         nf = L.oracle_face_create(self.h, direction)
@@ -165,6 +173,77 @@ class Oracle:
 
     def mindt(self):
         return lib().oracle_mindt(self.h)
+
+    def diag_measure(self, dtout, rz):
+        """one measurement of diagnostics_set on the current state:
+        (vector (NRg, 17), scalar (16,), equat (Neq, 18), mdot (12,))"""
+        L = lib()
+        nrg = int(self.params["NumR"])  # sim_N_global(R_DIR) on one rank
+        neq = L.oracle_diag_neq(self.h)
+        vec, sc, eq, md = np.zeros(nrg * 17), np.zeros(16), np.zeros(max(neq, 1) * 18), np.zeros(12)
+        L.oracle_diag_measure(self.h, float(dtout), _dp(np.ascontiguousarray(rz, dtype=np.float64)), _dp(vec),
+                              _dp(sc), _dp(eq), _dp(md))
+        return vec.reshape(nrg, 17), sc, eq[:neq * 18].reshape(neq, 18), md
+
+
+class Diagnostics:
+    """The diagnostics' schedule, running sums and text files
+    (src/Diagnostics/diagnostics_create_destroy.c:9-118, diagnostics_set.c,
+    diagnostics_print.c:11-92), one rank, around anything with a
+    diag_measure(dtout, rz) method: the oracle here, the GPU batch in tests.
+    The HDF5 equatorial dump is not written by this restatement."""
+
+    NUM_DIAG = 16
+
+    def __init__(self, source, params, t):
+        self.src = source
+        T = float(params["Time_Max"])
+        self.dtm = T / int(params["Num_Diag_Measure"])
+        self.dtd = T / int(params["Num_Diag_Dump"])
+        self.tm, self.td = self.dtm, self.dtd
+        while self.tm < t:
+            self.tm += self.dtm
+        while self.td < t:
+            self.td += self.dtd
+        self.toutprev = self.toutprev_dump = t
+        self.rz = np.array([params["R_Min"], params["R_Max"], params["Z_Min"], params["Z_Max"]], dtype=np.float64)
+        self.vec = None
+        self.scal = np.zeros(self.NUM_DIAG)
+        self.equat = None
+
+    def set(self, t, outdir=None):
+        """diagnostics_set: True when it measured"""
+        if not t > self.tm:
+            return False
+        v, sc, eq, md = self.src.diag_measure(t - self.toutprev, self.rz)
+        if outdir is not None:
+            with open(os.path.join(outdir, "DiagMdot.dat"), "a") as f:
+                f.write(" ".join("%e" % x for x in (t, *md)) + "\n")
+        self.vec = v.copy() if self.vec is None else self.vec + v
+        self.scal = self.scal + sc
+        self.equat = eq
+        self.toutprev = t
+        self.tm += self.dtm
+        return True
+
+    def print(self, t, outdir=None):
+        """diagnostics_print: True when it dumped"""
+        if not t > self.td:
+            return False
+        dt_dump = t - self.toutprev_dump
+        if outdir is not None:
+            vec = self.vec if self.vec is not None else np.zeros((0, 17))
+            with open(os.path.join(outdir, "DiagVector_%010.2f.dat" % t), "w") as f:
+                for row in vec:
+                    f.write("".join("%e " % (x / dt_dump) for x in row) + "\n")
+            with open(os.path.join(outdir, "DiagScalar.dat"), "a") as f:
+                f.write("%e " % t + "".join("%e " % (x / dt_dump) for x in self.scal) + "\n")
+        self.toutprev_dump = t
+        if self.vec is not None:
+            self.vec = np.zeros_like(self.vec)
+        self.scal = np.zeros(self.NUM_DIAG)
+        self.td += self.dtd
+        return True
 
 
 # ---------------------------------------------------------------- reference build

@@ -19,6 +19,10 @@
  *   disco_ref run   <parfile> <icfile> <out> <nsteps> [dumpstep ...]
  *   disco_ref unit  <parfile> <icfile> <out>
  *   disco_ref time  <parfile> <icfile> <nsteps>
+ *   disco_ref diag  <parfile> <icfile> <out> <nsteps>
+ *     (steps with the diagnostics of src/main.c:121-131; the text files the
+ *     reference writes land in the working directory; every measurement
+ *     also dumps the diagnostics' arrays at full precision)
  *
  * icfile layout (little endian):
  *   int32 magic 0x44534331, int32 NR (annuli incl. ghosts), int32 NZ
@@ -37,6 +41,7 @@
 #define FACE_PRIVATE_DEFS
 #define TIMESTEP_PRIVATE_DEFS
 #define PLANET_PRIVATE_DEFS
+#define DIAGNOSTICS_PRIVATE_DEFS
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -48,6 +53,7 @@
 #include "Face.h"
 #include "GravMass.h"
 #include "TimeStep.h"
+#include "Diagnostics.h"
 #include "header.h"
 
 static FILE *g_out = NULL;
@@ -329,6 +335,56 @@ int main(int argc, char **argv) {
       put_rec("dts", 0, dts, nsteps);
       fclose(g_out);
     }
+    MPI_Finalize();
+    return 0;
+  }
+  if (strcmp(mode, "diag") == 0) {
+    int nsteps = atoi(argv[5]);
+    g_out = fopen(argv[4], "wb");
+    dump_geom(sim);
+    FILE *so = freopen("/dev/null", "w", stdout);
+    (void)so;
+    /* src/main.c:121-131 */
+    struct Diagnostics *dg = diagnostics_create(sim, ts, mpi);
+    int32_t neq = dg->N_eq_cells;
+    put_rec("diag/N_eq", 1, &neq, 1);
+    put_rec("diag/N_p_global", 1, dg->N_p_global, sim_N_global(sim, R_DIR));
+    int nvec = sim_N_global(sim, R_DIR) * (dg->NUM_DIAG + 1);
+    int nmeas = 0, ndumps = 0, s2;
+    for (s2 = 1; s2 <= nsteps; ++s2) {
+      timestep_rk2(ts, sim, cells, gm, mpi);
+      double tm = dg->tdiag_measure;
+      diagnostics_set(dg, cells, sim, ts, mpi, gm);
+      if (dg->tdiag_measure != tm) {
+        char nm[64];
+        double t = ts->t;
+        snprintf(nm, sizeof nm, "meas%d/step_t", nmeas);
+        double st[2] = {(double)s2, t};
+        put_rec(nm, 0, st, 2);
+        snprintf(nm, sizeof nm, "meas%d/state", nmeas);
+        dump_state(nm, cells, sim, gm, ts);
+        snprintf(nm, sizeof nm, "meas%d/vector", nmeas);
+        put_rec(nm, 0, dg->VectorDiag[0], nvec);
+        snprintf(nm, sizeof nm, "meas%d/scalar", nmeas);
+        put_rec(nm, 0, dg->ScalarDiag, dg->NUM_DIAG);
+        snprintf(nm, sizeof nm, "meas%d/equat", nmeas);
+        put_rec(nm, 0, dg->EquatDiag[0], (int64_t)dg->N_eq_cells * (dg->NUM_DIAG + 2));
+        ++nmeas;
+      }
+      double td = dg->tdiag_dump;
+      diagnostics_print(dg, ts, sim, mpi);
+      if (dg->tdiag_dump != td) {
+        char nm[64];
+        snprintf(nm, sizeof nm, "dump%d/step_t", ndumps);
+        double st[2] = {(double)s2, ts->t};
+        put_rec(nm, 0, st, 2);
+        ++ndumps;
+      }
+    }
+    int32_t cnt[2] = {nmeas, ndumps};
+    put_rec("diag/counts", 1, cnt, 2);
+    diagnostics_destroy(dg, sim);
+    fclose(g_out);
     MPI_Finalize();
     return 0;
   }

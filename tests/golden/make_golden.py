@@ -47,6 +47,17 @@ def cases():
                               HiResSigma=0.3, HiResR0a=1.0, ZLogScale=0.5)
     c["disk3d_refNp_hires"] = ((p, k, kw), dict(steps=6, dumps=[1, 6], unit=True))
     c["vortex_r64_100"] = (configs.vortex(64), dict(steps=100, dumps=[100], unit=False, light=True))
+    # diagnostics (src/Diagnostics): measured every step; the vortex and the
+    # 3D disk dump every few measurements (time-averaged profiles), the binary
+    # (whose first step is long) every step
+    p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=1, PLM=2.0, RiemannSolver=1, NoInnerBC=1,
+                              Time_Max=10.0, Num_Diag_Measure=400, Num_Diag_Dump=100)
+    c["diag_vortex_scalar"] = ((p, k, dict(kw, x0=1.8)), dict(diag=10))
+    p, k, kw = configs.binary(16, r_min=0.3, r_max=3.0, Time_Max=1.0, Num_Diag_Measure=2000, Num_Diag_Dump=1200)
+    c["diag_binary_r16"] = ((p, k, kw), dict(diag=8))
+    p, k, kw = configs.disk3d(8, 4, 24, R_Min=0.5, R_Max=2.5, Time_Max=10.0, Num_Diag_Measure=100,
+                              Num_Diag_Dump=40)
+    c["diag_disk3d_r8z4"] = ((p, k, kw), dict(diag=8))
     return c
 
 
@@ -71,6 +82,24 @@ def make_case(name, spec, opts, tmp):
     out = {"params": np.array(json.dumps(params)), "N_p": N_p, "N_p_ref_formula": g0["N_p"],
            "r_faces": g["r_faces"], "z_faces": g["z_faces"], "tiph0": g["tiph0"], "dphi0": g["dphi0"],
            "prim0": prim0.ravel(), "dims": g["dims"]}
+    if opts.get("diag"):
+        # the text files the reference appends to land in a directory of their own
+        dd = os.path.join(tmp, name + "_diag")
+        os.makedirs(dd, exist_ok=True)
+        orc.run_reference(["diag", par, ic, os.path.join(dd, "d.bin"), str(opts["diag"])], dd)
+        r = orc.read_records(os.path.join(dd, "d.bin"))
+        for k, v in r.items():
+            if "/state/" in k and not k.endswith(("/prim", "/tiph", "/dphi", "/gravmass", "/t_dt")):
+                continue
+            if k.startswith(("diag/", "meas", "dump")):
+                out["diag/" + k if not k.startswith("diag/") else k] = v
+        for fn in sorted(os.listdir(dd)):
+            if fn.endswith(".dat"):
+                with open(os.path.join(dd, fn)) as f:
+                    out["file/" + fn] = np.array(f.read())
+        path = os.path.join(HERE, name + ".npz")
+        np.savez_compressed(path, **out)
+        return path
     if opts.get("unit"):
         u = orc.read_records(_run(["unit", par, ic, os.path.join(tmp, "u.bin")], tmp, "u.bin"))
         for k, v in u.items():
