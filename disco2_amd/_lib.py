@@ -116,6 +116,15 @@ SIGNATURES = {
     "disco_calibrate_copy": (cd, [cll]),
     "disco_batch_torque_loss": (ci, [vp, ci, pd]),
     "disco_batch_adjoint": (ci, [vp, ci, ci, pd, pd, pd, pd]),
+    "disco_batch_diag_sizes": (ci, [vp, ci, pi, pi]),
+    "disco_batch_diag_measure": (ci, [vp, ci, cd, pd, pd, pd, pd, pd]),
+    "disco_batch_set_scalars": (None, [vp, ci, pd]),
+    "diagnostics_create": (vp, [vp, vp, vp]),
+    "diagnostics_destroy": (None, [vp, vp]),
+    "diagnostics_set": (None, [vp, vp, vp, vp, vp, vp]),
+    "diagnostics_print": (None, [vp, vp, vp, vp]),
+    "diagnostics_tdiag_measure": (cd, [vp]),
+    "diagnostics_tdiag_dump": (cd, [vp]),
     "disco_last_error": (ci, []),
     "disco_last_error_msg": (ccp, []),
     "disco_clear_error": (None, []),

@@ -10,6 +10,8 @@
 #include "../../include/disco.h"
 #include "engine.h"
 
+bool diag_h5_write(const char *fname, const double *data, size_t n0, size_t n1);  // diagnostics.cpp
+
 // ------------------------------------------------------------------ errors
 static int g_err = 0;
 static std::string g_msg;
@@ -563,5 +565,135 @@ void disco_batch_scalars(DiscoBatch *b, int d, double *o) {
     g[4] = s.gm[p].RK_r; g[5] = s.gm[p].RK_phi; g[6] = s.gm[p].RK_M; g[7] = s.gm[p].RK_omega;
     g[8] = s.gm[p].Mdot; g[9] = s.gm[p].Macc;
   }
+}
+
+// ------------------------------------------------------------------ diagnostics
+// src/Diagnostics, one rank: the measurement on the device
+// (Engine::diag_measure); schedule, running sums and files as
+// diagnostics_create_destroy.c:9-118, diagnostics_set.c:13-303 and
+// diagnostics_print.c:11-92.
+}  // extern "C"
+struct Diagnostics {
+  int nrg = 0, neq = 0;
+  std::vector<double> vec, scal, equat;
+  double toutprev = 0, toutprev_dump = 0;
+  double tdiag_measure = 0, dtdiag_measure = 0, tdiag_dump = 0, dtdiag_dump = 0;
+  double rz[4] = {0, 0, 0, 0};
+};
+extern "C" {
+
+Diagnostics *diagnostics_create(Sim *sim, TimeStep *ts, MPIsetup *) {
+  Diagnostics *d = new Diagnostics;
+  const int imin = sim_Nghost_min(sim, 0), imax = sim_N(sim, 0) - sim_Nghost_max(sim, 0);
+  const int kmin = sim_Nghost_min(sim, 1), kmax = sim_N(sim, 1) - sim_Nghost_max(sim, 1);
+  for (int k = kmin; k < kmax; ++k) {
+    const double zp = sim_FacePos(sim, k, 1), zm = sim_FacePos(sim, k - 1, 1), z = 0.5 * (zm + zp);
+    for (int i = imin; i < imax; ++i)
+      for (int j = 0; j < sim_N_p(sim, i); ++j)
+        if ((fabs(zp) < 0.0000001) || (fabs(z) < 0.0000001)) ++d->neq;
+  }
+  d->nrg = sim_N_global(sim, 0);
+  d->vec.assign((size_t)d->nrg * 17, 0.0);
+  d->scal.assign(16, 0.0);
+  d->equat.assign((size_t)d->neq * 18, 0.0);
+  const double t = timestep_get_t(ts);
+  d->toutprev = d->toutprev_dump = t;
+  d->dtdiag_measure = sim_get_T_MAX(sim) / sim_NUM_DIAG_MEASURE(sim);
+  d->tdiag_measure = d->dtdiag_measure;
+  d->dtdiag_dump = sim_get_T_MAX(sim) / sim_NUM_DIAG_DUMP(sim);
+  d->tdiag_dump = d->dtdiag_dump;
+  while (d->tdiag_measure < t) d->tdiag_measure += d->dtdiag_measure;
+  while (d->tdiag_dump < t) d->tdiag_dump += d->dtdiag_dump;
+  d->rz[0] = sim_MIN(sim, 0);
+  d->rz[1] = sim_MAX(sim, 0);
+  d->rz[2] = sim_MIN(sim, 1);
+  d->rz[3] = sim_MAX(sim, 1);
+  return d;
+}
+
+void diagnostics_destroy(Diagnostics *d, Sim *) { delete d; }
+
+double diagnostics_tdiag_measure(Diagnostics *d) { return d->tdiag_measure; }
+double diagnostics_tdiag_dump(Diagnostics *d) { return d->tdiag_dump; }
+
+void diagnostics_set(Diagnostics *d, Cell ***c, Sim *, TimeStep *ts, MPIsetup *, GravMass *gm) {
+  const double t = timestep_get_t(ts);
+  if (!(t > d->tdiag_measure)) return;
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  scalars_from_host(e, ts, gm);
+  std::vector<double> v(d->vec.size()), s(16), md(12);
+  if (e->diag_measure(0, t - d->toutprev, d->rz, v.data(), s.data(), d->equat.data(), md.data()) != 0) return;
+  for (int p = 0; p < 2; ++p) {  // the masses' accretion as the host GravMass holds it
+    md[4 + p] = gm[p].Mdot;
+    md[6 + p] = gm[p].Macc;
+  }
+  if (FILE *f = fopen("DiagMdot.dat", "a")) {
+    fprintf(f, "%e", t);
+    for (double x : md) fprintf(f, " %e", x);
+    fprintf(f, "\n");
+    fclose(f);
+  }
+  for (size_t x = 0; x < v.size(); ++x) d->vec[x] += v[x];
+  for (int n = 0; n < 16; ++n) d->scal[n] += s[n];
+  d->toutprev = t;
+  d->tdiag_measure += d->dtdiag_measure;
+}
+
+void diagnostics_print(Diagnostics *d, TimeStep *ts, Sim *, MPIsetup *) {
+  const double t = timestep_get_t(ts);
+  if (!(t > d->tdiag_dump)) return;
+  const double dt_dump = t - d->toutprev_dump;
+  char fe[256], fv[256];
+  snprintf(fe, sizeof fe, "DiagEquat_%010.2f.h5", t);
+  snprintf(fv, sizeof fv, "DiagVector_%010.2f.dat", t);
+  FILE *vf = fopen(fv, "w"), *sf = fopen("DiagScalar.dat", "a");
+  if (vf) {
+    for (int i = 0; i < d->nrg; ++i) {
+      for (int n = 0; n < 17; ++n) fprintf(vf, "%e ", d->vec[(size_t)i * 17 + n] / dt_dump);
+      fprintf(vf, "\n");
+    }
+    fclose(vf);
+  }
+  if (sf) {
+    fprintf(sf, "%e ", t);
+    for (int n = 0; n < 16; ++n) fprintf(sf, "%e ", d->scal[n] / dt_dump);
+    fprintf(sf, "\n");
+    fclose(sf);
+  }
+  diag_h5_write(fe, d->equat.data(), (size_t)d->neq, 18);
+  d->toutprev_dump = t;
+  std::fill(d->vec.begin(), d->vec.end(), 0.0);
+  std::fill(d->scal.begin(), d->scal.end(), 0.0);
+  d->tdiag_dump += d->dtdiag_dump;
+}
+
+int disco_batch_diag_sizes(DiscoBatch *b, int d, int *nrg, int *neq) {
+  if (d < 0 || d >= b->e->ndisk) {
+    disco_set_error(14, "diagnostics: no such disk");
+    return -1;
+  }
+  return b->e->diag_sizes(d, nrg, neq);
+}
+int disco_batch_diag_measure(DiscoBatch *b, int d, double dtout, const double *rz, double *vec, double *scal,
+                             double *equat, double *mdot) {
+  return b->e->diag_measure(d, dtout, rz, vec, scal, equat, mdot);
+}
+void disco_batch_set_scalars(DiscoBatch *b, int d, const double *in) {
+  Engine *e = b->e;
+  DiskScalars s = e->get_scalars(d);
+  s.t = in[0];
+  s.dt = in[1];
+  s.RK = in[2];
+  for (int p = 0; p < 2; ++p) {
+    const double *g = in + 3 + 10 * p;
+    s.gm[p].r = g[0]; s.gm[p].phi = g[1]; s.gm[p].M = g[2]; s.gm[p].omega = g[3];
+    s.gm[p].RK_r = g[4]; s.gm[p].RK_phi = g[5]; s.gm[p].RK_M = g[6]; s.gm[p].RK_omega = g[7];
+    s.gm[p].Mdot = g[8]; s.gm[p].Macc = g[9];
+  }
+  e->put_scalars(d, s);
+  launch_scalar(e->mesh(), e->plan, e->st, SC_MASS);
+  e->check_error();
 }
 }

@@ -1,0 +1,190 @@
+// Host side of the diagnostics: one device measurement of a disk
+// (Engine::diag_measure), and the writer of the reference's output files.
+// The class API (diagnostics_create / _set / _print / _destroy,
+// src/Headers/Diagnostics.h:29-38) is in capi.cpp on top of these.
+#include <dlfcn.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "diagnostics.h"
+#include "disco_internal.h"
+#include "engine.h"
+
+int Engine::diag_sizes(int d, int *nrg, int *neq) const {
+  const DiskParams &P = hdp[d];
+  const int imin = P.NgMinR, imax = P.NR - P.NgMaxR, kmin = P.NgMinZ, kmax = P.NZ - P.NgMaxZ;
+  *nrg = imax - imin;
+  int n = 0;  // diagnostics_create_destroy.c:34-51
+  for (int k = kmin; k < kmax; ++k) {
+    const AnnInfo &A = hann[P.ann_base + k * P.NR];
+    const double zp = A.zp, z = 0.5 * (A.zm + A.zp);
+    if ((fabs(zp) < 0.0000001) || (fabs(z) < 0.0000001))
+      for (int i = imin; i < imax; ++i) n += hann[P.ann_base + k * P.NR + i].N;
+  }
+  *neq = n;
+  return 0;
+}
+
+int Engine::diag_measure(int d, double dtout, const double *rz, double *vec, double *scal, double *equat,
+                         double *mdot) {
+  if (!ok) return -1;
+  if (d < 0 || d >= ndisk) {
+    disco_set_error(14, "diagnostics: no such disk");
+    return -1;
+  }
+  const DiskParams &P = hdp[d];
+  if (P.cooling == 1) {
+    disco_set_error(14, "diagnostics: the Cool column needs Cooling 0 (the engine keeps no per-cell Cool)");
+    return -1;
+  }
+  int nrg, neq;
+  diag_sizes(d, &nrg, &neq);
+  const int imin = P.NgMinR, kmin = P.NgMinZ, kmax = P.NZ - P.NgMaxZ;
+  std::vector<int> eqb(P.NZ, -1), ioff(nrg, 0);
+  {
+    int pos = 0, per_layer = 0;
+    for (int ii = 0; ii < nrg; ++ii) {
+      ioff[ii] = per_layer;
+      per_layer += hann[P.ann_base + kmin * P.NR + imin + ii].N;
+    }
+    for (int k = kmin; k < kmax; ++k) {
+      const AnnInfo &A = hann[P.ann_base + k * P.NR];
+      const double zp = A.zp, z = 0.5 * (A.zm + A.zp);
+      if ((fabs(zp) < 0.0000001) || (fabs(z) < 0.0000001)) {
+        eqb[k] = pos;
+        pos += per_layer;
+      }
+    }
+  }
+  // device scratch: layer bases, annulus offsets, profiles, mass partials,
+  // scalars, equatorial cells, masses
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t o_eqb = 0, o_ioff = o_eqb + al(sizeof(int) * P.NZ), o_vec = o_ioff + al(sizeof(int) * nrg),
+               o_mp = o_vec + al(8 * (size_t)nrg * DIAG_NV), o_sc = o_mp + al(8 * (size_t)nrg * 5),
+               o_eq = o_sc + al(8 * DIAG_NS), o_out = o_eq + al(8 * (size_t)neq * DIAG_NE), total = o_out + al(8 * 8);
+  if (total > diag_cap) {
+    if (diag_mem) hipFree(diag_mem);
+    diag_mem = nullptr;
+    diag_cap = 0;
+    if (!hip_check(hipMalloc(&diag_mem, total), "diagnostics")) return -1;
+    diag_cap = total;
+  }
+  char *b = (char *)diag_mem;
+  DiagArgs g;
+  g.disk = d;
+  g.nrg = nrg;
+  g.imin = imin;
+  g.kmin = kmin;
+  g.kmax = kmax;
+  g.dtout = dtout;
+  for (int x = 0; x < 4; ++x) g.rz[x] = rz[x];
+  g.eq_base = (const int *)(b + o_eqb);
+  g.ioff = (const int *)(b + o_ioff);
+  g.vec = (double *)(b + o_vec);
+  g.mpart = (double *)(b + o_mp);
+  g.scal = (double *)(b + o_sc);
+  g.equat = (double *)(b + o_eq);
+  g.out_mass = (double *)(b + o_out);
+  hip_check(hipMemcpyAsync(b + o_eqb, eqb.data(), sizeof(int) * P.NZ, hipMemcpyHostToDevice, st), "diagnostics");
+  if (nrg) hip_check(hipMemcpyAsync(b + o_ioff, ioff.data(), sizeof(int) * nrg, hipMemcpyHostToDevice, st), "diagnostics");
+  launch_diag(mesh(), g, st);
+  double out[8];
+  hip_check(hipMemcpyAsync(vec, g.vec, 8 * (size_t)nrg * DIAG_NV, hipMemcpyDeviceToHost, st), "diagnostics");
+  hip_check(hipMemcpyAsync(scal, g.scal, 8 * DIAG_NS, hipMemcpyDeviceToHost, st), "diagnostics");
+  if (neq) hip_check(hipMemcpyAsync(equat, g.equat, 8 * (size_t)neq * DIAG_NE, hipMemcpyDeviceToHost, st), "diagnostics");
+  hip_check(hipMemcpyAsync(out, g.out_mass, 8 * 8, hipMemcpyDeviceToHost, st), "diagnostics");
+  if (check_error()) return -1;
+  const DiskScalars s = get_scalars(d);
+  mdot[0] = out[6];  // Mdot_near_req1 (the reference's probe reads the v_z profile column)
+  mdot[1] = out[7];  // r_near_req1
+  mdot[2] = out[0];  // mass inside r = 1
+  mdot[3] = out[5];  // inner wedge
+  mdot[4] = s.gm[0].Mdot;
+  mdot[5] = s.gm[1].Mdot;
+  mdot[6] = s.gm[0].Macc;
+  mdot[7] = s.gm[1].Macc;
+  mdot[8] = out[1];
+  mdot[9] = out[2];
+  mdot[10] = out[3];
+  mdot[11] = out[4];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The equatorial dump of diagnostics_print (diagnostics_print.c:40-80) is an
+// HDF5 file.  The library is loaded when first needed (DISCO_HDF5_LIB, else
+// libhdf5.so on the loader path or in /opt/conda/lib); without it the .h5
+// file is skipped and diag_h5_write returns false.
+namespace {
+
+typedef int64_t hid_t;
+typedef int herr_t;
+typedef unsigned long long hsize_t;
+
+struct H5Api {
+  bool tried = false, ok = false;
+  herr_t (*open)(void);
+  hid_t (*fcreate)(const char *, unsigned, hid_t, hid_t);
+  hid_t (*screate_simple)(int, const hsize_t *, const hsize_t *);
+  hid_t (*dcreate2)(hid_t, const char *, hid_t, hid_t, hid_t, hid_t, hid_t);
+  hid_t (*dget_space)(hid_t);
+  herr_t (*dwrite)(hid_t, hid_t, hid_t, hid_t, hid_t, const void *);
+  herr_t (*dclose)(hid_t);
+  herr_t (*sclose)(hid_t);
+  herr_t (*fclose)(hid_t);
+  hid_t *native_double;
+};
+
+H5Api &h5() {
+  static H5Api a;
+  if (a.tried) return a;
+  a.tried = true;
+  const char *env = getenv("DISCO_HDF5_LIB");
+  const char *names[] = {env, "libhdf5.so.103", "libhdf5.so", "/opt/conda/lib/libhdf5.so.103", "/opt/conda/lib/libhdf5.so"};
+  void *L = nullptr;
+  for (const char *n : names)
+    if (n && (L = dlopen(n, RTLD_NOW | RTLD_LOCAL))) break;
+  if (!L) return a;
+  a.open = (herr_t(*)(void))dlsym(L, "H5open");
+  a.fcreate = (hid_t(*)(const char *, unsigned, hid_t, hid_t))dlsym(L, "H5Fcreate");
+  a.screate_simple = (hid_t(*)(int, const hsize_t *, const hsize_t *))dlsym(L, "H5Screate_simple");
+  a.dcreate2 = (hid_t(*)(hid_t, const char *, hid_t, hid_t, hid_t, hid_t, hid_t))dlsym(L, "H5Dcreate2");
+  a.dget_space = (hid_t(*)(hid_t))dlsym(L, "H5Dget_space");
+  a.dwrite = (herr_t(*)(hid_t, hid_t, hid_t, hid_t, hid_t, const void *))dlsym(L, "H5Dwrite");
+  a.dclose = (herr_t(*)(hid_t))dlsym(L, "H5Dclose");
+  a.sclose = (herr_t(*)(hid_t))dlsym(L, "H5Sclose");
+  a.fclose = (herr_t(*)(hid_t))dlsym(L, "H5Fclose");
+  a.native_double = (hid_t *)dlsym(L, "H5T_NATIVE_DOUBLE_g");
+  a.ok = a.open && a.fcreate && a.screate_simple && a.dcreate2 && a.dget_space && a.dwrite && a.dclose && a.sclose &&
+         a.fclose && a.native_double && a.open() >= 0;
+  return a;
+}
+
+}  // namespace
+
+// dataset "EQUAT" of n0 x n1 doubles, the file truncated (H5F_ACC_TRUNC, H5P_DEFAULT)
+bool diag_h5_write(const char *fname, const double *data, size_t n0, size_t n1) {
+  H5Api &a = h5();
+  if (!a.ok) return false;
+  const hid_t P_DEFAULT = 0;
+  const unsigned ACC_TRUNC = 0x0002u;
+  hid_t f = a.fcreate(fname, ACC_TRUNC, P_DEFAULT, P_DEFAULT);
+  if (f < 0) return false;
+  hsize_t dims[2] = {(hsize_t)n0, (hsize_t)n1};
+  hid_t fs = a.screate_simple(2, dims, nullptr), ms = a.screate_simple(2, dims, nullptr);
+  hid_t ds = a.dcreate2(f, "EQUAT", *a.native_double, fs, P_DEFAULT, P_DEFAULT, P_DEFAULT);
+  a.sclose(fs);
+  fs = a.dget_space(ds);
+  herr_t st = a.dwrite(ds, *a.native_double, ms, fs, P_DEFAULT, data);
+  a.dclose(ds);
+  a.sclose(fs);
+  a.sclose(ms);
+  a.fclose(f);
+  return st >= 0;
+}

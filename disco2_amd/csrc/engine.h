@@ -89,6 +89,14 @@ struct Engine {
   void adjoint_redirect(const AdjSubRec &r);  // the next substep writes into r
   void adjoint_restore(const double *src, bool rk);
   int torque_loss(int nsteps, double *loss);
+
+  // diagnostics (diagnostics.cpp): one measurement of diagnostics_set on disk
+  // d (src/Diagnostics/diagnostics_set.c:13-303): vec [NumR][17], scal[16],
+  // equat [N_eq][18], mdot[12] as the DiagMdot.dat line after its t
+  int diag_sizes(int d, int *nrg, int *neq) const;
+  int diag_measure(int d, double dtout, const double *rz, double *vec, double *scal, double *equat, double *mdot);
+  void *diag_mem = nullptr;
+  size_t diag_cap = 0;
   int adjoint(int nsteps, int every, double *loss, double *g_alpha, double *g_q, double *g_prim0);
 };
 

@@ -31,6 +31,7 @@ struct Face;
 struct GravMass;
 struct MPIsetup;
 struct TimeStep;
+struct Diagnostics;
 
 /* ---- MPIsetup: src/Headers/MPIsetup.h:21-38 (single rank) ---- */
 struct MPIsetup *mpisetup_create(int argc, char **argv);          /* MPIsetup.h:21 */
@@ -209,6 +210,21 @@ int timestep_n(struct TimeStep *, int index, int direction);       /* TimeStep.h
 void timestep_set_nri(struct TimeStep *, int, int);                /* TimeStep.h:38 */
 void timestep_set_nzk(struct TimeStep *, int, int);                /* TimeStep.h:39 */
 
+/* ---- Diagnostics: src/Headers/Diagnostics.h:29-38 (single rank) ----
+ * The measurement runs on the GPU; the schedule (every T_MAX/Num_Diag_Measure,
+ * dumps every T_MAX/Num_Diag_Dump), the time-averaged sums and the files in
+ * the working directory are the reference's: DiagMdot.dat and DiagScalar.dat
+ * (appended), DiagVector_<t>.dat and DiagEquat_<t>.h5 (dataset EQUAT; written
+ * when an HDF5 library can be loaded: DISCO_HDF5_LIB, libhdf5.so, or
+ * /opt/conda/lib).  Cooling must be off (there is no per-cell Cool). */
+struct Diagnostics *diagnostics_create(struct Sim *, struct TimeStep *, struct MPIsetup *); /* Diagnostics.h:29 */
+void diagnostics_destroy(struct Diagnostics *, struct Sim *);                             /* Diagnostics.h:30 */
+void diagnostics_set(struct Diagnostics *, struct Cell ***, struct Sim *, struct TimeStep *, struct MPIsetup *,
+                     struct GravMass *);                                                  /* Diagnostics.h:33 */
+void diagnostics_print(struct Diagnostics *, struct TimeStep *, struct Sim *, struct MPIsetup *); /* Diagnostics.h:35 */
+double diagnostics_tdiag_measure(struct Diagnostics *);                                  /* Diagnostics.h:37 */
+double diagnostics_tdiag_dump(struct Diagnostics *);                                     /* Diagnostics.h:38 */
+
 /* ---- batched launch (no reference counterpart) ----
  * Many independent disks in one set of launches: each disk has its own
  * Sim (mesh and parameters), GravMass pair, time and dt.  The cell state
@@ -252,6 +268,20 @@ double disco_calibrate_copy(long long n);
 int disco_batch_torque_loss(struct DiscoBatch *, int nsteps, double *loss);
 int disco_batch_adjoint(struct DiscoBatch *, int nsteps, int checkpoint_every, double *loss, double *grad_alpha,
                         double *grad_q, double *grad_prim0);
+
+/* Diagnostics of one disk of a batch (no reference counterpart: the
+ * measurement of diagnostics_set, src/Diagnostics/diagnostics_set.c:13-303,
+ * without its schedule or files).  sizes: interior annuli and equatorial
+ * cells.  measure: dtout = t - (time of the previous measurement);
+ * rz = {R_Min, R_Max, Z_Min, Z_Max}; vector [nrg][17] and scalar[16] as the
+ * reference adds them to its time averages, equat [neq][18], mdot[12] the
+ * DiagMdot.dat line after its t.  Returns 0 or -1 (error set). */
+int disco_batch_diag_sizes(struct DiscoBatch *, int disk, int *nrg, int *neq);
+int disco_batch_diag_measure(struct DiscoBatch *, int disk, double dtout, const double *rz, double *vector,
+                             double *scalar, double *equat, double *mdot);
+/* t, dt, RK, then 2 x {r, phi, M, omega, RK_r, RK_phi, RK_M, RK_omega, Mdot, Macc}
+ * (the layout of disco_batch_scalars) */
+void disco_batch_set_scalars(struct DiscoBatch *, int disk, const double *in);
 
 /* ---- errors and version (no reference counterpart; the reference exits) ---- */
 int disco_last_error(void);

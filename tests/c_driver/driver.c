@@ -64,9 +64,15 @@ int main(int argc, char **argv) {
   cell_calc_cons(cells, sim);
   if (disco_last_error()) return fail("initial data");
 
+  /* src/main.c:121-131: the diagnostics write their files into the
+   * working directory when the parfile's schedule asks for them */
+  struct Diagnostics *dg = diagnostics_create(sim, ts, mpi);
   for (int n = 0; n < nsteps; ++n) {
     timestep_rk2(ts, sim, cells, gm, mpi);
     if (disco_last_error()) return fail("timestep_rk2");
+    diagnostics_set(dg, cells, sim, ts, mpi, gm);
+    diagnostics_print(dg, ts, sim, mpi);
+    if (disco_last_error()) return fail("diagnostics");
   }
   cell_syncproc_r(cells, sim, mpi);
   cell_syncproc_z(cells, sim, mpi);
@@ -81,6 +87,7 @@ int main(int argc, char **argv) {
   fclose(f);
   free(buf);
 
+  diagnostics_destroy(dg, sim);
   cell_destroy(cells, sim);
   sim_destroy(sim);
   gravMass_destroy(gm);

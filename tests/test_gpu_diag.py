@@ -1,0 +1,183 @@
+"""Diagnostics on the GPU against the reference's own diagnostics
+(tests/golden/diag_*.npz, written by src/Diagnostics compiled into the
+reference build).
+
+* Measurement (disco_batch_diag_measure) on the reference's state at each of
+  its measurements, driven through the reference's schedule
+  (oracle.oracle.Diagnostics): the time-averaged radial profiles, the
+  scalars and the equatorial cells bit for bit; the mass counts of the
+  DiagMdot.dat line to 1e-13 of the oracle (the GPU sums them per annulus,
+  the reference in one running sum) and the files' text equal to the
+  reference's.
+* The class API in a C driver (src/main.c:121-131 with diagnostics_set /
+  diagnostics_print after every step): the files it writes against the
+  reference's files for the same run: identical text for the vortex (whose
+  steps are bitwise), 1e-6 (the files' %e precision) for the binary disk;
+  the HDF5 equatorial dump read back.
+NUM_Q = 5: the reference's passive-scalar column reads past its primitive
+array (see tests/test_diag_oracle.py); it is not compared there.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from golden_util import Golden, golden_names
+
+pytestmark = pytest.mark.gpu
+
+DIAG = [n for n in golden_names() if n.startswith("diag_")]
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def masks(g):
+    mv, me, ms = np.ones(17, bool), np.ones(18, bool), np.ones(16, bool)
+    if g.NQ == 5:
+        mv[8] = me[9] = ms[7] = False
+    return mv, me, ms
+
+
+def passive_col(fn):
+    return {"DiagScalar.dat": 8, "DiagMdot.dat": None}.get(fn, 8)
+
+
+def compare_text(mine, ref, nq5, fn, rel=0.0):
+    lm, lr = mine.splitlines(), ref.splitlines()
+    assert len(lm) == len(lr), fn
+    skip = passive_col(fn) if nq5 else None
+    for a, b in zip(lm, lr):
+        x, y = a.split(), b.split()
+        assert len(x) == len(y), fn
+        for c, (u, v) in enumerate(zip(x, y)):
+            if c == skip:
+                continue
+            if rel == 0.0:
+                assert u == v, f"{fn} column {c}: {u} != {v}"
+            else:
+                fu, fv = float(u), float(v)
+                assert abs(fu - fv) <= rel * max(abs(fv), 1e-300), f"{fn} column {c}: {u} != {v}"
+
+
+class GpuSource:
+    def __init__(self, batch):
+        self.b = batch
+
+    def diag_measure(self, dtout, rz):
+        return self.b.diag_measure(dtout, rz)
+
+
+@pytest.mark.parametrize("name", DIAG)
+def test_measurement_matches_reference(name, tmp_path):
+    from disco2_amd.api import Batch, Setup
+    from oracle.oracle import Diagnostics, Oracle
+    g = Golden(name)
+    s = Setup(g.params, N_p=g.N_p)
+    b = Batch([s])
+    b.upload("prim", g.prim0)
+    b.init()
+    o = Oracle(g.params, g.N_p, g.r_faces, g.z_faces)
+    o.set_state(tiph=g.tiph0, prim=g.prim0)
+    o.init()
+    dg = Diagnostics(GpuSource(b), g.params, 0.0)
+    do = Diagnostics(o, g.params, 0.0)
+    mv, me, ms = masks(g)
+    n_meas, n_dump = (int(x) for x in g.get("diag/counts"))
+    dump_steps = {int(g.get(f"diag/dump{d}/step_t")[0]) for d in range(n_dump)}
+    print(f"{name}: {n_meas} measurements, {n_dump} dumps", flush=True)
+    ref_mdot = [[float(x) for x in line.split()] for line in str(g.get("file/DiagMdot.dat")).splitlines()]
+    for m in range(n_meas):
+        step, t = g.get(f"diag/meas{m}/step_t")
+        pre = f"diag/meas{m}/state"
+        gm = g.get(pre + "/gravmass").reshape(2, 12)
+        t_dt = g.get(pre + "/t_dt")
+        sc = np.concatenate([[t_dt[0], t_dt[1], 0.0], gm[0, :10], gm[1, :10]])
+        for f in ("prim", "tiph", "dphi"):
+            b.upload(f, g.get(pre + "/" + f))
+            o.put(f, g.get(pre + "/" + f))
+        b.set_scalars(sc)
+        o.set_scalars(sc)
+        v, _, _, md = b.diag_measure(t - dg.toutprev, dg.rz)
+        _, _, _, mo = o.diag_measure(t - do.toutprev, do.rz)
+        assert dg.set(t, str(tmp_path)) and do.set(t)
+        assert np.array_equal(dg.vec[:, mv], g.get(f"diag/meas{m}/vector").reshape(-1, 17)[:, mv]), f"meas {m} vector"
+        assert np.array_equal(dg.scal[ms], g.get(f"diag/meas{m}/scalar")[ms]), f"meas {m} scalar"
+        assert np.array_equal(dg.equat[:, me], g.get(f"diag/meas{m}/equat").reshape(-1, 18)[:, me]), f"meas {m} equat"
+        assert np.allclose(md, mo, rtol=1e-13, atol=0.0), f"meas {m} mass counts {md} vs {mo}"
+        assert np.allclose(md, ref_mdot[m][1:], rtol=1e-6, atol=1e-300), f"meas {m} DiagMdot line"
+        assert dg.print(t, str(tmp_path)) == (int(step) in dump_steps)
+        do.print(t)
+    for k in g.d:
+        if k.startswith("file/") and not k.endswith("DiagMdot.dat"):
+            fn = k[5:]
+            compare_text(open(tmp_path / fn).read(), str(g.get(k)), g.NQ == 5, fn)
+    b.close()
+
+
+def read_h5_equat(path, n):
+    for name in ("libhdf5.so.103", "libhdf5.so", "/opt/conda/lib/libhdf5.so.103", "/opt/conda/lib/libhdf5.so"):
+        try:
+            L = ctypes.CDLL(name)
+            break
+        except OSError:
+            L = None
+    if L is None:
+        return None
+    hid = ctypes.c_int64
+    L.H5open()
+    L.H5Fopen.restype = hid
+    L.H5Fopen.argtypes = [ctypes.c_char_p, ctypes.c_uint, hid]
+    L.H5Dopen2.restype = hid
+    L.H5Dopen2.argtypes = [hid, ctypes.c_char_p, hid]
+    L.H5Dread.argtypes = [hid, hid, hid, hid, hid, ctypes.c_void_p]
+    L.H5Dclose.argtypes = [hid]
+    L.H5Fclose.argtypes = [hid]
+    native = hid.in_dll(L, "H5T_NATIVE_DOUBLE_g").value
+    f = L.H5Fopen(str(path).encode(), 0, 0)
+    assert f >= 0
+    d = L.H5Dopen2(f, b"EQUAT", 0)
+    assert d >= 0
+    buf = np.empty(n * 18)
+    assert L.H5Dread(d, native, 0, 0, 0, buf.ctypes.data) >= 0
+    L.H5Dclose(d)
+    L.H5Fclose(f)
+    return buf.reshape(n, 18)
+
+
+@pytest.mark.parametrize("name", ["diag_vortex_scalar", "diag_binary_r16"])
+def test_c_driver_writes_the_reference_files(name, tmp_path):
+    from disco2_amd.params import write_parfile
+    from test_c_driver import build_driver
+    g = Golden(name)
+    par = str(tmp_path / "run.par")
+    write_parfile(par, g.params)
+    np.ascontiguousarray(g.prim0).tofile(str(tmp_path / "prim.bin"))
+    exe = build_driver(str(tmp_path))
+    n_meas, n_dump = (int(x) for x in g.get("diag/counts"))
+    steps = int(g.get(f"diag/meas{n_meas - 1}/step_t")[0])
+    print(f"C driver with diagnostics: {name}, {steps} steps", flush=True)
+    r = subprocess.run([exe, par, "prim.bin", str(steps), "out.bin"], capture_output=True, text=True, timeout=120,
+                       cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    ref_files = {k[5:]: str(g.get(k)) for k in g.d if k.startswith("file/")}
+    mine = sorted(f for f in os.listdir(tmp_path) if f.startswith("Diag") and f.endswith(".dat"))
+    assert mine == sorted(ref_files)
+    exact = name == "diag_vortex_scalar"
+    for fn, ref in ref_files.items():
+        compare_text(open(tmp_path / fn).read(), ref, g.NQ == 5, fn, rel=0.0 if exact else 1e-6)
+    # the equatorial dump of the last dump step: the last measurement's cells
+    last_dump_step = int(g.get(f"diag/dump{n_dump - 1}/step_t")[0])
+    t_dump = g.get(f"diag/dump{n_dump - 1}/step_t")[1]
+    m = max(k for k in range(n_meas) if int(g.get(f"diag/meas{k}/step_t")[0]) <= last_dump_step)
+    ref_eq = g.get(f"diag/meas{m}/equat").reshape(-1, 18)
+    h5 = tmp_path / ("DiagEquat_%010.2f.h5" % t_dump)
+    assert h5.exists()
+    eq = read_h5_equat(h5, ref_eq.shape[0])
+    if eq is None:
+        pytest.skip("no HDF5 library to read the equatorial dump back")
+    _, me, _ = masks(g)
+    if exact:
+        assert np.array_equal(eq[:, me], ref_eq[:, me])
+    else:
+        assert np.allclose(eq[:, me], ref_eq[:, me], rtol=1e-9, atol=1e-14)
