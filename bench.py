@@ -210,6 +210,27 @@ def adjoint_leg(steps=200, every=50):
     return res
 
 
+def diagnostics_leg(batch, params, reps=10):
+    """The reference's diagnostics measurement (diagnostics_set) on the bench
+    disk: wall clock of disco_batch_diag_measure, the two device kernels and
+    the copies of the radial profiles (NumR x 17) and scalars; and once more
+    with the equatorial cells (one per zone, 18 doubles) copied to the host,
+    which the reference does at every dump."""
+    import numpy as np
+    rz = np.array([params["R_Min"], params["R_Max"], params["Z_Min"], params["Z_Max"]], dtype=np.float64)
+    batch.diag_measure(1.0, rz, equat=False)  # allocates the scratch
+    batch.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.diag_measure(1.0, rz, equat=False)
+    wall = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    _, _, eq, _ = batch.diag_measure(1.0, rz, equat=True)
+    wall_eq = time.perf_counter() - t0
+    return {"ms_per_measurement": round(1e3 * wall, 4), "ms_with_equatorial_copy": round(1e3 * wall_eq, 4),
+            "equatorial_cells": int(eq.shape[0]), "measurements": reps}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +248,8 @@ def main(argv=None):
     ap.add_argument("--adjoint-every", type=int, default=50)
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint leg and print its JSON (for profiling it)")
+    ap.add_argument("--diag-only", action="store_true",
+                    help="run only the diagnostics leg on the workload and print its JSON (for profiling it)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one disk or batch per "
@@ -246,6 +269,10 @@ def main(argv=None):
     batch = Batch([setup])
     batch.upload("prim", prim0)
     batch.init()
+    if args.diag_only:
+        print(json.dumps({"workload": args.workload, "diagnostics": diagnostics_leg(batch, params)}))
+        batch.close()
+        return 0
     graph = not args.no_graph
     print(f"[bench] {args.workload}: {setup.nzones} zones ({n_int} interior), warmup {args.warmup}",
           file=sys.stderr, flush=True)
@@ -302,6 +329,9 @@ def main(argv=None):
         print("[bench] batched disks", file=sys.stderr, flush=True)
         batched = batched_leg(args.batched_steps, 2)
 
+    print("[bench] diagnostics", file=sys.stderr, flush=True)
+    diagnostics = diagnostics_leg(batch, params)
+
     adjoint = None
     if not args.no_adjoint:
         print("[bench] adjoint", file=sys.stderr, flush=True)
@@ -336,6 +366,7 @@ def main(argv=None):
         "kernels": kernels,
         "batched": batched,
         "adjoint": adjoint,
+        "diagnostics": diagnostics,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -119,6 +119,7 @@ SIGNATURES = {
     "disco_batch_diag_sizes": (ci, [vp, ci, pi, pi]),
     "disco_batch_diag_measure": (ci, [vp, ci, cd, pd, pd, pd, pd, pd]),
     "disco_batch_set_scalars": (None, [vp, ci, pd]),
+    "disco_batch_diag_equat": (ci, [vp, pd]),
     "diagnostics_create": (vp, [vp, vp, vp]),
     "diagnostics_destroy": (None, [vp, vp]),
     "diagnostics_set": (None, [vp, vp, vp, vp, vp, vp]),

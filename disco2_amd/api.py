@@ -139,11 +139,12 @@ class Batch:
         lib().disco_batch_set_scalars(self.h, disk, dptr(np.ascontiguousarray(values, dtype=np.float64)))
         check()
 
-    def diag_measure(self, dtout, rz, disk=0):
+    def diag_measure(self, dtout, rz, disk=0, equat=True):
         """One measurement of the reference's diagnostics_set on the GPU:
         (vector (NumR, 17), scalar (16,), equat (Neq, 18), mdot (12,)) as
         the reference adds them to its time averages (dtout = t - previous
-        measurement; rz = R_Min, R_Max, Z_Min, Z_Max)."""
+        measurement; rz = R_Min, R_Max, Z_Min, Z_Max).  equat=False leaves
+        the equatorial cells on the GPU (returns None for them)."""
         L = lib()
         nrg, neq = ctypes.c_int(), ctypes.c_int()
         L.disco_batch_diag_sizes(self.h, disk, ctypes.byref(nrg), ctypes.byref(neq))
@@ -151,11 +152,11 @@ class Batch:
         vec, sc = np.zeros(nrg.value * 17), np.zeros(16)
         eq, md = np.zeros(max(neq.value, 1) * 18), np.zeros(12)
         rc = L.disco_batch_diag_measure(self.h, disk, float(dtout), dptr(np.ascontiguousarray(rz, dtype=np.float64)),
-                                        dptr(vec), dptr(sc), dptr(eq), dptr(md))
+                                        dptr(vec), dptr(sc), dptr(eq) if equat else None, dptr(md))
         check()
         if rc != 0:
             raise RuntimeError("diagnostics failed")
-        return vec.reshape(nrg.value, 17), sc, eq[:neq.value * 18].reshape(neq.value, 18), md
+        return vec.reshape(nrg.value, 17), sc, eq[:neq.value * 18].reshape(neq.value, 18) if equat else None, md
 
     def torque_loss(self, nsteps):
         """Run nsteps RK2 steps; the time-averaged gravitational torque on the

@@ -574,6 +574,7 @@ void disco_batch_scalars(DiscoBatch *b, int d, double *o) {
 // diagnostics_print.c:11-92.
 }  // extern "C"
 struct Diagnostics {
+  Engine *eng = nullptr;  // of the last measurement
   int nrg = 0, neq = 0;
   std::vector<double> vec, scal, equat;
   double toutprev = 0, toutprev_dump = 0;
@@ -624,7 +625,8 @@ void diagnostics_set(Diagnostics *d, Cell ***c, Sim *, TimeStep *ts, MPIsetup *,
   Engine *e = h->eng;
   scalars_from_host(e, ts, gm);
   std::vector<double> v(d->vec.size()), s(16), md(12);
-  if (e->diag_measure(0, t - d->toutprev, d->rz, v.data(), s.data(), d->equat.data(), md.data()) != 0) return;
+  if (e->diag_measure(0, t - d->toutprev, d->rz, v.data(), s.data(), nullptr, md.data()) != 0) return;
+  d->eng = e;  // the equatorial cells stay on the device until a dump
   for (int p = 0; p < 2; ++p) {  // the masses' accretion as the host GravMass holds it
     md[4 + p] = gm[p].Mdot;
     md[6 + p] = gm[p].Macc;
@@ -662,6 +664,7 @@ void diagnostics_print(Diagnostics *d, TimeStep *ts, Sim *, MPIsetup *) {
     fprintf(sf, "\n");
     fclose(sf);
   }
+  if (d->eng && d->eng->diag_equat(d->equat.data()) != 0) return;
   diag_h5_write(fe, d->equat.data(), (size_t)d->neq, 18);
   d->toutprev_dump = t;
   std::fill(d->vec.begin(), d->vec.end(), 0.0);
@@ -680,6 +683,7 @@ int disco_batch_diag_measure(DiscoBatch *b, int d, double dtout, const double *r
                              double *equat, double *mdot) {
   return b->e->diag_measure(d, dtout, rz, vec, scal, equat, mdot);
 }
+int disco_batch_diag_equat(DiscoBatch *b, double *equat) { return b->e->diag_equat(equat); }
 void disco_batch_set_scalars(DiscoBatch *b, int d, const double *in) {
   Engine *e = b->e;
   DiskScalars s = e->get_scalars(d);

@@ -97,7 +97,10 @@ int Engine::diag_measure(int d, double dtout, const double *rz, double *vec, dou
   double out[8];
   hip_check(hipMemcpyAsync(vec, g.vec, 8 * (size_t)nrg * DIAG_NV, hipMemcpyDeviceToHost, st), "diagnostics");
   hip_check(hipMemcpyAsync(scal, g.scal, 8 * DIAG_NS, hipMemcpyDeviceToHost, st), "diagnostics");
-  if (neq) hip_check(hipMemcpyAsync(equat, g.equat, 8 * (size_t)neq * DIAG_NE, hipMemcpyDeviceToHost, st), "diagnostics");
+  if (neq && equat)
+    hip_check(hipMemcpyAsync(equat, g.equat, 8 * (size_t)neq * DIAG_NE, hipMemcpyDeviceToHost, st), "diagnostics");
+  diag_equat_dev = g.equat;
+  diag_equat_n = (size_t)neq * DIAG_NE;
   hip_check(hipMemcpyAsync(out, g.out_mass, 8 * 8, hipMemcpyDeviceToHost, st), "diagnostics");
   if (check_error()) return -1;
   const DiskScalars s = get_scalars(d);
@@ -114,6 +117,18 @@ int Engine::diag_measure(int d, double dtout, const double *rz, double *vec, dou
   mdot[10] = out[3];
   mdot[11] = out[4];
   return 0;
+}
+
+// the equatorial cells of the last measurement (they stay on the device
+// until asked for: the reference writes them only when it dumps)
+int Engine::diag_equat(double *out) {
+  if (!diag_equat_dev) {
+    disco_set_error(14, "diagnostics: nothing measured yet");
+    return -1;
+  }
+  if (diag_equat_n)
+    hip_check(hipMemcpyAsync(out, diag_equat_dev, 8 * diag_equat_n, hipMemcpyDeviceToHost, st), "diagnostics");
+  return check_error() ? -1 : 0;
 }
 
 // ---------------------------------------------------------------------------

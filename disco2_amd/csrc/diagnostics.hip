@@ -87,6 +87,7 @@ __global__ __launch_bounds__(TILE) void k_diag_ann(DevMesh m, DiagArgs g) {
       __syncthreads();
       if (t < NT) {
         const int n = min(TILE, A.N - j0);
+#pragma unroll 16
         for (int cc = 0; cc < n; ++cc) acc += s[t][cc];
       }
       __syncthreads();
@@ -97,38 +98,65 @@ __global__ __launch_bounds__(TILE) void k_diag_ann(DevMesh m, DiagArgs g) {
   if (t >= 12 && t < NT) g.mpart[(size_t)ii * NM + (t - 12)] = acc;
 }
 
-// scalars, masses, inner wedge, Mdot probe, scalings (diagnostics_set.c:193-245)
-__global__ __launch_bounds__(64) void k_diag_fin(DevMesh m, DiagArgs g) {
+// scalars, masses, inner wedge, Mdot probe, scalings (diagnostics_set.c:193-245).
+// Every sum runs in the reference's order; the workgroup stages each chunk
+// of its terms in LDS and one thread per sum adds them in order.
+constexpr int FT = 256;
+__global__ __launch_bounds__(FT) void k_diag_fin(DevMesh m, DiagArgs g) {
   const int t = threadIdx.x;
   const DiskParams &P = m.dp[g.disk];
   const int nrg = g.nrg;
   const double RMIN = g.rz[0], RMAX = g.rz[1], ZMIN = g.rz[2], ZMAX = g.rz[3];
-  __shared__ double s_scal[NS];
-  if (t < NS) {
-    double v = 0.0;
-    for (int ii = 0; ii < nrg; ++ii) {
+  __shared__ double s[NS + NM][FT];
+  double acc = 0.0;  // thread t < NS: scalar t; NS <= t < NS + NM: mass sum t - NS
+  // ScalarDiag_temp[n] += VectorDiag_temp[i][n+1] * (rp*rp - rm*rm), i in order
+  for (int c0 = 0; c0 < nrg; c0 += FT) {
+    const int ii = c0 + t;
+    if (ii < nrg) {
       const AnnInfo A = m.ann[P.ann_base + g.kmin * P.NR + g.imin + ii];
-      v += g.vec[(size_t)ii * NV + t + 1] * (A.rp * A.rp - A.rm * A.rm);
+      const double area = A.rp * A.rp - A.rm * A.rm;
+#pragma unroll
+      for (int n = 0; n < NS; ++n) s[n][t] = g.vec[(size_t)ii * NV + n + 1] * area;
+#pragma unroll
+      for (int n = 0; n < NM; ++n) s[NS + n][t] = g.mpart[(size_t)ii * NM + n];
     }
-    s_scal[t] = v;
-  } else if (t < NS + NM) {
-    double v = 0.0;
-    for (int ii = 0; ii < nrg; ++ii) v += g.mpart[(size_t)ii * NM + (t - NS)];
-    g.out_mass[t - NS] = v;
-  } else if (t == NS + NM) {
-    // cell (0, j, 0) for j below N_p of the annulus the previous loop left
-    // (imax): the first cells of layer 0, beyond annulus 0 when j >= N_p[0]
+    __syncthreads();
+    if (t < NS + NM) {
+      const int n = min(FT, nrg - c0);
+#pragma unroll 16
+      for (int cc = 0; cc < n; ++cc) acc += s[t][cc];
+    }
+    __syncthreads();
+  }
+  if (t < NS) g.scal[t] = acc * (g.dtout / ((ZMAX - ZMIN) * (RMAX * RMAX - RMIN * RMIN)));
+  if (t >= NS && t < NS + NM) g.out_mass[t - NS] = acc;
+  // inner wedge: cell (0, j, 0) for j below N_p of the annulus the previous
+  // loop left (imax), i.e. the first cells of layer 0, beyond annulus 0 when
+  // j >= N_p[0]
+  {
     const AnnInfo A0 = m.ann[P.ann_base];
     const double rp = A0.rp, rm = A0.rm;
     const int n = m.ann[P.ann_base + g.imin + nrg].N;
     double w = 0.0;
-    for (int j = 0; j < n; ++j) {
-      const long long c = A0.zbase + j;
-      const double dV = 0.5 * (rp * rp - rm * rm) * m.dphi[c];
-      w += m.prim[RHO * m.Ntot + c] * dV;
+    for (int c0 = 0; c0 < n; c0 += FT) {
+      const int j = c0 + t;
+      if (j < n) {
+        const long long c = A0.zbase + j;
+        const double dV = 0.5 * (rp * rp - rm * rm) * m.dphi[c];
+        s[0][t] = m.prim[RHO * m.Ntot + c] * dV;
+      }
+      __syncthreads();
+      if (t == 0) {
+        const int nn = min(FT, n - c0);
+#pragma unroll 16
+        for (int cc = 0; cc < nn; ++cc) w += s[0][cc];
+      }
+      __syncthreads();
     }
-    g.out_mass[NM] = w;
-  } else if (t == NS + NM + 1) {
+    if (t == 0) g.out_mass[NM] = w;
+  }
+  // the Mdot probe: the first annulus with r > 1 (profiles not yet scaled)
+  if (t == 0) {
     int found = 0;
     double Mdot = 0.0, rr = 0.0;
     for (int ii = 0; ii < nrg && !found; ++ii) {
@@ -143,14 +171,13 @@ __global__ __launch_bounds__(64) void k_diag_fin(DevMesh m, DiagArgs g) {
     g.out_mass[NM + 2] = rr;
   }
   __syncthreads();
-  if (t < NS) g.scal[t] = s_scal[t] * (g.dtout / ((ZMAX - ZMIN) * (RMAX * RMAX - RMIN * RMIN)));
   const double f = g.dtout / (ZMAX - ZMIN);
-  for (int x = t; x < nrg * NV; x += 64) g.vec[x] = g.vec[x] * f;
+  for (int x = t; x < nrg * NV; x += FT) g.vec[x] = g.vec[x] * f;
 }
 
 }  // namespace
 
 void launch_diag(const DevMesh &m, const DiagArgs &g, hipStream_t st) {
   hipLaunchKernelGGL(k_diag_ann, dim3(g.nrg), dim3(TILE), 0, st, m, g);
-  hipLaunchKernelGGL(k_diag_fin, dim3(1), dim3(64), 0, st, m, g);
+  hipLaunchKernelGGL(k_diag_fin, dim3(1), dim3(FT), 0, st, m, g);
 }

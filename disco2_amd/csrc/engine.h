@@ -94,9 +94,13 @@ struct Engine {
   // d (src/Diagnostics/diagnostics_set.c:13-303): vec [NumR][17], scal[16],
   // equat [N_eq][18], mdot[12] as the DiagMdot.dat line after its t
   int diag_sizes(int d, int *nrg, int *neq) const;
+  // (equat may be null: the cells stay on the device for diag_equat)
   int diag_measure(int d, double dtout, const double *rz, double *vec, double *scal, double *equat, double *mdot);
+  int diag_equat(double *out);  // the last measurement's equatorial cells
   void *diag_mem = nullptr;
   size_t diag_cap = 0;
+  double *diag_equat_dev = nullptr;
+  size_t diag_equat_n = 0;
   int adjoint(int nsteps, int every, double *loss, double *g_alpha, double *g_q, double *g_prim0);
 };
 

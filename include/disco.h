@@ -279,6 +279,9 @@ int disco_batch_adjoint(struct DiscoBatch *, int nsteps, int checkpoint_every, d
 int disco_batch_diag_sizes(struct DiscoBatch *, int disk, int *nrg, int *neq);
 int disco_batch_diag_measure(struct DiscoBatch *, int disk, double dtout, const double *rz, double *vector,
                              double *scalar, double *equat, double *mdot);
+/* equat may be NULL: the equatorial cells then stay on the device, and
+ * disco_batch_diag_equat copies those of the last measurement */
+int disco_batch_diag_equat(struct DiscoBatch *, double *equat);
 /* t, dt, RK, then 2 x {r, phi, M, omega, RK_r, RK_phi, RK_M, RK_omega, Mdot, Macc}
  * (the layout of disco_batch_scalars) */
 void disco_batch_set_scalars(struct DiscoBatch *, int disk, const double *in);
