@@ -120,6 +120,19 @@ SIGNATURES = {
     "disco_batch_diag_measure": (ci, [vp, ci, cd, pd, pd, pd, pd, pd]),
     "disco_batch_set_scalars": (None, [vp, ci, pd]),
     "disco_batch_diag_equat": (ci, [vp, pd]),
+    "disco_batch_checkpoint_write": (ci, [vp, ci, ccp]),
+    "disco_batch_checkpoint_read": (ci, [vp, ci, ccp]),
+    "io_create": (vp, [vp]),
+    "io_destroy": (None, [vp]),
+    "io_allocbuf": (None, [vp, vp]),
+    "io_deallocbuf": (None, [vp]),
+    "io_setbuf": (None, [vp, vp, vp, vp]),
+    "io_readbuf": (None, [vp, vp, vp, vp]),
+    "io_hdf5_out": (None, [vp, vp, vp]),
+    "io_hdf5_in": (None, [vp, vp, vp]),
+    "io_tcheck": (cd, [vp]),
+    "io_nfile": (ci, [vp]),
+    "io_setup": (None, [vp, vp, vp]),
     "diagnostics_create": (vp, [vp, vp, vp]),
     "diagnostics_destroy": (None, [vp, vp]),
     "diagnostics_set": (None, [vp, vp, vp, vp, vp, vp]),

@@ -158,6 +158,22 @@ class Batch:
             raise RuntimeError("diagnostics failed")
         return vec.reshape(nrg.value, 17), sc, eq[:neq.value * 18].reshape(neq.value, 18) if equat else None, md
 
+    def checkpoint(self, path, disk=0):
+        """Write the disk's time, masses and cells in the reference's
+        checkpoint layout (HDF5: T, GravMass, Data)."""
+        rc = lib().disco_batch_checkpoint_write(self.h, disk, str(path).encode())
+        check()
+        if rc != 0:
+            raise RuntimeError("checkpoint failed")
+
+    def restart(self, path, disk=0):
+        """Read a checkpoint into the disk as the reference restarts (after
+        init(); dphi is left as it is)."""
+        rc = lib().disco_batch_checkpoint_read(self.h, disk, str(path).encode())
+        check()
+        if rc != 0:
+            raise RuntimeError("restart failed")
+
     def torque_loss(self, nsteps):
         """Run nsteps RK2 steps; the time-averaged gravitational torque on the
         gas (the loss of the adjoint)."""

@@ -24,8 +24,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-Wno-unused-function", "-Wno-unused-result"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
-SOURCES = ["kernels.hip", "adjoint.hip", "diagnostics.hip", "engine.cpp", "adjoint_host.cpp", "diagnostics.cpp",
-           "capi.cpp", "sim_host.cpp"]
+SOURCES = ["kernels.hip", "adjoint.hip", "diagnostics.hip", "checkpoint.hip", "engine.cpp", "adjoint_host.cpp",
+           "diagnostics.cpp", "checkpoint.cpp", "h5.cpp", "capi.cpp", "sim_host.cpp"]
 
 
 def _compile(src: str) -> str:

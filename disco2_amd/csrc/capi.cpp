@@ -11,6 +11,7 @@
 #include "engine.h"
 
 bool diag_h5_write(const char *fname, const double *data, size_t n0, size_t n1);  // diagnostics.cpp
+#include "h5.h"
 
 // ------------------------------------------------------------------ errors
 static int g_err = 0;
@@ -700,4 +701,108 @@ void disco_batch_set_scalars(DiscoBatch *b, int d, const double *in) {
   launch_scalar(e->mesh(), e->plan, e->st, SC_MASS);
   e->check_error();
 }
+
+// ------------------------------------------------------------------ checkpoints
+// src/Headers/IO.h:19-34, one rank.  The cell rows are packed and unpacked on
+// the GPU (checkpoint.hip); the file layout is the reference's
+// (io_hdf5.c:15-155: T, GravMass [2][4], Data [Ncells][NUM_Q + 3]).
+}  // extern "C"
+struct IO {
+  double tcheck = 0, dtcheck = 0;
+  int nfile = 0;
+  char filename[256] = {0};
+  double GravMassBuffer[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  std::vector<double> buffer;
+};
+extern "C" {
+
+IO *io_create(Sim *) { return new IO; }
+void io_destroy(IO *io) { delete io; }
+void io_allocbuf(IO *io, Sim *sim) { io->buffer.assign((size_t)sim->Ncells * (sim->NUM_Q + 3), 0.0); }
+void io_deallocbuf(IO *io) { std::vector<double>().swap(io->buffer); }
+double io_tcheck(IO *io) { return io->tcheck; }
+int io_nfile(IO *io) { return io->nfile; }
+
+// io_setup.c:10-19
+void io_setup(IO *io, Sim *sim, TimeStep *ts) {
+  io->nfile = 0;
+  io->dtcheck = sim_get_T_MAX(sim) / sim->NUM_CHECKPOINTS;
+  io->tcheck = io->dtcheck;
+  while (io->tcheck < timestep_get_t(ts)) {
+    io->tcheck += io->dtcheck;
+    ++io->nfile;
+  }
+  snprintf(io->filename, sizeof io->filename, "checkpoint_%04d.h5", io->nfile);
+}
+
+// io_buffer.c:26-55
+void io_setbuf(IO *io, Cell ***c, Sim *sim, GravMass *gm) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  if (io->buffer.size() != (size_t)sim->Ncells * (sim->NUM_Q + 3)) io_allocbuf(io, sim);
+  if (h->eng->checkpoint_pack(0, io->buffer.data())) return;
+  for (int p = 0; p < 2; ++p) {
+    io->GravMassBuffer[p][0] = gm[p].r;
+    io->GravMassBuffer[p][1] = gm[p].phi;
+    io->GravMassBuffer[p][2] = gm[p].M;
+    io->GravMassBuffer[p][3] = gm[p].omega;
+  }
+}
+
+// io_buffer.c:57-81
+void io_readbuf(IO *io, Cell ***c, Sim *sim, GravMass *gm) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  if (io->buffer.size() != (size_t)sim->Ncells * (sim->NUM_Q + 3)) {
+    disco_set_error(15, "io_readbuf: no checkpoint rows (io_allocbuf, io_hdf5_in)");
+    return;
+  }
+  if (h->eng->checkpoint_unpack(0, io->buffer.data())) return;
+  h->stale = true;
+  for (int p = 0; p < 2; ++p)
+    gravMass_set_chkpt(gm, p, io->GravMassBuffer[p][0], io->GravMassBuffer[p][1], io->GravMassBuffer[p][2],
+                       io->GravMassBuffer[p][3]);
+}
+
+// io_hdf5.c:15-155: the file named by io_setup / the previous call
+void io_hdf5_out(IO *io, Sim *sim, TimeStep *ts) {
+  if (!h5_available()) {
+    disco_set_error(15, "checkpoints need the HDF5 library (DISCO_HDF5_LIB or libhdf5.so)");
+    return;
+  }
+  double T = timestep_get_t(ts);
+  const H5Dataset ds[3] = {{"T", 1, {1, 1}, &T},
+                           {"GravMass", 2, {2, 4}, &io->GravMassBuffer[0][0]},
+                           {"Data", 2, {(unsigned long long)sim->Ncells, (unsigned long long)(sim->NUM_Q + 3)},
+                            io->buffer.data()}};
+  if (io->buffer.size() != (size_t)sim->Ncells * (sim->NUM_Q + 3) || !h5_write_file(io->filename, ds, 3)) {
+    disco_set_error(15, std::string("io_hdf5_out: cannot write ") + io->filename);
+    return;
+  }
+  io->tcheck += io->dtcheck;
+  ++io->nfile;
+  snprintf(io->filename, sizeof io->filename, "checkpoint_%04d.h5", io->nfile);
+}
+
+// io_hdf5.c:158-265: input.h5 in the working directory
+void io_hdf5_in(IO *io, Sim *sim, TimeStep *ts) {
+  if (!h5_available()) {
+    disco_set_error(15, "checkpoints need the HDF5 library (DISCO_HDF5_LIB or libhdf5.so)");
+    return;
+  }
+  long long dims[2];
+  double T = 0.0;
+  const long long n = (long long)sim->Ncells * (sim->NUM_Q + 3);
+  io->buffer.assign((size_t)n, 0.0);
+  if (h5_read_dataset("input.h5", "T", &T, 1, dims) != 1 ||
+      h5_read_dataset("input.h5", "GravMass", &io->GravMassBuffer[0][0], 8, dims) != 8 ||
+      h5_read_dataset("input.h5", "Data", io->buffer.data(), n, dims) != n) {
+    disco_set_error(15, "io_hdf5_in: input.h5 does not hold this mesh's T, GravMass and Data");
+    return;
+  }
+  timestep_set_t(ts, T);
+}
+
+int disco_batch_checkpoint_write(DiscoBatch *b, int d, const char *fname) { return b->e->checkpoint_write(d, fname); }
+int disco_batch_checkpoint_read(DiscoBatch *b, int d, const char *fname) { return b->e->checkpoint_read(d, fname); }
 }

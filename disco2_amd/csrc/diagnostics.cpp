@@ -1,8 +1,7 @@
 // Host side of the diagnostics: one device measurement of a disk
-// (Engine::diag_measure), and the writer of the reference's output files.
+// (Engine::diag_measure), and the equatorial HDF5 dump.
 // The class API (diagnostics_create / _set / _print / _destroy,
 // src/Headers/Diagnostics.h:29-38) is in capi.cpp on top of these.
-#include <dlfcn.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -15,6 +14,7 @@
 #include "diagnostics.h"
 #include "disco_internal.h"
 #include "engine.h"
+#include "h5.h"
 
 int Engine::diag_sizes(int d, int *nrg, int *neq) const {
   const DiskParams &P = hdp[d];
@@ -131,75 +131,9 @@ int Engine::diag_equat(double *out) {
   return check_error() ? -1 : 0;
 }
 
-// ---------------------------------------------------------------------------
-// The equatorial dump of diagnostics_print (diagnostics_print.c:40-80) is an
-// HDF5 file.  The library is loaded when first needed (DISCO_HDF5_LIB, else
-// libhdf5.so on the loader path or in /opt/conda/lib); without it the .h5
-// file is skipped and diag_h5_write returns false.
-namespace {
-
-typedef int64_t hid_t;
-typedef int herr_t;
-typedef unsigned long long hsize_t;
-
-struct H5Api {
-  bool tried = false, ok = false;
-  herr_t (*open)(void);
-  hid_t (*fcreate)(const char *, unsigned, hid_t, hid_t);
-  hid_t (*screate_simple)(int, const hsize_t *, const hsize_t *);
-  hid_t (*dcreate2)(hid_t, const char *, hid_t, hid_t, hid_t, hid_t, hid_t);
-  hid_t (*dget_space)(hid_t);
-  herr_t (*dwrite)(hid_t, hid_t, hid_t, hid_t, hid_t, const void *);
-  herr_t (*dclose)(hid_t);
-  herr_t (*sclose)(hid_t);
-  herr_t (*fclose)(hid_t);
-  hid_t *native_double;
-};
-
-H5Api &h5() {
-  static H5Api a;
-  if (a.tried) return a;
-  a.tried = true;
-  const char *env = getenv("DISCO_HDF5_LIB");
-  const char *names[] = {env, "libhdf5.so.103", "libhdf5.so", "/opt/conda/lib/libhdf5.so.103", "/opt/conda/lib/libhdf5.so"};
-  void *L = nullptr;
-  for (const char *n : names)
-    if (n && (L = dlopen(n, RTLD_NOW | RTLD_LOCAL))) break;
-  if (!L) return a;
-  a.open = (herr_t(*)(void))dlsym(L, "H5open");
-  a.fcreate = (hid_t(*)(const char *, unsigned, hid_t, hid_t))dlsym(L, "H5Fcreate");
-  a.screate_simple = (hid_t(*)(int, const hsize_t *, const hsize_t *))dlsym(L, "H5Screate_simple");
-  a.dcreate2 = (hid_t(*)(hid_t, const char *, hid_t, hid_t, hid_t, hid_t, hid_t))dlsym(L, "H5Dcreate2");
-  a.dget_space = (hid_t(*)(hid_t))dlsym(L, "H5Dget_space");
-  a.dwrite = (herr_t(*)(hid_t, hid_t, hid_t, hid_t, hid_t, const void *))dlsym(L, "H5Dwrite");
-  a.dclose = (herr_t(*)(hid_t))dlsym(L, "H5Dclose");
-  a.sclose = (herr_t(*)(hid_t))dlsym(L, "H5Sclose");
-  a.fclose = (herr_t(*)(hid_t))dlsym(L, "H5Fclose");
-  a.native_double = (hid_t *)dlsym(L, "H5T_NATIVE_DOUBLE_g");
-  a.ok = a.open && a.fcreate && a.screate_simple && a.dcreate2 && a.dget_space && a.dwrite && a.dclose && a.sclose &&
-         a.fclose && a.native_double && a.open() >= 0;
-  return a;
-}
-
-}  // namespace
-
-// dataset "EQUAT" of n0 x n1 doubles, the file truncated (H5F_ACC_TRUNC, H5P_DEFAULT)
+// The equatorial dump of diagnostics_print (diagnostics_print.c:40-80):
+// dataset EQUAT of n0 x n1 doubles
 bool diag_h5_write(const char *fname, const double *data, size_t n0, size_t n1) {
-  H5Api &a = h5();
-  if (!a.ok) return false;
-  const hid_t P_DEFAULT = 0;
-  const unsigned ACC_TRUNC = 0x0002u;
-  hid_t f = a.fcreate(fname, ACC_TRUNC, P_DEFAULT, P_DEFAULT);
-  if (f < 0) return false;
-  hsize_t dims[2] = {(hsize_t)n0, (hsize_t)n1};
-  hid_t fs = a.screate_simple(2, dims, nullptr), ms = a.screate_simple(2, dims, nullptr);
-  hid_t ds = a.dcreate2(f, "EQUAT", *a.native_double, fs, P_DEFAULT, P_DEFAULT, P_DEFAULT);
-  a.sclose(fs);
-  fs = a.dget_space(ds);
-  herr_t st = a.dwrite(ds, *a.native_double, ms, fs, P_DEFAULT, data);
-  a.dclose(ds);
-  a.sclose(fs);
-  a.sclose(ms);
-  a.fclose(f);
-  return st >= 0;
+  const H5Dataset ds = {"EQUAT", 2, {(unsigned long long)n0, (unsigned long long)n1}, data};
+  return h5_write_file(fname, &ds, 1);
 }

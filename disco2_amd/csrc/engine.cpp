@@ -289,7 +289,7 @@ Engine::~Engine() {
   if (graph) hipGraphDestroy(graph);
   void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
-                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot, adj_mem, diag_mem};
+                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot, adj_mem, diag_mem, ckpt_mem};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (ev0) hipEventDestroy(ev0);

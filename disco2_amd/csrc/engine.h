@@ -97,6 +97,15 @@ struct Engine {
   // (equat may be null: the cells stay on the device for diag_equat)
   int diag_measure(int d, double dtout, const double *rz, double *vec, double *scal, double *equat, double *mdot);
   int diag_equat(double *out);  // the last measurement's equatorial cells
+
+  // checkpoints (checkpoint.cpp): disk d's cell rows {tiph, r, z, prim}
+  // (src/IO/io_buffer.c), nzones[d] x (NQ + 3) doubles on the host
+  int checkpoint_pack(int d, double *rows);
+  int checkpoint_unpack(int d, const double *rows);
+  int checkpoint_write(int d, const char *fname);
+  int checkpoint_read(int d, const char *fname);
+  void *ckpt_mem = nullptr;
+  size_t ckpt_cap = 0;
   void *diag_mem = nullptr;
   size_t diag_cap = 0;
   double *diag_equat_dev = nullptr;

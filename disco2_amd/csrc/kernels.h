@@ -107,3 +107,8 @@ void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ)
 void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int dir);
 void launch_plm(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int which);
 void launch_copy(double *dst, const double *src, long long n, hipStream_t st);
+
+// checkpoint rows (checkpoint.hip): nann = the disk's annuli (NR x NZ);
+// unpack writes the primitives and tiph (the current state's buffers)
+void launch_ckpt_pack(const DevMesh &m, int disk, int nann, double *out, hipStream_t st);
+void launch_ckpt_unpack(const DevMesh &m, int disk, int nann, const double *in, double *tiph, hipStream_t st);

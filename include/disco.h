@@ -32,6 +32,7 @@ struct GravMass;
 struct MPIsetup;
 struct TimeStep;
 struct Diagnostics;
+struct IO;
 
 /* ---- MPIsetup: src/Headers/MPIsetup.h:21-38 (single rank) ---- */
 struct MPIsetup *mpisetup_create(int argc, char **argv);          /* MPIsetup.h:21 */
@@ -225,6 +226,24 @@ void diagnostics_print(struct Diagnostics *, struct TimeStep *, struct Sim *, st
 double diagnostics_tdiag_measure(struct Diagnostics *);                                  /* Diagnostics.h:37 */
 double diagnostics_tdiag_dump(struct Diagnostics *);                                     /* Diagnostics.h:38 */
 
+/* ---- IO (checkpoints): src/Headers/IO.h:19-34 (single rank) ----
+ * Files of the reference's layout: checkpoint_NNNN.h5 written by io_hdf5_out,
+ * input.h5 read by io_hdf5_in, each with datasets T, GravMass [2][4]
+ * (r, phi, M, omega) and Data [Ncells][NUM_Q+3] (tiph, r, z, prim per cell,
+ * ghosts included, reference cell order).  Needs an HDF5 library at run time
+ * (DISCO_HDF5_LIB, libhdf5.so, /opt/conda/lib): otherwise the error is set. */
+struct IO *io_create(struct Sim *);                                          /* IO.h:20 */
+void io_destroy(struct IO *);                                                /* IO.h:21 */
+void io_allocbuf(struct IO *, struct Sim *);                                 /* IO.h:23 */
+void io_deallocbuf(struct IO *);                                             /* IO.h:24 */
+void io_setbuf(struct IO *, struct Cell ***, struct Sim *, struct GravMass *);  /* IO.h:25 */
+void io_readbuf(struct IO *, struct Cell ***, struct Sim *, struct GravMass *); /* IO.h:26 */
+void io_hdf5_out(struct IO *, struct Sim *, struct TimeStep *);              /* IO.h:28 */
+void io_hdf5_in(struct IO *, struct Sim *, struct TimeStep *);               /* IO.h:29 */
+double io_tcheck(struct IO *);                                               /* IO.h:31 */
+int io_nfile(struct IO *);                                                   /* IO.h:32 */
+void io_setup(struct IO *, struct Sim *, struct TimeStep *);                 /* IO.h:34 */
+
 /* ---- batched launch (no reference counterpart) ----
  * Many independent disks in one set of launches: each disk has its own
  * Sim (mesh and parameters), GravMass pair, time and dt.  The cell state
@@ -282,6 +301,14 @@ int disco_batch_diag_measure(struct DiscoBatch *, int disk, double dtout, const 
 /* equat may be NULL: the equatorial cells then stay on the device, and
  * disco_batch_diag_equat copies those of the last measurement */
 int disco_batch_diag_equat(struct DiscoBatch *, double *equat);
+/* A checkpoint of one disk in the reference's file layout (see IO above):
+ * write its time, masses and cells; read them back into a disk as the
+ * reference restarts (src/main.c:95-105): t, the masses' r, phi, M, omega
+ * (phi cleaned to [0, 2 pi]), tiph and the primitives (v_z = 0 in 2D);
+ * dphi is left as it is (a fresh batch: 2 pi / N_p, as cell_create).
+ * Returns 0 or -1 (error set). */
+int disco_batch_checkpoint_write(struct DiscoBatch *, int disk, const char *fname);
+int disco_batch_checkpoint_read(struct DiscoBatch *, int disk, const char *fname);
 /* t, dt, RK, then 2 x {r, phi, M, omega, RK_r, RK_phi, RK_M, RK_omega, Mdot, Macc}
  * (the layout of disco_batch_scalars) */
 void disco_batch_set_scalars(struct DiscoBatch *, int disk, const double *in);

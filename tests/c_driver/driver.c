@@ -4,7 +4,11 @@
  *
  *   driver <parfile> <prim_in> <nsteps> <state_out>
  *
- * prim_in: NUM_Q * Nzones doubles (q-major, the reference's cell order).
+ * prim_in: NUM_Q * Nzones doubles (q-major, the reference's cell order);
+ * with Restart 1 in the parfile the state comes from input.h5 instead
+ * (src/main.c:95-100) and prim_in is not read.  Checkpoints
+ * (checkpoint_NNNN.h5) and diagnostics files are written into the working
+ * directory when the parfile's schedule asks for them.
  * state_out: prim (NUM_Q * Nzones), tiph (Nzones), then t and dt.
  * Exit status: 0 ok, 2 usage/io, 3 library error (message on stderr). */
 #include <stdio.h>
@@ -49,20 +53,32 @@ int main(int argc, char **argv) {
   long long nz = cell_nzones(cells);
   int nq = sim_NUM_Q(sim);
   double *buf = (double *)malloc(sizeof(double) * (size_t)(nq + 1) * (size_t)nz);
-  FILE *f = fopen(argv[2], "rb");
-  if (!f || fread(buf, sizeof(double), (size_t)nq * nz, f) != (size_t)nq * nz) {
-    fprintf(stderr, "cannot read %lld zones x %d from %s\n", nz, nq, argv[2]);
-    return 2;
-  }
-  fclose(f);
+  FILE *f;
 
-  (*gravMass_init_ptr(sim))(gm, sim);
-  cell_upload(cells, 0, buf);
+  /* src/main.c:92-105 */
+  struct IO *io = io_create(sim);
+  if (sim_Restart(sim) == 1) {
+    io_allocbuf(io, sim);
+    io_hdf5_in(io, sim, ts);
+    io_readbuf(io, cells, sim, gm);
+    io_deallocbuf(io);
+    if (disco_last_error()) return fail("restart");
+  } else {
+    f = fopen(argv[2], "rb");
+    if (!f || fread(buf, sizeof(double), (size_t)nq * nz, f) != (size_t)nq * nz) {
+      fprintf(stderr, "cannot read %lld zones x %d from %s\n", nz, nq, argv[2]);
+      return 2;
+    }
+    fclose(f);
+    (*gravMass_init_ptr(sim))(gm, sim);
+    cell_upload(cells, 0, buf);
+  }
   gravMass_clean_pi(gm, sim);
   cell_syncproc_r(cells, sim, mpi);
   cell_syncproc_z(cells, sim, mpi);
   cell_calc_cons(cells, sim);
   if (disco_last_error()) return fail("initial data");
+  io_setup(io, sim, ts);
 
   /* src/main.c:121-131: the diagnostics write their files into the
    * working directory when the parfile's schedule asks for them */
@@ -73,6 +89,13 @@ int main(int argc, char **argv) {
     diagnostics_set(dg, cells, sim, ts, mpi, gm);
     diagnostics_print(dg, ts, sim, mpi);
     if (disco_last_error()) return fail("diagnostics");
+    if (timestep_get_t(ts) > io_tcheck(io)) { /* src/main.c:134-139 */
+      io_allocbuf(io, sim);
+      io_setbuf(io, cells, sim, gm);
+      io_hdf5_out(io, sim, ts);
+      io_deallocbuf(io);
+      if (disco_last_error()) return fail("checkpoint");
+    }
   }
   cell_syncproc_r(cells, sim, mpi);
   cell_syncproc_z(cells, sim, mpi);
@@ -88,6 +111,7 @@ int main(int argc, char **argv) {
   free(buf);
 
   diagnostics_destroy(dg, sim);
+  io_destroy(io);
   cell_destroy(cells, sim);
   sim_destroy(sim);
   gravMass_destroy(gm);
