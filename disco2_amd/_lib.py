@@ -120,6 +120,8 @@ SIGNATURES = {
     "disco_batch_diag_measure": (ci, [vp, ci, cd, pd, pd, pd, pd, pd]),
     "disco_batch_set_scalars": (None, [vp, ci, pd]),
     "disco_batch_diag_equat": (ci, [vp, pd]),
+    "disco_batch_initial_data": (ci, [vp, ci]),
+    "disco_initial_prim": (ci, [vp, pd, pd, pd]),
     "disco_batch_checkpoint_write": (ci, [vp, ci, ccp]),
     "disco_batch_checkpoint_read": (ci, [vp, ci, ccp]),
     "io_create": (vp, [vp]),

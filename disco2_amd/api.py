@@ -52,6 +52,16 @@ class Setup:
         return {"N_p": self.N_p, "r_faces": self.r_faces, "z_faces": self.z_faces,
                 "tiph": self.tiph0, "dphi": self.dphi0}
 
+    def initial_prim(self, prim=None):
+        """The primitives the disk's InitialDataType plugin sets (on the
+        host), over prim (default: zeros, as cell_create leaves them)."""
+        out = np.zeros((self.NQ, self.nzones)) if prim is None else np.array(prim, dtype=np.float64, order="C")
+        rc = lib().disco_initial_prim(self.sim, dptr(self.tiph0), dptr(self.dphi0), dptr(out))
+        check()
+        if rc != 0:
+            raise RuntimeError("initial data failed")
+        return out
+
     def interior_zones(self) -> int:
         L = lib()
         i0, i1 = L.sim_Nghost_min(self.sim, 0), self.NR - L.sim_Nghost_max(self.sim, 0)
@@ -157,6 +167,14 @@ class Batch:
         if rc != 0:
             raise RuntimeError("diagnostics failed")
         return vec.reshape(nrg.value, 17), sc, eq[:neq.value * 18].reshape(neq.value, 18) if equat else None, md
+
+    def initial_data(self, disk=0):
+        """Set the disk's primitives with its InitialDataType plugin (the
+        reference's src/main.c:103-104); run before init()."""
+        rc = lib().disco_batch_initial_data(self.h, disk)
+        check()
+        if rc != 0:
+            raise RuntimeError("initial data failed")
 
     def checkpoint(self, path, disk=0):
         """Write the disk's time, masses and cells in the reference's

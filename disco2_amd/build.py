@@ -2,9 +2,11 @@
 
     python -m disco2_amd.build
 
-Every object is compiled by hipcc with -ffp-contract=off, so neither the
-host geometry nor the device kernels contract a*b+c into an FMA: results
-then follow the reference's IEEE operation order.
+Every object is compiled with -ffp-contract=off, so neither the host code
+nor the device kernels contract a*b+c into an FMA: results then follow the
+reference's IEEE operation order.  The initial-data plugins are compiled by
+g++, the reference's compiler family, so that the C library calls they make
+(pow, exp, ...) are folded or called as in the reference build.
 """
 from __future__ import annotations
 
@@ -24,8 +26,10 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-Wno-unused-function", "-Wno-unused-result"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
+CXX = os.environ.get("DISCO_CXX", "g++")
 SOURCES = ["kernels.hip", "adjoint.hip", "diagnostics.hip", "checkpoint.hip", "engine.cpp", "adjoint_host.cpp",
-           "diagnostics.cpp", "checkpoint.cpp", "h5.cpp", "capi.cpp", "sim_host.cpp"]
+           "diagnostics.cpp", "checkpoint.cpp", "h5.cpp", "capi.cpp", "sim_host.cpp", "initial_data.cpp"]
+GCC_SOURCES = {"initial_data.cpp"}
 
 
 def _compile(src: str) -> str:
@@ -33,6 +37,8 @@ def _compile(src: str) -> str:
     obj = os.path.join(BUILD, src + ".o")
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip", *DEVICE, *COMMON, "-c", path, "-o", obj]
+    elif src in GCC_SOURCES:
+        cmd = [CXX, "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-c", path, "-o", obj]
     else:
         cmd = [HIPCC, *COMMON, "-D__HIP_PLATFORM_AMD__", "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -7,8 +7,9 @@ truncating aspect-ratio formula (src/Sim/sim_set.c:17-29).
 """
 from __future__ import annotations
 
-from .params import (make_params, C_RIGID, C_KEPLER, C_FIXED, A_KEPLER, GRAV_NONE,
-                     GRAV_SINGLE, GRAV_BINARY, HLLC, HLL)
+from .params import (make_params, C_RIGID, C_KEPLER, C_FIXED, A_KEPLER, A_MILOS, GRAV_NONE,
+                     GRAV_SINGLE, GRAV_BINARY, HLLC, HLL, BOUND_FIXED, BOUND_OUTFLOW,
+                     SHEAR, VORTEX, SSTEST, TORUS, MILOS_MACFADYEN, RAD_DOM, MIDDLE)
 
 
 def vortex(numr=64, r_min=0.5, r_max=5.0, **over):
@@ -38,6 +39,47 @@ def disk3d(numr=256, numz=64, npcap=1024, **over):
                     RiemannSolver=HLL, PLM=1.0, CFL=0.4, N_p_round=1)
     p.update(over)
     return p, "disk3d", {"seed": 1}
+
+
+# Set-ups whose initial state comes from the reference's own initial-data
+# plugins (InitialDataType, src/Cell/initial_data/cell_init_*.c), with fixed r
+# boundaries and boundary damping where the plugin allows them.  ic_kind
+# "plugin": the disk's primitives are set by Batch.initial_data().
+def plugin(kind, numr=16, **over):
+    base = dict(NumR=numr, NumZ=1, N_p_round=1, CFL=0.4, PLM=1.0)
+    if kind == "shear":  # viscous spreading of a Gaussian shear layer at x = 3
+        base.update(InitialDataType=SHEAR, R_Min=2.0, R_Max=4.0, Adiabatic_Index=1.4, w_numeric_type=C_FIXED,
+                    GravMassType=GRAV_NONE, Explicit_Viscosity=0.05, ViscConst=1, BoundTypeR=BOUND_FIXED)
+    elif kind == "vortex":  # the Gresho-like vortex of radius 1 and its passive marker
+        base.update(InitialDataType=VORTEX, R_Min=0.2, R_Max=0.95, Adiabatic_Index=1.4, w_numeric_type=C_RIGID,
+                    GravMassType=GRAV_NONE, NUM_N=1, BoundTypeR=BOUND_FIXED)
+    elif kind == "sstest":
+        base.update(InitialDataType=SSTEST, R_Min=0.5, R_Max=8.0, Adiabatic_Index=5.0 / 3.0,
+                    w_numeric_type=C_KEPLER, GravMassType=GRAV_BINARY, MassRatio=1.0, G_EPS=0.05,
+                    Explicit_Viscosity=0.01, PoRho_r1=0.01, BoundTypeR=BOUND_FIXED, DAMP_TIME=1.0,
+                    RDAMP_INNER=1.0, RDAMP_OUTER=6.0)
+    elif kind == "middle":
+        base.update(InitialDataType=MIDDLE, R_Min=0.5, R_Max=6.0, Adiabatic_Index=5.0 / 3.0,
+                    w_numeric_type=C_KEPLER, w_analytic_type=A_MILOS, GravMassType=GRAV_BINARY, MassRatio=0.5,
+                    G_EPS=0.05, Explicit_Viscosity=0.01, PoRho_r1=0.01, BoundTypeR=BOUND_OUTFLOW,
+                    DAMP_TIME=0.5, RDAMP_INNER=1.0, RDAMP_OUTER=4.5)
+    elif kind == "milos":
+        base.update(InitialDataType=MILOS_MACFADYEN, R_Min=1.0, R_Max=12.0, Adiabatic_Index=5.0 / 3.0,
+                    w_numeric_type=C_KEPLER, GravMassType=GRAV_BINARY, MassRatio=0.5, G_EPS=0.05,
+                    Explicit_Viscosity=0.01, PoRho_r1=0.01, Rho_Floor=1e-5, BoundTypeR=BOUND_FIXED,
+                    DAMP_TIME=2.0, RDAMP_INNER=2.0, RDAMP_OUTER=10.0)
+    elif kind == "rad_dom":
+        base.update(InitialDataType=RAD_DOM, R_Min=1.0, R_Max=5.0, Adiabatic_Index=5.0 / 3.0,
+                    w_numeric_type=C_KEPLER, GravMassType=GRAV_SINGLE, G_EPS=0.0, Explicit_Viscosity=0.1,
+                    BoundTypeR=BOUND_FIXED)
+    elif kind == "torus":  # 3D torus around a single mass, periodic in z
+        base.update(InitialDataType=TORUS, R_Min=0.5, R_Max=1.5, NumZ=4, Z_Min=-0.3, Z_Max=0.3,
+                    Adiabatic_Index=5.0 / 3.0, w_numeric_type=C_KEPLER, GravMassType=GRAV_SINGLE, Grav_2D=0,
+                    G_EPS=0.0, RiemannSolver=HLL, BoundTypeR=BOUND_OUTFLOW)
+    else:
+        raise ValueError(f"unknown plugin set-up {kind!r}")
+    base.update(over)
+    return make_params(**base), "plugin", {}
 
 
 # The benchmark workloads (sizes of the measurement definition)

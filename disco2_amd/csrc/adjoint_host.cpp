@@ -55,6 +55,8 @@ bool Engine::adjoint_scope() {
     why = "constant or alpha-law viscosity";
   else if (P.bound_r != BOUND_OUTFLOW)
     why = "outflow r boundaries";
+  else if (plan.n_binit > 0)
+    why = "no boundary damping";
   if (why) {
     disco_set_error(12, std::string("the adjoint supports ") + why);
     return false;

@@ -9,6 +9,7 @@
 
 #include "../../include/disco.h"
 #include "engine.h"
+#include "initial_data.h"
 
 bool diag_h5_write(const char *fname, const double *data, size_t n0, size_t n1);  // diagnostics.cpp
 #include "h5.h"
@@ -150,6 +151,65 @@ double cell_gradp(Cell *c, int q) { CellRef *r = R(c); return r->h->gradp[q * r-
 double cell_tiph(Cell *c) { CellRef *r = R(c); return r->h->tiph[r->zone]; }
 double cell_dphi(Cell *c) { CellRef *r = R(c); return r->h->dphi[r->zone]; }
 double cell_wiph(Cell *c) { CellRef *r = R(c); return r->h->wiph[r->zone]; }
+
+// ------------------------------------------------------------------ initial data
+// The plugins of src/Cell/initial_data/*.c run on the host over the cell
+// mirror (initial_data.cpp); the device copy follows before the next launch.
+static void cell_init_type(Cell ***c, Sim *s, GravMass *g, int type) {
+  CellsHandle *h = H(c);
+  mirror_pull(h);
+  if (disco_cell_init(s, type, g, h->tiph.data(), h->dphi.data(), h->prim.data(), h->nz, h->NQ))
+    h->dirty_prim = true;
+}
+static void cell_single_init_type(Cell *cell, Sim *s, int type, int i) {
+  CellRef *r = R(cell);
+  CellsHandle *h = r->h;
+  double o[DISCO_MAXQ];
+  for (int q = 0; q < h->NQ; ++q) o[q] = h->prim[q * h->nz + r->zone];
+  if (disco_single_init(s, type, i, h->tiph[r->zone], h->dphi[r->zone], o, h->NQ) <= 0) return;
+  for (int q = 0; q < h->NQ; ++q) h->prim[q * h->nz + r->zone] = o[q];
+  h->dirty_prim = true;
+}
+#define DISCO_PLUGIN(name, type)                                                                        \
+  void cell_init_##name(Cell ***c, Sim *s, GravMass *g, MPIsetup *) { cell_init_type(c, s, g, type); } \
+  void cell_single_init_##name(Cell *c, Sim *s, GravMass *, int i, int, int) { cell_single_init_type(c, s, type, i); }
+DISCO_PLUGIN(shear, ID_SHEAR)
+DISCO_PLUGIN(vortex, ID_VORTEX)
+DISCO_PLUGIN(SStest, ID_SSTEST)
+DISCO_PLUGIN(torus, ID_TORUS)
+DISCO_PLUGIN(milos_macfadyen, ID_MILOS_MACFADYEN)
+DISCO_PLUGIN(rad_dom, ID_RAD_DOM)
+DISCO_PLUGIN(middle, ID_MIDDLE)
+#undef DISCO_PLUGIN
+
+// src/Cell/cell_init_function_ptr.c:6-46; NULL (error set) where the
+// reference prints an error and exits
+void (*cell_init_ptr(Sim *s))(Cell ***, Sim *, GravMass *, MPIsetup *) {
+  switch (s->InitialDataType) {
+    case ID_SHEAR: return &cell_init_shear;
+    case ID_VORTEX: return &cell_init_vortex;
+    case ID_TORUS: return &cell_init_torus;
+    case ID_MILOS_MACFADYEN: return &cell_init_milos_macfadyen;
+    case ID_RAD_DOM: return &cell_init_rad_dom;
+    case ID_MIDDLE: return &cell_init_middle;
+    case ID_SSTEST: return &cell_init_SStest;
+  }
+  disco_set_error(8, "not a valid global initial data routine choice");
+  return nullptr;
+}
+void (*cell_single_init_ptr(Sim *s))(Cell *, Sim *, GravMass *, int, int, int) {
+  switch (s->InitialDataType) {
+    case ID_SHEAR: return &cell_single_init_shear;
+    case ID_VORTEX: return &cell_single_init_vortex;
+    case ID_TORUS: return &cell_single_init_torus;
+    case ID_MILOS_MACFADYEN: return &cell_single_init_milos_macfadyen;
+    case ID_RAD_DOM: return &cell_single_init_rad_dom;
+    case ID_MIDDLE: return &cell_single_init_middle;
+    case ID_SSTEST: return &cell_single_init_SStest;
+  }
+  disco_set_error(8, "not a valid single point initial data routine choice");
+  return nullptr;
+}
 
 void cell_set_prim(Cell ***c, int i, int j, int k, int q, double v) {  // src/Cell/cell_modify_data.c:13-15
   CellsHandle *h = H(c);
@@ -803,6 +863,32 @@ void io_hdf5_in(IO *io, Sim *sim, TimeStep *ts) {
   timestep_set_t(ts, T);
 }
 
+// the InitialDataType plugin over host arrays of one disk (host only)
+int disco_initial_prim(Sim *s, const double *tiph, const double *dphi, double *prim) {
+  long long NL = 0;
+  for (int n : s->N_p) NL += n;
+  const long long nz = NL * (long long)(s->z_faces.size() - 1);
+  return disco_cell_init(s, s->InitialDataType, nullptr, tiph, dphi, prim, nz, s->NUM_Q) ? 0 : -1;
+}
+
+// the disk's InitialDataType plugin (src/main.c:103-104) over its current
+// cells, on the host
+int disco_batch_initial_data(DiscoBatch *b, int d) {
+  Engine *e = b->e;
+  if (d < 0 || d >= e->ndisk) {
+    disco_set_error(9, "initial data: no such disk");
+    return -1;
+  }
+  const long long nz = e->nzones[d];
+  std::vector<double> prim((size_t)e->NQ * nz), tiph(nz), dphi(nz);
+  e->download(d, 0, prim.data());
+  e->download(d, 5, tiph.data());
+  e->download(d, 7, dphi.data());
+  Sim *S = e->sims[d];
+  if (!disco_cell_init(S, S->InitialDataType, nullptr, tiph.data(), dphi.data(), prim.data(), nz, e->NQ)) return -1;
+  e->upload(d, 0, prim.data());
+  return 0;
+}
 int disco_batch_checkpoint_write(DiscoBatch *b, int d, const char *fname) { return b->e->checkpoint_write(d, fname); }
 int disco_batch_checkpoint_read(DiscoBatch *b, int d, const char *fname) { return b->e->checkpoint_read(d, fname); }
 }

@@ -1,5 +1,7 @@
 // Internal host-side definitions of the opaque classes of include/disco.h.
 #pragma once
+#include <stdint.h>
+
 #include <string>
 #include <vector>
 
@@ -65,6 +67,36 @@ struct TimeStep {  // src/Headers/TimeStep.h:10-18 fields
   double t = 0, dt = 0, RK = 0;
   std::vector<int> nri, nzk;
   int Nfr = 0, Nfz = 0;
+};
+
+// glibc's additive-feedback rand() seeded with srand(seed), as the reference
+// draws it (src/Cell/cell_create_destroy.c:64-71 and the torus initial data,
+// src/Cell/initial_data/cell_init_torus.c:31,79): state s[0] = seed,
+// s[i] = 16807 s[i-1] mod (2^31 - 1) for i < 31, s[i] = s[i-31] for
+// 31 <= i < 34, then s[i] = s[i-31] + s[i-3] mod 2^32; the n-th output is
+// s[n+344] >> 1.  The last 34 words are kept in a ring.
+struct GlibcRand {
+  uint32_t r[34];
+  int k = 0;  // r[(k + m) % 34] holds s[n - 34 + m] for the next output n
+  explicit GlibcRand(unsigned seed) {
+    uint32_t s[344];
+    int64_t w = seed == 0 ? 1 : seed;
+    s[0] = (uint32_t)w;
+    for (int i = 1; i < 31; ++i) {
+      w = (16807LL * w) % 2147483647LL;
+      if (w < 0) w += 2147483647LL;
+      s[i] = (uint32_t)w;
+    }
+    for (int i = 31; i < 34; ++i) s[i] = s[i - 31];
+    for (int i = 34; i < 344; ++i) s[i] = s[i - 31] + s[i - 3];
+    for (int m = 0; m < 34; ++m) r[m] = s[310 + m];
+  }
+  int next() {
+    const uint32_t v = r[(k + 3) % 34] + r[(k + 31) % 34];
+    r[k] = v;
+    k = (k + 1) % 34;
+    return (int)(v >> 1);
+  }
 };
 
 // Per-cell reference handed out by cell_single / face_*_pointer.

@@ -19,6 +19,8 @@ enum { C_FIXED, C_WCELL, C_RIGID, C_KEPLER, C_OMEGA20, C_MILOS, C_POWERLAW };
 enum { A_FIXED, A_KEPLER, A_OMEGA20, A_MILOS, A_POWERLAW };
 enum { BOUND_FIXED, BOUND_OUTFLOW, BOUND_PERIODIC };
 enum { HLLC_SOLVER = 0, HLL_SOLVER = 1 };
+// InitialDataType, src/Headers/header.h:8
+enum { ID_SHEAR, ID_VORTEX, ID_SSTEST, ID_TORUS, ID_MILOS_MACFADYEN, ID_RAD_DOM, ID_MIDDLE };
 
 // sticky device error codes
 enum {
@@ -56,6 +58,22 @@ struct DiskScalars {
   // launch-uniform terms, recomputed whenever the masses move:
   // (xbh, ybh) = (r cos phi, r sin phi) of each mass, epsn = pow(G_EPS, PHI_ORDER)
   double xbh[2], ybh[2], epsn, pad3;
+};
+
+// Per annulus: what the fixed r boundaries (cell_boundary_fixed_r,
+// src/Cell/cell_boundary.c:208-232) and the boundary damping (cell_bc_damp,
+// src/Cell/cell_misc.c:98-132) need of the initial-data plugin's single-cell
+// function.  The parts that depend on the annulus alone are tabulated on the
+// host at set-up with the C library the reference uses; the parts that
+// depend on the cell's angle are evaluated on the device.
+enum { INIT_FIXED = 1, INIT_DAMP = 2, INIT_SET = 4 };
+struct InitRow {
+  double prim[5];   // rho, P, v_r, omega, v_z (INIT_SET: the plugin writes them)
+  double a;         // shear: rOm_a(r, 1) / r
+  double c0, c1;    // shear: v0 / sqrt(2 pi nu t0) and 4 nu t0
+  double rate;      // damping rate (INIT_DAMP)
+  int flags;        // INIT_FIXED: a fixed-boundary ghost annulus
+  int pad;
 };
 
 struct AnnInfo {

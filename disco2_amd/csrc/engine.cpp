@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../../include/disco.h"
+#include "initial_data.h"
 
 bool hip_check(hipError_t e, const char *what) {
   if (e == hipSuccess) return true;
@@ -38,7 +39,9 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     return;
   }
   std::vector<double> hrf, hzf;
-  std::vector<int2> ztiles, utiles, itiles, rft, zft, bc0, bc1, szt;
+  std::vector<int2> ztiles, utiles, itiles, rft, zft, bc0, bc1, szt, bit;
+  std::vector<InitRow> hinit;
+  bool any_init = false;
   long long zb = 0, fb = 0;
   int rb = 0;
   // r pairs of all disks first, then z pairs
@@ -47,11 +50,6 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     Sim *S = sims[d];
     if (S->NUM_C != 5) {
       disco_set_error(7, "NUM_C must be 5");
-      return;
-    }
-    if (S->BoundTypeR == BOUND_FIXED || (S->N_global[1] > 1 && S->BoundTypeZ == BOUND_FIXED) ||
-        S->DAMP_TIME > 0.0) {
-      disco_set_error(8, "fixed boundaries and damping need an initial-data plugin: not supported");
       return;
     }
     if (S->W_NUMERIC_TYPE == C_WCELL) {
@@ -85,6 +83,12 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     P.itile0 = (int)itiles.size();
     zbase.push_back(zb);
     nzones.push_back(NL * NZ);
+    // fixed r boundaries and damping: rows of the single-cell initial data
+    std::vector<InitRow> rows;
+    bool any = false;
+    if (!disco_init_rows(S, P.NQ, rows, &any)) return;
+    hinit.insert(hinit.end(), rows.begin(), rows.end());
+    any_init |= any;
     for (int k = 0; k < NZ; ++k)
       for (int i = 0; i < NR; ++i) {
         AnnInfo A;
@@ -113,6 +117,8 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
         }
         if (P.NglobalZ > 1 && (k < P.NgMinZ || k >= NZ - P.NgMaxZ))
           for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) szt.push_back(make_int2(a, j0));
+        if (rows[(size_t)k * NR + i].flags & (INIT_FIXED | INIT_DAMP))
+          for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) bit.push_back(make_int2(a, j0));
       }
     P.itile1 = (int)itiles.size();
     zb += NL * NZ;
@@ -204,6 +210,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   good &= dalloc(&d_err, 1, "err");
   good &= dalloc(&d_dt_tile, itiles.empty() ? 1 : itiles.size(), "dt tiles");
   good &= dalloc(&d_ann_rot, (size_t)nann, "annulus rotation");
+  if (any_init) good &= dalloc(&d_init, hinit.size(), "init rows") && h2d(d_init, hinit.data(), hinit.size());
   if (!good) return;
   hipMemset(d_err, 0, sizeof(int));
   hipMemset(d_prim[0], 0, nq * sizeof(double));
@@ -243,7 +250,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   };
   size_t o_z = add(ztiles, plan.n_ztiles), o_u = add(utiles, plan.n_utiles), o_i = add(itiles, plan.n_itiles), o_r = add(rft, plan.n_rftiles),
          o_zf = add(zft, plan.n_zftiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
-         o_s = add(szt, plan.n_sz);
+         o_s = add(szt, plan.n_sz), o_bi = add(bit, plan.n_binit);
   if (!dalloc(&d_tiles, all.size(), "tiles") || !h2d(d_tiles, all.data(), all.size())) return;
   plan.ztiles = d_tiles + o_z;
   plan.utiles = d_tiles + o_u;
@@ -253,6 +260,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   plan.bc0 = d_tiles + o_b0;
   plan.bc1 = d_tiles + o_b1;
   plan.sz = d_tiles + o_s;
+  plan.binit = d_tiles + o_bi;
   plan.ndisk = n;
   plan.any_visc = 0;
   for (auto &P : hdp) plan.any_visc |= P.visc > 0.0;
@@ -289,7 +297,7 @@ Engine::~Engine() {
   if (graph) hipGraphDestroy(graph);
   void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
-                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot, adj_mem, diag_mem, ckpt_mem};
+                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot, d_init, adj_mem, diag_mem, ckpt_mem};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (ev0) hipEventDestroy(ev0);
@@ -330,6 +338,7 @@ DevMesh Engine::mesh() const {
   m.err = d_err;
   m.dt_tile = d_dt_tile;
   m.ann_rot = d_ann_rot;
+  m.init = d_init;
   m.u_save = adj_usave;
   m.torque_tile = adj_torque;
   m.ftype = adj_ftype;

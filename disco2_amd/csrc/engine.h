@@ -47,6 +47,7 @@ struct Engine {
   int *d_err = nullptr;
   double *d_dt_tile = nullptr;
   double4 *d_ann_rot = nullptr;
+  InitRow *d_init = nullptr;  // fixed r boundaries and damping, per annulus (null when no disk has them)
   int2 *d_tiles = nullptr;
   int cur = 0;  // index of the prim / tiph / dphi buffers holding the current state
   bool w_static = false;  // no disk moves its annuli with the gas (C_RIGID): wiph depends on r alone

@@ -25,6 +25,7 @@ struct DevMesh {
   double *Fface;          // [NQ+3][NFtot] increments of the r/z overlap faces
   double *dt_tile;        // CFL minimum of each interior tile (k_mindt), reduced per disk by k_scalar
   double4 *ann_rot;       // per annulus: rOm_a, rdrOm_a, dtOm_a at the centre, rdrOm_a at the outer face
+  const InitRow *init;    // per annulus: fixed r boundaries and damping (null when unused)
   // adjoint recording (null in plain runs): k_update stores the conserved
   // state it converts (u_save, NQ x Ntot) and its tiles' torque sums
   double *u_save, *torque_tile;
@@ -58,6 +59,7 @@ struct LaunchPlan {
   const int2 *bc0; int n_bc0;           // ghost annuli 0 and NR-2
   const int2 *bc1; int n_bc1;           // ghost annulus NR-1
   const int2 *sz; int n_sz;             // z ghost layers
+  const int2 *binit; int n_binit;       // fixed-boundary ghost annuli and damped annuli
 };
 
 // Optional per-kernel timing: when g_kprof is set, every launch is bracketed

@@ -977,6 +977,62 @@ __global__ __launch_bounds__(TILE) void k_sync_z(DevMesh m, const int2 *tiles) {
 }
 
 // ----------------------------------------------------------------------------
+// Fixed r boundaries, cell_boundary_fixed_r (src/Cell/cell_boundary.c:208-232),
+// then boundary damping, cell_bc_damp (src/Cell/cell_misc.c:98-132), on the
+// annuli the host marked in their InitRow.  A ghost cell of a fixed boundary
+// takes the single-cell initial data (src/Cell/initial_data/*.c) at its new
+// edge; a damped cell relaxes towards the initial data by 1 - exp(-rate dt).
+// Per cell the boundary comes first, as in the reference.
+// ----------------------------------------------------------------------------
+template <int NQ>
+__device__ __forceinline__ void single_init(int type, const InitRow &R, double r, double tiph, double dphi,
+                                            double *pr) {
+  if (type == ID_SHEAR) {  // cell_init_shear.c:11-55
+    const double t = tiph - .5 * dphi;
+    const double x = r * cos(t) - 3.;
+    const double y = r * sin(t);
+    double vy = 0.0;
+    if (fabs(y) < 50.) vy = R.c0 * exp(-x * x / R.c1);
+    pr[RHO] = 1.0;
+    pr[PPP] = 0.01;
+    pr[URR] = vy * sin(t);
+    pr[UPP] = vy * cos(t) / r - R.a;
+    pr[UZZ] = 0.0;
+    return;
+  }
+  if (!(R.flags & INIT_SET)) return;  // the vortex outside its core writes nothing
+#pragma unroll
+  for (int q = 0; q < 5; ++q) pr[q] = R.prim[q];
+  if (type == ID_VORTEX && NQ > 5) pr[5] = cos(tiph - .5 * dphi) < 0.0 ? 0.0 : 1.0;  // cell_init_vortex.c:146,152
+}
+
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_bc_init(DevMesh m, const int2 *tiles, double fac) {
+  const int2 tl = tiles[xcd_order()];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const InitRow R = m.init[tl.x];
+  const DiskParams &P = disk_of(m, A.disk);
+  const long long c = A.zbase + j;
+  const double r = 0.5 * (A.rm + A.rp);
+  double pr[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) pr[q] = m.prim[q * m.Ntot + c];
+  if (R.flags & INIT_FIXED) single_init<NQ>(P.initial_data_type, R, r, m.tiph[c], m.dphi[c], pr);
+  if (R.flags & INIT_DAMP) {  // NUM_Q = 5 and initial data of r alone (the host checks)
+    const double dt = fac * m.sc[A.disk].dt;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const double dprim = R.prim[q] - pr[q];
+      pr[q] += (1. - exp(-R.rate * dt)) * dprim;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) m.prim[q * m.Ntot + c] = pr[q];
+}
+
+// ----------------------------------------------------------------------------
 // CFL time step over interior cells, cell_mindt, src/Cell/cell_mindt.c:10-100.
 // Each workgroup stores the minimum of its tile with a plain store (no
 // device-wide atomics: thousands of them on one word serialise); k_scalar's
@@ -1246,6 +1302,8 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   m2.dphi = m.dphi_new;
   if (L.n_bc0) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc0), B, 0, st, m2, L.bc0));
   if (L.n_bc1) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc1), B, 0, st, m2, L.bc1));
+  if (L.n_binit)
+    KLAUNCH(st, "k_bc_init", hipLaunchKernelGGL(k_bc_init<NQ>, dim3(L.n_binit), B, 0, st, m2, L.binit, fac));
   if (L.n_sz) KLAUNCH(st, "k_sync_z", hipLaunchKernelGGL(k_sync_z<NQ>, dim3(L.n_sz), B, 0, st, m2, L.sz));
 }
 

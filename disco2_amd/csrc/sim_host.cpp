@@ -385,35 +385,7 @@ double sim_dtOm_a(Sim *s, double r, double a) {
 // ------------------------------------------------------------------ phi offsets
 // The reference seeds the C library generator with srand(666 + rank) and
 // draws one offset per (k, i) annulus, k outer, i inner
-// (src/Cell/cell_create_destroy.c:64-71).  That generator is glibc's
-// additive-feedback rand(): state r[0] = seed, r[i] = 16807 r[i-1] mod
-// (2^31 - 1) for i < 31, r[i] = r[i-31] for 31 <= i < 34, then
-// r[i] = r[i-31] + r[i-3] mod 2^32, and the k-th output is r[k+344] >> 1.
-namespace {
-struct GlibcRand {
-  std::vector<uint32_t> r;
-  size_t k = 0;
-  explicit GlibcRand(unsigned seed) {
-    r.resize(34);
-    int64_t w = seed == 0 ? 1 : seed;
-    r[0] = (uint32_t)w;
-    for (int i = 1; i < 31; ++i) {
-      w = (16807LL * w) % 2147483647LL;
-      if (w < 0) w += 2147483647LL;
-      r[i] = (uint32_t)w;
-    }
-    for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
-    for (int i = 34; i < 344; ++i) r.push_back(r[i - 31] + r[i - 3]);
-    k = 344;
-  }
-  int next() {
-    uint32_t v = r[k - 31] + r[k - 3];
-    r.push_back(v);
-    ++k;
-    return (int)(v >> 1);
-  }
-};
-}  // namespace
+// (src/Cell/cell_create_destroy.c:64-71); GlibcRand restates that generator.
 
 // tiph and dphi of every zone exactly as cell_create leaves them
 // (src/Cell/cell_create_destroy.c:64-91) followed by cell_clean_pi

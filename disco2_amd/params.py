@@ -36,7 +36,8 @@ EXTRA_KEYS = {"N_p_round": int}
 C_FIXED, C_WCELL, C_RIGID, C_KEPLER, C_OMEGA20, C_MILOS, C_POWERLAW = range(7)
 A_FIXED, A_KEPLER, A_OMEGA20, A_MILOS, A_POWERLAW = range(5)
 BOUND_FIXED, BOUND_OUTFLOW, BOUND_PERIODIC = range(3)
-SHEAR, VORTEX = 0, 1
+# InitialDataType (src/Headers/header.h:8)
+SHEAR, VORTEX, SSTEST, TORUS, MILOS_MACFADYEN, RAD_DOM, MIDDLE = range(7)
 GRAV_NONE, GRAV_SINGLE, GRAV_BINARY = range(3)
 HLLC, HLL = 0, 1
 

@@ -155,6 +155,32 @@ double cell_gradp(struct Cell *, int q);                           /* Cell.h:49 
 double cell_tiph(struct Cell *);                                   /* Cell.h:51 */
 double cell_dphi(struct Cell *);                                   /* Cell.h:52 */
 double cell_wiph(struct Cell *);                                   /* Cell.h:53 */
+/* Initial-data plugins (src/Cell/initial_data/cell_init_*.c), on the host over the
+ * cells' host copy.  The global functions set every cell the reference sets
+ * from its edge and the masses; the single-cell functions set one cell (the
+ * torus's sets the error: the reference exits there).  cell_init_ptr and
+ * cell_single_init_ptr return NULL with the error set for an unknown
+ * InitialDataType, where the reference exits. */
+/* the InitialDataType plugin over host arrays of one disk (tiph, dphi per
+ * zone; prim q-major [NUM_Q][zones], in and out); host only, no reference
+ * counterpart (the plugins write theCells).  Returns 0 or -1 (error set). */
+int disco_initial_prim(struct Sim *, const double *tiph, const double *dphi, double *prim);
+void cell_init_shear(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *);           /* Cell.h:30 */
+void cell_single_init_shear(struct Cell *, struct Sim *, struct GravMass *, int, int, int);          /* Cell.h:31 */
+void cell_init_vortex(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *);          /* Cell.h:32 */
+void cell_single_init_vortex(struct Cell *, struct Sim *, struct GravMass *, int, int, int);         /* Cell.h:33 */
+void cell_init_torus(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *);           /* Cell.h:34 */
+void cell_single_init_torus(struct Cell *, struct Sim *, struct GravMass *, int, int, int);          /* Cell.h:35 */
+void cell_init_milos_macfadyen(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *); /* Cell.h:36 */
+void cell_single_init_milos_macfadyen(struct Cell *, struct Sim *, struct GravMass *, int, int, int); /* Cell.h:37 */
+void cell_init_rad_dom(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *);         /* Cell.h:38 */
+void cell_single_init_rad_dom(struct Cell *, struct Sim *, struct GravMass *, int, int, int);        /* Cell.h:39 */
+void cell_init_middle(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *);          /* Cell.h:40 */
+void cell_single_init_middle(struct Cell *, struct Sim *, struct GravMass *, int, int, int);         /* Cell.h:41 */
+void cell_init_SStest(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *);          /* Cell.h:42 */
+void cell_single_init_SStest(struct Cell *, struct Sim *, struct GravMass *, int, int, int);         /* Cell.h:43 */
+void (*cell_init_ptr(struct Sim *))(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *); /* Cell.h:44 */
+void (*cell_single_init_ptr(struct Sim *))(struct Cell *, struct Sim *, struct GravMass *, int, int, int); /* Cell.h:45 */
 void cell_set_prim(struct Cell ***, int i, int j, int k, int q, double v); /* Cell.h:89 */
 void cell_set_tiph(struct Cell ***, int i, int j, int k, double v);        /* Cell.h:90 */
 void cell_clean_pi(struct Cell ***, struct Sim *);                 /* Cell.h:60 */
@@ -301,6 +327,10 @@ int disco_batch_diag_measure(struct DiscoBatch *, int disk, double dtout, const 
 /* equat may be NULL: the equatorial cells then stay on the device, and
  * disco_batch_diag_equat copies those of the last measurement */
 int disco_batch_diag_equat(struct DiscoBatch *, double *equat);
+/* The disk's InitialDataType plugin (the step of src/main.c:103-104) over
+ * its current cells, on the host, bit for bit the reference's; run it
+ * before disco_batch_init.  Returns 0 or -1 (error set). */
+int disco_batch_initial_data(struct DiscoBatch *, int disk);
 /* A checkpoint of one disk in the reference's file layout (see IO above):
  * write its time, masses and cells; read them back into a disk as the
  * reference restarts (src/main.c:95-105): t, the masses' r, phi, M, omega

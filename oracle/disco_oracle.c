@@ -1341,6 +1341,276 @@ static void sync_z(OracleSim *s) {
   }
 }
 
+/* ---------------- initial data: src/Cell/initial_data/cell_init_*.c -------
+ * The seven plugins of src/Cell/cell_init_function_ptr.c:6-46.  The global
+ * functions write the cells' primitives; the single-cell ones write one
+ * cell, given its annulus i and edges.  Both keep the reference's loops and
+ * expression order; the torus draws from the C library's own rand(). */
+enum { ID_SHEAR, ID_VORTEX, ID_SSTEST, ID_TORUS, ID_MILOS, ID_RAD_DOM, ID_MIDDLE };
+
+static double ann_r(const OracleSim *s, int i) { return 0.5 * (rpos(s, i - 1) + rpos(s, i)); }
+
+/* the five primitives of the single-cell function of the disk's type for a
+ * cell of annulus i with edge t_iph and width d_phi (cell_single_init_*):
+ * returns 1 when it writes them, 0 when it writes nothing (the vortex
+ * outside its core), -1 for the torus (cell_init_torus.c:12-15 exits);
+ * *passive gets the vortex's passive value when there is one */
+static int single_init(const OracleSim *s, int i, double t_iph, double d_phi, double *pr, double *passive) {
+  double r = ann_r(s, i);
+  switch (s->ip[OI_InitialDataType]) {
+    case ID_SHEAR: { /* cell_init_shear.c:11-55 */
+      double rho = 1.0, Pp = 0.01, v0 = 1.0, t0 = 0.25;
+      double t = t_iph - .5 * d_phi;
+      double x = r * cos(t) - 3.;
+      double y = r * sin(t);
+      double nu = s->dp[OD_EXPLICIT_VISCOSITY] > 0.0 ? s->dp[OD_EXPLICIT_VISCOSITY] : 0.05;
+      double vy = 0.0;
+      if (fabs(y) < 50.) vy = v0 / sqrt(2. * M_PI * nu * t0) * exp(-x * x / (4. * nu * t0));
+      double vr = vy * sin(t);
+      double omega = vy * cos(t) / r;
+      pr[RHO] = rho; pr[PPP] = Pp; pr[URR] = vr; pr[UPP] = omega - rOm_a(s, r, 1.) / r; pr[UZZ] = 0.0;
+      return 1;
+    }
+    case ID_VORTEX: { /* cell_init_vortex.c:128-154 */
+      double rho = 1.0, P0 = 1.0, R = 1.0, A = 0.5, g = 3.0;
+      double t = t_iph - .5 * d_phi;
+      if (r >= R) return 0;
+      double Pp = P0 * (1. - A * exp(g * (r / R - R / (R - r) + 1.)));
+      double vp = sqrt(A * g * P0 / rho * (2. * R - r) / R * r * r / pow(R - r, 2.) * exp(g * (r / R - R / (R - r) + 1.)));
+      *passive = cos(t) < 0.0 ? 0.0 : 1.0;
+      pr[RHO] = rho; pr[PPP] = Pp; pr[URR] = 0.0; pr[UPP] = vp / r; pr[UZZ] = 0.0;
+      return 1;
+    }
+    case ID_SSTEST:
+    case ID_MIDDLE: { /* cell_init_SStest.c:12-47, cell_init_middle.c:12-47 */
+      int mid = s->ip[OI_InitialDataType] == ID_MIDDLE;
+      double alpha = s->dp[OD_EXPLICIT_VISCOSITY];
+      double r0 = mid ? 2.5 : 5.;
+      int d = mid ? 10 : 4;
+      double rho = pow(r, -3. / 5.) * exp(-pow(r / r0, -d));
+      double P = s->dp[OD_PoRho_r1] * pow(r, -3. / 2.) * exp(-pow(r / r0, -d));
+      double r_drP_o_P = -1.5 + d * pow(r / r0, -d);
+      if (rho < (mid ? 1.e-3 : 1.e-4)) rho = mid ? 1.e-3 : 1.e-4;
+      if (P < (mid ? 1.e-6 : 1.e-7)) P = mid ? 1.e-6 : 1.e-7;
+      double o2 = 1. / r / r / r * pow(1. + 3. / 16. / r / r, 2) + r_drP_o_P * P / rho / r / r;
+      double omega = sqrt(o2);
+      double a = 1.0;
+      omega = (pow(r, W_A_MILOS_INDEX) * omega + pow(a, W_A_MILOS_INDEX - 1.5)) /
+              (pow(r, W_A_MILOS_INDEX) + pow(a, W_A_MILOS_INDEX));
+      double vr = mid ? -3. * alpha / r / omega * P / rho * (2. + r_drP_o_P) : 0.0;
+      pr[RHO] = rho; pr[PPP] = P; pr[URR] = vr;
+      pr[UPP] = (mid ? omega : pow(r, -1.5)) - rOm_a(s, r, a) / r;
+      pr[UZZ] = 0.0;
+      return 1;
+    }
+    case ID_MILOS: { /* cell_init_milos_macfadyen.c:11-35 */
+      double Mach = 1. / sqrt(s->dp[OD_PoRho_r1]);
+      double Gam = s->dp[OD_GAMMALAW];
+      double rho0 = 1.0, rs = 10., delta_exp = 3.0, xi_exp = 2.0;
+      double rho = rho0 * (pow(rs / r, delta_exp)) * exp(-pow(rs / r, xi_exp));
+      double PoRho = 1. / Mach / Mach / Gam;
+      pr[RHO] = rho; pr[PPP] = PoRho * rho; pr[URR] = 0.0; pr[UPP] = 0.0; pr[UZZ] = 0.0;
+      return 1;
+    }
+    case ID_RAD_DOM: { /* cell_init_rad_dom.c:103-131 */
+      double blah = 0.0;
+      double xi = 0.005, a_o_M = 100;
+      double alpha = s->dp[OD_EXPLICIT_VISCOSITY];
+      double Gam = s->dp[OD_GAMMALAW];
+      double rho0 = 2. / (9. * xi * alpha * Gam) * sqrt(a_o_M);
+      double P0 = 8. * xi / (9. * alpha * Gam) * sqrt(a_o_M);
+      double rho = rho0 * pow(r, 1.5);
+      double P = P0 * pow(r, -1.5);
+      double o2 = 1. / r / r / r - 3. / 2. * P / rho / r / r;
+      double omega = sqrt(o2);
+      double vr = -1.5 * alpha * Gam * (P / rho) * sqrt(r) - blah * pow(r, -4.5);
+      pr[RHO] = rho; pr[PPP] = P; pr[URR] = vr; pr[UPP] = omega - 1. / pow(r, 1.5); pr[UZZ] = 0.0;
+      return 1;
+    }
+  }
+  return -1;
+}
+
+/* the single-cell function applied to cell z of annulus i; it also writes
+ * the cell's wiph, which cell_clear_w resets before any use */
+static int single_cell(OracleSim *s, int i, int z) {
+  double pr[5], passive = 0.0;
+  int type = s->ip[OI_InitialDataType];
+  int set = single_init(s, i, s->tiph[z], s->dphi[z], pr, &passive);
+  if (set < 0) return 17;
+  if (set == 0) return 0;
+  for (int q = 0; q < 5; ++q) P(s, q, z) = pr[q];
+  if (type == ID_VORTEX && s->NC < s->NQ) P(s, s->NC, z) = passive;
+  if (type == ID_RAD_DOM)
+    s->wiph[z] = pow(ann_r(s, i), -.5); /* cell_init_rad_dom.c:128 */
+  else if (type != ID_VORTEX)
+    s->wiph[z] = 0.0;
+  return 0;
+}
+
+static int fixed_r(OracleSim *s) { /* cell_boundary_fixed_r, cell_boundary.c:208-232 */
+  int e;
+  if (s->ip[OI_NoInnerBC] != 1)
+    for (int i = 0; i < s->ip[OI_NgMinR]; ++i)
+      for (int k = 0; k < s->NZ; ++k)
+        for (int j = 0; j < s->N_p[i]; ++j)
+          if ((e = single_cell(s, i, zid(s, k, i, j)))) return e;
+  for (int i = s->NR - 1; i > s->NR - s->ip[OI_NgMaxR] - 1; --i)
+    for (int k = 0; k < s->NZ; ++k)
+      for (int j = 0; j < s->N_p[i]; ++j)
+        if ((e = single_cell(s, i, zid(s, k, i, j)))) return e;
+  return 0;
+}
+
+static int fixed_z(OracleSim *s) { /* cell_boundary_fixed_z, cell_boundary.c:234-254 */
+  int e;
+  for (int i = 0; i < s->NR; ++i)
+    for (int k = 0; k < s->ip[OI_NgMinZ]; ++k)
+      for (int j = 0; j < s->N_p[i]; ++j)
+        if ((e = single_cell(s, i, zid(s, k, i, j)))) return e;
+  for (int i = 0; i < s->NR; ++i)
+    for (int k = s->NZ - 1; k > s->NZ - s->ip[OI_NgMaxZ] - 1; --k)
+      for (int j = 0; j < s->N_p[i]; ++j)
+        if ((e = single_cell(s, i, zid(s, k, i, j)))) return e;
+  return 0;
+}
+
+/* cell_bc_damp, cell_misc.c:98-132.  The reference's initialCell comes from
+ * an uninitialised malloc (cell_create_destroy.c:12-16): what the plugin does
+ * not write, and the cell's edges, are NaN here. */
+static int bc_damp(OracleSim *s, double dt) {
+  double RMIN = s->dp[OD_RMIN], RMAX = s->dp[OD_RMAX];
+  double RDI = s->dp[OD_RDAMP_INNER], RDO = s->dp[OD_RDAMP_OUTER];
+  double R0 = 1. / s->dp[OD_DAMP_TIME] / 2. / M_PI;
+  double init[16];
+  for (int i = 0; i < s->NR; ++i) {
+    double rp = rpos(s, i), rm = rpos(s, i - 1);
+    double r = .5 * (rp + rm);
+    if (r < RDI || r > RDO) {
+      double rate = R0 * (r - RDI) / (RMIN - RDI);
+      if (r > RDO) rate = R0 * (r - RDO) / (RMAX - RDO);
+      for (int k = 0; k < s->NZ; ++k)
+        for (int j = 0; j < s->N_p[i]; ++j) {
+          double passive = NAN;
+          for (int q = 0; q < s->NQ; ++q) init[q] = NAN;
+          if (single_init(s, i, NAN, NAN, init, &passive) < 0) return 17;
+          if (s->ip[OI_InitialDataType] == ID_VORTEX && s->NC < s->NQ) init[s->NC] = passive;
+          int z = zid(s, k, i, j);
+          for (int q = 0; q < s->NQ; ++q) {
+            double dprim = init[q] - P(s, q, z);
+            P(s, q, z) += (1. - exp(-rate * dt)) * dprim;
+          }
+        }
+    }
+  }
+  return 0;
+}
+
+/* the global functions, cell_init_*.c; returns 0, or 17 for an unknown type */
+int oracle_initial_data(OracleSim *s) {
+  grav_init(s); /* src/main.c:103 sets the masses first */
+  int type = s->ip[OI_InitialDataType];
+  if (type == ID_TORUS) { /* cell_init_torus.c:17-90 */
+    double GAMMALAW = s->dp[OD_GAMMALAW];
+    double M = 1.0, r_inner = 0.7, r_max = 1.0, q = 2.;
+    double sx = 2. * q - q;
+    double GammaFac = (GAMMALAW - 1.) / GAMMALAW;
+    double Phi_inner = -1.0 / r_inner;
+    double Phi_max = -1.0 / r_max;
+    double Rho_norm = pow(GammaFac * (Phi_inner - Phi_max + pow(r_inner, -sx) / sx - pow(1.0, -sx) / sx),
+                          1.0 / (GAMMALAW - 1.0));
+    srand(666);
+    for (int k = 0; k < s->NZ; ++k) {
+      double z = s->ip[OI_NglobalZ] != 1 ? 0.5 * (zpos(s, k - 1) + zpos(s, k)) : 0.0;
+      for (int i = 0; i < s->NR; ++i) {
+        double r = ann_r(s, i);
+        double R = sqrt(r * r + z * z);
+        double Phi = -M / R;
+        double omega = pow(r, -q);
+        double PoRho = GammaFac * (Phi_inner - Phi + pow(r_inner, -sx) / sx - pow(r, -sx) / sx);
+        if (PoRho < 1.0e-5) PoRho = 1.0e-5;
+        double rho = pow(PoRho, 1.0 / (GAMMALAW - 1.0)) / Rho_norm;
+        double Pp = pow(PoRho, 1.0 / GammaFac) / Rho_norm;
+        if (PoRho < 1.0e-5) Pp = rho;
+        for (int j = 0; j < s->N_p[i]; ++j) {
+          double delta = .02 * ((double)rand() / (double)RAND_MAX - .5);
+          int c = zid(s, k, i, j);
+          P(s, RHO, c) = rho; P(s, PPP, c) = Pp; P(s, URR, c) = 0.0;
+          P(s, UPP, c) = omega * (1. + delta) - pow(r, -2.);
+          P(s, UZZ, c) = 0.0;
+        }
+      }
+    }
+    return 0;
+  }
+  if (type == ID_MILOS) { /* cell_init_milos_macfadyen.c:37-152 */
+    double rs = 10.0, rho0 = 1.0;
+    double Mach = 1. / sqrt(s->dp[OD_PoRho_r1]);
+    double delta_exp = 3.0, xi_exp = 2.0;
+    double Gam = s->dp[OD_GAMMALAW];
+    double cp = 1. / Mach, fac = 0.0001, Mtotal = 1.0, sep = 1.0;
+    double massratio = s->dp[OD_MassRatio];
+    double M0, M1, r0, r1;
+    if (fabs(massratio) > 1.e-8) {
+      M0 = Mtotal / (1. + massratio);
+      M1 = Mtotal / (1. + 1. / massratio);
+      r0 = M1 / Mtotal * sep;
+      r1 = M0 / Mtotal * sep;
+    } else {
+      M0 = 1.0; M1 = 0.0; r0 = 0.0; r1 = 0.0;
+    }
+    double eps0 = s->dp[OD_G_EPS] * r0, eps1 = s->dp[OD_G_EPS] * r1;
+    double DISK_ALPHA = s->dp[OD_EXPLICIT_VISCOSITY];
+    for (int k = 0; k < s->NZ; ++k)
+      for (int i = 0; i < s->NR; ++i) {
+        double r = ann_r(s, i);
+        double cs = r < 1. ? 1. / Mach : sqrt(1. / r) / Mach;
+        for (int j = 0; j < s->N_p[i]; ++j) {
+          int c = zid(s, k, i, j);
+          double phi = s->tiph[c] - 0.5 * s->dphi[c];
+          double d0 = grav_dist(s, 0, r, phi, 0.), d1 = grav_dist(s, 1, r, phi, 0.);
+          double cs0 = d0 < 0.5 ? sqrt(1. / .5) / Mach * sqrt(M0) : sqrt(1. / d0) / Mach * sqrt(M0);
+          double cs1 = d1 < 0.5 ? sqrt(1. / .5) / Mach * sqrt(M1) : sqrt(1. / d1) / Mach * sqrt(M1);
+          double PoRho = (cs0 * cs0 + cs1 * cs1) / Gam;
+          int n = s->ip[OI_PHI_ORDER];
+          double Pot1 = M0 / pow(pow(d0, n) + pow(eps0, n), 1. / n);
+          double Pot2 = M1 / pow(pow(d1, n) + pow(eps1, n), 1. / n);
+          double rho = rho0 * (pow(rs / r, delta_exp)) * exp(-pow(rs / r, xi_exp));
+          double omega = sqrt(1.) / pow(r, 1.5);
+          omega *= 1. + 3. / 16. / r / r;
+          double O2 = omega * omega + cs * cs / r / r * (2. * rs * rs / r / r - 3.);
+          double vr;
+          if (r < 3.0) {
+            omega = 1. / pow(r, 1.5);
+            vr = 0.0;
+          } else {
+            omega = sqrt(O2);
+            vr = -3.0 / sqrt(r) * DISK_ALPHA * (1.0 / Mach) * (1.0 / Mach) * (1.0 - delta_exp + xi_exp * pow(r / rs, -xi_exp));
+          }
+          if (rho < (100. * s->dp[OD_RHO_FLOOR])) rho = 100. * s->dp[OD_RHO_FLOOR];
+          double Pp = PoRho * rho;
+          P(s, RHO, c) = rho * exp(fac * (Pot1 + Pot2) / cp / cp);
+          P(s, PPP, c) = Pp * exp(fac * (Pot1 + Pot2) / cp / cp);
+          P(s, URR, c) = vr;
+          P(s, UPP, c) = omega - sqrt(1.) / pow(r, 1.5);
+          P(s, UZZ, c) = 0.0;
+        }
+      }
+    return 0;
+  }
+  if (type < ID_SHEAR || type > ID_MIDDLE) return 17;
+  /* shear, vortex, rad_dom: the global loops evaluate what the single-cell
+   * functions do; SStest and middle also zero v_r for r < 1 (:81) */
+  for (int k = 0; k < s->NZ; ++k)
+    for (int i = 0; i < s->NR; ++i)
+      for (int j = 0; j < s->N_p[i]; ++j) {
+        int c = zid(s, k, i, j);
+        if (single_cell(s, i, c)) return 17;
+        if ((type == ID_SSTEST || type == ID_MIDDLE) && ann_r(s, i) < 1.) P(s, URR, c) = 0.0;
+      }
+  return 0;
+}
+
 /* ---------------- driver ---------------- */
 void oracle_init(OracleSim *s) { /* src/main.c:103-119 (IC plugin replaced by the caller's prim) */
   grav_init(s);
@@ -1391,12 +1661,12 @@ static int substep(OracleSim *s, double fac) { /* src/TimeStep/timestep_routines
   calc_prim(s);
   if (s->ip[OI_BoundTypeR] == BOUND_OUTFLOW)
     outflow_r(s);
-  else if (s->ip[OI_BoundTypeR] == BOUND_FIXED)
-    return 14; /* needs an initial-data plugin: not on this path */
-  if (s->ip[OI_NglobalZ] > 1 && s->ip[OI_BoundTypeZ] == BOUND_FIXED) return 15;
-  /* outflow_z (cell_boundary.c:141-206) only writes layers 0 and NZ-1, which
-   * the single-rank periodic z sync below overwrites: it has no effect. */
-  if (s->dp[OD_DAMP_TIME] > 0.0) return 16;
+  else if (s->ip[OI_BoundTypeR] == BOUND_FIXED && (e = fixed_r(s)))
+    return e;
+  /* outflow_z (cell_boundary.c:141-206) and fixed_z only write the ghost
+   * layers, which the single-rank periodic z sync below overwrites. */
+  if (s->ip[OI_NglobalZ] > 1 && s->ip[OI_BoundTypeZ] == BOUND_FIXED && (e = fixed_z(s))) return e;
+  if (s->dp[OD_DAMP_TIME] > 0.0 && (e = bc_damp(s, dt))) return e;
   sync_z(s);
   calc_cons(s);
   return 0;

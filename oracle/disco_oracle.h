@@ -27,7 +27,8 @@ enum {
 enum {
   OD_GAMMALAW, OD_CFL, OD_PLM, OD_EXPLICIT_VISCOSITY, OD_G_EPS, OD_RHO_FLOOR,
   OD_CS_FLOOR, OD_CS_CAP, OD_VEL_CAP, OD_MassRatio, OD_OrbShrinkTscale,
-  OD_OrbShrinkT0, OD_T_MAX, OD_DAMP_TIME, OD_PoRho_r1, OD_COUNT
+  OD_OrbShrinkT0, OD_T_MAX, OD_DAMP_TIME, OD_PoRho_r1, OD_RDAMP_INNER, OD_RDAMP_OUTER,
+  OD_RMIN, OD_RMAX, OD_COUNT
 };
 
 typedef struct OracleSim OracleSim;
@@ -40,6 +41,9 @@ int oracle_nzones(OracleSim *s);
 void oracle_set_tiph(OracleSim *s, const double *tiph);
 void oracle_set_dphi(OracleSim *s, const double *dphi);
 void oracle_set_prim(OracleSim *s, const double *prim);
+/* the InitialDataType plugin (src/main.c:103-104, after gravMass init) over
+ * the current cells; 0, or 17 for an unknown type */
+int oracle_initial_data(OracleSim *s);
 /* gravMass init + clean_pi + z-periodic sync + calc_cons (src/main.c:103-119) */
 void oracle_init(OracleSim *s);
 /* one timestep_rk2; returns 0, or a nonzero error code */

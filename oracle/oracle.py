@@ -23,7 +23,7 @@ I_SLOTS = ["InitialDataType", "GravMassType", "BoundTypeR", "BoundTypeZ", "NoInn
            "RhoSinkOn", "PHI_ORDER", "Cooling", "NumZ", None, None, None, None, "NUM_C"]
 D_SLOTS = ["Adiabatic_Index", "CFL", "PLM", "Explicit_Viscosity", "G_EPS", "Rho_Floor",
            "Cs_Floor", "Cs_Cap", "Vel_Cap", "MassRatio", "OrbShrinkTscale", "OrbShrinkT0",
-           "Time_Max", "DAMP_TIME", "PoRho_r1"]
+           "Time_Max", "DAMP_TIME", "PoRho_r1", "RDAMP_INNER", "RDAMP_OUTER", "R_Min", "R_Max"]
 
 _lib = None
 
@@ -51,6 +51,8 @@ def lib():
             getattr(L, name).restype = None
         L.oracle_nzones.argtypes = [vp]
         L.oracle_nzones.restype = ctypes.c_int
+        L.oracle_initial_data.argtypes = [vp]
+        L.oracle_initial_data.restype = ctypes.c_int
         L.oracle_rk2.argtypes = [vp]
         L.oracle_rk2.restype = ctypes.c_int
         L.oracle_run.argtypes = [vp, ctypes.c_int]
@@ -125,6 +127,12 @@ class Oracle:
             L.oracle_set_dphi(self.h, _dp(np.ascontiguousarray(dphi, dtype=np.float64)))
         if prim is not None:
             L.oracle_set_prim(self.h, _dp(np.ascontiguousarray(prim, dtype=np.float64)))
+
+    def initial_data(self):
+        """the InitialDataType plugin over the current cells (before init())"""
+        e = lib().oracle_initial_data(self.h)
+        if e:
+            raise RuntimeError(f"oracle error {e}")
 
     def init(self):
         lib().oracle_init(self.h)
@@ -263,8 +271,11 @@ def read_records(path) -> dict:
     return d
 
 
-def write_icfile(path, NR, NZ, NQ, N_p=None, tiph=None, prim=None):
+def write_icfile(path, NR, NZ, NQ, N_p=None, tiph=None, prim=None, plugin=False):
+    """plugin: the reference then runs its InitialDataType plugin over the
+    cells (after any tiph / prim given here)"""
     flags = (1 if N_p is not None else 0) | (2 if tiph is not None else 0) | (4 if prim is not None else 0)
+    flags |= 8 if plugin else 0
     with open(path, "wb") as f:
         f.write(struct.pack("<5i", 0x44534331, NR, NZ, NQ, flags))
         if N_p is not None:

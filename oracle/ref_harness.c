@@ -9,7 +9,8 @@
  *
  * It replays the set-up order of the reference driver (src/main.c:60-119),
  * replacing the initial-data plugin (src/main.c:103-104) by an initial
- * state read from a file, and optionally overriding the per-annulus N_p
+ * state read from a file (or running the plugin itself after it, flag 8),
+ * and optionally overriding the per-annulus N_p
  * array set by sim_set_N_p (src/Sim/sim_set.c:10-33).  It then calls
  * timestep_rk2 (src/TimeStep/timestep_algorithm.c:11-41) step by step and
  * dumps the cell state through the private struct layouts.
@@ -27,7 +28,7 @@
  * icfile layout (little endian):
  *   int32 magic 0x44534331, int32 NR (annuli incl. ghosts), int32 NZ
  *   (layers incl. ghosts), int32 NQ, int32 flags (1: N_p given, 2: tiph
- *   given, 4: prim given)
+ *   given, 4: prim given, 8: then run the InitialDataType plugin)
  *   [int32 N_p[NR]] [double tiph[Nzones]] [double prim[NQ][Nzones]]
  * zones are ordered k (layer), i (annulus), j (cell), as in the reference's
  * cell_create (src/Cell/cell_create_destroy.c:41-48).
@@ -245,6 +246,7 @@ int main(int argc, char **argv) {
     }
     free(buf);
     fclose(ic);
+    if (hdr[4] & 8) (*cell_init_ptr(sim))(cells, sim, gm, mpi);
   }
   /* src/main.c:106-119 */
   gravMass_clean_pi(gm, sim);

@@ -4,7 +4,8 @@
  *
  *   driver <parfile> <prim_in> <nsteps> <state_out>
  *
- * prim_in: NUM_Q * Nzones doubles (q-major, the reference's cell order);
+ * prim_in: NUM_Q * Nzones doubles (q-major, the reference's cell order),
+ * or "plugin" for the parfile's InitialDataType plugin (src/main.c:103-104);
  * with Restart 1 in the parfile the state comes from input.h5 instead
  * (src/main.c:95-100) and prim_in is not read.  Checkpoints
  * (checkpoint_NNNN.h5) and diagnostics files are written into the working
@@ -13,6 +14,7 @@
  * Exit status: 0 ok, 2 usage/io, 3 library error (message on stderr). */
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "disco.h"
 
@@ -63,6 +65,11 @@ int main(int argc, char **argv) {
     io_readbuf(io, cells, sim, gm);
     io_deallocbuf(io);
     if (disco_last_error()) return fail("restart");
+  } else if (strcmp(argv[2], "plugin") == 0) {
+    (*gravMass_init_ptr(sim))(gm, sim);
+    void (*init)(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *) = cell_init_ptr(sim);
+    if (!init) return fail("cell_init_ptr");
+    (*init)(cells, sim, gm, mpi);
   } else {
     f = fopen(argv[2], "rb");
     if (!f || fread(buf, sizeof(double), (size_t)nq * nz, f) != (size_t)nq * nz) {

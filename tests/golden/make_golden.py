@@ -6,7 +6,8 @@ been built (``make -C oracle ref``):
     python tests/golden/make_golden.py [case ...]
 
 For every case it writes the parfile, asks the reference for its mesh,
-builds the seeded initial state (disco2_amd.ics), runs the reference's
+builds the seeded initial state (disco2_amd.ics) or lets the reference's
+own initial-data plugin set it (the plugin cases), runs the reference's
 unit pieces (faces, PLM slopes, the first substep) and a few RK2 steps, and
 stores inputs and outputs as a compressed .npz.  The fixtures are data only.
 """
@@ -58,6 +59,22 @@ def cases():
     p, k, kw = configs.disk3d(8, 4, 24, R_Min=0.5, R_Max=2.5, Time_Max=10.0, Num_Diag_Measure=100,
                               Num_Diag_Dump=40)
     c["diag_disk3d_r8z4"] = ((p, k, kw), dict(diag=8))
+    # the reference's own initial-data plugins (src/Cell/initial_data), with
+    # fixed r boundaries and boundary damping; prim0 is the plugin's output
+    c["plugin_shear_fixed"] = (configs.plugin("shear", 16, NPCAP=64), dict(steps=8, dumps=[1, 8], unit=True, plugin=True))
+    c["plugin_vortex_fixed"] = (configs.plugin("vortex", 16, R_Max=0.9),
+                                dict(steps=8, dumps=[1, 8], unit=True, plugin=True))
+    c["plugin_sstest_fixed_damp"] = (configs.plugin("sstest", 16), dict(steps=8, dumps=[1, 8], unit=True, plugin=True))
+    c["plugin_middle_outflow_damp"] = (configs.plugin("middle", 16),
+                                       dict(steps=8, dumps=[1, 8], unit=True, plugin=True))
+    c["plugin_milos_fixed_damp"] = (configs.plugin("milos", 16), dict(steps=8, dumps=[1, 8], unit=True, plugin=True))
+    c["plugin_rad_dom_fixed"] = (configs.plugin("rad_dom", 16), dict(steps=8, dumps=[1, 8], unit=True, plugin=True))
+    c["plugin_torus3d"] = (configs.plugin("torus", 10, NPCAP=32), dict(steps=6, dumps=[1, 6], unit=True, plugin=True))
+    # (no viscosity: the reference exits once v_z != 0 meets viscosity)
+    c["plugin_middle3d_fixed_rz"] = (configs.plugin("middle", 8, R_Min=1.0, R_Max=4.0, NumZ=4, Z_Min=-0.2, Z_Max=0.2,
+                                                    NPCAP=48, Grav_2D=0, BoundTypeR=0, BoundTypeZ=0, DAMP_TIME=-1.0,
+                                                    Explicit_Viscosity=-1.0),
+                                     dict(steps=6, dumps=[1, 6], unit=True, plugin=True))
     return c
 
 
@@ -76,9 +93,16 @@ def make_case(name, spec, opts, tmp):
     g = orc.read_records(_run(["geom", par, icg, os.path.join(tmp, "g.bin")], tmp, "g.bin"))
     geom = {"N_p": N_p, "r_faces": g["r_faces"], "z_faces": g["z_faces"],
             "tiph": g["tiph0"], "dphi": g["dphi0"]}
-    prim0 = ics.make_ic(ic_kind, geom, params, **ic_kw)
     ic = os.path.join(tmp, "ic.bin")
-    orc.write_icfile(ic, NR, NZ, NQ, N_p=N_p, prim=prim0)
+    if opts.get("plugin"):
+        # the reference's InitialDataType plugin sets the cells; its output
+        # (the state before the first step) is the fixture's prim0
+        orc.write_icfile(ic, NR, NZ, NQ, N_p=N_p, plugin=True)
+        r0 = orc.read_records(_run(["run", par, ic, os.path.join(tmp, "r0.bin"), "0"], tmp, "r0.bin"))
+        prim0 = r0["step0/prim"].reshape(NQ, -1)
+    else:
+        prim0 = ics.make_ic(ic_kind, geom, params, **ic_kw)
+        orc.write_icfile(ic, NR, NZ, NQ, N_p=N_p, prim=prim0)
     out = {"params": np.array(json.dumps(params)), "N_p": N_p, "N_p_ref_formula": g0["N_p"],
            "r_faces": g["r_faces"], "z_faces": g["z_faces"], "tiph0": g["tiph0"], "dphi0": g["dphi0"],
            "prim0": prim0.ravel(), "dims": g["dims"]}
@@ -107,6 +131,9 @@ def make_case(name, spec, opts, tmp):
                 out["unit/" + k] = v
     args = ["run", par, ic, os.path.join(tmp, "r.bin"), str(opts["steps"])] + [str(d) for d in opts["dumps"]]
     r = orc.read_records(_run(args, tmp, "r.bin"))
+    missing = [d for d in opts["dumps"] if f"step{d}/prim" not in r]
+    if missing or "dts" not in r:  # the reference exits with status 0 on some errors
+        raise RuntimeError(f"{name}: the reference stopped before steps {missing}")
     for k, v in r.items():
         if k.startswith("step0/") or k in ("r_faces", "z_faces", "N_p", "dims"):
             continue
