@@ -4,9 +4,10 @@
 
 Every object is compiled with -ffp-contract=off, so neither the host code
 nor the device kernels contract a*b+c into an FMA: results then follow the
-reference's IEEE operation order.  The initial-data plugins are compiled by
-g++, the reference's compiler family, so that the C library calls they make
-(pow, exp, ...) are folded or called as in the reference build.
+reference's IEEE operation order.  The initial-data plugins and the analytic
+frame rotation are compiled by g++, the reference's compiler family, so that
+the C library calls they make (pow, exp, ...) are folded or called as in the
+reference build.
 """
 from __future__ import annotations
 
@@ -28,8 +29,8 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-p
           "-munsafe-fp-atomics"]
 CXX = os.environ.get("DISCO_CXX", "g++")
 SOURCES = ["kernels.hip", "adjoint.hip", "diagnostics.hip", "checkpoint.hip", "engine.cpp", "adjoint_host.cpp",
-           "diagnostics.cpp", "checkpoint.cpp", "h5.cpp", "capi.cpp", "sim_host.cpp", "initial_data.cpp"]
-GCC_SOURCES = {"initial_data.cpp"}
+           "diagnostics.cpp", "checkpoint.cpp", "h5.cpp", "capi.cpp", "sim_host.cpp", "initial_data.cpp", "frame.cpp"]
+GCC_SOURCES = {"initial_data.cpp", "frame.cpp"}
 
 
 def _compile(src: str) -> str:

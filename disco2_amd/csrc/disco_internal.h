@@ -131,9 +131,12 @@ struct Face {  // host copy of one face list (unit-level access)
 // error reporting shared by all translation units
 void disco_set_error(int code, const std::string &msg);
 
-// host geometry (sim_host.cpp)
+// host geometry (sim_host.cpp) and frame rotation (frame.cpp)
 void sim_param_arrays(const Sim *s, DiskParams *dp);
 double disco_rOm_a(int type, double r, double a);
 double disco_rdrOm_a(int type, double r, double a);
 double disco_dtOm_a(int type, double r, double a, double tscale);
+// per annulus (k-major, as the annulus table), at separation a: rOm_a,
+// rdrOm_a, dtOm_a at the centre and rdrOm_a at the outer face (frame.cpp)
+void disco_frame_rows(const Sim *S, double a, std::vector<double> &out);
 void disco_cell_offsets(const Sim *s, std::vector<double> &tiph, std::vector<double> &dphi);

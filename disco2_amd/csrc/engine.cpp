@@ -281,6 +281,18 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   }
   plan.n_rpairs = n_rpairs;
   plan.n_zpairs = n_zpairs;
+  // frames that do not depend on the binary separation: tabulated on the
+  // host once (k_scalar recomputes the A_MILOS frame when the masses move)
+  for (int d = 0; d < n; ++d) {
+    const DiskParams &P = hdp[d];
+    if (P.w_analytic == A_FIXED || P.w_analytic == A_MILOS) continue;
+    static_assert(sizeof(double4) == 4 * sizeof(double), "double4 rows");
+    std::vector<double> rows;
+    disco_frame_rows(sims[d], 1.0, rows);
+    if (!hip_check(hipMemcpy(d_ann_rot + P.ann_base, rows.data(), rows.size() * sizeof(double),
+                             hipMemcpyHostToDevice), "frame rows"))
+      return;
+  }
   // initial mesh edges: cell_create + cell_clean_pi + tiph sync (src/main.c:77-85)
   for (int d = 0; d < n; ++d) {
     std::vector<double> t, dp;

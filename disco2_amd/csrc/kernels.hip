@@ -1222,9 +1222,10 @@ __global__ __launch_bounds__(SCALAR_THREADS) void k_scalar(DevMesh m, int ndisk,
   DiskScalars &s = m.sc[d];
   if (ops & (SC_DT_FOLD | SC_STEP_BEGIN)) fold_dt(m, P, s);
   if (threadIdx.x == 0) scalar_ops(P, s, ops);
-  // the analytic frame rotation (src/Sim/sim_w_a.c) of every annulus, at the
-  // mass separation a the next kernels will see: after any mass move
-  if ((ops & (SC_SUB2_END | SC_EULER_END | SC_SUB_BEGIN | SC_MASS)) && P.w_analytic != A_FIXED) {
+  // the A_MILOS frame rotation (src/Sim/sim_w_a.c) of every annulus, at the
+  // mass separation a the next kernels will see: after any mass move (the
+  // other frames do not depend on a: the host tabulates them once)
+  if ((ops & (SC_SUB2_END | SC_EULER_END | SC_SUB_BEGIN | SC_MASS)) && P.w_analytic == A_MILOS) {
     __syncthreads();
     const double a = s.gm[0].r + s.gm[1].r;
     const int na = P.NR * P.NZ;

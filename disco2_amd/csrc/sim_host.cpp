@@ -332,48 +332,7 @@ void sim_set_misc(Sim *s, MPIsetup *) {
 }
 }
 
-// analytic frame rotation, src/Sim/sim_w_a.c:8-73 (host copies)
-#define W_A_MILOS_INDEX 8
-double disco_rOm_a(int type, double r, double a) {
-  switch (type) {
-    case A_FIXED: return 0.0;
-    case A_KEPLER: return pow(r, -0.5);
-    case A_OMEGA20: return r * 20;
-    case A_MILOS:
-      return r * (pow(r, W_A_MILOS_INDEX - 1.5) + pow(a, W_A_MILOS_INDEX - 1.5)) /
-             (pow(r, W_A_MILOS_INDEX) + pow(a, W_A_MILOS_INDEX));
-    case A_POWERLAW: return pow(r, -1.0);
-  }
-  disco_set_error(5, "unknown w_analytic_type");
-  return 0.0;
-}
-double disco_rdrOm_a(int type, double r, double a) {
-  switch (type) {
-    case A_FIXED: return 0.0;
-    case A_KEPLER: return -1.5 * pow(r, -1.5);
-    case A_OMEGA20: return 0.0;
-    case A_MILOS: {
-      int n = W_A_MILOS_INDEX;
-      double Omega_a = (pow(r, n - 1.5) + pow(a, n - 1.5)) / (pow(r, n) + pow(a, n));
-      double arn = pow(a / r, n);
-      return (-1.5 + n * arn / (1. + arn)) * Omega_a + (1.5 - n) * pow(a / r, n - 1.5) / (1. + arn) * pow(r, -1.5);
-    }
-    case A_POWERLAW: return -2. * pow(r, -2.0);
-  }
-  disco_set_error(5, "unknown w_analytic_type");
-  return 0.0;
-}
-double disco_dtOm_a(int type, double r, double a, double tau) {
-  if (type == A_MILOS) {
-    double a0 = 1.0;
-    double dta_o_a = -.25 * pow(a / a0, -4) / tau;
-    int n = W_A_MILOS_INDEX;
-    double Omega_a = (pow(r, n - 1.5) + pow(a, n - 1.5)) / (pow(r, n) + pow(a, n));
-    double arn = pow(a / r, n);
-    if (a > 0.0) return dta_o_a * (Omega_a * (-1.5 + n / (1. + arn)) + (1.5 - n) * pow(r, -1.5) / (1. + arn));
-  }
-  return 0.0;
-}
+// analytic frame rotation: disco_rOm_a and friends (frame.cpp)
 extern "C" {
 double sim_rOm_a(Sim *s, double r, double a) { return disco_rOm_a(s->W_ANALYTIC_TYPE, r, a); }
 double sim_rdrOm_a(Sim *s, double r, double a) { return disco_rdrOm_a(s->W_ANALYTIC_TYPE, r, a); }

@@ -9,6 +9,8 @@ reference reader.  Two extra keys understood only by this library
 """
 from __future__ import annotations
 
+import re
+
 # (key, type) in the reference's read order, src/Sim/sim_readpar.c:55-106
 PARAM_KEYS = [
     ("Restart", int), ("InitialDataType", int), ("GravMassType", int),
@@ -91,3 +93,30 @@ def parfile_text(params: dict) -> str:
 def write_parfile(path, params: dict) -> None:
     with open(path, "w") as f:
         f.write(parfile_text(params))
+
+
+_NUMBER = re.compile(r"\s*([+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?)")
+
+
+def read_parfile(path) -> dict:
+    """The parameters of a parfile, read with the reference's rule (readvar,
+    src/Sim/sim_readpar.c:10-46): the first line whose first word is the key;
+    the value follows after any of the characters "\\t :=>_"; integers are
+    the truncated double.  A missing key raises KeyError (the reference's
+    reader reports it and fails)."""
+    with open(path) as f:
+        lines = f.readlines()
+    out = {}
+    for key, typ in list(PARAM_KEYS) + list(EXTRA_KEYS.items()):
+        line = next((s for s in lines if s.split() and s.split()[0] == key), None)
+        if line is None:
+            if key in EXTRA_KEYS:
+                continue
+            raise KeyError(f"cant find {key}")
+        rest = line[len(key):].lstrip("\t :=>_")
+        m = _NUMBER.match(rest)
+        if not m:
+            raise ValueError(f"{key}: no number in {line.strip()!r}")
+        v = float(m.group(1))
+        out[key] = int(v) if typ is int else v
+    return out

@@ -13,6 +13,7 @@ stores inputs and outputs as a compressed .npz.  The fixtures are data only.
 """
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 import sys
@@ -25,10 +26,11 @@ sys.path.insert(0, ROOT)
 
 from disco2_amd import configs, ics  # noqa: E402
 from disco2_amd.geometry import n_p_round  # noqa: E402
-from disco2_amd.params import write_parfile  # noqa: E402
+from disco2_amd.params import read_parfile, write_parfile  # noqa: E402
 from oracle import oracle as orc  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+REF_PARFILES = "/root/reference/parfiles"
 
 
 def cases():
@@ -75,10 +77,49 @@ def cases():
                                                     NPCAP=48, Grav_2D=0, BoundTypeR=0, BoundTypeZ=0, DAMP_TIME=-1.0,
                                                     Explicit_Viscosity=-1.0),
                                      dict(steps=6, dumps=[1, 6], unit=True, plugin=True))
+    # the reference's shipped parfiles (parfiles/), run as they are with their
+    # own plugins; large meshes, so only digests of the states are kept
+    for nm, pf, steps in (("code_shear", "code_tests/shear.par", 4), ("code_vortex", "code_tests/vortex.par", 10),
+                          ("code_SStest", "code_tests/SStest.par", 40), ("code_torus", "torus.par", 3)):
+        c[nm] = ((os.path.join(REF_PARFILES, pf), "plugin", {}), dict(steps=steps, digest=True))
     return c
 
 
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def make_digest_case(name, parfile, opts, tmp):
+    """the reference on one of its own parfiles: SHA-256 of the state arrays
+    at the start and after the steps, t, dt and the masses"""
+    params = read_parfile(parfile)
+    par = os.path.join(tmp, name + ".par")
+    write_parfile(par, params)
+    g = orc.read_records(_run(["geom", par, "-", os.path.join(tmp, "g.bin")], tmp, "g.bin"))
+    NR, NZ, NQ = (int(x) for x in g["dims"][:3])
+    ic = os.path.join(tmp, "ic.bin")
+    orc.write_icfile(ic, NR, NZ, NQ, plugin=True)
+    steps = opts["steps"]
+    r = orc.read_records(_run(["run", par, ic, os.path.join(tmp, "r.bin"), str(steps), str(steps)], tmp, "r.bin"))
+    if f"step{steps}/prim" not in r:
+        raise RuntimeError(f"{name}: the reference stopped early")
+    out = {"params": np.array(json.dumps(params)), "N_p": g["N_p"], "r_faces": g["r_faces"], "z_faces": g["z_faces"],
+           "dims": g["dims"], "parfile": np.array(os.path.relpath(parfile, os.path.dirname(REF_PARFILES))),
+           "digest/tiph0": np.array(sha(g["tiph0"])), "digest/dphi0": np.array(sha(g["dphi0"]))}
+    for st in (0, steps):
+        for f in ("prim", "cons", "tiph", "dphi", "RKtiph"):
+            out[f"digest/step{st}/{f}"] = np.array(sha(r[f"step{st}/{f}"]))
+        out[f"digest/step{st}/t_dt"] = r[f"step{st}/t_dt"]
+        out[f"digest/step{st}/gravmass"] = r[f"step{st}/gravmass"]
+    out["digest/steps"] = np.array([steps], dtype=np.int32)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
 def make_case(name, spec, opts, tmp):
+    if opts.get("digest"):
+        return make_digest_case(name, spec[0], opts, tmp)
     params, ic_kind, ic_kw = spec
     par = os.path.join(tmp, name + ".par")
     write_parfile(par, params)
