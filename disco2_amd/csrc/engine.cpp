@@ -527,6 +527,7 @@ void Engine::run(int nsteps, bool use_graph) {
       gexec = nullptr;
     }
     int cur0 = cur;
+    if (w_static) enqueue_set_w();  // a mesh velocity of r alone: set once, before the capture, not in the graph
     if (!hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture")) return;
     enqueue_rk2();
     if (!hip_check(hipStreamEndCapture(st, &graph), "capture end")) return;

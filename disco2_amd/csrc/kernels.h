@@ -44,7 +44,7 @@ enum { FACE_II = 0, FACE_IO = 1, FACE_OO = 2, FACE_OI = 3 };
 
 // k_scalar runs one workgroup of SCALAR_THREADS per disk: all of them reduce
 // the disk's tile minima (SC_DT_FOLD), thread 0 does the scalar updates.
-#define SCALAR_THREADS 256
+#define SCALAR_THREADS 1024
 
 struct LaunchPlan {
   int ndisk, nann;

@@ -1091,9 +1091,17 @@ __global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
 __device__ __forceinline__ void fold_dt(const DevMesh &m, const DiskParams &P, DiskScalars &s) {
   __shared__ double s_m[SCALAR_THREADS / 64];
   double v = 1.e100;
-  for (int it = P.itile0 + (int)threadIdx.x; it < P.itile1; it += SCALAR_THREADS) {
-    const double b = m.dt_tile[it];
-    if (b > 0.0 && b < v) v = b;
+  // four independent loads in flight per thread
+  for (int it = P.itile0 + (int)threadIdx.x; it < P.itile1; it += 4 * SCALAR_THREADS) {
+    double b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int iu = it + u * SCALAR_THREADS;
+      b[u] = iu < P.itile1 ? m.dt_tile[iu] : 1.e100;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (b[u] > 0.0 && b[u] < v) v = b[u];
   }
   for (int off = 32; off > 0; off >>= 1) {
     double o = __shfl_down(v, off);
@@ -1117,7 +1125,7 @@ __device__ __forceinline__ void fold_dt(const DevMesh &m, const DiskParams &P, D
 // src/TimeStep/timestep_algorithm.c:11-41, src/TimeStep/timestep_set.c:16-21,
 // src/GravMass/GravMass_misc.c:9-77.
 // ----------------------------------------------------------------------------
-__device__ void d_gm_move(const DiskParams &P, DiskScalars &s, double t, double dt) {
+__device__ __forceinline__ void d_gm_move(const DiskParams &P, DiskScalars &s, double t, double dt) {
   double m0 = s.gm[0].M, m1 = s.gm[1].M;
   double M = m0 + m1;
   double a_0 = 1.0, tau_0 = P.orb_tscale, T0 = P.orb_t0;
@@ -1147,7 +1155,7 @@ __device__ void d_gm_move(const DiskParams &P, DiskScalars &s, double t, double 
 
 // launch-uniform gravity terms of the current mass positions (computed
 // with the same device functions the per-cell code would call)
-__device__ void d_mass_terms(const DiskParams &P, DiskScalars &s) {
+__device__ __forceinline__ void d_mass_terms(const DiskParams &P, DiskScalars &s) {
   for (int p = 0; p < 2; ++p) {
     s.xbh[p] = s.gm[p].r * cos(s.gm[p].phi);
     s.ybh[p] = s.gm[p].r * sin(s.gm[p].phi);
@@ -1155,7 +1163,7 @@ __device__ void d_mass_terms(const DiskParams &P, DiskScalars &s) {
   s.epsn = pow(P.g_eps, P.phi_order);
 }
 
-__device__ void scalar_ops(const DiskParams &P, DiskScalars &s, int ops) {
+__device__ __forceinline__ void scalar_ops(const DiskParams &P, DiskScalars &s, int ops) {
   if (ops & SC_STEP_BEGIN) {  // timestep_set_dt, gravMass_copy, RK = 0
     double dt = __longlong_as_double((long long)s.dtmin_bits);
     if (s.t + dt > P.t_max) dt = P.t_max - s.t;
@@ -1221,7 +1229,14 @@ __global__ __launch_bounds__(SCALAR_THREADS) void k_scalar(DevMesh m, int ndisk,
   const DiskParams &P = m.dp[d];
   DiskScalars &s = m.sc[d];
   if (ops & (SC_DT_FOLD | SC_STEP_BEGIN)) fold_dt(m, P, s);
-  if (threadIdx.x == 0) scalar_ops(P, s, ops);
+  if (threadIdx.x == 0) {
+    // on register copies: the operations' dependent chains would otherwise
+    // go through global memory (the two structs may alias)
+    const DiskParams Pl = P;
+    DiskScalars sl = s;
+    scalar_ops(Pl, sl, ops);
+    s = sl;
+  }
   // the A_MILOS frame rotation (src/Sim/sim_w_a.c) of every annulus, at the
   // mass separation a the next kernels will see: after any mass move (the
   // other frames do not depend on a: the host tabulates them once)
