@@ -406,45 +406,77 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
   const int pLf = i <= P.NR - 2 ? rpair_of(P, k, i) : -1;
   const bool visc = a.visc;
   const double tc = clean_pi(m.tiph[z]), dc = m.dphi[z];
+  // the cell's faces in the walk order of k_plm_rz: as R, its runs in the
+  // pair below (at most two, the second is the wrap), then as L one range
+  // in the pair above; range u is faces [fb[u] + b[u], fb[u] + e[u])
+  long long fb[3] = {0, 0, 0};
+  int b[3] = {0, 0, 0}, e[3] = {0, 0, 0};
+  double pr[3] = {0, 0, 0}, pp[3] = {0, 0, 0}, del[3] = {0, 0, 0};
+  if (pR >= 0) {
+    const PairInfo PI = m.pairs[pR];
+    const PairState S = m.pst[pR];
+    int t = j - S.jpmin;
+    if (t < 0) t += A.N;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (t <= S.T) {
+        fb[u] = PI.face_base;
+        b[u] = m.run_start[PI.run_base + t];
+        e[u] = m.run_start[PI.run_base + t + 1];
+        pr[u] = PI.r;
+        pp[u] = PI.deltaPerp;
+        del[u] = PI.deltaR;
+      }
+      t += A.N;
+    }
+  }
+  if (pLf >= 0) {
+    const PairInfo PI = m.pairs[pLf];
+    const PairState S = m.pst[pLf];
+    int f1 = (j + 1 < A.N) ? m.fo_r[z + 1] : S.count;
+    if (f1 > S.count) f1 = S.count;
+    int f0 = m.fo_r[z];
+    if (f0 < 0) f0 = 0;
+    fb[2] = PI.face_base;
+    b[2] = f0;
+    e[2] = f1;
+    pr[2] = PI.r;
+    pp[2] = PI.deltaPerp;
+    del[2] = PI.deltaL;
+  }
   double ap[5] = {0, 0, 0, 0, 0}, ag[5] = {0, 0, 0, 0, 0}, agp[5] = {0, 0, 0, 0, 0}, rot = 0.0;
   double g[5] = {0, 0, 0, 0, 0};
-  // the slope sum (faces as R, then as L, as k_plm_rz) and the gather
-  auto acc = [&](const PairInfo &PI, long long gi) {
-    const double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
-    for (int q = 0; q < 5; ++q) g[q] += m.Fface[q * NF + gi] * dA;
-  };
-  // as R: primR = p - g deltaR - gp dpR (Riemann part), W_R = p - dpR gp (viscous)
-  if (pR >= 0)
-    faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) {
-      acc(PI, gi);
-      const double dpR = wrap_pm(-m.fcm[gi] + (tc - .5 * dc));
+  // the slope sum and the gather.  As R: primR = p - g deltaR - gp dpR
+  // (Riemann part), W_R = p - dpR gp (viscous); as L: primL = p + g deltaL
+  // + gp dpL, W_L = p + dpL gp
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    const bool asL = u == 2;
+    for (int f = b[u]; f < e[u]; ++f) {
+      const long long gi = fb[u] + f;
+      const double dA = pr[u] * m.fdphi[gi] * pp[u];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) g[q] += m.Fface[q * NF + gi] * dA;
+      const double dpo = asL ? wrap_pm(m.fcm[gi] - (tc - .5 * dc)) : wrap_pm(-m.fcm[gi] + (tc - .5 * dc));
+      const int hs = asL ? AFR_HL : AFR_HR, ws = asL ? AFR_WL : AFR_WR;
+#pragma unroll
       for (int q = 0; q < 5; ++q) {
-        const double h = a.fr[(AFR_HR + q) * NF + gi];
-        const double s = visc ? h + a.fr[(AFR_WR + q) * NF + gi] : h;
-        ap[q] += s;
-        ag[q] -= PI.deltaR * h;
-        agp[q] -= dpR * s;
+        const double h = a.fr[(hs + q) * NF + gi];
+        const double sv = visc ? h + a.fr[(ws + q) * NF + gi] : h;
+        ap[q] += sv;
+        if (asL) {
+          ag[q] += del[u] * h;
+          agp[q] += dpo * sv;
+        } else {
+          ag[q] -= del[u] * h;
+          agp[q] -= dpo * sv;
+        }
       }
       if (visc)
         for (int kk = 0; kk < 2; ++kk) agp[URR + kk] += a.fr[(AFR_G + kk) * NF + gi];
-      rot += a.fr[(AFR_ROT + 1) * NF + gi];
-    });
-  // as L: primL = p + g deltaL + gp dpL, W_L = p + dpL gp
-  if (pLf >= 0)
-    faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) {
-      acc(PI, gi);
-      const double dpL = wrap_pm(m.fcm[gi] - (tc - .5 * dc));
-      for (int q = 0; q < 5; ++q) {
-        const double h = a.fr[(AFR_HL + q) * NF + gi];
-        const double s = visc ? h + a.fr[(AFR_WL + q) * NF + gi] : h;
-        ap[q] += s;
-        ag[q] += PI.deltaL * h;
-        agp[q] += dpL * s;
-      }
-      if (visc)
-        for (int kk = 0; kk < 2; ++kk) agp[URR + kk] += a.fr[(AFR_G + kk) * NF + gi];
-      rot += a.fr[AFR_ROT * NF + gi];
-    });
+      rot += a.fr[(asL ? AFR_ROT : AFR_ROT + 1) * NF + gi];
+    }
+  }
   // the r slope's adjoint: its r faces' plus (viscous) its phi faces' (agrad)
   double gb[5];
   for (int q = 0; q < 5; ++q) {
@@ -464,37 +496,51 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
   else
     dAtot = dz * (A.rp + A.rm) * dp;
   for (int q = 0; q < 5; ++q) g[q] /= dAtot;
+  // replay of the sequential limiter: which face's S it took last
   const double PLM = P.plm;
   int src[5] = {-1, -1, -1, -1, -1};
   int n = 0;
-  auto lim = [&](const PairInfo &PI, long long gi) {
-    for (int q = 0; q < 5; ++q) {
-      const double S = m.Fface[q * NF + gi];
-      if (S * g[q] < 0.0) {
-        g[q] = 0.0;
-        src[q] = -2;
-      } else if (fabs(PLM * S) < fabs(g[q])) {
-        g[q] = PLM * S;
-        src[q] = n;
+#pragma unroll
+  for (int u = 0; u < 3; ++u)
+    for (int f = b[u]; f < e[u]; ++f) {
+      double S[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) S[q] = m.Fface[q * NF + fb[u] + f];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        if (S[q] * g[q] < 0.0) {
+          g[q] = 0.0;
+          src[q] = -2;
+        } else if (fabs(PLM * S[q]) < fabs(g[q])) {
+          g[q] = PLM * S[q];
+          src[q] = n;
+        }
       }
+      ++n;
     }
-    ++n;
-  };
-  if (pR >= 0) faces_as_R(m, pR, j, A.N, lim);
-  if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, lim);
+  // each face's share of the slope's adjoint: dA/dAtot to every face when the
+  // sum survived, PLM to the face the limiter took last, nothing when zeroed;
+  // and the adjoint of the face's width (AFR_WB from the L cell, + 1 from R)
   n = 0;
-  auto put = [&](const PairInfo &PI, long long gi, double *dst, int side) {
-    const double dA = PI.r * m.fdphi[gi] * PI.deltaPerp;
-    double wb = 0.0;
-    for (int q = 0; q < 5; ++q) {
-      dst[q * NF + gi] = src[q] == -1 ? dA / dAtot * gb[q] : (src[q] == n ? PLM * gb[q] : 0.0);
-      if (src[q] == -1) wb += gb[q] * m.Fface[q * NF + gi] / dAtot;  // d g / d dA_f
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    double *dst = u == 2 ? a.aSL : a.aSR;
+    for (int f = b[u]; f < e[u]; ++f) {
+      const long long gi = fb[u] + f;
+      const double dA = pr[u] * m.fdphi[gi] * pp[u];
+      double S[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) S[q] = m.Fface[q * NF + gi];
+      double wb = 0.0;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        dst[q * NF + gi] = src[q] == -1 ? dA / dAtot * gb[q] : (src[q] == n ? PLM * gb[q] : 0.0);
+        if (src[q] == -1) wb += gb[q] * S[q] / dAtot;  // d g / d dA_f
+      }
+      a.fr[(AFR_WB + (u == 2 ? 0 : 1)) * NF + gi] = wb * pr[u] * pp[u];
+      ++n;
     }
-    a.fr[(AFR_WB + side) * NF + gi] = wb * PI.r * PI.deltaPerp;
-    ++n;
-  };
-  if (pR >= 0) faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) { put(PI, gi, a.aSR, 1); });
-  if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) { put(PI, gi, a.aSL, 0); });
+  }
 }
 
 // face slope S = (W_R - W_L)/(deltaL + deltaR), W_L = p_L + dpL gp_L,
