@@ -34,8 +34,8 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
       return;
     }
   }
-  if (NQ != 5 && NQ != 6) {
-    disco_set_error(7, "NUM_Q must be 5 or 6 (NUM_C 5 with at most one passive scalar)");
+  if (NQ < 5 || NQ > DISCO_MAXQ) {
+    disco_set_error(7, "NUM_Q must be 5 to 8 (NUM_C 5 with at most three passive scalars)");
     return;
   }
   std::vector<double> hrf, hzf;

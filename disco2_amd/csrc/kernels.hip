@@ -1323,12 +1323,18 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   if (L.n_sz) KLAUNCH(st, "k_sync_z", hipLaunchKernelGGL(k_sync_z<NQ>, dim3(L.n_sz), B, 0, st, m2, L.sz));
 }
 
+// NUM_Q = 5 + NUM_N, NUM_N = 0..3: one instantiation of every kernel each
+#define DISCO_NQ_DISPATCH(NQ, CALL)      \
+  switch (NQ) {                          \
+    case 5: { constexpr int Q = 5; CALL; } break; \
+    case 6: { constexpr int Q = 6; CALL; } break; \
+    case 7: { constexpr int Q = 7; CALL; } break; \
+    default: { constexpr int Q = 8; CALL; } break; \
+  }
+
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
                     int recompute_w, int pre_ops, int post_ops) {
-  if (NQ == 5)
-    launch_substep_q<5>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
-  else
-    launch_substep_q<6>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
+  DISCO_NQ_DISPATCH(NQ, launch_substep_q<Q>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops));
 }
 
 bool prepare_launches(const LaunchPlan &L) {
@@ -1348,10 +1354,8 @@ void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st) {
 
 void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ) {
   if (!L.n_itiles) return;
-  if (NQ == 5)
-    KLAUNCH(st, "k_mindt", hipLaunchKernelGGL(k_mindt<5>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles));
-  else
-    KLAUNCH(st, "k_mindt", hipLaunchKernelGGL(k_mindt<6>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles));
+  DISCO_NQ_DISPATCH(NQ, KLAUNCH(st, "k_mindt",
+                                hipLaunchKernelGGL(k_mindt<Q>, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles)));
 }
 
 // plain copy of n doubles: the reference point for HBM byte counters
@@ -1365,30 +1369,21 @@ void launch_copy(double *dst, const double *src, long long n, hipStream_t st) {
 }
 
 void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int dir) {
-  if (NQ == 5)
-    launch_faces_q<5>(m, L, st, dir);
-  else
-    launch_faces_q<6>(m, L, st, dir);
+  DISCO_NQ_DISPATCH(NQ, launch_faces_q<Q>(m, L, st, dir));
+}
+
+template <int NQ>
+static void launch_plm_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int which) {
+  if (which < 0) {
+    hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
+  } else {
+    const int2 *ft = which == 0 ? L.rftiles : L.zftiles;
+    int nft = which == 0 ? L.n_rftiles : L.n_zftiles;
+    if (nft) hipLaunchKernelGGL(k_face_slope<NQ>, dim3(nft), dim3(TILE), 0, st, m, ft);
+    hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles, which);
+  }
 }
 
 void launch_plm(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int which) {
-  if (NQ == 5) {
-    if (which < 0)
-      hipLaunchKernelGGL(k_plm_p<5>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
-    else {
-      const int2 *ft = which == 0 ? L.rftiles : L.zftiles;
-      int nft = which == 0 ? L.n_rftiles : L.n_zftiles;
-      if (nft) hipLaunchKernelGGL(k_face_slope<5>, dim3(nft), dim3(TILE), 0, st, m, ft);
-      hipLaunchKernelGGL(k_plm_rz<5>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles, which);
-    }
-  } else {
-    if (which < 0)
-      hipLaunchKernelGGL(k_plm_p<6>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
-    else {
-      const int2 *ft = which == 0 ? L.rftiles : L.zftiles;
-      int nft = which == 0 ? L.n_rftiles : L.n_zftiles;
-      if (nft) hipLaunchKernelGGL(k_face_slope<6>, dim3(nft), dim3(TILE), 0, st, m, ft);
-      hipLaunchKernelGGL(k_plm_rz<6>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles, which);
-    }
-  }
+  DISCO_NQ_DISPATCH(NQ, launch_plm_q<Q>(m, L, st, which));
 }

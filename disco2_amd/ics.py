@@ -79,8 +79,8 @@ def vortex(geom: dict, gamma: float, seed: int = 1, disk: int = 0, beta: float =
     vp = -du * np.sin(phi) + dv * np.cos(phi)
     prim = np.zeros((nq, len(r)))
     prim[RHO], prim[PPP], prim[URR], prim[UPP] = rho, P, vr, vp / r
-    if nq > 5:  # passive scalar: marks the vortex core
-        prim[5] = np.exp(-rr2)
+    for q in range(5, nq):  # passive scalars: mark the vortex core at widths 1, 1/2, 1/3
+        prim[q] = np.exp(-(q - 4) * rr2)
     return prim
 
 
