@@ -140,6 +140,7 @@ def batched_leg(steps, warmup, ndisk=64):
     for d, pr in enumerate(prims):
         big.upload("prim", pr, disk=d)
     big.init()
+    k, cand = big.autotune(3)
     big.run(warmup)
     big.synchronize()
     t0 = time.perf_counter()
@@ -173,7 +174,8 @@ def batched_leg(steps, warmup, ndisk=64):
                                   "graph": True},
             "per_disk_eager_launches": {"value": n_int * steps / t_eager, "ms_per_step": 1e3 * t_eager / steps,
                                         "graph": False},
-            "speedup": t_plain / t_batched, "speedup_vs_eager": t_eager / t_batched, "unit": "zone-updates/s"}
+            "speedup": t_plain / t_batched, "speedup_vs_eager": t_eager / t_batched, "unit": "zone-updates/s",
+            "schedule": {"chosen": k, "ms_per_step": {str(c): round(v, 4) for c, v in cand.items()}}}
 
 
 def adjoint_leg(steps=200, every=50):
@@ -251,6 +253,8 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch schedule")
+    ap.add_argument("--autotune-steps", type=int, default=5)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--no-adjoint", action="store_true")
@@ -284,6 +288,11 @@ def main(argv=None):
         batch.close()
         return 0
     graph = not args.no_graph
+    schedule = None
+    if not args.no_autotune:
+        k, cand = batch.autotune(args.autotune_steps)
+        schedule = {"chosen": k, "ms_per_step": {str(c): round(v, 4) for c, v in cand.items()}}
+        print(f"[bench] autotuned schedule {k}: {schedule['ms_per_step']}", file=sys.stderr, flush=True)
     print(f"[bench] {args.workload}: {setup.nzones} zones ({n_int} interior), warmup {args.warmup}",
           file=sys.stderr, flush=True)
     batch.run(args.warmup, graph=graph)
@@ -365,7 +374,7 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": args.workload, "NumR": params["NumR"], "NumZ": params["NumZ"],
+        "config": {"workload": args.workload, "schedule": schedule, "NumR": params["NumR"], "NumZ": params["NumZ"],
                    "zones": setup.nzones, "interior_zones": n_int, "riemann": "HLLC" if params["RiemannSolver"] == 0 else "HLL",
                    "graph": graph, "parallelism": "single disk, 1 GPU"},
         "roofline": roof,

@@ -121,6 +121,9 @@ SIGNATURES = {
     "disco_batch_set_scalars": (None, [vp, ci, pd]),
     "disco_batch_diag_equat": (ci, [vp, pd]),
     "disco_batch_initial_data": (ci, [vp, ci]),
+    "disco_batch_autotune": (ci, [vp, ci, pd]),
+    "disco_batch_set_schedule": (ci, [vp, ci]),
+    "disco_batch_schedule": (ci, [vp]),
     "disco_initial_prim": (ci, [vp, pd, pd, pd]),
     "disco_batch_checkpoint_write": (ci, [vp, ci, ccp]),
     "disco_batch_checkpoint_read": (ci, [vp, ci, ccp]),

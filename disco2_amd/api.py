@@ -168,6 +168,23 @@ class Batch:
             raise RuntimeError("diagnostics failed")
         return vec.reshape(nrg.value, 17), sc, eq[:neq.value * 18].reshape(neq.value, 18) if equat else None, md
 
+    def autotune(self, nsteps=10):
+        """Time every launch schedule on a scratch copy of the batch and keep
+        the fastest; returns (chosen schedule, {schedule: ms/step})."""
+        ms = np.zeros(8)
+        k = lib().disco_batch_autotune(self.h, int(nsteps), dptr(ms))
+        check()
+        if k < 0:
+            raise RuntimeError("autotune failed")
+        return k, {i: float(v) for i, v in enumerate(ms) if v >= 0}
+
+    def set_schedule(self, k):
+        lib().disco_batch_set_schedule(self.h, int(k))
+        check()
+
+    def schedule(self):
+        return lib().disco_batch_schedule(self.h)
+
     def initial_data(self, disk=0):
         """Set the disk's primitives with its InitialDataType plugin (the
         reference's src/main.c:103-104); run before init()."""

@@ -28,6 +28,9 @@ __device__ __forceinline__ int xcd_order() {
   return x * q + min(x, r) + (b >> 3);
 }
 
+// the workgroup's tile under the mesh's launch schedule
+__device__ __forceinline__ int tile_of(const DevMesh &m) { return m.sched_xcd ? xcd_order() : (int)blockIdx.x; }
+
 __device__ __forceinline__ const DiskParams &disk_of(const DevMesh &m, int d) { return m.dp[d]; }
 
 __device__ __forceinline__ int rpair_of(const DiskParams &P, int k, int i) { return P.rpair_base + k * (P.NR - 1) + i; }

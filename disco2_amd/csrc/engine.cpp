@@ -4,6 +4,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 
 #include "../../include/disco.h"
@@ -278,6 +279,18 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     plan.stage_r = mr <= cap ? mr : 0;
     plan.stage_z = mz <= cap ? mz : 0;
     if (!prepare_launches(plan)) plan.stage_r = plan.stage_z = 0;
+    stage_r_max = plan.stage_r;
+    stage_z_max = plan.stage_z;
+  }
+  {  // the face builder's pairs, largest outer annulus first (stable), per direction
+    std::vector<int> perm(hpairs.size());
+    for (size_t q = 0; q < perm.size(); ++q) perm[q] = (int)q;
+    auto bigger = [&](int x, int y) { return hann[hpairs[x].ann_out].N > hann[hpairs[y].ann_out].N; };
+    std::stable_sort(perm.begin(), perm.begin() + n_rpairs, bigger);
+    std::stable_sort(perm.begin() + n_rpairs, perm.end(), bigger);
+    if (!dalloc(&d_pair_perm, perm.empty() ? 1 : perm.size(), "pair order") ||
+        (!perm.empty() && !h2d(d_pair_perm, perm.data(), perm.size())))
+      return;
   }
   plan.n_rpairs = n_rpairs;
   plan.n_zpairs = n_zpairs;
@@ -309,7 +322,8 @@ Engine::~Engine() {
   if (graph) hipGraphDestroy(graph);
   void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
-                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot, d_init, adj_mem, diag_mem, ckpt_mem};
+                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot, d_init, d_pair_perm, adj_mem, diag_mem,
+                  ckpt_mem};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (ev0) hipEventDestroy(ev0);
@@ -351,6 +365,8 @@ DevMesh Engine::mesh() const {
   m.dt_tile = d_dt_tile;
   m.ann_rot = d_ann_rot;
   m.init = d_init;
+  m.sched_xcd = sched.xcd;
+  m.pair_perm = sched.largest_first ? d_pair_perm : nullptr;
   m.u_save = adj_usave;
   m.torque_tile = adj_torque;
   m.ftype = adj_ftype;
@@ -543,6 +559,71 @@ void Engine::run(int nsteps, bool use_graph) {
       enqueue_rk2();
   }
   hip_check(hipEventRecord(ev1, st), "event");
+}
+
+void Engine::set_schedule(const Schedule &s) {
+  sched = s;
+  plan.stage_r = s.stage ? stage_r_max : 0;
+  plan.stage_z = s.stage ? stage_z_max : 0;
+  if (gexec) {  // the next graph run captures the new launches
+    hipGraphExecDestroy(gexec);
+    hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+  graph_cur = -1;
+}
+
+// Candidate k: xcd = k & 1, largest_first = (k >> 1) & 1, stage = (k >> 2) & 1.
+// Each runs on a scratch engine holding a copy of the batch's state (the
+// batch itself does not move), graph-replayed for nsteps after two warm-up
+// steps.  Schedules change only the order in which workgroups run, never the
+// arithmetic, so every candidate gives the same bits.
+int Engine::autotune(int nsteps, double *ms, int nms) {
+  if (!ok) return -1;
+  if (nsteps < 1) nsteps = 1;
+  Engine t(ndisk, sims.data());
+  if (!t.ok) return -1;
+  for (int d = 0; d < ndisk; ++d) {
+    std::vector<double> buf((size_t)NQ * nzones[d]);
+    download(d, 0, buf.data());
+    t.upload(d, 0, buf.data());
+    for (int f : {5, 6, 7}) {
+      download(d, f, buf.data());
+      t.upload(d, f, buf.data());
+    }
+    t.put_scalars(d, get_scalars(d));
+  }
+  double best = 1e300;
+  int bestk = -1;
+  for (int k = 0; k < 8; ++k) {
+    Schedule s;
+    s.xcd = k & 1;
+    s.largest_first = (k >> 1) & 1;
+    s.stage = (k >> 2) & 1;
+    if (s.stage && !stage_r_max && !stage_z_max) {  // nothing to stage: the same launches as k - 4
+      if (k < nms) ms[k] = -1.0;
+      continue;
+    }
+    t.set_schedule(s);
+    t.run(2, true);
+    t.synchronize();
+    t.run(nsteps, true);
+    t.synchronize();
+    const double v = t.last_ms / nsteps;
+    if (k < nms) ms[k] = v;
+    if (v < best) {
+      best = v;
+      bestk = k;
+    }
+  }
+  if (bestk < 0) return -1;
+  Schedule s;
+  s.xcd = bestk & 1;
+  s.largest_first = (bestk >> 1) & 1;
+  s.stage = (bestk >> 2) & 1;
+  set_schedule(s);
+  return bestk;
 }
 
 void Engine::synchronize() {

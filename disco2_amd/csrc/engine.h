@@ -57,6 +57,19 @@ struct Engine {
   void *adj_mem = nullptr;  // the adjoint's working memory, kept for later calls
   size_t adj_cap = 0;
 
+  // launch schedule (the autotuner's choice; the results do not depend on it):
+  // XCD-banded tiles, the face builder's pairs largest first, the outer
+  // annulus staged in LDS
+  struct Schedule {
+    int xcd = 1, largest_first = 0, stage = 1;
+  } sched;
+  int stage_r_max = 0, stage_z_max = 0;  // LDS staging the plan allows
+  int *d_pair_perm = nullptr;
+  void set_schedule(const Schedule &s);
+  // times the candidate schedules on a scratch copy of the batch and keeps
+  // the fastest; ms[k] per candidate k (bits xcd | largest_first << 1 | stage << 2)
+  int autotune(int nsteps, double *ms, int nms);
+
   LaunchPlan plan{};
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;

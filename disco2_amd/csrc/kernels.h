@@ -32,6 +32,11 @@ struct DevMesh {
   signed char *ftype;     // the face builder records each face's bounding edges here
   long long Ntot, NFtot;
   int *err;
+  // launch schedule (chosen by the batch's autotuner): workgroup -> tile in
+  // XCD bands or in launch order; face-builder pairs in table order or the
+  // permutation pair_perm (largest outer annulus first)
+  int sched_xcd;
+  const int *pair_perm;
 };
 
 // which edges bound an overlap face, left then right (I: inner annulus,

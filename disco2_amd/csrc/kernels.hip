@@ -261,7 +261,8 @@ __global__ __launch_bounds__(FTB) void k_faces(DevMesh m, int pair0, int nstage)
   __shared__ double s_d[FTB];
   __shared__ int s_j[FTB];
   __shared__ int s_w[FTB / 64];
-  const int p = pair0 + xcd_order();
+  const int b = pair0 + tile_of(m);
+  const int p = m.pair_perm ? m.pair_perm[b] : b;
   const PairInfo P = m.pairs[p];
   const AnnInfo A = m.ann[P.ann_in];
   const AnnInfo B = m.ann[P.ann_out];
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(FTB) void k_faces(DevMesh m, int pair0, int nstage)
 // One wavefront per annulus; C_RIGID sums in the reference's cell order.
 // ----------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_set_w(DevMesh m, int ann0) {
-  const int a = ann0 + xcd_order();
+  const int a = ann0 + tile_of(m);
   const AnnInfo A = m.ann[a];
   const DiskParams &P = disk_of(m, A.disk);
   const int lane = threadIdx.x;
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(64) void k_set_w(DevMesh m, int ann0) {
 // ----------------------------------------------------------------------------
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
-  const int2 tl = tiles[xcd_order()];
+  const int2 tl = tiles[tile_of(m)];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
 // first NQ rows of the face-increment buffer (the fluxes overwrite them later).
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_face_slope(DevMesh m, const int2 *ftiles) {
-  const int2 tl = ftiles[xcd_order()];
+  const int2 tl = ftiles[tile_of(m)];
   const int p = tl.x;
   const int f = tl.y + threadIdx.x;
   if (f >= m.pst[p].count) return;
@@ -388,7 +389,7 @@ __global__ __launch_bounds__(TILE) void k_face_slope(DevMesh m, const int2 *ftil
 #endif
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, int dir) {
-  const int2 tl = tiles[xcd_order()];
+  const int2 tl = tiles[tile_of(m)];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -549,7 +550,7 @@ __device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A,
 // (with FLUX_SPLIT the two parts run as two launches)
 template <int NQ, int PART>
 __global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
-  const int2 tl = ftiles[xcd_order()];
+  const int2 tl = ftiles[tile_of(m)];
   const int p = tl.x;
   const int f = tl.y + threadIdx.x;
   if (f >= m.pst[p].count) return;
@@ -660,7 +661,7 @@ __device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy
 
 template <int NQ>
 __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
-  const int ti = xcd_order();
+  const int ti = tile_of(m);
   const int2 tl = tiles[ti];
   const AnnInfo A = m.ann[tl.x];
   const int t = threadIdx.x;
@@ -914,7 +915,7 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
 // ----------------------------------------------------------------------------
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_bc_r(DevMesh m, const int2 *tiles) {
-  const int2 tl = tiles[xcd_order()];
+  const int2 tl = tiles[tile_of(m)];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -955,7 +956,7 @@ __global__ __launch_bounds__(TILE) void k_bc_r(DevMesh m, const int2 *tiles) {
 // layers, src/Cell/cell_sync.c:156-205.
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_sync_z(DevMesh m, const int2 *tiles) {
-  const int2 tl = tiles[xcd_order()];
+  const int2 tl = tiles[tile_of(m)];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -1008,7 +1009,7 @@ __device__ __forceinline__ void single_init(int type, const InitRow &R, double r
 
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_bc_init(DevMesh m, const int2 *tiles, double fac) {
-  const int2 tl = tiles[xcd_order()];
+  const int2 tl = tiles[tile_of(m)];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -1068,7 +1069,7 @@ __device__ __forceinline__ double cell_dt(const DevMesh &m, const int2 tl, const
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
   __shared__ double s_m[TILE / 64];
-  const int it = xcd_order();
+  const int it = tile_of(m);
   const int2 tl = tiles[it];
   const AnnInfo A = m.ann[tl.x];
   double v = cell_dt<NQ>(m, tl, A);

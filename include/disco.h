@@ -293,6 +293,18 @@ double disco_batch_last_ms(struct DiscoBatch *);
  * up to maxk distinct kernel names (32 chars each), total ms and launch
  * counts; returns the number of names */
 int disco_batch_profile(struct DiscoBatch *, int nsteps, char *names, double *ms, int *counts, int maxk);
+/* Launch schedule of a batch (the results do not depend on it): k = xcd |
+ * largest_first << 1 | stage << 2, with xcd: workgroups take their tiles in
+ * bands per XCD (else in launch order); largest_first: the face builder
+ * takes annulus pairs largest outer annulus first; stage: it stages the
+ * outer annulus's edges in LDS.  The default is 5 (xcd, stage).
+ * disco_batch_autotune times every candidate for nsteps graph-replayed steps
+ * on a scratch copy of the batch (the batch does not move), keeps the
+ * fastest and returns it (-1 on error); ms8[k] gets each candidate's
+ * ms/step (-1: not run, the same launches as k - 4). */
+int disco_batch_autotune(struct DiscoBatch *, int nsteps, double *ms8);
+int disco_batch_set_schedule(struct DiscoBatch *, int k);
+int disco_batch_schedule(struct DiscoBatch *);
 /* device copy of n doubles (8n bytes read, 8n written); returns ms */
 double disco_calibrate_copy(long long n);
 
