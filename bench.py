@@ -266,7 +266,7 @@ def main(argv=None):
                     help="run only the diagnostics leg on the workload and print its JSON (for profiling it)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (one disk or batch per "
+        print("bench.py measures one GPU: disco2_amd has no multi-GPU path (one disk or batch per "
               "GPU); run with --gpus 1", file=sys.stderr)
         return 2
     if args.adjoint_only:
