@@ -74,11 +74,10 @@ def kernel_alg_bytes(name, nz, nfaces_r, nq, visc):
         # per zone: prim, grad, gradp, tiph, dphi
         return nfaces_r * (24 + 8 + d * fx) + nz * d * (3 * nq + 2)
     if name == "k_plm_r":
-        # per face: slope S (NQ) and overlap length; per zone: dphi, cursors in, grad out
-        return nfaces_r * d * (nq + 1) + nz * (d + 8) + nz * d * nq
-    if name == "k_face_slope_r":
-        # per face: cell indices, centre phi in, S out; per zone: prim, gradp, tiph, dphi
-        return nfaces_r * (24 + d * nq) + nz * d * (2 * nq + 2)
+        # the face slopes are evaluated in place from the faces' two cells:
+        # per face: cell indices, centre phi, overlap length; per zone: prim,
+        # gradp, tiph, dphi and the cursors in, grad out
+        return nfaces_r * (16 + 2 * d) + nz * (d * (2 * nq + 2) + 8) + nz * d * nq
     if name == "k_plm_p":
         return nz * d * (nq + 1) + nz * d * nq
     if name == "k_mindt":

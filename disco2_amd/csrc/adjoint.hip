@@ -336,7 +336,7 @@ __global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_rflux_h(DevMesh m, A
     RFaceHydroAdj h;
     vjp_rface_hydro(P, c, F.geo, a.fac * m.sc[PI.disk].dt, F.Xb, sb, h);
     const long long NF = m.NFtot, g = F.g;
-    {  // the face slope, as k_face_slope stores it for k_plm_rz
+    {  // the face slope S, stored for ka_plm_r's replay of the limiter
       double S[5];
       face_slope<5>(m, PI, g, S);
       for (int q = 0; q < 5; ++q) m.Fface[q * NF + g] = S[q];

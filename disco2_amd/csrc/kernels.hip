@@ -364,29 +364,39 @@ __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
 // gather: the cell's faces as R (previous pair) then as L (next pair), in
 // face order, for the sum and again for the limiter.
 // ----------------------------------------------------------------------------
-// The face difference S = (W_R - W_L)/(deltaL + deltaR) of every overlap
-// face (cell_plm.c:36-49 and 86-99), computed once per face and stored in the
-// first NQ rows of the face-increment buffer (the fluxes overwrite them later).
+// The face difference S = (W_R - W_L)/(deltaL + deltaR) (cell_plm.c:36-49 and
+// 86-99) is computed where it is used, from the face's two cells, with
+// face_slope's arithmetic: the cell's own state is loaded once, the other
+// cell's per face (a neighbour's cells sit in the same L2).  Each face is
+// evaluated by both of its cells and never goes through HBM.  The slopes of
+// a cell's first PlmCache::NC faces wait in LDS (a private column per thread, no
+// barrier) between the sum and the limiter's walk; the rest, rare, are
+// evaluated again.  LDS rather than registers keeps the kernel at four waves
+// per SIMD.
 template <int NQ>
-__global__ __launch_bounds__(TILE) void k_face_slope(DevMesh m, const int2 *ftiles) {
-  const int2 tl = ftiles[tile_of(m)];
-  const int p = tl.x;
-  const int f = tl.y + threadIdx.x;
-  if (f >= m.pst[p].count) return;
-  const PairInfo PI = m.pairs[p];
-  const long long g = PI.face_base + f;
-  double S[NQ];
-  face_slope<NQ>(m, PI, g, S);
+struct PlmCache {
+  static constexpr int NC = NQ <= 6 ? 4 : 3;  // 40-48 KiB per workgroup
+};
+// S of face g of a cell whose centre angle clean_pi(tiph) - dphi/2 is phi_o,
+// primitives w and phi slopes gp; own_left: the cell is the face's L
+template <int NQ>
+__device__ __forceinline__ void own_face_slope(const DevMesh &m, double phi_o, const double *w, const double *gp,
+                                               bool own_left, double dsum, long long g, double *S) {
+  const int oth = own_left ? m.fR[g] : m.fL[g];
+  const double cm = m.fcm[g];
+  const double ph = clean_pi(m.tiph[oth]) - .5 * m.dphi[oth];
+  const double pL = own_left ? phi_o : ph;
+  const double pR = own_left ? ph : phi_o;
+  const double dpL = wrap_pm(cm - pL);
+  const double dpR = wrap_pm(pR - cm);
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = S[q];
+  for (int q = 0; q < NQ; ++q) {
+    const double v = m.prim[q * m.Ntot + oth], gv = m.gradp[q * m.Ntot + oth];
+    const double WL = own_left ? w[q] + dpL * gp[q] : v + dpL * gv;
+    const double WR = own_left ? v - dpR * gv : w[q] - dpR * gp[q];
+    S[q] = (WR - WL) / dsum;
+  }
 }
-
-// The slopes of a cell's first PLM_CACHE faces stay in registers between the
-// sum and the limiter's walk (slot selects on constant indices); the rest are
-// read again.
-#ifndef PLM_CACHE
-#define PLM_CACHE 4
-#endif
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, int dir) {
   const int2 tl = tiles[tile_of(m)];
@@ -411,10 +421,10 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
   // The cell's faces in the reference's order: as R, its runs in the pair
   // below (faces_as_R: at most two, the second is the wrap), then as L one
   // range in the pair above (faces_as_L); range u is faces [fb[u] + b[u],
-  // fb[u] + e[u]) with its pair's r and deltaPerp.
+  // fb[u] + e[u]) with its pair's r, deltaPerp and deltaL + deltaR.
   long long fb[3] = {0, 0, 0};
   int b[3] = {0, 0, 0}, e[3] = {0, 0, 0};
-  double pr[3] = {0, 0, 0}, pp[3] = {0, 0, 0};
+  double pr[3] = {0, 0, 0}, pp[3] = {0, 0, 0}, ds[3] = {1, 1, 1};
   if (pR >= 0) {
     const PairInfo PI = m.pairs[pR];
     const PairState S = m.pst[pR];
@@ -428,6 +438,7 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
         e[u] = m.run_start[PI.run_base + t + 1];
         pr[u] = PI.r;
         pp[u] = PI.deltaPerp;
+        ds[u] = PI.deltaR + PI.deltaL;
       }
       t += A.N;
     }
@@ -444,29 +455,36 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
     e[2] = f1;
     pr[2] = PI.r;
     pp[2] = PI.deltaPerp;
+    ds[2] = PI.deltaR + PI.deltaL;
   }
-  double cS[PLM_CACHE > 0 ? PLM_CACHE : 1][NQ];
+  const double dp = m.dphi[z];
+  const double phi_o = clean_pi(m.tiph[z]) - .5 * dp;
+  double w[NQ], gp[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    w[q] = m.prim[q * m.Ntot + z];
+    gp[q] = m.gradp[q * m.Ntot + z];
+  }
+  constexpr int NC = PlmCache<NQ>::NC;
+  __shared__ double s_S[NC][NQ][TILE];
+  const int t = threadIdx.x;
   int n = 0;
 #pragma unroll
   for (int u = 0; u < 3; ++u)
     for (int f = b[u]; f < e[u]; ++f) {
       const long long gi = fb[u] + f;
       double S[NQ];
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) S[q] = m.Fface[q * m.NFtot + gi];
+      own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], gi, S);
       double dA = pr[u] * m.fdphi[gi] * pp[u];
 #pragma unroll
       for (int q = 0; q < NQ; ++q) g[q] += S[q] * dA;
+      if (n < NC)
 #pragma unroll
-      for (int c = 0; c < PLM_CACHE; ++c)
-        if (n == c)
-#pragma unroll
-          for (int q = 0; q < NQ; ++q) cS[c][q] = S[q];
+        for (int q = 0; q < NQ; ++q) s_S[n][q][t] = S[q];
       ++n;
     }
   const double dz = A.zp - A.zm;
   const double rp = A.rp, rm = A.rm;
-  const double dp = m.dphi[z];
   double dAtot;
   if (dir == 0) {
     if (i == 0)
@@ -486,16 +504,11 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
   for (int u = 0; u < 3; ++u)
     for (int f = b[u]; f < e[u]; ++f) {
       double S[NQ];
-      if (n < PLM_CACHE) {
+      if (n < NC) {
 #pragma unroll
-        for (int c = 0; c < PLM_CACHE; ++c)
-          if (n == c)
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) S[q] = cS[c][q];
+        for (int q = 0; q < NQ; ++q) S[q] = s_S[n][q][t];
       } else {
-        const long long gi = fb[u] + f;
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) S[q] = m.Fface[q * m.NFtot + gi];
+        own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], fb[u] + f, S);
       }
       ++n;
 #pragma unroll
@@ -1260,7 +1273,7 @@ __global__ __launch_bounds__(SCALAR_THREADS) void k_scalar(DevMesh m, int ndisk,
 // ----------------------------------------------------------------------------
 KProf *g_kprof = nullptr;
 
-// overlap faces of the r (dir 0) or z (dir 1) pairs, with their slopes S
+// overlap faces of the r (dir 0) or z (dir 1) pairs
 template <int NQ>
 static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir) {
   const int np = dir == 0 ? L.n_rpairs : L.n_zpairs;
@@ -1269,11 +1282,6 @@ static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st
   const int nst = dir == 0 ? L.stage_r : L.stage_z;
   KLAUNCH(st, dir == 0 ? "k_faces_r" : "k_faces_z",
           hipLaunchKernelGGL(k_faces, dim3(np), dim3(FTB), (size_t)nst * 2 * sizeof(double), st, m, p0, nst));
-  const int2 *ft = dir == 0 ? L.rftiles : L.zftiles;
-  const int nft = dir == 0 ? L.n_rftiles : L.n_zftiles;
-  if (nft)
-    KLAUNCH(st, dir == 0 ? "k_face_slope_r" : "k_face_slope_z",
-            hipLaunchKernelGGL(k_face_slope<NQ>, dim3(nft), dim3(TILE), 0, st, m, ft));
 }
 
 // the r or z faces' increments: one kernel, or (FLUX_SPLIT) the Riemann and
@@ -1378,9 +1386,6 @@ static void launch_plm_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, 
   if (which < 0) {
     hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
   } else {
-    const int2 *ft = which == 0 ? L.rftiles : L.zftiles;
-    int nft = which == 0 ? L.n_rftiles : L.n_zftiles;
-    if (nft) hipLaunchKernelGGL(k_face_slope<NQ>, dim3(nft), dim3(TILE), 0, st, m, ft);
     hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles, which);
   }
 }
