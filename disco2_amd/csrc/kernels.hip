@@ -256,7 +256,11 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
 }
 
 // nstage: LDS doubles per array for the outer annulus (0: none staged)
-__global__ __launch_bounds__(FTB) void k_faces(DevMesh m, int pair0, int nstage) {
+// Two workgroups per CU (all 512 annulus pairs of a 512-annulus disk
+// resident at once) need 8 waves per SIMD: left alone the compiler takes 106
+// SGPRs, which allows 7, so one workgroup per CU and two rounds.
+__global__ __launch_bounds__(FTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_faces(DevMesh m, int pair0,
+                                                                                           int nstage) {
   extern __shared__ double s_stage[];  // Tout [nstage], Dout [nstage]
   __shared__ double s_d[FTB];
   __shared__ int s_j[FTB];
