@@ -446,17 +446,29 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
   }
   double ap[5] = {0, 0, 0, 0, 0}, ag[5] = {0, 0, 0, 0, 0}, agp[5] = {0, 0, 0, 0, 0}, rot = 0.0;
   double g[5] = {0, 0, 0, 0, 0};
+  // the face slopes of the cell's first NC faces wait in LDS (a private
+  // column per thread, no barrier) for the limiter replay and the shares
+  constexpr int NC = 4;
+  __shared__ double s_S[NC][5][TILE];
+  const int t = threadIdx.x;
+  int n = 0;
   // the slope sum and the gather.  As R: primR = p - g deltaR - gp dpR
   // (Riemann part), W_R = p - dpR gp (viscous); as L: primL = p + g deltaL
   // + gp dpL, W_L = p + dpL gp
 #pragma unroll
   for (int u = 0; u < 3; ++u) {
     const bool asL = u == 2;
-    for (int f = b[u]; f < e[u]; ++f) {
+    for (int f = b[u]; f < e[u]; ++f, ++n) {
       const long long gi = fb[u] + f;
       const double dA = pr[u] * m.fdphi[gi] * pp[u];
+      double S[5];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) g[q] += m.Fface[q * NF + gi] * dA;
+      for (int q = 0; q < 5; ++q) S[q] = m.Fface[q * NF + gi];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) g[q] += S[q] * dA;
+      if (n < NC)
+#pragma unroll
+        for (int q = 0; q < 5; ++q) s_S[n][q][t] = S[q];
       const double dpo = asL ? wrap_pm(m.fcm[gi] - (tc - .5 * dc)) : wrap_pm(-m.fcm[gi] + (tc - .5 * dc));
       const int hs = asL ? AFR_HL : AFR_HR, ws = asL ? AFR_WL : AFR_WR;
 #pragma unroll
@@ -499,13 +511,21 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
   // replay of the sequential limiter: which face's S it took last
   const double PLM = P.plm;
   int src[5] = {-1, -1, -1, -1, -1};
-  int n = 0;
+  auto slope = [&](int nn, long long gi, double *S) {
+    if (nn < NC) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) S[q] = s_S[nn][q][t];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) S[q] = m.Fface[q * NF + gi];
+    }
+  };
+  n = 0;
 #pragma unroll
   for (int u = 0; u < 3; ++u)
     for (int f = b[u]; f < e[u]; ++f) {
       double S[5];
-#pragma unroll
-      for (int q = 0; q < 5; ++q) S[q] = m.Fface[q * NF + fb[u] + f];
+      slope(n, fb[u] + f, S);
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         if (S[q] * g[q] < 0.0) {
@@ -529,8 +549,7 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
       const long long gi = fb[u] + f;
       const double dA = pr[u] * m.fdphi[gi] * pp[u];
       double S[5];
-#pragma unroll
-      for (int q = 0; q < 5; ++q) S[q] = m.Fface[q * NF + gi];
+      slope(n, gi, S);
       double wb = 0.0;
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
