@@ -36,7 +36,7 @@ B_ALG_PER_ZONE_UPDATE = 320  # 8 * (6*NUM_Q + 10) bytes at NUM_Q = 5
 # Counters cannot be read from inside the timed run, so the figure is
 # reported only for the workload and kernel it was measured on.
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-MEASURED_TRAFFIC_KIB = {("binary2d_r512", "k_update"): (136831, 72642)}
+MEASURED_TRAFFIC_KIB = {("binary2d_r512", "k_update"): (136885, 72652)}
 
 # FP64 arithmetic, measured the same way (rocprofv3 --pmc
 # SQ_INSTS_VALU_FLOPS_FP64 ..., profiles/binary2d_r512_fp64_flops.md): the
@@ -45,7 +45,7 @@ MEASURED_TRAFFIC_KIB = {("binary2d_r512", "k_update"): (136831, 72642)}
 # lanes; masked lanes at tile ends are counted, so these are upper bounds.
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector, published
 MEASURED_FLOP_PER_LAUNCH = {("binary2d_r512", "k_update"): 2.258e7 * 64}
-MEASURED_FLOP_PER_STEP = {"binary2d_r512": 1.0902e8 * 64}  # every kernel of one RK2 step
+MEASURED_FLOP_PER_STEP = {"binary2d_r512": 1.1626e8 * 64}  # every kernel of one RK2 step
 
 
 def r_face_count(n_p, nz_layers):
@@ -80,6 +80,10 @@ def kernel_alg_bytes(name, nz, nfaces_r, nq, visc):
         return nfaces_r * (16 + 2 * d) + nz * (d * (2 * nq + 2) + 8) + nz * d * nq
     if name == "k_plm_p":
         return nz * d * (nq + 1) + nz * d * nq
+    if name == "k_plm_p_dt":
+        # the phi slopes plus the CFL minimum of the same cells (tiph for the
+        # alpha viscosity's position)
+        return nz * d * (nq + 2) + nz * d * nq
     if name == "k_mindt":
         return nz * d * (nq + 2)
     if name == "k_faces_r":

@@ -80,7 +80,7 @@ struct AnnInfo {
   long long zbase;   // first zone of the annulus
   int N;             // N_p
   int disk, k, i;
-  int pad;
+  int itile;         // its first tile in the interior (CFL) tile list; -1 for ghost annuli and layers
   double rm, rp, zm, zp;  // face positions bounding the annulus
 };
 

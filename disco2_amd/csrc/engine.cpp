@@ -103,11 +103,12 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
         A.rp = S->r_faces[i + 1];
         A.zm = S->z_faces[k];
         A.zp = S->z_faces[k + 1];
+        bool interior = i >= P.NgMinR && i < NR - P.NgMaxR && k >= P.NgMinZ && k < NZ - P.NgMaxZ;
+        A.itile = interior ? (int)itiles.size() : -1;
         int a = (int)hann.size();
         hann.push_back(A);
         for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) ztiles.push_back(make_int2(a, j0));
         for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE - 1) utiles.push_back(make_int2(a, j0));
-        bool interior = i >= P.NgMinR && i < NR - P.NgMaxR && k >= P.NgMinZ && k < NZ - P.NgMaxZ;
         if (interior)
           for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) itiles.push_back(make_int2(a, j0));
         if (P.bound_r == BOUND_OUTFLOW) {
@@ -508,12 +509,13 @@ void Engine::enqueue_substep(double fac, int copy_first, int recompute_w, int pr
 }
 
 // timestep_rk2, src/TimeStep/timestep_algorithm.c:11-41.  The per-disk
-// scalar updates between substeps ride in one launch after each update.
+// scalar updates between substeps ride in one launch after each update; the
+// CFL tile minima of the step (timestep_set_dt) ride on the first substep's
+// phi slopes, which read the same state.
 void Engine::enqueue_rk2() {
-  DevMesh m = mesh();
   enqueue_set_w();
-  launch_mindt(m, plan, st, NQ);
-  enqueue_substep(1.0, 1, 0, SC_STEP_BEGIN | SC_SUB_BEGIN, SC_AFTER_UPDATE | SC_SUB1_END | SC_SUB_BEGIN);
+  enqueue_substep(1.0, 1, 0, SC_PLM_DT | SC_STEP_BEGIN | SC_SUB_BEGIN,
+                  SC_AFTER_UPDATE | SC_SUB1_END | SC_SUB_BEGIN);
   enqueue_substep(0.5, 0, w_static ? 0 : 1, 0, SC_AFTER_UPDATE | SC_SUB2_END);
 }
 
@@ -528,10 +530,8 @@ void Engine::enqueue_set_w() {
 
 // timestep_forward_euler, src/TimeStep/timestep_algorithm.c:43-57
 void Engine::enqueue_euler() {
-  DevMesh m = mesh();
   enqueue_set_w();
-  launch_mindt(m, plan, st, NQ);
-  enqueue_substep(1.0, 1, 0, SC_STEP_BEGIN | SC_SUB_BEGIN, SC_AFTER_UPDATE | SC_EULER_END);
+  enqueue_substep(1.0, 1, 0, SC_PLM_DT | SC_STEP_BEGIN | SC_SUB_BEGIN, SC_AFTER_UPDATE | SC_EULER_END);
 }
 
 void Engine::run(int nsteps, bool use_graph) {

@@ -103,6 +103,8 @@ enum {
   SC_RESET_DT = 64,
   SC_MASS = 128,        // launch-uniform gravity terms
   SC_DT_FOLD = 256,     // CFL minimum from the tile minima (runs first; STEP_BEGIN implies it)
+  SC_PLM_DT = 512,      // pre_ops of launch_substep: the phi slopes take the CFL tile minima, and
+                        // the pre-substep scalar launch (with its fold) follows them
 };
 
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,

@@ -333,34 +333,98 @@ __global__ __launch_bounds__(64) void k_set_w(DevMesh m, int ann0) {
 }
 
 // ----------------------------------------------------------------------------
-// phi slopes, cell_plm_p, src/Cell/cell_plm.c:116-161
+// CFL time step over interior cells, cell_mindt, src/Cell/cell_mindt.c:10-100.
+// Each workgroup stores the minimum of its tile with a plain store (no
+// device-wide atomics: thousands of them on one word serialise); k_scalar's
+// SC_DT_FOLD reduces a disk's tiles.  min is exact, so the order is free.
 // ----------------------------------------------------------------------------
 template <int NQ>
+__device__ __forceinline__ double cell_dt(const DevMesh &m, const int2 tl, const AnnInfo &A) {
+  const int j = tl.y + threadIdx.x;
+  const DiskParams &P = disk_of(m, A.disk);
+  double dt = 1.e100;
+  if (j < A.N) {
+    const long long c = A.zbase + j;
+    const double rm = A.rm, rp = A.rp;
+    const double dz = A.zp - A.zm;
+    const double dr = rp - rm;
+    const double r = .5 * (rp + rm);
+    const double wj = m.wiph[tl.x];
+    double w = .5 * (wj + wj);
+    double dx = dr;
+    double rdphi = .5 * (rp + rm) * m.dphi[c];
+    if (rdphi < dr) dx = rdphi;
+    if (dx > dz) dx = dz;
+    double pr[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) pr[q] = m.prim[q * m.Ntot + c];
+    const double tc = (P.visc > 0.0 && P.visc_const != 1) ? clean_pi(m.tiph[c]) - 0.5 * m.dphi[c] : 0.0;
+    double d = cell_dt_value(P, m.sc[A.disk], pr, r, dx, w, tc);
+    if (d < dt) dt = d;
+  }
+  return dt;
+}
+
+// the workgroup's minimum of v, stored by thread 0 at dt_tile[slot]
+__device__ __forceinline__ void store_tile_min(const DevMesh &m, double v, int slot) {
+  __shared__ double s_m[TILE / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(v, off);
+    if (o < v) v = o;
+  }
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = s_m[0];
+    for (int w = 1; w < TILE / 64; ++w)
+      if (s_m[w] < b) b = s_m[w];
+    m.dt_tile[slot] = b;
+  }
+}
+
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
+  const int it = tile_of(m);
+  const int2 tl = tiles[it];
+  const AnnInfo A = m.ann[tl.x];
+  store_tile_min(m, cell_dt<NQ>(m, tl, A), it);
+}
+
+
+// ----------------------------------------------------------------------------
+// phi slopes, cell_plm_p, src/Cell/cell_plm.c:116-161
+// ----------------------------------------------------------------------------
+// DT: the first substep's launch also takes the CFL minimum of its tile
+// (k_mindt's value, from the same prim and dphi) when the tile is interior,
+// into the tile's slot of the interior tile list, saving k_mindt's pass.
+template <int NQ, bool DT>
 __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
   const int2 tl = tiles[tile_of(m)];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
-  if (j >= A.N) return;
-  const double PLM = m.dp[A.disk].plm;
-  const int N = A.N;
-  const long long c = A.zbase + j;
-  const long long cL = A.zbase + (j == 0 ? N - 1 : j - 1);
-  const long long cR = A.zbase + (j == N - 1 ? 0 : j + 1);
-  const double dpL = .5 * (m.dphi[c] + m.dphi[cL]);
-  const double dpR = .5 * (m.dphi[c] + m.dphi[cR]);
+  if (j < A.N) {
+    const double PLM = m.dp[A.disk].plm;
+    const int N = A.N;
+    const long long c = A.zbase + j;
+    const long long cL = A.zbase + (j == 0 ? N - 1 : j - 1);
+    const long long cR = A.zbase + (j == N - 1 ? 0 : j + 1);
+    const double dpL = .5 * (m.dphi[c] + m.dphi[cL]);
+    const double dpR = .5 * (m.dphi[c] + m.dphi[cR]);
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const double *pq = m.prim + q * m.Ntot;
-    double vc = pq[c], vL = pq[cL], vR = pq[cR];
-    double sL = (vc - vL) / dpL;
-    double sR = (vR - vc) / dpR;
-    double sM = (vR - vL) / (dpL + dpR);
-    double s = PLM * sL;
-    if (fabs(s) > PLM * fabs(sR)) s = PLM * sR;
-    if (fabs(s) > fabs(sM)) s = sM;
-    if (sL * sR < 0.0) s = 0.0;
-    m.gradp[q * m.Ntot + c] = s;
+    for (int q = 0; q < NQ; ++q) {
+      const double *pq = m.prim + q * m.Ntot;
+      double vc = pq[c], vL = pq[cL], vR = pq[cR];
+      double sL = (vc - vL) / dpL;
+      double sR = (vR - vc) / dpR;
+      double sM = (vR - vL) / (dpL + dpR);
+      double s = PLM * sL;
+      if (fabs(s) > PLM * fabs(sR)) s = PLM * sR;
+      if (fabs(s) > fabs(sM)) s = sM;
+      if (sL * sR < 0.0) s = 0.0;
+      m.gradp[q * m.Ntot + c] = s;
+    }
   }
+  if (DT && A.itile >= 0) store_tile_min(m, cell_dt<NQ>(m, tl, A), A.itile + tl.y / TILE);
 }
 
 // ----------------------------------------------------------------------------
@@ -1050,60 +1114,6 @@ __global__ __launch_bounds__(TILE) void k_bc_init(DevMesh m, const int2 *tiles, 
   for (int q = 0; q < NQ; ++q) m.prim[q * m.Ntot + c] = pr[q];
 }
 
-// ----------------------------------------------------------------------------
-// CFL time step over interior cells, cell_mindt, src/Cell/cell_mindt.c:10-100.
-// Each workgroup stores the minimum of its tile with a plain store (no
-// device-wide atomics: thousands of them on one word serialise); k_scalar's
-// SC_DT_FOLD reduces a disk's tiles.  min is exact, so the order is free.
-// ----------------------------------------------------------------------------
-template <int NQ>
-__device__ __forceinline__ double cell_dt(const DevMesh &m, const int2 tl, const AnnInfo &A) {
-  const int j = tl.y + threadIdx.x;
-  const DiskParams &P = disk_of(m, A.disk);
-  double dt = 1.e100;
-  if (j < A.N) {
-    const long long c = A.zbase + j;
-    const double rm = A.rm, rp = A.rp;
-    const double dz = A.zp - A.zm;
-    const double dr = rp - rm;
-    const double r = .5 * (rp + rm);
-    const double wj = m.wiph[tl.x];
-    double w = .5 * (wj + wj);
-    double dx = dr;
-    double rdphi = .5 * (rp + rm) * m.dphi[c];
-    if (rdphi < dr) dx = rdphi;
-    if (dx > dz) dx = dz;
-    double pr[5];
-#pragma unroll
-    for (int q = 0; q < 5; ++q) pr[q] = m.prim[q * m.Ntot + c];
-    const double tc = (P.visc > 0.0 && P.visc_const != 1) ? clean_pi(m.tiph[c]) - 0.5 * m.dphi[c] : 0.0;
-    double d = cell_dt_value(P, m.sc[A.disk], pr, r, dx, w, tc);
-    if (d < dt) dt = d;
-  }
-  return dt;
-}
-
-template <int NQ>
-__global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
-  __shared__ double s_m[TILE / 64];
-  const int it = tile_of(m);
-  const int2 tl = tiles[it];
-  const AnnInfo A = m.ann[tl.x];
-  double v = cell_dt<NQ>(m, tl, A);
-  for (int off = 32; off > 0; off >>= 1) {
-    double o = __shfl_down(v, off);
-    if (o < v) v = o;
-  }
-  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = s_m[0];
-    for (int w = 1; w < TILE / 64; ++w)
-      if (s_m[w] < b) b = s_m[w];
-    m.dt_tile[it] = b;
-  }
-}
-
 // SC_DT_FOLD: dtmin of disk d = min(dtmin, the positive tile minima below
 // 1e100) over the disk's interior tiles; every thread of the workgroup calls it.
 __device__ __forceinline__ void fold_dt(const DevMesh &m, const DiskParams &P, DiskScalars &s) {
@@ -1310,8 +1320,13 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
                              int recompute_w, int pre_ops, int post_ops) {
   const dim3 B(TILE);
   const dim3 SD(L.ndisk), S64(SCALAR_THREADS);
-  if (pre_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, pre_ops, 0.0));
-  KLAUNCH(st, "k_plm_p", hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
+  if (pre_ops & SC_PLM_DT) {  // the CFL tile minima ride on the phi slopes; the fold follows them
+    KLAUNCH(st, "k_plm_p_dt", hipLaunchKernelGGL((k_plm_p<NQ, true>), dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
+    KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, pre_ops & ~SC_PLM_DT, 0.0));
+  } else {
+    if (pre_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, pre_ops, 0.0));
+    KLAUNCH(st, "k_plm_p", hipLaunchKernelGGL((k_plm_p<NQ, false>), dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
+  }
   launch_faces_q<NQ>(m, L, st, 0);
   launch_faces_q<NQ>(m, L, st, 1);
   if (recompute_w) KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
@@ -1388,7 +1403,7 @@ void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ,
 template <int NQ>
 static void launch_plm_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int which) {
   if (which < 0) {
-    hipLaunchKernelGGL(k_plm_p<NQ>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
+    hipLaunchKernelGGL((k_plm_p<NQ, false>), dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles);
   } else {
     hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), dim3(TILE), 0, st, m, L.ztiles, which);
   }
