@@ -754,6 +754,38 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
   const DiskParams &P = disk_of(m, A.disk);
   const DiskScalars &sc = m.sc[A.disk];
   const double4 rot = ann_rotation(m, P, tl.x);
+  // the cell's r-face ranges (the heads of the update's dependent gathers),
+  // loaded before the phi faces so that their latency hides behind the
+  // Riemann solves: as R, up to two runs of pair (i-1, i); as L, one range
+  // of pair (i, i+1)
+  int rb[3] = {0, 0, 0}, re[3] = {0, 0, 0};
+  if (t < n) {
+    const long long cz = A.zbase + j;
+    if (A.i >= 1) {
+      const int pr_ = rpair_of(P, A.k, A.i - 1);
+      const PairInfo PI = m.pairs[pr_];
+      const PairState S = m.pst[pr_];
+      int tt = j - S.jpmin;
+      if (tt < 0) tt += A.N;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (tt <= S.T) {
+          rb[u] = m.run_start[PI.run_base + tt];
+          re[u] = m.run_start[PI.run_base + tt + 1];
+        }
+        tt += A.N;
+      }
+    }
+    if (A.i <= P.NR - 2) {
+      const PairState S = m.pst[rpair_of(P, A.k, A.i)];
+      int f1 = (j + 1 < A.N) ? m.fo_r[cz + 1] : S.count;
+      if (f1 > S.count) f1 = S.count;
+      int f0 = m.fo_r[cz];
+      if (f0 < 0) f0 = 0;
+      rb[2] = f0;
+      re[2] = f1;
+    }
+  }
   // phi faces: each is solved once, by the thread of its left cell (thread 0
   // also solves the face shared with the previous tile)
   if (t <= n) {
@@ -823,14 +855,18 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
       }
     }
     // r faces in face order: as R of pair (i-1, i), then as L of (i, i+1)
-    if (i >= 1)
-      faces_as_R(m, rpair_of(P, k, i - 1), j, N, [&](const PairInfo &PI, long long g) {
-        add_face_R<NQ>(m, m.Fface, m.NFtot, g, u, PI.r + PI.deltaR, visc);
-      });
-    if (i <= P.NR - 2)
-      faces_as_L(m, rpair_of(P, k, i), m.fo_r, c, j, N, [&](const PairInfo &PI, long long g) {
-        add_face_L<NQ>(m, m.Fface, m.NFtot, g, u, PI.r - PI.deltaL, visc);
-      });
+    if (i >= 1) {
+      const PairInfo PI = m.pairs[rpair_of(P, k, i - 1)];
+      const double rc = PI.r + PI.deltaR;
+#pragma unroll
+      for (int w = 0; w < 2; ++w)
+        for (int f = rb[w]; f < re[w]; ++f) add_face_R<NQ>(m, m.Fface, m.NFtot, PI.face_base + f, u, rc, visc);
+    }
+    if (i <= P.NR - 2) {
+      const PairInfo PI = m.pairs[rpair_of(P, k, i)];
+      const double rc = PI.r - PI.deltaL;
+      for (int f = rb[2]; f < re[2]; ++f) add_face_L<NQ>(m, m.Fface, m.NFtot, PI.face_base + f, u, rc, visc);
+    }
     if (P.NglobalZ != 1) {
       if (k >= 1)
         faces_as_R(m, zpair_of(P, k - 1, i), j, N, [&](const PairInfo &PI, long long g) {
