@@ -1031,8 +1031,7 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
 // outer ghost NR-1 (from the new NR-2).
 // ----------------------------------------------------------------------------
 template <int NQ>
-__global__ __launch_bounds__(TILE) void k_bc_r(DevMesh m, const int2 *tiles) {
-  const int2 tl = tiles[tile_of(m)];
+__device__ __forceinline__ void bc_r_tile(const DevMesh &m, const int2 tl) {
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -1067,6 +1066,10 @@ __global__ __launch_bounds__(TILE) void k_bc_r(DevMesh m, const int2 *tiles) {
   const double dA = dz * r_face * m.dphi[c];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) m.prim[q * m.Ntot + c] = v[q] / dA;
+}
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_bc_r(DevMesh m, const int2 *tiles) {
+  bc_r_tile<NQ>(m, tiles[tile_of(m)]);
 }
 
 // z halo on one rank: periodic copy of (prim, RKcons, tiph) into the ghost
@@ -1152,15 +1155,16 @@ __global__ __launch_bounds__(TILE) void k_bc_init(DevMesh m, const int2 *tiles, 
 
 // SC_DT_FOLD: dtmin of disk d = min(dtmin, the positive tile minima below
 // 1e100) over the disk's interior tiles; every thread of the workgroup calls it.
+template <int NT>
 __device__ __forceinline__ void fold_dt(const DevMesh &m, const DiskParams &P, DiskScalars &s) {
-  __shared__ double s_m[SCALAR_THREADS / 64];
+  __shared__ double s_m[NT / 64];
   double v = 1.e100;
   // four independent loads in flight per thread
-  for (int it = P.itile0 + (int)threadIdx.x; it < P.itile1; it += 4 * SCALAR_THREADS) {
+  for (int it = P.itile0 + (int)threadIdx.x; it < P.itile1; it += 4 * NT) {
     double b[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int iu = it + u * SCALAR_THREADS;
+      const int iu = it + u * NT;
       b[u] = iu < P.itile1 ? m.dt_tile[iu] : 1.e100;
     }
 #pragma unroll
@@ -1175,7 +1179,7 @@ __device__ __forceinline__ void fold_dt(const DevMesh &m, const DiskParams &P, D
   __syncthreads();
   if (threadIdx.x == 0) {
     double b = s_m[0];
-    for (int w = 1; w < SCALAR_THREADS / 64; ++w)
+    for (int w = 1; w < NT / 64; ++w)
       if (s_m[w] < b) b = s_m[w];
     if (b < 1.e100) {
       const unsigned long long bits = (unsigned long long)__double_as_longlong(b);
@@ -1288,11 +1292,12 @@ __device__ __forceinline__ void scalar_ops(const DiskParams &P, DiskScalars &s, 
   if (ops & SC_MASS) d_mass_terms(P, s);
 }
 
-__global__ __launch_bounds__(SCALAR_THREADS) void k_scalar(DevMesh m, int ndisk, int ops, double arg) {
-  const int d = blockIdx.x;  // one workgroup per disk
+// the scalar operations of disk d, by one workgroup of NT threads
+template <int NT>
+__device__ __forceinline__ void scalar_block(const DevMesh &m, int d, int ops) {
   const DiskParams &P = m.dp[d];
   DiskScalars &s = m.sc[d];
-  if (ops & (SC_DT_FOLD | SC_STEP_BEGIN)) fold_dt(m, P, s);
+  if (ops & (SC_DT_FOLD | SC_STEP_BEGIN)) fold_dt<NT>(m, P, s);
   if (threadIdx.x == 0) {
     // on register copies: the operations' dependent chains would otherwise
     // go through global memory (the two structs may alias)
@@ -1308,7 +1313,7 @@ __global__ __launch_bounds__(SCALAR_THREADS) void k_scalar(DevMesh m, int ndisk,
     __syncthreads();
     const double a = s.gm[0].r + s.gm[1].r;
     const int na = P.NR * P.NZ;
-    for (int k = threadIdx.x; k < na; k += SCALAR_THREADS) {
+    for (int k = threadIdx.x; k < na; k += NT) {
       const AnnInfo &A = m.ann[P.ann_base + k];
       const double r = .5 * (A.rp + A.rm);
       m.ann_rot[P.ann_base + k] =
@@ -1316,6 +1321,23 @@ __global__ __launch_bounds__(SCALAR_THREADS) void k_scalar(DevMesh m, int ndisk,
                        d_dtOm_a(P.w_analytic, r, a, P.orb_tscale), d_rdrOm_a(P.w_analytic, A.rp, a));
     }
   }
+}
+__global__ __launch_bounds__(SCALAR_THREADS) void k_scalar(DevMesh m, int ndisk, int ops, double arg) {
+  scalar_block<SCALAR_THREADS>(m, blockIdx.x, ops);  // one workgroup per disk
+}
+
+// The outflow boundary's first stage and, in the workgroups past its tiles,
+// the post-update scalar operations of every disk, in one launch: the two
+// are independent (the boundary remap reads no scalar, the scalar operations
+// touch no cell) and each alone is a few microseconds of launch latency.
+// ops must not fold the CFL minimum (a fold wants SCALAR_THREADS).
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_bc_r_scalar(DevMesh m, const int2 *tiles, int ntiles, int ops) {
+  const int b = tile_of(m);
+  if (b < ntiles)
+    bc_r_tile<NQ>(m, tiles[b]);
+  else
+    scalar_block<TILE>(m, b - ntiles, ops);
 }
 
 // ----------------------------------------------------------------------------
@@ -1375,12 +1397,17 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   }
   KLAUNCH(st, "k_update",
           hipLaunchKernelGGL(k_update<NQ>, dim3(L.n_utiles), B, 0, st, m, L.utiles, fac, copy_first));
-  if (post_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, post_ops, 0.0));
   DevMesh m2 = m;  // after the update the new state lives in the *_new buffers
   m2.tiph = m.tiph_new;
   m2.prim = m.prim_new;
   m2.dphi = m.dphi_new;
-  if (L.n_bc0) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc0), B, 0, st, m2, L.bc0));
+  if (post_ops && L.n_bc0 && !(post_ops & (SC_DT_FOLD | SC_STEP_BEGIN))) {
+    KLAUNCH(st, "k_bc_r_scalar", hipLaunchKernelGGL(k_bc_r_scalar<NQ>, dim3(L.n_bc0 + L.ndisk), B, 0, st, m2, L.bc0,
+                                                    L.n_bc0, post_ops));
+  } else {
+    if (post_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, post_ops, 0.0));
+    if (L.n_bc0) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc0), B, 0, st, m2, L.bc0));
+  }
   if (L.n_bc1) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc1), B, 0, st, m2, L.bc1));
   if (L.n_binit)
     KLAUNCH(st, "k_bc_init", hipLaunchKernelGGL(k_bc_init<NQ>, dim3(L.n_binit), B, 0, st, m2, L.binit, fac));
