@@ -1030,8 +1030,30 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
 // neighbour.  stage 0: inner ghost (i = 0) and outer ghost NR-2; stage 1:
 // outer ghost NR-1 (from the new NR-2).
 // ----------------------------------------------------------------------------
+// the new primitives of cell c2 of outer ghost annulus i2 = NR-2 (stage 0
+// below: the overlap-area-weighted remap of annulus NR-3), with stage 0's
+// arithmetic, for stage 1 when both run in one launch
 template <int NQ>
-__device__ __forceinline__ void bc_r_tile(const DevMesh &m, const int2 tl) {
+__device__ __forceinline__ void outer_ghost_value(const DevMesh &m, const DiskParams &P, int k, int i2, int c2,
+                                                  double *w) {
+  const AnnInfo A2 = m.ann[P.ann_base + k * P.NR + i2];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) w[q] = 0.0;
+  faces_as_R(m, rpair_of(P, k, i2 - 1), (int)(c2 - A2.zbase), A2.N, [&](const PairInfo &PI, long long g) {
+    const int cL = m.fL[g];
+    double dA = PI.r * m.fdphi[g] * PI.deltaPerp;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) w[q] += m.prim[q * m.Ntot + cL] * dA;
+  });
+  const double dA = (A2.zp - A2.zm) * A2.rm * m.dphi[c2];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) w[q] = w[q] / dA;
+}
+
+// nested: stage 1's annulus NR-1 runs in the same launch as stage 0 and
+// recomputes the NR-2 values it reads instead of loading them
+template <int NQ>
+__device__ __forceinline__ void bc_r_tile(const DevMesh &m, const int2 tl, bool nested) {
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -1056,11 +1078,19 @@ __device__ __forceinline__ void bc_r_tile(const DevMesh &m, const int2 tl) {
     });
   } else {
     r_face = A.rm;
+    const bool from_ghost = nested && i == P.NR - 1;
     faces_as_R(m, rpair_of(P, k, i - 1), j, A.N, [&](const PairInfo &PI, long long g) {
       const int cL = m.fL[g];
       double dA = PI.r * m.fdphi[g] * PI.deltaPerp;
+      if (from_ghost) {
+        double w[NQ];
+        outer_ghost_value<NQ>(m, P, k, i - 1, cL, w);
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) v[q] += m.prim[q * m.Ntot + cL] * dA;
+        for (int q = 0; q < NQ; ++q) v[q] += w[q] * dA;
+      } else {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) v[q] += m.prim[q * m.Ntot + cL] * dA;
+      }
     });
   }
   const double dA = dz * r_face * m.dphi[c];
@@ -1069,7 +1099,7 @@ __device__ __forceinline__ void bc_r_tile(const DevMesh &m, const int2 tl) {
 }
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_bc_r(DevMesh m, const int2 *tiles) {
-  bc_r_tile<NQ>(m, tiles[tile_of(m)]);
+  bc_r_tile<NQ>(m, tiles[tile_of(m)], false);
 }
 
 // z halo on one rank: periodic copy of (prim, RKcons, tiph) into the ghost
@@ -1326,16 +1356,17 @@ __global__ __launch_bounds__(SCALAR_THREADS) void k_scalar(DevMesh m, int ndisk,
   scalar_block<SCALAR_THREADS>(m, blockIdx.x, ops);  // one workgroup per disk
 }
 
-// The outflow boundary's first stage and, in the workgroups past its tiles,
-// the post-update scalar operations of every disk, in one launch: the two
-// are independent (the boundary remap reads no scalar, the scalar operations
-// touch no cell) and each alone is a few microseconds of launch latency.
-// ops must not fold the CFL minimum (a fold wants SCALAR_THREADS).
+// The outflow boundary (both stages: annulus NR-1 recomputes the NR-2
+// values it needs) and, in the workgroups past its tiles, the post-update
+// scalar operations of every disk, in one launch: the parts are independent
+// (the boundary remap reads no scalar, the scalar operations touch no cell)
+// and each alone is a few microseconds of launch latency.  ops must not fold
+// the CFL minimum (a fold wants SCALAR_THREADS).
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_bc_r_scalar(DevMesh m, const int2 *tiles, int ntiles, int ops) {
   const int b = tile_of(m);
   if (b < ntiles)
-    bc_r_tile<NQ>(m, tiles[b]);
+    bc_r_tile<NQ>(m, tiles[b], true);
   else
     scalar_block<TILE>(m, b - ntiles, ops);
 }
@@ -1401,14 +1432,17 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   m2.tiph = m.tiph_new;
   m2.prim = m.prim_new;
   m2.dphi = m.dphi_new;
-  if (post_ops && L.n_bc0 && !(post_ops & (SC_DT_FOLD | SC_STEP_BEGIN))) {
-    KLAUNCH(st, "k_bc_r_scalar", hipLaunchKernelGGL(k_bc_r_scalar<NQ>, dim3(L.n_bc0 + L.ndisk), B, 0, st, m2, L.bc0,
-                                                    L.n_bc0, post_ops));
+  // the two boundary stages' tile lists are adjacent in the tile array
+  const bool one_launch = L.n_bc0 && L.bc1 == L.bc0 + L.n_bc0 && !(post_ops & (SC_DT_FOLD | SC_STEP_BEGIN));
+  if (one_launch) {
+    const int nt = L.n_bc0 + L.n_bc1;
+    KLAUNCH(st, "k_bc_r_scalar", hipLaunchKernelGGL(k_bc_r_scalar<NQ>, dim3(nt + (post_ops ? L.ndisk : 0)), B, 0, st,
+                                                    m2, L.bc0, nt, post_ops));
   } else {
     if (post_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, post_ops, 0.0));
     if (L.n_bc0) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc0), B, 0, st, m2, L.bc0));
+    if (L.n_bc1) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc1), B, 0, st, m2, L.bc1));
   }
-  if (L.n_bc1) KLAUNCH(st, "k_bc_r", hipLaunchKernelGGL(k_bc_r<NQ>, dim3(L.n_bc1), B, 0, st, m2, L.bc1));
   if (L.n_binit)
     KLAUNCH(st, "k_bc_init", hipLaunchKernelGGL(k_bc_init<NQ>, dim3(L.n_binit), B, 0, st, m2, L.binit, fac));
   if (L.n_sz) KLAUNCH(st, "k_sync_z", hipLaunchKernelGGL(k_sync_z<NQ>, dim3(L.n_sz), B, 0, st, m2, L.sz));
