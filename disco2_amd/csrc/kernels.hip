@@ -258,7 +258,8 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
 // nstage: LDS doubles per array for the outer annulus (0: none staged)
 // Two workgroups per CU (all 512 annulus pairs of a 512-annulus disk
 // resident at once) need 8 waves per SIMD: left alone the compiler takes 106
-// SGPRs, which allows 7, so one workgroup per CU and two rounds.
+// SGPRs, which allows 7, so one workgroup per CU and two passes over the
+// CUs.
 __global__ __launch_bounds__(FTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_faces(DevMesh m, int pair0,
                                                                                            int nstage) {
   extern __shared__ double s_stage[];  // Tout [nstage], Dout [nstage]
