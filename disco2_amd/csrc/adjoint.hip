@@ -206,8 +206,7 @@ __device__ __forceinline__ void phi_face_at(const DevMesh &m, const AdjMesh &a, 
 #ifndef ADJ_HYDRO_WAVES
 #define ADJ_HYDRO_WAVES 1
 #endif
-__global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_phi_h(DevMesh m, AdjMesh a, const int2 *tiles, int slot0) {
-  const int ti = xcd_order();
+__device__ __forceinline__ void phi_h_tile(const DevMesh &m, const AdjMesh &a, const int2 *tiles, int ti, int slot0) {
   const int2 tl = tiles[ti];
   const AnnInfo A = m.ann[tl.x];
   const DiskParams &P = disk_of(m, A.disk);
@@ -245,8 +244,7 @@ __global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_phi_h(DevMesh m, Adj
 }
 
 // phi faces backward, viscous part
-__global__ __launch_bounds__(TILE) void ka_phi_v(DevMesh m, AdjMesh a, const int2 *tiles, int slot0) {
-  const int ti = xcd_order();
+__device__ __forceinline__ void phi_v_tile(const DevMesh &m, const AdjMesh &a, const int2 *tiles, int ti, int slot0) {
   const int2 tl = tiles[ti];
   const AnnInfo A = m.ann[tl.x];
   const DiskParams &P = disk_of(m, A.disk);
@@ -320,8 +318,7 @@ __device__ __forceinline__ void r_face_at(const DevMesh &m, const AdjMesh &a, co
 // The r-face backward kernels write what each face hands its two cells into
 // per-face slots of a.fr (q-major over faces, AFR_* offsets in units of
 // NFtot); ka_plm_r gathers them per cell over the cell's face ranges.
-__global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_rflux_h(DevMesh m, AdjMesh a, const int2 *ftiles, int slot0) {
-  const int ti = xcd_order();
+__device__ __forceinline__ void rflux_h_tile(const DevMesh &m, const AdjMesh &a, const int2 *ftiles, int ti, int slot0) {
   const int2 tl = ftiles[ti];
   const int p = tl.x;
   const int f = tl.y + threadIdx.x;
@@ -354,8 +351,7 @@ __global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_rflux_h(DevMesh m, A
   reduce_store(sb, a, slot0 + ti);
 }
 
-__global__ __launch_bounds__(TILE) void ka_rflux_v(DevMesh m, AdjMesh a, const int2 *ftiles, int slot0) {
-  const int ti = xcd_order();
+__device__ __forceinline__ void rflux_v_tile(const DevMesh &m, const AdjMesh &a, const int2 *ftiles, int ti, int slot0) {
   const int2 tl = ftiles[ti];
   const int p = tl.x;
   const int f = tl.y + threadIdx.x;
@@ -384,10 +380,32 @@ __global__ __launch_bounds__(TILE) void ka_rflux_v(DevMesh m, AdjMesh a, const i
   reduce_store(sb, a, slot0 + ti);
 }
 
+// The phi faces' and the r faces' backward of one part (Riemann or viscous)
+// in one launch: workgroups [0, nu) take the phi-face tiles, the rest the
+// r-face tiles.  They are independent (the phi faces add into the cells'
+// adjoints, the r faces write per-face slots) and their scalar partials go
+// to consecutive slots from slot0, as two launches would write them.
+__global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_faces_h(DevMesh m, AdjMesh a, const int2 *utiles, int nu,
+                                                                    const int2 *ftiles, int slot0) {
+  const int b = xcd_order();
+  if (b < nu)
+    phi_h_tile(m, a, utiles, b, slot0);
+  else
+    rflux_h_tile(m, a, ftiles, b - nu, slot0 + nu);
+}
+__global__ __launch_bounds__(TILE) void ka_faces_v(DevMesh m, AdjMesh a, const int2 *utiles, int nu, const int2 *ftiles,
+                                                   int slot0) {
+  const int b = xcd_order();
+  if (b < nu)
+    phi_v_tile(m, a, utiles, b, slot0);
+  else
+    rflux_v_tile(m, a, ftiles, b - nu, slot0 + nu);
+}
+
 // r slope backward (k_plm_rz, r direction).  The cell gathers what its r
 // faces' flux adjoints hand it (primitives, r and phi slopes, its annulus
 // rotation) in the same walk that replays the slope sum; then it replays
-// the sequential limiter (on the face slopes ka_rflux_h left in Fface) and
+// the sequential limiter (on the face slopes rflux_h_tile left in Fface) and
 // hands each face its share of the slope's adjoint: dA/dAtot to every face
 // when the sum survived, PLM to the face whose S the limiter took last,
 // nothing when it was zeroed.  Each face gets one value from its L cell
@@ -862,16 +880,13 @@ void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan
   const dim3 B(TILE);
   if (L.n_bc1) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc1), B, 0, st, m, a, L.bc1);
   if (L.n_bc0) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc0), B, 0, st, m, a, L.bc0);
-  // part slots: ka_update [0, nz), ka_phi_h [nz, nz+nu), ka_rflux_h [nz+nu, nz+nu+nrf),
-  // and with viscosity ka_phi_v and ka_rflux_v after them
+  // part slots: ka_update [0, nz), the phi faces [nz, nz+nu), the r faces
+  // [nz+nu, nz+nu+nrf) (Riemann part), and with viscosity the same two
+  // ranges of the viscous part after them
   const int nz = L.n_ztiles, nu = L.n_utiles, nrf = L.n_rftiles;
   hipLaunchKernelGGL(ka_update, dim3(nz), B, 0, st, m, a, L.ztiles);
-  hipLaunchKernelGGL(ka_phi_h, dim3(nu), B, 0, st, m, a, L.utiles, nz);
-  if (nrf) hipLaunchKernelGGL(ka_rflux_h, dim3(nrf), B, 0, st, m, a, L.rftiles, nz + nu);
-  if (a.visc) {
-    hipLaunchKernelGGL(ka_phi_v, dim3(nu), B, 0, st, m, a, L.utiles, nz + nu + nrf);
-    if (nrf) hipLaunchKernelGGL(ka_rflux_v, dim3(nrf), B, 0, st, m, a, L.rftiles, nz + 2 * nu + nrf);
-  }
+  hipLaunchKernelGGL(ka_faces_h, dim3(nu + nrf), B, 0, st, m, a, L.utiles, nu, L.rftiles, nz);
+  if (a.visc) hipLaunchKernelGGL(ka_faces_v, dim3(nu + nrf), B, 0, st, m, a, L.utiles, nu, L.rftiles, nz + nu + nrf);
   hipLaunchKernelGGL(ka_plm_r, dim3(nz), B, 0, st, m, a, L.ztiles);
   hipLaunchKernelGGL(ka_slope, dim3(nz), B, 0, st, m, a, L.ztiles);
   hipLaunchKernelGGL(ka_plm_p, dim3(nz), B, 0, st, m, a, L.ztiles);
