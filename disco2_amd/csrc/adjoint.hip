@@ -730,8 +730,12 @@ __global__ __launch_bounds__(TILE) void ka_dt(DevMesh m, AdjMesh a, const int2 *
 
 // sum of n tile partials into *out (one workgroup)
 #define REDUCE_THREADS 1024
+// workgroup b sums the n slots of partial buffer b (SA_FIELDS rows of npart)
+// into out[b]
 __global__ __launch_bounds__(REDUCE_THREADS) void ka_reduce(const double *part, long long npart, int n,
                                                              ScalarAdj *out) {
+  part += (long long)blockIdx.x * SA_FIELDS * npart;
+  out += blockIdx.x;
   __shared__ double sh[SA_FIELDS][REDUCE_THREADS / 64];
   double v[SA_FIELDS];
 #pragma unroll
@@ -875,8 +879,7 @@ __global__ __launch_bounds__(TILE) void ka_chain_acc(int nann, const DiskScalars
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
-                            ScalarAdj *out) {
+void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st) {
   const dim3 B(TILE);
   if (L.n_bc1) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc1), B, 0, st, m, a, L.bc1);
   if (L.n_bc0) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc0), B, 0, st, m, a, L.bc0);
@@ -890,7 +893,10 @@ void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan
   hipLaunchKernelGGL(ka_plm_r, dim3(nz), B, 0, st, m, a, L.ztiles);
   hipLaunchKernelGGL(ka_slope, dim3(nz), B, 0, st, m, a, L.ztiles);
   hipLaunchKernelGGL(ka_plm_p, dim3(nz), B, 0, st, m, a, L.ztiles);
-  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(REDUCE_THREADS), 0, st, (const double *)a.part, a.npart,
+}
+void launch_adjoint_reduce(const AdjMesh &a, const LaunchPlan &L, hipStream_t st, int nbuf, ScalarAdj *out) {
+  const int nz = L.n_ztiles, nu = L.n_utiles, nrf = L.n_rftiles;
+  hipLaunchKernelGGL(ka_reduce, dim3(nbuf), dim3(REDUCE_THREADS), 0, st, (const double *)a.part, a.npart,
                      a.visc ? nz + 2 * nu + 2 * nrf : nz + nu + nrf, out);
 }
 

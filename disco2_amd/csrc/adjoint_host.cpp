@@ -197,7 +197,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     screc = ar.take<DiskScalars>((size_t)3 * every);
     ckpt_sc = ar.take<DiskScalars>((size_t)nseg);
     fin_sc = ar.take<DiskScalars>(1);
-    part = ar.take<double>(9 * npart);
+    part = ar.take<double>(2 * 9 * npart);  // the two substeps' partial sums
     pick = ar.take<int>(1);
     for (auto &r : rec) {
       r.prim = ar.take<double>(nq);
@@ -359,7 +359,8 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
       a.aPn = aPn;
       a.fac = 0.5;
       a.copy_first = 0;
-      launch_adjoint_substep(m2, a, plan, st, out + 0);
+      a.part = part;
+      launch_adjoint_substep(m2, a, plan, st);
       launch_adjoint_rot(m2, a, plan, st, th2);
       std::swap(aP, aPn);
       // substep 1 (fac 1, RKcons = prim2cons(prim))
@@ -368,8 +369,12 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
       a.aPn = aPn;
       a.fac = 1.0;
       a.copy_first = 1;
-      launch_adjoint_substep(m1, a, plan, st, out + 1);
+      a.part = part + 9 * npart;
+      launch_adjoint_substep(m1, a, plan, st);
       std::swap(aP, aPn);
+      // both substeps' scalar adjoints, out[0] and out[1], in one launch
+      a.part = part;
+      launch_adjoint_reduce(a, plan, st, 2, out);
       // the step's dt: its CFL minimum, at the step-start state with the
       // scalars k_mindt saw
       launch_adjoint_chain_dt(m1, nann, scA, scB, out + 1, out + 0, th2, Ltheta, lamT, cs, st);

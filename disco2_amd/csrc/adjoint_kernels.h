@@ -40,8 +40,12 @@ struct AdjChain {
   double Lphi[2], Lr[2], LM[2], Lalpha, lam_dt;
 };
 
-void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
-                            ScalarAdj *out);
+// the backward kernels of one substep; their per-tile partial sums of the
+// scalar adjoints go to a.part (launch_adjoint_reduce sums them)
+void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st);
+// out[b] = the sum of the substep partials in a.part + b * 9 * a.npart, for
+// b < nbuf, one workgroup each
+void launch_adjoint_reduce(const AdjMesh &a, const LaunchPlan &L, hipStream_t st, int nbuf, ScalarAdj *out);
 // pick: device int scratch for the index of the cell that sets dt; dtbits:
 // the step's CFL minimum (device)
 void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
