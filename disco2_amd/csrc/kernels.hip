@@ -339,28 +339,34 @@ __global__ __launch_bounds__(64) void k_set_w(DevMesh m, int ann0) {
 // device-wide atomics: thousands of them on one word serialise); k_scalar's
 // SC_DT_FOLD reduces a disk's tiles.  min is exact, so the order is free.
 // ----------------------------------------------------------------------------
+// the CFL step of cell c (local index j) of annulus A, tile tl, from its
+// primitives pr and width dp
+__device__ __forceinline__ double cell_dt_at(const DevMesh &m, const int2 tl, const AnnInfo &A, long long c,
+                                             const double *pr, double dp) {
+  const DiskParams &P = disk_of(m, A.disk);
+  const double rm = A.rm, rp = A.rp;
+  const double dz = A.zp - A.zm;
+  const double dr = rp - rm;
+  const double r = .5 * (rp + rm);
+  const double wj = m.wiph[tl.x];
+  double w = .5 * (wj + wj);
+  double dx = dr;
+  double rdphi = .5 * (rp + rm) * dp;
+  if (rdphi < dr) dx = rdphi;
+  if (dx > dz) dx = dz;
+  const double tc = (P.visc > 0.0 && P.visc_const != 1) ? clean_pi(m.tiph[c]) - 0.5 * dp : 0.0;
+  return cell_dt_value(P, m.sc[A.disk], pr, r, dx, w, tc);
+}
 template <int NQ>
 __device__ __forceinline__ double cell_dt(const DevMesh &m, const int2 tl, const AnnInfo &A) {
   const int j = tl.y + threadIdx.x;
-  const DiskParams &P = disk_of(m, A.disk);
   double dt = 1.e100;
   if (j < A.N) {
     const long long c = A.zbase + j;
-    const double rm = A.rm, rp = A.rp;
-    const double dz = A.zp - A.zm;
-    const double dr = rp - rm;
-    const double r = .5 * (rp + rm);
-    const double wj = m.wiph[tl.x];
-    double w = .5 * (wj + wj);
-    double dx = dr;
-    double rdphi = .5 * (rp + rm) * m.dphi[c];
-    if (rdphi < dr) dx = rdphi;
-    if (dx > dz) dx = dz;
     double pr[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) pr[q] = m.prim[q * m.Ntot + c];
-    const double tc = (P.visc > 0.0 && P.visc_const != 1) ? clean_pi(m.tiph[c]) - 0.5 * m.dphi[c] : 0.0;
-    double d = cell_dt_value(P, m.sc[A.disk], pr, r, dx, w, tc);
+    double d = cell_dt_at(m, tl, A, c, pr, m.dphi[c]);
     if (d < dt) dt = d;
   }
   return dt;
@@ -403,18 +409,27 @@ __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
   const int2 tl = tiles[tile_of(m)];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
+  double dtc = 1.e100;  // this cell's CFL step (DT, interior tiles)
   if (j < A.N) {
     const double PLM = m.dp[A.disk].plm;
     const int N = A.N;
     const long long c = A.zbase + j;
     const long long cL = A.zbase + (j == 0 ? N - 1 : j - 1);
     const long long cR = A.zbase + (j == N - 1 ? 0 : j + 1);
-    const double dpL = .5 * (m.dphi[c] + m.dphi[cL]);
-    const double dpR = .5 * (m.dphi[c] + m.dphi[cR]);
+    const double dpc = m.dphi[c];
+    const double dpL = .5 * (dpc + m.dphi[cL]);
+    const double dpR = .5 * (dpc + m.dphi[cR]);
+    double pr[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) pr[q] = m.prim[q * m.Ntot + c];
+    if (DT && A.itile >= 0) {
+      const double d = cell_dt_at(m, tl, A, c, pr, dpc);
+      if (d < dtc) dtc = d;  // as cell_dt: a NaN or huge step counts as 1e100
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const double *pq = m.prim + q * m.Ntot;
-      double vc = pq[c], vL = pq[cL], vR = pq[cR];
+      double vc = pr[q], vL = pq[cL], vR = pq[cR];
       double sL = (vc - vL) / dpL;
       double sR = (vR - vc) / dpR;
       double sM = (vR - vL) / (dpL + dpR);
@@ -425,7 +440,7 @@ __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
       m.gradp[q * m.Ntot + c] = s;
     }
   }
-  if (DT && A.itile >= 0) store_tile_min(m, cell_dt<NQ>(m, tl, A), A.itile + tl.y / TILE);
+  if (DT && A.itile >= 0) store_tile_min(m, dtc, A.itile + tl.y / TILE);
 }
 
 // ----------------------------------------------------------------------------
