@@ -40,6 +40,8 @@ def cases():
     c["vortex_scalar_hll_noinner"] = ((p, k, dict(kw, x0=1.8)), dict(steps=12, dumps=[1, 12], unit=True))
     p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=3, PLM=2.0)
     c["vortex_three_scalars"] = ((p, k, dict(kw, x0=1.8)), dict(steps=8, dumps=[1, 8], unit=True))
+    p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=2, PLM=1.5)
+    c["vortex_two_scalars"] = ((p, k, dict(kw, x0=1.6)), dict(steps=8, dumps=[1, 8], unit=True))
     c["binary_r16"] = (configs.binary(16, r_min=0.3, r_max=3.0), dict(steps=20, dumps=[1, 2, 20], unit=True))
     p, k, kw = configs.binary(16, r_min=0.3, r_max=3.0, q=0.5, w_numeric_type=0, w_analytic_type=1,
                               RLogScale=1.0, RhoSinkOn=1, ViscConst=0)
