@@ -338,9 +338,14 @@ void cell_calc_prim(Cell ***c, Sim *s) {
   mirror_push(h);
 }
 
+// src/Cell/cell_misc.c:134-143.  The next step recomputes wiph (the engine's
+// cached mesh velocity is dropped), as the reference's timestep_rk2 always
+// runs cell_set_w after this.
 void cell_clear_w(Cell ***c, Sim *) {
   CellsHandle *h = H(c);
+  mirror_push(h);
   hipMemsetAsync(h->eng->d_wiph, 0, sizeof(double) * h->eng->nann, h->eng->st);
+  h->eng->w_ready = false;
   h->stale = true;
 }
 void cell_set_w(Cell ***c, Sim *) {

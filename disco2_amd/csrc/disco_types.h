@@ -39,6 +39,7 @@ struct DiskParams {
   long long zone_base;
   int NL;                     // zones per layer
   int itile0, itile1;         // this disk's range in the interior (CFL) tile list
+  int utile0, utile1;         // this disk's range in the update tile list
   int w_numeric, w_analytic, riemann, visc_const, grav2d, sink, phi_order, cooling;
   int initial_data_type, grav_type, bound_r, no_inner_bc;
   double gamma, cfl, plm, visc, g_eps, rho_floor, cs_floor, cs_cap, vel_cap;

@@ -82,6 +82,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     P.zone_base = zb;
     P.ann_base = (int)hann.size();
     P.itile0 = (int)itiles.size();
+    P.utile0 = (int)utiles.size();
     zbase.push_back(zb);
     nzones.push_back(NL * NZ);
     // fixed r boundaries and damping: rows of the single-cell initial data
@@ -123,6 +124,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
           for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) bit.push_back(make_int2(a, j0));
       }
     P.itile1 = (int)itiles.size();
+    P.utile1 = (int)utiles.size();
     zb += NL * NZ;
     // r pairs (k, i)-(k, i+1): constants of build_jloop, src/Face/face_create_destroy.c:60-68
     P.rpair_base = (int)hpairs.size();
@@ -212,6 +214,9 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   good &= dalloc(&d_err, 1, "err");
   good &= dalloc(&d_dt_tile, itiles.empty() ? 1 : itiles.size(), "dt tiles");
   good &= dalloc(&d_ann_rot, (size_t)nann, "annulus rotation");
+  bool any_sink = false;
+  for (auto &P : hdp) any_sink |= P.sink == 1;
+  if (any_sink) good &= dalloc(&d_mdot_tile, 2 * utiles.size(), "sink tiles");
   if (any_init) good &= dalloc(&d_init, hinit.size(), "init rows") && h2d(d_init, hinit.data(), hinit.size());
   if (!good) return;
   hipMemset(d_err, 0, sizeof(int));
@@ -323,7 +328,7 @@ Engine::~Engine() {
   if (graph) hipGraphDestroy(graph);
   void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
-                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_ann_rot, d_init, d_pair_perm, adj_mem, diag_mem,
+                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_mdot_tile, d_ann_rot, d_init, d_pair_perm, adj_mem, diag_mem,
                   ckpt_mem};
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -364,6 +369,7 @@ DevMesh Engine::mesh() const {
   m.NFtot = NFtot;
   m.err = d_err;
   m.dt_tile = d_dt_tile;
+  m.mdot_tile = d_mdot_tile;
   m.ann_rot = d_ann_rot;
   m.init = d_init;
   m.sched_xcd = sched.xcd;
@@ -551,6 +557,7 @@ void Engine::run(int nsteps, bool use_graph) {
     cur = cur0;  // capture does not execute; a step swaps the edge buffers twice
     graph_cur = cur;
   }
+  if (use_graph && w_static) enqueue_set_w();  // the graph holds no set_w; a no-op once wiph is set
   hip_check(hipEventRecord(ev0, st), "event");
   for (int s = 0; s < nsteps; ++s) {
     if (use_graph)

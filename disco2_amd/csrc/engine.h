@@ -46,6 +46,7 @@ struct Engine {
   double *d_Fface = nullptr;
   int *d_err = nullptr;
   double *d_dt_tile = nullptr;
+  double *d_mdot_tile = nullptr;  // sink accretion per update tile (when some disk has RhoSinkOn)
   double4 *d_ann_rot = nullptr;
   InitRow *d_init = nullptr;  // fixed r boundaries and damping, per annulus (null when no disk has them)
   int2 *d_tiles = nullptr;

@@ -24,6 +24,7 @@ struct DevMesh {
   int *run_start;
   double *Fface;          // [NQ+3][NFtot] increments of the r/z overlap faces
   double *dt_tile;        // CFL minimum of each interior tile (k_mindt), reduced per disk by k_scalar
+  double *mdot_tile;      // sink: per update tile, the two masses' accretion (2 per tile; null without sinks)
   double4 *ann_rot;       // per annulus: rOm_a, rdrOm_a, dtOm_a at the centre, rdrOm_a at the outer face
   const InitRow *init;    // per annulus: fixed r boundaries and damping (null when unused)
   // adjoint recording (null in plain runs): k_update stores the conserved

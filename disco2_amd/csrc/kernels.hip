@@ -1018,7 +1018,9 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
     __syncthreads();
   }
   if (P.sink == 1) {
-    // Mdot partial sums of the block (one annulus, one disk)
+    // the tile's accretion of each mass, summed in a fixed order into the
+    // tile's own slot; k_scalar folds a disk's slots in tile order, so Mdot
+    // does not depend on the order in which workgroups run
     for (int off = 32; off > 0; off >>= 1) {
       md0 += __shfl_down(md0, off);
       md1 += __shfl_down(md1, off);
@@ -1034,8 +1036,8 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
         t0 += s_md[0][w];
         t1 += s_md[1][w];
       }
-      atomicAdd(&m.sc[A.disk].mdot_acc[0], t0);
-      atomicAdd(&m.sc[A.disk].mdot_acc[1], t1);
+      m.mdot_tile[2 * ti] = t0;
+      m.mdot_tile[2 * ti + 1] = t1;
     }
   }
 }
@@ -1338,12 +1340,45 @@ __device__ __forceinline__ void scalar_ops(const DiskParams &P, DiskScalars &s, 
   if (ops & SC_MASS) d_mass_terms(P, s);
 }
 
+// SC_AFTER_UPDATE with a sink: mdot_acc of disk d = the sum of its update
+// tiles' accretion slots (k_update), in a fixed order: thread t adds tiles
+// t, t + NT, ... in turn, then the waves' sums are added in order.
+template <int NT>
+__device__ __forceinline__ void fold_mdot(const DevMesh &m, const DiskParams &P, DiskScalars &s) {
+  __shared__ double s_a[2][NT / 64];
+  double v0 = 0.0, v1 = 0.0;
+  for (int it = P.utile0 + (int)threadIdx.x; it < P.utile1; it += NT) {
+    v0 += m.mdot_tile[2 * it];
+    v1 += m.mdot_tile[2 * it + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_down(v0, off);
+    v1 += __shfl_down(v1, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_a[0][threadIdx.x >> 6] = v0;
+    s_a[1][threadIdx.x >> 6] = v1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int w = 0; w < NT / 64; ++w) {
+      t0 += s_a[0][w];
+      t1 += s_a[1][w];
+    }
+    s.mdot_acc[0] = t0;
+    s.mdot_acc[1] = t1;
+  }
+  __syncthreads();
+}
+
 // the scalar operations of disk d, by one workgroup of NT threads
 template <int NT>
 __device__ __forceinline__ void scalar_block(const DevMesh &m, int d, int ops) {
   const DiskParams &P = m.dp[d];
   DiskScalars &s = m.sc[d];
   if (ops & (SC_DT_FOLD | SC_STEP_BEGIN)) fold_dt<NT>(m, P, s);
+  if ((ops & SC_AFTER_UPDATE) && P.sink == 1) fold_mdot<NT>(m, P, s);
   if (threadIdx.x == 0) {
     // on register copies: the operations' dependent chains would otherwise
     // go through global memory (the two structs may alias)
