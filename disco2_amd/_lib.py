@@ -115,6 +115,7 @@ SIGNATURES = {
     "disco_batch_profile": (ci, [vp, ci, ctypes.c_char_p, pd, pi, ci]),
     "disco_calibrate_copy": (cd, [cll]),
     "disco_batch_torque_loss": (ci, [vp, ci, pd]),
+    "disco_batch_torque_loss_noalloc": (ci, [vp, ci, pd]),
     "disco_batch_adjoint": (ci, [vp, ci, ci, pd, pd, pd, pd]),
     "disco_batch_diag_sizes": (ci, [vp, ci, pi, pi]),
     "disco_batch_diag_measure": (ci, [vp, ci, cd, pd, pd, pd, pd, pd]),

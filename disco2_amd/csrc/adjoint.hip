@@ -10,6 +10,8 @@
 // bitwise reproducible run to run; it does not need to be).  Launch-uniform
 // scalars (dt, alpha, the masses and their positions) are summed per tile
 // and reduced per substep.
+#include <mutex>
+
 #include "adjoint.h"
 #include "adjoint_kernels.h"
 #include "device_common.h"
@@ -934,4 +936,33 @@ void launch_adjoint_chain_acc(int nann, const DiskScalars *pre, const DiskScalar
                               const ScalarAdj *S0, const ScalarAdj *S1, const ScalarAdj *S2, const double *th1,
                               const double *th2, double *Ltheta, AdjChain *cs, hipStream_t st) {
   hipLaunchKernelGGL(ka_chain_acc, dim3(1), dim3(TILE), 0, st, nann, pre, A, B, S0, S1, S2, th1, th2, Ltheta, cs);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free torque loss (disco_batch_torque_loss_noalloc):
+// slot 0 is the loss accumulator, slots 1.. the per-tile torque sums that
+// k_update writes.  One copy per GPU, held until the process ends; every
+// call on a GPU shares it, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[TORQUE_SCRATCH_DOUBLES];
+
+__global__ __launch_bounds__(64) void ka_zero(double *p, int n) {
+  for (int i = threadIdx.x; i < n; i += 64) p[i] = 0.0;
+}
+void launch_zero(double *p, int n, hipStream_t st) { hipLaunchKernelGGL(ka_zero, dim3(1), dim3(64), 0, st, p, n); }
+
+// The device address of this GPU's scratch (the array's name is only a
+// host-side stand-in): looked up once per device, with that device current.
+double *torque_scratch_device() {
+  static std::mutex mu;
+  static double *addr[TORQUE_SCRATCH_MAX_DEVICES] = {};
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TORQUE_SCRATCH_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+    addr[dev] = (double *)p;
+  }
+  return addr[dev];
 }

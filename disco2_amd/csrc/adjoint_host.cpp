@@ -128,6 +128,33 @@ int Engine::torque_loss(int nsteps, double *loss) {
   return check_error() ? -1 : 0;
 }
 
+// torque_loss without device allocation: slot 0 of the fixed scratch is the
+// accumulator (cleared by a fill kernel), slots 1.. the tile sums, which
+// every k_update tile overwrites before ka_sum reads them.
+int Engine::torque_loss_noalloc(int nsteps, double *loss) {
+  if (!ok || !adjoint_scope()) return -1;
+  if ((long long)plan.n_utiles + 1 > TORQUE_SCRATCH_DOUBLES) {
+    disco_set_error(16, "torque_loss_noalloc: " + std::to_string(plan.n_utiles) +
+                            " update tiles exceed the fixed scratch of " +
+                            std::to_string(TORQUE_SCRATCH_DOUBLES - 1));
+    return -1;
+  }
+  double *scr = torque_scratch_device();
+  if (!scr) {
+    disco_set_error(16, "torque_loss_noalloc: no device address for the fixed scratch");
+    return -1;
+  }
+  double *acc = scr, *torque = scr + 1;
+  launch_zero(acc, 1, st);
+  const double t0 = get_scalars(0).t;
+  for (int n = 0; n < nsteps; ++n) adjoint_step(nullptr, nullptr, torque, acc);
+  const double T = get_scalars(0).t - t0;
+  double I = 0.0;
+  hip_check(hipMemcpy(&I, acc, 8, hipMemcpyDeviceToHost), "torque D2H");
+  *loss = T > 0.0 ? I / T : 0.0;
+  return check_error() ? -1 : 0;
+}
+
 namespace {
 
 // device buffers of one adjoint run, carved from one allocation

@@ -63,3 +63,12 @@ void launch_adjoint_chain_dt(const DevMesh &m, int nann, const DiskScalars *A, c
 void launch_adjoint_chain_acc(int nann, const DiskScalars *pre, const DiskScalars *A, const DiskScalars *B,
                               const ScalarAdj *S0, const ScalarAdj *S1, const ScalarAdj *S2, const double *th1,
                               const double *th2, double *Ltheta, AdjChain *cs, hipStream_t st);
+
+// Fixed device scratch of the allocation-free torque loss: the accumulator
+// and one slot per k_update tile (512 KiB: batches of up to 65,535 update
+// tiles, about 16.7M zones), on up to 64 GPUs per process.
+#define TORQUE_SCRATCH_DOUBLES (1 << 16)
+#define TORQUE_SCRATCH_MAX_DEVICES 64
+// this GPU's scratch address (null on error); p[0, n) = 0 on the stream
+double *torque_scratch_device();
+void launch_zero(double *p, int n, hipStream_t st);

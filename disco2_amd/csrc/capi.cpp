@@ -548,6 +548,9 @@ void disco_batch_synchronize(DiscoBatch *b) {
 }
 double disco_batch_last_ms(DiscoBatch *b) { return b->e->last_ms; }
 int disco_batch_torque_loss(DiscoBatch *b, int nsteps, double *loss) { return b->e->torque_loss(nsteps, loss); }
+int disco_batch_torque_loss_noalloc(DiscoBatch *b, int nsteps, double *loss) {
+  return b->e->torque_loss_noalloc(nsteps, loss);
+}
 int disco_batch_adjoint(DiscoBatch *b, int nsteps, int every, double *loss, double *g_alpha, double *g_q,
                         double *g_prim0) {
   return b->e->adjoint(nsteps, every, loss, g_alpha, g_q, g_prim0);

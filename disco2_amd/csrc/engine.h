@@ -104,6 +104,9 @@ struct Engine {
   void adjoint_redirect(const AdjSubRec &r);  // the next substep writes into r
   void adjoint_restore(const double *src, bool rk);
   int torque_loss(int nsteps, double *loss);
+  // torque_loss with its accumulator and tile sums in the library's fixed
+  // device scratch: no device allocation (adjoint_host.cpp)
+  int torque_loss_noalloc(int nsteps, double *loss);
 
   // diagnostics (diagnostics.cpp): one measurement of diagnostics_set on disk
   // d (src/Diagnostics/diagnostics_set.c:13-303): vec [NumR][17], scal[16],

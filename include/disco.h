@@ -325,6 +325,15 @@ double disco_calibrate_copy(long long n);
 int disco_batch_torque_loss(struct DiscoBatch *, int nsteps, double *loss);
 int disco_batch_adjoint(struct DiscoBatch *, int nsteps, int checkpoint_every, double *loss, double *grad_alpha,
                         double *grad_q, double *grad_prim0);
+/* No reference counterpart: disco_batch_torque_loss for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same loss, same kernels, same bits; its
+ * accumulator and per-tile sums live in a fixed device array of the library
+ * (512 KiB per GPU, held until the process ends), so it calls no hipMalloc or
+ * hipFree.  Limit: batches of at most 65,535 update tiles (255 cells each,
+ * about 16.7M zones); larger ones set the error and return -1.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU race. */
+int disco_batch_torque_loss_noalloc(struct DiscoBatch *, int nsteps, double *loss);
 
 /* Diagnostics of one disk of a batch (no reference counterpart: the
  * measurement of diagnostics_set, src/Diagnostics/diagnostics_set.c:13-303,
