@@ -872,26 +872,19 @@ void io_hdf5_in(IO *io, Sim *sim, TimeStep *ts) {
 }
 
 // launch schedules (no reference counterpart)
-int disco_batch_autotune(DiscoBatch *b, int nsteps, double *ms8) {
-  double tmp[8];
-  return b->e->autotune(nsteps, ms8 ? ms8 : tmp, 8);
+int disco_batch_autotune(DiscoBatch *b, int nsteps, double *ms) {
+  double tmp[DISCO_NSCHEDULES];
+  return b->e->autotune(nsteps, ms ? ms : tmp, DISCO_NSCHEDULES);
 }
 int disco_batch_set_schedule(DiscoBatch *b, int k) {
-  if (k < 0 || k > 7) {
-    disco_set_error(9, "schedule: 0 to 7");
+  if (k < 0 || k >= DISCO_NSCHEDULES) {
+    disco_set_error(9, "schedule: 0 to DISCO_NSCHEDULES - 1");
     return -1;
   }
-  Engine::Schedule s;
-  s.xcd = k & 1;
-  s.largest_first = (k >> 1) & 1;
-  s.stage = (k >> 2) & 1;
-  b->e->set_schedule(s);
+  b->e->set_schedule(Engine::Schedule::of(k));
   return 0;
 }
-int disco_batch_schedule(DiscoBatch *b) {
-  const Engine::Schedule &s = b->e->sched;
-  return s.xcd | s.largest_first << 1 | s.stage << 2;
-}
+int disco_batch_schedule(DiscoBatch *b) { return b->e->sched.code(); }
 
 // the InitialDataType plugin over host arrays of one disk (host only)
 int disco_initial_prim(Sim *s, const double *tiph, const double *dphi, double *prim) {

@@ -85,8 +85,9 @@ __device__ __forceinline__ void faces_as_R(const DevMesh &m, int p, int jo, int 
 
 // analytic frame rotation of an annulus (filled by k_scalar whenever the
 // masses move; zero for the non-rotating frame)
+template <int FEAT = F_ALL>
 __device__ __forceinline__ double4 ann_rotation(const DevMesh &m, const DiskParams &P, int a) {
-  if (P.w_analytic == A_FIXED) return make_double4(0.0, 0.0, 0.0, 0.0);
+  if (!(FEAT & F_FRAME) || P.w_analytic == A_FIXED) return make_double4(0.0, 0.0, 0.0, 0.0);
   return m.ann_rot[a];
 }
 

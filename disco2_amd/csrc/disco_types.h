@@ -22,6 +22,21 @@ enum { HLLC_SOLVER = 0, HLL_SOLVER = 1 };
 // InitialDataType, src/Headers/header.h:8
 enum { ID_SHEAR, ID_VORTEX, ID_SSTEST, ID_TORUS, ID_MILOS_MACFADYEN, ID_RAD_DOM, ID_MIDDLE };
 
+// Physics paths a compiled kernel variant keeps.  The batch's launch plan
+// picks the smallest variant holding every path its disks take; a cleared
+// bit removes that path from the code (and its registers), a set bit keeps
+// the disk's runtime choice.  Every variant does the same arithmetic on the
+// paths it keeps.
+enum {
+  F_VISC = 1,   // explicit viscosity (riemann_visc_flux, cell_add_visc_src, viscous CFL)
+  F_HLLC = 2,   // RiemannSolver 0
+  F_HLL = 4,    // RiemannSolver 1
+  F_SINK = 8,   // RhoSinkOn
+  F_COOL = 16,  // Cooling
+  F_FRAME = 32, // w_analytic_type != 0 (frame rotation terms, split fictitious forces)
+  F_ALL = 63
+};
+
 // sticky device error codes
 enum {
   DERR_NONE = 0,
@@ -82,6 +97,9 @@ struct AnnInfo {
   int N;             // N_p
   int disk, k, i;
   int itile;         // its first tile in the interior (CFL) tile list; -1 for ghost annuli and layers
+  int utile;         // its first slot in the update tile list (table order; the slot of a tile
+                     // does not depend on the order in which a schedule lists the tiles)
+  int pad_;
   double rm, rp, zm, zp;  // face positions bounding the annulus
 };
 

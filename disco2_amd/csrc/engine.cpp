@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <tuple>
 #include <string>
 
 #include "../../include/disco.h"
@@ -106,6 +107,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
         A.zp = S->z_faces[k + 1];
         bool interior = i >= P.NgMinR && i < NR - P.NgMaxR && k >= P.NgMinZ && k < NZ - P.NgMaxZ;
         A.itile = interior ? (int)itiles.size() : -1;
+        A.utile = (int)utiles.size();
         int a = (int)hann.size();
         hann.push_back(A);
         for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) ztiles.push_back(make_int2(a, j0));
@@ -247,6 +249,24 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     h2d(d_dt_tile, none.data(), none.size());
   }
 
+  // Column order (a schedule option): the tiles of each disk's annuli
+  // listed annulus column by column (i outer, k inner) instead of layer by
+  // layer, so a 3D cell's z neighbours are processed next to it; the face
+  // tiles of pairs the same way.  2D disks keep their order.
+  auto zone_key = [&](const int2 &t) {
+    const AnnInfo &A = hann[t.x];
+    return std::make_tuple(A.disk, A.i, A.k);
+  };
+  auto face_key = [&](const int2 &t) {
+    const AnnInfo &A = hann[hpairs[t.x].ann_in];
+    return std::make_tuple(A.disk, A.i, A.k);
+  };
+  std::vector<int2> ztiles_c = ztiles, utiles_c = utiles, rft_c = rft, zft_c = zft;
+  auto by = [](auto key) { return [key](const int2 &a, const int2 &b) { return key(a) < key(b); }; };
+  std::stable_sort(ztiles_c.begin(), ztiles_c.end(), by(zone_key));
+  std::stable_sort(utiles_c.begin(), utiles_c.end(), by(zone_key));
+  std::stable_sort(rft_c.begin(), rft_c.end(), by(face_key));
+  std::stable_sort(zft_c.begin(), zft_c.end(), by(face_key));
   // one tile array on the device, sliced per list
   std::vector<int2> all;
   auto add = [&](std::vector<int2> &v, int &count) {
@@ -258,6 +278,8 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   size_t o_z = add(ztiles, plan.n_ztiles), o_u = add(utiles, plan.n_utiles), o_i = add(itiles, plan.n_itiles), o_r = add(rft, plan.n_rftiles),
          o_zf = add(zft, plan.n_zftiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
          o_s = add(szt, plan.n_sz), o_bi = add(bit, plan.n_binit);
+  int nc;
+  const size_t o_zc = add(ztiles_c, nc), o_uc = add(utiles_c, nc), o_rc = add(rft_c, nc), o_zfc = add(zft_c, nc);
   if (!dalloc(&d_tiles, all.size(), "tiles") || !h2d(d_tiles, all.data(), all.size())) return;
   plan.ztiles = d_tiles + o_z;
   plan.utiles = d_tiles + o_u;
@@ -268,9 +290,23 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   plan.bc1 = d_tiles + o_b1;
   plan.sz = d_tiles + o_s;
   plan.binit = d_tiles + o_bi;
+  tiles_row[0] = plan.ztiles;
+  tiles_row[1] = plan.utiles;
+  tiles_row[2] = plan.rftiles;
+  tiles_row[3] = plan.zftiles;
+  tiles_col[0] = d_tiles + o_zc;
+  tiles_col[1] = d_tiles + o_uc;
+  tiles_col[2] = d_tiles + o_rc;
+  tiles_col[3] = d_tiles + o_zfc;
   plan.ndisk = n;
   plan.any_visc = 0;
-  for (auto &P : hdp) plan.any_visc |= P.visc > 0.0;
+  plan.feat = 0;
+  for (auto &P : hdp) {
+    plan.any_visc |= P.visc > 0.0;
+    plan.feat |= (P.visc > 0.0 ? F_VISC : 0) | (P.riemann == HLL_SOLVER ? F_HLL : F_HLLC) |
+                 (P.sink == 1 ? F_SINK : 0) | (P.cooling == 1 ? F_COOL : 0) |
+                 (P.w_analytic != A_FIXED ? F_FRAME : 0);
+  }
   plan.nann = nann;
   {  // largest outer annulus of each pair direction, staged in LDS if it fits
     int mr = 0, mz = 0;
@@ -570,6 +606,12 @@ void Engine::run(int nsteps, bool use_graph) {
 
 void Engine::set_schedule(const Schedule &s) {
   sched = s;
+  plan.fuse = s.fuse;
+  const int2 *const *tl = s.col ? tiles_col : tiles_row;
+  plan.ztiles = tl[0];
+  plan.utiles = tl[1];
+  plan.rftiles = tl[2];
+  plan.zftiles = tl[3];
   plan.stage_r = s.stage ? stage_r_max : 0;
   plan.stage_z = s.stage ? stage_z_max : 0;
   if (gexec) {  // the next graph run captures the new launches
@@ -601,36 +643,54 @@ int Engine::autotune(int nsteps, double *ms, int nms) {
     }
     t.put_scalars(d, get_scalars(d));
   }
-  double best = 1e300;
-  int bestk = -1;
-  for (int k = 0; k < 8; ++k) {
-    Schedule s;
-    s.xcd = k & 1;
-    s.largest_first = (k >> 1) & 1;
-    s.stage = (k >> 2) & 1;
-    if (s.stage && !stage_r_max && !stage_z_max) {  // nothing to stage: the same launches as k - 4
-      if (k < nms) ms[k] = -1.0;
-      continue;
-    }
-    t.set_schedule(s);
+  for (int k = 0; k < nms; ++k) ms[k] = -1.0;
+  auto time_k = [&](int k) {
+    t.set_schedule(Schedule::of(k));
     t.run(2, true);
     t.synchronize();
     t.run(nsteps, true);
     t.synchronize();
     const double v = t.last_ms / nsteps;
     if (k < nms) ms[k] = v;
+    return v;
+  };
+  // the launch orders first (faces stored), then the fused face
+  // evaluations under the fastest order
+  double best = 1e300;
+  int bestk = -1;
+  for (int k0 = 0; k0 < 16; ++k0) {
+    const int k = (k0 & 7) | (k0 >> 3) << 5;  // xcd, largest_first, stage, col
+    if (!schedule_distinct(k)) continue;  // the same launches as another candidate
+    const double v = time_k(k);
     if (v < best) {
       best = v;
       bestk = k;
     }
   }
   if (bestk < 0) return -1;
-  Schedule s;
-  s.xcd = bestk & 1;
-  s.largest_first = (bestk >> 1) & 1;
-  s.stage = (bestk >> 2) & 1;
-  set_schedule(s);
+  const int order = bestk;
+  for (int f = 1; f < 4; ++f) {
+    const int k = order | f << 3;
+    if (!schedule_distinct(k)) continue;
+    const double v = time_k(k);
+    if (v < best) {
+      best = v;
+      bestk = k;
+    }
+  }
+  set_schedule(Schedule::of(bestk));
   return bestk;
+}
+
+// false when candidate k launches exactly what a smaller candidate launches:
+// LDS staging with nothing to stage, fused r faces with z faces present
+bool Engine::schedule_distinct(int k) const {
+  const Schedule s = Schedule::of(k);
+  if (s.stage && !stage_r_max && !stage_z_max) return false;
+  if ((s.fuse & 1) && n_zpairs) return false;
+  if ((s.fuse & 2) && !n_zpairs) return false;
+  if (s.col && !n_zpairs) return false;  // 2D disks: column order is table order
+  return true;
 }
 
 void Engine::synchronize() {

@@ -61,14 +61,29 @@ struct Engine {
   // launch schedule (the autotuner's choice; the results do not depend on it):
   // XCD-banded tiles, the face builder's pairs largest first, the outer
   // annulus staged in LDS
+  // fuse: bit 0 / 1 evaluate the r / z faces' increments inside the update
+  // instead of storing them (each face is then solved by both of its cells)
+  // col: 3D disks' tiles listed annulus column by annulus column
   struct Schedule {
-    int xcd = 1, largest_first = 0, stage = 1;
+    int xcd = 1, largest_first = 0, stage = 1, fuse = 0, col = 0;
+    int code() const { return xcd | largest_first << 1 | stage << 2 | fuse << 3 | col << 5; }
+    static Schedule of(int k) {
+      Schedule s;
+      s.xcd = k & 1;
+      s.largest_first = (k >> 1) & 1;
+      s.stage = (k >> 2) & 1;
+      s.fuse = (k >> 3) & 3;
+      s.col = (k >> 5) & 1;
+      return s;
+    }
   } sched;
+  const int2 *tiles_row[4] = {}, *tiles_col[4] = {};  // zone, update, r face, z face tiles per order
+  bool schedule_distinct(int k) const;
   int stage_r_max = 0, stage_z_max = 0;  // LDS staging the plan allows
   int *d_pair_perm = nullptr;
   void set_schedule(const Schedule &s);
   // times the candidate schedules on a scratch copy of the batch and keeps
-  // the fastest; ms[k] per candidate k (bits xcd | largest_first << 1 | stage << 2)
+  // the fastest; ms[k] per candidate k (Schedule::code; -1: not timed)
   int autotune(int nsteps, double *ms, int nms);
 
   LaunchPlan plan{};

@@ -62,6 +62,8 @@ struct LaunchPlan {
   const int2 *zftiles; int n_zftiles;   // z faces
   int stage_r, stage_z;                 // face builder: outer-annulus cells staged in LDS (0: none)
   int any_visc;                         // some disk has viscosity
+  int feat;                             // physics paths its disks take (F_* bits)
+  int fuse;                             // schedule: r (1) / z (2) face increments evaluated inside k_update
   const int2 *bc0; int n_bc0;           // ghost annuli 0 and NR-2
   const int2 *bc1; int n_bc1;           // ghost annulus NR-1
   const int2 *sz; int n_sz;             // z ghost layers
@@ -111,6 +113,8 @@ enum {
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
                     int recompute_w, int pre_ops, int post_ops);
 bool prepare_launches(const LaunchPlan &L);  // kernel attributes the plan needs
+// the compiled variant (F_* set) that launch_substep uses for these needs
+int feature_variant(int need, int NQ);
 void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op);
 void launch_set_w(const DevMesh &m, const LaunchPlan &L, hipStream_t st);
 void launch_mindt(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ);

@@ -341,6 +341,7 @@ __global__ __launch_bounds__(64) void k_set_w(DevMesh m, int ann0) {
 // ----------------------------------------------------------------------------
 // the CFL step of cell c (local index j) of annulus A, tile tl, from its
 // primitives pr and width dp
+template <int FEAT>
 __device__ __forceinline__ double cell_dt_at(const DevMesh &m, const int2 tl, const AnnInfo &A, long long c,
                                              const double *pr, double dp) {
   const DiskParams &P = disk_of(m, A.disk);
@@ -354,10 +355,10 @@ __device__ __forceinline__ double cell_dt_at(const DevMesh &m, const int2 tl, co
   double rdphi = .5 * (rp + rm) * dp;
   if (rdphi < dr) dx = rdphi;
   if (dx > dz) dx = dz;
-  const double tc = (P.visc > 0.0 && P.visc_const != 1) ? clean_pi(m.tiph[c]) - 0.5 * dp : 0.0;
-  return cell_dt_value(P, m.sc[A.disk], pr, r, dx, w, tc);
+  const double tc = ((FEAT & F_VISC) && P.visc > 0.0 && P.visc_const != 1) ? clean_pi(m.tiph[c]) - 0.5 * dp : 0.0;
+  return cell_dt_value<FEAT>(P, m.sc[A.disk], pr, r, dx, w, tc);
 }
-template <int NQ>
+template <int NQ, int FEAT = F_ALL>
 __device__ __forceinline__ double cell_dt(const DevMesh &m, const int2 tl, const AnnInfo &A) {
   const int j = tl.y + threadIdx.x;
   double dt = 1.e100;
@@ -366,7 +367,7 @@ __device__ __forceinline__ double cell_dt(const DevMesh &m, const int2 tl, const
     double pr[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) pr[q] = m.prim[q * m.Ntot + c];
-    double d = cell_dt_at(m, tl, A, c, pr, m.dphi[c]);
+    double d = cell_dt_at<FEAT>(m, tl, A, c, pr, m.dphi[c]);
     if (d < dt) dt = d;
   }
   return dt;
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(TILE) void k_mindt(DevMesh m, const int2 *tiles) {
 // DT: the first substep's launch also takes the CFL minimum of its tile
 // (k_mindt's value, from the same prim and dphi) when the tile is interior,
 // into the tile's slot of the interior tile list, saving k_mindt's pass.
-template <int NQ, bool DT>
+template <int NQ, bool DT, int FEAT = F_ALL>
 __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
   const int2 tl = tiles[tile_of(m)];
   const AnnInfo A = m.ann[tl.x];
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(TILE) void k_plm_p(DevMesh m, const int2 *tiles) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) pr[q] = m.prim[q * m.Ntot + c];
     if (DT && A.itile >= 0) {
-      const double d = cell_dt_at(m, tl, A, c, pr, dpc);
+      const double d = cell_dt_at<FEAT>(m, tl, A, c, pr, dpc);
       if (d < dtc) dtc = d;  // as cell_dt: a NaN or huge step counts as 1e100
     }
 #pragma unroll
@@ -615,7 +616,7 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
 // ----------------------------------------------------------------------------
 // Face (j, j+1) of an annulus, riemann_setup_p (src/Riemann/Riemann_setup.c:46-80)
 // + riemann_AddFlux: the increments X = dt*dA*F and viscous Y.
-template <int NQ>
+template <int NQ, int FEAT>
 __device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A, int a, int j, const DiskParams &P,
                                              const DiskScalars &sc, double dt, double rdr_om, double *X, double *Y) {
   const long long cL = A.zbase + j;
@@ -632,7 +633,7 @@ __device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A,
     primR[q] = m.prim[q * m.Ntot + cR] - 0.5 * m.gradp[q * m.Ntot + cR] * dpR;
   }
   ViscIn vi;
-  if (P.visc > 0.0) {
+  if ((FEAT & F_VISC) && P.visc > 0.0) {
     visc_cells(m, vi, cL, cR);
     vi.cm = vi.tL;
     vi.r_cell_L = r;
@@ -640,23 +641,18 @@ __device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A,
     vi.rdr_om = rdr_om;
   }
   Y[0] = Y[1] = Y[2] = 0.0;
-  return d_face_flux<NQ>(primL, primR, 1, r, dA, m.wiph[a], dt, P, sc, &vi, X, Y);
+  return d_face_flux<NQ, 0, FEAT>(primL, primR, 1, r, dA, m.wiph[a], dt, P, sc, &vi, X, Y);
 }
 
-// PART 0: Riemann and viscous increments; 1: Riemann only; 2: viscous only
-// (with FLUX_SPLIT the two parts run as two launches)
-template <int NQ, int PART>
-__global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
-  const int2 tl = ftiles[tile_of(m)];
-  const int p = tl.x;
-  const int f = tl.y + threadIdx.x;
-  if (f >= m.pst[p].count) return;
-  const PairInfo PI = m.pairs[p];
-  const DiskParams &P = disk_of(m, PI.disk);
-  if (PART == 2 && !(P.visc > 0.0)) return;
-  const DiskScalars &sc = m.sc[PI.disk];
-  const double dt = fac * sc.dt;
-  const long long g = PI.face_base + f;
+// The increments riemann_AddFlux adds for r/z overlap face g of pair PI
+// (riemann_setup_rz, src/Riemann/Riemann_setup.c:11-44, and the flux of
+// Riemann_routines.c:314-397): X = dt*dA*F and the viscous Y.  Stored per
+// face by k_flux_rz, or evaluated inside k_update by each of the face's two
+// cells (the fused schedule): the same arithmetic either way.
+template <int NQ, int FEAT>
+__device__ __forceinline__ int rz_face_increments(const DevMesh &m, const PairInfo &PI, const DiskParams &P,
+                                                  const DiskScalars &sc, double dt, long long g, double *X,
+                                                  double *Y) {
   const int cL = m.fL[g], cR = m.fR[g];
   const double cm = m.fcm[g];
   const double tL = clean_pi(m.tiph[cL]), tR = clean_pi(m.tiph[cR]);
@@ -676,10 +672,10 @@ __global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const i
     primR[q] = m.prim[q * m.Ntot + cR] - m.grad[q * m.Ntot + cR] * PI.deltaR - m.gradp[q * m.Ntot + cR] * dpR;
   }
   ViscIn vi;
-  if (P.visc > 0.0) {
+  if ((FEAT & F_VISC) && P.visc > 0.0) {
     visc_cells(m, vi, cL, cR);
     vi.cm = cm;
-    const double4 rot = ann_rotation(m, P, PI.ann_in);
+    const double4 rot = ann_rotation<FEAT>(m, P, PI.ann_in);
     if (PI.dir == 0) {
       vi.r_cell_L = r - PI.deltaL;
       vi.r_cell_R = r + PI.deltaR;
@@ -690,13 +686,27 @@ __global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const i
       vi.rdr_om = rot.y;  // z faces sit at the annulus centre
     }
   }
-  double X[NQ], Y[3] = {0.0, 0.0, 0.0};
-  int e = d_face_flux<NQ, PART>(primL, primR, PI.dir == 0 ? 0 : 2, r, dA, 0.0, dt, P, sc, &vi, X, Y);
+  Y[0] = Y[1] = Y[2] = 0.0;
+  return d_face_flux<NQ, 0, FEAT>(primL, primR, PI.dir == 0 ? 0 : 2, r, dA, 0.0, dt, P, sc, &vi, X, Y);
+}
+
+// every r (or z) overlap face once: its increments to Fface
+template <int NQ, int FEAT>
+__global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
+  const int2 tl = ftiles[tile_of(m)];
+  const int p = tl.x;
+  const int f = tl.y + threadIdx.x;
+  if (f >= m.pst[p].count) return;
+  const PairInfo PI = m.pairs[p];
+  const DiskParams &P = disk_of(m, PI.disk);
+  const DiskScalars &sc = m.sc[PI.disk];
+  const long long g = PI.face_base + f;
+  double X[NQ], Y[3];
+  int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, fac * sc.dt, g, X, Y);
   if (e) set_err(m.err, e);
-  if (PART != 2)
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = X[q];
-  if (PART != 1 && P.visc > 0.0) {
+  for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = X[q];
+  if ((FEAT & F_VISC) && P.visc > 0.0) {
 #pragma unroll
     for (int v = 0; v < 3; ++v) m.Fface[(NQ + v) * m.NFtot + g] = Y[v];
   }
@@ -711,6 +721,28 @@ __global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const i
 //   variables at the start of the substep are prim2cons of the primitives,
 //   which is what cell_calc_cons left there (:112).
 // ----------------------------------------------------------------------------
+// the increments of a face evaluated in place (fused schedule): the same
+// additions as add_face_R / add_face_L from the stored ones
+template <int NQ>
+__device__ __forceinline__ void add_inc_R(const double *X, const double *Y, double *u, double r_cell, bool visc) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) u[q] += X[q];
+  if (visc) {
+    u[SRR] += Y[0] / r_cell;
+    u[LLL] += Y[1];
+    u[TAU] += Y[2];
+  }
+}
+template <int NQ>
+__device__ __forceinline__ void add_inc_L(const double *X, const double *Y, double *u, double r_cell, bool visc) {
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) u[q] += -X[q];
+  if (visc) {
+    u[SRR] += -Y[0] / r_cell;
+    u[LLL] += -Y[1];
+    u[TAU] += -Y[2];
+  }
+}
 template <int NQ>
 __device__ __forceinline__ void add_face_R(const DevMesh &m, const double *F, long long stride, long long g,
                                            double *u, double r_cell, bool visc) {
@@ -756,7 +788,10 @@ __device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy
   return t;
 }
 
-template <int NQ>
+// FUSE: FUSE_R / FUSE_Z evaluate the r / z faces' increments in place (each
+// face by both of its cells) instead of gathering the stored ones
+enum { FUSE_R = 1, FUSE_Z = 2 };
+template <int NQ, int FEAT, int FUSE>
 __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
   const int ti = tile_of(m);
   const int2 tl = tiles[ti];
@@ -769,7 +804,7 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
   double md0 = 0.0, md1 = 0.0, tq = 0.0;
   const DiskParams &P = disk_of(m, A.disk);
   const DiskScalars &sc = m.sc[A.disk];
-  const double4 rot = ann_rotation(m, P, tl.x);
+  const double4 rot = ann_rotation<FEAT>(m, P, tl.x);
   // the cell's r-face ranges (the heads of the update's dependent gathers),
   // loaded before the phi faces so that their latency hides behind the
   // Riemann solves: as R, up to two runs of pair (i-1, i); as L, one range
@@ -808,7 +843,7 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
     int jf = tl.y - 1 + t;
     if (jf < 0) jf += A.N;
     double X[NQ], Y[3];
-    int e = phi_face_flux<NQ>(m, A, tl.x, jf, P, sc, fac * sc.dt, rot.y, X, Y);
+    int e = phi_face_flux<NQ, FEAT>(m, A, tl.x, jf, P, sc, fac * sc.dt, rot.y, X, Y);
     if (e) set_err(m.err, e);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) s_F[q][t] = X[q];
@@ -819,7 +854,7 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
   if (t < n) {
     const double dt = fac * sc.dt;
     const double RK = sc.RK;
-    const bool visc = P.visc > 0.0;
+    const bool visc = (FEAT & F_VISC) && P.visc > 0.0;
     const long long c = A.zbase + j;
     const int N = A.N, k = A.k, i = A.i;
     const double rp = A.rp, rm = A.rm, zp = A.zp, zm = A.zm;
@@ -876,21 +911,53 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
       const double rc = PI.r + PI.deltaR;
 #pragma unroll
       for (int w = 0; w < 2; ++w)
-        for (int f = rb[w]; f < re[w]; ++f) add_face_R<NQ>(m, m.Fface, m.NFtot, PI.face_base + f, u, rc, visc);
+        for (int f = rb[w]; f < re[w]; ++f) {
+          if (FUSE & FUSE_R) {
+            double X[NQ], Y[3];
+            const int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, PI.face_base + f, X, Y);
+            if (e) set_err(m.err, e);
+            add_inc_R<NQ>(X, Y, u, rc, visc);
+          } else {
+            add_face_R<NQ>(m, m.Fface, m.NFtot, PI.face_base + f, u, rc, visc);
+          }
+        }
     }
     if (i <= P.NR - 2) {
       const PairInfo PI = m.pairs[rpair_of(P, k, i)];
       const double rc = PI.r - PI.deltaL;
-      for (int f = rb[2]; f < re[2]; ++f) add_face_L<NQ>(m, m.Fface, m.NFtot, PI.face_base + f, u, rc, visc);
+      for (int f = rb[2]; f < re[2]; ++f) {
+        if (FUSE & FUSE_R) {
+          double X[NQ], Y[3];
+          const int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, PI.face_base + f, X, Y);
+          if (e) set_err(m.err, e);
+          add_inc_L<NQ>(X, Y, u, rc, visc);
+        } else {
+          add_face_L<NQ>(m, m.Fface, m.NFtot, PI.face_base + f, u, rc, visc);
+        }
+      }
     }
     if (P.NglobalZ != 1) {
       if (k >= 1)
         faces_as_R(m, zpair_of(P, k - 1, i), j, N, [&](const PairInfo &PI, long long g) {
-          add_face_R<NQ>(m, m.Fface, m.NFtot, g, u, PI.r, visc);
+          if (FUSE & FUSE_Z) {
+            double X[NQ], Y[3];
+            const int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, g, X, Y);
+            if (e) set_err(m.err, e);
+            add_inc_R<NQ>(X, Y, u, PI.r, visc);
+          } else {
+            add_face_R<NQ>(m, m.Fface, m.NFtot, g, u, PI.r, visc);
+          }
         });
       if (k <= P.NZ - 2)
         faces_as_L(m, zpair_of(P, k, i), m.fo_z, c, j, N, [&](const PairInfo &PI, long long g) {
-          add_face_L<NQ>(m, m.Fface, m.NFtot, g, u, PI.r, visc);
+          if (FUSE & FUSE_Z) {
+            double X[NQ], Y[3];
+            const int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, g, X, Y);
+            if (e) set_err(m.err, e);
+            add_inc_L<NQ>(X, Y, u, PI.r, visc);
+          } else {
+            add_face_L<NQ>(m, m.Fface, m.NFtot, g, u, PI.r, visc);
+          }
         });
     }
     // cell_add_src, src/Cell/cell_source.c:73-214
@@ -908,7 +975,7 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
       double Fp;
       add_geom_grav_src(u, pr, r, dV, dt, cphi, sphi, zc, rot.x, rot.y, rot.z, P, sc, &Fp);
       if (m.torque_tile && interior_cell(P, i, k)) tq += dt * dV * rho * Fp * r;
-      if (P.sink == 1) {  // get_rho_sink, :15-47
+      if ((FEAT & F_SINK) && P.sink == 1) {  // get_rho_sink, :15-47
         double xpos = r * cphi, ypos = r * sphi;
         double xbh0 = sc.xbh[0], ybh0 = sc.ybh[0];
         double xbh1 = sc.xbh[1], ybh1 = sc.ybh[1];
@@ -937,7 +1004,7 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
           }
         }
       }
-      if (P.cooling == 1) {
+      if ((FEAT & F_COOL) && P.cooling == 1) {
         if (P.initial_data_type == 0) {
           double cooling_term = (Pp - .01) / (P.gamma - 1.) * dV / 1.e-2;
           if (cooling_term < 0.0) cooling_term = 0.0;
@@ -967,8 +1034,9 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
                   (rdrOm_a + r * m.grad[UPP * m.Ntot + c] + m.gradp[URR * m.Ntot + c]);
       }
     }
-    // cell_add_split_fictitious, cell_add_split_fictitious.c:11-44
-    {
+    // cell_add_split_fictitious, cell_add_split_fictitious.c:11-44 (a no-op
+    // without a frame rotation: kappa^2 = 0)
+    if (FEAT & F_FRAME) {
       double rOm = rot.x;
       double Omega = rOm / r;
       double dr_r2Om = 2. * rOm + r * rot.y;
@@ -1013,11 +1081,11 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
     if (threadIdx.x == 0) {
       double t0 = 0.0;
       for (int w = 0; w < TILE / 64; ++w) t0 += s_md[0][w];
-      m.torque_tile[ti] = t0;
+      m.torque_tile[A.utile + tl.y / (TILE - 1)] = t0;
     }
     __syncthreads();
   }
-  if (P.sink == 1) {
+  if ((FEAT & F_SINK) && P.sink == 1) {
     // the tile's accretion of each mass, summed in a fixed order into the
     // tile's own slot; k_scalar folds a disk's slots in tile order, so Mdot
     // does not depend on the order in which workgroups run
@@ -1036,8 +1104,9 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
         t0 += s_md[0][w];
         t1 += s_md[1][w];
       }
-      m.mdot_tile[2 * ti] = t0;
-      m.mdot_tile[2 * ti + 1] = t1;
+      const int slot = A.utile + tl.y / (TILE - 1);
+      m.mdot_tile[2 * slot] = t0;
+      m.mdot_tile[2 * slot + 1] = t1;
     }
   }
 }
@@ -1438,30 +1507,33 @@ static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st
           hipLaunchKernelGGL(k_faces, dim3(np), dim3(FTB), (size_t)nst * 2 * sizeof(double), st, m, p0, nst));
 }
 
-// the r or z faces' increments: one kernel, or (FLUX_SPLIT) the Riemann and
-// the viscous parts as two, each with fewer registers
-#ifndef FLUX_SPLIT
-#define FLUX_SPLIT 0
-#endif
-template <int NQ>
-static void launch_flux_q(const DevMesh &m, const int2 *ft, int nft, int any_visc, hipStream_t st, double fac,
-                          const char *name, const char *name_visc) {
-  const dim3 B(TILE);
-  if (FLUX_SPLIT) {
-    KLAUNCH(st, name, hipLaunchKernelGGL((k_flux_rz<NQ, 1>), dim3(nft), B, 0, st, m, ft, fac));
-    if (any_visc) KLAUNCH(st, name_visc, hipLaunchKernelGGL((k_flux_rz<NQ, 2>), dim3(nft), B, 0, st, m, ft, fac));
-  } else {
-    KLAUNCH(st, name, hipLaunchKernelGGL((k_flux_rz<NQ, 0>), dim3(nft), B, 0, st, m, ft, fac));
-  }
+template <int NQ, int FEAT>
+static void launch_flux_q(const DevMesh &m, const int2 *ft, int nft, hipStream_t st, double fac, const char *name) {
+  KLAUNCH(st, name, hipLaunchKernelGGL((k_flux_rz<NQ, FEAT>), dim3(nft), dim3(TILE), 0, st, m, ft, fac));
 }
 
-template <int NQ>
+template <int NQ, int FEAT>
+static void launch_update_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, double fac, int copy_first,
+                            int fuse) {
+  const dim3 G(L.n_utiles), B(TILE);
+  if (fuse == (FUSE_R | FUSE_Z))
+    KLAUNCH(st, "k_update", hipLaunchKernelGGL((k_update<NQ, FEAT, FUSE_R | FUSE_Z>), G, B, 0, st, m, L.utiles, fac, copy_first));
+  else if (fuse == FUSE_R)
+    KLAUNCH(st, "k_update", hipLaunchKernelGGL((k_update<NQ, FEAT, FUSE_R>), G, B, 0, st, m, L.utiles, fac, copy_first));
+  else if (fuse == FUSE_Z)
+    KLAUNCH(st, "k_update", hipLaunchKernelGGL((k_update<NQ, FEAT, FUSE_Z>), G, B, 0, st, m, L.utiles, fac, copy_first));
+  else
+    KLAUNCH(st, "k_update", hipLaunchKernelGGL((k_update<NQ, FEAT, 0>), G, B, 0, st, m, L.utiles, fac, copy_first));
+}
+
+template <int NQ, int FEAT>
 static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, double fac, int copy_first,
                              int recompute_w, int pre_ops, int post_ops) {
   const dim3 B(TILE);
   const dim3 SD(L.ndisk), S64(SCALAR_THREADS);
   if (pre_ops & SC_PLM_DT) {  // the CFL tile minima ride on the phi slopes; the fold follows them
-    KLAUNCH(st, "k_plm_p_dt", hipLaunchKernelGGL((k_plm_p<NQ, true>), dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
+    KLAUNCH(st, "k_plm_p_dt",
+            hipLaunchKernelGGL((k_plm_p<NQ, true, FEAT>), dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
     KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, pre_ops & ~SC_PLM_DT, 0.0));
   } else {
     if (pre_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, pre_ops, 0.0));
@@ -1471,14 +1543,15 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   launch_faces_q<NQ>(m, L, st, 1);
   if (recompute_w) KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
   KLAUNCH(st, "k_plm_r", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0));
-  if (L.n_rftiles)
-    launch_flux_q<NQ>(m, L.rftiles, L.n_rftiles, L.any_visc, st, fac, "k_flux_r", "k_flux_r_visc");
+  // the fused schedule evaluates the r faces inside k_update, which then
+  // needs the r slopes: only without z faces (the z slopes overwrite them)
+  const int fuse = (L.n_zpairs ? 0 : (L.fuse & FUSE_R)) | (L.fuse & FUSE_Z);
+  if (L.n_rftiles && !(fuse & FUSE_R)) launch_flux_q<NQ, FEAT>(m, L.rftiles, L.n_rftiles, st, fac, "k_flux_r");
   if (L.n_zpairs) {
     KLAUNCH(st, "k_plm_z", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 1));
-    launch_flux_q<NQ>(m, L.zftiles, L.n_zftiles, L.any_visc, st, fac, "k_flux_z", "k_flux_z_visc");
+    if (!(fuse & FUSE_Z)) launch_flux_q<NQ, FEAT>(m, L.zftiles, L.n_zftiles, st, fac, "k_flux_z");
   }
-  KLAUNCH(st, "k_update",
-          hipLaunchKernelGGL(k_update<NQ>, dim3(L.n_utiles), B, 0, st, m, L.utiles, fac, copy_first));
+  launch_update_q<NQ, FEAT>(m, L, st, fac, copy_first, fuse);
   DevMesh m2 = m;  // after the update the new state lives in the *_new buffers
   m2.tiph = m.tiph_new;
   m2.prim = m.prim_new;
@@ -1508,9 +1581,34 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
     default: { constexpr int Q = 8; CALL; } break; \
   }
 
+// The compiled physics variants (feature sets, F_*): NUM_Q 5 has a variant
+// per common configuration, every NUM_Q the general one.  The plan's feat is
+// the union of what its disks need.
+static const int FEAT_VARIANTS[] = {F_HLLC, F_HLL, F_VISC | F_HLLC, F_ALL};
+int feature_variant(int need, int NQ) {
+  if (NQ == 5)
+    for (int v : FEAT_VARIANTS)
+      if ((need & ~v) == 0) return v;
+  return F_ALL;
+}
+
+template <int NQ>
+static void launch_substep_nq(const DevMesh &m, const LaunchPlan &L, hipStream_t st, double fac, int copy_first,
+                              int recompute_w, int pre_ops, int post_ops) {
+  const int v = feature_variant(L.feat, NQ);
+  if (NQ == 5 && v == F_HLLC)
+    launch_substep_q<NQ, F_HLLC>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
+  else if (NQ == 5 && v == F_HLL)
+    launch_substep_q<NQ, F_HLL>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
+  else if (NQ == 5 && v == (F_VISC | F_HLLC))
+    launch_substep_q<NQ, F_VISC | F_HLLC>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
+  else
+    launch_substep_q<NQ, F_ALL>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops);
+}
+
 void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, double fac, int copy_first,
                     int recompute_w, int pre_ops, int post_ops) {
-  DISCO_NQ_DISPATCH(NQ, launch_substep_q<Q>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops));
+  DISCO_NQ_DISPATCH(NQ, launch_substep_nq<Q>(m, L, st, fac, copy_first, recompute_w, pre_ops, post_ops));
 }
 
 bool prepare_launches(const LaunchPlan &L) {

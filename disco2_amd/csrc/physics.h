@@ -224,6 +224,7 @@ DHD __forceinline__ void add_geom_grav_src(double *u, const double *pr, double r
 // CFL time step of one cell, src/Cell/cell_mindt.c:44-90: pr its
 // primitives, r its centre radius, dx its smallest extent, w the mesh
 // velocity, tc its centre angle (read only by the alpha viscosity).
+template <int FEAT = F_ALL>
 DHD __forceinline__ double cell_dt_value(const DiskParams &P, const DiskScalars &sc, const double *pr, double r,
                                          double dx, double w, double tc) {
   double Pp = pr[PPP];
@@ -235,7 +236,7 @@ DHD __forceinline__ double cell_dt_value(const DiskParams &P, const DiskScalars 
   double cf2 = cs * cs;
   double a = sqrt(cf2) + sqrt(vr * vr + vp * vp + vz * vz);
   double d = P.cfl * dx / a;
-  if (P.visc > 0.0) {
+  if ((FEAT & F_VISC) && P.visc > 0.0) {
     double nu;
     if (P.visc_const == 1) {
       nu = P.visc;
@@ -291,8 +292,10 @@ DHD __forceinline__ void d_set_flux(const double *prim, double *F, double r, int
 }
 
 // PART 0: the whole flux; 1: the Riemann increments X only; 2: the viscous
-// increments Y only (the two parts share no intermediate but dt*dA)
-template <int NQ, int PART = 0>
+// increments Y only (the two parts share no intermediate but dt*dA).  FEAT:
+// the solvers and the viscous flux this variant keeps (F_HLL | F_HLLC both:
+// the disk's RiemannSolver picks at run time).
+template <int NQ, int PART = 0, int FEAT = F_ALL>
 DHD __forceinline__ int d_face_flux(const double *primL, const double *primR, int dir, double r,
                                            double dA, double w, double dt, const DiskParams &P,
                                            const DiskScalars &sc, const ViscIn *vi, double *X, double *Y) {
@@ -337,7 +340,7 @@ DHD __forceinline__ int d_face_flux(const double *primL, const double *primR, in
       d_prim2cons<NQ>(pr, Uk, r, 1.0, G, P.NC);
 #pragma unroll
       for (int q = 0; q < NQ; ++q) F[q] = Fk[q] - w * Uk[q];
-    } else if (P.riemann == HLL_SOLVER) {  // Riemann_routines.c:74-85
+    } else if ((FEAT & F_HLL) && (!(FEAT & F_HLLC) || P.riemann == HLL_SOLVER)) {  // Riemann_routines.c:74-85
       double FL[NQ], FR[NQ], UL[NQ], UR[NQ];
       d_set_flux<NQ>(primL, FL, r, n0, n1, n2, G);
       d_set_flux<NQ>(primR, FR, r, n0, n1, n2, G);
@@ -395,7 +398,7 @@ DHD __forceinline__ int d_face_flux(const double *primL, const double *primR, in
 #pragma unroll
     for (int q = 0; q < NQ; ++q) X[q] = dtdA * F[q];
   }
-  if (PART != 1 && P.visc > 0.0) {  // riemann_visc_flux, :193-311
+  if (PART != 1 && (FEAT & F_VISC) && P.visc > 0.0) {  // riemann_visc_flux, :193-311
     // only the entries the flux formulas use are formed: Avg for all five
     // primitives, the r and phi gradients of v_r and omega
     double Avg[5], Grr[2], Gph[2];

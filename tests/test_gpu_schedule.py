@@ -2,8 +2,10 @@
 
 A schedule changes only which workgroup runs which tile (XCD bands or launch
 order) and in which order the face builder takes the annulus pairs (table
-order or largest first), and whether it stages the outer annulus in LDS;
-never the arithmetic or its order.  So every schedule must give the same
+order or largest first), whether it stages the outer annulus in LDS, and
+whether the update evaluates the r / z faces' fluxes itself (each face by
+both of its cells) instead of reading them stored per face; never the
+arithmetic or its order.  So every schedule must give the same
 bits, and the autotuner, which times the candidates on a scratch copy of the
 batch, must leave the batch's state as it was.
 """
@@ -15,11 +17,12 @@ from compare import bitwise_equal, describe
 pytestmark = pytest.mark.gpu
 
 
-def batch():
+def batch(with3d=True):
     from disco2_amd import configs, ics
     from disco2_amd.api import Batch, Setup
-    specs = [configs.binary(48, r_min=0.4, r_max=4.0, q=0.3), configs.vortex(40),
-             configs.disk3d(10, 6, 40, R_Min=0.5, R_Max=2.5)]
+    specs = [configs.binary(48, r_min=0.4, r_max=4.0, q=0.3), configs.vortex(40)]
+    if with3d:
+        specs.append(configs.disk3d(10, 6, 40, R_Min=0.5, R_Max=2.5))
     setups = [Setup(p) for p, _, _ in specs]
     b = Batch(setups)
     for d, ((p, kind, kw), s) in enumerate(zip(specs, setups)):
@@ -32,11 +35,13 @@ def state(b, n):
     return [(b.download("prim", disk=d), b.download("tiph", disk=d), b.scalars(d)) for d in range(n)]
 
 
-def test_every_schedule_gives_the_same_bits():
+@pytest.mark.parametrize("with3d,fused", [(True, 2), (False, 1)])
+def test_every_schedule_gives_the_same_bits(with3d, fused):
     ref = None
-    for k in range(8):
-        print(f"schedule {k}", flush=True)
-        b, n = batch()
+    extra = [5 | 1 << 5, 6 | 1 << 5 | fused << 3] if with3d else []
+    for k in list(range(8)) + [5 | fused << 3, 2 | fused << 3] + extra:
+        print(f"schedule {k} (3D disk: {with3d})", flush=True)
+        b, n = batch(with3d)
         b.set_schedule(k)
         assert b.schedule() == k
         b.run(6)
