@@ -145,6 +145,45 @@ SIGNATURES = {
     "diagnostics_print": (None, [vp, vp, vp, vp]),
     "diagnostics_tdiag_measure": (cd, [vp]),
     "diagnostics_tdiag_dump": (cd, [vp]),
+    "cell_single_init_ptr": (vp, [vp]),
+    "cell_init_ptr": (vp, [vp]),
+    "cell_clear_w": (None, [vp, vp]),
+    "cell_calc_prim": (None, [vp, vp]),
+    "cell_prim2cons": (None, [pd, pd, cd, cd, vp]),
+    "cell_cons2prim": (None, [pd, pd, cd, cd, vp]),
+    "cell_Cool": (cd, [vp, ci, ci, ci]),
+    "cell_single_create": (vp, [vp]),
+    "cell_single_destroy": (None, [vp]),
+    "cell_add_cons": (None, [vp, ci, cd]),
+    "cell_copy": (None, [vp, vp]),
+    "cell_adjust_RK_cons": (None, [vp, vp, cd]),
+    "cell_add_src": (None, [vp, vp, vp, cd]),
+    "cell_add_visc_src": (None, [vp, vp, vp, cd]),
+    "cell_add_visc_src_old": (None, [vp, vp, cd]),
+    "cell_add_split_fictitious": (None, [vp, vp, vp, cd]),
+    "cell_update_phi": (None, [vp, vp, vp, cd, cd]),
+    "cell_update_dphi": (None, [vp, vp]),
+    "cell_boundary_outflow_r": (None, [vp, vp, vp, vp, vp]),
+    "cell_boundary_outflow_z": (None, [vp, vp, vp, vp, vp]),
+    "cell_boundary_fixed_r": (None, [vp, vp, vp, vp, vp]),
+    "cell_boundary_fixed_z": (None, [vp, vp, vp, vp, vp]),
+    "cell_bc_damp": (None, [vp, vp, vp, cd, vp]),
+    "cell_print": (None, [vp, ci, ci, ci]),
+    "riemann_create": (vp, [vp]),
+    "riemann_destroy": (None, [vp]),
+    "riemann_setup_rz": (None, [vp, vp, vp, ci, ci]),
+    "riemann_setup_p": (None, [vp, vp, vp, ci, ci, ci, ci]),
+    "riemann_AddFlux": (None, [vp, vp, vp, cd]),
+    "diagnostics_reset": (None, [vp, vp, vp, vp]),
+    "sim_ZeroPsiBndry": (ci, [vp]),
+    "sim_NoInnerBC": (ci, [vp]),
+    "sim_BoundTypeR": (ci, [vp]),
+    "sim_BoundTypeZ": (ci, [vp]),
+    "sim_DAMP_TIME": (cd, [vp]),
+    "sim_EXPLICIT_VISCOSITY": (cd, [vp]),
+    "timestep_set_dt": (None, [vp, vp, vp, vp]),
+    "timestep_set_RK": (None, [vp, cd]),
+    "timestep_update_t": (None, [vp]),
     "disco_last_error": (ci, []),
     "disco_last_error_msg": (ccp, []),
     "disco_clear_error": (None, []),

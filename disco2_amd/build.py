@@ -28,7 +28,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
 CXX = os.environ.get("DISCO_CXX", "g++")
-SOURCES = ["kernels.hip", "adjoint.hip", "diagnostics.hip", "checkpoint.hip", "engine.cpp", "adjoint_host.cpp",
+SOURCES = ["kernels.hip", "unit.hip", "adjoint.hip", "diagnostics.hip", "checkpoint.hip", "engine.cpp", "adjoint_host.cpp",
            "diagnostics.cpp", "checkpoint.cpp", "h5.cpp", "capi.cpp", "sim_host.cpp", "initial_data.cpp", "frame.cpp"]
 GCC_SOURCES = {"initial_data.cpp", "frame.cpp"}
 

@@ -142,15 +142,20 @@ long long cell_zone_index(Cell *c) { return reinterpret_cast<CellRef *>(c)->zone
 
 static CellRef *R(Cell *c) {
   CellRef *r = reinterpret_cast<CellRef *>(c);
-  mirror_pull(r->h);
+  if (!r->own) mirror_pull(r->h);
   return r;
 }
-double cell_prim(Cell *c, int q) { CellRef *r = R(c); return r->h->prim[q * r->h->nz + r->zone]; }
-double cell_grad(Cell *c, int q) { CellRef *r = R(c); return r->h->grad[q * r->h->nz + r->zone]; }
-double cell_gradp(Cell *c, int q) { CellRef *r = R(c); return r->h->gradp[q * r->h->nz + r->zone]; }
-double cell_tiph(Cell *c) { CellRef *r = R(c); return r->h->tiph[r->zone]; }
-double cell_dphi(Cell *c) { CellRef *r = R(c); return r->h->dphi[r->zone]; }
-double cell_wiph(Cell *c) { CellRef *r = R(c); return r->h->wiph[r->zone]; }
+// a cell of its own keeps NQ primitives, then tiph, dphi, wiph; it has no
+// slopes (the reference's cell_single_create allocates none)
+double cell_prim(Cell *c, int q) {
+  CellRef *r = R(c);
+  return r->own ? r->own[q] : r->h->prim[q * r->h->nz + r->zone];
+}
+double cell_grad(Cell *c, int q) { CellRef *r = R(c); return r->own ? 0.0 : r->h->grad[q * r->h->nz + r->zone]; }
+double cell_gradp(Cell *c, int q) { CellRef *r = R(c); return r->own ? 0.0 : r->h->gradp[q * r->h->nz + r->zone]; }
+double cell_tiph(Cell *c) { CellRef *r = R(c); return r->own ? r->own[r->NQ] : r->h->tiph[r->zone]; }
+double cell_dphi(Cell *c) { CellRef *r = R(c); return r->own ? r->own[r->NQ + 1] : r->h->dphi[r->zone]; }
+double cell_wiph(Cell *c) { CellRef *r = R(c); return r->own ? r->own[r->NQ + 2] : r->h->wiph[r->zone]; }
 
 // ------------------------------------------------------------------ initial data
 // The plugins of src/Cell/initial_data/*.c run on the host over the cell
@@ -163,6 +168,13 @@ static void cell_init_type(Cell ***c, Sim *s, GravMass *g, int type) {
 }
 static void cell_single_init_type(Cell *cell, Sim *s, int type, int i) {
   CellRef *r = R(cell);
+  if (r->own) {  // a cell of its own (cell_single_create)
+    double o[DISCO_MAXQ];
+    for (int q = 0; q < r->NQ; ++q) o[q] = r->own[q];
+    if (disco_single_init(s, type, i, r->own[r->NQ], r->own[r->NQ + 1], o, r->NQ) <= 0) return;
+    for (int q = 0; q < r->NQ; ++q) r->own[q] = o[q];
+    return;
+  }
   CellsHandle *h = r->h;
   double o[DISCO_MAXQ];
   for (int q = 0; q < h->NQ; ++q) o[q] = h->prim[q * h->nz + r->zone];
@@ -275,8 +287,12 @@ void cell_syncproc_z(Cell ***c, Sim *s, MPIsetup *) {
   e->upload(0, 5, h->tiph.data());
   e->upload(0, 2, rk.data());
 }
-// Conserved variables are always derived from the primitives on the device.
-void cell_calc_cons(Cell ***c, Sim *) { mirror_push(H(c)); }
+// src/Cell/cell_conversion.c:35-55: the conserved state is prim2cons of the
+// primitives again (derived on the device when a unit-level call needs it)
+void cell_calc_cons(Cell ***c, Sim *) {
+  mirror_push(H(c));
+  H(c)->eng->ucons_live = false;
+}
 
 // src/Cell/cell_conversion.c:11-33
 void cell_prim2cons(double *prim, double *cons, double r, double dV, Sim *s) {
@@ -316,26 +332,18 @@ void cell_cons2prim(double *cons, double *prim, double r, double dV, Sim *s) {
   prim[UZZ] = vz;
   for (int q = s->NUM_C; q < s->NUM_Q; ++q) prim[q] = cons[q] / cons[DDD];
 }
-// cell_calc_prim of cons = prim2cons(prim), src/Cell/cell_conversion.c:105-122
-void cell_calc_prim(Cell ***c, Sim *s) {
+// cell_calc_prim, src/Cell/cell_conversion.c:105-122, from the unit-level
+// conserved state (prim2cons of the primitives unless a unit-level call
+// changed it)
+void cell_calc_prim(Cell ***c, Sim *) {
   CellsHandle *h = H(c);
-  mirror_pull(h);
-  const DiskParams &P = h->eng->hdp[0];
-  for (int a = 0; a < P.NR * P.NZ; ++a) {
-    const AnnInfo &A = h->eng->hann[a];
-    double rp = A.rp, rm = A.rm, dz = A.zp - A.zm, r = .5 * (rp + rm);
-    for (int j = 0; j < A.N; ++j) {
-      long long z = A.zbase + j;
-      double p[8], u[8];
-      for (int q = 0; q < h->NQ; ++q) p[q] = h->prim[q * h->nz + z];
-      double dV = .5 * (rp * rp - rm * rm) * h->dphi[z] * dz;
-      cell_prim2cons(p, u, r, dV, s);
-      cell_cons2prim(u, p, r, dV, s);
-      for (int q = 0; q < h->NQ; ++q) h->prim[q * h->nz + z] = p[q];
-    }
-  }
-  h->dirty_prim = true;
   mirror_push(h);
+  Engine *e = h->eng;
+  double *uc = e->unit_cons();
+  if (!uc) return;
+  launch_unit(e->mesh(), e->plan, e->st, e->NQ, UNIT_CALC_PRIM, uc, 0.0, 0.0, nullptr, nullptr, 0);
+  e->check_error();
+  h->stale = true;
 }
 
 // src/Cell/cell_misc.c:134-143.  The next step recomputes wiph (the engine's
@@ -486,6 +494,7 @@ void timestep_rk2(TimeStep *ts, Sim *, Cell ***c, GravMass *gm, MPIsetup *) {
   Engine *e = h->eng;
   scalars_from_host(e, ts, gm);
   e->enqueue_rk2();
+  e->ucons_live = false;  // cell_calc_cons ends every substep
   scalars_to_host(e, ts, gm);
   e->check_error();
   h->stale = true;
@@ -496,6 +505,7 @@ void timestep_forward_euler(TimeStep *ts, Sim *, Cell ***c, GravMass *gm, MPIset
   Engine *e = h->eng;
   scalars_from_host(e, ts, gm);
   e->enqueue_euler();
+  e->ucons_live = false;  // cell_calc_cons ends every substep
   scalars_to_host(e, ts, gm);
   e->check_error();
   h->stale = true;
@@ -508,6 +518,7 @@ void timestep_substep(TimeStep *ts, Cell ***c, Sim *, GravMass *gm, MPIsetup *, 
   Engine *e = h->eng;
   scalars_from_host(e, ts, gm);
   e->enqueue_substep(fac, 0, 1, SC_SUB_BEGIN, SC_AFTER_UPDATE);
+  e->ucons_live = false;  // cell_calc_cons ends every substep
   scalars_to_host(e, ts, gm);
   e->check_error();
   h->stale = true;
@@ -915,3 +926,289 @@ int disco_batch_initial_data(DiscoBatch *b, int d) {
 int disco_batch_checkpoint_write(DiscoBatch *b, int d, const char *fname) { return b->e->checkpoint_write(d, fname); }
 int disco_batch_checkpoint_read(DiscoBatch *b, int d, const char *fname) { return b->e->checkpoint_read(d, fname); }
 }
+
+// ------------------------------------------------------------------ unit level
+// The pieces timestep_substep calls (src/TimeStep/timestep_routines.c:16-113),
+// exported one by one with the reference's signatures (src/Headers/Cell.h,
+// Riemann.h).  Each is a device launch on the handle's current state and on
+// the unit-level conserved state (the reference's cell->cons), which follows
+// the primitives until one of these calls changes it (cell_calc_cons makes
+// it prim2cons of the primitives again, as do timestep_rk2 / _substep /
+// _forward_euler, which end with cell_calc_cons).  The engine's own steps
+// start from prim2cons of the primitives.
+struct Riemann {  // riemann_setup_* fill the face, riemann_AddFlux solves it
+  CellsHandle *h = nullptr;
+  UnitFace f{};
+  bool ready = false;
+};
+
+static int ann_of_zone(const Engine *e, long long z) {
+  // annuli are in zone order: the last one starting at or before z
+  int lo = 0, hi = e->nann - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) / 2;
+    if (e->hann[mid].zbase <= z)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// the launch that the unit-level source routines share; part as UNIT_SRC
+static void unit_src(Cell ***c, GravMass *gm, double dt, int part) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  if (gm) scalars_from_host(e, nullptr, gm);
+  double *uc = e->unit_cons();
+  if (!uc) return;
+  launch_unit(e->mesh(), e->plan, e->st, e->NQ, UNIT_SRC, uc, dt, 0.0, e->d_cool, e->d_umdot, part);
+  if (part == 0 && gm) {  // gravMass_set_Mdot with the accretion summed in tile order (cell_source.c:206-213)
+    std::vector<double> md(2 * (size_t)e->plan.n_ztiles);
+    hip_check(hipMemcpyAsync(md.data(), e->d_umdot, md.size() * sizeof(double), hipMemcpyDeviceToHost, e->st),
+              "sink D2H");
+    hip_check(hipStreamSynchronize(e->st), "synchronize");
+    double s0 = 0.0, s1 = 0.0;
+    for (int t = 0; t < e->plan.n_ztiles; ++t) {
+      s0 += md[2 * t];
+      s1 += md[2 * t + 1];
+    }
+    gm[0].Mdot = s0;
+    gm[1].Mdot = s1;
+  }
+}
+
+// a plugin's single-cell function other than the disk's own cannot run on
+// the device (its rows are the disk's)
+static bool own_plugin(Sim *s, void (*fn)(Cell *, Sim *, GravMass *, int, int, int), const char *who) {
+  if (fn != cell_single_init_ptr(s)) {
+    disco_set_error(17, std::string(who) + ": only the Sim's own single-cell initial data (cell_single_init_ptr) "
+                                           "runs on the device");
+    return false;
+  }
+  return true;
+}
+static void unit_bc(Cell ***c, int mode, double dt) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  InitRow *rows = e->unit_rows();
+  if (!rows) return;
+  launch_unit_bc(e->mesh(), e->plan, e->st, e->NQ, rows, mode, dt);
+  h->stale = true;
+}
+
+extern "C" {
+
+// src/Cell/cell_create_destroy.c:12-21: a cell of its own (its primitives,
+// edge, width and mesh velocity; zero until set)
+Cell *cell_single_create(Sim *s) {
+  CellRef *r = new CellRef();
+  r->h = nullptr;
+  r->zone = -1;
+  r->NQ = s->NUM_Q;
+  r->own = new double[r->NQ + 3]();
+  return reinterpret_cast<Cell *>(r);
+}
+void cell_single_destroy(Cell *c) {
+  CellRef *r = reinterpret_cast<CellRef *>(c);
+  if (!r || !r->own) return;
+  delete[] r->own;
+  delete r;
+}
+
+// src/Cell/cell_access.c:32-34: the cooling rate cell_add_src stored (0
+// until a unit-level cell_add_src with Cooling 1 ran)
+double cell_Cool(Cell ***c, int i, int j, int k) {
+  CellsHandle *h = H(c);
+  Engine *e = h->eng;
+  if (!e->d_cool) return 0.0;
+  double v = 0.0;
+  hip_check(hipStreamSynchronize(e->st), "synchronize");
+  hip_check(hipMemcpy(&v, e->d_cool + zone_of(h, i, j, k), sizeof(double), hipMemcpyDeviceToHost), "Cool D2H");
+  return v;
+}
+
+// src/Cell/cell_modify_data.c:11-13
+void cell_add_cons(Cell *cell, int q, double add) {
+  CellRef *r = reinterpret_cast<CellRef *>(cell);
+  if (r->own || q < 0 || q >= r->h->NQ) {
+    disco_set_error(17, "cell_add_cons: a cell of a Cell *** handle and 0 <= q < NUM_Q");
+    return;
+  }
+  mirror_push(r->h);
+  Engine *e = r->h->eng;
+  double *uc = e->unit_cons();
+  if (!uc) return;
+  launch_unit_add(uc, (long long)q * e->Ntot + r->zone, add, e->st);
+}
+
+void cell_copy(Cell ***c, Sim *) {  // src/Cell/cell_misc.c:28-41
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  double *uc = e->unit_cons();
+  if (!uc) return;
+  launch_unit(e->mesh(), e->plan, e->st, e->NQ, UNIT_COPY, uc, 0.0, 0.0, nullptr, nullptr, 0);
+}
+void cell_adjust_RK_cons(Cell ***c, Sim *, double RK) {  // src/Cell/cell_misc.c:43-57
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  double *uc = e->unit_cons();
+  if (!uc) return;
+  launch_unit(e->mesh(), e->plan, e->st, e->NQ, UNIT_ADJUST, uc, RK, 0.0, nullptr, nullptr, 0);
+}
+
+// src/Cell/cell_source.c:73-300, src/Cell/cell_add_split_fictitious.c:11-44
+void cell_add_src(Cell ***c, Sim *, GravMass *gm, double dt) { unit_src(c, gm, dt, 0); }
+void cell_add_visc_src(Cell ***c, Sim *, GravMass *gm, double dt) { unit_src(c, gm, dt, 1); }
+void cell_add_visc_src_old(Cell ***c, Sim *, double dt) { unit_src(c, nullptr, dt, 2); }
+void cell_add_split_fictitious(Cell ***c, Sim *, GravMass *gm, double dt) { unit_src(c, gm, dt, 3); }
+
+// src/Cell/cell_misc.c:59-96
+void cell_update_phi(Cell ***c, Sim *, GravMass *gm, double RK, double dt) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  if (gm) scalars_from_host(e, nullptr, gm);  // the frame rotation at the masses' separation
+  launch_unit(e->mesh(), e->plan, e->st, e->NQ, UNIT_UPDATE_PHI, nullptr, RK, dt, nullptr, nullptr, 0);
+  h->stale = true;
+}
+void cell_update_dphi(Cell ***c, Sim *) {
+  CellsHandle *h = H(c);
+  mirror_push(h);
+  Engine *e = h->eng;
+  launch_unit(e->mesh(), e->plan, e->st, e->NQ, UNIT_UPDATE_DPHI, nullptr, 0.0, 0.0, nullptr, nullptr, 0);
+  h->stale = true;
+}
+
+// src/Cell/cell_boundary.c:14-139 with the r faces of the last face_create
+// (the ghost-annulus tile lists exist for BoundTypeR outflow)
+void cell_boundary_outflow_r(Cell ***c, Face *, Sim *s, MPIsetup *, TimeStep *) {
+  CellsHandle *h = H(c);
+  if (s->BoundTypeR != BOUND_OUTFLOW) {
+    disco_set_error(17, "cell_boundary_outflow_r: the Sim's BoundTypeR is not outflow");
+    return;
+  }
+  mirror_push(h);
+  Engine *e = h->eng;
+  launch_bc_outflow_r(e->mesh(), e->plan, e->st, e->NQ);
+  h->stale = true;
+}
+// src/Cell/cell_boundary.c:141-206 with the z faces of the last face_create
+void cell_boundary_outflow_z(Cell ***c, Face *, Sim *s, MPIsetup *, TimeStep *) {
+  CellsHandle *h = H(c);
+  if (s->N_global[1] <= 1) return;
+  mirror_push(h);
+  Engine *e = h->eng;
+  launch_unit(e->mesh(), e->plan, e->st, e->NQ, UNIT_OUTFLOW_Z, nullptr, 0.0, 0.0, nullptr, nullptr, 0);
+  h->stale = true;
+}
+// src/Cell/cell_boundary.c:208-254 and src/Cell/cell_misc.c:98-132
+void cell_boundary_fixed_r(Cell ***c, Sim *s, GravMass *, MPIsetup *,
+                           void (*fn)(Cell *, Sim *, GravMass *, int, int, int)) {
+  if (own_plugin(s, fn, "cell_boundary_fixed_r")) unit_bc(c, UNIT_FIXED_R, 0.0);
+}
+void cell_boundary_fixed_z(Cell ***c, Sim *s, GravMass *, MPIsetup *,
+                           void (*fn)(Cell *, Sim *, GravMass *, int, int, int)) {
+  if (own_plugin(s, fn, "cell_boundary_fixed_z")) unit_bc(c, UNIT_FIXED_Z, 0.0);
+}
+void cell_bc_damp(Cell ***c, Sim *s, GravMass *, double dt, void (*fn)(Cell *, Sim *, GravMass *, int, int, int)) {
+  if (!(s->DAMP_TIME > 0.0)) {
+    disco_set_error(17, "cell_bc_damp: DAMP_TIME must be positive (the damping rate is 1/DAMP_TIME/2/pi)");
+    return;
+  }
+  if (own_plugin(s, fn, "cell_bc_damp")) unit_bc(c, UNIT_DAMP, dt);
+}
+
+// declared at src/Headers/Cell.h:91 and never defined in the reference: the
+// cell's primitives, edge and width on stdout
+void cell_print(Cell ***c, int i, int j, int k) {
+  Cell *x = cell_single(c, i, j, k);
+  printf("cell (%d %d %d): tiph %e dphi %e prim", i, j, k, cell_tiph(x), cell_dphi(x));
+  for (int q = 0; q < H(c)->NQ; ++q) printf(" %e", cell_prim(x, q));
+  printf("\n");
+}
+
+// ------------------------------------------------------------------ Riemann
+// src/Riemann/Riemann_create_destroy.c:9-53, Riemann_setup.c:11-80,
+// Riemann_routines.c:314-397: one face at a time, solved on the device
+Riemann *riemann_create(Sim *) { return new Riemann(); }
+void riemann_destroy(Riemann *r) { delete r; }
+void riemann_setup_rz(Riemann *R, Face *F, Sim *, int n, int direction) {
+  if (!F || n < 0 || n >= (int)F->L.size() || (direction != 0 && direction != 2)) {
+    disco_set_error(18, "riemann_setup_rz: no such face (direction 0 for r faces, 2 for z)");
+    R->ready = false;
+    return;
+  }
+  R->h = F->h;
+  UnitFace &f = R->f;
+  f.kind = direction;
+  f.cL = F->L[n];
+  f.cR = F->R[n];
+  f.ann_L = ann_of_zone(F->h->eng, f.cL);
+  f.r = F->r[n];
+  f.dA = F->dA[n];
+  f.cm = F->cm[n];
+  f.deltaL = F->deltaL[n];
+  f.deltaR = F->deltaR[n];
+  f.r_cell_L = direction == 0 ? f.r - f.deltaL : f.r;
+  f.r_cell_R = direction == 0 ? f.r + f.deltaR : f.r;
+  R->ready = true;
+}
+void riemann_setup_p(Riemann *R, Cell ***c, Sim *s, int i, int j_low, int k, int) {
+  CellsHandle *h = H(c);
+  const int N = sim_N_p(s, i);
+  if (i < 0 || i >= sim_N(s, 0) || k < 0 || k >= sim_N(s, 1) || j_low < 0 || j_low >= N) {
+    disco_set_error(18, "riemann_setup_p: no such cell");
+    R->ready = false;
+    return;
+  }
+  const int j_hi = j_low == N - 1 ? 0 : j_low + 1;
+  R->h = h;
+  UnitFace &f = R->f;
+  f.kind = 1;
+  f.cL = zone_of(h, i, j_low, k);
+  f.cR = zone_of(h, i, j_hi, k);
+  f.ann_L = k * h->eng->hdp[0].NR + i;
+  const double zm = sim_FacePos(s, k - 1, 1), zp = sim_FacePos(s, k, 1);
+  const double dz = zp - zm;
+  const double rm = sim_FacePos(s, i - 1, 0), rp = sim_FacePos(s, i, 0);
+  const double dr = rp - rm;
+  const double r = .5 * (rp + rm);
+  f.dA = dr * dz;
+  f.r = r;
+  f.cm = 0.0;  // the left cell's tiph, read on the device
+  f.deltaL = f.deltaR = 0.0;
+  f.r_cell_L = r;
+  f.r_cell_R = r;
+  R->ready = true;
+}
+void riemann_AddFlux(Riemann *R, Sim *, GravMass *gm, double dt) {
+  if (!R->ready) {
+    disco_set_error(18, "riemann_AddFlux before riemann_setup_rz / riemann_setup_p");
+    return;
+  }
+  CellsHandle *h = R->h;
+  mirror_push(h);
+  Engine *e = h->eng;
+  if (gm) scalars_from_host(e, nullptr, gm);
+  double *uc = e->unit_cons();
+  if (!uc) return;
+  launch_unit_face(e->mesh(), e->st, e->NQ, uc, R->f, dt);
+}
+
+// declared at src/Headers/Diagnostics.h:31 and never defined in the
+// reference: starts new time averages at the current time
+void diagnostics_reset(Diagnostics *d, Cell ***, Sim *, TimeStep *ts) {
+  std::fill(d->vec.begin(), d->vec.end(), 0.0);
+  std::fill(d->scal.begin(), d->scal.end(), 0.0);
+  d->toutprev = d->toutprev_dump = timestep_get_t(ts);
+}
+// declared at src/Headers/Sim.h:89 and never defined in the reference; no
+// parfile key sets it (src/Sim/sim_readpar.c:55-106): 0
+int sim_ZeroPsiBndry(Sim *) { return 0; }
+
+}  // extern "C"

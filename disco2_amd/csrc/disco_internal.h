@@ -119,6 +119,10 @@ struct CellsHandle {
 struct CellRef {
   CellsHandle *h;
   long long zone;
+  // a cell of its own (cell_single_create): NQ primitives, then tiph, dphi,
+  // wiph; null for a cell of a Cell *** handle
+  double *own = nullptr;
+  int NQ = 0;
 };
 
 struct Face {  // host copy of one face list (unit-level access)

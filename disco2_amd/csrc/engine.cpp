@@ -365,7 +365,7 @@ Engine::~Engine() {
   void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
                   d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_mdot_tile, d_ann_rot, d_init, d_pair_perm, adj_mem, diag_mem,
-                  ckpt_mem};
+                  ckpt_mem, d_ucons, d_cool, d_umdot, d_urows};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (ev0) hipEventDestroy(ev0);
@@ -462,6 +462,10 @@ void Engine::download(int d, int field, double *dst) {
       break;
     }
     case 1: {  // prim2cons with cell_calc_cons volumes (src/Cell/cell_conversion.c:35-55)
+      if (ucons_live && d_ucons) {  // the unit-level conserved state
+        get(d_ucons, NQ, dst);
+        break;
+      }
       std::vector<double> pr((size_t)NQ * nz), dp(nz);
       get(d_prim[cur], NQ, pr.data());
       get(d_dphi[cur], 1, dp.data());
@@ -707,4 +711,36 @@ int Engine::check_error() {
   if (e == DERR_VISC_VZ)
     disco_set_error(31, "viscosity needs v_z = 0 (the reference exits in riemann_visc_flux)");
   return e;
+}
+
+// The unit-level conserved state: allocated on the first unit-level call,
+// and while not live it is what cell_calc_cons would leave (prim2cons of the
+// primitives), derived on demand.
+double *Engine::unit_cons() {
+  if (!d_ucons) {
+    if (!dalloc(&d_ucons, (size_t)NQ * Ntot, "unit cons") || !dalloc(&d_cool, (size_t)Ntot, "cool") ||
+        !dalloc(&d_umdot, 2 * (size_t)(plan.n_ztiles ? plan.n_ztiles : 1), "unit sink"))
+      return nullptr;
+    hipMemsetAsync(d_cool, 0, sizeof(double) * Ntot, st);
+    ucons_live = false;
+  }
+  if (!ucons_live) {
+    launch_unit(mesh(), plan, st, NQ, UNIT_DERIVE, d_ucons, 0.0, 0.0, nullptr, nullptr, 0);
+    ucons_live = true;
+  }
+  return d_ucons;
+}
+
+// the plugin's per-annulus single-cell rows of every disk, as fixed
+// boundaries and damping would see them whatever the disk's boundary type
+InitRow *Engine::unit_rows() {
+  if (d_urows) return d_urows;
+  std::vector<InitRow> all;
+  for (int d = 0; d < ndisk; ++d) {
+    std::vector<InitRow> rows;
+    if (!disco_init_rows(sims[d], NQ, rows, nullptr, true)) return nullptr;
+    all.insert(all.end(), rows.begin(), rows.end());
+  }
+  if (!dalloc(&d_urows, all.size(), "unit rows") || !h2d(d_urows, all.data(), all.size())) return nullptr;
+  return d_urows;
 }

@@ -7,6 +7,7 @@
 
 #include "disco_internal.h"
 #include "kernels.h"
+#include "unit.h"
 
 // What the adjoint keeps of one substep: the state it produced (ghosts
 // included) and everything its backward reads.  The forward writes these
@@ -137,6 +138,14 @@ struct Engine {
   int checkpoint_unpack(int d, const double *rows);
   int checkpoint_write(int d, const char *fname);
   int checkpoint_read(int d, const char *fname);
+  // unit-level class API (capi.cpp, unit.hip): the reference's cell->cons as
+  // its own array while a driver calls the substep's pieces one by one; it
+  // follows the primitives (prim2cons) until a unit-level call changes it
+  double *d_ucons = nullptr, *d_cool = nullptr, *d_umdot = nullptr;
+  InitRow *d_urows = nullptr;
+  bool ucons_live = false;
+  double *unit_cons();      // the live conserved state (allocated and derived on first use); null on error
+  InitRow *unit_rows();     // the disk's single-cell plugin rows on the device; null on error
   void *ckpt_mem = nullptr;
   size_t ckpt_cap = 0;
   void *diag_mem = nullptr;

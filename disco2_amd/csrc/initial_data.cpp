@@ -349,15 +349,17 @@ int disco_single_init(const Sim *S, int type, int i, double tiph, double dphi, d
   return -1;
 }
 
-bool disco_init_rows(const Sim *S, int NQ, std::vector<InitRow> &rows, bool *any) {
+bool disco_init_rows(const Sim *S, int NQ, std::vector<InitRow> &rows, bool *any, bool values_always) {
   const int NR = (int)S->N_p.size(), NZ = (int)S->z_faces.size() - 1;
   const bool fixed_r = S->BoundTypeR == BOUND_FIXED;
   const bool fixed_z = S->N_global[1] > 1 && S->BoundTypeZ == BOUND_FIXED;
   const bool damp = S->DAMP_TIME > 0.0;
   rows.assign((size_t)NR * NZ, InitRow());
   for (auto &w : rows) memset(&w, 0, sizeof(w));
+  bool any_ = false;
+  if (!any) any = &any_;
   *any = false;
-  if (!fixed_r && !fixed_z && !damp) return true;
+  if (!fixed_r && !fixed_z && !damp && !values_always) return true;
   if (!valid_type(S->InitialDataType, "single point")) return false;
   const int type = S->InitialDataType;
   if (type == ID_TORUS) {

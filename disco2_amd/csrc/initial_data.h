@@ -28,4 +28,6 @@ int disco_single_init(const Sim *S, int type, int i, double tiph, double dphi, d
 // damping; damping whose single-cell values depend on the uninitialised
 // cell cell_bc_damp passes, src/Cell/cell_misc.c:107,123).  *any is set when
 // some row asks for device work.
-bool disco_init_rows(const Sim *S, int NQ, std::vector<InitRow> &rows, bool *any);
+// values_always: the rows' single-cell values also where the disk's set-up
+// uses no fixed boundary or damping (the class API's unit-level calls).
+bool disco_init_rows(const Sim *S, int NQ, std::vector<InitRow> &rows, bool *any, bool values_always = false);

@@ -126,3 +126,12 @@ void launch_copy(double *dst, const double *src, long long n, hipStream_t st);
 // unpack writes the primitives and tiph (the current state's buffers)
 void launch_ckpt_pack(const DevMesh &m, int disk, int nann, double *out, hipStream_t st);
 void launch_ckpt_unpack(const DevMesh &m, int disk, int nann, const double *in, double *tiph, hipStream_t st);
+
+// NUM_Q = 5 + NUM_N, NUM_N = 0..3: one instantiation of every kernel each
+#define DISCO_NQ_DISPATCH(NQ, CALL)      \
+  switch (NQ) {                          \
+    case 5: { constexpr int Q = 5; CALL; } break; \
+    case 6: { constexpr int Q = 6; CALL; } break; \
+    case 7: { constexpr int Q = 7; CALL; } break; \
+    default: { constexpr int Q = 8; CALL; } break; \
+  }

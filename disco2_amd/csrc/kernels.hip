@@ -7,6 +7,7 @@
 // atomics), which keeps the results deterministic and bitwise comparable.
 #include <hip/hip_runtime.h>
 
+#include "cell_ops.h"
 #include "device_common.h"
 
 // ----------------------------------------------------------------------------
@@ -960,99 +961,24 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
           }
         });
     }
-    // cell_add_src, src/Cell/cell_source.c:73-214
+    // cell_add_src, cell_add_visc_src, cell_add_split_fictitious
+    // (src/Cell/cell_source.c:73-277, cell_add_split_fictitious.c:11-44)
     const double traw = m.tiph[c];
     const double tcl = clean_pi(traw);
-    const GravState g0 = sc.gm[0], g1 = sc.gm[1];
     {
       double phi = tcl - .5 * dphi;
       double cphi, sphi;
       sincos(phi, &sphi, &cphi);
-      double rho = pr[RHO], Pp = pr[PPP];
       double dV = dphi * .5 * (rp * rp - rm * rm) * dz;
       double zc = .5 * (zp + zm);
-      double rdrOm_a = rot.y;
-      double Fp;
-      add_geom_grav_src(u, pr, r, dV, dt, cphi, sphi, zc, rot.x, rot.y, rot.z, P, sc, &Fp);
-      if (m.torque_tile && interior_cell(P, i, k)) tq += dt * dV * rho * Fp * r;
-      if ((FEAT & F_SINK) && P.sink == 1) {  // get_rho_sink, :15-47
-        double xpos = r * cphi, ypos = r * sphi;
-        double xbh0 = sc.xbh[0], ybh0 = sc.ybh[0];
-        double xbh1 = sc.xbh[1], ybh1 = sc.ybh[1];
-        double r0 = sqrt((xpos - xbh0) * (xpos - xbh0) + (ypos - ybh0) * (ypos - ybh0));
-        double r1 = sqrt((xpos - xbh1) * (xpos - xbh1) + (ypos - ybh1) * (ypos - ybh1));
-        double M0 = g0.M, M1 = g1.M;
-        double Mtotal = M0 + M1;
-        double alpha = P.visc;
-        double one_o_nu = 1.0 / (alpha * Pp / rho) * sqrt(pow(r0, -3) * M0 / Mtotal + pow(r1, -3) * M1 / Mtotal);
-        double t_visc0 = 1.0 * 2. / 3. * r0 * r0 * one_o_nu;
-        double t_visc1 = 1.0 * 2. / 3. * r1 * r1 * one_o_nu;
-        if (t_visc0 < 10. * dt) t_visc0 = 10. * dt;
-        if (t_visc1 < 10. * dt) t_visc1 = 10. * dt;
-        int imin = P.NgMinR == 1 ? 0 : P.NgMinR;
-        bool interior = i >= imin && i < P.NR - P.NgMaxR && k >= P.NgMinZ && k < P.NZ - P.NgMaxZ;
-        for (int pm = 0; pm < P.grav_type && pm < 2; ++pm) {
-          double drho = 0.0;
-          if (pm == 0 && r0 < 0.5) drho = rho / t_visc0;
-          if (pm == 1 && r1 < 0.5) drho = rho / t_visc1;
-          u[RHO] -= drho * dt * dV;
-          if (interior) {
-            if (pm == 0)
-              md0 += drho * dV;
-            else
-              md1 += drho * dV;
-          }
-        }
-      }
-      if ((FEAT & F_COOL) && P.cooling == 1) {
-        if (P.initial_data_type == 0) {
-          double cooling_term = (Pp - .01) / (P.gamma - 1.) * dV / 1.e-2;
-          if (cooling_term < 0.0) cooling_term = 0.0;
-          u[TAU] -= cooling_term * dt;
-        } else {
-          double cooling_cap = u[TAU] * 1.0;
-          double cooling_term = 9. / 4. * P.visc * pow(P.porho_r1, -3) / rho * pow(Pp / rho, 4.) * dt * dV;
-          if (cooling_term > cooling_cap) cooling_term = cooling_cap;
-          u[TAU] -= cooling_term;
-        }
-      }
-      // cell_add_visc_src, :216-277
-      if (visc) {
-        double xpos = r * cphi, ypos = r * sphi;
-        double xbh0 = sc.xbh[0], ybh0 = sc.ybh[0];
-        double xbh1 = sc.xbh[1], ybh1 = sc.ybh[1];
-        double dist_bh0 = sqrt((xpos - xbh0) * (xpos - xbh0) + (ypos - ybh0) * (ypos - ybh0));
-        double dist_bh1 = sqrt((xpos - xbh1) * (xpos - xbh1) + (ypos - ybh1) * (ypos - ybh1));
-        double eps = P.g_eps;
-        double Sigma_nu;
-        if (P.visc_const == 1)
-          Sigma_nu = rho * P.visc;
-        else
-          Sigma_nu = P.visc * Pp /
-                     sqrt(pow_m1p5(dist_bh0 * dist_bh0 + eps * eps) * g0.M + pow_m1p5(dist_bh1 * dist_bh1 + eps * eps) * g1.M);
-        u[TAU] += dt * dV * Sigma_nu * rdrOm_a *
-                  (rdrOm_a + r * m.grad[UPP * m.Ntot + c] + m.gradp[URR * m.Ntot + c]);
-      }
+      const double Fp =
+          src_grav_sink_cool<FEAT>(P, sc, rot, i, k, r, dV, zc, cphi, sphi, pr, u, dt, &md0, &md1, nullptr);
+      if (m.torque_tile && interior_cell(P, i, k)) tq += dt * dV * pr[RHO] * Fp * r;
+      if (visc)
+        src_visc<FEAT>(P, sc, rot, r, dV, cphi, sphi, pr, m.grad[UPP * m.Ntot + c], m.gradp[URR * m.Ntot + c], u,
+                       dt);
     }
-    // cell_add_split_fictitious, cell_add_split_fictitious.c:11-44 (a no-op
-    // without a frame rotation: kappa^2 = 0)
-    if (FEAT & F_FRAME) {
-      double rOm = rot.x;
-      double Omega = rOm / r;
-      double dr_r2Om = 2. * rOm + r * rot.y;
-      double kappa2 = 2. * Omega / r * dr_r2Om;
-      if (kappa2 > 0.) {
-        double kappa = sqrt(kappa2);
-        double l0 = u[LLL];
-        double rhov0 = u[SRR];
-        double skt, ckt;
-        sincos(kappa * dt, &skt, &ckt);
-        double l = l0 * ckt - u[SRR] * dr_r2Om / kappa * skt;
-        double rhov = rhov0 * ckt + 2.0 * Omega * u[LLL] / r / kappa * skt;
-        u[LLL] = l;
-        u[SRR] = rhov;
-      }
-    }
+    src_fict<FEAT>(rot, r, dt, u);
     // cell_update_phi + cell_update_dphi (the left neighbour's new edge is
     // recomputed here from its old state)
     const double wtot = m.wiph[tl.x] + rot.x;
@@ -1222,28 +1148,6 @@ __global__ __launch_bounds__(TILE) void k_sync_z(DevMesh m, const int2 *tiles) {
 // edge; a damped cell relaxes towards the initial data by 1 - exp(-rate dt).
 // Per cell the boundary comes first, as in the reference.
 // ----------------------------------------------------------------------------
-template <int NQ>
-__device__ __forceinline__ void single_init(int type, const InitRow &R, double r, double tiph, double dphi,
-                                            double *pr) {
-  if (type == ID_SHEAR) {  // cell_init_shear.c:11-55
-    const double t = tiph - .5 * dphi;
-    const double x = r * cos(t) - 3.;
-    const double y = r * sin(t);
-    double vy = 0.0;
-    if (fabs(y) < 50.) vy = R.c0 * exp(-x * x / R.c1);
-    pr[RHO] = 1.0;
-    pr[PPP] = 0.01;
-    pr[URR] = vy * sin(t);
-    pr[UPP] = vy * cos(t) / r - R.a;
-    pr[UZZ] = 0.0;
-    return;
-  }
-  if (!(R.flags & INIT_SET)) return;  // the vortex outside its core writes nothing
-#pragma unroll
-  for (int q = 0; q < 5; ++q) pr[q] = R.prim[q];
-  if (type == ID_VORTEX && NQ > 5) pr[5] = cos(tiph - .5 * dphi) < 0.0 ? 0.0 : 1.0;  // cell_init_vortex.c:146,152
-}
-
 template <int NQ>
 __global__ __launch_bounds__(TILE) void k_bc_init(DevMesh m, const int2 *tiles, double fac) {
   const int2 tl = tiles[tile_of(m)];
@@ -1572,14 +1476,6 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   if (L.n_sz) KLAUNCH(st, "k_sync_z", hipLaunchKernelGGL(k_sync_z<NQ>, dim3(L.n_sz), B, 0, st, m2, L.sz));
 }
 
-// NUM_Q = 5 + NUM_N, NUM_N = 0..3: one instantiation of every kernel each
-#define DISCO_NQ_DISPATCH(NQ, CALL)      \
-  switch (NQ) {                          \
-    case 5: { constexpr int Q = 5; CALL; } break; \
-    case 6: { constexpr int Q = 6; CALL; } break; \
-    case 7: { constexpr int Q = 7; CALL; } break; \
-    default: { constexpr int Q = 8; CALL; } break; \
-  }
 
 // The compiled physics variants (feature sets, F_*): NUM_Q 5 has a variant
 // per common configuration, every NUM_Q the general one.  The plan's feat is
@@ -1657,4 +1553,13 @@ static void launch_plm_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, 
 
 void launch_plm(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int which) {
   DISCO_NQ_DISPATCH(NQ, launch_plm_q<Q>(m, L, st, which));
+}
+
+// cell_boundary_outflow_r on its own (the class API): both stages as two
+// launches on the current state
+void launch_bc_outflow_r(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ) {
+  DISCO_NQ_DISPATCH(NQ, {
+    if (L.n_bc0) hipLaunchKernelGGL(k_bc_r<Q>, dim3(L.n_bc0), dim3(TILE), 0, st, m, L.bc0);
+    if (L.n_bc1) hipLaunchKernelGGL(k_bc_r<Q>, dim3(L.n_bc1), dim3(TILE), 0, st, m, L.bc1);
+  });
 }

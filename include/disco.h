@@ -33,6 +33,7 @@ struct MPIsetup;
 struct TimeStep;
 struct Diagnostics;
 struct IO;
+struct Riemann;
 
 /* ---- MPIsetup: src/Headers/MPIsetup.h:21-38 (single rank) ---- */
 struct MPIsetup *mpisetup_create(int argc, char **argv);          /* MPIsetup.h:21 */
@@ -112,6 +113,7 @@ int sim_GravMassType(struct Sim *);                                /* Sim.h:130 
 double sim_rOm_a(struct Sim *, double r, double a);                /* Sim.h:137 */
 double sim_rdrOm_a(struct Sim *, double r, double a);              /* Sim.h:138 */
 double sim_dtOm_a(struct Sim *, double r, double a);               /* Sim.h:139 */
+int sim_ZeroPsiBndry(struct Sim *);  /* Sim.h:89: declared, never defined in the reference; returns 0 */
 /* No reference counterpart: programmatic set-up instead of a parfile.
  * sim_set_param takes any parfile key (src/Sim/sim_readpar.c:55-106) and
  * returns 0, or 1 for an unknown key.  sim_set_N_p_array replaces the N_p
@@ -155,6 +157,11 @@ double cell_gradp(struct Cell *, int q);                           /* Cell.h:49 
 double cell_tiph(struct Cell *);                                   /* Cell.h:51 */
 double cell_dphi(struct Cell *);                                   /* Cell.h:52 */
 double cell_wiph(struct Cell *);                                   /* Cell.h:53 */
+double cell_Cool(struct Cell ***, int i, int j, int k);            /* Cell.h:54 (0 until a cooling cell_add_src) */
+/* a cell of its own (Cell.h:27-28): primitives, tiph, dphi, wiph, zero until
+ * set; the single-cell initial-data functions write it */
+struct Cell *cell_single_create(struct Sim *);                     /* Cell.h:27 */
+void cell_single_destroy(struct Cell *);                           /* Cell.h:28 */
 /* Initial-data plugins (src/Cell/initial_data/cell_init_*.c), on the host over the
  * cells' host copy.  The global functions set every cell the reference sets
  * from its edge and the masses; the single-cell functions set one cell (the
@@ -196,6 +203,49 @@ void cell_set_w(struct Cell ***, struct Sim *);                    /* Cell.h:67 
 void cell_plm_p(struct Cell ***, struct Sim *);                    /* Cell.h:73 */
 void cell_plm_rz(struct Cell ***, struct Sim *, struct Face *, struct TimeStep *,
                  struct MPIsetup *, int direction);                /* Cell.h:72 */
+/* The rest of what timestep_substep calls (src/TimeStep/timestep_routines.c:
+ * 16-113), each a device launch on the current state and on the conserved
+ * state the reference keeps in cell->cons.  That conserved state follows the
+ * primitives (prim2cons) until one of these calls changes it; cell_calc_cons
+ * makes it prim2cons of the primitives again, and timestep_rk2 / _substep /
+ * _forward_euler end with it, as in the reference.  The engine's own steps
+ * start from prim2cons of the primitives (the reference's state after
+ * cell_calc_cons).  Cell angles are read as stored: call cell_clean_pi first,
+ * as timestep_substep does. */
+void cell_copy(struct Cell ***, struct Sim *);                     /* Cell.h:87 */
+void cell_adjust_RK_cons(struct Cell ***, struct Sim *, double RK); /* Cell.h:88 */
+void cell_add_cons(struct Cell *, int q, double add);              /* Cell.h:56 */
+void cell_add_src(struct Cell ***, struct Sim *, struct GravMass *, double dt);      /* Cell.h:57 (sets Mdot) */
+void cell_add_visc_src(struct Cell ***, struct Sim *, struct GravMass *, double dt); /* Cell.h:58 */
+void cell_add_visc_src_old(struct Cell ***, struct Sim *, double dt);               /* Cell.h:59 */
+void cell_update_phi(struct Cell ***, struct Sim *, struct GravMass *, double RK, double dt); /* Cell.h:61 */
+void cell_update_dphi(struct Cell ***, struct Sim *);              /* Cell.h:62 */
+void cell_add_split_fictitious(struct Cell ***, struct Sim *, struct GravMass *, double dt); /* Cell.h:63 */
+/* boundaries (Cell.h:75-79): the outflow ones use the faces of the last
+ * face_create in their direction; outflow_r needs BoundTypeR outflow; the
+ * fixed ones and the damping accept only the Sim's own single-cell function
+ * (cell_single_init_ptr), whose values the device evaluates */
+void cell_boundary_outflow_r(struct Cell ***, struct Face *, struct Sim *, struct MPIsetup *, struct TimeStep *);
+void cell_boundary_outflow_z(struct Cell ***, struct Face *, struct Sim *, struct MPIsetup *, struct TimeStep *);
+void cell_boundary_fixed_r(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *,
+                           void (*)(struct Cell *, struct Sim *, struct GravMass *, int, int, int));
+void cell_boundary_fixed_z(struct Cell ***, struct Sim *, struct GravMass *, struct MPIsetup *,
+                           void (*)(struct Cell *, struct Sim *, struct GravMass *, int, int, int));
+void cell_bc_damp(struct Cell ***, struct Sim *, struct GravMass *, double dt,
+                  void (*)(struct Cell *, struct Sim *, struct GravMass *, int, int, int));
+void cell_print(struct Cell ***, int i, int j, int k);  /* Cell.h:91: declared, never defined in the reference */
+
+/* ---- Riemann: src/Headers/Riemann.h:37-42 ----
+ * One face per riemann_AddFlux, solved on the device and added to the two
+ * cells' conserved state (the reference's per-face create / setup / AddFlux
+ * / destroy loop of timestep_routines.c:42-71).  direction: 0 (r faces) or 2
+ * (z faces) for setup_rz, 1 for setup_p. */
+struct Riemann *riemann_create(struct Sim *);                                          /* Riemann.h:37 */
+void riemann_destroy(struct Riemann *);                                                /* Riemann.h:38 */
+void riemann_setup_rz(struct Riemann *, struct Face *, struct Sim *, int n, int direction); /* Riemann.h:40 */
+void riemann_setup_p(struct Riemann *, struct Cell ***, struct Sim *, int i, int j_low, int k,
+                     int direction);                                                   /* Riemann.h:41 */
+void riemann_AddFlux(struct Riemann *, struct Sim *, struct GravMass *, double dt);    /* Riemann.h:42 */
 /* No reference counterpart: bulk transfer of whole fields, zones in the
  * reference's (k, i, j) order, NQ-vectors q-major.  field: 0 prim, 3 grad,
  * 4 gradp (NQ*Nzones); 5 tiph, 7 dphi, 8 wiph (Nzones); 1 cons (derived). */
@@ -251,6 +301,9 @@ void diagnostics_set(struct Diagnostics *, struct Cell ***, struct Sim *, struct
 void diagnostics_print(struct Diagnostics *, struct TimeStep *, struct Sim *, struct MPIsetup *); /* Diagnostics.h:35 */
 double diagnostics_tdiag_measure(struct Diagnostics *);                                  /* Diagnostics.h:37 */
 double diagnostics_tdiag_dump(struct Diagnostics *);                                     /* Diagnostics.h:38 */
+/* Diagnostics.h:31: declared, never defined in the reference; here it starts
+ * new time averages at the current time */
+void diagnostics_reset(struct Diagnostics *, struct Cell ***, struct Sim *, struct TimeStep *);
 
 /* ---- IO (checkpoints): src/Headers/IO.h:19-34 (single rank) ----
  * Files of the reference's layout: checkpoint_NNNN.h5 written by io_hdf5_out,

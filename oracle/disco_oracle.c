@@ -1691,6 +1691,21 @@ int oracle_rk2(OracleSim *s) { /* src/TimeStep/timestep_algorithm.c:11-41 */
   return 0;
 }
 
+int oracle_forward_euler(OracleSim *s) { /* src/TimeStep/timestep_algorithm.c:43-57 */
+  int e;
+  oracle_set_w(s);
+  if (s->err) return s->err;
+  s->dt = oracle_mindt(s); /* timestep_set_dt */
+  if (s->t + s->dt > s->dp[OD_T_MAX]) s->dt = s->dp[OD_T_MAX] - s->t;
+  cell_copy(s);
+  grav_copy(s);
+  s->RK = 0.0;
+  if ((e = substep(s, 1.0))) return s->err = e;
+  grav_move(s, s->t, 1.0 * s->dt);
+  s->t += s->dt; /* timestep_update_t; no Macc update */
+  return 0;
+}
+
 int oracle_run(OracleSim *s, int nsteps) {
   for (int n = 0; n < nsteps; ++n) {
     int e = oracle_rk2(s);

@@ -49,6 +49,8 @@ void oracle_init(OracleSim *s);
 /* one timestep_rk2; returns 0, or a nonzero error code */
 int oracle_rk2(OracleSim *s);
 int oracle_run(OracleSim *s, int nsteps);
+/* one timestep_forward_euler (src/TimeStep/timestep_algorithm.c:43-57) */
+int oracle_forward_euler(OracleSim *s);
 double oracle_torque(OracleSim *s);
 /* field: 0 prim, 1 cons, 2 RKcons, 3 grad, 4 gradp (NQ*Nz each);
  *        5 tiph, 6 RKtiph, 7 dphi, 8 wiph (Nz each) */

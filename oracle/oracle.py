@@ -57,6 +57,8 @@ def lib():
         L.oracle_rk2.restype = ctypes.c_int
         L.oracle_run.argtypes = [vp, ctypes.c_int]
         L.oracle_run.restype = ctypes.c_int
+        L.oracle_forward_euler.argtypes = [vp]
+        L.oracle_forward_euler.restype = ctypes.c_int
         L.oracle_get.argtypes = [vp, ctypes.c_int, dp]
         L.oracle_put.argtypes = [vp, ctypes.c_int, dp]
         L.oracle_face_create.argtypes = [vp, ctypes.c_int]
@@ -141,6 +143,13 @@ class Oracle:
         e = lib().oracle_run(self.h, n)
         if e:
             raise RuntimeError(f"oracle error {e}")
+
+    def euler(self, n=1):
+        """n timestep_forward_euler steps"""
+        for _ in range(n):
+            e = lib().oracle_forward_euler(self.h)
+            if e:
+                raise RuntimeError(f"oracle error {e}")
 
     def get(self, field):
         n = self.Nz * (self.NQ if FIELDS[field] <= 4 else 1)

@@ -295,6 +295,25 @@ int main(int argc, char **argv) {
     return 0;
   }
 
+  if (strcmp(mode, "euler") == 0) {
+    /* timestep_forward_euler (src/TimeStep/timestep_algorithm.c:43-57), the
+     * reference's one-substep step that src/main.c never calls */
+    int nsteps = atoi(argv[5]), s;
+    g_out = fopen(argv[4], "wb");
+    dump_geom(sim);
+    FILE *so = freopen("/dev/null", "w", stdout);
+    (void)so;
+    for (s = 1; s <= nsteps; ++s) {
+      char nm[32];
+      timestep_forward_euler(ts, sim, cells, gm, mpi);
+      snprintf(nm, sizeof nm, "euler%d", s);
+      dump_state(nm, cells, sim, gm, ts);
+    }
+    fclose(g_out);
+    MPI_Finalize();
+    return 0;
+  }
+
   if (strcmp(mode, "run") == 0 || strcmp(mode, "time") == 0) {
     int timing = strcmp(mode, "time") == 0;
     int nsteps = atoi(argv[timing ? 4 : 5]);

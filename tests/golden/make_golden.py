@@ -35,21 +35,22 @@ REF_PARFILES = "/root/reference/parfiles"
 
 def cases():
     c = {}
-    c["vortex_r16"] = (configs.vortex(16), dict(steps=20, dumps=[1, 2, 20], unit=True))
+    c["vortex_r16"] = (configs.vortex(16), dict(steps=20, dumps=[1, 2, 20], unit=True, euler=2))
     p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=1, PLM=2.0, RiemannSolver=1, NoInnerBC=1)
     c["vortex_scalar_hll_noinner"] = ((p, k, dict(kw, x0=1.8)), dict(steps=12, dumps=[1, 12], unit=True))
     p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=3, PLM=2.0)
     c["vortex_three_scalars"] = ((p, k, dict(kw, x0=1.8)), dict(steps=8, dumps=[1, 8], unit=True))
     p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=2, PLM=1.5)
     c["vortex_two_scalars"] = ((p, k, dict(kw, x0=1.6)), dict(steps=8, dumps=[1, 8], unit=True))
-    c["binary_r16"] = (configs.binary(16, r_min=0.3, r_max=3.0), dict(steps=20, dumps=[1, 2, 20], unit=True))
+    c["binary_r16"] = (configs.binary(16, r_min=0.3, r_max=3.0), dict(steps=20, dumps=[1, 2, 20], unit=True,
+                                                                          euler=2))
     p, k, kw = configs.binary(16, r_min=0.3, r_max=3.0, q=0.5, w_numeric_type=0, w_analytic_type=1,
                               RLogScale=1.0, RhoSinkOn=1, ViscConst=0)
     c["binary_akepler_log_sink"] = ((p, k, kw), dict(steps=12, dumps=[1, 12], unit=True))
     p, k, kw = configs.binary(16, r_min=0.3, r_max=3.0, alpha=0.02, ViscConst=1, w_numeric_type=2)
     c["binary_viscconst_rigid"] = ((p, k, kw), dict(steps=8, dumps=[1, 8], unit=True))
     p, k, kw = configs.disk3d(8, 4, 24, R_Min=0.5, R_Max=2.5)
-    c["disk3d_r8z4"] = ((p, k, kw), dict(steps=10, dumps=[1, 2, 10], unit=True))
+    c["disk3d_r8z4"] = ((p, k, kw), dict(steps=10, dumps=[1, 2, 10], unit=True, euler=2))
     p, k, kw = configs.disk3d(8, 4, 20, R_Min=0.5, R_Max=2.5, N_p_round=0, HiResFac=1.0,
                               HiResSigma=0.3, HiResR0a=1.0, ZLogScale=0.5)
     c["disk3d_refNp_hires"] = ((p, k, kw), dict(steps=6, dumps=[1, 6], unit=True))
@@ -173,6 +174,11 @@ def make_case(name, spec, opts, tmp):
         u = orc.read_records(_run(["unit", par, ic, os.path.join(tmp, "u.bin")], tmp, "u.bin"))
         for k, v in u.items():
             if k.startswith(("faces_", "gradp", "grad_", "init/cons", "sub1/")):
+                out["unit/" + k] = v
+    if opts.get("euler"):  # timestep_forward_euler steps from the same initial state
+        u = orc.read_records(_run(["euler", par, ic, os.path.join(tmp, "e.bin"), str(opts["euler"])], tmp, "e.bin"))
+        for k, v in u.items():
+            if k.startswith("euler"):
                 out["unit/" + k] = v
     args = ["run", par, ic, os.path.join(tmp, "r.bin"), str(opts["steps"])] + [str(d) for d in opts["dumps"]]
     r = orc.read_records(_run(args, tmp, "r.bin"))

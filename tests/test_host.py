@@ -37,6 +37,20 @@ def test_library_exports_every_declared_symbol():
     assert len(declared_symbols()) > 150
 
 
+def test_every_reference_class_function_is_declared_and_exported():
+    """Every function the reference's class headers declare (src/Headers/*.h,
+    listed in tests/golden/reference_api.json) is declared in include/disco.h
+    and exported by the library: the drop-in boundary is the whole class API."""
+    import json
+    from disco2_amd._lib import lib
+    api = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_api.json")))
+    assert len(api) == 199
+    declared = set(declared_symbols())
+    L = lib()
+    missing = [f"{h}:{n} {name}" for h, n, name in api if name not in declared or not hasattr(L, name)]
+    assert not missing, missing
+
+
 @pytest.mark.parametrize("name", golden_names())
 def test_mesh_matches_reference(name):
     from disco2_amd.api import Setup

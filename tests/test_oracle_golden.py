@@ -74,3 +74,20 @@ def test_first_substep_matches_reference():
     o.step(1)
     # after a whole step the first-substep snapshot RKcons equals the initial cons
     assert bitwise_equal(o.get("RKcons"), g.get("unit/sub1/RKcons"))
+
+
+@pytest.mark.parametrize("name", [n for n in golden_names() if Golden(n).has("unit/euler1/prim")])
+def test_forward_euler_bitwise(name):
+    """timestep_forward_euler (src/TimeStep/timestep_algorithm.c:43-57): the
+    oracle's two steps against the reference's."""
+    g = Golden(name)
+    o = make_oracle(g)
+    for st in (1, 2):
+        o.euler(1)
+        for f in ("prim", "cons", "RKcons", "tiph", "RKtiph", "dphi", "wiph"):
+            k = f"unit/euler{st}/{f}"
+            assert bitwise_equal(o.get(f), g.get(k)), f"euler {st} {f}: {describe(o.get(f), g.get(k))}"
+        sc = o.scalars()
+        assert bitwise_equal(sc[:2], g.get(f"unit/euler{st}/t_dt"))
+        gm = g.get(f"unit/euler{st}/gravmass").reshape(2, 12)[:, :10].ravel()
+        assert bitwise_equal(sc[3:], gm)
