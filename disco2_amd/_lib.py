@@ -184,6 +184,8 @@ SIGNATURES = {
     "timestep_set_dt": (None, [vp, vp, vp, vp]),
     "timestep_set_RK": (None, [vp, cd]),
     "timestep_update_t": (None, [vp]),
+    "disco_cell_ledger_enable": (ci, [vp, ci]),
+    "disco_cell_ledger": (ci, [vp, pd]),
     "disco_last_error": (ci, []),
     "disco_last_error_msg": (ccp, []),
     "disco_clear_error": (None, []),

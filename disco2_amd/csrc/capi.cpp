@@ -1212,3 +1212,9 @@ void diagnostics_reset(Diagnostics *d, Cell ***, Sim *, TimeStep *ts) {
 int sim_ZeroPsiBndry(Sim *) { return 0; }
 
 }  // extern "C"
+
+extern "C" {
+// the conservation ledger of a class-API handle (no reference counterpart)
+int disco_cell_ledger_enable(Cell ***c, int on) { return H(c)->eng->ledger_enable(on != 0); }
+int disco_cell_ledger(Cell ***c, double *out) { return H(c)->eng->ledger_last(0, out); }
+}

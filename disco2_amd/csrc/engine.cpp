@@ -365,7 +365,7 @@ Engine::~Engine() {
   void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
                   d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_mdot_tile, d_ann_rot, d_init, d_pair_perm, adj_mem, diag_mem,
-                  ckpt_mem, d_ucons, d_cool, d_umdot, d_urows};
+                  ckpt_mem, d_ucons, d_cool, d_umdot, d_urows, d_led_torque};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (ev0) hipEventDestroy(ev0);
@@ -411,7 +411,7 @@ DevMesh Engine::mesh() const {
   m.sched_xcd = sched.xcd;
   m.pair_perm = sched.largest_first ? d_pair_perm : nullptr;
   m.u_save = adj_usave;
-  m.torque_tile = adj_torque;
+  m.torque_tile = adj_torque ? adj_torque : d_led_torque;
   m.ftype = adj_ftype;
   return m;
 }
@@ -743,4 +743,55 @@ InitRow *Engine::unit_rows() {
   }
   if (!dalloc(&d_urows, all.size(), "unit rows") || !h2d(d_urows, all.data(), all.size())) return nullptr;
   return d_urows;
+}
+
+int Engine::ledger_enable(bool on) {
+  if (on && !d_led_torque) {
+    if (!dalloc(&d_led_torque, (size_t)plan.n_utiles, "ledger")) return -1;
+    hipMemsetAsync(d_led_torque, 0, sizeof(double) * plan.n_utiles, st);
+  } else if (!on && d_led_torque) {
+    hip_check(hipStreamSynchronize(st), "synchronize");
+    hipFree(d_led_torque);
+    d_led_torque = nullptr;
+  }
+  return 0;
+}
+
+// The last substep's boundary r-face increments and torque term of disk d:
+// out[0, NQ + 3) what the faces between the inner ghost annulus and the first
+// interior annulus added to their R cell (dt*dA*F[q], then dt*dA*Fvisc[SRR,
+// LLL, TAU] when viscous), out[NQ + 3, 2 (NQ + 3)) the same for the faces
+// between the last interior annulus and the outer ghosts, out[2 (NQ + 3)]
+// the torque term sum dt dV rho F_phi r of the interior cells; sums in face
+// and tile order
+int Engine::ledger_last(int d, double *out) {
+  const DiskParams &P = hdp[d];
+  if (!d_led_torque || P.NglobalZ != 1 || P.NgMinR != 1) {
+    disco_set_error(19, "ledger: enable it before the substep; 2D disks with one inner ghost annulus");
+    return -1;
+  }
+  if (!hip_check(hipStreamSynchronize(st), "synchronize")) return -1;
+  const int nv = NQ + 3;
+  const int pairs[2] = {P.rpair_base, P.rpair_base + P.NR - 3};
+  for (int side = 0; side < 2; ++side) {
+    const PairInfo &Q = hpairs[pairs[side]];
+    PairState ps;
+    hip_check(hipMemcpy(&ps, d_pst + pairs[side], sizeof(ps), hipMemcpyDeviceToHost), "ledger D2H");
+    std::vector<double> row(ps.count > 0 ? ps.count : 1);
+    for (int v = 0; v < nv; ++v) {
+      double s = 0.0;
+      if (v < NQ || P.visc > 0.0) {
+        hip_check(hipMemcpy(row.data(), d_Fface + (size_t)v * NFtot + Q.face_base, sizeof(double) * ps.count,
+                            hipMemcpyDeviceToHost), "ledger D2H");
+        for (int n = 0; n < ps.count; ++n) s += row[n];
+      }
+      out[side * nv + v] = s;
+    }
+  }
+  std::vector<double> tq(plan.n_utiles);
+  hip_check(hipMemcpy(tq.data(), d_led_torque, sizeof(double) * tq.size(), hipMemcpyDeviceToHost), "ledger D2H");
+  double t = 0.0;
+  for (int x = P.utile0; x < P.utile1; ++x) t += tq[x];
+  out[2 * nv] = t;
+  return check_error() ? -1 : 0;
 }

@@ -145,6 +145,12 @@ struct Engine {
   InitRow *d_urows = nullptr;
   bool ucons_live = false;
   double *unit_cons();      // the live conserved state (allocated and derived on first use); null on error
+  // conservation ledger (2D disks): with d_led_torque set, k_update stores
+  // its tiles' torque terms; ledger_last reads what the last substep's
+  // boundary r faces and torque added (layout of disco_cell_ledger)
+  double *d_led_torque = nullptr;
+  int ledger_enable(bool on);
+  int ledger_last(int disk, double *out);
   InitRow *unit_rows();     // the disk's single-cell plugin rows on the device; null on error
   void *ckpt_mem = nullptr;
   size_t ckpt_cap = 0;

@@ -246,6 +246,17 @@ void riemann_setup_rz(struct Riemann *, struct Face *, struct Sim *, int n, int 
 void riemann_setup_p(struct Riemann *, struct Cell ***, struct Sim *, int i, int j_low, int k,
                      int direction);                                                   /* Riemann.h:41 */
 void riemann_AddFlux(struct Riemann *, struct Sim *, struct GravMass *, double dt);    /* Riemann.h:42 */
+/* No reference counterpart: the conservation ledger of the last substep of a
+ * 2D disk (after disco_cell_ledger_enable(cells, 1) and a timestep_* call):
+ * out[0, NQ+3): what the r faces between the inner ghost annulus and the
+ * first interior annulus added to their R cell, dt*dA*F[q] for q < NQ, then
+ * dt*dA*Fvisc[SRR], [LLL], [TAU] (0 without viscosity); out[NQ+3, 2(NQ+3)):
+ * the same for the faces between the last interior annulus and the outer
+ * ghosts; out[2(NQ+3)]: the torque term sum of dt dV rho F_phi r over the
+ * interior cells (the LLL source of src/Cell/cell_source.c:171).  Returns 0
+ * or -1 (error set). */
+int disco_cell_ledger_enable(struct Cell ***, int on);
+int disco_cell_ledger(struct Cell ***, double *out);
 /* No reference counterpart: bulk transfer of whole fields, zones in the
  * reference's (k, i, j) order, NQ-vectors q-major.  field: 0 prim, 3 grad,
  * 4 gradp (NQ*Nzones); 5 tiph, 7 dphi, 8 wiph (Nzones); 1 cons (derived). */

@@ -52,6 +52,12 @@ struct OracleSim {
   int *nri, *nzk;
   int err;
   double torque; /* sum of dt*dV*rho*F_phi*r over interior cells and substeps (the adjoint's loss) */
+  /* ledger of the last substep: what the r faces between the inner ghost
+   * annulus and the first interior one ([0]) and between the last interior
+   * annulus and the outer ghosts ([1]) added to their R cell, dt*dA*F[q]
+   * (q < NQ) and dt*dA*Fvisc[SRR, LLL, TAU]; and that substep's torque term */
+  double led[2][8 + 3];
+  double led_torque;
 };
 
 /* sim_FacePos (src/Sim/sim_access.c:26-36): index -1 is the first face */
@@ -1155,8 +1161,10 @@ static void add_src(OracleSim *s, double dt) { /* :73-214 */
         U(s, SRR, c) += dt * dV * rho * (Fr * sint + F_centrifugal_r);
         U(s, LLL, c) += dt * dV * rho * (Fp - r * dtOm) * r;
         if (i >= s->ip[OI_NgMinR] && i < s->NR - s->ip[OI_NgMaxR] && k >= s->ip[OI_NgMinZ] &&
-            k < s->NZ - s->ip[OI_NgMaxZ])
+            k < s->NZ - s->ip[OI_NgMaxZ]) {
           s->torque += dt * dV * rho * Fp * r;
+          s->led_torque += dt * dV * rho * Fp * r;
+        }
         U(s, SZZ, c) += dt * dV * rho * Fr * cost;
         U(s, TAU, c) += dt * dV * rho * ((Fr * sint + F_centrifugal_r) * vr + Fr * vz * cost + (Fp + F_euler_phi) * vp);
         if (s->ip[OI_RhoSinkOn] == 1) {
@@ -1639,10 +1647,20 @@ static int substep(OracleSim *s, double fac) { /* src/TimeStep/timestep_routines
         setup_p(s, &R, i, j, k);
         if ((e = add_flux(s, &R, dt))) return e;
       }
+  memset(s->led, 0, sizeof(s->led));
+  s->led_torque = 0.0;
+  const int in0 = s->nri[0], in1 = s->nri[1];
+  const int out0 = s->nri[s->NR - 3], out1 = s->nri[s->NR - 2];
   for (int n = 0; n < s->nri[s->NR - 1]; ++n) {
     riemann_init(&R);
     setup_rz(s, &R, 0, n, 0);
     if ((e = add_flux(s, &R, dt))) return e;
+    int side = (n >= in0 && n < in1) ? 0 : (n >= out0 && n < out1) ? 1 : -1;
+    if (side >= 0) { /* test infrastructure: the conservation ledger */
+      for (int q = 0; q < s->NQ; ++q) s->led[side][q] += dt * R.dA * R.F[q];
+      if (s->dp[OD_EXPLICIT_VISCOSITY] > 0.)
+        for (int v = 0; v < 3; ++v) s->led[side][8 + v] += dt * R.dA * R.Fvisc[v == 0 ? SRR : v == 1 ? LLL : TAU];
+    }
   }
   if (s->ip[OI_NglobalZ] != 1) {
     oracle_plm_rz(s, 1);
@@ -1704,6 +1722,39 @@ int oracle_forward_euler(OracleSim *s) { /* src/TimeStep/timestep_algorithm.c:43
   grav_move(s, s->t, 1.0 * s->dt);
   s->t += s->dt; /* timestep_update_t; no Macc update */
   return 0;
+}
+
+/* timestep_rk2 in pieces (test infrastructure for the conservation
+ * ledger): the step's set-up, one substep with its mass move, the end */
+int oracle_step_begin(OracleSim *s) {
+  oracle_set_w(s);
+  if (s->err) return s->err;
+  s->dt = oracle_mindt(s);
+  if (s->t + s->dt > s->dp[OD_T_MAX]) s->dt = s->dp[OD_T_MAX] - s->t;
+  cell_copy(s);
+  grav_copy(s);
+  s->RK = 0.0;
+  return 0;
+}
+int oracle_step_substep(OracleSim *s, double fac) {
+  int e = substep(s, fac);
+  if (e) return s->err = e;
+  grav_move(s, s->t, fac * s->dt);
+  s->RK = 0.5;
+  return 0;
+}
+void oracle_step_end(OracleSim *s) {
+  s->t += s->dt;
+  for (int p = 0; p < 2; ++p) s->gm[p].Macc = s->gm[p].Macc + s->gm[p].Mdot * s->dt;
+}
+/* the last substep's ledger: led[0], led[1] (NQ + 3 each), torque */
+void oracle_ledger(OracleSim *s, double *out) {
+  const int n = s->NQ + 3;
+  for (int v = 0; v < n; ++v) {
+    out[v] = s->led[0][v < s->NQ ? v : 8 + (v - s->NQ)];
+    out[n + v] = s->led[1][v < s->NQ ? v : 8 + (v - s->NQ)];
+  }
+  out[2 * n] = s->led_torque;
 }
 
 int oracle_run(OracleSim *s, int nsteps) {

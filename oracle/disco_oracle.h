@@ -49,6 +49,18 @@ void oracle_init(OracleSim *s);
 /* one timestep_rk2; returns 0, or a nonzero error code */
 int oracle_rk2(OracleSim *s);
 int oracle_run(OracleSim *s, int nsteps);
+/* timestep_rk2 in pieces (for the conservation ledger): set-up (set_w, dt,
+ * cell_copy, gravMass_copy, RK = 0), one substep with its gravMass_move
+ * (then RK = 0.5), the end (t += dt, Macc) */
+int oracle_step_begin(OracleSim *s);
+int oracle_step_substep(OracleSim *s, double fac);
+void oracle_step_end(OracleSim *s);
+/* the last substep's ledger: the increments dt*dA*F[q] (q < NQ) and
+ * dt*dA*Fvisc[SRR, LLL, TAU] that the r faces between the inner ghost annulus
+ * and the first interior annulus added to their R cell (NQ + 3 values), the
+ * same for the faces between the last interior annulus and the outer ghosts
+ * (NQ + 3), and the substep's torque term sum dt dV rho F_phi r (1) */
+void oracle_ledger(OracleSim *s, double *out);
 /* one timestep_forward_euler (src/TimeStep/timestep_algorithm.c:43-57) */
 int oracle_forward_euler(OracleSim *s);
 double oracle_torque(OracleSim *s);

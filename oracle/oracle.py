@@ -59,6 +59,14 @@ def lib():
         L.oracle_run.restype = ctypes.c_int
         L.oracle_forward_euler.argtypes = [vp]
         L.oracle_forward_euler.restype = ctypes.c_int
+        L.oracle_step_begin.argtypes = [vp]
+        L.oracle_step_begin.restype = ctypes.c_int
+        L.oracle_step_substep.argtypes = [vp, ctypes.c_double]
+        L.oracle_step_substep.restype = ctypes.c_int
+        L.oracle_step_end.argtypes = [vp]
+        L.oracle_step_end.restype = None
+        L.oracle_ledger.argtypes = [vp, dp]
+        L.oracle_ledger.restype = None
         L.oracle_get.argtypes = [vp, ctypes.c_int, dp]
         L.oracle_put.argtypes = [vp, ctypes.c_int, dp]
         L.oracle_face_create.argtypes = [vp, ctypes.c_int]
@@ -143,6 +151,28 @@ class Oracle:
         e = lib().oracle_run(self.h, n)
         if e:
             raise RuntimeError(f"oracle error {e}")
+
+    def step_begin(self):
+        """timestep_rk2's set-up before its first substep"""
+        e = lib().oracle_step_begin(self.h)
+        if e:
+            raise RuntimeError(f"oracle error {e}")
+
+    def substep(self, fac):
+        """one substep (fac 1.0, then 0.5) with its gravMass_move"""
+        e = lib().oracle_step_substep(self.h, fac)
+        if e:
+            raise RuntimeError(f"oracle error {e}")
+
+    def step_end(self):
+        lib().oracle_step_end(self.h)
+
+    def ledger(self):
+        """the last substep's boundary r-face increments (inner, outer: NQ + 3
+        each) and torque term"""
+        out = np.empty(2 * (self.NQ + 3) + 1)
+        lib().oracle_ledger(self.h, _dp(out))
+        return out
 
     def euler(self, n=1):
         """n timestep_forward_euler steps"""
