@@ -70,6 +70,17 @@ static void put_rec(const char *name, int type, const void *data, int64_t count)
 }
 
 static int g_NR, g_NZ, g_NQ, g_Nz;
+/* several MPI ranks (the timing mode only): the initial-state file holds
+ * the whole mesh (one rank's layout, ghosts included) and every rank takes
+ * its own annuli and layers: local annulus i is global annulus
+ * i - Nghost_min + N0 + 1 (one inner ghost annulus globally), local layer k
+ * global layer k - Nghost_min + N0 + 2 (two ghost layers) */
+static int g_multi = 0, g_gNR = 0, g_gNZ = 0;
+static int32_t *g_gNp = NULL;
+static int glob_i(struct Sim *s, int i) { return i - s->Nghost_min[R_DIR] + s->N0[R_DIR] + 1; }
+static int glob_k(struct Sim *s, int k) {
+  return g_gNZ == 1 ? 0 : k - s->Nghost_min[Z_DIR] + s->N0[Z_DIR] + 2;
+}
 static struct Cell *g_base;
 
 static int zone_of(struct Cell *c) { return (int)(c - g_base); }
@@ -185,12 +196,26 @@ int main(int argc, char **argv) {
       fprintf(stderr, "bad icfile\n");
       return 4;
     }
-    if (hdr[1] != g_NR || hdr[2] != g_NZ) {
+    g_multi = mpisetup_NumProcs(mpi) > 1;
+    if (g_multi && strcmp(mode, "time") != 0) {
+      fprintf(stderr, "several ranks: timing mode only\n");
+      return 5;
+    }
+    if (!g_multi && (hdr[1] != g_NR || hdr[2] != g_NZ)) {
       fprintf(stderr, "icfile dims %d %d != %d %d\n", hdr[1], hdr[2], g_NR, g_NZ);
       return 5;
     }
+    g_gNR = hdr[1];
+    g_gNZ = hdr[2];
     if (hdr[4] & 1) {
-      if (fread(sim->N_p, 4, (size_t)g_NR, ic) != (size_t)g_NR) return 6;
+      if (g_multi) {
+        int i;
+        g_gNp = malloc(sizeof(int32_t) * (size_t)g_gNR);
+        if (fread(g_gNp, 4, (size_t)g_gNR, ic) != (size_t)g_gNR) return 6;
+        for (i = 0; i < g_NR; ++i) sim->N_p[i] = g_gNp[glob_i(sim, i)];
+      } else if (fread(sim->N_p, 4, (size_t)g_NR, ic) != (size_t)g_NR) {
+        return 6;
+      }
     }
   }
   sim_set_misc(sim, mpi);
@@ -239,7 +264,26 @@ int main(int argc, char **argv) {
       if (fread(buf, 8, (size_t)g_Nz, ic) != (size_t)g_Nz) return 8;
       for (z = 0; z < g_Nz; ++z) g_base[z].tiph = buf[z];
     }
-    if (hdr[4] & 4) {
+    if ((hdr[4] & 4) && g_multi) {
+      /* this rank's rows of the global primitives (q-major, zone order) */
+      int64_t gNL = 0, *goff = malloc(sizeof(int64_t) * (size_t)(g_gNR + 1));
+      int i, k;
+      if (!g_gNp) return 9;
+      goff[0] = 0;
+      for (i = 0; i < g_gNR; ++i) goff[i + 1] = goff[i] + g_gNp[i];
+      gNL = goff[g_gNR];
+      const long base = 20 + 4L * g_gNR + ((hdr[4] & 2) ? 8L * gNL * g_gNZ : 0);
+      for (q = 0; q < g_NQ; ++q)
+        for (k = 0; k < g_NZ; ++k)
+          for (i = 0; i < g_NR; ++i) {
+            const int I = glob_i(sim, i), K = glob_k(sim, k), n = sim_N_p(sim, i);
+            const int64_t gz = (int64_t)q * gNL * g_gNZ + (int64_t)K * gNL + goff[I];
+            if (fseek(ic, base + 8L * gz, SEEK_SET) != 0 || fread(buf, 8, (size_t)n, ic) != (size_t)n) return 9;
+            struct Cell *row = cells[k][i];
+            for (z = 0; z < n; ++z) row[z].prim[q] = buf[z];
+          }
+      free(goff);
+    } else if (hdr[4] & 4) {
       if (fread(buf, 8, (size_t)g_Nz * g_NQ, ic) != (size_t)g_Nz * g_NQ) return 9;
       for (q = 0; q < g_NQ; ++q)
         for (z = 0; z < g_Nz; ++z) g_base[z].prim[q] = buf[(size_t)q * g_Nz + z];
@@ -347,11 +391,19 @@ int main(int argc, char **argv) {
     clock_gettime(CLOCK_MONOTONIC, &t1);
     double el = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
     if (timing) {
-      int ninterior = 0, i;
+      /* interior zones of all ranks; the slowest rank's time */
+      long long ninterior = 0, nall = 0;
+      int i;
       for (i = sim->Nghost_min[R_DIR]; i < g_NR - sim->Nghost_max[R_DIR]; ++i) ninterior += sim_N_p(sim, i);
       ninterior *= sim->N_noghost[Z_DIR];
-      fprintf(stderr, "{\"seconds\": %.6f, \"steps\": %d, \"interior_zones\": %d, \"zone_updates_per_s\": %.6e}\n",
-              el, nsteps, ninterior, (double)ninterior * nsteps / el);
+      double elmax = el;
+      MPI_Allreduce(&ninterior, &nall, 1, MPI_LONG_LONG, MPI_SUM, MPI_COMM_WORLD);
+      MPI_Allreduce(&el, &elmax, 1, MPI_DOUBLE, MPI_MAX, MPI_COMM_WORLD);
+      if (mpisetup_MyProc(mpi) == 0)
+        fprintf(stderr,
+                "{\"seconds\": %.6f, \"steps\": %d, \"interior_zones\": %lld, \"ranks\": %d, "
+                "\"zone_updates_per_s\": %.6e}\n",
+                elmax, nsteps, nall, mpisetup_NumProcs(mpi), (double)nall * nsteps / elmax);
     } else {
       put_rec("dts", 0, dts, nsteps);
       fclose(g_out);
