@@ -249,24 +249,10 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     h2d(d_dt_tile, none.data(), none.size());
   }
 
-  // Column order (a schedule option): the tiles of each disk's annuli
-  // listed annulus column by column (i outer, k inner) instead of layer by
-  // layer, so a 3D cell's z neighbours are processed next to it; the face
-  // tiles of pairs the same way.  2D disks keep their order.
-  auto zone_key = [&](const int2 &t) {
-    const AnnInfo &A = hann[t.x];
-    return std::make_tuple(A.disk, A.i, A.k);
-  };
-  auto face_key = [&](const int2 &t) {
-    const AnnInfo &A = hann[hpairs[t.x].ann_in];
-    return std::make_tuple(A.disk, A.i, A.k);
-  };
-  std::vector<int2> ztiles_c = ztiles, utiles_c = utiles, rft_c = rft, zft_c = zft;
-  auto by = [](auto key) { return [key](const int2 &a, const int2 &b) { return key(a) < key(b); }; };
-  std::stable_sort(ztiles_c.begin(), ztiles_c.end(), by(zone_key));
-  std::stable_sort(utiles_c.begin(), utiles_c.end(), by(zone_key));
-  std::stable_sort(rft_c.begin(), rft_c.end(), by(face_key));
-  std::stable_sort(zft_c.begin(), zft_c.end(), by(face_key));
+  // the table-order lists the layer-block schedules reorder (build_zblock)
+  h_ztiles = ztiles;
+  h_utiles = utiles;
+  h_zft = zft;
   // one tile array on the device, sliced per list
   std::vector<int2> all;
   auto add = [&](std::vector<int2> &v, int &count) {
@@ -278,8 +264,6 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   size_t o_z = add(ztiles, plan.n_ztiles), o_u = add(utiles, plan.n_utiles), o_i = add(itiles, plan.n_itiles), o_r = add(rft, plan.n_rftiles),
          o_zf = add(zft, plan.n_zftiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
          o_s = add(szt, plan.n_sz), o_bi = add(bit, plan.n_binit);
-  int nc;
-  const size_t o_zc = add(ztiles_c, nc), o_uc = add(utiles_c, nc), o_rc = add(rft_c, nc), o_zfc = add(zft_c, nc);
   if (!dalloc(&d_tiles, all.size(), "tiles") || !h2d(d_tiles, all.data(), all.size())) return;
   plan.ztiles = d_tiles + o_z;
   plan.utiles = d_tiles + o_u;
@@ -290,14 +274,9 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   plan.bc1 = d_tiles + o_b1;
   plan.sz = d_tiles + o_s;
   plan.binit = d_tiles + o_bi;
-  tiles_row[0] = plan.ztiles;
-  tiles_row[1] = plan.utiles;
-  tiles_row[2] = plan.rftiles;
-  tiles_row[3] = plan.zftiles;
-  tiles_col[0] = d_tiles + o_zc;
-  tiles_col[1] = d_tiles + o_uc;
-  tiles_col[2] = d_tiles + o_rc;
-  tiles_col[3] = d_tiles + o_zfc;
+  tl_zb[0][0] = plan.ztiles;
+  tl_zb[0][1] = plan.utiles;
+  tl_zb[0][2] = plan.zftiles;
   plan.ndisk = n;
   plan.any_visc = 0;
   plan.feat = 0;
@@ -364,7 +343,8 @@ Engine::~Engine() {
   if (graph) hipGraphDestroy(graph);
   void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
-                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_dt_tile, d_mdot_tile, d_ann_rot, d_init, d_pair_perm, adj_mem, diag_mem,
+                  d_fo_z, d_run, d_Fface, d_err, d_tiles, d_tiles_zb[1], d_tiles_zb[2], d_tiles_zb[3],
+                  d_tiles_zb[4], d_tiles_zb[5], d_tiles_zb[6], d_tiles_zb[7], d_dt_tile, d_mdot_tile, d_ann_rot, d_init, d_pair_perm, adj_mem, diag_mem,
                   ckpt_mem, d_ucons, d_cool, d_umdot, d_urows, d_led_torque};
   for (void *p : ptrs)
     if (p) hipFree(p);
@@ -608,14 +588,45 @@ void Engine::run(int nsteps, bool use_graph) {
   hip_check(hipEventRecord(ev1, st), "event");
 }
 
-void Engine::set_schedule(const Schedule &s) {
+// Layer-block order (schedule field zb): the zone, update and z-face tiles of
+// each disk listed in blocks of 2^zb layers, annulus column by annulus
+// column inside a block (layer block, then i, then k), so that a 3D cell's z
+// neighbours are processed close to it while its r neighbours stay within a
+// block; zb = 0 is the table order (layer by layer).  The lists of a zb are
+// built on first use.
+bool Engine::build_zblock(int zb) {
+  if (tl_zb[zb][0]) return true;
+  const int mask = (1 << zb) - 1;
+  auto zone_key = [&](const int2 &t) {
+    const AnnInfo &A = hann[t.x];
+    return std::make_tuple(A.disk, A.k >> zb, A.i, A.k & mask);
+  };
+  auto face_key = [&](const int2 &t) {
+    const AnnInfo &A = hann[hpairs[t.x].ann_in];
+    return std::make_tuple(A.disk, A.k >> zb, A.i, A.k & mask);
+  };
+  std::vector<int2> zt = h_ztiles, ut = h_utiles, zf = h_zft;
+  std::stable_sort(zt.begin(), zt.end(), [&](const int2 &a, const int2 &b) { return zone_key(a) < zone_key(b); });
+  std::stable_sort(ut.begin(), ut.end(), [&](const int2 &a, const int2 &b) { return zone_key(a) < zone_key(b); });
+  std::stable_sort(zf.begin(), zf.end(), [&](const int2 &a, const int2 &b) { return face_key(a) < face_key(b); });
+  std::vector<int2> all(zt);
+  all.insert(all.end(), ut.begin(), ut.end());
+  all.insert(all.end(), zf.begin(), zf.end());
+  if (!dalloc(&d_tiles_zb[zb], all.size(), "tiles") || !h2d(d_tiles_zb[zb], all.data(), all.size())) return false;
+  tl_zb[zb][0] = d_tiles_zb[zb];
+  tl_zb[zb][1] = d_tiles_zb[zb] + zt.size();
+  tl_zb[zb][2] = d_tiles_zb[zb] + zt.size() + ut.size();
+  return true;
+}
+
+void Engine::set_schedule(const Schedule &s0) {
+  Schedule s = s0;
+  if (!build_zblock(s.zb)) s.zb = 0;
   sched = s;
   plan.fuse = s.fuse;
-  const int2 *const *tl = s.col ? tiles_col : tiles_row;
-  plan.ztiles = tl[0];
-  plan.utiles = tl[1];
-  plan.rftiles = tl[2];
-  plan.zftiles = tl[3];
+  plan.ztiles = tl_zb[s.zb][0];
+  plan.utiles = tl_zb[s.zb][1];
+  plan.zftiles = tl_zb[s.zb][2];
   plan.stage_r = s.stage ? stage_r_max : 0;
   plan.stage_z = s.stage ? stage_z_max : 0;
   if (gexec) {  // the next graph run captures the new launches
@@ -658,30 +669,24 @@ int Engine::autotune(int nsteps, double *ms, int nms) {
     if (k < nms) ms[k] = v;
     return v;
   };
-  // the launch orders first (faces stored), then the fused face
-  // evaluations under the fastest order
+  // the launch orders first (faces stored, table order), then the layer
+  // blocks under the fastest order, then the fused face evaluations
   double best = 1e300;
   int bestk = -1;
-  for (int k0 = 0; k0 < 16; ++k0) {
-    const int k = (k0 & 7) | (k0 >> 3) << 5;  // xcd, largest_first, stage, col
-    if (!schedule_distinct(k)) continue;  // the same launches as another candidate
+  auto consider = [&](int k) {
+    if (!schedule_distinct(k)) return;  // the same launches as another candidate
     const double v = time_k(k);
     if (v < best) {
       best = v;
       bestk = k;
     }
-  }
+  };
+  for (int k = 0; k < 8; ++k) consider(k);
   if (bestk < 0) return -1;
   const int order = bestk;
-  for (int f = 1; f < 4; ++f) {
-    const int k = order | f << 3;
-    if (!schedule_distinct(k)) continue;
-    const double v = time_k(k);
-    if (v < best) {
-      best = v;
-      bestk = k;
-    }
-  }
+  for (int zb = 1; zb < 8; ++zb) consider(order | zb << 5);
+  const int blocked = bestk;
+  for (int f = 1; f < 4; ++f) consider(blocked | f << 3);
   set_schedule(Schedule::of(bestk));
   return bestk;
 }
@@ -693,7 +698,11 @@ bool Engine::schedule_distinct(int k) const {
   if (s.stage && !stage_r_max && !stage_z_max) return false;
   if ((s.fuse & 1) && n_zpairs) return false;
   if ((s.fuse & 2) && !n_zpairs) return false;
-  if (s.col && !n_zpairs) return false;  // 2D disks: column order is table order
+  if (s.zb && !n_zpairs) return false;  // 2D disks: every layer-block order is table order
+  int nzmax = 1;
+  for (const DiskParams &P : hdp) nzmax = P.NZ > nzmax ? P.NZ : nzmax;
+  if (s.zb && (1 << (s.zb - 1)) >= nzmax) return false;  // blocks already span every layer at zb - 1
+  return true;
   return true;
 }
 

@@ -64,21 +64,25 @@ struct Engine {
   // annulus staged in LDS
   // fuse: bit 0 / 1 evaluate the r / z faces' increments inside the update
   // instead of storing them (each face is then solved by both of its cells)
-  // col: 3D disks' tiles listed annulus column by annulus column
+  // zb: 3D disks' zone, update and z-face tiles in blocks of 2^zb layers,
+  // annulus column by annulus column inside a block (0: table order)
   struct Schedule {
-    int xcd = 1, largest_first = 0, stage = 1, fuse = 0, col = 0;
-    int code() const { return xcd | largest_first << 1 | stage << 2 | fuse << 3 | col << 5; }
+    int xcd = 1, largest_first = 0, stage = 1, fuse = 0, zb = 0;
+    int code() const { return xcd | largest_first << 1 | stage << 2 | fuse << 3 | zb << 5; }
     static Schedule of(int k) {
       Schedule s;
       s.xcd = k & 1;
       s.largest_first = (k >> 1) & 1;
       s.stage = (k >> 2) & 1;
       s.fuse = (k >> 3) & 3;
-      s.col = (k >> 5) & 1;
+      s.zb = (k >> 5) & 7;
       return s;
     }
   } sched;
-  const int2 *tiles_row[4] = {}, *tiles_col[4] = {};  // zone, update, r face, z face tiles per order
+  std::vector<int2> h_ztiles, h_utiles, h_zft;       // table-order lists
+  int2 *d_tiles_zb[8] = {};                          // [zone | update | z face] tiles per layer block
+  const int2 *tl_zb[8][3] = {};                      // (zb = 0: slices of d_tiles)
+  bool build_zblock(int zb);
   bool schedule_distinct(int k) const;
   int stage_r_max = 0, stage_z_max = 0;  // LDS staging the plan allows
   int *d_pair_perm = nullptr;
