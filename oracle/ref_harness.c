@@ -21,6 +21,10 @@
  *   disco_ref unit  <parfile> <icfile> <out>
  *   disco_ref time  <parfile> <icfile> <nsteps>
  *   disco_ref diag  <parfile> <icfile> <out> <nsteps>
+ *   disco_ref io    <parfile> <icfile> <out> <nsteps> <more>
+ *     (nsteps RK2 steps, then the checkpoint buffer of io_setbuf and
+ *     io_setup's schedule at that time; then the restart of src/main.c:77-119
+ *     from that buffer into fresh cells and masses, and <more> steps on)
  *     (steps with the diagnostics of src/main.c:121-131; the text files the
  *     reference writes land in the working directory; every measurement
  *     also dumps the diagnostics' arrays at full precision)
@@ -43,6 +47,7 @@
 #define TIMESTEP_PRIVATE_DEFS
 #define PLANET_PRIVATE_DEFS
 #define DIAGNOSTICS_PRIVATE_DEFS
+#define IO_PRIVATE_DEFS
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -55,6 +60,7 @@
 #include "GravMass.h"
 #include "TimeStep.h"
 #include "Diagnostics.h"
+#include "IO.h"
 #include "header.h"
 
 static FILE *g_out = NULL;
@@ -408,6 +414,67 @@ int main(int argc, char **argv) {
       put_rec("dts", 0, dts, nsteps);
       fclose(g_out);
     }
+    MPI_Finalize();
+    return 0;
+  }
+  if (strcmp(mode, "io") == 0) {
+    /* io_buffer.c / io_setup.c in the order of src/main.c:132-139 (writing)
+     * and src/main.c:77-119 (Restart 1); io_hdf5.c needs parallel HDF5 and
+     * is not built: the buffer goes from io_setbuf straight to io_readbuf,
+     * and the time is restored as io_hdf5_in does (io_hdf5.c:196-198) */
+    int nsteps = atoi(argv[5]), more = atoi(argv[6]), s2;
+    g_out = fopen(argv[4], "wb");
+    dump_geom(sim);
+    FILE *so = freopen("/dev/null", "w", stdout);
+    (void)so;
+    for (s2 = 1; s2 <= nsteps; ++s2) timestep_rk2(ts, sim, cells, gm, mpi);
+    dump_state("io/state", cells, sim, gm, ts);
+    struct IO *io = io_create(sim);
+    io_allocbuf(io, sim);
+    io_setbuf(io, cells, sim, gm);
+    put_rec("io/buffer", 0, io->buffer[0], (int64_t)sim_Ncells(sim) * (g_NQ + 3));
+    put_rec("io/gravmass_buffer", 0, io->GravMassBuffer[0], 8);
+    const double tsave = timestep_get_t(ts);
+    io_setup(io, sim, ts);
+    {
+      double tc = io_tcheck(io);
+      int32_t nf = io_nfile(io);
+      put_rec("io/tcheck", 0, &tc, 1);
+      put_rec("io/nfile", 1, &nf, 1);
+    }
+    /* restart: fresh cells (cell_create reseeds its phi offsets), masses
+     * and time step.  gravMass_create leaves Mdot and Macc unset and
+     * io_readbuf does not set them: they are zeroed here, and the cases this
+     * mode serves have no sink. */
+    struct GravMass *gm2 = gravMass_create(sim_NumGravMass(sim));
+    memset(gm2, 0, sizeof(struct GravMass) * (size_t)sim_NumGravMass(sim));
+    struct Cell ***cells2 = cell_create(sim, mpi);
+    cell_clean_pi(cells2, sim);
+    cell_syncproc_r(cells2, sim, mpi);
+    cell_syncproc_z(cells2, sim, mpi);
+    struct TimeStep *ts2 = timestep_create(sim);
+    timestep_set_t(ts2, tsave);
+    io_readbuf(io, cells2, sim, gm2);
+    io_deallocbuf(io);
+    gravMass_clean_pi(gm2, sim);
+    struct IO *io2 = io_create(sim);
+    io_setup(io2, sim, ts2);
+    {
+      double tc = io_tcheck(io2);
+      int32_t nf = io_nfile(io2);
+      put_rec("io/restart_tcheck", 0, &tc, 1);
+      put_rec("io/restart_nfile", 1, &nf, 1);
+    }
+    cell_syncproc_r(cells2, sim, mpi);
+    cell_syncproc_z(cells2, sim, mpi);
+    cell_calc_cons(cells2, sim);
+    g_base = cells2[0][0];
+    dump_state("io/restart0", cells2, sim, gm2, ts2);
+    for (s2 = 1; s2 <= more; ++s2) timestep_rk2(ts2, sim, cells2, gm2, mpi);
+    dump_state("io/restart", cells2, sim, gm2, ts2);
+    io_destroy(io);
+    io_destroy(io2);
+    fclose(g_out);
     MPI_Finalize();
     return 0;
   }

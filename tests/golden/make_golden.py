@@ -8,7 +8,8 @@ been built (``make -C oracle ref``):
 For every case it writes the parfile, asks the reference for its mesh,
 builds the seeded initial state (disco2_amd.ics) or lets the reference's
 own initial-data plugin set it (the plugin cases), runs the reference's
-unit pieces (faces, PLM slopes, the first substep) and a few RK2 steps, and
+unit pieces (faces, PLM slopes, the first substep), a few RK2 steps and,
+for some cases, the checkpoint buffer and a restart from it, and
 stores inputs and outputs as a compressed .npz.  The fixtures are data only.
 """
 from __future__ import annotations
@@ -35,7 +36,7 @@ REF_PARFILES = "/root/reference/parfiles"
 
 def cases():
     c = {}
-    c["vortex_r16"] = (configs.vortex(16), dict(steps=20, dumps=[1, 2, 20], unit=True, euler=2))
+    c["vortex_r16"] = (configs.vortex(16), dict(steps=20, dumps=[1, 2, 20], unit=True, euler=2, io=(5, 5)))
     p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=1, PLM=2.0, RiemannSolver=1, NoInnerBC=1)
     c["vortex_scalar_hll_noinner"] = ((p, k, dict(kw, x0=1.8)), dict(steps=12, dumps=[1, 12], unit=True))
     p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=3, PLM=2.0)
@@ -43,17 +44,20 @@ def cases():
     p, k, kw = configs.vortex(12, r_min=0.5, r_max=3.5, NUM_N=2, PLM=1.5)
     c["vortex_two_scalars"] = ((p, k, dict(kw, x0=1.6)), dict(steps=8, dumps=[1, 8], unit=True))
     c["binary_r16"] = (configs.binary(16, r_min=0.3, r_max=3.0), dict(steps=20, dumps=[1, 2, 20], unit=True,
-                                                                          euler=2))
+                                                                          euler=2, io=(5, 5)))
     p, k, kw = configs.binary(16, r_min=0.3, r_max=3.0, q=0.5, w_numeric_type=0, w_analytic_type=1,
                               RLogScale=1.0, RhoSinkOn=1, ViscConst=0)
     c["binary_akepler_log_sink"] = ((p, k, kw), dict(steps=12, dumps=[1, 12], unit=True))
     p, k, kw = configs.binary(16, r_min=0.3, r_max=3.0, alpha=0.02, ViscConst=1, w_numeric_type=2)
     c["binary_viscconst_rigid"] = ((p, k, kw), dict(steps=8, dumps=[1, 8], unit=True))
     p, k, kw = configs.disk3d(8, 4, 24, R_Min=0.5, R_Max=2.5)
-    c["disk3d_r8z4"] = ((p, k, kw), dict(steps=10, dumps=[1, 2, 10], unit=True, euler=2))
+    c["disk3d_r8z4"] = ((p, k, kw), dict(steps=10, dumps=[1, 2, 10], unit=True, euler=2, io=(3, 3)))
     p, k, kw = configs.disk3d(8, 4, 20, R_Min=0.5, R_Max=2.5, N_p_round=0, HiResFac=1.0,
                               HiResSigma=0.3, HiResR0a=1.0, ZLogScale=0.5)
     c["disk3d_refNp_hires"] = ((p, k, kw), dict(steps=6, dumps=[1, 6], unit=True))
+    # the checkpoint schedule of io_setup (a checkpoint every 0.025 in time)
+    c["vortex_io_schedule"] = (configs.vortex(12, Time_Max=1.0, Num_Checkpoints=40),
+                               dict(steps=1, dumps=[1], unit=False, light=True, io=(6, 3)))
     c["vortex_r64_100"] = (configs.vortex(64), dict(steps=100, dumps=[100], unit=False, light=True))
     # diagnostics (src/Diagnostics): measured every step; the vortex and the
     # 3D disk dump every few measurements (time-averaged profiles), the binary
@@ -180,6 +184,13 @@ def make_case(name, spec, opts, tmp):
         for k, v in u.items():
             if k.startswith("euler"):
                 out["unit/" + k] = v
+    if opts.get("io"):  # checkpoint buffer (io_setbuf), io_setup, restart (io_readbuf) and steps after it
+        n, more = opts["io"]
+        u = orc.read_records(_run(["io", par, ic, os.path.join(tmp, "i.bin"), str(n), str(more)], tmp, "i.bin"))
+        for k, v in u.items():
+            if k.startswith("io/") and not k.endswith(("/cons", "/RKcons", "/RKtiph", "/wiph")):
+                out[k] = v
+        out["io/steps"] = np.array([n, more], dtype=np.int32)
     args = ["run", par, ic, os.path.join(tmp, "r.bin"), str(opts["steps"])] + [str(d) for d in opts["dumps"]]
     r = orc.read_records(_run(args, tmp, "r.bin"))
     missing = [d for d in opts["dumps"] if f"step{d}/prim" not in r]

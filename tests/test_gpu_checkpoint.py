@@ -1,10 +1,15 @@
 """Checkpoints and restart (src/IO) on the GPU.
 
-The reference's IO needs parallel HDF5 (io_hdf5.c:37-38 H5Pset_fapl_mpio),
-which this image lacks, so no file written by the reference is at hand: the
-file layout is the one io_setbuf / io_readbuf / io_hdf5_out / io_hdf5_in
-define, and the tests pin it by content, against the oracle (the CPU
-restatement that is bitwise the reference):
+The reference's io_hdf5.c needs parallel HDF5 (io_hdf5.c:37-38
+H5Pset_fapl_mpio), which this image lacks, so no file written by the
+reference is at hand.  Its io_buffer.c, io_setup.c, io_create_destroy.c and
+io_access.c are built into the reference harness, and the golden fixtures
+hold io_setbuf's rows, io_setup's schedule and the reference's restart from
+those rows (the io/ records): the file's Data, GravMass and T and the
+restarted run are checked against them, through the batch API and through
+the class API (io_setbuf, io_hdf5_out, io_hdf5_in, io_readbuf, io_setup).
+Larger cases are checked against the oracle, whose restart rules
+(tests/ckpt_util.py) the CPU tests pin to the same records:
 
 * layout: the file of disco_batch_checkpoint_write after some RK2 steps,
   read back with the HDF5 library, holds T = t, GravMass = (r, phi, M,
@@ -27,6 +32,10 @@ import subprocess
 
 import numpy as np
 import pytest
+
+import ckpt_util
+from compare import bitwise_equal, describe, max_rel, zone_rel
+from golden_util import Golden, golden_names
 
 pytestmark = pytest.mark.gpu
 
@@ -104,33 +113,11 @@ def check(a, b, exact, what):
 
 
 def centres(s):
-    rc = 0.5 * (s.r_faces[:-1] + s.r_faces[1:])
-    zc = 0.5 * (s.z_faces[:-1] + s.z_faces[1:])
-    r = np.concatenate([np.full(n, rc[i]) for i, n in enumerate(s.N_p)])
-    return np.tile(r, s.NZ), np.repeat(zc, int(s.N_p.sum()))
+    return ckpt_util.centres(s.N_p, s.r_faces, s.z_faces)
 
 
 def restart_oracle(p, s, data, T, gm):
-    """the reference's restart on the oracle: tiph and primitives from the
-    file (v_z = 0 in 2D), dphi from cell_create, t and the masses"""
-    from oracle.oracle import Oracle
-    prim = data[:, 3:].T.copy()
-    if s.NZ == 1:
-        prim[4] = 0.0
-    o = Oracle(p, s.N_p, s.r_faces, s.z_faces)
-    o.set_state(tiph=data[:, 0].copy(), prim=prim, dphi=s.dphi0)
-    o.init()
-    sc = o.scalars()
-    sc[0] = T
-    for m in range(2):
-        sc[3 + 10 * m: 3 + 10 * m + 4] = gm[m]
-    for k in (4, 14):  # gravMass_clean_pi, GravMass_misc.c:9-17
-        while sc[k] > 2. * np.pi:
-            sc[k] -= 2. * np.pi
-        while sc[k] < 0.0:
-            sc[k] += 2. * np.pi
-    o.set_scalars(sc)
-    return o
+    return ckpt_util.restart_oracle(p, s.N_p, s.r_faces, s.z_faces, s.dphi0, data, T, gm)
 
 
 @pytest.mark.parametrize("kind", ["vortex", "binary"])
@@ -225,3 +212,127 @@ def test_c_driver_checkpoints_and_restarts(tmp_path):
     assert np.array_equal(prim, o2.get("prim"))
     assert np.array_equal(tiph, o2.get("tiph"))
     assert t == o2.scalars()[0]
+
+
+# fixtures with the reference's checkpoint rows and restart; vortex cases
+# have no libm transcendental and are compared bit for bit
+IO_CASES = [n for n in golden_names() if Golden(n).has("io/buffer")]
+IO_BITWISE = {"vortex_r16", "vortex_io_schedule"}
+
+
+def same(name, got, ref, what):
+    if name in IO_BITWISE:
+        assert bitwise_equal(got, ref), f"{what}: {describe(got, ref)}"
+    elif np.ndim(ref) == 2:
+        assert zone_rel(got, ref) < 1e-11, f"{what}: {zone_rel(got, ref)}"
+    else:
+        assert max_rel(got, ref) < 1e-11, f"{what}: {max_rel(got, ref)}"
+
+
+def check_file(name, g, path):
+    """the file's T, GravMass and Data against io_setbuf's rows"""
+    T, gm, data = read_h5(path, "T"), read_h5(path, "GravMass"), read_h5(path, "Data")
+    ref = g.get("io/buffer").reshape(g.Nz, g.NQ + 3)
+    assert data.shape == ref.shape and gm.shape == (2, 4) and T.shape == (1,)
+    assert bitwise_equal(data[:, 1:3], ref[:, 1:3])  # the centres: no arithmetic beyond the mesh
+    same(name, data[:, 0], ref[:, 0], "tiph")
+    same(name, data[:, 3:].T.copy(), ref[:, 3:].T.copy(), "prim")
+    same(name, T, g.get("io/state/t_dt")[:1], "T")
+    same(name, gm.ravel(), g.get("io/gravmass_buffer"), "GravMass")
+
+
+def check_state(name, g, key, prim, tiph, dphi, t):
+    same(name, prim, g.get(f"{key}/prim"), f"{key} prim")
+    same(name, tiph, g.get(f"{key}/tiph"), f"{key} tiph")
+    same(name, dphi, g.get(f"{key}/dphi"), f"{key} dphi")
+    same(name, np.array([t]), g.get(f"{key}/t_dt")[:1], f"{key} t")
+
+
+@pytest.mark.parametrize("name", IO_CASES)
+def test_batch_checkpoint_matches_reference_rows(name, tmp_path):
+    """disco_batch_checkpoint_write after the fixture's steps holds the
+    reference's io_setbuf rows; disco_batch_checkpoint_read into a fresh disk
+    gives the reference's restarted state, and the steps after it its state"""
+    from disco2_amd.api import Batch, Setup
+    g = Golden(name)
+    n, more = (int(x) for x in g.get("io/steps"))
+    print(f"checkpoint rows: {name}, {n} steps, restart, {more} more", flush=True)
+    b = Batch([Setup(g.params)])
+    b.upload("prim", g.prim0)
+    b.init()
+    b.run(n)
+    path = tmp_path / "ck.h5"
+    b.checkpoint(path)
+    b.close()
+    check_file(name, g, path)
+    b2 = Batch([Setup(g.params)])
+    b2.upload("prim", g.prim0)
+    b2.init()
+    b2.restart(path)
+    check_state(name, g, "io/restart0", b2.download("prim"), b2.download("tiph"), b2.download("dphi"),
+                b2.scalars()[0])
+    b2.run(more)
+    check_state(name, g, "io/restart", b2.download("prim"), b2.download("tiph"), b2.download("dphi"),
+                b2.scalars()[0])
+    b2.close()
+
+
+@pytest.mark.parametrize("name", IO_CASES)
+def test_class_api_checkpoint_matches_reference_rows(name, tmp_path, monkeypatch):
+    """src/main.c's checkpoint and restart through the class API:
+    io_setbuf + io_setup + io_hdf5_out after the fixture's steps, then
+    cell_create, io_hdf5_in (input.h5), io_readbuf, gravMass_clean_pi,
+    io_setup, the syncs and cell_calc_cons (src/main.c:74-119) and more steps"""
+    from test_gpu_class_api import Driver, VP
+    g = Golden(name)
+    n, more = (int(x) for x in g.get("io/steps"))
+    print(f"class API checkpoint: {name}, {n} steps, restart, {more} more", flush=True)
+    monkeypatch.chdir(tmp_path)
+    d = Driver(g)
+    L, sim, mpi = d.L, d.sim, d.mpi
+    for _ in range(n):
+        L.timestep_rk2(d.ts, sim, d.cells, d.gm, mpi)
+    io = L.io_create(sim)
+    L.io_allocbuf(io, sim)
+    L.io_setbuf(io, d.cells, sim, d.gm)
+    L.io_setup(io, sim, d.ts)
+    assert L.io_tcheck(io) == g.get("io/tcheck")[0] and L.io_nfile(io) == g.get("io/nfile")[0]
+    fname = "checkpoint_%04d.h5" % L.io_nfile(io)
+    L.io_hdf5_out(io, sim, d.ts)
+    L.io_deallocbuf(io)
+    L.io_destroy(io)
+    d.check()
+    d.close()
+    check_file(name, g, tmp_path / fname)
+    shutil.copy(tmp_path / fname, tmp_path / "input.h5")
+    # the restart: src/main.c:74-119 with Restart 1
+    r = Driver.__new__(Driver)
+    r.L, r.check, r.dptr, r.g, r.s, r.sim, r.mpi = L, d.check, d.dptr, g, d.s, sim, mpi
+    r.gm = L.gravMass_create(2)
+    r.cells = L.cell_create(sim, mpi)
+    L.cell_clean_pi(r.cells, sim)
+    L.cell_syncproc_r(r.cells, sim, mpi)
+    L.cell_syncproc_z(r.cells, sim, mpi)
+    r.ts = L.timestep_create(sim)
+    io = L.io_create(sim)
+    L.io_allocbuf(io, sim)
+    L.io_hdf5_in(io, sim, r.ts)
+    L.io_readbuf(io, r.cells, sim, r.gm)
+    L.io_deallocbuf(io)
+    L.gravMass_clean_pi(r.gm, sim)
+    L.io_setup(io, sim, r.ts)
+    assert L.io_tcheck(io) == g.get("io/restart_tcheck")[0] and L.io_nfile(io) == g.get("io/restart_nfile")[0]
+    L.io_destroy(io)
+    L.cell_syncproc_r(r.cells, sim, mpi)
+    L.cell_syncproc_z(r.cells, sim, mpi)
+    L.cell_calc_cons(r.cells, sim)
+    r.check()
+    check_state(name, g, "io/restart0", r.field("prim"), r.field("tiph"), r.field("dphi"), L.timestep_get_t(r.ts))
+    for _ in range(more):
+        L.timestep_rk2(r.ts, sim, r.cells, r.gm, mpi)
+    r.check()
+    check_state(name, g, "io/restart", r.field("prim"), r.field("tiph"), r.field("dphi"), L.timestep_get_t(r.ts))
+    gm = np.array([[L.gravMass_r(r.gm, p), L.gravMass_phi(r.gm, p), L.gravMass_M(r.gm, p),
+                    L.gravMass_omega(r.gm, p)] for p in range(2)])
+    same(name, gm.ravel(), g.get("io/restart/gravmass").reshape(2, 12)[:, :4].ravel(), "masses after restart")
+    r.close()

@@ -17,7 +17,7 @@ from golden_util import Golden, golden_names
 pytestmark = pytest.mark.gpu
 
 BITWISE = {"vortex_r16", "vortex_r64_100", "vortex_scalar_hll_noinner", "plugin_vortex_fixed", "vortex_three_scalars",
-           "vortex_two_scalars"}
+           "vortex_two_scalars", "vortex_io_schedule"}
 # per-zone relative tolerance on density, energy and pressure (positive
 # quantities) and on every conserved component scaled by the zone's
 # conserved-vector norm, for the cases with transcendental physics

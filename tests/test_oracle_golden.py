@@ -91,3 +91,68 @@ def test_forward_euler_bitwise(name):
         assert bitwise_equal(sc[:2], g.get(f"unit/euler{st}/t_dt"))
         gm = g.get(f"unit/euler{st}/gravmass").reshape(2, 12)[:, :10].ravel()
         assert bitwise_equal(sc[3:], gm)
+
+
+def io_names():
+    return [n for n in golden_names() if Golden(n).has("io/buffer")]
+
+
+@pytest.mark.parametrize("name", io_names())
+def test_checkpoint_rows_are_the_reference_buffer(name):
+    """io_setbuf's rows (io_buffer.c:26-55) hold, per cell in cell order,
+    tiph, the annulus and layer centres and the primitives of the state, and
+    GravMassBuffer the masses' r, phi, M and omega"""
+    from ckpt_util import centres
+    g = Golden(name)
+    buf = g.get("io/buffer").reshape(g.Nz, g.NQ + 3)
+    assert bitwise_equal(buf[:, 0], g.get("io/state/tiph"))
+    r, z = centres(g.N_p, g.r_faces, g.z_faces)
+    assert bitwise_equal(buf[:, 1], r) and bitwise_equal(buf[:, 2], z)
+    assert bitwise_equal(buf[:, 3:].T, g.get("io/state/prim"))
+    gm = g.get("io/state/gravmass").reshape(2, 12)[:, :4]
+    assert bitwise_equal(g.get("io/gravmass_buffer"), gm.ravel())
+
+
+@pytest.mark.parametrize("name", io_names())
+def test_restart_rules_match_reference(name):
+    """the restart the checkpoint tests run on the oracle (tests/ckpt_util.py)
+    is the reference's: io_readbuf into fresh cells and masses, then the
+    set-up and RK2 steps, bit for bit"""
+    from ckpt_util import restart_oracle
+    g = Golden(name)
+    buf = g.get("io/buffer").reshape(g.Nz, g.NQ + 3)
+    T = g.get("io/state/t_dt")[0]
+    o = restart_oracle(g.params, g.N_p, g.r_faces, g.z_faces, g.dphi0, buf, T,
+                       g.get("io/gravmass_buffer").reshape(2, 4))
+    for pfx in ("io/restart0", "io/restart"):
+        if pfx == "io/restart":
+            o.step(int(g.get("io/steps")[1]))
+        for f in ("prim", "tiph", "dphi"):
+            assert bitwise_equal(o.get(f), g.get(f"{pfx}/{f}")), f"{pfx} {f}: {describe(o.get(f), g.get(f'{pfx}/{f}'))}"
+        sc = o.scalars()
+        nt = 1 if pfx == "io/restart0" else 2  # (no dt before the first step)
+        assert bitwise_equal(sc[:nt], g.get(f"{pfx}/t_dt")[:nt]), pfx
+        gm = g.get(f"{pfx}/gravmass").reshape(2, 12)
+        for m in range(2):
+            assert bitwise_equal(sc[3 + 10 * m: 3 + 10 * m + 4], gm[m, :4]), f"{pfx} mass {m}"
+
+
+@pytest.mark.parametrize("name", io_names())
+def test_io_setup_schedule_matches_reference(name):
+    """io_setup (io_setup.c:10-19) through the C ABI, host only: the next
+    checkpoint time and the file number at the checkpointed time and after
+    the restart"""
+    from disco2_amd._lib import lib
+    from disco2_amd.api import Setup
+    g = Golden(name)
+    L = lib()
+    s = Setup(g.params)
+    ts = L.timestep_create(s.sim)
+    io = L.io_create(s.sim)
+    L.timestep_set_t(ts, float(g.get("io/state/t_dt")[0]))
+    L.io_setup(io, s.sim, ts)
+    assert L.io_tcheck(io) == g.get("io/tcheck")[0]
+    assert L.io_nfile(io) == g.get("io/nfile")[0]
+    assert L.io_tcheck(io) == g.get("io/restart_tcheck")[0] and L.io_nfile(io) == g.get("io/restart_nfile")[0]
+    L.io_destroy(io)
+    L.timestep_destroy(ts)
