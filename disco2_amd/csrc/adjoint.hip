@@ -674,8 +674,8 @@ __global__ __launch_bounds__(TILE) void ka_plm_p(DevMesh m, AdjMesh a, const int
 // CFL step backward.  The step's dt is the minimum of the interior cells'
 // steps; on a smooth disk whole annuli can tie bit for bit, and the minimum
 // is then the step of any one of them: the first tied cell (lowest zone)
-// takes lam_dt times its gradient.  ka_dt_pick finds it, ka_dt applies it.
-// m.sc holds the scalars the forward k_mindt saw.
+// takes lam_dt times its gradient.  ka_dt_pick finds it, ka_chain_dt
+// applies it.  m.sc holds the scalars the forward's CFL minimum saw.
 __device__ __forceinline__ double cfl_cell(const DevMesh &m, const int2 tl, const AnnInfo &A, const DiskParams &P,
                                            const DiskScalars &sc, long long c, double *pr, double *r_out,
                                            double *dx_out, double *w_out, double *tc_out) {
@@ -706,30 +706,6 @@ __global__ __launch_bounds__(TILE) void ka_dt_pick(DevMesh m, const int2 *tiles,
   const double d = cfl_cell(m, tl, A, disk_of(m, A.disk), m.sc[A.disk], c, pr, &r, &dx, &w, &tc);
   if ((unsigned long long)__double_as_longlong(d) == dtbits) atomicMin(pick, (int)c);
 }
-__global__ __launch_bounds__(TILE) void ka_dt(DevMesh m, AdjMesh a, const int2 *tiles, const int *pick,
-                                              int slot0) {
-  const int ti = xcd_order();
-  const int2 tl = tiles[ti];
-  const AnnInfo A = m.ann[tl.x];
-  const DiskParams &P = disk_of(m, A.disk);
-  const DiskScalars &sc = m.sc[A.disk];
-  const int j = tl.y + threadIdx.x;
-  ScalarAdj sb = {};
-  if (j < A.N && (long long)*pick == A.zbase + j) {
-    const long long c = A.zbase + j;
-    double pr[5], r, dx, w, tc;
-    cfl_cell(m, tl, A, P, sc, c, pr, &r, &dx, &w, &tc);
-    {
-      double prb[5] = {0, 0, 0, 0, 0};
-      double tcb = 0.0;
-      vjp_cell_dt(P, sc, pr, r, dx, w, tc, *a.lam_dt, prb, sb, &tcb);
-      add5(a.aP, m.Ntot, c, prb);
-      if (tcb != 0.0) atomicAdd(&a.arot[c], tcb);
-    }
-  }
-  reduce_store(sb, a, slot0 + ti);
-}
-
 // sum of n tile partials into *out (one workgroup)
 #define REDUCE_THREADS 1024
 // workgroup b sums the n slots of partial buffer b (SA_FIELDS rows of npart)
@@ -781,21 +757,6 @@ __global__ __launch_bounds__(REDUCE_THREADS) void ka_reduce(const double *part, 
   }
 }
 
-// sum of the tiles' torque into *acc (one workgroup; acc accumulates)
-__global__ __launch_bounds__(TILE) void ka_sum(const double *v, int n, double *acc) {
-  __shared__ double sh[TILE / 64];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += TILE) s += v[i];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < TILE / 64; ++w) t += sh[w];
-    *acc += t;
-  }
-}
-
 // per annulus sum of the cells' rotation adjoints (one workgroup per annulus)
 __global__ __launch_bounds__(TILE) void ka_rot(DevMesh m, AdjMesh a, double *out) {
   __shared__ double sh[TILE / 64];
@@ -832,11 +793,14 @@ __device__ __forceinline__ void chain_pos(const DiskScalars &sc, const ScalarAdj
 // The step's dt enters: both substeps' dt_s (dt, dt/2), the phase the binary
 // advances (omega dt) and every annulus's turn (Omega_a dt, Omega_a = w_a / r_a,
 // cell_misc.c:73-75), and T (lamT).  lam_dt = 0 when dt was clipped (the CFL
-// minimum did not set it).
-__global__ __launch_bounds__(TILE) void ka_chain_dt(DevMesh m, int nann, const DiskScalars *A,
+// minimum did not set it).  Then the CFL step's own backward: lam_dt times
+// the gradient of the step of the cell ka_dt_pick chose, into aP and the
+// cell's rotation adjoint, and its scalar adjoints into *S0 (one cell, one
+// thread); pick is reset for the next step.  m.sc: the scalars the forward's CFL minimum saw.
+__global__ __launch_bounds__(TILE) void ka_chain_dt(DevMesh m, AdjMesh a, int nann, const DiskScalars *A,
                                                     const DiskScalars *B, const ScalarAdj *S1, const ScalarAdj *S2,
                                                     const double *th2, const double *Ltheta, double lamT,
-                                                    AdjChain *cs) {
+                                                    AdjChain *cs, int *pick, ScalarAdj *S0) {
   __shared__ double sh[TILE / 64];
   double s = 0.0;
   for (int k = threadIdx.x; k < nann; k += TILE) {
@@ -855,7 +819,31 @@ __global__ __launch_bounds__(TILE) void ka_chain_dt(DevMesh m, int nann, const D
     const double lam = 1.0 * S1->dt + 0.5 * S2->dt + omega * (cs->Lphi[0] + cs->Lphi[1] + lphi2[0] + lphi2[1]) +
                        lamT + t;
     const double dtmin = __longlong_as_double((long long)A->dtmin_bits);
-    cs->lam_dt = A->dt == dtmin ? lam : 0.0;
+    const double lam_dt = A->dt == dtmin ? lam : 0.0;
+    cs->lam_dt = lam_dt;
+    ScalarAdj sb = {};
+    const int c = *pick;
+    if (c >= 0 && c < m.Ntot) {
+      int lo = 0, hi = nann - 1;  // the annulus holding cell c (zbase ascending)
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (m.ann[mid].zbase <= c)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const AnnInfo An = m.ann[lo];
+      const DiskParams &P = disk_of(m, An.disk);
+      double pr[5], r, dx, w, tc;
+      cfl_cell(m, make_int2(lo, 0), An, P, m.sc[An.disk], c, pr, &r, &dx, &w, &tc);
+      double prb[5] = {0, 0, 0, 0, 0};
+      double tcb = 0.0;
+      vjp_cell_dt(P, m.sc[An.disk], pr, r, dx, w, tc, lam_dt, prb, sb, &tcb);
+      add5(a.aP, m.Ntot, c, prb);
+      a.arot[c] += tcb;
+    }
+    *S0 = sb;
+    *pick = 0x7fffffff;
   }
 }
 
@@ -902,18 +890,9 @@ void launch_adjoint_reduce(const AdjMesh &a, const LaunchPlan &L, hipStream_t st
                      a.visc ? nz + 2 * nu + 2 * nrf : nz + nu + nrf, out);
 }
 
-void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
-                       const unsigned long long *dtbits, int *pick, ScalarAdj *out) {
-  if (!L.n_itiles) return;
-  hipMemsetAsync(pick, 0x7f, sizeof(int), st);  // "no cell yet": a large positive index
-  hipLaunchKernelGGL(ka_dt_pick, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles, dtbits, pick);
-  hipLaunchKernelGGL(ka_dt, dim3(L.n_itiles), dim3(TILE), 0, st, m, a, L.itiles, (const int *)pick, 0);
-  hipLaunchKernelGGL(ka_reduce, dim3(1), dim3(REDUCE_THREADS), 0, st, (const double *)a.part, a.npart, L.n_itiles,
-                     out);
-}
-
-void launch_torque_sum(const double *tiles, int n, double *acc, hipStream_t st) {
-  hipLaunchKernelGGL(ka_sum, dim3(1), dim3(TILE), 0, st, tiles, n, acc);
+void launch_adjoint_pick(const DevMesh &m, const LaunchPlan &L, hipStream_t st, const unsigned long long *dtbits,
+                         int *pick) {
+  if (L.n_itiles) hipLaunchKernelGGL(ka_dt_pick, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles, dtbits, pick);
 }
 
 void launch_adjoint_rot(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st, double *out) {
@@ -926,10 +905,12 @@ void launch_copy_scalars(DiskScalars *dst, const DiskScalars *src, hipStream_t s
   hipLaunchKernelGGL(ka_copy_scalars, dim3(1), dim3(64), 0, st, dst, src);
 }
 
-void launch_adjoint_chain_dt(const DevMesh &m, int nann, const DiskScalars *A, const DiskScalars *B,
-                             const ScalarAdj *S1, const ScalarAdj *S2, const double *th2, const double *Ltheta,
-                             double lamT, AdjChain *cs, hipStream_t st) {
-  hipLaunchKernelGGL(ka_chain_dt, dim3(1), dim3(TILE), 0, st, m, nann, A, B, S1, S2, th2, Ltheta, lamT, cs);
+void launch_adjoint_chain_dt(const DevMesh &m, const AdjMesh &a, int nann, const DiskScalars *A,
+                             const DiskScalars *B, const ScalarAdj *S1, const ScalarAdj *S2, const double *th2,
+                             const double *Ltheta, double lamT, AdjChain *cs, int *pick, ScalarAdj *S0,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(ka_chain_dt, dim3(1), dim3(TILE), 0, st, m, a, nann, A, B, S1, S2, th2, Ltheta, lamT, cs, pick,
+                     S0);
 }
 
 void launch_adjoint_chain_acc(int nann, const DiskScalars *pre, const DiskScalars *A, const DiskScalars *B,

@@ -81,25 +81,23 @@ void Engine::adjoint_redirect(const AdjSubRec &r) {
   adj_ftype = r.ftype;  // the face builder records the face types
 }
 
-// One RK2 step exactly as enqueue_rk2 launches it, with the pre-substep
-// scalar ops as their own launch so the scalars can be recorded between
-// them; every k_update adds its torque to acc.
+// One RK2 step as enqueue_rk2 launches it, recording: the scalars (the
+// scalar launches write screc[0..2]: before and after the step's opening
+// operations, after the first substep's), the update tiles' torque, added up
+// into acc by the substeps' scalar launches, and (rec) the substeps' state,
+// slopes, faces and converted conserved states.
 void Engine::adjoint_step(const AdjSubRec *rec, DiskScalars *screc, double *torque, double *acc) {
-  DevMesh m = mesh();
   enqueue_set_w();
-  launch_mindt(m, plan, st, NQ);
-  if (screc) launch_copy_scalars(screc, d_sc, st);
-  launch_scalar(m, plan, st, SC_STEP_BEGIN | SC_SUB_BEGIN);
-  if (screc) launch_copy_scalars(screc + 1, d_sc, st);
+  adj_sc_rec = screc;
   adj_torque = torque;
+  adj_torque_acc = acc;
   if (rec) adjoint_redirect(rec[0]);
-  enqueue_substep(1.0, 1, 0, 0, SC_AFTER_UPDATE | SC_SUB1_END | SC_SUB_BEGIN);
-  launch_torque_sum(torque, plan.n_utiles, acc, st);
-  if (screc) launch_copy_scalars(screc + 2, d_sc, st);
+  enqueue_substep(1.0, 1, 0, SC_PLM_DT | SC_STEP_BEGIN | SC_SUB_BEGIN, SC_AFTER_UPDATE | SC_SUB1_END | SC_SUB_BEGIN);
   if (rec) adjoint_redirect(rec[1]);
   enqueue_substep(0.5, 0, w_static ? 0 : 1, 0, SC_AFTER_UPDATE | SC_SUB2_END);
-  launch_torque_sum(torque, plan.n_utiles, acc, st);
+  adj_sc_rec = nullptr;
   adj_torque = nullptr;
+  adj_torque_acc = nullptr;
   adj_usave = nullptr;
   adj_ftype = nullptr;
 }
@@ -130,7 +128,8 @@ int Engine::torque_loss(int nsteps, double *loss) {
 
 // torque_loss without device allocation: slot 0 of the fixed scratch is the
 // accumulator (cleared by a fill kernel), slots 1.. the tile sums, which
-// every k_update tile overwrites before ka_sum reads them.
+// every k_update tile overwrites before the substep's scalar launch adds
+// them up.
 int Engine::torque_loss_noalloc(int nsteps, double *loss) {
   if (!ok || !adjoint_scope()) return -1;
   if ((long long)plan.n_utiles + 1 > TORQUE_SCRATCH_DOUBLES) {
@@ -187,15 +186,18 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   const size_t npairs = hpairs.size();
 
   // one allocation: the adjoint fields, the checkpoints, the record of one
-  // segment (2 * every substeps); laid out once to size it, once to carve it
-  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr, *arot, *torque, *acc, *ckpt, *fin, *th1, *th2,
-      *Ltheta;
-  DiskScalars *screc, *ckpt_sc, *fin_sc;
+  // segment (2 * every substeps) -- two records with several segments, so
+  // that the recompute of one segment (on st_aux) overlaps the reverse sweep
+  // of the next (on st); laid out once to size it, once to carve it
+  const int nbuf = nseg > 1 ? 2 : 1;
+  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr, *aFS, *arot, *torque, *acc, *ckpt, *fin, *th1,
+      *th2, *Ltheta;
+  DiskScalars *screcs[2] = {nullptr, nullptr}, *ckpt_sc, *fin_sc;
   double *part;
   ScalarAdj *out;
   AdjChain *cs;
   int *pick;
-  std::vector<AdjSubRec> rec(2 * every);
+  std::vector<AdjSubRec> recs[2] = {std::vector<AdjSubRec>(2 * every), std::vector<AdjSubRec>(nbuf > 1 ? 2 * every : 0)};
   Arena ar;
   size_t zero_bytes = 0;
   auto layout = [&]() {
@@ -216,31 +218,34 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     aSL = ar.take<double>(NQ * nf);
     aSR = ar.take<double>(NQ * nf);
     fr = ar.take<double>((size_t)AFR_N * nf);
+    aFS = ar.take<double>((size_t)NQ * nf);  // the reverse sweep's face slopes (the forward's Fface is in use)
     torque = ar.take<double>((size_t)plan.n_utiles);
     ckpt = ar.take<double>(wA * nseg);
     fin = ar.take<double>(nseg > 1 ? wA : 0);
     th1 = ar.take<double>((size_t)nann);
     th2 = ar.take<double>((size_t)nann);
-    screc = ar.take<DiskScalars>((size_t)3 * every);
+    for (int b = 0; b < nbuf; ++b) screcs[b] = ar.take<DiskScalars>((size_t)3 * every);
     ckpt_sc = ar.take<DiskScalars>((size_t)nseg);
     fin_sc = ar.take<DiskScalars>(1);
     part = ar.take<double>(2 * 9 * npart);  // the two substeps' partial sums
     pick = ar.take<int>(1);
-    for (auto &r : rec) {
-      r.prim = ar.take<double>(nq);
-      r.grad = ar.take<double>(nq);
-      r.gradp = ar.take<double>(nq);
-      r.usave = ar.take<double>(nq);
-      r.tiph = ar.take<double>(nz);
-      r.dphi = ar.take<double>(nz);
-      r.fdphi = ar.take<double>(nf);
-      r.fcm = ar.take<double>(nf);
-      r.fL = ar.take<int>(nf);
-      r.fR = ar.take<int>(nf);
-      r.fo_r = ar.take<int>(nz);
-      r.run = ar.take<int>((size_t)nrun);
-      r.pst = ar.take<PairState>(npairs);
-      r.ftype = ar.take<signed char>(nf);
+    for (int b = 0; b < nbuf; ++b) {
+      for (auto &r : recs[b]) {
+        r.prim = ar.take<double>(nq);
+        r.grad = ar.take<double>(nq);
+        r.gradp = ar.take<double>(nq);
+        r.usave = ar.take<double>(nq);
+        r.tiph = ar.take<double>(nz);
+        r.dphi = ar.take<double>(nz);
+        r.fdphi = ar.take<double>(nf);
+        r.fcm = ar.take<double>(nf);
+        r.fL = ar.take<int>(nf);
+        r.fR = ar.take<int>(nf);
+        r.fo_r = ar.take<int>(nz);
+        r.run = ar.take<int>((size_t)nrun);
+        r.pst = ar.take<PairState>(npairs);
+        r.ftype = ar.take<signed char>(nf);
+      }
     }
   };
   layout();
@@ -296,19 +301,24 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     hipMemcpyAsync(c + nq + 2 * nz, d_RKtiph, nz * 8, hipMemcpyDeviceToDevice, st);
   };
 
+  // the record of segment seg: the last in record 0, then alternating
+  auto buf_of = [&](int seg) { return nbuf > 1 ? (nseg - 1 - seg) & 1 : 0; };
+
   // forward: the loss, a checkpoint at the start of every segment, and the
-  // record of the last segment
-  const int last0 = (nseg - 1) * every;
+  // record of the last nbuf segments (one per record)
+  const int rec0 = (nseg - nbuf) * every;
   const double t0 = get_scalars(0).t;
   for (int n = 0; n < nsteps; ++n) {
     if (n % every == 0) {
       snapshot(ckpt + wA * (n / every));
       launch_copy_scalars(ckpt_sc + n / every, d_sc, st);
     }
-    if (n >= last0)
-      adjoint_step(&rec[2 * (n - last0)], screc + 3 * (n - last0), torque, acc);
-    else
+    if (n >= rec0) {
+      const int b = buf_of(n / every), k = n % every;
+      adjoint_step(&recs[b][2 * k], screcs[b] + 3 * k, torque, acc);
+    } else {
       adjoint_step(nullptr, nullptr, torque, acc);
+    }
   }
   end_record();
   const double T = get_scalars(0).t - t0;
@@ -323,7 +333,8 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   }
 
   // one substep's mesh: input state `in`, everything else from its record
-  const DevMesh base = mesh();
+  DevMesh base = mesh();
+  base.Fface = aFS;
   auto sub_mesh = [&](const AdjSubRec &in, const AdjSubRec &o, DiskScalars *sc) {
     DevMesh m = base;
     m.prim = in.prim;
@@ -364,19 +375,68 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   a.lam_dt = &cs->lam_dt;
   a.arot = arot;
   a.visc = P.visc > 0.0;
+  // the streams: the reverse sweep on st, the recomputes on st_aux.  A
+  // recompute writes the engine's state and one record; it waits for the
+  // loss pass and for the sweep that last read its record, and the sweep of
+  // its segment waits for it.  The sweep reads only its record, its
+  // checkpoint and the constant mesh (its face-slope scratch is aFS).
+  const hipStream_t sm = st;
+  hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_back[2] = {nullptr, nullptr}, ev_fwd = nullptr;
+  bool used[2] = {true, true};  // the loss pass filled both records
+  if (nbuf > 1) {
+    bool good = st_aux || hip_check(hipStreamCreate(&st_aux), "adjoint stream");
+    for (int b = 0; b < 2 && good; ++b)
+      good = hip_check(hipEventCreateWithFlags(&ev_rec[b], hipEventDisableTiming), "adjoint event") &&
+             hip_check(hipEventCreateWithFlags(&ev_back[b], hipEventDisableTiming), "adjoint event");
+    good = good && hip_check(hipEventCreateWithFlags(&ev_fwd, hipEventDisableTiming), "adjoint event");
+    if (!good) return -1;
+    hipEventRecord(ev_fwd, sm);
+    hipStreamWaitEvent(st_aux, ev_fwd, 0);
+  }
+  // A segment's recompute from its checkpoint, recording it, in pieces: the
+  // host issues one recompute step and one sweep step in turn, so that both
+  // streams hold queued work (issued whole, the recompute's launches kept
+  // the host busy for as long as the GPU took to run them, and the sweep
+  // queued behind them found an idle stream only when they were done)
+  auto recompute_begin = [&](int seg) {
+    const int b = buf_of(seg);
+    if (used[b]) hipStreamWaitEvent(st_aux, ev_back[b], 0);
+    st = st_aux;
+    adjoint_restore(ckpt + wA * seg, false);
+    launch_copy_scalars(d_sc, ckpt_sc + seg, st);
+    st = sm;
+  };
+  auto recompute_step = [&](int seg, int k) {  // step seg * every + k
+    const int b = buf_of(seg);
+    st = st_aux;
+    adjoint_step(&recs[b][2 * k], screcs[b] + 3 * k, torque, acc);
+    st = sm;
+  };
+  auto recompute_end = [&](int seg) {
+    st = st_aux;
+    end_record();
+    st = sm;
+    hipEventRecord(ev_rec[buf_of(seg)], st_aux);
+  };
+  // "no cell yet" for the first step's CFL pick (ka_chain_dt resets it)
+  hipMemsetAsync(pick, 0x7f, sizeof(int), sm);
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int n0 = seg * every, n1 = std::min(nsteps, n0 + every);
-    if (seg != nseg - 1) {  // recompute the segment from its checkpoint, recording it
-      adjoint_restore(ckpt + wA * seg, false);
-      launch_copy_scalars(d_sc, ckpt_sc + seg, st);
-      for (int n = n0; n < n1; ++n) adjoint_step(&rec[2 * (n - n0)], screc + 3 * (n - n0), torque, acc);
-      end_record();
-    }
+    const int b = buf_of(seg);
+    // the previous segment's recompute (when the loss pass did not record
+    // it) overlaps this segment's sweep; an earlier segment is whole
+    const int rn = seg - 1 >= 0 && seg - 1 < nseg - nbuf ? every : 0;
+    if (rn) recompute_begin(seg - 1);
+    if (seg < nseg - nbuf) hipStreamWaitEvent(sm, ev_rec[b], 0);
+    const std::vector<AdjSubRec> &rec = recs[b];
+    DiskScalars *const screc = screcs[b];
     AdjSubRec start = rec[0];
     start.prim = ckpt + wA * seg;
     start.tiph = start.prim + nq;
     start.dphi = start.tiph + nz;
-    for (int n = n1 - 1; n >= n0; --n) {
+    for (int n = n1 - 1, r = 0; n >= n0 || r < rn; --n, ++r) {
+      if (r < rn) recompute_step(seg - 1, r);
+      if (n < n0) continue;
       const int k = n - n0;
       DiskScalars *scP = screc + 3 * k, *scA = scP + 1, *scB = scP + 2;
       const AdjSubRec &in1 = k == 0 ? start : rec[2 * k - 1];
@@ -387,8 +447,8 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
       a.fac = 0.5;
       a.copy_first = 0;
       a.part = part;
-      launch_adjoint_substep(m2, a, plan, st);
-      launch_adjoint_rot(m2, a, plan, st, th2);
+      launch_adjoint_substep(m2, a, plan, sm);
+      launch_adjoint_rot(m2, a, plan, sm, th2);
       std::swap(aP, aPn);
       // substep 1 (fac 1, RKcons = prim2cons(prim))
       const DevMesh m1 = sub_mesh(in1, rec[2 * k], scA);
@@ -397,25 +457,37 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
       a.fac = 1.0;
       a.copy_first = 1;
       a.part = part + 9 * npart;
-      launch_adjoint_substep(m1, a, plan, st);
+      launch_adjoint_substep(m1, a, plan, sm);
       std::swap(aP, aPn);
       // both substeps' scalar adjoints, out[0] and out[1], in one launch
       a.part = part;
-      launch_adjoint_reduce(a, plan, st, 2, out);
+      launch_adjoint_reduce(a, plan, sm, 2, out);
       // the step's dt: its CFL minimum, at the step-start state with the
-      // scalars k_mindt saw
-      launch_adjoint_chain_dt(m1, nann, scA, scB, out + 1, out + 0, th2, Ltheta, lamT, cs, st);
+      // scalars the forward's CFL minimum saw
       DevMesh md = m1;
       md.sc = scP;
       a.aP = aPn;  // the step-start adjoint
-      launch_adjoint_dt(md, a, plan, st, &scA->dtmin_bits, pick, out + 2);
-      launch_adjoint_rot(m1, a, plan, st, th1);  // substep 1 and the CFL step
-      launch_adjoint_chain_acc(nann, scP, scA, scB, out + 2, out + 1, out + 0, th1, th2, Ltheta, cs, st);
+      launch_adjoint_pick(md, plan, sm, &scA->dtmin_bits, pick);
+      launch_adjoint_chain_dt(md, a, nann, scA, scB, out + 1, out + 0, th2, Ltheta, lamT, cs, pick, out + 2, sm);
+      launch_adjoint_rot(m1, a, plan, sm, th1);  // substep 1 and the CFL step
+      launch_adjoint_chain_acc(nann, scP, scA, scB, out + 2, out + 1, out + 0, th1, th2, Ltheta, cs, sm);
+    }
+    if (rn) recompute_end(seg - 1);
+    if (nbuf > 1) {
+      hipEventRecord(ev_back[b], sm);
+      used[b] = true;
     }
   }
   AdjChain C;
   hip_check(hipMemcpy(&C, cs, sizeof(C), hipMemcpyDeviceToHost), "adjoint D2H");
   hip_check(hipMemcpy(g_prim0, aPn, nq * 8, hipMemcpyDeviceToHost), "adjoint D2H");
+  // (the blocking copies have waited for both streams; segment 0's
+  // recompute, the last, precedes segment 0's sweep on st)
+  for (int b = 0; b < 2; ++b) {
+    if (ev_rec[b]) hipEventDestroy(ev_rec[b]);
+    if (ev_back[b]) hipEventDestroy(ev_back[b]);
+  }
+  if (ev_fwd) hipEventDestroy(ev_fwd);
   if (nseg > 1) {
     adjoint_restore(fin, false);
     launch_copy_scalars(d_sc, fin_sc, st);

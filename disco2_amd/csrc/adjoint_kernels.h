@@ -46,19 +46,22 @@ void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan
 // out[b] = the sum of the substep partials in a.part + b * 9 * a.npart, for
 // b < nbuf, one workgroup each
 void launch_adjoint_reduce(const AdjMesh &a, const LaunchPlan &L, hipStream_t st, int nbuf, ScalarAdj *out);
-// pick: device int scratch for the index of the cell that sets dt; dtbits:
-// the step's CFL minimum (device)
-void launch_adjoint_dt(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
-                       const unsigned long long *dtbits, int *pick, ScalarAdj *out);
-void launch_torque_sum(const double *tiles, int n, double *acc, hipStream_t st);
+// pick: device int scratch (0x7fffffff: none yet) that receives the lowest
+// interior cell whose CFL step is the step's minimum dtbits (device)
+void launch_adjoint_pick(const DevMesh &m, const LaunchPlan &L, hipStream_t st, const unsigned long long *dtbits,
+                         int *pick);
 // per annulus sum of a.arot into out[annulus]; clears a.arot
 void launch_adjoint_rot(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st, double *out);
 void launch_copy_scalars(DiskScalars *dst, const DiskScalars *src, hipStream_t st);
 // lam_dt of a step from its substeps' scalar adjoints S1, S2, the second
-// substep's annulus rotation adjoints th2 and the sums so far (m: annuli, wiph)
-void launch_adjoint_chain_dt(const DevMesh &m, int nann, const DiskScalars *A, const DiskScalars *B,
-                             const ScalarAdj *S1, const ScalarAdj *S2, const double *th2, const double *Ltheta,
-                             double lamT, AdjChain *cs, hipStream_t st);
+// substep's annulus rotation adjoints th2 and the sums so far (m: annuli,
+// wiph, and the scalars the forward's CFL minimum saw); then the CFL step's backward at the
+// picked cell into a.aP and a.arot, its scalar adjoints into S0, and pick
+// reset
+void launch_adjoint_chain_dt(const DevMesh &m, const AdjMesh &a, int nann, const DiskScalars *A,
+                             const DiskScalars *B, const ScalarAdj *S1, const ScalarAdj *S2, const double *th2,
+                             const double *Ltheta, double lamT, AdjChain *cs, int *pick, ScalarAdj *S0,
+                             hipStream_t st);
 // adds a step's contributions (S0: the CFL step's) to the sums
 void launch_adjoint_chain_acc(int nann, const DiskScalars *pre, const DiskScalars *A, const DiskScalars *B,
                               const ScalarAdj *S0, const ScalarAdj *S1, const ScalarAdj *S2, const double *th1,

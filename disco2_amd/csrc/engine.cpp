@@ -350,6 +350,7 @@ Engine::~Engine() {
     if (p) hipFree(p);
   if (ev0) hipEventDestroy(ev0);
   if (ev1) hipEventDestroy(ev1);
+  if (st_aux) hipStreamDestroy(st_aux);
   if (st) hipStreamDestroy(st);
 }
 
@@ -392,6 +393,8 @@ DevMesh Engine::mesh() const {
   m.pair_perm = sched.largest_first ? d_pair_perm : nullptr;
   m.u_save = adj_usave;
   m.torque_tile = adj_torque ? adj_torque : d_led_torque;
+  m.torque_acc = adj_torque ? adj_torque_acc : nullptr;
+  m.sc_rec = adj_sc_rec;
   m.ftype = adj_ftype;
   return m;
 }

@@ -55,6 +55,8 @@ struct Engine {
   bool w_static = false;  // no disk moves its annuli with the gas (C_RIGID): wiph depends on r alone
   bool w_ready = false;   // wiph has been set (a launch for it is enqueued)
   double *adj_usave = nullptr, *adj_torque = nullptr;  // adjoint recording (mesh() passes them on)
+  double *adj_torque_acc = nullptr;
+  DiskScalars *adj_sc_rec = nullptr;
   signed char *adj_ftype = nullptr;
   void *adj_mem = nullptr;  // the adjoint's working memory, kept for later calls
   size_t adj_cap = 0;
@@ -93,6 +95,9 @@ struct Engine {
 
   LaunchPlan plan{};
   hipStream_t st = nullptr;
+  // the adjoint's recompute stream (default priority): a segment's forward
+  // recompute runs on it while the reverse sweep of the next segment runs on st
+  hipStream_t st_aux = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_cur = -1;

@@ -30,6 +30,12 @@ struct DevMesh {
   // adjoint recording (null in plain runs): k_update stores the conserved
   // state it converts (u_save, NQ x Ntot) and its tiles' torque sums
   double *u_save, *torque_tile;
+  // and, in the launches that run a disk's scalar operations: torque_acc[d]
+  // accumulates the sum of its update tiles' torque (after each update), and
+  // sc_rec[3 d ...] receives the scalars of an RK2 step: before and after the
+  // step's opening operations (SC_STEP_BEGIN), after the first substep's
+  double *torque_acc;
+  DiskScalars *sc_rec;
   signed char *ftype;     // the face builder records each face's bounding edges here
   long long Ntot, NFtot;
   int *err;

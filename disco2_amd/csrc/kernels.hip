@@ -1345,13 +1345,33 @@ __device__ __forceinline__ void fold_mdot(const DevMesh &m, const DiskParams &P,
   __syncthreads();
 }
 
+// SC_AFTER_UPDATE with torque_acc: torque_acc[d] += the sum of the disk's
+// update tiles' torque (k_update), thread t adding tiles t, t + NT, ... in
+// turn, then the waves' sums in order
+template <int NT>
+__device__ __forceinline__ void fold_torque(const DevMesh &m, const DiskParams &P, int d) {
+  __shared__ double s_a[NT / 64];
+  double v = 0.0;
+  for (int it = P.utile0 + (int)threadIdx.x; it < P.utile1; it += NT) v += m.torque_tile[it];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if ((threadIdx.x & 63) == 0) s_a[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < NT / 64; ++w) t += s_a[w];
+    m.torque_acc[d] += t;
+  }
+}
+
 // the scalar operations of disk d, by one workgroup of NT threads
 template <int NT>
 __device__ __forceinline__ void scalar_block(const DevMesh &m, int d, int ops) {
   const DiskParams &P = m.dp[d];
   DiskScalars &s = m.sc[d];
+  if (m.sc_rec && (ops & SC_STEP_BEGIN) && threadIdx.x == 0) m.sc_rec[3 * d] = s;  // (before the folds write s)
   if (ops & (SC_DT_FOLD | SC_STEP_BEGIN)) fold_dt<NT>(m, P, s);
   if ((ops & SC_AFTER_UPDATE) && P.sink == 1) fold_mdot<NT>(m, P, s);
+  if ((ops & SC_AFTER_UPDATE) && m.torque_acc) fold_torque<NT>(m, P, d);
   if (threadIdx.x == 0) {
     // on register copies: the operations' dependent chains would otherwise
     // go through global memory (the two structs may alias)
@@ -1359,6 +1379,8 @@ __device__ __forceinline__ void scalar_block(const DevMesh &m, int d, int ops) {
     DiskScalars sl = s;
     scalar_ops(Pl, sl, ops);
     s = sl;
+    if (m.sc_rec && (ops & SC_STEP_BEGIN)) m.sc_rec[3 * d + 1] = sl;
+    if (m.sc_rec && (ops & SC_SUB1_END)) m.sc_rec[3 * d + 2] = sl;
   }
   // the A_MILOS frame rotation (src/Sim/sim_w_a.c) of every annulus, at the
   // mass separation a the next kernels will see: after any mass move (the
