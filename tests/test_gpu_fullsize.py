@@ -20,7 +20,23 @@
   the step shrinks (relative difference 1.4e-4 at h = 1e-6, 9.8e-6 at 1e-7,
   3.1e-6 at 1e-8, 1.1e-6 at 3e-9, the one-sided differences closing in
   linearly in h), so q is differenced with h = 3e-9 and alpha with 1e-7
-  (1.5e-7).
+  (1.5e-7);
+* the same workload, its seeded state untied (P, v_r and omega perturbed by
+  1e-3 relative, as the per-zone checks of test_gpu_adjoint.py), at the
+  benchmark's checkpoint interval (every 20 steps): dL/d q and dL/d alpha
+  against one-sided and central differences at h = 1e-6 ... 1e-8.  At
+  h = 1e-6 the central difference agrees to 1e-5 (alpha: 1.4e-6) unless a
+  kink of the loss falls inside the step (the CFL minimum and the limiters
+  switch): then the two one-sided differences disagree by more than 1e-3
+  while one of them is within 1e-3 of the adjoint (q: 1.2e-4 and 1.3e-2, the
+  central difference 6.3e-3).  Some step of the sweep must agree to 1e-6
+  (q: the minus side at 1e-7, 3.9e-7; alpha: 7.8e-8 at 3e-7);
+* dL/d(initial primitives) of 48 zones spread over the full-size disk (six
+  annuli from the inner edge to the outer, two cells each, rho, P, v_r,
+  omega) against central differences at h = 1e-6, 1e-7, 1e-8 relative:
+  each to 1e-4 of |fd| + 1e-3 max|g| at one of them (a kink can fall inside
+  the wider steps: annulus 1's omega misses by 30% at h = 1e-6, where its
+  minus-side difference jumps, and agrees to 1e-7 at h = 1e-7).
 """
 import numpy as np
 import pytest
@@ -135,3 +151,81 @@ def test_adjoint_full_size_matches_differences():
     assert abs(L - loss()) <= 1e-12 * abs(L)
     assert abs(ga - fa) <= 1e-6 * abs(fa)
     assert abs(gq - fq) <= 5e-6 * abs(fq)
+
+
+@pytest.fixture(scope="module")
+def untied_adjoint():
+    from disco2_amd import configs, ics
+    from disco2_amd.api import Batch, Setup
+    p, kind, kw = configs.c5_adjoint()
+    s = Setup(p)
+    pr = ics.make_ic(kind, s.geometry(), p, **kw)
+    rs = np.random.RandomState(7)
+    n = pr.shape[1]
+    cs = np.sqrt(pr[1] / pr[0])
+    pr[1] *= 1 + 1e-3 * rs.uniform(-1, 1, n)
+    pr[2] += 1e-3 * cs * rs.uniform(-1, 1, n)
+    pr[3] *= 1 + 1e-3 * rs.uniform(-1, 1, n)
+    steps, every = 200, 20
+    print(f"binary2d_r256_adjoint untied: {steps} steps, checkpoints every {every}", flush=True)
+    b = Batch([s])
+    b.upload("prim", pr)
+    b.init()
+    L, ga, gq, gp = b.adjoint(steps, checkpoint_every=every)
+    b.close()
+
+    def loss(prim=None, **over):
+        bb = Batch([Setup(dict(p, **over))])
+        bb.upload("prim", pr if prim is None else prim)
+        bb.init()
+        try:
+            return bb.torque_loss(steps)
+        finally:
+            bb.close()
+
+    return dict(p=p, s=s, prim0=pr, L=L, ga=ga, gq=gq, gp=gp, loss=loss)
+
+
+@pytest.mark.parametrize("name", ["q", "alpha"])
+def test_adjoint_full_size_parameters_untied(untied_adjoint, name):
+    u = untied_adjoint
+    key = "MassRatio" if name == "q" else "Explicit_Viscosity"
+    x0, g = u["p"][key], (u["gq"] if name == "q" else u["ga"])
+    L0 = u["loss"]()
+    assert abs(u["L"] - L0) <= 1e-12 * abs(L0)
+    rows = {}
+    for h in (1e-6, 3e-7, 1e-7, 3e-8, 1e-8):
+        Lp, Lm = u["loss"](**{key: x0 * (1 + h)}), u["loss"](**{key: x0 * (1 - h)})
+        fp, fm, fc = (Lp - L0) / (h * x0), (L0 - Lm) / (h * x0), (Lp - Lm) / (2 * h * x0)
+        rows[h] = [abs(v - g) / abs(g) for v in (fc, fp, fm)]
+        print(f"dL/d{name} h {h:.0e}: adjoint {g:.10e} central {fc:.10e} one-sided + {fp:.10e} - {fm:.10e}; "
+              f"relative differences {rows[h][0]:.1e} {rows[h][1]:.1e} {rows[h][2]:.1e}")
+    c6, p6, m6 = rows[1e-6]
+    if c6 > 1e-5:  # a kink inside [x0 - h, x0 + h]: the two sides disagree, one is near the adjoint
+        assert abs(p6 - m6) > 1e-3 and min(p6, m6) < 1e-3, rows[1e-6]
+    assert min(min(r) for r in rows.values()) <= 1e-6, rows
+
+
+def test_adjoint_full_size_initial_state_untied(untied_adjoint):
+    u = untied_adjoint
+    s, prim0, gp = u["s"], u["prim0"], u["gp"]
+    aoff = np.concatenate([[0], np.cumsum(s.N_p)])
+    gmax = np.max(np.abs(gp))
+    worst, n = 0.0, 0
+    for i in (1, 2, s.NR // 4, s.NR // 2, 3 * s.NR // 4, s.NR - 3):
+        for j in (3, int(s.N_p[i]) // 2):
+            z = int(aoff[i] + j)
+            for q in (0, 1, 2, 3):  # rho, P, v_r, omega
+                x0 = prim0[q, z]
+                errs = []
+                for hr in (1e-6, 1e-7, 1e-8):
+                    h = hr * max(abs(x0), 1e-4)
+                    a, c = prim0.copy(), prim0.copy()
+                    a[q, z] = x0 + h
+                    c[q, z] = x0 - h
+                    fd = (u["loss"](a) - u["loss"](c)) / (2 * h)
+                    errs.append(abs(gp[q, z] - fd) / (abs(fd) + 1e-3 * gmax))
+                worst = max(worst, min(errs))
+                n += 1
+                assert min(errs) <= 1e-4, f"zone {z} (annulus {i}) q {q}: adjoint {gp[q, z]:.6e}, errors {errs}"
+    print(f"{n} initial-state derivatives at full size, worst {worst:.2e} of |fd| + 1e-3 max|g|")
