@@ -244,8 +244,9 @@ def counter_passes(workload, timeout=240):
     return out
 
 
-def counters_child(workload):
-    """the run the counter passes profile (see counter_passes)"""
+def counters_child(workload, schedule=None):
+    """the run the counter passes profile (see counter_passes); schedule: a
+    launch schedule to set instead of the default"""
     from disco2_amd import ics
     from disco2_amd._lib import lib
     from disco2_amd.api import Batch, Setup
@@ -255,6 +256,8 @@ def counters_child(workload):
     b = Batch([s])
     b.upload("prim", ics.make_ic(ic_kind, s.geometry(), params, **ic_kw))
     b.init()
+    if schedule is not None:
+        b.set_schedule(schedule)
     lib().disco_calibrate_copy(CALIB_DOUBLES)
     b.run(1, graph=False)
     lib().disco_calibrate_copy(64)
@@ -398,7 +401,10 @@ def measure_workload(name, args, batch_out=None):
     batch.init()
     graph = not args.no_graph
     schedule = None
-    if not args.no_autotune:
+    if args.schedule is not None:
+        batch.set_schedule(args.schedule)
+        schedule = {"chosen": args.schedule, "ms_per_step": None}
+    elif not args.no_autotune:
         k, cand = batch.autotune(args.autotune_steps)
         schedule = {"chosen": k, "ms_per_step": {str(c): round(v, 4) for c, v in cand.items()}}
         print(f"[bench] {name}: autotuned schedule {k}: {schedule['ms_per_step']}", file=sys.stderr, flush=True)
@@ -489,6 +495,8 @@ def main(argv=None):
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch schedule")
     ap.add_argument("--autotune-steps", type=int, default=5)
+    ap.add_argument("--schedule", type=int, default=None,
+                    help="use this launch schedule (0 .. DISCO_NSCHEDULES - 1) instead of autotuning")
     ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--counters-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-batched", action="store_true")
@@ -502,7 +510,7 @@ def main(argv=None):
                     help="run only the diagnostics leg on the secondary workload and print its JSON")
     args = ap.parse_args(argv)
     if args.counters_child:
-        return counters_child(args.workload)
+        return counters_child(args.workload, args.schedule)
     if args.gpus != 1:
         print("bench.py measures one GPU: disco2_amd has no multi-GPU path (one disk or batch per "
               "GPU); run with --gpus 1", file=sys.stderr)

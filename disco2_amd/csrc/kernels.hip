@@ -745,7 +745,7 @@ __device__ __forceinline__ void add_inc_L(const double *X, const double *Y, doub
   }
 }
 template <int NQ>
-__device__ __forceinline__ void add_face_R(const DevMesh &m, const double *F, long long stride, long long g,
+__device__ __forceinline__ void add_face_R(const double *F, long long stride, long long g,
                                            double *u, double r_cell, bool visc) {
 #pragma unroll
   for (int q = 0; q < NQ; ++q) u[q] += F[q * stride + g];
@@ -756,7 +756,7 @@ __device__ __forceinline__ void add_face_R(const DevMesh &m, const double *F, lo
   }
 }
 template <int NQ>
-__device__ __forceinline__ void add_face_L(const DevMesh &m, const double *F, long long stride, long long g,
+__device__ __forceinline__ void add_face_L(const double *F, long long stride, long long g,
                                            double *u, double r_cell, bool visc) {
 #pragma unroll
   for (int q = 0; q < NQ; ++q) u[q] += -F[q * stride + g];
@@ -789,6 +789,77 @@ __device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy
   return t;
 }
 
+// The face ranges of cell j (zone z) of an annulus of N cells: as R of pair
+// pR, up to two runs (the second is the wrap), into b[0..1], e[0..1]; as L
+// of pair pL, one range, into b[2], e[2] (faces_as_R / faces_as_L order).
+// Absent pairs (-1) leave empty ranges.
+__device__ __forceinline__ void cell_face_ranges(const DevMesh &m, int pR, int pL, const int *fo, long long z, int j,
+                                                 int N, int *b, int *e) {
+  b[0] = b[1] = b[2] = 0;
+  e[0] = e[1] = e[2] = 0;
+  if (pR >= 0) {
+    const PairInfo PI = m.pairs[pR];
+    const PairState S = m.pst[pR];
+    int tt = j - S.jpmin;
+    if (tt < 0) tt += N;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (tt <= S.T) {
+        b[u] = m.run_start[PI.run_base + tt];
+        e[u] = m.run_start[PI.run_base + tt + 1];
+      }
+      tt += N;
+    }
+  }
+  if (pL >= 0) {
+    const PairState S = m.pst[pL];
+    int f1 = (j + 1 < N) ? fo[z + 1] : S.count;
+    if (f1 > S.count) f1 = S.count;
+    int f0 = fo[z];
+    if (f0 < 0) f0 = 0;
+    b[2] = f0;
+    e[2] = f1;
+  }
+}
+
+// add the stored increments of faces [b, e) of one range in face order, as
+// add_face_R (R) or add_face_L; two faces' loads are issued before their adds
+template <int NQ, bool R>
+__device__ __forceinline__ void add_faces(const double *F, long long stride, long long base, int b, int e, double *u,
+                                          double r_cell, bool visc) {
+  constexpr int NV = NQ + 3;
+  int f = b;
+  for (; f + 1 < e; f += 2) {
+    double x0[NV], x1[NV];
+    const long long g = base + f;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      x0[q] = F[q * stride + g];
+      x1[q] = F[q * stride + g + 1];
+    }
+    if (visc) {
+#pragma unroll
+      for (int v = NQ; v < NV; ++v) {
+        x0[v] = F[v * stride + g];
+        x1[v] = F[v * stride + g + 1];
+      }
+    }
+    if (R) {
+      add_inc_R<NQ>(x0, x0 + NQ, u, r_cell, visc);
+      add_inc_R<NQ>(x1, x1 + NQ, u, r_cell, visc);
+    } else {
+      add_inc_L<NQ>(x0, x0 + NQ, u, r_cell, visc);
+      add_inc_L<NQ>(x1, x1 + NQ, u, r_cell, visc);
+    }
+  }
+  if (f < e) {
+    if (R)
+      add_face_R<NQ>(F, stride, base + f, u, r_cell, visc);
+    else
+      add_face_L<NQ>(F, stride, base + f, u, r_cell, visc);
+  }
+}
+
 // FUSE: FUSE_R / FUSE_Z evaluate the r / z faces' increments in place (each
 // face by both of its cells) instead of gathering the stored ones
 enum { FUSE_R = 1, FUSE_Z = 2 };
@@ -806,37 +877,19 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
   const DiskParams &P = disk_of(m, A.disk);
   const DiskScalars &sc = m.sc[A.disk];
   const double4 rot = ann_rotation<FEAT>(m, P, tl.x);
-  // the cell's r-face ranges (the heads of the update's dependent gathers),
-  // loaded before the phi faces so that their latency hides behind the
-  // Riemann solves: as R, up to two runs of pair (i-1, i); as L, one range
-  // of pair (i, i+1)
-  int rb[3] = {0, 0, 0}, re[3] = {0, 0, 0};
-  if (t < n) {
+  // the cell's r- and z-face ranges (the heads of the update's dependent
+  // gathers), loaded before the phi faces so that their latency hides behind
+  // the Riemann solves: as R, up to two runs of pair (i-1, i) / (k-1, k); as
+  // L, one range of pair (i, i+1) / (k, k+1)
+  int rb[3], re[3], zb[3] = {0, 0, 0}, ze[3] = {0, 0, 0};
+  {
     const long long cz = A.zbase + j;
-    if (A.i >= 1) {
-      const int pr_ = rpair_of(P, A.k, A.i - 1);
-      const PairInfo PI = m.pairs[pr_];
-      const PairState S = m.pst[pr_];
-      int tt = j - S.jpmin;
-      if (tt < 0) tt += A.N;
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        if (tt <= S.T) {
-          rb[u] = m.run_start[PI.run_base + tt];
-          re[u] = m.run_start[PI.run_base + tt + 1];
-        }
-        tt += A.N;
-      }
-    }
-    if (A.i <= P.NR - 2) {
-      const PairState S = m.pst[rpair_of(P, A.k, A.i)];
-      int f1 = (j + 1 < A.N) ? m.fo_r[cz + 1] : S.count;
-      if (f1 > S.count) f1 = S.count;
-      int f0 = m.fo_r[cz];
-      if (f0 < 0) f0 = 0;
-      rb[2] = f0;
-      re[2] = f1;
-    }
+    const bool live = t < n;
+    cell_face_ranges(m, live && A.i >= 1 ? rpair_of(P, A.k, A.i - 1) : -1,
+                     live && A.i <= P.NR - 2 ? rpair_of(P, A.k, A.i) : -1, m.fo_r, cz, j, A.N, rb, re);
+    if (P.NglobalZ != 1)
+      cell_face_ranges(m, live && A.k >= 1 ? zpair_of(P, A.k - 1, A.i) : -1,
+                       live && A.k <= P.NZ - 2 ? zpair_of(P, A.k, A.i) : -1, m.fo_z, cz, j, A.N, zb, ze);
   }
   // phi faces: each is solved once, by the thread of its left cell (thread 0
   // also solves the face shared with the previous tile)
@@ -906,60 +959,45 @@ __global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const 
         phiL();
       }
     }
-    // r faces in face order: as R of pair (i-1, i), then as L of (i, i+1)
-    if (i >= 1) {
-      const PairInfo PI = m.pairs[rpair_of(P, k, i - 1)];
-      const double rc = PI.r + PI.deltaR;
+    // r faces in face order: as R of pair (i-1, i), then as L of (i, i+1);
+    // then the z faces the same way
 #pragma unroll
-      for (int w = 0; w < 2; ++w)
-        for (int f = rb[w]; f < re[w]; ++f) {
-          if (FUSE & FUSE_R) {
-            double X[NQ], Y[3];
-            const int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, PI.face_base + f, X, Y);
-            if (e) set_err(m.err, e);
-            add_inc_R<NQ>(X, Y, u, rc, visc);
+    for (int dir = 0; dir < 2; ++dir) {
+      if (dir == 1 && P.NglobalZ == 1) break;
+      const bool fuse = dir == 0 ? (FUSE & FUSE_R) != 0 : (FUSE & FUSE_Z) != 0;
+      const int *b = dir == 0 ? rb : zb, *e = dir == 0 ? re : ze;
+      const bool lo = dir == 0 ? i >= 1 : k >= 1, hi = dir == 0 ? i <= P.NR - 2 : k <= P.NZ - 2;
+      if (lo) {
+        const PairInfo PI = m.pairs[dir == 0 ? rpair_of(P, k, i - 1) : zpair_of(P, k - 1, i)];
+        const double rc = dir == 0 ? PI.r + PI.deltaR : PI.r;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          if (fuse) {
+            for (int f = b[w]; f < e[w]; ++f) {
+              double X[NQ], Y[3];
+              const int er = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, PI.face_base + f, X, Y);
+              if (er) set_err(m.err, er);
+              add_inc_R<NQ>(X, Y, u, rc, visc);
+            }
           } else {
-            add_face_R<NQ>(m, m.Fface, m.NFtot, PI.face_base + f, u, rc, visc);
+            add_faces<NQ, true>(m.Fface, m.NFtot, PI.face_base, b[w], e[w], u, rc, visc);
           }
-        }
-    }
-    if (i <= P.NR - 2) {
-      const PairInfo PI = m.pairs[rpair_of(P, k, i)];
-      const double rc = PI.r - PI.deltaL;
-      for (int f = rb[2]; f < re[2]; ++f) {
-        if (FUSE & FUSE_R) {
-          double X[NQ], Y[3];
-          const int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, PI.face_base + f, X, Y);
-          if (e) set_err(m.err, e);
-          add_inc_L<NQ>(X, Y, u, rc, visc);
-        } else {
-          add_face_L<NQ>(m, m.Fface, m.NFtot, PI.face_base + f, u, rc, visc);
         }
       }
-    }
-    if (P.NglobalZ != 1) {
-      if (k >= 1)
-        faces_as_R(m, zpair_of(P, k - 1, i), j, N, [&](const PairInfo &PI, long long g) {
-          if (FUSE & FUSE_Z) {
+      if (hi) {
+        const PairInfo PI = m.pairs[dir == 0 ? rpair_of(P, k, i) : zpair_of(P, k, i)];
+        const double rc = dir == 0 ? PI.r - PI.deltaL : PI.r;
+        if (fuse) {
+          for (int f = b[2]; f < e[2]; ++f) {
             double X[NQ], Y[3];
-            const int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, g, X, Y);
-            if (e) set_err(m.err, e);
-            add_inc_R<NQ>(X, Y, u, PI.r, visc);
-          } else {
-            add_face_R<NQ>(m, m.Fface, m.NFtot, g, u, PI.r, visc);
+            const int er = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, PI.face_base + f, X, Y);
+            if (er) set_err(m.err, er);
+            add_inc_L<NQ>(X, Y, u, rc, visc);
           }
-        });
-      if (k <= P.NZ - 2)
-        faces_as_L(m, zpair_of(P, k, i), m.fo_z, c, j, N, [&](const PairInfo &PI, long long g) {
-          if (FUSE & FUSE_Z) {
-            double X[NQ], Y[3];
-            const int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, g, X, Y);
-            if (e) set_err(m.err, e);
-            add_inc_L<NQ>(X, Y, u, PI.r, visc);
-          } else {
-            add_face_L<NQ>(m, m.Fface, m.NFtot, g, u, PI.r, visc);
-          }
-        });
+        } else {
+          add_faces<NQ, false>(m.Fface, m.NFtot, PI.face_base, b[2], e[2], u, rc, visc);
+        }
+      }
     }
     // cell_add_src, cell_add_visc_src, cell_add_split_fictitious
     // (src/Cell/cell_source.c:73-277, cell_add_split_fictitious.c:11-44)
