@@ -555,20 +555,36 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
   __shared__ double s_S[NC][NQ][TILE];
   const int t = threadIdx.x;
   int n = 0;
+  // the slope sum in face order; two faces per trip, so that the second
+  // face's loads are in flight with the first's
+  auto add_face = [&](const double *S, double dA) {
 #pragma unroll
-  for (int u = 0; u < 3; ++u)
-    for (int f = b[u]; f < e[u]; ++f) {
+    for (int q = 0; q < NQ; ++q) g[q] += S[q] * dA;
+    if (n < NC)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) s_S[n][q][t] = S[q];
+    ++n;
+  };
+#pragma unroll
+  for (int u = 0; u < 3; ++u) {
+    int f = b[u];
+    for (; f + 1 < e[u]; f += 2) {
+      const long long gi = fb[u] + f;
+      double S0[NQ], S1[NQ];
+      own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], gi, S0);
+      own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], gi + 1, S1);
+      const double dA0 = pr[u] * m.fdphi[gi] * pp[u];
+      const double dA1 = pr[u] * m.fdphi[gi + 1] * pp[u];
+      add_face(S0, dA0);
+      add_face(S1, dA1);
+    }
+    if (f < e[u]) {
       const long long gi = fb[u] + f;
       double S[NQ];
       own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], gi, S);
-      double dA = pr[u] * m.fdphi[gi] * pp[u];
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) g[q] += S[q] * dA;
-      if (n < NC)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) s_S[n][q][t] = S[q];
-      ++n;
+      add_face(S, pr[u] * m.fdphi[gi] * pp[u]);
     }
+  }
   const double dz = A.zp - A.zm;
   const double rp = A.rp, rm = A.rm;
   double dAtot;
