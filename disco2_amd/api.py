@@ -171,7 +171,7 @@ class Batch:
     def autotune(self, nsteps=10):
         """Time every launch schedule on a scratch copy of the batch and keep
         the fastest; returns (chosen schedule, {schedule: ms/step})."""
-        ms = np.zeros(256)  # DISCO_NSCHEDULES
+        ms = np.zeros(1024)  # DISCO_NSCHEDULES
         k = lib().disco_batch_autotune(self.h, int(nsteps), dptr(ms))
         check()
         if k < 0:

@@ -632,6 +632,7 @@ void Engine::set_schedule(const Schedule &s0) {
   plan.zftiles = tl_zb[s.zb][2];
   plan.stage_r = s.stage ? stage_r_max : 0;
   plan.stage_z = s.stage ? stage_z_max : 0;
+  plan.face_threads = s.face_threads();
   if (gexec) {  // the next graph run captures the new launches
     hipGraphExecDestroy(gexec);
     hipGraphDestroy(graph);
@@ -641,7 +642,7 @@ void Engine::set_schedule(const Schedule &s0) {
   graph_cur = -1;
 }
 
-// Candidate k: xcd = k & 1, largest_first = (k >> 1) & 1, stage = (k >> 2) & 1.
+// Candidate k: Schedule::of(k) (xcd, largest_first, stage, fuse, zb, fw).
 // Each runs on a scratch engine holding a copy of the batch's state (the
 // batch itself does not move), graph-replayed for nsteps after two warm-up
 // steps.  Schedules change only the order in which workgroups run, never the
@@ -673,7 +674,8 @@ int Engine::autotune(int nsteps, double *ms, int nms) {
     return v;
   };
   // the launch orders first (faces stored, table order), then the layer
-  // blocks under the fastest order, then the fused face evaluations
+  // blocks under the fastest order, then the fused face evaluations, then
+  // the face builder's workgroup size
   double best = 1e300;
   int bestk = -1;
   auto consider = [&](int k) {
@@ -690,6 +692,8 @@ int Engine::autotune(int nsteps, double *ms, int nms) {
   for (int zb = 1; zb < 8; ++zb) consider(order | zb << 5);
   const int blocked = bestk;
   for (int f = 1; f < 4; ++f) consider(blocked | f << 3);
+  const int fused = bestk;
+  for (int w = 1; w < 3; ++w) consider(fused | w << 8);
   set_schedule(Schedule::of(bestk));
   return bestk;
 }
@@ -705,7 +709,7 @@ bool Engine::schedule_distinct(int k) const {
   int nzmax = 1;
   for (const DiskParams &P : hdp) nzmax = P.NZ > nzmax ? P.NZ : nzmax;
   if (s.zb && (1 << (s.zb - 1)) >= nzmax) return false;  // blocks already span every layer at zb - 1
-  return true;
+  if (s.fw == 3) return false;                            // (not a width)
   return true;
 }
 

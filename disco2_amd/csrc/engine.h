@@ -69,8 +69,8 @@ struct Engine {
   // zb: 3D disks' zone, update and z-face tiles in blocks of 2^zb layers,
   // annulus column by annulus column inside a block (0: table order)
   struct Schedule {
-    int xcd = 1, largest_first = 0, stage = 1, fuse = 0, zb = 0;
-    int code() const { return xcd | largest_first << 1 | stage << 2 | fuse << 3 | zb << 5; }
+    int xcd = 1, largest_first = 0, stage = 1, fuse = 0, zb = 0, fw = 0;
+    int code() const { return xcd | largest_first << 1 | stage << 2 | fuse << 3 | zb << 5 | fw << 8; }
     static Schedule of(int k) {
       Schedule s;
       s.xcd = k & 1;
@@ -78,8 +78,10 @@ struct Engine {
       s.stage = (k >> 2) & 1;
       s.fuse = (k >> 3) & 3;
       s.zb = (k >> 5) & 7;
+      s.fw = (k >> 8) & 3;
       return s;
     }
+    int face_threads() const { return 1024 >> (fw < 3 ? fw : 0); }
   } sched;
   std::vector<int2> h_ztiles, h_utiles, h_zft;       // table-order lists
   int2 *d_tiles_zb[8] = {};                          // [zone | update | z face] tiles per layer block

@@ -70,6 +70,7 @@ struct LaunchPlan {
   int any_visc;                         // some disk has viscosity
   int feat;                             // physics paths its disks take (F_* bits)
   int fuse;                             // schedule: r (1) / z (2) face increments evaluated inside k_update
+  int face_threads;                     // schedule: threads per face-builder workgroup (1024, 512, 256)
   const int2 *bc0; int n_bc0;           // ghost annuli 0 and NR-2
   const int2 *bc1; int n_bc1;           // ghost annulus NR-1
   const int2 *sz; int n_sz;             // z ghost layers

@@ -112,7 +112,8 @@ __device__ int guess_outer(double e, const double *Tout, int Nout) {
   return c;
 }
 
-#define FTB 1024  // threads of the face builder (one workgroup per annulus pair)
+// threads of the face builder (one workgroup per annulus pair): 1024, 512
+// or 256, a launch-schedule choice (the faces do not depend on it)
 
 // exclusive prefix sum over the workgroup (wave shuffles, then wave totals)
 __device__ int block_excl_scan(int v, int *s_w, int *total) {
@@ -152,6 +153,7 @@ __device__ int block_excl_scan(int v, int *s_w, int *total) {
 // the reference's; if any did not (ties between edges), thread 0 replays the
 // sequential walk.  The outer annulus's edges are staged in LDS when they
 // fit, so the walks' dependent loads are LDS reads.
+template <int FTB>
 __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, const AnnInfo &A, const AnnInfo &B,
                                            int p, const double *Tout, const double *Dout, double *s_d, int *s_j,
                                            int *s_w) {
@@ -261,6 +263,7 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
 // resident at once) need 8 waves per SIMD: left alone the compiler takes 106
 // SGPRs, which allows 7, so one workgroup per CU and two passes over the
 // CUs.
+template <int FTB>
 __global__ __launch_bounds__(FTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_faces(DevMesh m, int pair0,
                                                                                            int nstage) {
   extern __shared__ double s_stage[];  // Tout [nstage], Dout [nstage]
@@ -279,9 +282,9 @@ __global__ __launch_bounds__(FTB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
       s_stage[nstage + i] = Dg[i];
     }
     __syncthreads();
-    faces_pair(m, P, A, B, p, s_stage, s_stage + nstage, s_d, s_j, s_w);
+    faces_pair<FTB>(m, P, A, B, p, s_stage, s_stage + nstage, s_d, s_j, s_w);
   } else {
-    faces_pair(m, P, A, B, p, Tg, Dg, s_d, s_j, s_w);
+    faces_pair<FTB>(m, P, A, B, p, Tg, Dg, s_d, s_j, s_w);
   }
 }
 
@@ -1483,8 +1486,14 @@ static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st
   const int p0 = dir == 0 ? 0 : L.n_rpairs;
   if (!np) return;
   const int nst = dir == 0 ? L.stage_r : L.stage_z;
-  KLAUNCH(st, dir == 0 ? "k_faces_r" : "k_faces_z",
-          hipLaunchKernelGGL(k_faces, dim3(np), dim3(FTB), (size_t)nst * 2 * sizeof(double), st, m, p0, nst));
+  const size_t lds = (size_t)nst * 2 * sizeof(double);
+  const char *nm = dir == 0 ? "k_faces_r" : "k_faces_z";
+  if (L.face_threads == 256)
+    KLAUNCH(st, nm, hipLaunchKernelGGL(k_faces<256>, dim3(np), dim3(256), lds, st, m, p0, nst));
+  else if (L.face_threads == 512)
+    KLAUNCH(st, nm, hipLaunchKernelGGL(k_faces<512>, dim3(np), dim3(512), lds, st, m, p0, nst));
+  else
+    KLAUNCH(st, nm, hipLaunchKernelGGL(k_faces<1024>, dim3(np), dim3(1024), lds, st, m, p0, nst));
 }
 
 template <int NQ, int FEAT>
@@ -1586,8 +1595,13 @@ void launch_substep(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int N
 bool prepare_launches(const LaunchPlan &L) {
   const int nst = L.stage_r > L.stage_z ? L.stage_r : L.stage_z;
   if (!nst) return true;
-  return hipFuncSetAttribute((const void *)k_faces, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)(nst * 2 * sizeof(double))) == hipSuccess;
+  const int bytes = (int)(nst * 2 * sizeof(double));
+  return hipFuncSetAttribute((const void *)k_faces<1024>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) ==
+             hipSuccess &&
+         hipFuncSetAttribute((const void *)k_faces<512>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) ==
+             hipSuccess &&
+         hipFuncSetAttribute((const void *)k_faces<256>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) ==
+             hipSuccess;
 }
 
 void launch_scalar(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int op) {

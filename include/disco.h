@@ -358,7 +358,8 @@ double disco_batch_last_ms(struct DiscoBatch *);
  * counts; returns the number of names */
 int disco_batch_profile(struct DiscoBatch *, int nsteps, char *names, double *ms, int *counts, int maxk);
 /* Launch schedule of a batch (the results do not depend on it, bit for bit):
- * k = xcd | largest_first << 1 | stage << 2 | fuse << 3 | zb << 5, with xcd:
+ * k = xcd | largest_first << 1 | stage << 2 | fuse << 3 | zb << 5 | fw << 8,
+ * with xcd:
  * workgroups take their tiles in bands per XCD (else in launch order);
  * largest_first: the face builder takes annulus pairs largest outer annulus
  * first; stage: it stages the outer annulus's edges in LDS; fuse (0-3): the
@@ -366,15 +367,16 @@ int disco_batch_profile(struct DiscoBatch *, int nsteps, char *names, double *ms
  * by both of each face's cells instead of being stored per face (r only for
  * batches without z faces); zb (0-7): 3D disks' cell and z-face tiles are
  * listed in blocks of 2^zb layers, annulus column by annulus column inside a
- * block (0: layer by layer), so that z neighbours run close together.  The
- * default is 5 (xcd, stage, nothing fused, layer order).
+ * block (0: layer by layer), so that z neighbours run close together; fw
+ * (0-2): the face builder runs 1024 >> fw threads per annulus pair.  The
+ * default is 5 (xcd, stage, nothing fused, layer order, 1024 threads).
  * disco_batch_autotune times the launch orders (xcd, largest_first, stage),
- * then the layer blocks of the fastest, then the fused
- * variants of the fastest, each for nsteps graph-replayed steps on a scratch
+ * then the layer blocks of the fastest, then the fused variants of the
+ * fastest, then its face-builder widths, each for nsteps graph-replayed steps on a scratch
  * copy of the batch (the batch does not move), keeps the fastest and returns
  * it (-1 on error); ms[k] (DISCO_NSCHEDULES entries, may be NULL) gets each
  * candidate's ms/step, -1 where not timed. */
-#define DISCO_NSCHEDULES 256
+#define DISCO_NSCHEDULES 1024
 int disco_batch_autotune(struct DiscoBatch *, int nsteps, double *ms);
 int disco_batch_set_schedule(struct DiscoBatch *, int k);
 int disco_batch_schedule(struct DiscoBatch *);

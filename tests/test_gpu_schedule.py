@@ -4,8 +4,9 @@ A schedule changes only which workgroup runs which tile (XCD bands or launch
 order) and in which order the face builder takes the annulus pairs (table
 order or largest first), whether it stages the outer annulus in LDS, and
 whether the update evaluates the r / z faces' fluxes itself (each face by
-both of its cells) instead of reading them stored per face; never the
-arithmetic or its order.  So every schedule must give the same
+both of its cells) instead of reading them stored per face, and how many
+threads the face builder runs per annulus pair; never the arithmetic or its
+order.  So every schedule must give the same
 bits, and the autotuner, which times the candidates on a scratch copy of the
 batch, must leave the batch's state as it was.
 """
@@ -39,7 +40,8 @@ def state(b, n):
 def test_every_schedule_gives_the_same_bits(with3d, fused):
     ref = None
     extra = [5 | 1 << 5, 6 | 1 << 5 | fused << 3] if with3d else []
-    for k in list(range(8)) + [5 | fused << 3, 2 | fused << 3] + extra:
+    widths = [5 | 1 << 8, 4 | 2 << 8, 7 | 2 << 8]  # face builder at 512 / 256 threads
+    for k in list(range(8)) + [5 | fused << 3, 2 | fused << 3] + extra + widths:
         print(f"schedule {k} (3D disk: {with3d})", flush=True)
         b, n = batch(with3d)
         b.set_schedule(k)
