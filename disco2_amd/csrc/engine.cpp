@@ -205,6 +205,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   size_t nq = (size_t)NQ * Ntot;
   good &= dalloc(&d_prim[0], nq, "prim") && dalloc(&d_prim[1], nq, "prim") && dalloc(&d_RKcons, nq, "RKcons");
   good &= dalloc(&d_grad, nq, "grad") && dalloc(&d_gradp, nq, "gradp");
+  if (n_zpairs) good &= dalloc(&d_grad_r, nq, "grad_r");  // both slope directions in one pass
   good &= dalloc(&d_tiph[0], Ntot, "tiph") && dalloc(&d_tiph[1], Ntot, "tiph");
   good &= dalloc(&d_RKtiph, Ntot, "RKtiph") && dalloc(&d_dphi[0], Ntot, "dphi") && dalloc(&d_dphi[1], Ntot, "dphi");
   good &= dalloc(&d_wiph, (size_t)nann, "wiph");
@@ -341,7 +342,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
 Engine::~Engine() {
   if (gexec) hipGraphExecDestroy(gexec);
   if (graph) hipGraphDestroy(graph);
-  void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp,
+  void *ptrs[] = {d_dp, d_sc, d_ann, d_pairs, d_pst, d_rf, d_zf, d_prim[0], d_prim[1], d_RKcons, d_grad, d_gradp, d_grad_r,
                   d_tiph[0], d_tiph[1], d_RKtiph, d_dphi[0], d_dphi[1], d_wiph, d_fL, d_fR, d_fdphi, d_fcm, d_fo_r,
                   d_fo_z, d_run, d_Fface, d_err, d_tiles, d_tiles_zb[1], d_tiles_zb[2], d_tiles_zb[3],
                   d_tiles_zb[4], d_tiles_zb[5], d_tiles_zb[6], d_tiles_zb[7], d_dt_tile, d_mdot_tile, d_ann_rot, d_init, d_pair_perm, adj_mem, diag_mem,
@@ -367,6 +368,7 @@ DevMesh Engine::mesh() const {
   m.prim_new = d_prim[1 - cur];
   m.RKcons = d_RKcons;
   m.grad = d_grad;
+  m.grad_r = d_grad_r;
   m.gradp = d_gradp;
   m.tiph = d_tiph[cur];
   m.tiph_new = d_tiph[1 - cur];

@@ -38,7 +38,8 @@ struct Engine {
   PairInfo *d_pairs = nullptr;
   PairState *d_pst = nullptr;
   double *d_rf = nullptr, *d_zf = nullptr;
-  double *d_prim[2] = {nullptr, nullptr}, *d_RKcons = nullptr, *d_grad = nullptr, *d_gradp = nullptr;
+  double *d_prim[2] = {nullptr, nullptr}, *d_RKcons = nullptr, *d_grad = nullptr, *d_gradp = nullptr,
+         *d_grad_r = nullptr;
   double *d_tiph[2] = {nullptr, nullptr};
   double *d_RKtiph = nullptr, *d_dphi[2] = {nullptr, nullptr}, *d_wiph = nullptr;
   int *d_fL = nullptr, *d_fR = nullptr;

@@ -12,6 +12,7 @@ struct DevMesh {
   PairState *pst;
   const double *rf, *zf;  // face-position pools (per-disk offsets geo_r / geo_z)
   double *prim, *RKcons, *grad, *gradp;
+  double *grad_r;         // 3D: the r slopes, beside the z slopes in grad (null in 2D)
   double *prim_new;       // primitives after the substep (k_update writes them)
   const double *tiph;     // cell edges at the start of the substep
   double *tiph_new;       // cell edges after the substep

@@ -486,14 +486,13 @@ __device__ __forceinline__ void own_face_slope(const DevMesh &m, double phi_o, c
     S[q] = (WR - WL) / dsum;
   }
 }
+// The r (dir 0) or z (dir 1) slope of cell j (zone z) of annulus A into
+// out[q * Ntot + z], from the cell's primitives w, phi slopes gp, centre angle
+// phi_o and width dp; s_S: the thread's LDS column of cached face slopes.
 template <int NQ>
-__global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, int dir) {
-  const int2 tl = tiles[tile_of(m)];
-  const AnnInfo A = m.ann[tl.x];
-  const int j = tl.y + threadIdx.x;
-  if (j >= A.N) return;
-  const DiskParams &P = disk_of(m, A.disk);
-  const long long z = A.zbase + j;
+__device__ __forceinline__ void plm_cell_dir(const DevMesh &m, const AnnInfo &A, const DiskParams &P, int j,
+                                             long long z, int dir, const double *w, const double *gp, double phi_o,
+                                             double dp, double (*s_S)[NQ][TILE], double *out) {
   const int k = A.k, i = A.i;
   double g[NQ];
 #pragma unroll
@@ -546,16 +545,7 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
     pp[2] = PI.deltaPerp;
     ds[2] = PI.deltaR + PI.deltaL;
   }
-  const double dp = m.dphi[z];
-  const double phi_o = clean_pi(m.tiph[z]) - .5 * dp;
-  double w[NQ], gp[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    w[q] = m.prim[q * m.Ntot + z];
-    gp[q] = m.gradp[q * m.Ntot + z];
-  }
   constexpr int NC = PlmCache<NQ>::NC;
-  __shared__ double s_S[NC][NQ][TILE];
   const int t = threadIdx.x;
   int n = 0;
   // the slope sum in face order; two faces per trip, so that the second
@@ -625,7 +615,51 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
       }
     }
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) m.grad[q * m.Ntot + z] = g[q];
+  for (int q = 0; q < NQ; ++q) out[q * m.Ntot + z] = g[q];
+}
+
+// the cell's primitives, phi slopes, centre angle and width
+template <int NQ>
+__device__ __forceinline__ void plm_own(const DevMesh &m, long long z, double *w, double *gp, double *phi_o,
+                                        double *dp) {
+  *dp = m.dphi[z];
+  *phi_o = clean_pi(m.tiph[z]) - .5 * *dp;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    w[q] = m.prim[q * m.Ntot + z];
+    gp[q] = m.gradp[q * m.Ntot + z];
+  }
+}
+
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, int dir) {
+  const int2 tl = tiles[tile_of(m)];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const long long z = A.zbase + j;
+  double w[NQ], gp[NQ], phi_o, dp;
+  plm_own<NQ>(m, z, w, gp, &phi_o, &dp);
+  __shared__ double s_S[PlmCache<NQ>::NC][NQ][TILE];
+  plm_cell_dir<NQ>(m, A, disk_of(m, A.disk), j, z, dir, w, gp, phi_o, dp, s_S, m.grad);
+}
+
+// Both slopes of a 3D disk's cells in one pass (the cell's own state loaded
+// once): the r slopes into grad_r, then the z slopes into m.grad, which then
+// holds what the reference's grad holds after its z pass.
+template <int NQ>
+__global__ __launch_bounds__(TILE) void k_plm_rz2(DevMesh m, const int2 *tiles, double *grad_r) {
+  const int2 tl = tiles[tile_of(m)];
+  const AnnInfo A = m.ann[tl.x];
+  const int j = tl.y + threadIdx.x;
+  if (j >= A.N) return;
+  const long long z = A.zbase + j;
+  const DiskParams &P = disk_of(m, A.disk);
+  double w[NQ], gp[NQ], phi_o, dp;
+  plm_own<NQ>(m, z, w, gp, &phi_o, &dp);
+  __shared__ double s_S[PlmCache<NQ>::NC][NQ][TILE];
+  plm_cell_dir<NQ>(m, A, P, j, z, 0, w, gp, phi_o, dp, s_S, grad_r);
+  plm_cell_dir<NQ>(m, A, P, j, z, 1, w, gp, phi_o, dp, s_S, m.grad);
 }
 
 // ----------------------------------------------------------------------------
@@ -1531,13 +1565,22 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
   launch_faces_q<NQ>(m, L, st, 0);
   launch_faces_q<NQ>(m, L, st, 1);
   if (recompute_w) KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
-  KLAUNCH(st, "k_plm_r", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0));
+  // 3D: both slope directions in one pass, the r slopes kept apart for the r fluxes
+  const bool both = L.n_zpairs && m.grad_r;
+  if (both)
+    KLAUNCH(st, "k_plm_rz", hipLaunchKernelGGL(k_plm_rz2<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, m.grad_r));
+  else
+    KLAUNCH(st, "k_plm_r", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 0));
   // the fused schedule evaluates the r faces inside k_update, which then
   // needs the r slopes: only without z faces (the z slopes overwrite them)
   const int fuse = (L.n_zpairs ? 0 : (L.fuse & FUSE_R)) | (L.fuse & FUSE_Z);
-  if (L.n_rftiles && !(fuse & FUSE_R)) launch_flux_q<NQ, FEAT>(m, L.rftiles, L.n_rftiles, st, fac, "k_flux_r");
+  if (L.n_rftiles && !(fuse & FUSE_R)) {
+    DevMesh mr = m;
+    if (both) mr.grad = m.grad_r;
+    launch_flux_q<NQ, FEAT>(mr, L.rftiles, L.n_rftiles, st, fac, "k_flux_r");
+  }
   if (L.n_zpairs) {
-    KLAUNCH(st, "k_plm_z", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 1));
+    if (!both) KLAUNCH(st, "k_plm_z", hipLaunchKernelGGL(k_plm_rz<NQ>, dim3(L.n_ztiles), B, 0, st, m, L.ztiles, 1));
     if (!(fuse & FUSE_Z)) launch_flux_q<NQ, FEAT>(m, L.zftiles, L.n_zftiles, st, fac, "k_flux_z");
   }
   launch_update_q<NQ, FEAT>(m, L, st, fac, copy_first, fuse);
