@@ -489,58 +489,84 @@ __device__ __forceinline__ void own_face_slope(const DevMesh &m, double phi_o, c
 // The r (dir 0) or z (dir 1) slope of cell j (zone z) of annulus A into
 // out[q * Ntot + z], from the cell's primitives w, phi slopes gp, centre angle
 // phi_o and width dp; s_S: the thread's LDS column of cached face slopes.
+// The face ranges of cell j (zone z) of an annulus of N cells: as R of pair
+// pR, up to two runs (the second is the wrap), into b[0..1], e[0..1]; as L
+// of pair pL, one range, into b[2], e[2] (faces_as_R / faces_as_L order).
+// Absent pairs (-1) leave empty ranges.
+__device__ __forceinline__ void cell_face_ranges(const DevMesh &m, int pR, int pL, const int *fo, long long z, int j,
+                                                 int N, int *b, int *e) {
+  b[0] = b[1] = b[2] = 0;
+  e[0] = e[1] = e[2] = 0;
+  if (pR >= 0) {
+    const PairInfo PI = m.pairs[pR];
+    const PairState S = m.pst[pR];
+    int tt = j - S.jpmin;
+    if (tt < 0) tt += N;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      if (tt <= S.T) {
+        b[u] = m.run_start[PI.run_base + tt];
+        e[u] = m.run_start[PI.run_base + tt + 1];
+      }
+      tt += N;
+    }
+  }
+  if (pL >= 0) {
+    const PairState S = m.pst[pL];
+    int f1 = (j + 1 < N) ? fo[z + 1] : S.count;
+    if (f1 > S.count) f1 = S.count;
+    int f0 = fo[z];
+    if (f0 < 0) f0 = 0;
+    b[2] = f0;
+    e[2] = f1;
+  }
+}
+
+// the pairs below (pR) and above (pL) of the cell in direction dir (-1: none)
+__device__ __forceinline__ void plm_pairs(const AnnInfo &A, const DiskParams &P, int dir, int *pR, int *pL) {
+  const int k = A.k, i = A.i;
+  *pR = *pL = -1;
+  if (dir == 0) {
+    if (i >= 1) *pR = rpair_of(P, k, i - 1);
+    if (i <= P.NR - 2) *pL = rpair_of(P, k, i);
+  } else {
+    if (k >= 1) *pR = zpair_of(P, k - 1, i);
+    if (k <= P.NZ - 2) *pL = zpair_of(P, k, i);
+  }
+}
+
+// b, e: the cell's face ranges in direction dir (cell_face_ranges), when the
+// caller has loaded them already
 template <int NQ>
 __device__ __forceinline__ void plm_cell_dir(const DevMesh &m, const AnnInfo &A, const DiskParams &P, int j,
                                              long long z, int dir, const double *w, const double *gp, double phi_o,
-                                             double dp, double (*s_S)[NQ][TILE], double *out) {
-  const int k = A.k, i = A.i;
+                                             double dp, double (*s_S)[NQ][TILE], double *out, const int *b,
+                                             const int *e) {
+  const int i = A.i;
   double g[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) g[q] = 0.0;
-  int pR = -1, pLf = -1;
-  const int *fo = dir == 0 ? m.fo_r : m.fo_z;
-  if (dir == 0) {
-    if (i >= 1) pR = rpair_of(P, k, i - 1);
-    if (i <= P.NR - 2) pLf = rpair_of(P, k, i);
-  } else {
-    if (k >= 1) pR = zpair_of(P, k - 1, i);
-    if (k <= P.NZ - 2) pLf = zpair_of(P, k, i);
-  }
+  int pR, pLf;
+  plm_pairs(A, P, dir, &pR, &pLf);
   // The cell's faces in the reference's order: as R, its runs in the pair
   // below (faces_as_R: at most two, the second is the wrap), then as L one
   // range in the pair above (faces_as_L); range u is faces [fb[u] + b[u],
   // fb[u] + e[u]) with its pair's r, deltaPerp and deltaL + deltaR.
   long long fb[3] = {0, 0, 0};
-  int b[3] = {0, 0, 0}, e[3] = {0, 0, 0};
   double pr[3] = {0, 0, 0}, pp[3] = {0, 0, 0}, ds[3] = {1, 1, 1};
   if (pR >= 0) {
     const PairInfo PI = m.pairs[pR];
-    const PairState S = m.pst[pR];
-    int t = j - S.jpmin;
-    if (t < 0) t += A.N;
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      if (t <= S.T) {
-        fb[u] = PI.face_base;
-        b[u] = m.run_start[PI.run_base + t];
-        e[u] = m.run_start[PI.run_base + t + 1];
-        pr[u] = PI.r;
-        pp[u] = PI.deltaPerp;
-        ds[u] = PI.deltaR + PI.deltaL;
-      }
-      t += A.N;
+      fb[u] = PI.face_base;
+      pr[u] = PI.r;
+      pp[u] = PI.deltaPerp;
+      ds[u] = PI.deltaR + PI.deltaL;
     }
   }
   if (pLf >= 0) {
     const PairInfo PI = m.pairs[pLf];
-    const PairState S = m.pst[pLf];
-    int f1 = (j + 1 < A.N) ? fo[z + 1] : S.count;
-    if (f1 > S.count) f1 = S.count;
-    int f0 = fo[z];
-    if (f0 < 0) f0 = 0;
     fb[2] = PI.face_base;
-    b[2] = f0;
-    e[2] = f1;
     pr[2] = PI.r;
     pp[2] = PI.deltaPerp;
     ds[2] = PI.deltaR + PI.deltaL;
@@ -641,7 +667,11 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
   double w[NQ], gp[NQ], phi_o, dp;
   plm_own<NQ>(m, z, w, gp, &phi_o, &dp);
   __shared__ double s_S[PlmCache<NQ>::NC][NQ][TILE];
-  plm_cell_dir<NQ>(m, A, disk_of(m, A.disk), j, z, dir, w, gp, phi_o, dp, s_S, m.grad);
+  const DiskParams &P = disk_of(m, A.disk);
+  int pR, pL, b[3], e[3];
+  plm_pairs(A, P, dir, &pR, &pL);
+  cell_face_ranges(m, pR, pL, dir == 0 ? m.fo_r : m.fo_z, z, j, A.N, b, e);
+  plm_cell_dir<NQ>(m, A, P, j, z, dir, w, gp, phi_o, dp, s_S, m.grad, b, e);
 }
 
 // Both slopes of a 3D disk's cells in one pass (the cell's own state loaded
@@ -655,11 +685,18 @@ __global__ __launch_bounds__(TILE) void k_plm_rz2(DevMesh m, const int2 *tiles, 
   if (j >= A.N) return;
   const long long z = A.zbase + j;
   const DiskParams &P = disk_of(m, A.disk);
+  // (loading the z ranges before the r slope, to overlap their latency,
+  // measured 2% slower: more registers held through the r pass)
   double w[NQ], gp[NQ], phi_o, dp;
   plm_own<NQ>(m, z, w, gp, &phi_o, &dp);
   __shared__ double s_S[PlmCache<NQ>::NC][NQ][TILE];
-  plm_cell_dir<NQ>(m, A, P, j, z, 0, w, gp, phi_o, dp, s_S, grad_r);
-  plm_cell_dir<NQ>(m, A, P, j, z, 1, w, gp, phi_o, dp, s_S, m.grad);
+#pragma unroll
+  for (int dir = 0; dir < 2; ++dir) {
+    int pR, pL, b[3], e[3];
+    plm_pairs(A, P, dir, &pR, &pL);
+    cell_face_ranges(m, pR, pL, dir == 0 ? m.fo_r : m.fo_z, z, j, A.N, b, e);
+    plm_cell_dir<NQ>(m, A, P, j, z, dir, w, gp, phi_o, dp, s_S, dir == 0 ? grad_r : m.grad, b, e);
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -842,38 +879,6 @@ __device__ __forceinline__ double new_tiph(double traw, double rkt_in, bool copy
   return t;
 }
 
-// The face ranges of cell j (zone z) of an annulus of N cells: as R of pair
-// pR, up to two runs (the second is the wrap), into b[0..1], e[0..1]; as L
-// of pair pL, one range, into b[2], e[2] (faces_as_R / faces_as_L order).
-// Absent pairs (-1) leave empty ranges.
-__device__ __forceinline__ void cell_face_ranges(const DevMesh &m, int pR, int pL, const int *fo, long long z, int j,
-                                                 int N, int *b, int *e) {
-  b[0] = b[1] = b[2] = 0;
-  e[0] = e[1] = e[2] = 0;
-  if (pR >= 0) {
-    const PairInfo PI = m.pairs[pR];
-    const PairState S = m.pst[pR];
-    int tt = j - S.jpmin;
-    if (tt < 0) tt += N;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (tt <= S.T) {
-        b[u] = m.run_start[PI.run_base + tt];
-        e[u] = m.run_start[PI.run_base + tt + 1];
-      }
-      tt += N;
-    }
-  }
-  if (pL >= 0) {
-    const PairState S = m.pst[pL];
-    int f1 = (j + 1 < N) ? fo[z + 1] : S.count;
-    if (f1 > S.count) f1 = S.count;
-    int f0 = fo[z];
-    if (f0 < 0) f0 = 0;
-    b[2] = f0;
-    e[2] = f1;
-  }
-}
 
 // add the stored increments of faces [b, e) of one range in face order, as
 // add_face_R (R) or add_face_L; two faces' loads are issued before their adds
