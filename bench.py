@@ -80,11 +80,15 @@ def kernel_touched_bytes(name, nz, nf_r, nf_z, nq, visc):
         # per face: cell indices, centre phi, overlap length in, increments out;
         # per zone: prim, grad, gradp, tiph, dphi
         return nf_dir * (24 + 8 + d * fx) + nz * d * (3 * nq + 2)
-    if name.startswith("k_plm_r") or name.startswith("k_plm_z"):
+    if name in ("k_plm_r", "k_plm_z"):
         # face slopes evaluated in place from the faces' two cells: per face cell
         # indices, centre phi, overlap length; per zone prim, gradp, tiph, dphi,
         # cursors in, grad out
         return nf_dir * (16 + 2 * d) + nz * (d * (2 * nq + 2) + 8) + nz * d * nq
+    if name == "k_plm_rz":
+        # both directions in one pass: the r and z faces, the zone inputs once,
+        # two cursor pairs, the r and z slopes out
+        return (nf_r + nf_z) * (16 + 2 * d) + nz * (d * (2 * nq + 2) + 16) + 2 * nz * d * nq
     if name == "k_plm_p":
         return nz * d * (nq + 1) + nz * d * nq
     if name == "k_plm_p_dt":
