@@ -286,56 +286,79 @@ struct ScalarAdj {
   double dt, alpha, M[2], xbh[2], ybh[2], loss;
 };
 
-// nu of the alpha viscosity at (r, angle with cos ct / sin st), given P/rho
-// (Riemann_routines.c:259-263, cell_mindt.c:84) and its VJP.
-DHD __forceinline__ double visc_nu(const DiskParams &P, const DiskScalars &sc, double r, double ct, double st,
-                                   double Pp, double rho, double *Qout) {
-  if (P.visc_const == 1) return P.visc;
+// nu of the alpha viscosity at (r, angle), given P/rho
+// (Riemann_routines.c:259-263, cell_mindt.c:84), with what its VJP reuses:
+// the angle's cos / sin, the point (x, y), each mass's offset, s = d^2 +
+// eps^2 and s^-1.5, Q = sum M s^-1.5 and sqrt(Q).  The same operations as
+// the forward's, so nu has its bits.
+struct ViscAt {
+  double ct, st, x, y, nu, Q, sq;
+  double dx[2], dy[2], s[2], pm[2];
+};
+DHD __forceinline__ void visc_at_cs(const DiskParams &P, const DiskScalars &sc, double r, double ct, double st,
+                                    double Pp, double rho, ViscAt &V) {
+  V.ct = ct;
+  V.st = st;
+  if (P.visc_const == 1) {
+    V.nu = P.visc;
+    return;
+  }
   const double eps = P.g_eps;
-  const double d0 = d_grav_dist(sc.xbh[0], sc.ybh[0], r, ct, st, 0.);
-  const double d1 = d_grav_dist(sc.xbh[1], sc.ybh[1], r, ct, st, 0.);
-  const double Q = pow_m1p5(d0 * d0 + eps * eps) * sc.gm[0].M + pow_m1p5(d1 * d1 + eps * eps) * sc.gm[1].M;
-  if (Qout) *Qout = Q;
-  return P.visc * Pp / rho / sqrt(Q);
+  V.x = r * ct;
+  V.y = r * st;
+  for (int p = 0; p < 2; ++p) {
+    V.dx[p] = V.x - sc.xbh[p];
+    V.dy[p] = V.y - sc.ybh[p];
+    const double d = sqrt(V.dx[p] * V.dx[p] + V.dy[p] * V.dy[p]);
+    V.s[p] = d * d + eps * eps;
+    V.pm[p] = pow_m1p5(V.s[p]);
+  }
+  V.Q = V.pm[0] * sc.gm[0].M + V.pm[1] * sc.gm[1].M;
+  V.sq = sqrt(V.Q);
+  V.nu = P.visc * Pp / rho / V.sq;
 }
-// angb (optional): adjoint of the angle whose cos / sin are ct / st
-DHD __forceinline__ void vjp_visc_nu(const DiskParams &P, const DiskScalars &sc, double r, double ct, double st,
-                                     double Pp, double rho, double nub, double *Ppb, double *rhob, ScalarAdj &sb,
+DHD __forceinline__ void visc_at(const DiskParams &P, const DiskScalars &sc, double r, double ang, double Pp,
+                                 double rho, ViscAt &V) {
+  double st, ct;
+  sincos(ang, &st, &ct);
+  visc_at_cs(P, sc, r, ct, st, Pp, rho, V);
+}
+DHD __forceinline__ double visc_nu(const DiskParams &P, const DiskScalars &sc, double r, double ct, double st,
+                                   double Pp, double rho) {
+  ViscAt V;
+  visc_at_cs(P, sc, r, ct, st, Pp, rho, V);
+  return V.nu;
+}
+// VJP of nu at V; angb (optional): adjoint of the angle whose cos / sin are V.ct / V.st
+DHD __forceinline__ void vjp_visc_nu(const DiskParams &P, const DiskScalars &sc, const ViscAt &V, double Pp,
+                                     double rho, double nub, double *Ppb, double *rhob, ScalarAdj &sb,
                                      double *angb = nullptr) {
   if (P.visc_const == 1) {
     sb.alpha += nub;
     return;
   }
-  const double eps = P.g_eps;
-  double Q;
-  const double nu = visc_nu(P, sc, r, ct, st, Pp, rho, &Q);
-  const double sq = sqrt(Q);
-  sb.alpha += Pp / rho / sq * nub;
-  *Ppb += P.visc / (rho * sq) * nub;
-  *rhob += -P.visc * Pp / (rho * rho * sq) * nub;
-  const double Qb = -nu / (2. * Q) * nub;
-  const double x = r * ct, y = r * st;
+  sb.alpha += Pp / rho / V.sq * nub;
+  *Ppb += P.visc / (rho * V.sq) * nub;
+  *rhob += -P.visc * Pp / (rho * rho * V.sq) * nub;
+  const double Qb = -V.nu / (2. * V.Q) * nub;
   for (int p = 0; p < 2; ++p) {
-    const double dx = x - sc.xbh[p], dy = y - sc.ybh[p];
-    const double d = sqrt(dx * dx + dy * dy);
-    const double s = d * d + eps * eps;
-    sb.M[p] += pow_m1p5(s) * Qb;
-    const double sbar = sc.gm[p].M * (-1.5) * pow_m1p5(s) / s * Qb;
+    const double dx = V.dx[p], dy = V.dy[p], s = V.s[p];
+    sb.M[p] += V.pm[p] * Qb;
+    const double sbar = sc.gm[p].M * (-1.5) * V.pm[p] / s * Qb;
     // s = d^2 + eps^2 with d^2 = dx^2 + dy^2, (x, y) = r (cos, sin)(angle)
     sb.xbh[p] += -2. * dx * sbar;
     sb.ybh[p] += -2. * dy * sbar;
-    if (angb) *angb += 2. * (dx * (-y) + dy * x) * sbar;
+    if (angb) *angb += 2. * (dx * (-V.y) + dy * V.x) * sbar;
   }
 }
 
 // The three viscous flux components (Riemann_routines.c:286-301) from the
 // face averages Avg[5], the r and phi gradients of (v_r, omega) Grr[2],
-// Gph[2], for an r (n0) or phi (n1) face; nu at the face centre cm.
-DHD __forceinline__ void visc_flux_v(const DiskParams &P, const DiskScalars &sc, int n0, double r, double cm,
-                                     double rdr, const double *Avg, const double *Grr, const double *Gph, double *v) {
-  double st, ct;
-  sincos(cm, &st, &ct);
-  const double nu = visc_nu(P, sc, r, ct, st, Avg[PPP], Avg[RHO], nullptr);
+// Gph[2], for an r (n0) or phi (n1) face; V: nu at the face centre
+// (visc_at with Avg's P and rho).
+DHD __forceinline__ void visc_flux_v(const ViscAt &V, int n0, double r, double rdr, const double *Avg,
+                                     const double *Grr, const double *Gph, double *v) {
+  const double nu = V.nu;
   const double rho = Avg[RHO], vr = Avg[URR], om = Avg[UPP];
   const double K = -nu * rho;
   const double A1 = r * Grr[0] - vr - r * r * Gph[1];
@@ -352,13 +375,11 @@ DHD __forceinline__ void visc_flux_v(const DiskParams &P, const DiskScalars &sc,
     v[2] = K * (-r * om * B1 + vr * B2);
   }
 }
-DHD __forceinline__ void vjp_visc_flux(const DiskParams &P, const DiskScalars &sc, int n0, double r, double cm,
+DHD __forceinline__ void vjp_visc_flux(const DiskParams &P, const DiskScalars &sc, const ViscAt &V, int n0, double r,
                                        double rdr, const double *Avg, const double *Grr, const double *Gph,
                                        const double *vb, double *Avgb, double *Grrb, double *Gphb, ScalarAdj &sb,
                                        double *cmb) {
-  double st, ct;
-  sincos(cm, &st, &ct);
-  const double nu = visc_nu(P, sc, r, ct, st, Avg[PPP], Avg[RHO], nullptr);
+  const double nu = V.nu;
   const double rho = Avg[RHO], vr = Avg[URR], om = Avg[UPP];
   const double K = -nu * rho;
   const double A1 = r * Grr[0] - vr - r * r * Gph[1];
@@ -396,7 +417,7 @@ DHD __forceinline__ void vjp_visc_flux(const DiskParams &P, const DiskScalars &s
   Avgb[UPP] += omb;
   Avgb[RHO] += -nu * Kb;
   const double nub = -rho * Kb;
-  vjp_visc_nu(P, sc, r, ct, st, Avg[PPP], Avg[RHO], nub, &Avgb[PPP], &Avgb[RHO], sb, cmb);
+  vjp_visc_nu(P, sc, V, Avg[PPP], Avg[RHO], nub, &Avgb[PPP], &Avgb[RHO], sb, cmb);
 }
 
 // Gravity of one mass at a cell, projected on r and phi (cell_source.c:49-71).
@@ -454,12 +475,14 @@ DHD __forceinline__ void vjp_face_flux(const DiskParams &P, const DiskScalars &s
   for (int q = 0; q < 5; ++q) sdt += F[q] * Xb[q];
   if (P.visc > 0.0 && dir != 2) {
     double v[3], vb[3];
-    visc_flux_v(P, sc, dir == 0, r, cm, rdr, Avg, Grr, Gph, v);
+    ViscAt V;
+    visc_at(P, sc, r, cm, Avg[PPP], Avg[RHO], V);
+    visc_flux_v(V, dir == 0, r, rdr, Avg, Grr, Gph, v);
     for (int k = 0; k < 3; ++k) {
       vb[k] = dtdA * Yb[k];
       sdt += v[k] * Yb[k];
     }
-    vjp_visc_flux(P, sc, dir == 0, r, cm, rdr, Avg, Grr, Gph, vb, Avgb, Grrb, Gphb, sb, cmb);
+    vjp_visc_flux(P, sc, V, dir == 0, r, rdr, Avg, Grr, Gph, vb, Avgb, Grrb, Gphb, sb, cmb);
   }
   sb.dt += dA * sdt;
   if (dAb) *dAb += dt * sdt;
@@ -522,15 +545,15 @@ DHD __forceinline__ void vjp_cell_dt(const DiskParams &P, const DiskScalars &sc,
   double d0b = db;
   double Ppb = 0.0, rhob = 0.0;
   if (P.visc > 0.0) {
-    double ct, st;
-    sincos(tc, &st, &ct);
-    const double nu = visc_nu(P, sc, r, ct, st, Pp, rho, nullptr);
+    ViscAt V;
+    visc_at(P, sc, r, tc, Pp, rho, V);
+    const double nu = V.nu;
     const double tv = .25 * dx * dx / nu;
     const double den = 1. + d0 / tv;
     d0b = db / (den * den);
     const double tvb = db * d0 * d0 / (tv * tv * den * den);
     const double nub = -tvb * tv / nu;
-    vjp_visc_nu(P, sc, r, ct, st, Pp, rho, nub, &Ppb, &rhob, sb, tcb);
+    vjp_visc_nu(P, sc, V, Pp, rho, nub, &Ppb, &rhob, sb, tcb);
   }
   const double ab = -d0 / a * d0b;
   if (c > 0.0) {
@@ -686,7 +709,9 @@ DHD __forceinline__ void vjp_rface_visc(const DiskParams &P, const DiskScalars &
     Gph[k] = .5 * (c.gpL[URR + k] + c.gpR[URR + k]) / g.r;
   }
   double vf[3], vb[3], Avgb[5] = {0, 0, 0, 0, 0}, Grrb[2] = {0, 0};
-  visc_flux_v(P, sc, 1, g.r, g.cm, g.rdr, Avg, Grr, Gph, vf);
+  ViscAt V;
+  visc_at(P, sc, g.r, g.cm, Avg[PPP], Avg[RHO], V);
+  visc_flux_v(V, 1, g.r, g.rdr, Avg, Grr, Gph, vf);
   double sdt = 0.0;
   for (int k = 0; k < 3; ++k) {
     vb[k] = dt * g.dA * Yb[k];
@@ -696,7 +721,7 @@ DHD __forceinline__ void vjp_rface_visc(const DiskParams &P, const DiskScalars &
   v.dAb = dt * sdt;
   v.cmb = 0.0;
   v.Gphb[0] = v.Gphb[1] = 0.0;
-  vjp_visc_flux(P, sc, 1, g.r, g.cm, g.rdr, Avg, Grr, Gph, vb, Avgb, Grrb, v.Gphb, sb, &v.cmb);
+  vjp_visc_flux(P, sc, V, 1, g.r, g.rdr, Avg, Grr, Gph, vb, Avgb, Grrb, v.Gphb, sb, &v.cmb);
   for (int q = 0; q < 5; ++q) v.WLb[q] = v.WRb[q] = .5 * Avgb[q];
   for (int k = 0; k < 2; ++k) {
     v.WLb[URR + k] -= Grrb[k] / dsum;
@@ -814,7 +839,9 @@ DHD __forceinline__ void vjp_phiface_visc(const DiskParams &P, const DiskScalars
     Gph[k] = (c.pR[URR + k] - c.pL[URR + k]) / (g.tR - g.tL) / g.r;
     v.Grrb[k] = v.Gphb[k] = 0.0;
   }
-  visc_flux_v(P, sc, 0, g.r, g.tL, g.rdr, Avg, Grr, Gph, vf);
+  ViscAt V;
+  visc_at(P, sc, g.r, g.tL, Avg[PPP], Avg[RHO], V);
+  visc_flux_v(V, 0, g.r, g.rdr, Avg, Grr, Gph, vf);
   double sdt = 0.0;
   for (int k = 0; k < 3; ++k) {
     vb[k] = dt * g.dA * Yb[k];
@@ -822,7 +849,7 @@ DHD __forceinline__ void vjp_phiface_visc(const DiskParams &P, const DiskScalars
   }
   sb.dt += g.dA * sdt;
   v.tLb = 0.0;
-  vjp_visc_flux(P, sc, 0, g.r, g.tL, g.rdr, Avg, Grr, Gph, vb, v.Avgb, v.Grrb, v.Gphb, sb, &v.tLb);
+  vjp_visc_flux(P, sc, V, 0, g.r, g.rdr, Avg, Grr, Gph, vb, v.Avgb, v.Grrb, v.Gphb, sb, &v.tLb);
 }
 
 // *tLb: adjoint of the face's angle (the viscosity is evaluated at cm = tL);

@@ -30,12 +30,14 @@ __device__ __forceinline__ void add5(double *a, long long N, long long c, const 
     if (v[q] != 0.0) atomicAdd(&a[q * N + c], v[q]);
 }
 
-// workgroup sum of the scalar adjoints into slot `slot` of the partial sums
-// (every thread calls; row k of a.part holds field k of every slot)
+// workgroup sum of nine values per thread; thread k < 9 stores sum k at
+// dst[k * stride] (every thread calls)
 #define SA_FIELDS 9
-__device__ void reduce_store(const ScalarAdj &s, const AdjMesh &a, int slot) {
+__device__ void block_sum9(const double *v0, double *dst, long long stride) {
   __shared__ double sh[SA_FIELDS][1024 / 64];
-  double v[SA_FIELDS] = {s.dt, s.alpha, s.M[0], s.M[1], s.xbh[0], s.xbh[1], s.ybh[0], s.ybh[1], s.loss};
+  double v[SA_FIELDS];
+#pragma unroll
+  for (int k = 0; k < SA_FIELDS; ++k) v[k] = v0[k];
 #pragma unroll
   for (int k = 0; k < SA_FIELDS; ++k)
     for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
@@ -47,8 +49,14 @@ __device__ void reduce_store(const ScalarAdj &s, const AdjMesh &a, int slot) {
   if (threadIdx.x < SA_FIELDS) {
     double t = 0.0;
     for (int i = 0; i < nw; ++i) t += sh[threadIdx.x][i];
-    a.part[threadIdx.x * a.npart + slot] = t;
+    dst[threadIdx.x * stride] = t;
   }
+}
+// workgroup sum of the scalar adjoints into slot `slot` of the partial sums
+// (every thread calls; row k of a.part holds field k of every slot)
+__device__ void reduce_store(const ScalarAdj &s, const AdjMesh &a, int slot) {
+  const double v[SA_FIELDS] = {s.dt, s.alpha, s.M[0], s.M[1], s.xbh[0], s.xbh[1], s.ybh[0], s.ybh[1], s.loss};
+  block_sum9(v, a.part + slot, a.npart);
 }
 
 __device__ __forceinline__ void load_pair(const DevMesh &m, long long cL, long long cR, FacePair &c) {
@@ -88,8 +96,29 @@ __device__ __forceinline__ void face_geo_to_rot(const DevMesh &m, const AdjMesh 
 // Outflow remap backward (k_bc_r): a ghost cell's adjoint goes to the cells
 // it was averaged from, weighted as the forward weighted their primitives.
 // The mesh view has the substep's faces and the new dphi.
-__global__ __launch_bounds__(TILE) void ka_bc(DevMesh m, AdjMesh a, const int2 *tiles) {
-  const int2 tl = tiles[xcd_order()];
+__device__ void rot_annulus(const DevMesh &m, const double *src, int ann, double *out);
+__device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, const unsigned long long *dtp,
+                          int *pick);
+// workgroups past the nmain tiles: one per annulus of the rotation sums,
+// then (x.pick) one for the step's CFL cell
+__global__ __launch_bounds__(TILE) void ka_bc(DevMesh m, AdjMesh a, const int2 *tiles, int nmain, int nrot,
+                                              const int2 *itiles, int nitiles, AdjExtra x) {
+  const int b = xcd_order();
+  if (b >= nmain + nrot) {
+    DevMesh mp = m;  // the step-start state and the scalars its CFL minimum saw
+    mp.prim = (double *)x.pick_prim;
+    mp.tiph = x.pick_tiph;
+    mp.dphi = (double *)x.pick_dphi;
+    mp.sc = (DiskScalars *)x.pick_sc;
+    mp.dt_tile = (double *)x.pick_dt_tile;
+    pick_cell(mp, itiles, nitiles, x.dtbits, x.pick);
+    return;
+  }
+  if (b >= nmain) {
+    rot_annulus(m, x.rot_src, b - nmain, x.rot_out);
+    return;
+  }
+  const int2 tl = tiles[b];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
   if (j >= A.N) return;
@@ -132,7 +161,14 @@ __global__ __launch_bounds__(TILE) void ka_bc(DevMesh m, AdjMesh a, const int2 *
 // Cell update backward (k_update phase 2): cons2prim, the RK blend and the
 // RK snapshot, prim2cons, the sources and the torque term.  Writes the
 // adjoint of the cell's conserved state (au) and starts aP.
-__global__ __launch_bounds__(TILE) void ka_update(DevMesh m, AdjMesh a, const int2 *tiles) {
+// minimum waves per SIMD of the update and viscous-part kernels (1: no bound)
+#ifndef ADJ_UPDATE_WAVES
+#define ADJ_UPDATE_WAVES 1
+#endif
+#ifndef ADJ_VISC_WAVES
+#define ADJ_VISC_WAVES 1
+#endif
+__global__ __launch_bounds__(TILE, ADJ_UPDATE_WAVES) void ka_update(DevMesh m, AdjMesh a, const int2 *tiles) {
   const int ti = xcd_order();
   const int2 tl = tiles[ti];
   const AnnInfo A = m.ann[tl.x];
@@ -395,7 +431,7 @@ __global__ __launch_bounds__(TILE, ADJ_HYDRO_WAVES) void ka_faces_h(DevMesh m, A
   else
     rflux_h_tile(m, a, ftiles, b - nu, slot0 + nu);
 }
-__global__ __launch_bounds__(TILE) void ka_faces_v(DevMesh m, AdjMesh a, const int2 *utiles, int nu, const int2 *ftiles,
+__global__ __launch_bounds__(TILE, ADJ_VISC_WAVES) void ka_faces_v(DevMesh m, AdjMesh a, const int2 *utiles, int nu, const int2 *ftiles,
                                                    int slot0) {
   const int b = xcd_order();
   if (b < nu)
@@ -583,20 +619,18 @@ __global__ __launch_bounds__(TILE) void ka_plm_r(DevMesh m, AdjMesh a, const int
 }
 
 // face slope S = (W_R - W_L)/(deltaL + deltaR), W_L = p_L + dpL gp_L,
-// W_R = p_R - dpR gp_R, backward, gathered per cell over its r faces
-__global__ __launch_bounds__(TILE) void ka_slope(DevMesh m, AdjMesh a, const int2 *tiles) {
-  const int2 tl = tiles[xcd_order()];
-  const AnnInfo A = m.ann[tl.x];
-  const int j = tl.y + threadIdx.x;
-  if (j >= A.N) return;
-  const DiskParams &P = disk_of(m, A.disk);
-  const long long z = A.zbase + j;
+// W_R = p_R - dpR gp_R, backward, gathered per cell over its r faces: cell j
+// (zone z) of annulus A receives ap (its primitives), agp (its phi slopes)
+// and rot (its annulus rotation)
+__device__ __forceinline__ void slope_cell(const DevMesh &m, const AdjMesh &a, const AnnInfo &A, const DiskParams &P,
+                                           int j, long long z, double *ap, double *agp, double *rot_out) {
   const long long NF = m.NFtot;
   const int k = A.k, i = A.i;
   const int pR = i >= 1 ? rpair_of(P, k, i - 1) : -1;
   const int pLf = i <= P.NR - 2 ? rpair_of(P, k, i) : -1;
   const double tc = clean_pi(m.tiph[z]), dc = m.dphi[z];
-  double gpc[5], ap[5] = {0, 0, 0, 0, 0}, agp[5] = {0, 0, 0, 0, 0}, rot = 0.0;
+  double gpc[5], rot = 0.0;
+  for (int q = 0; q < 5; ++q) ap[q] = agp[q] = 0.0;
   load5(m.gradp, m.Ntot, z, gpc);
   // one face: Sb the slope's adjoint, W_Lb = -Sb/d, W_Rb = Sb/d; dpLb, dpRb
   // the adjoints of the two offsets, which move the face centre and edges
@@ -631,11 +665,7 @@ __global__ __launch_bounds__(TILE) void ka_slope(DevMesh m, AdjMesh a, const int
   };
   if (pR >= 0) faces_as_R(m, pR, j, A.N, [&](const PairInfo &PI, long long gi) { face(PI, gi, false); });
   if (pLf >= 0) faces_as_L(m, pLf, m.fo_r, z, j, A.N, [&](const PairInfo &PI, long long gi) { face(PI, gi, true); });
-  for (int q = 0; q < 5; ++q) {
-    a.aP[q * m.Ntot + z] += ap[q];
-    a.agradp[q * m.Ntot + z] += agp[q];
-  }
-  a.arot[z] += rot;
+  *rot_out = rot;
 }
 
 // phi slope backward (k_plm_p), gathered: cell c receives the weight on it
@@ -650,32 +680,71 @@ __device__ __forceinline__ void plm_p_cell(const DevMesh &m, const AnnInfo &A, d
   const double dpR = .5 * (m.dphi[c] + m.dphi[cR]);
   plm_p_weights(m.prim[q * m.Ntot + cL], m.prim[q * m.Ntot + c], m.prim[q * m.Ntot + cR], dpL, dpR, plm, wL, wc, wR);
 }
-__global__ __launch_bounds__(TILE) void ka_plm_p(DevMesh m, AdjMesh a, const int2 *tiles) {
-  const int2 tl = tiles[xcd_order()];
+__device__ void reduce_block(const AdjExtra &x, const AdjMesh &a, int rb, int n);
+// The face-slope backward and the phi-slope backward in one launch, over
+// tiles of TILE - 2 cells j0 .. j0+n-1.  Thread t takes cell j0 - 1 + t (mod
+// N) for t = 0 .. n+1: the face-slope gather (slope_cell) completes that
+// cell's phi-slope adjoint agp, and the cell's phi limiter (k_plm_p) hands
+// it on: wL agp to the left neighbour's primitives, wc agp to its own, wR
+// agp to the right neighbour's; the outer two wait in LDS.  After a barrier
+// each of the n cells adds its own share and its neighbours' (the two halo
+// cells, their own tiles' cells, only hand theirs on).  Each cell's limiter
+// weights are evaluated once, and the phi-slope adjoint never goes back to
+// HBM (nothing after this substep reads it).  Workgroups past the nmain
+// tiles: the reduction's blocks (whole workgroups each).
+#ifndef ADJ_SLOPE_WAVES
+#define ADJ_SLOPE_WAVES 1
+#endif
+__global__ __launch_bounds__(TILE, ADJ_SLOPE_WAVES) void ka_slope_p(DevMesh m, AdjMesh a, const int2 *tiles,
+                                                                    int nmain, int nslots, AdjExtra x) {
+  const int b = xcd_order();
+  if (b >= nmain) {
+    reduce_block(x, a, b - nmain, nslots);
+    return;
+  }
+  __shared__ double s_toL[5][TILE], s_toR[5][TILE];
+  const int2 tl = tiles[b];
   const AnnInfo A = m.ann[tl.x];
-  const int j = tl.y + threadIdx.x;
-  if (j >= A.N) return;
   const DiskParams &P = disk_of(m, A.disk);
   const int N = A.N;
-  const int jl = j == 0 ? N - 1 : j - 1, jr = j == N - 1 ? 0 : j + 1;
-  const long long c = A.zbase + j;
-  for (int q = 0; q < 5; ++q) {
-    double wL, wc, wR, s;
-    plm_p_cell(m, A, P.plm, j, q, &wL, &wc, &wR);
-    s = wc * a.agradp[q * m.Ntot + c];
-    plm_p_cell(m, A, P.plm, jl, q, &wL, &wc, &wR);  // left neighbour: its right weight
-    s += wR * a.agradp[q * m.Ntot + A.zbase + jl];
-    plm_p_cell(m, A, P.plm, jr, q, &wL, &wc, &wR);  // right neighbour: its left weight
-    s += wL * a.agradp[q * m.Ntot + A.zbase + jr];
-    a.aP[q * m.Ntot + c] += s;
+  const int n = min(TILE - 2, N - tl.y);
+  const int t = threadIdx.x;
+  int j = tl.y - 1 + t;
+  if (j < 0) j += N;
+  if (j >= N) j -= N;
+  const long long z = A.zbase + j;
+  double ap[5], own[5], rot = 0.0;
+  if (t <= n + 1) {
+    double agp[5];
+    slope_cell(m, a, A, P, j, z, ap, agp, &rot);
+    for (int q = 0; q < 5; ++q) {
+      const double g = a.agradp[q * m.Ntot + z] + agp[q];
+      double wL, wc, wR;
+      plm_p_cell(m, A, P.plm, j, q, &wL, &wc, &wR);
+      own[q] = wc * g;
+      s_toL[q][t] = wL * g;
+      s_toR[q][t] = wR * g;
+    }
   }
+  __syncthreads();
+  if (t < 1 || t > n) return;
+  for (int q = 0; q < 5; ++q) {
+    double s = own[q];
+    s += s_toR[q][t - 1];  // the left neighbour's right weight
+    s += s_toL[q][t + 1];  // the right neighbour's left weight
+    double v = a.aP[q * m.Ntot + z] + ap[q];
+    v += s;
+    a.aP[q * m.Ntot + z] = v;
+  }
+  a.arot[z] += rot;
 }
 
 // CFL step backward.  The step's dt is the minimum of the interior cells'
 // steps; on a smooth disk whole annuli can tie bit for bit, and the minimum
 // is then the step of any one of them: the first tied cell (lowest zone)
-// takes lam_dt times its gradient.  ka_dt_pick finds it, ka_chain_dt
-// applies it.  m.sc holds the scalars the forward's CFL minimum saw.
+// takes lam_dt times its gradient.  pick_cell finds it from the step's tile
+// minima (one workgroup of the step's first launch), ka_chain applies it.  m.sc holds the scalars the forward's CFL minimum
+// saw.
 __device__ __forceinline__ double cfl_cell(const DevMesh &m, const int2 tl, const AnnInfo &A, const DiskParams &P,
                                            const DiskScalars &sc, long long c, double *pr, double *r_out,
                                            double *dx_out, double *w_out, double *tc_out) {
@@ -694,77 +763,33 @@ __device__ __forceinline__ double cfl_cell(const DevMesh &m, const int2 tl, cons
   *tc_out = tc;
   return cell_dt_value(P, sc, pr, r, dx, w, tc);
 }
-__global__ __launch_bounds__(TILE) void ka_dt_pick(DevMesh m, const int2 *tiles, const unsigned long long *dtp,
-                                                   int *pick) {
-  const unsigned long long dtbits = *dtp;
-  const int2 tl = tiles[xcd_order()];
-  const AnnInfo A = m.ann[tl.x];
-  const int j = tl.y + threadIdx.x;
-  if (j >= A.N) return;
-  const long long c = A.zbase + j;
-  double pr[5], r, dx, w, tc;
-  const double d = cfl_cell(m, tl, A, disk_of(m, A.disk), m.sc[A.disk], c, pr, &r, &dx, &w, &tc);
-  if ((unsigned long long)__double_as_longlong(d) == dtbits) atomicMin(pick, (int)c);
-}
-// sum of n tile partials into *out (one workgroup)
-#define REDUCE_THREADS 1024
-// workgroup b sums the n slots of partial buffer b (SA_FIELDS rows of npart)
-// into out[b]
-__global__ __launch_bounds__(REDUCE_THREADS) void ka_reduce(const double *part, long long npart, int n,
-                                                             ScalarAdj *out) {
-  part += (long long)blockIdx.x * SA_FIELDS * npart;
-  out += blockIdx.x;
-  __shared__ double sh[SA_FIELDS][REDUCE_THREADS / 64];
+// Block rb of the reduction: buffer rb / ADJ_RED_BLOCKS of the two
+// substeps' partials (x.red_src: the second substep's, then the first's),
+// its share of the n slots, into x.red.
+__device__ void reduce_block(const AdjExtra &x, const AdjMesh &a, int rb, int n) {
+  const int buf = rb / ADJ_RED_BLOCKS, blk = rb % ADJ_RED_BLOCKS;
+  const double *part = x.red_src + (long long)buf * SA_FIELDS * a.npart;
+  const int per = (n + ADJ_RED_BLOCKS - 1) / ADJ_RED_BLOCKS;
+  const int i0 = blk * per, i1 = min(n, i0 + per);
   double v[SA_FIELDS];
 #pragma unroll
   for (int k = 0; k < SA_FIELDS; ++k) v[k] = 0.0;
-  // 4 slots per thread and pass, all 9 fields: 36 loads in flight
-  for (int i0 = threadIdx.x; i0 < n; i0 += 4 * REDUCE_THREADS) {
-    double x[SA_FIELDS][4];
+#pragma unroll 1
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x)
 #pragma unroll
-    for (int k = 0; k < SA_FIELDS; ++k)
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int i = i0 + u * REDUCE_THREADS;
-        x[k][u] = i < n ? part[k * npart + i] : 0.0;
-      }
-#pragma unroll
-    for (int k = 0; k < SA_FIELDS; ++k) v[k] += (x[k][0] + x[k][1]) + (x[k][2] + x[k][3]);
-  }
-#pragma unroll
-  for (int k = 0; k < SA_FIELDS; ++k)
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int k = 0; k < SA_FIELDS; ++k) sh[k][w] = v[k];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double t[SA_FIELDS];
-    for (int k = 0; k < SA_FIELDS; ++k) {
-      t[k] = 0.0;
-      for (int i = 0; i < REDUCE_THREADS / 64; ++i) t[k] += sh[k][i];
-    }
-    out->dt = t[0];
-    out->alpha = t[1];
-    out->M[0] = t[2];
-    out->M[1] = t[3];
-    out->xbh[0] = t[4];
-    out->xbh[1] = t[5];
-    out->ybh[0] = t[6];
-    out->ybh[1] = t[7];
-    out->loss = t[8];
-  }
+    for (int k = 0; k < SA_FIELDS; ++k) v[k] += part[k * a.npart + i];
+  block_sum9(v, x.red + (long long)rb * SA_FIELDS, 1);
 }
 
-// per annulus sum of the cells' rotation adjoints (one workgroup per annulus)
-__global__ __launch_bounds__(TILE) void ka_rot(DevMesh m, AdjMesh a, double *out) {
+// per annulus sum of the cells' rotation adjoints src into out[ann]; clears them
+__device__ void rot_annulus(const DevMesh &m, const double *src0, int ann, double *out) {
   __shared__ double sh[TILE / 64];
-  const AnnInfo A = m.ann[blockIdx.x];
+  double *src = (double *)src0;
+  const AnnInfo A = m.ann[ann];
   double s = 0.0;
   for (int j = threadIdx.x; j < A.N; j += TILE) {
-    s += a.arot[A.zbase + j];
-    a.arot[A.zbase + j] = 0.0;
+    s += src[A.zbase + j];
+    src[A.zbase + j] = 0.0;
   }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
@@ -772,7 +797,7 @@ __global__ __launch_bounds__(TILE) void ka_rot(DevMesh m, AdjMesh a, double *out
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < TILE / 64; ++w) t += sh[w];
-    out[blockIdx.x] = t;
+    out[ann] = t;
   }
 }
 
@@ -790,113 +815,171 @@ __device__ __forceinline__ void chain_pos(const DiskScalars &sc, const ScalarAdj
   }
 }
 
-// The step's dt enters: both substeps' dt_s (dt, dt/2), the phase the binary
-// advances (omega dt) and every annulus's turn (Omega_a dt, Omega_a = w_a / r_a,
+static __device__ __forceinline__ ScalarAdj scalar_adj_of(const double *v) {
+  ScalarAdj s;
+  s.dt = v[0];
+  s.alpha = v[1];
+  s.M[0] = v[2];
+  s.M[1] = v[3];
+  s.xbh[0] = v[4];
+  s.xbh[1] = v[5];
+  s.ybh[0] = v[6];
+  s.ybh[1] = v[7];
+  s.loss = v[8];
+  return s;
+}
+
+// The step's CFL cell, one workgroup: the lowest interior zone whose step
+// is the minimum *dtp (the first tile, in zone order, whose minimum it is,
+// then the lowest such cell in it; a tile whose cells missed it, were one
+// to, would pass to the next) into *pick (INT_MAX: none).  m: the step-start
+// state, the scalars the forward's CFL minimum saw, m.dt_tile its tile
+// minima.
+__device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, const unsigned long long *dtp,
+                          int *pick) {
+  __shared__ int s_i[TILE / 64];
+  const unsigned long long dtbits = *dtp;
+  auto block_min = [&](int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_down(v, off));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_i[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int r = s_i[0];
+    for (int w = 1; w < TILE / 64; ++w) r = min(r, s_i[w]);
+    return r;
+  };
+  int c = 0x7fffffff;
+  int from = 0;
+  while (c == 0x7fffffff && from < nitiles) {
+    int ft = 0x7fffffff;
+    for (int t = from + threadIdx.x; t < nitiles; t += TILE)
+      if ((unsigned long long)__double_as_longlong(m.dt_tile[t]) == dtbits) {
+        ft = t;
+        break;
+      }
+    const int tile = block_min(ft);
+    if (tile == 0x7fffffff) break;
+    const int2 tl = itiles[tile];
+    const AnnInfo At = m.ann[tl.x];
+    const int j = tl.y + threadIdx.x;
+    int cc = 0x7fffffff;
+    if (j < At.N) {
+      const long long z = At.zbase + j;
+      double pr[5], r, dx, w, tc;
+      const double d = cfl_cell(m, tl, At, disk_of(m, At.disk), m.sc[At.disk], z, pr, &r, &dx, &w, &tc);
+      if ((unsigned long long)__double_as_longlong(d) == dtbits) cc = (int)z;
+    }
+    c = block_min(cc);
+    from = tile + 1;
+  }
+  if (threadIdx.x == 0) *pick = c;
+}
+
+// The scalar chain of one step (one workgroup).  The two substeps' scalar
+// adjoints S2, S1 are the sums of the reduction's partials.  The step's dt
+// enters: both substeps' dt_s (dt, dt/2), the phase the binary advances
+// (omega dt) and every annulus's turn (Omega_a dt, Omega_a = w_a / r_a,
 // cell_misc.c:73-75), and T (lamT).  lam_dt = 0 when dt was clipped (the CFL
 // minimum did not set it).  Then the CFL step's own backward: lam_dt times
-// the gradient of the step of the cell ka_dt_pick chose, into aP and the
-// cell's rotation adjoint, and its scalar adjoints into *S0 (one cell, one
-// thread); pick is reset for the next step.  m.sc: the scalars the forward's CFL minimum saw.
-__global__ __launch_bounds__(TILE) void ka_chain_dt(DevMesh m, AdjMesh a, int nann, const DiskScalars *A,
-                                                    const DiskScalars *B, const ScalarAdj *S1, const ScalarAdj *S2,
-                                                    const double *th2, const double *Ltheta, double lamT,
-                                                    AdjChain *cs, int *pick, ScalarAdj *S0) {
+// the gradient of the step of the cell pick_cell chose (*pick), into aP, the
+// rotation adjoint of its annulus and its scalar adjoints S0.  The step's contributions to the sums: Ltheta += th1n + th2
+// (+ the CFL cell's), and the masses' phases, radii and masses and alpha in
+// cs.  th1n: the first substep's rotation sums of the step after this one
+// (summed one step late; Ltheta is read only by the chains of earlier
+// steps, so each of them still sees every later step's).  m.sc: the
+// scalars the forward's CFL minimum saw (pre).
+__global__ __launch_bounds__(TILE) void ka_chain(DevMesh m, AdjMesh a, const int *pick, int nann,
+                                                 const DiskScalars *pre,
+                                                 const DiskScalars *A, const DiskScalars *B, const double *red,
+                                                 const double *th1n, const double *th2, double *Ltheta, double lamT,
+                                                 AdjChain *cs) {
   __shared__ double sh[TILE / 64];
+  __shared__ double s_sa[2][SA_FIELDS];  // S2, S1
+  if (threadIdx.x < 2 * SA_FIELDS) {
+    const int buf = threadIdx.x / SA_FIELDS, k = threadIdx.x % SA_FIELDS;
+    double t = 0.0;
+    for (int b = 0; b < ADJ_RED_BLOCKS; ++b) t += red[(buf * ADJ_RED_BLOCKS + b) * SA_FIELDS + k];
+    s_sa[buf][k] = t;
+  }
   double s = 0.0;
   for (int k = threadIdx.x; k < nann; k += TILE) {
     const AnnInfo &an = m.ann[k];
-    s += m.wiph[k] / (.5 * (an.rp + an.rm)) * (Ltheta[k] + th2[k]);
+    const double v = Ltheta[k] + th1n[k] + th2[k];
+    s += m.wiph[k] / (.5 * (an.rp + an.rm)) * v;
+    Ltheta[k] = v;
   }
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-  __syncthreads();
+  __syncthreads();  // sh and s_sa
+  const int c = *pick;
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < TILE / 64; ++w) t += sh[w];
+    const ScalarAdj S2 = scalar_adj_of(s_sa[0]), S1 = scalar_adj_of(s_sa[1]);
     double lphi2[2], lr2[2];
-    chain_pos(*B, *S2, lphi2, lr2);
+    chain_pos(*B, S2, lphi2, lr2);
     const double omega = A->gm[0].omega;
-    const double lam = 1.0 * S1->dt + 0.5 * S2->dt + omega * (cs->Lphi[0] + cs->Lphi[1] + lphi2[0] + lphi2[1]) +
+    const double lam = 1.0 * S1.dt + 0.5 * S2.dt + omega * (cs->Lphi[0] + cs->Lphi[1] + lphi2[0] + lphi2[1]) +
                        lamT + t;
     const double dtmin = __longlong_as_double((long long)A->dtmin_bits);
     const double lam_dt = A->dt == dtmin ? lam : 0.0;
     cs->lam_dt = lam_dt;
-    ScalarAdj sb = {};
-    const int c = *pick;
+    ScalarAdj S0 = {};
     if (c >= 0 && c < m.Ntot) {
-      int lo = 0, hi = nann - 1;  // the annulus holding cell c (zbase ascending)
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
+      int l = 0, hi = nann - 1;  // the annulus holding cell c (zbase ascending)
+      while (l < hi) {
+        const int mid = (l + hi + 1) >> 1;
         if (m.ann[mid].zbase <= c)
-          lo = mid;
+          l = mid;
         else
           hi = mid - 1;
       }
-      const AnnInfo An = m.ann[lo];
+      const AnnInfo An = m.ann[l];
       const DiskParams &P = disk_of(m, An.disk);
       double pr[5], r, dx, w, tc;
-      cfl_cell(m, make_int2(lo, 0), An, P, m.sc[An.disk], c, pr, &r, &dx, &w, &tc);
+      cfl_cell(m, make_int2(l, 0), An, P, m.sc[An.disk], c, pr, &r, &dx, &w, &tc);
       double prb[5] = {0, 0, 0, 0, 0};
       double tcb = 0.0;
-      vjp_cell_dt(P, m.sc[An.disk], pr, r, dx, w, tc, lam_dt, prb, sb, &tcb);
+      vjp_cell_dt(P, m.sc[An.disk], pr, r, dx, w, tc, lam_dt, prb, S0, &tcb);
       add5(a.aP, m.Ntot, c, prb);
-      a.arot[c] += tcb;
+      Ltheta[l] += tcb;  // every thread's Ltheta writes precede the barriers above
     }
-    *S0 = sb;
-    *pick = 0x7fffffff;
-  }
-}
-
-__global__ __launch_bounds__(TILE) void ka_chain_acc(int nann, const DiskScalars *pre, const DiskScalars *A,
-                                                     const DiskScalars *B, const ScalarAdj *S0, const ScalarAdj *S1,
-                                                     const ScalarAdj *S2, const double *th1, const double *th2,
-                                                     double *Ltheta, AdjChain *cs) {
-  for (int k = threadIdx.x; k < nann; k += TILE) Ltheta[k] += th1[k] + th2[k];
-  if (threadIdx.x == 0) {
-    double lphi0[2], lr0[2], lphi1[2], lr1[2], lphi2[2], lr2[2];
-    chain_pos(*pre, *S0, lphi0, lr0);
-    chain_pos(*A, *S1, lphi1, lr1);
-    chain_pos(*B, *S2, lphi2, lr2);
+    double lphi0[2], lr0[2], lphi1[2], lr1[2];
+    chain_pos(*pre, S0, lphi0, lr0);
+    chain_pos(*A, S1, lphi1, lr1);
     for (int p = 0; p < 2; ++p) {
       cs->Lphi[p] += lphi1[p] + lphi2[p] + lphi0[p];
       cs->Lr[p] += lr1[p] + lr2[p] + lr0[p];
-      cs->LM[p] += S1->M[p] + S2->M[p] + S0->M[p];
+      cs->LM[p] += S1.M[p] + S2.M[p] + S0.M[p];
     }
-    cs->Lalpha += S1->alpha + S2->alpha + S0->alpha;
+    cs->Lalpha += S1.alpha + S2.alpha + S0.alpha;
   }
 }
 
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st) {
+void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
+                            const AdjExtra &x) {
   const dim3 B(TILE);
-  if (L.n_bc1) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc1), B, 0, st, m, a, L.bc1);
-  if (L.n_bc0) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc0), B, 0, st, m, a, L.bc0);
+  // the rotation sums and the CFL cell ride in the first boundary launch
+  const int nrot = x.rot_src ? L.nann : 0;
+  const int npick = x.pick ? 1 : 0;
+  if (L.n_bc1 + nrot + npick)
+    hipLaunchKernelGGL(ka_bc, dim3(L.n_bc1 + nrot + npick), B, 0, st, m, a, L.bc1, L.n_bc1, nrot, L.itiles,
+                       L.n_itiles, x);
+  if (L.n_bc0) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc0), B, 0, st, m, a, L.bc0, L.n_bc0, 0, L.itiles, 0, AdjExtra());
   // part slots: ka_update [0, nz), the phi faces [nz, nz+nu), the r faces
   // [nz+nu, nz+nu+nrf) (Riemann part), and with viscosity the same two
   // ranges of the viscous part after them
   const int nz = L.n_ztiles, nu = L.n_utiles, nrf = L.n_rftiles;
+  const int nslots = a.visc ? nz + 2 * nu + 2 * nrf : nz + nu + nrf;
+  const int nred = x.red ? 2 * ADJ_RED_BLOCKS : 0;
   hipLaunchKernelGGL(ka_update, dim3(nz), B, 0, st, m, a, L.ztiles);
   hipLaunchKernelGGL(ka_faces_h, dim3(nu + nrf), B, 0, st, m, a, L.utiles, nu, L.rftiles, nz);
   if (a.visc) hipLaunchKernelGGL(ka_faces_v, dim3(nu + nrf), B, 0, st, m, a, L.utiles, nu, L.rftiles, nz + nu + nrf);
   hipLaunchKernelGGL(ka_plm_r, dim3(nz), B, 0, st, m, a, L.ztiles);
-  hipLaunchKernelGGL(ka_slope, dim3(nz), B, 0, st, m, a, L.ztiles);
-  hipLaunchKernelGGL(ka_plm_p, dim3(nz), B, 0, st, m, a, L.ztiles);
-}
-void launch_adjoint_reduce(const AdjMesh &a, const LaunchPlan &L, hipStream_t st, int nbuf, ScalarAdj *out) {
-  const int nz = L.n_ztiles, nu = L.n_utiles, nrf = L.n_rftiles;
-  hipLaunchKernelGGL(ka_reduce, dim3(nbuf), dim3(REDUCE_THREADS), 0, st, (const double *)a.part, a.npart,
-                     a.visc ? nz + 2 * nu + 2 * nrf : nz + nu + nrf, out);
-}
-
-void launch_adjoint_pick(const DevMesh &m, const LaunchPlan &L, hipStream_t st, const unsigned long long *dtbits,
-                         int *pick) {
-  if (L.n_itiles) hipLaunchKernelGGL(ka_dt_pick, dim3(L.n_itiles), dim3(TILE), 0, st, m, L.itiles, dtbits, pick);
-}
-
-void launch_adjoint_rot(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st, double *out) {
-  hipLaunchKernelGGL(ka_rot, dim3(L.nann), dim3(TILE), 0, st, m, a, out);
+  hipLaunchKernelGGL(ka_slope_p, dim3(L.n_stiles + nred), B, 0, st, m, a, L.stiles, L.n_stiles, nslots, x);
 }
 
 void launch_copy_scalars(DiskScalars *dst, const DiskScalars *src, hipStream_t st) {
@@ -905,18 +988,11 @@ void launch_copy_scalars(DiskScalars *dst, const DiskScalars *src, hipStream_t s
   hipLaunchKernelGGL(ka_copy_scalars, dim3(1), dim3(64), 0, st, dst, src);
 }
 
-void launch_adjoint_chain_dt(const DevMesh &m, const AdjMesh &a, int nann, const DiskScalars *A,
-                             const DiskScalars *B, const ScalarAdj *S1, const ScalarAdj *S2, const double *th2,
-                             const double *Ltheta, double lamT, AdjChain *cs, int *pick, ScalarAdj *S0,
-                             hipStream_t st) {
-  hipLaunchKernelGGL(ka_chain_dt, dim3(1), dim3(TILE), 0, st, m, a, nann, A, B, S1, S2, th2, Ltheta, lamT, cs, pick,
-                     S0);
-}
-
-void launch_adjoint_chain_acc(int nann, const DiskScalars *pre, const DiskScalars *A, const DiskScalars *B,
-                              const ScalarAdj *S0, const ScalarAdj *S1, const ScalarAdj *S2, const double *th1,
-                              const double *th2, double *Ltheta, AdjChain *cs, hipStream_t st) {
-  hipLaunchKernelGGL(ka_chain_acc, dim3(1), dim3(TILE), 0, st, nann, pre, A, B, S0, S1, S2, th1, th2, Ltheta, cs);
+void launch_adjoint_chain(const DevMesh &m, const AdjMesh &a, const int *pick, int nann, const DiskScalars *pre,
+                          const DiskScalars *A, const DiskScalars *B, const double *red, const double *th1n,
+                          const double *th2, double *Ltheta, double lamT, AdjChain *cs, hipStream_t st) {
+  hipLaunchKernelGGL(ka_chain, dim3(1), dim3(TILE), 0, st, m, a, pick, nann, pre, A, B, red, th1n, th2, Ltheta, lamT,
+                     cs);
 }
 
 // ---------------------------------------------------------------------------

@@ -77,6 +77,7 @@ void Engine::adjoint_redirect(const AdjSubRec &r) {
   d_fo_r = r.fo_r;
   d_run = r.run;
   d_pst = r.pst;
+  d_dt_tile = r.dt_tile;
   adj_usave = r.usave;  // k_update records what cons2prim converts
   adj_ftype = r.ftype;  // the face builder records the face types
 }
@@ -190,13 +191,13 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   // that the recompute of one segment (on st_aux) overlaps the reverse sweep
   // of the next (on st); laid out once to size it, once to carve it
   const int nbuf = nseg > 1 ? 2 : 1;
-  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr, *aFS, *arot, *torque, *acc, *ckpt, *fin, *th1,
+  double *aP, *aPn, *aR, *au, *agrad, *agradp, *aSL, *aSR, *fr, *aFS, *arot2, *arot1, *torque, *acc, *ckpt, *fin, *th1,
       *th2, *Ltheta;
   DiskScalars *screcs[2] = {nullptr, nullptr}, *ckpt_sc, *fin_sc;
   double *part;
-  ScalarAdj *out;
-  AdjChain *cs;
+  double *red;
   int *pick;
+  AdjChain *cs;
   std::vector<AdjSubRec> recs[2] = {std::vector<AdjSubRec>(2 * every), std::vector<AdjSubRec>(nbuf > 1 ? 2 * every : 0)};
   Arena ar;
   size_t zero_bytes = 0;
@@ -204,10 +205,12 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     ar.off = 0;
     // first the fields that start at zero (one memset)
     aPn = ar.take<double>(nq);
-    arot = ar.take<double>(nz);
+    arot2 = ar.take<double>(nz);  // the cells' rotation adjoints of the second substep
+    arot1 = ar.take<double>(nz);  // and of the first
     acc = ar.take<double>(1);
     Ltheta = ar.take<double>((size_t)nann);
-    out = ar.take<ScalarAdj>(3);  // S2, S1, S0 of the current step
+    red = ar.take<double>(2 * ADJ_RED_BLOCKS * 9);  // the two substeps' scalar partials, reduced per block
+    pick = ar.take<int>(1);
     cs = ar.take<AdjChain>(1);
     zero_bytes = ar.off;
     aP = ar.take<double>(nq);
@@ -228,7 +231,6 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     ckpt_sc = ar.take<DiskScalars>((size_t)nseg);
     fin_sc = ar.take<DiskScalars>(1);
     part = ar.take<double>(2 * 9 * npart);  // the two substeps' partial sums
-    pick = ar.take<int>(1);
     for (int b = 0; b < nbuf; ++b) {
       for (auto &r : recs[b]) {
         r.prim = ar.take<double>(nq);
@@ -245,6 +247,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
         r.run = ar.take<int>((size_t)nrun);
         r.pst = ar.take<PairState>(npairs);
         r.ftype = ar.take<signed char>(nf);
+        r.dt_tile = ar.take<double>((size_t)std::max(ni_t, 1));
       }
     }
   };
@@ -268,11 +271,11 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
 
   // the engine's own buffers, pointed back at after each recorded segment
   struct Own {
-    double *prim[2], *tiph[2], *dphi[2], *grad, *gradp, *fdphi, *fcm;
+    double *prim[2], *tiph[2], *dphi[2], *grad, *gradp, *fdphi, *fcm, *dt_tile;
     int *fL, *fR, *fo_r, *run;
     PairState *pst;
   } own = {{d_prim[0], d_prim[1]}, {d_tiph[0], d_tiph[1]}, {d_dphi[0], d_dphi[1]}, d_grad, d_gradp, d_fdphi, d_fcm,
-           d_fL, d_fR, d_fo_r, d_run, d_pst};
+           d_dt_tile, d_fL, d_fR, d_fo_r, d_run, d_pst};
   auto end_record = [&]() {
     hipMemcpyAsync(own.prim[cur], d_prim[cur], nq * 8, hipMemcpyDeviceToDevice, st);
     hipMemcpyAsync(own.tiph[cur], d_tiph[cur], nz * 8, hipMemcpyDeviceToDevice, st);
@@ -293,6 +296,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     d_fo_r = own.fo_r;
     d_run = own.run;
     d_pst = own.pst;
+    d_dt_tile = own.dt_tile;
   };
   auto snapshot = [&](double *c) {
     hipMemcpyAsync(c, d_prim[cur], nq * 8, hipMemcpyDeviceToDevice, st);
@@ -373,7 +377,6 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   a.npart = (long long)npart;
   a.wl = wl;
   a.lam_dt = &cs->lam_dt;
-  a.arot = arot;
   a.visc = P.visc > 0.0;
   // the streams: the reverse sweep on st, the recomputes on st_aux.  A
   // recompute writes the engine's state and one record; it waits for the
@@ -418,8 +421,6 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     st = sm;
     hipEventRecord(ev_rec[buf_of(seg)], st_aux);
   };
-  // "no cell yet" for the first step's CFL pick (ka_chain_dt resets it)
-  hipMemsetAsync(pick, 0x7f, sizeof(int), sm);
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int n0 = seg * every, n1 = std::min(nsteps, n0 + every);
     const int b = buf_of(seg);
@@ -447,8 +448,22 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
       a.fac = 0.5;
       a.copy_first = 0;
       a.part = part;
-      launch_adjoint_substep(m2, a, plan, sm);
-      launch_adjoint_rot(m2, a, plan, sm, th2);
+      a.arot = arot2;
+      // riding in this substep's first launch: the rotation sums of the next
+      // step's first substep (th1, used by this step's chain) and the step's
+      // CFL cell (the step-start state with the scalars and tile minima the
+      // forward's CFL minimum saw)
+      AdjExtra x2;
+      x2.rot_src = arot1;
+      x2.pick_prim = in1.prim;
+      x2.pick_tiph = in1.tiph;
+      x2.pick_dphi = in1.dphi;
+      x2.pick_dt_tile = rec[2 * k].dt_tile;
+      x2.pick_sc = scP;
+      x2.dtbits = &scA->dtmin_bits;
+      x2.pick = pick;
+      x2.rot_out = th1;
+      launch_adjoint_substep(m2, a, plan, sm, x2);
       std::swap(aP, aPn);
       // substep 1 (fac 1, RKcons = prim2cons(prim))
       const DevMesh m1 = sub_mesh(in1, rec[2 * k], scA);
@@ -457,20 +472,23 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
       a.fac = 1.0;
       a.copy_first = 1;
       a.part = part + 9 * npart;
-      launch_adjoint_substep(m1, a, plan, sm);
+      a.arot = arot1;
+      // the second substep's rotation sums ride in this substep's first
+      // launch, both substeps' scalar partials' reduction in its last
+      AdjExtra x1;
+      x1.rot_src = arot2;
+      x1.rot_out = th2;
+      x1.red_src = part;
+      x1.red = red;
+      launch_adjoint_substep(m1, a, plan, sm, x1);
       std::swap(aP, aPn);
-      // both substeps' scalar adjoints, out[0] and out[1], in one launch
       a.part = part;
-      launch_adjoint_reduce(a, plan, sm, 2, out);
       // the step's dt: its CFL minimum, at the step-start state with the
       // scalars the forward's CFL minimum saw
       DevMesh md = m1;
       md.sc = scP;
       a.aP = aPn;  // the step-start adjoint
-      launch_adjoint_pick(md, plan, sm, &scA->dtmin_bits, pick);
-      launch_adjoint_chain_dt(md, a, nann, scA, scB, out + 1, out + 0, th2, Ltheta, lamT, cs, pick, out + 2, sm);
-      launch_adjoint_rot(m1, a, plan, sm, th1);  // substep 1 and the CFL step
-      launch_adjoint_chain_acc(nann, scP, scA, scB, out + 2, out + 1, out + 0, th1, th2, Ltheta, cs, sm);
+      launch_adjoint_chain(md, a, pick, nann, scP, scA, scB, red, th1, th2, Ltheta, lamT, cs, sm);
     }
     if (rn) recompute_end(seg - 1);
     if (nbuf > 1) {

@@ -40,32 +40,44 @@ struct AdjChain {
   double Lphi[2], Lr[2], LM[2], Lalpha, lam_dt;
 };
 
-// the backward kernels of one substep; their per-tile partial sums of the
-// scalar adjoints go to a.part (launch_adjoint_reduce sums them)
-void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st);
-// out[b] = the sum of the substep partials in a.part + b * 9 * a.npart, for
-// b < nbuf, one workgroup each
-void launch_adjoint_reduce(const AdjMesh &a, const LaunchPlan &L, hipStream_t st, int nbuf, ScalarAdj *out);
-// pick: device int scratch (0x7fffffff: none yet) that receives the lowest
-// interior cell whose CFL step is the step's minimum dtbits (device)
-void launch_adjoint_pick(const DevMesh &m, const LaunchPlan &L, hipStream_t st, const unsigned long long *dtbits,
-                         int *pick);
-// per annulus sum of a.arot into out[annulus]; clears a.arot
-void launch_adjoint_rot(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st, double *out);
+// Work that rides in the workgroups past the tiles of a substep's backward
+// launches instead of launches of its own: the per-annulus sums of a
+// rotation-adjoint buffer (rot_src non-null) and the step's CFL cell (pick
+// non-null), both in the substep's first boundary launch, and the reduction
+// of both substeps' scalar partials (red non-null, in its slope launch).
+// The two substeps keep their cells' rotation adjoints in two buffers, so
+// that one substep's sums run while the other writes.
+#define ADJ_RED_BLOCKS 32  // workgroups per partial buffer in the reduction
+struct AdjExtra {
+  double *rot_src = nullptr;  // per cell rotation adjoints, summed per annulus into rot_out and cleared
+  double *rot_out = nullptr;  // [nann]
+  const double *red_src = nullptr;  // the second substep's partial buffer, the first's after it
+  double *red = nullptr;            // [2][ADJ_RED_BLOCKS][9]: their partial sums
+  // the CFL cell: the step-start state, the scalars its CFL minimum saw, the
+  // interior tiles' minima and the minimum; *pick receives the lowest
+  // interior zone whose step is the minimum (INT_MAX: none)
+  const double *pick_prim = nullptr, *pick_tiph = nullptr, *pick_dphi = nullptr, *pick_dt_tile = nullptr;
+  const DiskScalars *pick_sc = nullptr;
+  const unsigned long long *dtbits = nullptr;
+  int *pick = nullptr;
+};
+
+// the backward kernels of one substep and the work of x; their per-tile
+// partial sums of the scalar adjoints go to a.part
+void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
+                            const AdjExtra &x);
 void launch_copy_scalars(DiskScalars *dst, const DiskScalars *src, hipStream_t st);
-// lam_dt of a step from its substeps' scalar adjoints S1, S2, the second
-// substep's annulus rotation adjoints th2 and the sums so far (m: annuli,
-// wiph, and the scalars the forward's CFL minimum saw); then the CFL step's backward at the
-// picked cell into a.aP and a.arot, its scalar adjoints into S0, and pick
-// reset
-void launch_adjoint_chain_dt(const DevMesh &m, const AdjMesh &a, int nann, const DiskScalars *A,
-                             const DiskScalars *B, const ScalarAdj *S1, const ScalarAdj *S2, const double *th2,
-                             const double *Ltheta, double lamT, AdjChain *cs, int *pick, ScalarAdj *S0,
-                             hipStream_t st);
-// adds a step's contributions (S0: the CFL step's) to the sums
-void launch_adjoint_chain_acc(int nann, const DiskScalars *pre, const DiskScalars *A, const DiskScalars *B,
-                              const ScalarAdj *S0, const ScalarAdj *S1, const ScalarAdj *S2, const double *th1,
-                              const double *th2, double *Ltheta, AdjChain *cs, hipStream_t st);
+// The scalar chain of one step, one workgroup: the two substeps' scalar
+// adjoints S2, S1 from red; lam_dt of the step from them, the second
+// substep's annulus rotation adjoints th2, the first substep's of the step
+// after this one (th1n: those are summed one step late) and the sums so far
+// (m: annuli, wiph, the step-start state and the scalars the forward's CFL
+// minimum saw); the CFL step's backward at the cell *pick into a.aP (its
+// rotation adjoint into Ltheta of its annulus, its scalar adjoints S0); the
+// step's contributions added to the sums (Ltheta += th1n + th2, cs).
+void launch_adjoint_chain(const DevMesh &m, const AdjMesh &a, const int *pick, int nann, const DiskScalars *pre,
+                          const DiskScalars *A, const DiskScalars *B, const double *red, const double *th1n,
+                          const double *th2, double *Ltheta, double lamT, AdjChain *cs, hipStream_t st);
 
 // Fixed device scratch of the allocation-free torque loss: the accumulator
 // and one slot per k_update tile (512 KiB: batches of up to 65,535 update

@@ -41,7 +41,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     return;
   }
   std::vector<double> hrf, hzf;
-  std::vector<int2> ztiles, utiles, itiles, rft, zft, bc0, bc1, szt, bit;
+  std::vector<int2> ztiles, utiles, stiles, itiles, rft, zft, bc0, bc1, szt, bit;
   std::vector<InitRow> hinit;
   bool any_init = false;
   long long zb = 0, fb = 0;
@@ -112,6 +112,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
         hann.push_back(A);
         for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) ztiles.push_back(make_int2(a, j0));
         for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE - 1) utiles.push_back(make_int2(a, j0));
+        for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE - 2) stiles.push_back(make_int2(a, j0));
         if (interior)
           for (int j0 = 0; j0 < A.N; j0 += DISCO_TILE) itiles.push_back(make_int2(a, j0));
         if (P.bound_r == BOUND_OUTFLOW) {
@@ -264,7 +265,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   };
   size_t o_z = add(ztiles, plan.n_ztiles), o_u = add(utiles, plan.n_utiles), o_i = add(itiles, plan.n_itiles), o_r = add(rft, plan.n_rftiles),
          o_zf = add(zft, plan.n_zftiles), o_b0 = add(bc0, plan.n_bc0), o_b1 = add(bc1, plan.n_bc1),
-         o_s = add(szt, plan.n_sz), o_bi = add(bit, plan.n_binit);
+         o_s = add(szt, plan.n_sz), o_bi = add(bit, plan.n_binit), o_st = add(stiles, plan.n_stiles);
   if (!dalloc(&d_tiles, all.size(), "tiles") || !h2d(d_tiles, all.data(), all.size())) return;
   plan.ztiles = d_tiles + o_z;
   plan.utiles = d_tiles + o_u;
@@ -275,6 +276,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   plan.bc1 = d_tiles + o_b1;
   plan.sz = d_tiles + o_s;
   plan.binit = d_tiles + o_bi;
+  plan.stiles = d_tiles + o_st;
   tl_zb[0][0] = plan.ztiles;
   tl_zb[0][1] = plan.utiles;
   tl_zb[0][2] = plan.zftiles;

@@ -15,6 +15,7 @@
 struct AdjSubRec {
   double *prim, *tiph, *dphi;
   double *grad, *gradp, *usave, *fdphi, *fcm;
+  double *dt_tile;  // the interior tiles' CFL minima (written in a step's first substep)
   int *fL, *fR, *fo_r, *run;
   PairState *pst;
   signed char *ftype;

@@ -64,6 +64,7 @@ struct LaunchPlan {
   int n_rpairs, n_zpairs;
   const int2 *ztiles; int n_ztiles;     // every zone
   const int2 *utiles; int n_utiles;     // every zone, DISCO_TILE - 1 cells per tile (k_update)
+  const int2 *stiles; int n_stiles;     // every zone, DISCO_TILE - 2 cells per tile (a phi neighbour each side)
   const int2 *itiles; int n_itiles;     // interior zones (CFL)
   const int2 *rftiles; int n_rftiles;   // r faces
   const int2 *zftiles; int n_zftiles;   // z faces
