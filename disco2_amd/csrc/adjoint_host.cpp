@@ -179,6 +179,12 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     return -1;
   }
   if (every < 1 || every > nsteps) every = nsteps;
+  // the second stream and the events of the checkpointed sweep, made once
+  // (by the first call, whatever its interval) and kept like the working
+  // memory
+  if (!st_aux && !hip_check(hipStreamCreate(&st_aux), "adjoint stream")) return -1;
+  for (hipEvent_t &e : adj_ev)
+    if (!e && !hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "adjoint event")) return -1;
   const size_t nz = (size_t)Ntot, nq = (size_t)NQ * Ntot, nf = (size_t)NFtot;
   const size_t wA = nq + 3 * nz;  // doubles per checkpoint: prim, tiph, dphi, RKtiph
   const int nseg = (nsteps + every - 1) / every;
@@ -384,15 +390,9 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   // its segment waits for it.  The sweep reads only its record, its
   // checkpoint and the constant mesh (its face-slope scratch is aFS).
   const hipStream_t sm = st;
-  hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_back[2] = {nullptr, nullptr}, ev_fwd = nullptr;
+  hipEvent_t *const ev_rec = adj_ev, *const ev_back = adj_ev + 2, ev_fwd = adj_ev[4];
   bool used[2] = {true, true};  // the loss pass filled both records
   if (nbuf > 1) {
-    bool good = st_aux || hip_check(hipStreamCreate(&st_aux), "adjoint stream");
-    for (int b = 0; b < 2 && good; ++b)
-      good = hip_check(hipEventCreateWithFlags(&ev_rec[b], hipEventDisableTiming), "adjoint event") &&
-             hip_check(hipEventCreateWithFlags(&ev_back[b], hipEventDisableTiming), "adjoint event");
-    good = good && hip_check(hipEventCreateWithFlags(&ev_fwd, hipEventDisableTiming), "adjoint event");
-    if (!good) return -1;
     hipEventRecord(ev_fwd, sm);
     hipStreamWaitEvent(st_aux, ev_fwd, 0);
   }
@@ -501,11 +501,6 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   hip_check(hipMemcpy(g_prim0, aPn, nq * 8, hipMemcpyDeviceToHost), "adjoint D2H");
   // (the blocking copies have waited for both streams; segment 0's
   // recompute, the last, precedes segment 0's sweep on st)
-  for (int b = 0; b < 2; ++b) {
-    if (ev_rec[b]) hipEventDestroy(ev_rec[b]);
-    if (ev_back[b]) hipEventDestroy(ev_back[b]);
-  }
-  if (ev_fwd) hipEventDestroy(ev_fwd);
   if (nseg > 1) {
     adjoint_restore(fin, false);
     launch_copy_scalars(d_sc, fin_sc, st);

@@ -354,6 +354,8 @@ Engine::~Engine() {
   if (ev0) hipEventDestroy(ev0);
   if (ev1) hipEventDestroy(ev1);
   if (st_aux) hipStreamDestroy(st_aux);
+  for (hipEvent_t e : adj_ev)
+    if (e) hipEventDestroy(e);
   if (st) hipStreamDestroy(st);
 }
 

@@ -102,6 +102,9 @@ struct Engine {
   // the adjoint's recompute stream (default priority): a segment's forward
   // recompute runs on it while the reverse sweep of the next segment runs on st
   hipStream_t st_aux = nullptr;
+  // the adjoint's second stream and its events (recompute done, sweep done
+  // per record, loss pass done), made by the first adjoint call
+  hipEvent_t adj_ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_cur = -1;
