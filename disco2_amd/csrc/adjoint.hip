@@ -97,12 +97,12 @@ __device__ __forceinline__ void face_geo_to_rot(const DevMesh &m, const AdjMesh 
 // it was averaged from, weighted as the forward weighted their primitives.
 // The mesh view has the substep's faces and the new dphi.
 __device__ void rot_annulus(const DevMesh &m, const double *src, int ann, double *out);
-__device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, const unsigned long long *dtp,
-                          int *pick);
+__device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, int nann,
+                          const unsigned long long *dtp, CflGrad *pick);
 // workgroups past the nmain tiles: one per annulus of the rotation sums,
 // then (x.pick) one for the step's CFL cell
 __global__ __launch_bounds__(TILE) void ka_bc(DevMesh m, AdjMesh a, const int2 *tiles, int nmain, int nrot,
-                                              const int2 *itiles, int nitiles, AdjExtra x) {
+                                              const int2 *itiles, int nitiles, int nann, AdjExtra x) {
   const int b = xcd_order();
   if (b >= nmain + nrot) {
     DevMesh mp = m;  // the step-start state and the scalars its CFL minimum saw
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(TILE) void ka_bc(DevMesh m, AdjMesh a, const int2 *
     mp.dphi = (double *)x.pick_dphi;
     mp.sc = (DiskScalars *)x.pick_sc;
     mp.dt_tile = (double *)x.pick_dt_tile;
-    pick_cell(mp, itiles, nitiles, x.dtbits, x.pick);
+    pick_cell(mp, itiles, nitiles, nann, x.dtbits, x.pick);
     return;
   }
   if (b >= nmain) {
@@ -832,11 +832,11 @@ static __device__ __forceinline__ ScalarAdj scalar_adj_of(const double *v) {
 // The step's CFL cell, one workgroup: the lowest interior zone whose step
 // is the minimum *dtp (the first tile, in zone order, whose minimum it is,
 // then the lowest such cell in it; a tile whose cells missed it, were one
-// to, would pass to the next) into *pick (INT_MAX: none).  m: the step-start
-// state, the scalars the forward's CFL minimum saw, m.dt_tile its tile
-// minima.
-__device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, const unsigned long long *dtp,
-                          int *pick) {
+// to, would pass to the next), and the gradient of its step (one thread),
+// into *pick.  m: the step-start state, the scalars the forward's CFL
+// minimum saw, m.dt_tile its tile minima.
+__device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, int nann,
+                          const unsigned long long *dtp, CflGrad *pick) {
   __shared__ int s_i[TILE / 64];
   const unsigned long long dtbits = *dtp;
   auto block_min = [&](int v) {
@@ -872,7 +872,29 @@ __device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, con
     c = block_min(cc);
     from = tile + 1;
   }
-  if (threadIdx.x == 0) *pick = c;
+  if (threadIdx.x != 0) return;
+  CflGrad g = {};
+  g.c = c;
+  if (c >= 0 && c < m.Ntot) {
+    int l = 0, hi = nann - 1;  // the annulus holding cell c (zbase ascending)
+    while (l < hi) {
+      const int mid = (l + hi + 1) >> 1;
+      if (m.ann[mid].zbase <= c)
+        l = mid;
+      else
+        hi = mid - 1;
+    }
+    g.ann = l;
+    const AnnInfo An = m.ann[l];
+    const DiskParams &P = disk_of(m, An.disk);
+    double pr[5], r, dx, w, tc;
+    cfl_cell(m, make_int2(l, 0), An, P, m.sc[An.disk], c, pr, &r, &dx, &w, &tc);
+    ScalarAdj sb = {};
+    vjp_cell_dt(P, m.sc[An.disk], pr, r, dx, w, tc, 1.0, g.prb, sb, &g.tcb);
+    const double v[9] = {sb.dt, sb.alpha, sb.M[0], sb.M[1], sb.xbh[0], sb.xbh[1], sb.ybh[0], sb.ybh[1], sb.loss};
+    for (int k = 0; k < 9; ++k) g.sa[k] = v[k];
+  }
+  *pick = g;
 }
 
 // The scalar chain of one step (one workgroup).  The two substeps' scalar
@@ -881,14 +903,15 @@ __device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, con
 // (omega dt) and every annulus's turn (Omega_a dt, Omega_a = w_a / r_a,
 // cell_misc.c:73-75), and T (lamT).  lam_dt = 0 when dt was clipped (the CFL
 // minimum did not set it).  Then the CFL step's own backward: lam_dt times
-// the gradient of the step of the cell pick_cell chose (*pick), into aP, the
-// rotation adjoint of its annulus and its scalar adjoints S0.  The step's contributions to the sums: Ltheta += th1n + th2
+// the gradient of the step of the cell pick_cell chose (*pick, with that
+// gradient), into aP, the rotation adjoint of its annulus and its scalar
+// adjoints S0.  The step's contributions to the sums: Ltheta += th1n + th2
 // (+ the CFL cell's), and the masses' phases, radii and masses and alpha in
 // cs.  th1n: the first substep's rotation sums of the step after this one
 // (summed one step late; Ltheta is read only by the chains of earlier
 // steps, so each of them still sees every later step's).  m.sc: the
 // scalars the forward's CFL minimum saw (pre).
-__global__ __launch_bounds__(TILE) void ka_chain(DevMesh m, AdjMesh a, const int *pick, int nann,
+__global__ __launch_bounds__(TILE) void ka_chain(DevMesh m, AdjMesh a, const CflGrad *pick, int nann,
                                                  const DiskScalars *pre,
                                                  const DiskScalars *A, const DiskScalars *B, const double *red,
                                                  const double *th1n, const double *th2, double *Ltheta, double lamT,
@@ -911,7 +934,6 @@ __global__ __launch_bounds__(TILE) void ka_chain(DevMesh m, AdjMesh a, const int
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
   __syncthreads();  // sh and s_sa
-  const int c = *pick;
   if (threadIdx.x == 0) {
     double t = 0.0;
     for (int w = 0; w < TILE / 64; ++w) t += sh[w];
@@ -925,24 +947,15 @@ __global__ __launch_bounds__(TILE) void ka_chain(DevMesh m, AdjMesh a, const int
     const double lam_dt = A->dt == dtmin ? lam : 0.0;
     cs->lam_dt = lam_dt;
     ScalarAdj S0 = {};
-    if (c >= 0 && c < m.Ntot) {
-      int l = 0, hi = nann - 1;  // the annulus holding cell c (zbase ascending)
-      while (l < hi) {
-        const int mid = (l + hi + 1) >> 1;
-        if (m.ann[mid].zbase <= c)
-          l = mid;
-        else
-          hi = mid - 1;
-      }
-      const AnnInfo An = m.ann[l];
-      const DiskParams &P = disk_of(m, An.disk);
-      double pr[5], r, dx, w, tc;
-      cfl_cell(m, make_int2(l, 0), An, P, m.sc[An.disk], c, pr, &r, &dx, &w, &tc);
-      double prb[5] = {0, 0, 0, 0, 0};
-      double tcb = 0.0;
-      vjp_cell_dt(P, m.sc[An.disk], pr, r, dx, w, tc, lam_dt, prb, S0, &tcb);
-      add5(a.aP, m.Ntot, c, prb);
-      Ltheta[l] += tcb;  // every thread's Ltheta writes precede the barriers above
+    const CflGrad &g = *pick;
+    if (g.c >= 0 && g.c < m.Ntot) {  // lam_dt times the gradient of the CFL cell's step
+      double prb[5];
+      for (int q = 0; q < 5; ++q) prb[q] = lam_dt * g.prb[q];
+      add5(a.aP, m.Ntot, g.c, prb);
+      Ltheta[g.ann] += lam_dt * g.tcb;  // every thread's Ltheta writes precede the barrier above
+      double v[9];
+      for (int k = 0; k < 9; ++k) v[k] = lam_dt * g.sa[k];
+      S0 = scalar_adj_of(v);
     }
     double lphi0[2], lr0[2], lphi1[2], lr1[2];
     chain_pos(*pre, S0, lphi0, lr0);
@@ -967,8 +980,9 @@ void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan
   const int npick = x.pick ? 1 : 0;
   if (L.n_bc1 + nrot + npick)
     hipLaunchKernelGGL(ka_bc, dim3(L.n_bc1 + nrot + npick), B, 0, st, m, a, L.bc1, L.n_bc1, nrot, L.itiles,
-                       L.n_itiles, x);
-  if (L.n_bc0) hipLaunchKernelGGL(ka_bc, dim3(L.n_bc0), B, 0, st, m, a, L.bc0, L.n_bc0, 0, L.itiles, 0, AdjExtra());
+                       L.n_itiles, L.nann, x);
+  if (L.n_bc0)
+    hipLaunchKernelGGL(ka_bc, dim3(L.n_bc0), B, 0, st, m, a, L.bc0, L.n_bc0, 0, L.itiles, 0, L.nann, AdjExtra());
   // part slots: ka_update [0, nz), the phi faces [nz, nz+nu), the r faces
   // [nz+nu, nz+nu+nrf) (Riemann part), and with viscosity the same two
   // ranges of the viscous part after them
@@ -988,7 +1002,7 @@ void launch_copy_scalars(DiskScalars *dst, const DiskScalars *src, hipStream_t s
   hipLaunchKernelGGL(ka_copy_scalars, dim3(1), dim3(64), 0, st, dst, src);
 }
 
-void launch_adjoint_chain(const DevMesh &m, const AdjMesh &a, const int *pick, int nann, const DiskScalars *pre,
+void launch_adjoint_chain(const DevMesh &m, const AdjMesh &a, const CflGrad *pick, int nann, const DiskScalars *pre,
                           const DiskScalars *A, const DiskScalars *B, const double *red, const double *th1n,
                           const double *th2, double *Ltheta, double lamT, AdjChain *cs, hipStream_t st) {
   hipLaunchKernelGGL(ka_chain, dim3(1), dim3(TILE), 0, st, m, a, pick, nann, pre, A, B, red, th1n, th2, Ltheta, lamT,

@@ -202,7 +202,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
   DiskScalars *screcs[2] = {nullptr, nullptr}, *ckpt_sc, *fin_sc;
   double *part;
   double *red;
-  int *pick;
+  CflGrad *pick;
   AdjChain *cs;
   std::vector<AdjSubRec> recs[2] = {std::vector<AdjSubRec>(2 * every), std::vector<AdjSubRec>(nbuf > 1 ? 2 * every : 0)};
   Arena ar;
@@ -216,7 +216,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
     acc = ar.take<double>(1);
     Ltheta = ar.take<double>((size_t)nann);
     red = ar.take<double>(2 * ADJ_RED_BLOCKS * 9);  // the two substeps' scalar partials, reduced per block
-    pick = ar.take<int>(1);
+    pick = ar.take<CflGrad>(1);
     cs = ar.take<AdjChain>(1);
     zero_bytes = ar.off;
     aP = ar.take<double>(nq);

@@ -55,11 +55,18 @@ struct AdjExtra {
   double *red = nullptr;            // [2][ADJ_RED_BLOCKS][9]: their partial sums
   // the CFL cell: the step-start state, the scalars its CFL minimum saw, the
   // interior tiles' minima and the minimum; *pick receives the lowest
-  // interior zone whose step is the minimum (INT_MAX: none)
+  // interior zone whose step is the minimum and the gradient of its step
   const double *pick_prim = nullptr, *pick_tiph = nullptr, *pick_dphi = nullptr, *pick_dt_tile = nullptr;
   const DiskScalars *pick_sc = nullptr;
   const unsigned long long *dtbits = nullptr;
-  int *pick = nullptr;
+  struct CflGrad *pick = nullptr;
+};
+// The cell whose CFL step is the step's dt (c = INT_MAX: none), its annulus,
+// and the gradient of its step: on its primitives (prb), its annulus's
+// rotation (tcb) and the scalars (sa: the ScalarAdj fields in order).
+struct CflGrad {
+  int c, ann;
+  double prb[5], tcb, sa[9];
 };
 
 // the backward kernels of one substep and the work of x; their per-tile
@@ -72,10 +79,11 @@ void launch_copy_scalars(DiskScalars *dst, const DiskScalars *src, hipStream_t s
 // substep's annulus rotation adjoints th2, the first substep's of the step
 // after this one (th1n: those are summed one step late) and the sums so far
 // (m: annuli, wiph, the step-start state and the scalars the forward's CFL
-// minimum saw); the CFL step's backward at the cell *pick into a.aP (its
-// rotation adjoint into Ltheta of its annulus, its scalar adjoints S0); the
-// step's contributions added to the sums (Ltheta += th1n + th2, cs).
-void launch_adjoint_chain(const DevMesh &m, const AdjMesh &a, const int *pick, int nann, const DiskScalars *pre,
+// minimum saw); the CFL step's backward, lam_dt times *pick's gradient,
+// into a.aP (its rotation adjoint into Ltheta of its annulus, its scalar
+// adjoints S0); the step's contributions added to the sums (Ltheta += th1n +
+// th2, cs).
+void launch_adjoint_chain(const DevMesh &m, const AdjMesh &a, const CflGrad *pick, int nann, const DiskScalars *pre,
                           const DiskScalars *A, const DiskScalars *B, const double *red, const double *th1n,
                           const double *th2, double *Ltheta, double lamT, AdjChain *cs, hipStream_t st);
 
