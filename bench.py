@@ -510,6 +510,8 @@ def main(argv=None):
     ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint leg and print its JSON (for profiling it)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched leg and print its JSON (for profiling it)")
     ap.add_argument("--diag-only", action="store_true",
                     help="run only the diagnostics leg on the secondary workload and print its JSON")
     args = ap.parse_args(argv)
@@ -521,6 +523,9 @@ def main(argv=None):
         return 2
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(args.adjoint_steps, args.adjoint_every)))
+        return 0
+    if args.batched_only:
+        print(json.dumps(batched_leg(args.batched_steps, args.warmup)))
         return 0
     if args.diag_only:
         from disco2_amd import configs, ics

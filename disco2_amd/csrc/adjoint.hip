@@ -1,15 +1,16 @@
 // Backward kernels of one substep for the hand-written adjoint (2D, NUM_Q 5,
 // non-rotating analytic frame, state-independent mesh motion).
 //
-// The forward substep is re-run first (with k_update recording the conserved
-// state it converts), so the faces, the phi and r slopes and the state are
-// in place.  Then, in reverse: the outflow remap, the cell update (cons2prim,
-// RK blend, prim2cons, sources, the torque of the loss), the phi faces, the
-// r faces, the r limiter, the face slopes, the phi limiter.  Cells receive
-// the contributions of their faces through atomic adds (the adjoint is not
-// bitwise reproducible run to run; it does not need to be).  Launch-uniform
-// scalars (dt, alpha, the masses and their positions) are summed per tile
-// and reduced per substep.
+// The forward's record holds each substep's input and output state, slopes,
+// faces and converted conserved state.  In reverse, per substep: the cell
+// update (cons2prim, RK blend, prim2cons, sources, the torque of the loss)
+// with the outflow remap's backward gathered by the remap's source cells,
+// the phi and r faces (Riemann and viscous parts), the r limiter, the face
+// slopes and the phi limiter.  Cells gather what their faces hand them (per
+// tile in LDS, or per-face slots): no atomics, every sum in a fixed order,
+// so the gradients are the same bits run to run.  Launch-uniform scalars
+// (dt, alpha, the masses and their positions) are summed per tile and
+// reduced per step.
 #include <mutex>
 
 #include "adjoint.h"
@@ -26,8 +27,7 @@ __device__ __forceinline__ void store5(double *a, long long N, long long c, cons
 }
 __device__ __forceinline__ void add5(double *a, long long N, long long c, const double *v) {
 #pragma unroll
-  for (int q = 0; q < 5; ++q)
-    if (v[q] != 0.0) atomicAdd(&a[q * N + c], v[q]);
+  for (int q = 0; q < 5; ++q) a[q * N + c] += v[q];
 }
 
 // workgroup sum of nine values per thread; thread k < 9 stores sum k at
@@ -84,78 +84,124 @@ __device__ __forceinline__ void face_rot_split(signed char type, double wb, doub
   *vin = wb * w_in + cb * c_in + tLb;
   *vout = wb * w_out + cb * c_out + tRb;
 }
-// the same, added to the two cells' entries of a.arot
-__device__ __forceinline__ void face_geo_to_rot(const DevMesh &m, const AdjMesh &a, long long g, long long cL,
-                                                long long cR, double wb, double cb, double tLb, double tRb) {
-  double vin, vout;
-  face_rot_split(m.ftype[g], wb, cb, tLb, tRb, &vin, &vout);
-  if (vin != 0.0) atomicAdd(&a.arot[cL], vin);
-  if (vout != 0.0) atomicAdd(&a.arot[cR], vout);
-}
 
-// Outflow remap backward (k_bc_r): a ghost cell's adjoint goes to the cells
-// it was averaged from, weighted as the forward weighted their primitives.
-// The mesh view has the substep's faces and the new dphi.
-__device__ void rot_annulus(const DevMesh &m, const double *src, int ann, double *out);
-__device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, int nann,
-                          const unsigned long long *dtp, CflGrad *pick);
-// workgroups past the nmain tiles: one per annulus of the rotation sums,
-// then (x.pick) one for the step's CFL cell
-__global__ __launch_bounds__(TILE) void ka_bc(DevMesh m, AdjMesh a, const int2 *tiles, int nmain, int nrot,
-                                              const int2 *itiles, int nitiles, int nann, AdjExtra x) {
-  const int b = xcd_order();
-  if (b >= nmain + nrot) {
-    DevMesh mp = m;  // the step-start state and the scalars its CFL minimum saw
-    mp.prim = (double *)x.pick_prim;
-    mp.tiph = x.pick_tiph;
-    mp.dphi = (double *)x.pick_dphi;
-    mp.sc = (DiskScalars *)x.pick_sc;
-    mp.dt_tile = (double *)x.pick_dt_tile;
-    pick_cell(mp, itiles, nitiles, nann, x.dtbits, x.pick);
-    return;
-  }
-  if (b >= nmain) {
-    rot_annulus(m, x.rot_src, b - nmain, x.rot_out);
-    return;
-  }
-  const int2 tl = tiles[b];
-  const AnnInfo A = m.ann[tl.x];
-  const int j = tl.y + threadIdx.x;
-  if (j >= A.N) return;
-  const DiskParams &P = disk_of(m, A.disk);
-  const long long c = A.zbase + j;
-  const int k = A.k, i = A.i;
-  double lam[5];
-  load5(a.aPn, m.Ntot, c, lam);
-  const double dAc = (A.zp - A.zm) * (i == 0 ? A.rp : A.rm) * m.dphi_new[c];
-  // ghost = sum over faces of prim[src] * dA_f / dAc, dA_f = r * width * dperp
-  if (i == 0) {
-    faces_as_L(m, rpair_of(P, k, 0), m.fo_r, c, j, A.N, [&](const PairInfo &PI, long long g) {
-      const long long s = m.fR[g];
-      const double w = PI.r * m.fdphi[g] * PI.deltaPerp / dAc;
-      double v[5], wb = 0.0;
+// Outflow remap backward (k_bc_r, cell_boundary_outflow_r), gathered.  The
+// remap fills ghost annulus NR-2 from the faces' overlap with NR-3, then
+// NR-1 from the new NR-2 (and annulus 0 from annulus 1 unless NoInnerBC):
+// ghost = sum over its faces of prim[src] * dA_f / dAc, dA_f = r * width *
+// dperp, dAc the ghost's face area.  Backward, each source cell gathers
+// what its ghosts hand it; a cell of NR-3 first adds to each NR-2 ghost the
+// NR-1 ghosts' shares.  A face's width also enters its dA: the adjoint of
+// the width goes to the rotations of the face's two annuli, and each of its
+// two cells takes its own side's share (face_rot_split), so no cell is
+// written by another's thread.  The mesh view has the substep's faces and
+// the new dphi.
+// dAc of ghost cell g of annulus An (inner: the inner ghost annulus)
+__device__ __forceinline__ double ghost_area(const DevMesh &m, const AnnInfo &An, long long g, bool inner) {
+  return (An.zp - An.zm) * (inner ? An.rp : An.rm) * m.dphi_new[g];
+}
+// the adjoint ghost cell g of annulus NR-2 (An, local index jg) holds after
+// the NR-1 ghosts' shares (the second stage's backward)
+__device__ __forceinline__ void ghost2_total(const DevMesh &m, const AdjMesh &a, const DiskParams &P, int k,
+                                             const AnnInfo &An, const AnnInfo &Ao, long long g, int jg, double *lam) {
+  load5(a.aPn, m.Ntot, g, lam);
+  faces_as_L(m, rpair_of(P, k, P.NR - 2), m.fo_r, g, jg, An.N, [&](const PairInfo &PI, long long f) {
+    const long long h = m.fR[f];
+    const double w = PI.r * m.fdphi[f] * PI.deltaPerp / ghost_area(m, Ao, h, false);
+    for (int q = 0; q < 5; ++q) lam[q] += a.aPn[q * m.Ntot + h] * w;
+  });
+}
+// the rotation adjoint of the L (asL) or R side of face f whose width
+// adjoint is wb
+__device__ __forceinline__ double side_rot(const DevMesh &m, long long f, double wb, bool asL) {
+  double vin, vout;
+  face_rot_split(m.ftype[f], wb, 0.0, 0.0, 0.0, &vin, &vout);
+  return asL ? vin : vout;
+}
+// cell c (local j) of annulus A: the remap's backward gathered into ob (a
+// source cell) and its sides' rotation adjoints of the remap's faces into
+// *rot (source and ghost cells)
+__device__ __forceinline__ void remap_gather(const DevMesh &m, const AdjMesh &a, const DiskParams &P,
+                                             const int2 tl, const AnnInfo &A, long long c, int j, double *ob,
+                                             double *rot) {
+  const int k = A.k, i = A.i, NR = P.NR;
+  const bool inner = P.no_inner_bc != 1;
+  double rt = 0.0;
+  if (i == NR - 3) {  // source of the NR-2 ghosts (with the NR-1 ghosts' shares)
+    const AnnInfo An = m.ann[tl.x + 1], Ao = m.ann[tl.x + 2];
+    faces_as_L(m, rpair_of(P, k, NR - 3), m.fo_r, c, j, A.N, [&](const PairInfo &PI, long long f) {
+      const long long g = m.fR[f];
+      double lam[5];
+      ghost2_total(m, a, P, k, An, Ao, g, (int)(g - An.zbase), lam);
+      const double s = PI.r * PI.deltaPerp / ghost_area(m, An, g, false);
+      const double w = m.fdphi[f] * s;
+      double wb = 0.0;
       for (int q = 0; q < 5; ++q) {
-        v[q] = lam[q] * w;
-        wb += lam[q] * m.prim_new[q * m.Ntot + s];
+        ob[q] += lam[q] * w;
+        wb += lam[q] * m.prim_new[q * m.Ntot + c];
       }
-      add5(a.aPn, m.Ntot, s, v);
-      face_geo_to_rot(m, a, g, c, s, wb * PI.r * PI.deltaPerp / dAc, 0.0, 0.0, 0.0);
-    });
-  } else {
-    faces_as_R(m, rpair_of(P, k, i - 1), j, A.N, [&](const PairInfo &PI, long long g) {
-      const long long s = m.fL[g];
-      const double w = PI.r * m.fdphi[g] * PI.deltaPerp / dAc;
-      double v[5], wb = 0.0;
-      for (int q = 0; q < 5; ++q) {
-        v[q] = lam[q] * w;
-        wb += lam[q] * m.prim_new[q * m.Ntot + s];
-      }
-      add5(a.aPn, m.Ntot, s, v);
-      face_geo_to_rot(m, a, g, s, c, wb * PI.r * PI.deltaPerp / dAc, 0.0, 0.0, 0.0);
+      rt += side_rot(m, f, wb * s, true);
     });
   }
-  const double z[5] = {0, 0, 0, 0, 0};
-  store5(a.aPn, m.Ntot, c, z);
+  if (i == NR - 2) {
+    // source of the NR-1 ghosts
+    const AnnInfo Ao = m.ann[tl.x + 1];
+    faces_as_L(m, rpair_of(P, k, NR - 2), m.fo_r, c, j, A.N, [&](const PairInfo &PI, long long f) {
+      const long long h = m.fR[f];
+      const double s = PI.r * PI.deltaPerp / ghost_area(m, Ao, h, false);
+      double wb = 0.0;
+      for (int q = 0; q < 5; ++q) wb += a.aPn[q * m.Ntot + h] * m.prim_new[q * m.Ntot + c];
+      rt += side_rot(m, f, wb * s, true);
+    });
+    // ghost of the NR-3 cells
+    double lam[5];
+    ghost2_total(m, a, P, k, A, Ao, c, j, lam);
+    const double s0 = (A.zp - A.zm) * A.rm * m.dphi_new[c];
+    faces_as_R(m, rpair_of(P, k, NR - 3), j, A.N, [&](const PairInfo &PI, long long f) {
+      const long long src = m.fL[f];
+      double wb = 0.0;
+      for (int q = 0; q < 5; ++q) wb += lam[q] * m.prim_new[q * m.Ntot + src];
+      rt += side_rot(m, f, wb * (PI.r * PI.deltaPerp / s0), false);
+    });
+  }
+  if (i == NR - 1) {  // ghost of the NR-2 cells
+    double lam[5];
+    load5(a.aPn, m.Ntot, c, lam);
+    const double s0 = (A.zp - A.zm) * A.rm * m.dphi_new[c];
+    faces_as_R(m, rpair_of(P, k, NR - 2), j, A.N, [&](const PairInfo &PI, long long f) {
+      const long long src = m.fL[f];
+      double wb = 0.0;
+      for (int q = 0; q < 5; ++q) wb += lam[q] * m.prim_new[q * m.Ntot + src];
+      rt += side_rot(m, f, wb * (PI.r * PI.deltaPerp / s0), false);
+    });
+  }
+  if (i == 1 && inner) {  // source of the inner ghosts (NR = 4: as well as the outer)
+    const AnnInfo Ai = m.ann[tl.x - 1];
+    faces_as_R(m, rpair_of(P, k, 0), j, A.N, [&](const PairInfo &PI, long long f) {
+      const long long g = m.fL[f];
+      const double s = PI.r * PI.deltaPerp / ghost_area(m, Ai, g, true);
+      const double w = m.fdphi[f] * s;
+      double wb = 0.0;
+      for (int q = 0; q < 5; ++q) {
+        const double lam = a.aPn[q * m.Ntot + g];
+        ob[q] += lam * w;
+        wb += lam * m.prim_new[q * m.Ntot + c];
+      }
+      rt += side_rot(m, f, wb * s, false);
+    });
+  }
+  if (i == 0 && inner) {  // inner ghost of the annulus-1 cells
+    double lam[5];
+    load5(a.aPn, m.Ntot, c, lam);
+    const double s0 = (A.zp - A.zm) * A.rp * m.dphi_new[c];
+    faces_as_L(m, rpair_of(P, k, 0), m.fo_r, c, j, A.N, [&](const PairInfo &PI, long long f) {
+      const long long src = m.fR[f];
+      double wb = 0.0;
+      for (int q = 0; q < 5; ++q) wb += lam[q] * m.prim_new[q * m.Ntot + src];
+      rt += side_rot(m, f, wb * (PI.r * PI.deltaPerp / s0), true);
+    });
+  }
+  *rot = rt;
 }
 
 // Cell update backward (k_update phase 2): cons2prim, the RK blend and the
@@ -168,8 +214,29 @@ __global__ __launch_bounds__(TILE) void ka_bc(DevMesh m, AdjMesh a, const int2 *
 #ifndef ADJ_VISC_WAVES
 #define ADJ_VISC_WAVES 1
 #endif
-__global__ __launch_bounds__(TILE, ADJ_UPDATE_WAVES) void ka_update(DevMesh m, AdjMesh a, const int2 *tiles) {
+__device__ void rot_annulus(const DevMesh &m, const double *src, int ann, double *out);
+__device__ void pick_cell(const DevMesh &m, const int2 *itiles, int nitiles, int nann,
+                          const unsigned long long *dtp, CflGrad *pick);
+// Workgroups past the nmain tiles: one per annulus of the rotation sums,
+// then (x.pick) one for the step's CFL cell.
+__global__ __launch_bounds__(TILE, ADJ_UPDATE_WAVES) void ka_update(DevMesh m, AdjMesh a, const int2 *tiles,
+                                                                    int nmain, int nrot, const int2 *itiles,
+                                                                    int nitiles, int nann, AdjExtra x) {
   const int ti = xcd_order();
+  if (ti >= nmain + nrot) {
+    DevMesh mp = m;  // the step-start state and the scalars its CFL minimum saw
+    mp.prim = (double *)x.pick_prim;
+    mp.tiph = x.pick_tiph;
+    mp.dphi = (double *)x.pick_dphi;
+    mp.sc = (DiskScalars *)x.pick_sc;
+    mp.dt_tile = (double *)x.pick_dt_tile;
+    pick_cell(mp, itiles, nitiles, nann, x.dtbits, x.pick);
+    return;
+  }
+  if (ti >= nmain) {
+    rot_annulus(m, x.rot_src, ti - nmain, x.rot_out);
+    return;
+  }
   const int2 tl = tiles[ti];
   const AnnInfo A = m.ann[tl.x];
   const DiskParams &P = disk_of(m, A.disk);
@@ -181,10 +248,16 @@ __global__ __launch_bounds__(TILE, ADJ_UPDATE_WAVES) void ka_update(DevMesh m, A
     const double rp = A.rp, rm = A.rm, zp = A.zp, zm = A.zm;
     const double r = .5 * (rp + rm), dz = zp - zm;
     const double dphi = m.dphi[c], dphin = m.dphi_new[c];
-    double pr[5], u[5], ob[5], ub[5] = {0, 0, 0, 0, 0}, prb[5] = {0, 0, 0, 0, 0}, u0b[5];
+    double pr[5], u[5], ob[5] = {0, 0, 0, 0, 0}, ub[5] = {0, 0, 0, 0, 0}, prb[5] = {0, 0, 0, 0, 0}, u0b[5];
     load5(m.prim, m.Ntot, c, pr);
     load5(m.u_save, m.Ntot, c, u);
-    load5(a.aPn, m.Ntot, c, ob);
+    // the remap overwrites the ghost annuli: their adjoint goes to the
+    // cells they were averaged from, which gather it
+    const int i = A.i, NR = P.NR;
+    const bool ghost = i >= NR - 2 || (i == 0 && P.no_inner_bc != 1);
+    if (!ghost) load5(a.aPn, m.Ntot, c, ob);
+    double rot = 0.0;
+    if (i >= NR - 3 || i <= 1) remap_gather(m, a, P, tl, A, c, j, ob, &rot);
     vjp_cons2prim(u, r, .5 * (rp * rp - rm * rm) * dphin * dz, P, ob, ub);
     store5(a.au, m.Ntot, c, ub);
     for (int q = 0; q < 5; ++q) {
@@ -201,7 +274,7 @@ __global__ __launch_bounds__(TILE, ADJ_UPDATE_WAVES) void ka_update(DevMesh m, A
     const double wl = interior_cell(P, A.i, A.k) ? a.wl : 0.0;
     double phib = 0.0;
     vjp_cell_sources(P, sc, pr, r, dVs, a.fac * sc.dt, phi, .5 * (zp + zm), ub, wl, prb, sb, &phib);
-    a.arot[c] += phib;
+    a.arot[c] += rot + phib;
     store5(a.aP, m.Ntot, c, prb);
   }
   reduce_store(sb, a, ti);
@@ -975,21 +1048,17 @@ __global__ __launch_bounds__(TILE) void ka_chain(DevMesh m, AdjMesh a, const Cfl
 void launch_adjoint_substep(const DevMesh &m, const AdjMesh &a, const LaunchPlan &L, hipStream_t st,
                             const AdjExtra &x) {
   const dim3 B(TILE);
-  // the rotation sums and the CFL cell ride in the first boundary launch
-  const int nrot = x.rot_src ? L.nann : 0;
-  const int npick = x.pick ? 1 : 0;
-  if (L.n_bc1 + nrot + npick)
-    hipLaunchKernelGGL(ka_bc, dim3(L.n_bc1 + nrot + npick), B, 0, st, m, a, L.bc1, L.n_bc1, nrot, L.itiles,
-                       L.n_itiles, L.nann, x);
-  if (L.n_bc0)
-    hipLaunchKernelGGL(ka_bc, dim3(L.n_bc0), B, 0, st, m, a, L.bc0, L.n_bc0, 0, L.itiles, 0, L.nann, AdjExtra());
   // part slots: ka_update [0, nz), the phi faces [nz, nz+nu), the r faces
   // [nz+nu, nz+nu+nrf) (Riemann part), and with viscosity the same two
   // ranges of the viscous part after them
   const int nz = L.n_ztiles, nu = L.n_utiles, nrf = L.n_rftiles;
   const int nslots = a.visc ? nz + 2 * nu + 2 * nrf : nz + nu + nrf;
   const int nred = x.red ? 2 * ADJ_RED_BLOCKS : 0;
-  hipLaunchKernelGGL(ka_update, dim3(nz), B, 0, st, m, a, L.ztiles);
+  // the rotation sums and the CFL cell ride in the update launch
+  const int nrot = x.rot_src ? L.nann : 0;
+  const int npick = x.pick ? 1 : 0;
+  hipLaunchKernelGGL(ka_update, dim3(nz + nrot + npick), B, 0, st, m, a, L.ztiles, nz, nrot, L.itiles, L.n_itiles,
+                     L.nann, x);
   hipLaunchKernelGGL(ka_faces_h, dim3(nu + nrf), B, 0, st, m, a, L.utiles, nu, L.rftiles, nz);
   if (a.visc) hipLaunchKernelGGL(ka_faces_v, dim3(nu + nrf), B, 0, st, m, a, L.utiles, nu, L.rftiles, nz + nu + nrf);
   hipLaunchKernelGGL(ka_plm_r, dim3(nz), B, 0, st, m, a, L.ztiles);

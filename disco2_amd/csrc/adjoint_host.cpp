@@ -449,7 +449,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
       a.copy_first = 0;
       a.part = part;
       a.arot = arot2;
-      // riding in this substep's first launch: the rotation sums of the next
+      // riding in this substep's update launch: the rotation sums of the next
       // step's first substep (th1, used by this step's chain) and the step's
       // CFL cell (the step-start state with the scalars and tile minima the
       // forward's CFL minimum saw)
@@ -473,7 +473,7 @@ int Engine::adjoint(int nsteps, int every, double *loss, double *g_alpha, double
       a.copy_first = 1;
       a.part = part + 9 * npart;
       a.arot = arot1;
-      // the second substep's rotation sums ride in this substep's first
+      // the second substep's rotation sums ride in this substep's update
       // launch, both substeps' scalar partials' reduction in its last
       AdjExtra x1;
       x1.rot_src = arot2;

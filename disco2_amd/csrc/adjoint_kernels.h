@@ -43,7 +43,7 @@ struct AdjChain {
 // Work that rides in the workgroups past the tiles of a substep's backward
 // launches instead of launches of its own: the per-annulus sums of a
 // rotation-adjoint buffer (rot_src non-null) and the step's CFL cell (pick
-// non-null), both in the substep's first boundary launch, and the reduction
+// non-null), both in the substep's update launch, and the reduction
 // of both substeps' scalar partials (red non-null, in its slope launch).
 // The two substeps keep their cells' rotation adjoints in two buffers, so
 // that one substep's sums run while the other writes.

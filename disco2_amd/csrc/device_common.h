@@ -15,6 +15,10 @@
 #ifndef FLUX_WAVES
 #define FLUX_WAVES 1
 #endif
+// the update's variant for HLL without viscosity (the 3D disk's)
+#ifndef UPDATE_WAVES_HLL
+#define UPDATE_WAVES_HLL UPDATE_WAVES
+#endif
 
 __device__ __forceinline__ void set_err(int *err, int code) { atomicCAS(err, 0, code); }
 

@@ -537,7 +537,7 @@ __device__ __forceinline__ void plm_pairs(const AnnInfo &A, const DiskParams &P,
 
 // b, e: the cell's face ranges in direction dir (cell_face_ranges), when the
 // caller has loaded them already
-template <int NQ>
+template <int NQ, int NC = PlmCache<NQ>::NC>
 __device__ __forceinline__ void plm_cell_dir(const DevMesh &m, const AnnInfo &A, const DiskParams &P, int j,
                                              long long z, int dir, const double *w, const double *gp, double phi_o,
                                              double dp, double (*s_S)[NQ][TILE], double *out, const int *b,
@@ -571,7 +571,6 @@ __device__ __forceinline__ void plm_cell_dir(const DevMesh &m, const AnnInfo &A,
     pp[2] = PI.deltaPerp;
     ds[2] = PI.deltaR + PI.deltaL;
   }
-  constexpr int NC = PlmCache<NQ>::NC;
   const int t = threadIdx.x;
   int n = 0;
   // the slope sum in face order; two faces per trip, so that the second
@@ -677,8 +676,16 @@ __global__ __launch_bounds__(TILE) void k_plm_rz(DevMesh m, const int2 *tiles, i
 // Both slopes of a 3D disk's cells in one pass (the cell's own state loaded
 // once): the r slopes into grad_r, then the z slopes into m.grad, which then
 // holds what the reference's grad holds after its z pass.
+// face slopes of a cell cached in LDS and minimum waves per SIMD of the
+// two-direction kernel
+#ifndef PLM2_NC
+#define PLM2_NC PlmCache<NQ>::NC
+#endif
+#ifndef PLM2_WAVES
+#define PLM2_WAVES 1
+#endif
 template <int NQ>
-__global__ __launch_bounds__(TILE) void k_plm_rz2(DevMesh m, const int2 *tiles, double *grad_r) {
+__global__ __launch_bounds__(TILE, PLM2_WAVES) void k_plm_rz2(DevMesh m, const int2 *tiles, double *grad_r) {
   const int2 tl = tiles[tile_of(m)];
   const AnnInfo A = m.ann[tl.x];
   const int j = tl.y + threadIdx.x;
@@ -689,13 +696,14 @@ __global__ __launch_bounds__(TILE) void k_plm_rz2(DevMesh m, const int2 *tiles, 
   // measured 2% slower: more registers held through the r pass)
   double w[NQ], gp[NQ], phi_o, dp;
   plm_own<NQ>(m, z, w, gp, &phi_o, &dp);
-  __shared__ double s_S[PlmCache<NQ>::NC][NQ][TILE];
+  constexpr int NC = PLM2_NC;
+  __shared__ double s_S[NC][NQ][TILE];
 #pragma unroll
   for (int dir = 0; dir < 2; ++dir) {
     int pR, pL, b[3], e[3];
     plm_pairs(A, P, dir, &pR, &pL);
     cell_face_ranges(m, pR, pL, dir == 0 ? m.fo_r : m.fo_z, z, j, A.N, b, e);
-    plm_cell_dir<NQ>(m, A, P, j, z, dir, w, gp, phi_o, dp, s_S, dir == 0 ? grad_r : m.grad, b, e);
+    plm_cell_dir<NQ, NC>(m, A, P, j, z, dir, w, gp, phi_o, dp, s_S, dir == 0 ? grad_r : m.grad, b, e);
   }
 }
 
@@ -921,8 +929,13 @@ __device__ __forceinline__ void add_faces(const double *F, long long stride, lon
 // FUSE: FUSE_R / FUSE_Z evaluate the r / z faces' increments in place (each
 // face by both of its cells) instead of gathering the stored ones
 enum { FUSE_R = 1, FUSE_Z = 2 };
+template <int FEAT>
+constexpr int update_waves() {
+  return FEAT == F_HLL ? UPDATE_WAVES_HLL : UPDATE_WAVES;
+}
 template <int NQ, int FEAT, int FUSE>
-__global__ __launch_bounds__(TILE, UPDATE_WAVES) void k_update(DevMesh m, const int2 *tiles, double fac, int copy_first) {
+__global__ __launch_bounds__(TILE, update_waves<FEAT>()) void k_update(DevMesh m, const int2 *tiles, double fac,
+                                                                       int copy_first) {
   const int ti = tile_of(m);
   const int2 tl = tiles[ti];
   const AnnInfo A = m.ann[tl.x];
