@@ -7,11 +7,13 @@
 * the same workload run twice gives identical bits (no atomics touch the
   state), and inside a batch with another disk it gives the bits it gives
   alone;
-* disk3d_r256_z64 (12.5M zones, 64 layers, HLL, periodic z): the oracle is
-  too slow at this size (about a minute per step on one core), so the test
-  checks what does not depend on it: a run twice gives identical bits, and
-  the disk beside a small 2D disk in a batch gives the bits it gives alone;
-  the state stays finite with positive density and pressure;
+* disk3d_r256_z64 (13.4M zones with ghosts, 64 layers, HLL, periodic z):
+  two RK2 steps against the oracle (about 35 s per step on one core), per
+  zone to 1e-11, the cell edges to 1e-13 rad, t and dt to 1e-13; a run of
+  three steps twice gives
+  identical bits, and the disk beside a small 2D disk in a batch gives the
+  bits it gives alone; the state stays finite with positive density and
+  pressure;
 * binary2d_r256_adjoint (221,930 interior zones, 200 steps, checkpoints every
   50): dL/d alpha and dL/d q against central differences of the GPU loss.
   Over 200 steps the loss is piecewise smooth and strongly curved in q (the
@@ -99,6 +101,30 @@ def test_binary_full_size_is_deterministic_and_batch_invariant():
         assert bitwise_equal(out[k][0], out[0][0]), describe(out[k][0], out[0][0])
         assert bitwise_equal(out[k][1], out[0][1])
         assert bitwise_equal(out[k][2], out[0][2])
+
+
+def test_disk3d_full_size_matches_oracle():
+    from disco2_amd import configs
+    from oracle.oracle import Oracle
+    p, s, prim0, b = start(configs.c3_disk3d())
+    steps = 2
+    print(f"disk3d_r256_z64: {s.nzones} zones, {steps} steps against the oracle", flush=True)
+    b.run(steps)
+    prim, tiph, sg = b.download("prim"), b.download("tiph"), b.scalars()
+    b.close()
+    o = Oracle(p, s.N_p, s.r_faces, s.z_faces)
+    o.set_state(tiph=s.tiph0, prim=prim0)
+    o.init()
+    o.step(steps)
+    rel = zone_rel(prim, o.get("prim"))
+    # the cell edges are angles of order one, some of them near zero: their
+    # difference is measured in radians
+    dt_abs = float(np.max(np.abs(tiph - o.get("tiph"))))
+    print(f"max per-zone relative difference {rel:.2e}, cell edges {dt_abs:.2e} rad")
+    assert rel < 1e-11
+    assert dt_abs < 1e-13
+    so = o.scalars()
+    assert abs(sg[0] - so[0]) <= 1e-13 * so[0] and abs(sg[1] - so[1]) <= 1e-13 * so[1]
 
 
 def test_disk3d_full_size_is_deterministic_and_batch_invariant():
