@@ -44,6 +44,49 @@ __device__ __forceinline__ bool interior_cell(const DiskParams &P, int i, int k)
   return i >= P.NgMinR && i < P.NR - P.NgMaxR && k >= P.NgMinZ && k < P.NZ - P.NgMaxZ;
 }
 
+// An overlap face: its L and R zones, width and centre angle.
+struct FaceGeo {
+  long long cL, cR;
+  double w, cm;
+};
+// Face f of a regular pair (PairState::reg: inner annulus of N cells from
+// zone zin, outer annulus of N cells from zout), from the two annuli's
+// edges with the face builder's arithmetic (jloop_cell): face 2j is inner
+// cell j's step A with outer cell jo = (jpmin + j) mod N, face 2j + 1 its
+// step C with the next outer cell.  m.tiph, m.dphi: the substep's edges.
+__device__ __forceinline__ FaceGeo pair_face(const DevMesh &m, int jpmin, int N, long long zin, long long zout, int f) {
+  const int j = f >> 1;
+  int jo = jpmin + j + (f & 1);
+  if (jo >= N) jo -= N;
+  FaceGeo g;
+  g.cL = zin + j;
+  g.cR = zout + jo;
+  const double To = m.tiph[zout + jo];
+  if (!(f & 1)) {
+    const int jm = j == 0 ? N - 1 : j - 1;
+    g.w = wrap_p(To - m.tiph[zin + jm]);
+    g.cm = To - .5 * g.w;
+  } else {
+    const double Ti = m.tiph[zin + j];
+    g.w = m.dphi[zout + jo] + wrap_pm(Ti - To);
+    g.cm = Ti - .5 * g.w;
+  }
+  return g;
+}
+// face g of pair PI (state S): computed for a regular pair, else loaded
+__device__ __forceinline__ FaceGeo face_geo(const DevMesh &m, const PairInfo &PI, const PairState &S, long long g) {
+  if (S.reg) {
+    const AnnInfo Ai = m.ann[PI.ann_in];
+    return pair_face(m, S.jpmin, Ai.N, Ai.zbase, m.ann[PI.ann_out].zbase, (int)(g - PI.face_base));
+  }
+  FaceGeo G;
+  G.cL = m.fL[g];
+  G.cR = m.fR[g];
+  G.w = m.fdphi[g];
+  G.cm = m.fcm[g];
+  return G;
+}
+
 // S = (W_R - W_L)/(deltaL + deltaR) of one face, cell_plm.c:36-49 and 86-99
 template <int NQ>
 __device__ __forceinline__ void face_slope(const DevMesh &m, const PairInfo &P, long long g, double *S) {

@@ -119,4 +119,9 @@ struct PairState {   // written by the face builder each substep
   int jpmin;         // outer cell containing the inner annulus' start edge
   int T;             // number of outer-cell advances (runs - 1)
   int seq;           // 1 if the sequential fallback built this pair
+  // 1: both annuli have N cells and every inner cell j has exactly two
+  // faces, 2j with outer cell (jpmin + j) mod N and 2j + 1 with the next
+  // one (no edges tie); the faces' cells, widths and centres then follow
+  // from the two annuli's edges alone (pair_face)
+  int reg;
 };

@@ -233,7 +233,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
   hipMemset(d_ann_rot, 0, (size_t)nann * sizeof(double4));
   {  // no faces yet: every gather over a pair is empty until k_faces runs
     std::vector<PairState> p0(hpairs.size());
-    for (auto &x : p0) x = PairState{0, 0, -1, 0};
+    for (auto &x : p0) x = PairState{0, 0, -1, 0, 0};
     h2d(d_pst, p0.data(), p0.size());
   }
   hipMemset(d_fo_r, 0, Ntot * sizeof(int));

@@ -206,7 +206,7 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
   o.zin = A.zbase;
   o.zout = B.zbase;
   o.err = m.err;
-  int bad = 0, carry = 0;
+  int bad = 0, carry = 0, irregular = 0;
   for (int base = 0; base < Nin; base += FTB) {
     const int j = base + tid;
     int gs = 0, ge = 0, cnt = 0;
@@ -216,6 +216,7 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
       int adv = ge - gs;
       if (adv < 0) adv += Nout;
       cnt = 1 + adv;
+      irregular |= cnt != 2;
     }
     int chunk;
     const int foff = carry + block_excl_scan(cnt, s_w, &chunk);
@@ -228,6 +229,7 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
     carry += chunk;
   }
   bad = __syncthreads_or(bad);
+  irregular = __syncthreads_or(irregular);
   int total = carry;
   if (bad && tid == 0) {
     int f = 0, b2 = 0, jp = jpmin;
@@ -254,6 +256,9 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
     st.jpmin = jpmin;
     st.T = T;
     st.seq = bad;
+    // every cell verified with two faces: the walk advanced one outer cell
+    // per inner cell from jpmin
+    st.reg = !bad && !irregular && Nin == Nout && total == 2 * Nin;
     m.pst[p] = st;
   }
 }
@@ -466,13 +471,12 @@ template <int NQ>
 struct PlmCache {
   static constexpr int NC = NQ <= 6 ? 4 : 3;  // 40-48 KiB per workgroup
 };
-// S of face g of a cell whose centre angle clean_pi(tiph) - dphi/2 is phi_o,
-// primitives w and phi slopes gp; own_left: the cell is the face's L
+// S of a face (centre cm, other cell oth) of a cell whose centre angle
+// clean_pi(tiph) - dphi/2 is phi_o, primitives w and phi slopes gp;
+// own_left: the cell is the face's L
 template <int NQ>
 __device__ __forceinline__ void own_face_slope(const DevMesh &m, double phi_o, const double *w, const double *gp,
-                                               bool own_left, double dsum, long long g, double *S) {
-  const int oth = own_left ? m.fR[g] : m.fL[g];
-  const double cm = m.fcm[g];
+                                               bool own_left, double dsum, long long oth, double cm, double *S) {
   const double ph = clean_pi(m.tiph[oth]) - .5 * m.dphi[oth];
   const double pL = own_left ? phi_o : ph;
   const double pR = own_left ? ph : phi_o;
@@ -498,27 +502,43 @@ __device__ __forceinline__ void cell_face_ranges(const DevMesh &m, int pR, int p
   b[0] = b[1] = b[2] = 0;
   e[0] = e[1] = e[2] = 0;
   if (pR >= 0) {
-    const PairInfo PI = m.pairs[pR];
     const PairState S = m.pst[pR];
     int tt = j - S.jpmin;
     if (tt < 0) tt += N;
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      if (tt <= S.T) {
-        b[u] = m.run_start[PI.run_base + tt];
-        e[u] = m.run_start[PI.run_base + tt + 1];
+    if (S.reg) {  // runs of two faces 2t - 1, 2t; the wrap cell's two runs are faces 0 and 2N - 1
+      if (tt == 0) {
+        e[0] = 1;
+        b[1] = 2 * N - 1;
+        e[1] = 2 * N;
+      } else {
+        b[0] = 2 * tt - 1;
+        e[0] = 2 * tt + 1;
       }
-      tt += N;
+    } else {
+      const PairInfo PI = m.pairs[pR];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (tt <= S.T) {
+          b[u] = m.run_start[PI.run_base + tt];
+          e[u] = m.run_start[PI.run_base + tt + 1];
+        }
+        tt += N;
+      }
     }
   }
   if (pL >= 0) {
     const PairState S = m.pst[pL];
-    int f1 = (j + 1 < N) ? fo[z + 1] : S.count;
-    if (f1 > S.count) f1 = S.count;
-    int f0 = fo[z];
-    if (f0 < 0) f0 = 0;
-    b[2] = f0;
-    e[2] = f1;
+    if (S.reg) {  // faces 2j, 2j + 1
+      b[2] = 2 * j;
+      e[2] = 2 * j + 2;
+    } else {
+      int f1 = (j + 1 < N) ? fo[z + 1] : S.count;
+      if (f1 > S.count) f1 = S.count;
+      int f0 = fo[z];
+      if (f0 < 0) f0 = 0;
+      b[2] = f0;
+      e[2] = f1;
+    }
   }
 }
 
@@ -552,25 +572,42 @@ __device__ __forceinline__ void plm_cell_dir(const DevMesh &m, const AnnInfo &A,
   // below (faces_as_R: at most two, the second is the wrap), then as L one
   // range in the pair above (faces_as_L); range u is faces [fb[u] + b[u],
   // fb[u] + e[u]) with its pair's r, deltaPerp and deltaL + deltaR.
-  long long fb[3] = {0, 0, 0};
+  // A regular pair's faces (PairState::reg) are computed from the two
+  // annuli's edges instead of loaded: rg[u], with its jpmin, cell count and
+  // the inner and outer annuli's first zones.
+  long long fb[3] = {0, 0, 0}, zi[3] = {0, 0, 0}, zo[3] = {0, 0, 0};
   double pr[3] = {0, 0, 0}, pp[3] = {0, 0, 0}, ds[3] = {1, 1, 1};
-  if (pR >= 0) {
-    const PairInfo PI = m.pairs[pR];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      fb[u] = PI.face_base;
-      pr[u] = PI.r;
-      pp[u] = PI.deltaPerp;
-      ds[u] = PI.deltaR + PI.deltaL;
+  int rg[3] = {0, 0, 0}, jpm[3] = {0, 0, 0}, np[3] = {0, 0, 0};
+  auto pair_consts = [&](int p, int u) {
+    const PairInfo PI = m.pairs[p];
+    const PairState S = m.pst[p];
+    fb[u] = PI.face_base;
+    pr[u] = PI.r;
+    pp[u] = PI.deltaPerp;
+    ds[u] = PI.deltaR + PI.deltaL;
+    rg[u] = S.reg;
+    if (S.reg) {
+      const AnnInfo Ai = m.ann[PI.ann_in];
+      jpm[u] = S.jpmin;
+      np[u] = Ai.N;
+      zi[u] = Ai.zbase;
+      zo[u] = m.ann[PI.ann_out].zbase;
     }
+  };
+  if (pR >= 0) {
+    pair_consts(pR, 0);
+    pair_consts(pR, 1);
   }
-  if (pLf >= 0) {
-    const PairInfo PI = m.pairs[pLf];
-    fb[2] = PI.face_base;
-    pr[2] = PI.r;
-    pp[2] = PI.deltaPerp;
-    ds[2] = PI.deltaR + PI.deltaL;
-  }
+  if (pLf >= 0) pair_consts(pLf, 2);
+  auto geo = [&](int u, long long gi) {
+    if (rg[u]) return pair_face(m, jpm[u], np[u], zi[u], zo[u], (int)(gi - fb[u]));
+    FaceGeo G;
+    G.cL = m.fL[gi];
+    G.cR = m.fR[gi];
+    G.w = m.fdphi[gi];
+    G.cm = m.fcm[gi];
+    return G;
+  };
   const int t = threadIdx.x;
   int n = 0;
   // the slope sum in face order; two faces per trip, so that the second
@@ -588,19 +625,20 @@ __device__ __forceinline__ void plm_cell_dir(const DevMesh &m, const AnnInfo &A,
     int f = b[u];
     for (; f + 1 < e[u]; f += 2) {
       const long long gi = fb[u] + f;
+      const FaceGeo G0 = geo(u, gi), G1 = geo(u, gi + 1);
       double S0[NQ], S1[NQ];
-      own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], gi, S0);
-      own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], gi + 1, S1);
-      const double dA0 = pr[u] * m.fdphi[gi] * pp[u];
-      const double dA1 = pr[u] * m.fdphi[gi + 1] * pp[u];
+      own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], u == 2 ? G0.cR : G0.cL, G0.cm, S0);
+      own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], u == 2 ? G1.cR : G1.cL, G1.cm, S1);
+      const double dA0 = pr[u] * G0.w * pp[u];
+      const double dA1 = pr[u] * G1.w * pp[u];
       add_face(S0, dA0);
       add_face(S1, dA1);
     }
     if (f < e[u]) {
-      const long long gi = fb[u] + f;
+      const FaceGeo G = geo(u, fb[u] + f);
       double S[NQ];
-      own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], gi, S);
-      add_face(S, pr[u] * m.fdphi[gi] * pp[u]);
+      own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], u == 2 ? G.cR : G.cL, G.cm, S);
+      add_face(S, pr[u] * G.w * pp[u]);
     }
   }
   const double dz = A.zp - A.zm;
@@ -628,7 +666,8 @@ __device__ __forceinline__ void plm_cell_dir(const DevMesh &m, const AnnInfo &A,
 #pragma unroll
         for (int q = 0; q < NQ; ++q) S[q] = s_S[n][q][t];
       } else {
-        own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], fb[u] + f, S);
+        const FaceGeo G = geo(u, fb[u] + f);
+        own_face_slope<NQ>(m, phi_o, w, gp, u == 2, ds[u], u == 2 ? G.cR : G.cL, G.cm, S);
       }
       ++n;
 #pragma unroll
@@ -750,10 +789,10 @@ __device__ __forceinline__ int phi_face_flux(const DevMesh &m, const AnnInfo &A,
 // cells (the fused schedule): the same arithmetic either way.
 template <int NQ, int FEAT>
 __device__ __forceinline__ int rz_face_increments(const DevMesh &m, const PairInfo &PI, const DiskParams &P,
-                                                  const DiskScalars &sc, double dt, long long g, double *X,
+                                                  const DiskScalars &sc, double dt, const FaceGeo &G, double *X,
                                                   double *Y) {
-  const int cL = m.fL[g], cR = m.fR[g];
-  const double cm = m.fcm[g];
+  const long long cL = G.cL, cR = G.cR;
+  const double cm = G.cm;
   const double tL = clean_pi(m.tiph[cL]), tR = clean_pi(m.tiph[cR]);
   const double dL = m.dphi[cL], dR = m.dphi[cR];
   double pL = tL - .5 * dL;
@@ -763,7 +802,7 @@ __device__ __forceinline__ int rz_face_increments(const DevMesh &m, const PairIn
   dpL = wrap_pm(dpL);
   dpR = wrap_pm(dpR);
   const double r = PI.r;
-  const double dA = r * m.fdphi[g] * PI.deltaPerp;
+  const double dA = r * G.w * PI.deltaPerp;
   double primL[NQ], primR[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
@@ -795,13 +834,14 @@ __global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const i
   const int2 tl = ftiles[tile_of(m)];
   const int p = tl.x;
   const int f = tl.y + threadIdx.x;
-  if (f >= m.pst[p].count) return;
+  const PairState S = m.pst[p];
+  if (f >= S.count) return;
   const PairInfo PI = m.pairs[p];
   const DiskParams &P = disk_of(m, PI.disk);
   const DiskScalars &sc = m.sc[PI.disk];
   const long long g = PI.face_base + f;
   double X[NQ], Y[3];
-  int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, fac * sc.dt, g, X, Y);
+  int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, fac * sc.dt, face_geo(m, PI, S, g), X, Y);
   if (e) set_err(m.err, e);
 #pragma unroll
   for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = X[q];
@@ -1039,14 +1079,17 @@ __global__ __launch_bounds__(TILE, update_waves<FEAT>()) void k_update(DevMesh m
       const int *b = dir == 0 ? rb : zb, *e = dir == 0 ? re : ze;
       const bool lo = dir == 0 ? i >= 1 : k >= 1, hi = dir == 0 ? i <= P.NR - 2 : k <= P.NZ - 2;
       if (lo) {
-        const PairInfo PI = m.pairs[dir == 0 ? rpair_of(P, k, i - 1) : zpair_of(P, k - 1, i)];
+        const int pq = dir == 0 ? rpair_of(P, k, i - 1) : zpair_of(P, k - 1, i);
+        const PairInfo PI = m.pairs[pq];
         const double rc = dir == 0 ? PI.r + PI.deltaR : PI.r;
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
           if (fuse) {
+            const PairState S = m.pst[pq];
             for (int f = b[w]; f < e[w]; ++f) {
               double X[NQ], Y[3];
-              const int er = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, PI.face_base + f, X, Y);
+              const int er =
+                  rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, face_geo(m, PI, S, PI.face_base + f), X, Y);
               if (er) set_err(m.err, er);
               add_inc_R<NQ>(X, Y, u, rc, visc);
             }
@@ -1056,12 +1099,15 @@ __global__ __launch_bounds__(TILE, update_waves<FEAT>()) void k_update(DevMesh m
         }
       }
       if (hi) {
-        const PairInfo PI = m.pairs[dir == 0 ? rpair_of(P, k, i) : zpair_of(P, k, i)];
+        const int pq = dir == 0 ? rpair_of(P, k, i) : zpair_of(P, k, i);
+        const PairInfo PI = m.pairs[pq];
         const double rc = dir == 0 ? PI.r - PI.deltaL : PI.r;
         if (fuse) {
+          const PairState S = m.pst[pq];
           for (int f = b[2]; f < e[2]; ++f) {
             double X[NQ], Y[3];
-            const int er = rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, PI.face_base + f, X, Y);
+            const int er =
+                rz_face_increments<NQ, FEAT>(m, PI, P, sc, dt, face_geo(m, PI, S, PI.face_base + f), X, Y);
             if (er) set_err(m.err, er);
             add_inc_L<NQ>(X, Y, u, rc, visc);
           }
