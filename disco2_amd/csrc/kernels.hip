@@ -153,10 +153,18 @@ __device__ int block_excl_scan(int v, int *s_w, int *total) {
 // the reference's; if any did not (ties between edges), thread 0 replays the
 // sequential walk.  The outer annulus's edges are staged in LDS when they
 // fit, so the walks' dependent loads are LDS reads.
+//
+// A pair of equal annuli whose cells all have two faces is regular
+// (PairState::reg): its faces follow from the edges alone, and the step's
+// kernels compute them where they use them.  With skip set, such a pair's
+// walks are verified without storing its faces, face offsets and runs;
+// the class API's face export, the adjoint's record and the outflow
+// boundary's pairs (r pairs from annuli 0, NR-3, NR-2, whose remap reads
+// the stored faces) always store them.
 template <int FTB>
 __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, const AnnInfo &A, const AnnInfo &B,
                                            int p, const double *Tout, const double *Dout, double *s_d, int *s_j,
-                                           int *s_w) {
+                                           int *s_w, int skip) {
   const int tid = threadIdx.x;
   const int Nin = A.N, Nout = B.N;
   const double *Tin = m.tiph + A.zbase, *Din = m.dphi + A.zbase;
@@ -192,6 +200,40 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
     }
   }
   const int jpmin = s_j[0] < 0 ? 0 : s_j[0];
+  if (skip && Nin == Nout && 2 * Nin <= P.cap && Nin + 1 < P.run_cap) {
+    const int NR = disk_of(m, P.disk).NR;
+    if (P.dir == 1 || (A.i != 0 && A.i < NR - 3)) {
+      // every cell two faces, each walk as guessed: regular, nothing stored
+      int irr = 0;
+      for (int base = 0; base < Nin; base += FTB) {
+        const int j = base + tid;
+        if (j < Nin) {
+          const int gs = (j == 0) ? jpmin : guess_outer(Tin[j - 1], Tout, Nout);
+          const int ge = guess_outer(Tin[j], Tout, Nout);
+          int adv = ge - gs;
+          if (adv < 0) adv += Nout;
+          int jpe, b2 = 0;
+          FaceOut none{};
+          const int n = adv == 1 ? jloop_cell<false>(j, gs, Tin, Din, Tout, Dout, Nin, Nout, 2 * j, none, &jpe, &b2)
+                                 : 0;
+          irr |= adv != 1 || b2 || n != 2 || (j + 1 < Nin && jpe != ge);
+        }
+      }
+      irr = __syncthreads_or(irr);
+      if (!irr) {
+        if (tid == 0) {
+          PairState st;
+          st.count = 2 * Nin;
+          st.jpmin = jpmin;
+          st.T = Nin;
+          st.seq = 0;
+          st.reg = 1;
+          m.pst[p] = st;
+        }
+        return;
+      }
+    }
+  }
   FaceOut o;
   o.fL = m.fL;
   o.fR = m.fR;
@@ -270,7 +312,7 @@ __device__ __forceinline__ void faces_pair(const DevMesh &m, const PairInfo &P, 
 // CUs.
 template <int FTB>
 __global__ __launch_bounds__(FTB) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_faces(DevMesh m, int pair0,
-                                                                                           int nstage) {
+                                                                                           int nstage, int skip) {
   extern __shared__ double s_stage[];  // Tout [nstage], Dout [nstage]
   __shared__ double s_d[FTB];
   __shared__ int s_j[FTB];
@@ -287,9 +329,9 @@ __global__ __launch_bounds__(FTB) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
       s_stage[nstage + i] = Dg[i];
     }
     __syncthreads();
-    faces_pair<FTB>(m, P, A, B, p, s_stage, s_stage + nstage, s_d, s_j, s_w);
+    faces_pair<FTB>(m, P, A, B, p, s_stage, s_stage + nstage, s_d, s_j, s_w, skip);
   } else {
-    faces_pair<FTB>(m, P, A, B, p, Tg, Dg, s_d, s_j, s_w);
+    faces_pair<FTB>(m, P, A, B, p, Tg, Dg, s_d, s_j, s_w, skip);
   }
 }
 
@@ -1578,8 +1620,10 @@ __global__ __launch_bounds__(TILE) void k_bc_r_scalar(DevMesh m, const int2 *til
 KProf *g_kprof = nullptr;
 
 // overlap faces of the r (dir 0) or z (dir 1) pairs
+// skip: regular pairs' faces need not be stored (the step's own kernels
+// compute them)
 template <int NQ>
-static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir) {
+static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int dir, int skip) {
   const int np = dir == 0 ? L.n_rpairs : L.n_zpairs;
   const int p0 = dir == 0 ? 0 : L.n_rpairs;
   if (!np) return;
@@ -1587,11 +1631,11 @@ static void launch_faces_q(const DevMesh &m, const LaunchPlan &L, hipStream_t st
   const size_t lds = (size_t)nst * 2 * sizeof(double);
   const char *nm = dir == 0 ? "k_faces_r" : "k_faces_z";
   if (L.face_threads == 256)
-    KLAUNCH(st, nm, hipLaunchKernelGGL(k_faces<256>, dim3(np), dim3(256), lds, st, m, p0, nst));
+    KLAUNCH(st, nm, hipLaunchKernelGGL(k_faces<256>, dim3(np), dim3(256), lds, st, m, p0, nst, skip));
   else if (L.face_threads == 512)
-    KLAUNCH(st, nm, hipLaunchKernelGGL(k_faces<512>, dim3(np), dim3(512), lds, st, m, p0, nst));
+    KLAUNCH(st, nm, hipLaunchKernelGGL(k_faces<512>, dim3(np), dim3(512), lds, st, m, p0, nst, skip));
   else
-    KLAUNCH(st, nm, hipLaunchKernelGGL(k_faces<1024>, dim3(np), dim3(1024), lds, st, m, p0, nst));
+    KLAUNCH(st, nm, hipLaunchKernelGGL(k_faces<1024>, dim3(np), dim3(1024), lds, st, m, p0, nst, skip));
 }
 
 template <int NQ, int FEAT>
@@ -1626,8 +1670,10 @@ static void launch_substep_q(const DevMesh &m, const LaunchPlan &L, hipStream_t 
     if (pre_ops) KLAUNCH(st, "k_scalar", hipLaunchKernelGGL(k_scalar, SD, S64, 0, st, m, L.ndisk, pre_ops, 0.0));
     KLAUNCH(st, "k_plm_p", hipLaunchKernelGGL((k_plm_p<NQ, false>), dim3(L.n_ztiles), B, 0, st, m, L.ztiles));
   }
-  launch_faces_q<NQ>(m, L, st, 0);
-  launch_faces_q<NQ>(m, L, st, 1);
+  // the adjoint's recording forward (m.ftype set) keeps every face
+  const int skip = m.ftype == nullptr;
+  launch_faces_q<NQ>(m, L, st, 0, skip);
+  launch_faces_q<NQ>(m, L, st, 1, skip);
   if (recompute_w) KLAUNCH(st, "k_set_w", hipLaunchKernelGGL(k_set_w, dim3(L.nann), dim3(64), 0, st, m, 0));
   // 3D: both slope directions in one pass, the r slopes kept apart for the r fluxes
   const bool both = L.n_zpairs && m.grad_r;
@@ -1736,7 +1782,7 @@ void launch_copy(double *dst, const double *src, long long n, hipStream_t st) {
 }
 
 void launch_faces(const DevMesh &m, const LaunchPlan &L, hipStream_t st, int NQ, int dir) {
-  DISCO_NQ_DISPATCH(NQ, launch_faces_q<Q>(m, L, st, dir));
+  DISCO_NQ_DISPATCH(NQ, launch_faces_q<Q>(m, L, st, dir, 0));
 }
 
 template <int NQ>
