@@ -16,8 +16,8 @@ Printed: one JSON line with the metric; the HBM roofline of the dominant
 kernel (the update, which carries the step's algorithmic state traffic:
 160 B per interior zone per substep, over its HIP-event duration) and of the
 whole step (320 B per zone-update); the HBM bytes and FP64 operations the
-kernels actually moved, read from rocprofv3 counter passes of this build
-(FETCH_SIZE, WRITE_SIZE and SQ_INSTS_VALU_FLOPS_FP64, the byte counters
+kernels actually moved, read from rocprofv3 counter passes of the library as
+built (FETCH_SIZE, WRITE_SIZE and SQ_INSTS_VALU_FLOPS_FP64, the byte counters
 scaled by what they read for a plain copy of known size); and the reference
 CPU code timed on this host, on all cores under MPI and on one core.
 """
@@ -186,7 +186,7 @@ def _kernel_key(name):
 
 
 def counter_passes(workload, timeout=240):
-    """HBM bytes and FP64 operations of this build, from three rocprofv3
+    """HBM bytes and FP64 operations of the library as built, from three rocprofv3
     counter passes over a child run (bench.py --counters-child): FETCH_SIZE,
     WRITE_SIZE (KiB per dispatch) and SQ_INSTS_VALU_FLOPS_FP64 (per wave
     instruction: x 64 lanes).  The child copies CALIB_DOUBLES doubles (the
@@ -483,7 +483,7 @@ def add_counters(res, cnt):
                                           "fp64_flop_per_launch": v["flop_per_launch"]}
                                       for k, v in sorted(cnt["kernels"].items())},
                        "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE / SQ_INSTS_VALU_FLOPS_FP64, one pass "
-                                 "each, over one eager step of this build (bench.py --counters-child)"}
+                                 "each, over one eager step of the library as built (bench.py --counters-child)"}
 
 
 def main(argv=None):
