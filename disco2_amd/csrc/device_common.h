@@ -15,14 +15,9 @@
 #ifndef FLUX_WAVES
 #define FLUX_WAVES 1
 #endif
-// the update's and the face fluxes' variants for HLL without viscosity
-// (the 3D disk's): at 6 waves per SIMD the flux kernel fits in 78 VGPRs
-// without spilling (88 and 5 waves when left alone; 8 waves spill)
+// the update's variant for HLL without viscosity (the 3D disk's)
 #ifndef UPDATE_WAVES_HLL
 #define UPDATE_WAVES_HLL UPDATE_WAVES
-#endif
-#ifndef FLUX_WAVES_HLL
-#define FLUX_WAVES_HLL 6
 #endif
 
 __device__ __forceinline__ void set_err(int *err, int code) { atomicCAS(err, 0, code); }

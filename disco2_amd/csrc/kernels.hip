@@ -870,13 +870,9 @@ __device__ __forceinline__ int rz_face_increments(const DevMesh &m, const PairIn
   return d_face_flux<NQ, 0, FEAT>(primL, primR, PI.dir == 0 ? 0 : 2, r, dA, 0.0, dt, P, sc, &vi, X, Y);
 }
 
-template <int FEAT>
-constexpr int flux_waves() {
-  return FEAT == F_HLL ? FLUX_WAVES_HLL : FLUX_WAVES;
-}
 // every r (or z) overlap face once: its increments to Fface
 template <int NQ, int FEAT>
-__global__ __launch_bounds__(TILE, flux_waves<FEAT>()) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
+__global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
   const int2 tl = ftiles[tile_of(m)];
   const int p = tl.x;
   const PairState S = m.pst[p];
