@@ -432,10 +432,9 @@ __device__ __forceinline__ void r_face_at(const DevMesh &m, const AdjMesh &a, co
 __device__ __forceinline__ void rflux_h_tile(const DevMesh &m, const AdjMesh &a, const int2 *ftiles, int ti, int slot0) {
   const int2 tl = ftiles[ti];
   const int p = tl.x;
-  const int n = m.pst[p].count;
+  const int f = tl.y + threadIdx.x;
   ScalarAdj sb = {};
-  for (FaceSpan s = face_span(tl, n); s.f < n; s.f += s.step) {
-    const int f = s.f;
+  if (f < m.pst[p].count) {
     const PairInfo PI = m.pairs[p];
     const DiskParams &P = disk_of(m, PI.disk);
     RFaceAt F;
@@ -466,10 +465,9 @@ __device__ __forceinline__ void rflux_h_tile(const DevMesh &m, const AdjMesh &a,
 __device__ __forceinline__ void rflux_v_tile(const DevMesh &m, const AdjMesh &a, const int2 *ftiles, int ti, int slot0) {
   const int2 tl = ftiles[ti];
   const int p = tl.x;
-  const int n = m.pst[p].count;
+  const int f = tl.y + threadIdx.x;
   ScalarAdj sb = {};
-  for (FaceSpan s = face_span(tl, n); s.f < n; s.f += s.step) {
-    const int f = s.f;
+  if (f < m.pst[p].count) {
     const PairInfo PI = m.pairs[p];
     const DiskParams &P = disk_of(m, PI.disk);
     const DiskScalars &sc = m.sc[PI.disk];

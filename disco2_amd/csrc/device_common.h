@@ -35,21 +35,6 @@ __device__ __forceinline__ int xcd_order() {
 // the workgroup's tile under the mesh's launch schedule
 __device__ __forceinline__ int tile_of(const DevMesh &m) { return m.sched_xcd ? xcd_order() : (int)blockIdx.x; }
 
-// A face tile (pair x, first face y) covers faces y + t.  A pair's tiles
-// cover its usual face count; its last tile, marked by y = -1 - f0, takes
-// faces f0 + t, f0 + t + TILE, ... up to the pair's actual count.
-// Loop: for (FaceSpan s = face_span(tl, n); s.f < n; s.f += s.step).
-struct FaceSpan {
-  int f, step;
-};
-__device__ __forceinline__ FaceSpan face_span(int2 tl, int count) {
-  const bool last = tl.y < 0;
-  FaceSpan s;
-  s.f = (last ? -1 - tl.y : tl.y) + (int)threadIdx.x;
-  s.step = last ? TILE : count;  // one trip in a tile that is not the last
-  return s;
-}
-
 __device__ __forceinline__ const DiskParams &disk_of(const DevMesh &m, int d) { return m.dp[d]; }
 
 __device__ __forceinline__ int rpair_of(const DiskParams &P, int k, int i) { return P.rpair_base + k * (P.NR - 1) + i; }

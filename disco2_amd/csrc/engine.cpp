@@ -184,12 +184,7 @@ Engine::Engine(int n, Sim **s) : ndisk(n), sims(s, s + n) {
     Q.run_base = rb;
     fb += Q.cap;
     rb += Q.run_cap;
-    // face tiles over the pair's usual face count, N_in + N_out (two faces
-    // per cell of an equal pair), not over its capacity (about 3 N): the
-    // last tile, marked by y = -1 - f0, also takes the faces past it
-    const int usual = hann[Q.ann_in].N + hann[Q.ann_out].N + 2;
-    for (int f0 = 0; f0 < usual; f0 += DISCO_TILE)
-      (Q.dir == 0 ? rft : zft).push_back(make_int2(p, f0 + DISCO_TILE >= usual ? -1 - f0 : f0));
+    for (int f0 = 0; f0 < Q.cap; f0 += DISCO_TILE) (Q.dir == 0 ? rft : zft).push_back(make_int2(p, f0));
   }
   Ntot = zb;
   NFtot = fb;

@@ -875,23 +875,21 @@ template <int NQ, int FEAT>
 __global__ __launch_bounds__(TILE, FLUX_WAVES) void k_flux_rz(DevMesh m, const int2 *ftiles, double fac) {
   const int2 tl = ftiles[tile_of(m)];
   const int p = tl.x;
+  const int f = tl.y + threadIdx.x;
   const PairState S = m.pst[p];
-  FaceSpan s = face_span(tl, S.count);
-  if (s.f >= S.count) return;
+  if (f >= S.count) return;
   const PairInfo PI = m.pairs[p];
   const DiskParams &P = disk_of(m, PI.disk);
   const DiskScalars &sc = m.sc[PI.disk];
-  for (; s.f < S.count; s.f += s.step) {
-    const long long g = PI.face_base + s.f;
-    double X[NQ], Y[3];
-    int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, fac * sc.dt, face_geo(m, PI, S, g), X, Y);
-    if (e) set_err(m.err, e);
+  const long long g = PI.face_base + f;
+  double X[NQ], Y[3];
+  int e = rz_face_increments<NQ, FEAT>(m, PI, P, sc, fac * sc.dt, face_geo(m, PI, S, g), X, Y);
+  if (e) set_err(m.err, e);
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = X[q];
-    if ((FEAT & F_VISC) && P.visc > 0.0) {
+  for (int q = 0; q < NQ; ++q) m.Fface[q * m.NFtot + g] = X[q];
+  if ((FEAT & F_VISC) && P.visc > 0.0) {
 #pragma unroll
-      for (int v = 0; v < 3; ++v) m.Fface[(NQ + v) * m.NFtot + g] = Y[v];
-    }
+    for (int v = 0; v < 3; ++v) m.Fface[(NQ + v) * m.NFtot + g] = Y[v];
   }
 }
 
