@@ -63,6 +63,21 @@ def test_vortex_c1_size_100_steps_bitwise():
     assert bitwise_equal(gpu["scalars"][:2], ref["scalars"][:2])
 
 
+def test_coarse_inner_annuli_bitwise():
+    """A vortex mesh whose inner ghost annulus has 3 cells against 9 in the
+    next (each inner cell meets 3-4 outer cells, so 4-5 faces per cell and
+    runs of several faces in the face builder), 16 in the one after: bit for
+    bit against the oracle after 10 steps."""
+    from disco2_amd import configs
+    from disco2_amd.api import Setup
+    p, kind, kw = configs.vortex(24, r_min=0.2)
+    assert [int(n) for n in Setup(p).N_p[:3]] == [3, 9, 16]
+    print("vortex NumR 24 from r = 0.2 (N_phi 3, 9, 16, ...), 10 steps", flush=True)
+    s, gpu, ref = run_both(p, kind, kw, 10)
+    for f in ("prim", "cons", "tiph", "dphi"):
+        assert bitwise_equal(gpu[f], ref[f]), f"{f}: {describe(gpu[f], ref[f])}"
+
+
 def test_disk3d_vs_oracle():
     from disco2_amd import configs
     p, kind, kw = configs.disk3d(16, 8, 48, R_Min=0.5, R_Max=3.0)
