@@ -159,6 +159,38 @@ def test_adjoint_is_reproducible(adj):
     assert np.array_equal(gp, adj["gp"])
 
 
+def test_inviscid_adjoint_matches_differences():
+    """Explicit_Viscosity 0: the step takes no viscous branch and neither
+    does the backward (no viscous face kernels).  dL/dq and chosen initial
+    primitives against central differences of the GPU loss on the untied
+    state; dL/d alpha is the derivative of the inviscid branch, 0 (the loss
+    has a kink at alpha = 0: from the right it grows with the viscous flux)."""
+    p, kind, kw = case(alpha=0.0)
+    prim0, s = initial_state(p, kind, kw, True)
+    print(f"inviscid adjoint: binary NumR 24, {STEPS} steps", flush=True)
+    loss, ga, gq, gp = gpu(p, prim0, adjoint=True)
+    assert ga == 0.0
+    assert abs(loss - oracle_loss(p, prim0)) <= 1e-11 * abs(loss)
+    fd = central(lambda x: gpu(case(alpha=0.0, q=x)[0], prim0), 0.5, 0.5e-6)
+    print(f"dL/dq: adjoint {gq:.10e}  GPU differences {fd:.10e}")
+    assert abs(gq - fd) <= REL * abs(fd)
+    aoff = np.concatenate([[0], np.cumsum(s.N_p)])
+    gmax = np.max(np.abs(gp))
+    for i in (1, s.NR // 2, s.NR - 3):
+        z = int(aoff[i] + 3)
+        for q in (0, 1, 2, 3):
+            x0 = prim0[q, z]
+            h = 1e-5 * max(abs(x0), 1e-4)
+
+            def f(x):
+                pr = prim0.copy()
+                pr[q, z] = x
+                return gpu(p, pr)
+
+            d = central(f, x0, h)
+            assert abs(gp[q, z] - d) / (abs(d) + 1e-2 * gmax) <= 1e-4, f"zone {z} q {q}: {gp[q, z]:.6e} vs {d:.6e}"
+
+
 def test_out_of_scope_is_rejected():
     """3D, another frame, state-dependent mesh motion: the adjoint refuses."""
     from disco2_amd import configs, ics
