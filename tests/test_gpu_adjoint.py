@@ -191,6 +191,40 @@ def test_inviscid_adjoint_matches_differences():
             assert abs(gp[q, z] - d) / (abs(d) + 1e-2 * gmax) <= 1e-4, f"zone {z} q {q}: {gp[q, z]:.6e} vs {d:.6e}"
 
 
+@pytest.mark.parametrize("variant", ["hll", "fixed_mesh"])
+def test_adjoint_variants_match_differences(variant):
+    """The HLL solver's backward, and a mesh at rest (w_numeric_type 1, no
+    annulus rotation to differentiate), on the untied state: the loss against
+    the oracle's, dL/d alpha and dL/d q against central differences of the
+    GPU loss and of the oracle's loss.  Steps: 1e-6 relative in q (the loss
+    is curved in q: at 1e-5 the truncation error is 3e-6 with HLL) and 1e-5
+    in alpha (the mesh at rest has a small dL/d alpha, 0.085 against 1.5
+    with HLL, and at 1e-6 the round-off of the losses' difference reaches
+    1e-6 of it: GPU and oracle differences 9e-7 apart)."""
+    from disco2_amd.params import C_FIXED, HLL
+    over = {"hll": {"RiemannSolver": HLL}, "fixed_mesh": {"w_numeric_type": C_FIXED}}[variant]
+
+    def params(alpha=0.01, q=0.5):
+        p = case(alpha=alpha, q=q)[0]
+        p.update(over)
+        return p
+
+    p = params()
+    _, kind, kw = case()
+    prim0, _ = initial_state(p, kind, kw, True)
+    print(f"adjoint {variant}: binary NumR 24, {STEPS} steps", flush=True)
+    loss, ga, gq, _ = gpu(p, prim0, adjoint=True)
+    assert abs(loss - oracle_loss(p, prim0)) <= 1e-11 * abs(loss)
+    for name, g, x0 in (("alpha", ga, 0.01), ("q", gq, 0.5)):
+        h = (1e-5 if name == "alpha" else 1e-6) * x0
+        mk = (lambda x: params(alpha=x)) if name == "alpha" else (lambda x: params(q=x))
+        fd = central(lambda x: gpu(mk(x), prim0), x0, h)
+        fdo = central(lambda x: oracle_loss(mk(x), prim0), x0, h)
+        print(f"dL/d{name}: adjoint {g:.10e}  GPU differences {fd:.10e}  oracle differences {fdo:.10e}")
+        assert abs(g - fd) <= REL * abs(fd)
+        assert abs(g - fdo) <= REL * abs(fdo)
+
+
 def test_out_of_scope_is_rejected():
     """3D, another frame, state-dependent mesh motion: the adjoint refuses."""
     from disco2_amd import configs, ics
