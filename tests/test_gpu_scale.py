@@ -134,6 +134,40 @@ def test_batch_reproduces_single_disks_bitwise():
         b.close()
 
 
+def test_tiny_disks_in_a_batch():
+    """The smallest disks: one interior annulus (NumR 1: 4 annuli with the
+    ghosts, 75 zones) up to NumR 8, every annulus shorter than a zone tile,
+    six of them in one batch: each bit for bit as alone after 10 steps, and
+    the smallest bit for bit against the oracle."""
+    from disco2_amd import configs, ics
+    from disco2_amd.api import Batch, Setup
+    specs = [configs.vortex(n, r_min=1.0, r_max=r) for n, r in ((1, 1.5), (2, 1.5), (3, 2.0), (4, 2.0), (6, 2.0),
+                                                                  (8, 3.0))]
+    setups = [Setup(p) for p, _, _ in specs]
+    assert setups[0].NR == 4 and setups[0].nzones == 75
+    prims = [ics.make_ic(k, s.geometry(), p, **kw) for (p, k, kw), s in zip(specs, setups)]
+    print("batch of 6 tiny disks (75 to 588 zones), 10 steps", flush=True)
+    big = Batch(setups)
+    for d, pr in enumerate(prims):
+        big.upload("prim", pr, disk=d)
+    big.init()
+    big.run(10)
+    got = [(big.download("prim", disk=d), big.download("tiph", disk=d)) for d in range(len(setups))]
+    big.close()
+    for d, (s, pr) in enumerate(zip(setups, prims)):
+        b = Batch([s])
+        b.upload("prim", pr)
+        b.init()
+        b.run(10)
+        assert bitwise_equal(got[d][0], b.download("prim")), f"disk {d}"
+        assert bitwise_equal(got[d][1], b.download("tiph")), f"disk {d}"
+        b.close()
+    p, kind, kw = specs[0]
+    _, gpu, ref = run_both(p, kind, kw, 10)
+    for f in ("prim", "cons", "tiph", "dphi"):
+        assert bitwise_equal(gpu[f], ref[f]), f"{f}: {describe(gpu[f], ref[f])}"
+
+
 def test_uniform_state_is_preserved():
     """Well-balanced geometry: a uniform state at rest without gravity stays
     uniform (the r-flux pressure difference equals the P/r source exactly in
