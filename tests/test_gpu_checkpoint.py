@@ -88,6 +88,8 @@ def case(kind):
     from disco2_amd import configs
     if kind == "vortex":
         return configs.vortex(24)
+    if kind == "disk3d":
+        return configs.disk3d(8, 4, 24, R_Min=0.5, R_Max=2.5)
     return configs.binary(32, r_min=0.5, r_max=4.0, q=0.5)
 
 
@@ -120,7 +122,7 @@ def restart_oracle(p, s, data, T, gm):
     return ckpt_util.restart_oracle(p, s.N_p, s.r_faces, s.z_faces, s.dphi0, data, T, gm)
 
 
-@pytest.mark.parametrize("kind", ["vortex", "binary"])
+@pytest.mark.parametrize("kind", ["vortex", "binary", "disk3d"])
 def test_batch_checkpoint_and_restart(kind, tmp_path):
     from disco2_amd import ics
     from disco2_amd.api import Batch, Setup
