@@ -9,6 +9,8 @@ reference build).
   DiagMdot.dat line to 1e-13 of the oracle (the GPU sums them per annulus,
   the reference in one running sum) and the files' text equal to the
   reference's.
+* At full size (binary2d_r512, 874,488 equatorial cells): one measurement
+  bit for bit against the oracle's.
 * The class API in a C driver (src/main.c:121-131 with diagnostics_set /
   diagnostics_print after every step): the files it writes against the
   reference's files for the same run: identical text for the vortex (whose
@@ -181,3 +183,35 @@ def test_c_driver_writes_the_reference_files(name, tmp_path):
         assert np.array_equal(eq[:, me], ref_eq[:, me])
     else:
         assert np.allclose(eq[:, me], ref_eq[:, me], rtol=1e-9, atol=1e-14)
+
+
+def test_full_size_measurement_matches_oracle():
+    """One measurement on the measured 2D workload's initial state
+    (binary2d_r512, 874,488 interior zones, all of them equatorial): the
+    radial profiles, the scalars and the equatorial cells bit for bit
+    against the oracle's (which reproduces the diag_* fixtures bit for bit),
+    the mass counts to 1e-13."""
+    from disco2_amd import configs, ics
+    from disco2_amd.api import Batch, Setup
+    from oracle.oracle import Oracle
+    p, kind, kw = configs.binary(512)
+    s = Setup(p)
+    prim0 = ics.make_ic(kind, s.geometry(), p, **kw)
+    print("diagnostics: binary2d_r512, one measurement", flush=True)
+    b = Batch([s])
+    b.upload("prim", prim0)
+    b.init()
+    o = Oracle(p, s.N_p, s.r_faces, s.z_faces)
+    o.set_state(tiph=s.tiph0, prim=prim0)
+    o.init()
+    rz = np.array([p["R_Min"], p["R_Max"], p["Z_Min"], p["Z_Max"]], dtype=np.float64)
+    vg, sg, eg, mg = b.diag_measure(0.25, rz)
+    vo, so, eo, mo = o.diag_measure(0.25, rz)
+    b.close()
+    mv, me, ms = np.ones(17, bool), np.ones(18, bool), np.ones(16, bool)
+    mv[8] = me[9] = ms[7] = False  # NUM_Q 5: the passive-scalar column (see the module's note)
+    assert vg.shape == vo.shape and eg.shape == eo.shape and len(eg) == 874488
+    assert np.array_equal(vg[:, mv], vo[:, mv])
+    assert np.array_equal(sg[ms], so[ms])
+    assert np.array_equal(eg[:, me], eo[:, me])
+    assert np.allclose(mg, mo, rtol=1e-13, atol=0.0)
